@@ -1,0 +1,203 @@
+// Shared device helpers for the gfx950 candidate-scoring kernels:
+// complex arithmetic, the LDS-resident mixed-radix Stockham FFT, wave
+// reductions, and the C-ABI error plumbing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include <algorithm>
+
+#define GIS_API extern "C" __attribute__((visibility("default")))
+
+// ---------------------------------------------------------------------------
+// Error reporting across the C ABI: every entry point returns 0 on success,
+// a positive hipError_t on a runtime failure, or a negative code for invalid
+// arguments; gis_last_error() returns the message of the last failure on the
+// calling thread.
+// ---------------------------------------------------------------------------
+enum {
+  GIS_OK = 0,
+  GIS_EINVAL = -1,      // invalid argument (shape, size, pointer)
+  GIS_EUNSUPPORTED = -2 // box size / crop size outside the compiled set
+};
+
+void gis_set_error(const char* fmt, ...);
+
+#define GIS_REQUIRE(cond, ...)              \
+  do {                                      \
+    if (!(cond)) {                          \
+      gis_set_error(__VA_ARGS__);           \
+      return GIS_EINVAL;                    \
+    }                                       \
+  } while (0)
+
+#define GIS_LAUNCH_CHECK(what)                                              \
+  do {                                                                      \
+    hipError_t e_ = hipGetLastError();                                      \
+    if (e_ != hipSuccess) {                                                 \
+      gis_set_error("%s: launch failed: %s", what, hipGetErrorString(e_));  \
+      return (int)e_;                                                       \
+    }                                                                       \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Complex helpers (float2 = re, im)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mul_mi(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------------------
+// Twiddle table: tw[k] = exp(-2 pi i k / L), k in [0, L), computed in double.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void fill_twiddles(float2* tw, int L, int tid, int nthr) {
+  for (int k = tid; k < L; k += nthr) {
+    double s, c;
+    sincospi(-2.0 * (double)k / (double)L, &s, &c);
+    tw[k] = make_float2((float)c, (float)s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS Stockham FFT, L = 2^a * 3^b (b <= 1): radix-4 stages, one radix-2 stage
+// if a is odd, one radix-3 stage if 3 | L.  Natural-order input and output.
+//
+// nseq sequences live in LDS at buf[q * sp + i]; the result ends in one of
+// the two ping-pong buffers, whose address is returned.  The caller must
+// __syncthreads() before the call (data ready); the function ends with a
+// barrier.  Forward: exp(-2 pi i / L); inverse: exp(+...), unscaled.
+// ---------------------------------------------------------------------------
+template <int L, int R, bool INV>
+__device__ __forceinline__ void stockham_stage(const float2* __restrict__ src, float2* __restrict__ dst,
+                                               int Ns, int nseq, int sp, const float2* __restrict__ tw,
+                                               int tid, int nthr) {
+  constexpr int LR = L / R;
+  const int total = nseq * LR;
+  const int step = L / (Ns * R);
+  for (int t = tid; t < total; t += nthr) {
+    const int q = t / LR;
+    const int j = t - q * LR;
+    const float2* S = src + q * sp;
+    float2* D = dst + q * sp;
+    const int k = j % Ns;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = S[j + r * LR];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      float2 w = tw[k * r * step];
+      if (INV) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+    if constexpr (R == 2) {
+      float2 a = v[0], b = v[1];
+      v[0] = cadd(a, b);
+      v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+      float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+      float2 c = cadd(v[1], v[3]), d = mul_mi<INV>(csub(v[1], v[3]));
+      v[0] = cadd(a, c);
+      v[2] = csub(a, c);
+      v[1] = cadd(b, d);
+      v[3] = csub(b, d);
+    } else if constexpr (R == 3) {
+      const float c1 = -0.5f;
+      const float s1 = INV ? 0.86602540378443864676f : -0.86602540378443864676f;
+      float2 a = v[0], b = v[1], c = v[2];
+      float2 t1 = cadd(b, c);
+      float2 t2 = make_float2(a.x + c1 * t1.x, a.y + c1 * t1.y);
+      float2 t3 = csub(b, c);  // (b - c) * i*s1
+      float2 t4 = make_float2(-s1 * t3.y, s1 * t3.x);
+      v[0] = cadd(a, t1);
+      v[1] = cadd(t2, t4);
+      v[2] = csub(t2, t4);
+    }
+    const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) D[base + r * Ns] = v[r];
+  }
+}
+
+template <int L>
+struct FftShape {
+  static constexpr int three = (L % 3 == 0) ? 1 : 0;
+  static constexpr int p2 = three ? L / 3 : L;
+  static constexpr int lg = (p2 >= 512) ? 9 : (p2 >= 256) ? 8 : (p2 >= 128) ? 7 : (p2 >= 64) ? 6
+                          : (p2 >= 32) ? 5 : (p2 >= 16) ? 4 : (p2 >= 8) ? 3 : (p2 >= 4) ? 2 : 1;
+  static_assert((1 << lg) == p2, "FFT length must be 2^a or 3*2^a");
+  static constexpr int n4 = lg / 2;
+  static constexpr int n2 = lg % 2;
+  static constexpr int stages = n4 + n2 + three;
+};
+
+template <int L, bool INV>
+__device__ float2* fft_lds(float2* a, float2* b, int nseq, int sp, const float2* tw, int tid, int nthr) {
+  using S = FftShape<L>;
+  float2* src = a;
+  float2* dst = b;
+  int Ns = 1;
+#pragma unroll
+  for (int s = 0; s < S::n4; ++s) {
+    stockham_stage<L, 4, INV>(src, dst, Ns, nseq, sp, tw, tid, nthr);
+    __syncthreads();
+    float2* t = src; src = dst; dst = t;
+    Ns *= 4;
+  }
+  if constexpr (S::n2) {
+    stockham_stage<L, 2, INV>(src, dst, Ns, nseq, sp, tw, tid, nthr);
+    __syncthreads();
+    float2* t = src; src = dst; dst = t;
+    Ns *= 2;
+  }
+  if constexpr (S::three) {
+    stockham_stage<L, 3, INV>(src, dst, Ns, nseq, sp, tw, tid, nthr);
+    __syncthreads();
+    float2* t = src; src = dst; dst = t;
+    Ns *= 3;
+  }
+  return src;
+}
+
+// ---------------------------------------------------------------------------
+// Block-wide reductions (blockDim.x a multiple of 64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sums `nv` doubles per thread over the block; result valid in all threads.
+// `scratch` must hold nv * (blockDim.x / 64) doubles of LDS.
+template <int NV>
+__device__ __forceinline__ void block_sum_f64(double (&v)[NV], double* scratch) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum_f64(v[i]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) scratch[i * nw + wid] = v[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += scratch[i * nw + w];
+    v[i] = s;
+  }
+  __syncthreads();
+}
+
+// Number of compute units of the current device (cached per device).
+int gis_cu_count();

@@ -1,0 +1,465 @@
+// Template bank: central-slice projection, whitening and packing.
+//
+// For every direction d (rotation R_d = Euler(rot, tilt, 0)^T) this computes
+// the unit-norm packed template of IntegratedProjector.build_templates
+// (v606:1077-1147):
+//   1. trilinear central-slice gather from the 3-D half spectrum with the
+//      Friedel conjugation where the rotated x < 0 (v606:856-896, 1099-1113)
+//   2. ifftshift rows, irfft2 (complex-to-real rule: the imaginary parts of
+//      columns 0 and N are dropped after the row transform), ifftshift, crop
+//      the padding (v606:1122-1130)
+//   3. whitening (v606:938-992): fft2, ring mean of |F| -> divide (DC ring
+//      kept), inverse; least-squares plane with x = row and y = column
+//      (v606:916-918), subtract, (x - mean) / std
+//   4. x mask, fft2, crop to n, pack to the rfft2 layout (v606:494-541),
+//      x FSC, / L2 norm (v606:1131-1146)
+// The bank is stored UNshifted ([M][n][n/2+1], rfft2 layout); the
+// correlation kernel folds the reference's pre-shift roll into its tables.
+//
+// gfx950 mapping: one 256-thread workgroup per direction (persistent grid),
+// every 1-D transform an LDS Stockham FFT, the 2-D passes staged through a
+// per-workgroup scratch slot in HBM (mostly L2/MALL resident).  Two real
+// rows travel in one complex FFT (real + i*imag) in the row passes.
+#include "common.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kLdsFloat2 = 8192;  // 64 KiB of LDS per workgroup
+
+__device__ __forceinline__ int ring_of(int fy, int kx) {
+  return (int)rintf(sqrtf((float)(fy * fy + kx * kx)));
+}
+
+template <int N>
+struct TplLayout {
+  static constexpr int L2 = 2 * N;       // padded length
+  static constexpr int M2 = N + 1;       // padded half-spectrum width
+  static constexpr int MH = N / 2 + 1;   // box half-spectrum width
+  static constexpr int NR = (int)(0.7072 * N) + 3;  // rings (upper bound)
+  static constexpr int FIXED = L2 + N + NR / 2 + 1 + 32;  // twiddles, ring divisors, reductions
+  static constexpr int CAP = (kLdsFloat2 - FIXED) / 2;    // per ping-pong buffer
+  static constexpr int TK1 = CAP / (L2 + 1);              // phase 1 columns per tile
+  static constexpr int RB2 = CAP / L2;                    // phase 2 complex rows per batch
+  static constexpr int TK3 = CAP / (N + 1);               // box-length column tiles
+  static constexpr int RB6 = CAP / N;                     // box-length complex rows
+  static_assert(TK1 >= 1 && RB2 >= 1, "LDS budget");
+  static constexpr size_t U_ELEMS = (size_t)N * M2;  // float2
+  static constexpr size_t W_ELEMS = (size_t)N * MH;  // float2
+  static constexpr size_t SLOT = U_ELEMS + W_ELEMS;
+};
+
+template <int N>
+__global__ void __launch_bounds__(kThreads)
+build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ rts, int M, int n,
+                       const float* __restrict__ mask, const float* __restrict__ fsc,
+                       float2* __restrict__ bank, float* __restrict__ norms, float2* __restrict__ scratch) {
+  using T = TplLayout<N>;
+  constexpr int L2 = T::L2, M2 = T::M2, MH = T::MH;
+  __shared__ float2 lds[kLdsFloat2];
+  float2* tw2 = lds;                       // L2 twiddles
+  float2* tw1 = tw2 + L2;                  // N twiddles
+  float* div = (float*)(tw1 + N);          // NR ring divisors
+  double* red = (double*)(div + T::NR + (T::NR & 1));  // reduction scratch (16 doubles)
+  float2* bufA = lds + T::FIXED;
+  float2* bufB = bufA + T::CAP;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  fill_twiddles(tw2, L2, tid, nthr);
+  fill_twiddles(tw1, N, tid, nthr);
+  float2* U = scratch + (size_t)blockIdx.x * T::SLOT;
+  float2* W = U + T::U_ELEMS;
+  float* Z = (float*)U;  // real image, aliases U once U is consumed
+  const int mh_n = n / 2 + 1;
+  const float inv_l2 = 1.0f / (float)L2;
+
+  for (int d = blockIdx.x; d < M; d += gridDim.x) {
+    const float* R = rts + 9 * d;
+    const float r00 = R[0], r01 = R[1], r10 = R[3], r11 = R[4], r20 = R[6], r21 = R[7];
+    __syncthreads();
+
+    // ---- phase 1: gather + inverse column FFT (length 2N) ----------------
+    for (int k0 = 0; k0 < M2; k0 += T::TK1) {
+      const int nk = min(T::TK1, M2 - k0);
+      for (int t = tid; t < nk * L2; t += nthr) {
+        const int q = t / L2, ip = t - q * L2;
+        const int kx = k0 + q;
+        const int f = ip < N ? ip : ip - L2;
+        const float fy = (float)f * inv_l2;
+        const float fx = (float)kx * inv_l2;
+        float gx = r00 * fx + r01 * fy;
+        float gy = r10 * fx + r11 * fy;
+        float gz = r20 * fx + r21 * fy;
+        const bool cj = gx < 0.0f;
+        if (cj) { gx = -gx; gy = -gy; gz = -gz; }
+        float cz = gz * (float)L2 + (float)N;
+        float cy = gy * (float)L2 + (float)N;
+        float cx = gx * (float)L2;
+        cz = fminf(fmaxf(cz, 0.0f), (float)(L2 - 1));
+        cy = fminf(fmaxf(cy, 0.0f), (float)(L2 - 1));
+        cx = fminf(fmaxf(cx, 0.0f), (float)(M2 - 1));
+        const float z0f = floorf(cz), y0f = floorf(cy), x0f = floorf(cx);
+        const float tz = cz - z0f, ty = cy - y0f, tx = cx - x0f;
+        const int z0 = (int)z0f, y0 = (int)y0f, x0 = (int)x0f;
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int dz = 0; dz < 2; ++dz) {
+          const int iz = z0 + dz;
+          if (iz > L2 - 1) continue;
+          const float wz = dz ? tz : 1.0f - tz;
+          const int kz = (iz + N) % L2;
+#pragma unroll
+          for (int dy = 0; dy < 2; ++dy) {
+            const int iy = y0 + dy;
+            if (iy > L2 - 1) continue;
+            const float wy = dy ? ty : 1.0f - ty;
+            const int ky = (iy + N) % L2;
+            const float2* row = D + ((size_t)kz * L2 + ky) * M2;
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+              const int ix = x0 + dx;
+              if (ix > M2 - 1) continue;
+              const float wx = dx ? tx : 1.0f - tx;
+              const float w = (wz * wy) * wx;
+              const float2 v = row[ix];
+              acc.x += w * v.x;
+              acc.y += w * v.y;
+            }
+          }
+        }
+        if (cj) acc.y = -acc.y;
+        bufA[q * (L2 + 1) + ip] = acc;
+      }
+      __syncthreads();
+      float2* res = fft_lds<L2, true>(bufA, bufB, nk, L2 + 1, tw2, tid, nthr);
+      // keep spatial rows y in [3N/2, 2N) u [0, N/2) -> projection row Y
+      for (int t = tid; t < nk * N; t += nthr) {
+        const int Y = t / nk, q = t - Y * nk;
+        const int y = (Y + 3 * N / 2) % L2;
+        U[(size_t)Y * M2 + k0 + q] = res[q * (L2 + 1) + y];
+      }
+      __syncthreads();
+    }
+
+    // ---- phase 2: complex-to-real rows (2N) -> crop -> forward FFT (N) ----
+    for (int y0 = 0; y0 < N; y0 += 2 * T::RB2) {
+      const int nrow = min(2 * T::RB2, N - y0);
+      const int nseq = (nrow + 1) / 2;
+      for (int t = tid; t < nseq * L2; t += nthr) {
+        const int q = t / L2, k = t - q * L2;
+        const int ya = y0 + 2 * q, yb = ya + 1;
+        // Hermitian extension of the half spectrum, Im of columns 0 and N dropped
+        const int ks = k <= N ? k : L2 - k;
+        float2 a = U[(size_t)ya * M2 + ks];
+        float2 b = yb < y0 + nrow ? U[(size_t)yb * M2 + ks] : make_float2(0.f, 0.f);
+        if (ks == 0 || ks == N) { a.y = 0.f; b.y = 0.f; }
+        if (k > N) { a.y = -a.y; b.y = -b.y; }
+        bufA[q * L2 + k] = make_float2(a.x - b.y, a.y + b.x);  // a + i*b
+      }
+      __syncthreads();
+      float2* res = fft_lds<L2, true>(bufA, bufB, nseq, L2, tw2, tid, nthr);
+      float2* oth = res == bufA ? bufB : bufA;
+      const float sc = 1.0f / ((float)L2 * (float)L2);
+      for (int t = tid; t < nseq * N; t += nthr) {
+        const int q = t / N, X = t - q * N;
+        const float2 v = res[q * L2 + (X + 3 * N / 2) % L2];
+        oth[q * N + X] = make_float2(v.x * sc, v.y * sc);  // projection rows ya (re), yb (im)
+      }
+      __syncthreads();
+      float2* spec = fft_lds<N, false>(oth, res, nseq, N, tw1, tid, nthr);
+      for (int t = tid; t < nseq * MH; t += nthr) {
+        const int q = t / MH, k = t - q * MH;
+        const float2 z = spec[q * N + k];
+        const float2 zc = cconj(spec[q * N + (N - k) % N]);
+        const int ya = y0 + 2 * q;
+        W[(size_t)ya * MH + k] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+        if (ya + 1 < y0 + nrow)
+          W[(size_t)(ya + 1) * MH + k] = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+      }
+      __syncthreads();
+    }
+
+    // ---- phase 3: forward column FFT (N) -> projection spectrum F ---------
+    for (int k0 = 0; k0 < MH; k0 += T::TK3) {
+      const int nk = min(T::TK3, MH - k0);
+      for (int t = tid; t < nk * N; t += nthr) {
+        const int Y = t / nk, q = t - Y * nk;
+        bufA[q * (N + 1) + Y] = W[(size_t)Y * MH + k0 + q];
+      }
+      __syncthreads();
+      float2* res = fft_lds<N, false>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
+      for (int t = tid; t < nk * N; t += nthr) {
+        const int ky = t / nk, q = t - ky * nk;
+        W[(size_t)ky * MH + k0 + q] = res[q * (N + 1) + ky];
+      }
+      __syncthreads();
+    }
+
+    // ---- phase 4: ring sums of |F| over the full plane (deterministic) ---
+    const int nr = ring_of(N / 2, N / 2) + 1;
+    for (int ring = tid; ring < nr; ring += nthr) {
+      float s = 0.0f;
+      int cnt = 0;
+      const float lo = (float)ring - 0.5f, hi = (float)ring + 0.5f;
+      for (int fy = -N / 2; fy < N / 2; ++fy) {
+        const float h2 = hi * hi - (float)(fy * fy);
+        if (h2 < 0.0f) continue;
+        const float l2 = lo * lo - (float)(fy * fy);
+        int ka = l2 > 0.0f ? (int)floorf(sqrtf(l2)) - 1 : 0;
+        int kb = (int)ceilf(sqrtf(h2)) + 1;
+        ka = max(ka, 0);
+        kb = min(kb, N / 2);
+        const int ky = fy < 0 ? fy + N : fy;
+        for (int kx = ka; kx <= kb; ++kx) {
+          if (ring_of(fy, kx) != ring) continue;
+          const int wgt = (kx == 0 || kx == N / 2) ? 1 : 2;
+          const float2 v = W[(size_t)ky * MH + kx];
+          const float a = sqrtf(v.x * v.x + v.y * v.y);
+          s += wgt == 2 ? a + a : a;
+          cnt += wgt;
+        }
+      }
+      div[ring] = (ring == 0 || cnt == 0) ? 1.0f : (float)cnt / s;
+    }
+    __syncthreads();
+
+    // ---- phase 5: scale by ring divisor, inverse column FFT (N) -----------
+    for (int k0 = 0; k0 < MH; k0 += T::TK3) {
+      const int nk = min(T::TK3, MH - k0);
+      for (int t = tid; t < nk * N; t += nthr) {
+        const int ky = t / nk, q = t - ky * nk;
+        const int fy = ky < N / 2 ? ky : ky - N;
+        const float g = div[ring_of(fy, k0 + q)];
+        const float2 v = W[(size_t)ky * MH + k0 + q];
+        bufA[q * (N + 1) + ky] = make_float2(v.x * g, v.y * g);
+      }
+      __syncthreads();
+      float2* res = fft_lds<N, true>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
+      for (int t = tid; t < nk * N; t += nthr) {
+        const int Y = t / nk, q = t - Y * nk;
+        W[(size_t)Y * MH + k0 + q] = res[q * (N + 1) + Y];
+      }
+      __syncthreads();
+    }
+
+    // ---- phase 6: complex-to-real rows (N) -> real image + plane sums ----
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};  // sum z, sum z^2, sum Y*z, sum X*z
+    for (int y0 = 0; y0 < N; y0 += 2 * T::RB6) {
+      const int nrow = min(2 * T::RB6, N - y0);
+      const int nseq = (nrow + 1) / 2;
+      for (int t = tid; t < nseq * N; t += nthr) {
+        const int q = t / N, k = t - q * N;
+        const int ya = y0 + 2 * q, yb = ya + 1;
+        const int ks = k <= N / 2 ? k : N - k;
+        float2 a = W[(size_t)ya * MH + ks];
+        float2 b = yb < y0 + nrow ? W[(size_t)yb * MH + ks] : make_float2(0.f, 0.f);
+        if (ks == 0 || ks == N / 2) { a.y = 0.f; b.y = 0.f; }
+        if (k > N / 2) { a.y = -a.y; b.y = -b.y; }
+        bufA[q * N + k] = make_float2(a.x - b.y, a.y + b.x);
+      }
+      __syncthreads();
+      float2* res = fft_lds<N, true>(bufA, bufB, nseq, N, tw1, tid, nthr);
+      const float sc = 1.0f / ((float)N * (float)N);
+      for (int t = tid; t < nseq * N; t += nthr) {
+        const int q = t / N, X = t - q * N;
+        const float2 v = res[q * N + X];
+        const int ya = y0 + 2 * q;
+        const float za = v.x * sc;
+        Z[(size_t)ya * N + X] = za;
+        acc[0] += za; acc[1] += (double)za * za; acc[2] += (double)ya * za; acc[3] += (double)X * za;
+        if (ya + 1 < y0 + nrow) {
+          const float zb = v.y * sc;
+          Z[(size_t)(ya + 1) * N + X] = zb;
+          acc[0] += zb; acc[1] += (double)zb * zb; acc[2] += (double)(ya + 1) * zb; acc[3] += (double)X * zb;
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- phase 7: least-squares plane, mean, std (float64) ---------------
+    block_sum_f64<4>(acc, red);
+    double pa, pb, pc, mean, inv_std;
+    {
+      const double nn = (double)N;
+      const double s1 = nn * (nn * (nn - 1.0) / 2.0);                       // sum of a coordinate
+      const double s2 = nn * ((nn - 1.0) * nn * (2.0 * nn - 1.0) / 6.0);   // sum of its square
+      const double sxy = (nn * (nn - 1.0) / 2.0) * (nn * (nn - 1.0) / 2.0);
+      const double cnt = nn * nn;
+      // normal equations [[s2, sxy, s1], [sxy, s2, s1], [s1, s1, cnt]] * (a, b, c) = (Sxz, Syz, Sz)
+      double A[3][4] = {{s2, sxy, s1, acc[2]}, {sxy, s2, s1, acc[3]}, {s1, s1, cnt, acc[0]}};
+      for (int c = 0; c < 3; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 3; ++r)
+          if (fabs(A[r][c]) > fabs(A[p][c])) p = r;
+        if (p != c)
+          for (int j = 0; j < 4; ++j) { double tt = A[c][j]; A[c][j] = A[p][j]; A[p][j] = tt; }
+        for (int r = c + 1; r < 3; ++r) {
+          const double f = A[r][c] / A[c][c];
+          for (int j = c; j < 4; ++j) A[r][j] -= f * A[c][j];
+        }
+      }
+      pc = A[2][3] / A[2][2];
+      pb = (A[1][3] - A[1][2] * pc) / A[1][1];
+      pa = (A[0][3] - A[0][1] * pb - A[0][2] * pc) / A[0][0];
+      // residual statistics from the moments
+      const double sz = acc[0], szz = acc[1], sxz = acc[2], syz = acc[3];
+      const double sum_res = sz - pa * s1 - pb * s1 - pc * cnt;
+      const double sum_plane2 = pa * pa * s2 + pb * pb * s2 + pc * pc * cnt + 2.0 * pa * pb * sxy +
+                                2.0 * pa * pc * s1 + 2.0 * pb * pc * s1;
+      const double sum_res2 = szz - 2.0 * (pa * sxz + pb * syz + pc * sz) + sum_plane2;
+      mean = sum_res / cnt;
+      double var = sum_res2 / cnt - mean * mean;
+      var = var > 0.0 ? var : 0.0;
+      const double sd = sqrt(var);
+      inv_std = sd > 1e-12 ? 1.0 / sd : 1.0;
+    }
+    const float fa = (float)pa, fb = (float)pb, fc = (float)pc, fmean = (float)mean;
+    const float fis = (float)inv_std;
+
+    // ---- phase 8: normalise, mask, forward row FFT (N) -------------------
+    for (int y0 = 0; y0 < N; y0 += 2 * T::RB6) {
+      const int nrow = min(2 * T::RB6, N - y0);
+      const int nseq = (nrow + 1) / 2;
+      for (int t = tid; t < nseq * N; t += nthr) {
+        const int q = t / N, X = t - q * N;
+        const int ya = y0 + 2 * q, yb = ya + 1;
+        float va = Z[(size_t)ya * N + X];
+        va = ((va - ((fa * (float)ya + fb * (float)X) + fc)) - fmean) * fis;
+        if (mask) va *= mask[(size_t)ya * N + X];
+        float vb = 0.f;
+        if (yb < y0 + nrow) {
+          vb = Z[(size_t)yb * N + X];
+          vb = ((vb - ((fa * (float)yb + fb * (float)X) + fc)) - fmean) * fis;
+          if (mask) vb *= mask[(size_t)yb * N + X];
+        }
+        bufA[q * N + X] = make_float2(va, vb);
+      }
+      __syncthreads();
+      float2* res = fft_lds<N, false>(bufA, bufB, nseq, N, tw1, tid, nthr);
+      for (int t = tid; t < nseq * MH; t += nthr) {
+        const int q = t / MH, k = t - q * MH;
+        const float2 z = res[q * N + k];
+        const float2 zc = cconj(res[q * N + (N - k) % N]);
+        const int ya = y0 + 2 * q;
+        W[(size_t)ya * MH + k] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+        if (ya + 1 < y0 + nrow)
+          W[(size_t)(ya + 1) * MH + k] = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+      }
+      __syncthreads();
+    }
+
+    // ---- phase 9: forward column FFT (N), crop to n, pack, FSC -----------
+    float2* out = bank + (size_t)d * n * mh_n;
+    double nrm[1] = {0.0};
+    const int kmax = n / 2;  // columns 0..n/2 are needed
+    for (int k0 = 0; k0 <= kmax; k0 += T::TK3) {
+      const int nk = min(T::TK3, kmax + 1 - k0);
+      for (int t = tid; t < nk * N; t += nthr) {
+        const int Y = t / nk, q = t - Y * nk;
+        bufA[q * (N + 1) + Y] = W[(size_t)Y * MH + k0 + q];
+      }
+      __syncthreads();
+      float2* res = fft_lds<N, false>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
+      for (int t = tid; t < nk * n; t += nthr) {
+        const int r = t / nk, q = t - r * nk;
+        const int c = k0 + q;
+        const int fy = r < n / 2 ? r : r - n;
+        float2 v;
+        if (c < n / 2 || n == N) {
+          v = res[q * (N + 1) + (fy < 0 ? fy + N : fy)];
+        } else {  // packed column n/2 holds frequency -n/2 = conj of (+n/2) at -fy
+          const int my = -fy;
+          v = cconj(res[q * (N + 1) + (my < 0 ? my + N : my)]);
+        }
+        const float g = fsc ? fsc[(size_t)r * mh_n + c] : 1.0f;
+        v.x *= g;
+        v.y *= g;
+        out[(size_t)r * mh_n + c] = v;
+        nrm[0] += (double)v.x * v.x + (double)v.y * v.y;
+      }
+      __syncthreads();
+    }
+
+    // ---- phase 10: unit L2 norm over the packed half plane ---------------
+    block_sum_f64<1>(nrm, red);
+    const float norm = (float)sqrt(nrm[0]);
+    for (int t = tid; t < n * mh_n; t += nthr) {
+      float2 v = out[t];
+      out[t] = make_float2(v.x / norm, v.y / norm);
+    }
+    if (norms && tid == 0) norms[d] = norm;
+  }
+}
+
+template <int N>
+int launch_build_templates(const float2* D, const float* rts, int M, int n, const float* mask,
+                           const float* fsc, float2* bank, float* norms, void* ws, size_t ws_bytes,
+                           hipStream_t st) {
+  using T = TplLayout<N>;
+  const size_t slot = T::SLOT * sizeof(float2);
+  int grid = (int)std::min<size_t>((size_t)M, ws_bytes / slot);
+  grid = std::min(grid, 4 * gis_cu_count());
+  GIS_REQUIRE(grid >= 1, "gis_build_templates: workspace of %zu bytes is below one slot (%zu)", ws_bytes, slot);
+  hipLaunchKernelGGL(build_templates_kernel<N>, dim3(grid), dim3(kThreads), 0, st, D, rts, M, n, mask, fsc,
+                     bank, norms, (float2*)ws);
+  GIS_LAUNCH_CHECK("gis_build_templates");
+  return GIS_OK;
+}
+
+template <int N>
+size_t tpl_slot_bytes() { return TplLayout<N>::SLOT * sizeof(float2); }
+
+}  // namespace
+
+static size_t slot_bytes_for(int N) {
+  switch (N) {
+    case 32: return tpl_slot_bytes<32>();
+    case 64: return tpl_slot_bytes<64>();
+    case 96: return tpl_slot_bytes<96>();
+    case 128: return tpl_slot_bytes<128>();
+    case 192: return tpl_slot_bytes<192>();
+    case 256: return tpl_slot_bytes<256>();
+    default: return 0;
+  }
+}
+
+// Workspace bytes that let gis_build_templates run `M` directions with full
+// parallelism (one scratch slot per resident workgroup).
+GIS_API size_t gis_build_templates_workspace(int N, int M) {
+  const size_t s = slot_bytes_for(N);
+  if (s == 0 || M <= 0) return 0;
+  const size_t g = std::min<size_t>((size_t)M, (size_t)4 * gis_cu_count());
+  return g * s;
+}
+
+// Replaces the per-direction loop of IntegratedProjector.build_templates
+// (v606:1077-1147).
+//   dft   [2N][2N][N+1] complex64 from gis_volume_dft
+//   rts   [M][3][3] float32 rotation matrices R = Euler(rot, tilt, 0)^T
+//   mask  [N][N] float32 real-space mask, or NULL (--discardMask)
+//   fsc   [n][n/2+1] float32 packed FSC weights, or NULL (all ones)
+//   bank  [M][n][n/2+1] complex64 output (unit L2 norm, UNshifted rfft2 layout)
+//   norms [M] float32 output of the pre-normalisation norms, or NULL
+GIS_API int gis_build_templates(const void* dft, int N, int n, const float* rts, int M, const float* mask,
+                                const float* fsc, void* bank, float* norms, void* ws, size_t ws_bytes,
+                                void* stream) {
+  GIS_REQUIRE(dft && rts && bank && ws, "gis_build_templates: null pointer");
+  GIS_REQUIRE(M >= 0, "gis_build_templates: M < 0");
+  GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0, "gis_build_templates: newbox %d must be even and <= box %d", n, N);
+  if (M == 0) return GIS_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const float2* D = (const float2*)dft;
+  float2* B = (float2*)bank;
+  switch (N) {
+    case 32: return launch_build_templates<32>(D, rts, M, n, mask, fsc, B, norms, ws, ws_bytes, st);
+    case 64: return launch_build_templates<64>(D, rts, M, n, mask, fsc, B, norms, ws, ws_bytes, st);
+    case 96: return launch_build_templates<96>(D, rts, M, n, mask, fsc, B, norms, ws, ws_bytes, st);
+    case 128: return launch_build_templates<128>(D, rts, M, n, mask, fsc, B, norms, ws, ws_bytes, st);
+    case 192: return launch_build_templates<192>(D, rts, M, n, mask, fsc, B, norms, ws, ws_bytes, st);
+    case 256: return launch_build_templates<256>(D, rts, M, n, mask, fsc, B, norms, ws, ws_bytes, st);
+    default:
+      gis_set_error("gis_build_templates: box %d not in {32, 64, 96, 128, 192, 256}", N);
+      return GIS_EUNSUPPORTED;
+  }
+}

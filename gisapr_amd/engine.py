@@ -1,0 +1,206 @@
+"""Device-resident candidate scoring: density -> template bank -> search -> score.
+
+``SearchEngine`` holds everything that is fixed per dataset on the HIP
+device (mask, packed FSC, direction matrices, CTF tables, correlation
+tables) and ``ParticleSet`` holds one particle stack's spectra, psi grids and
+local direction lists resident in HBM, so a candidate evaluation only runs
+the splat, the 3-D transform, the template bank, the correlation and the
+per-particle reduction.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import host, kernels
+from .density import splat_parameters
+
+
+@dataclass
+class ParticleSet:
+    images: torch.Tensor               # [B, N, N] float32 (device)
+    rot: np.ndarray
+    tilt: np.ndarray
+    psi: np.ndarray
+    line_index: np.ndarray
+    local: List[List[int]]             # per particle: global direction indices
+    fake_psi: np.ndarray               # [B, P] searched psi (record value)
+    eff_psi: np.ndarray                # [B, P] effective psi (0.5 deg quirk)
+    rotp: torch.Tensor                 # [B, P, 6]
+    defocus: Optional[torch.Tensor]    # [B, 3] or None
+    dir_off: torch.Tensor              # [B+1] int32
+    dir_idx: torch.Tensor              # [sum M_i] int32
+    spectra: Optional[torch.Tensor] = None   # [B, P, n, n/2+1] complex64
+    inv_norm: Optional[torch.Tensor] = None  # [B, P] float32
+
+    @property
+    def B(self) -> int:
+        return int(self.images.shape[0])
+
+    @property
+    def P(self) -> int:
+        return int(self.rotp.shape[1])
+
+    @property
+    def n_scores(self) -> int:
+        return int(self.dir_idx.shape[0]) * self.P
+
+    def nbytes_resident(self) -> int:
+        t = [self.images, self.rotp, self.dir_off, self.dir_idx, self.spectra, self.inv_norm, self.defocus]
+        return sum(x.numel() * x.element_size() for x in t if x is not None)
+
+
+@dataclass
+class SearchResult:
+    corr: kernels.CorrelationResult
+    score: torch.Tensor  # [1] float64: sum exp(rawCC / 1024)
+
+
+class SearchEngine:
+    """One search configuration (box, crop, psi sampling, CTF, mask, FSC, directions)."""
+
+    def __init__(self, *, box, new_box=None, apix, crop, psi_step, local_range=20.0,
+                 do_local_search=True, kk=0.0, voltage=300.0, cs=2.7, mask_radius=110, mask_edge=6,
+                 discard_mask=False, fsc_values=None, dir_rot, dir_tilt, device=None):
+        self.N = int(box)
+        self.n = int(self.N if new_box is None else new_box)
+        if self.N % 2 or self.n % 2:
+            raise ValueError("oriboxsize and newboxsize must be even")
+        if self.n > self.N:
+            raise ValueError("newboxsize cannot exceed oriboxsize")
+        if psi_step <= 0:
+            raise ValueError("psiStep must be positive")
+        if crop <= 0 or crop % 2 or crop > self.n:
+            raise ValueError("CCG crop size must be positive, even and <= newboxsize")
+        if self.N not in kernels.SUPPORTED_BOXES:
+            raise ValueError(f"box {self.N} is not one of {kernels.SUPPORTED_BOXES}")
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("gisapr_amd runs on HIP devices only")
+        self.apix = float(apix)
+        self.new_apix = self.N / self.n * self.apix
+        self.trans_scale = self.apix / self.new_apix
+        self.crop = int(crop)
+        self.psi_step = float(psi_step)
+        self.local_range = float(local_range)
+        self.do_local_search = bool(do_local_search)
+        self.dir_rot = np.asarray(dir_rot, np.float64)
+        self.dir_tilt = np.asarray(dir_tilt, np.float64)
+        dev = self.device
+        self.mask = None if discard_mask else torch.from_numpy(
+            host.generate_soft_mask((self.N, self.N), int(mask_radius), int(mask_edge))).to(dev)
+        fsc = np.ones(self.N, np.float32) if fsc_values is None else np.asarray(fsc_values, np.float32)
+        self.fsc_packed = torch.from_numpy(host.packed_fsc(fsc, self.n)).to(dev)
+        self.rts = torch.from_numpy(np.ascontiguousarray(host.rotation_matrices_t(self.dir_rot, self.dir_tilt))).to(dev)
+        self.ctf = host.CTFTables(self.n, self.new_apix, cs, voltage, kk)
+        self.ctf_tab = torch.from_numpy(self.ctf.table).to(dev)
+
+    # ---------------------------------------------------------------- particles
+    def add_particles(self, images, rot, tilt, psi, defocus=None, line_index=None,
+                      prepare=True) -> ParticleSet:
+        dev = self.device
+        images = torch.as_tensor(images, dtype=torch.float32).to(dev).contiguous()
+        B = int(images.shape[0])
+        if tuple(images.shape[1:]) != (self.N, self.N):
+            raise ValueError(f"particle images must be [B, {self.N}, {self.N}]")
+        rot, tilt, psi = (np.asarray(v, np.float64).reshape(B) for v in (rot, tilt, psi))
+        if self.do_local_search:
+            local = host.local_orientation_map(rot, tilt, self.dir_rot, self.dir_tilt, self.local_range)
+        else:
+            local = [list(range(len(self.dir_rot))) for _ in range(B)]
+        fakes, effs = [], []
+        for p in psi:
+            f, e = host.psi_sampling_for_record(float(p), self.psi_step, self.do_local_search, self.local_range)
+            fakes.append(f)
+            effs.append(e)
+        fake = np.asarray(fakes, np.float64)
+        eff = np.asarray(effs, np.float64)
+        rotp = np.stack([host.psi_grid_params(self.N, e) for e in effs]).astype(np.float32)
+        off = np.zeros(B + 1, np.int32)
+        off[1:] = np.cumsum([len(l) for l in local])
+        idx = np.asarray([k for l in local for k in l], np.int32)
+        dfo = None
+        if defocus is not None:
+            dfo = torch.from_numpy(np.ascontiguousarray(np.asarray(defocus, np.float32).reshape(B, 3))).to(dev)
+        ps = ParticleSet(
+            images=images, rot=rot, tilt=tilt, psi=psi,
+            line_index=np.arange(B) if line_index is None else np.asarray(line_index),
+            local=local, fake_psi=fake, eff_psi=eff,
+            rotp=torch.from_numpy(rotp).to(dev), defocus=dfo,
+            dir_off=torch.from_numpy(off).to(dev),
+            dir_idx=torch.from_numpy(idx if len(idx) else np.zeros(0, np.int32)).to(dev))
+        if prepare:
+            self.prepare(ps)
+        return ps
+
+    def prepare(self, ps: ParticleSet) -> ParticleSet:
+        ctf_tab = self.ctf_tab if ps.defocus is not None else None
+        ps.spectra, ps.inv_norm = kernels.prepare_particles(
+            ps.images, self.n, ps.rotp, self.mask, ctf_tab, ps.defocus, self.ctf.kconst,
+            out=ps.spectra, inv_norm=ps.inv_norm)
+        return ps
+
+    # ---------------------------------------------------------------- model
+    def density(self, coords, numbers, res, base: Optional[torch.Tensor] = None) -> torch.Tensor:
+        width, rp, kn = splat_parameters(self.apix, res)
+        xyz = torch.as_tensor(np.asarray(coords, np.float32)).to(self.device).contiguous()
+        zn = torch.as_tensor(np.asarray(numbers, np.float32)).to(self.device).contiguous()
+        vol = base.clone() if base is not None else None
+        return kernels.splat(xyz, zn, self.N, width, self.apix, rp, kn, vol)
+
+    def templates(self, volume: torch.Tensor, with_norms=False):
+        dft = kernels.volume_dft(volume)
+        return kernels.build_templates(dft, self.n, self.rts, self.mask, self.fsc_packed, with_norms)
+
+    # ---------------------------------------------------------------- search
+    def search(self, bank: torch.Tensor, ps: ParticleSet) -> SearchResult:
+        if ps.spectra is None:
+            self.prepare(ps)
+        if int(bank.shape[0]) != len(self.dir_rot):
+            raise ValueError("template bank does not cover the engine's direction table")
+        # dir_off / dir_idx were built on the host against this direction table
+        corr = kernels.correlate(bank, ps.spectra, ps.inv_norm, ps.dir_off, ps.dir_idx, self.crop,
+                                 validate=False)
+        return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best))
+
+    def score_volume(self, volume: torch.Tensor, ps: ParticleSet) -> SearchResult:
+        return self.search(self.templates(volume), ps)
+
+    # ---------------------------------------------------------------- records
+    def records(self, res: SearchResult, ps: ParticleSet) -> List[dict]:
+        """Per-particle results (make_result_records fields, v606:1452-1516)."""
+        c = res.corr
+        best = c.best.cpu().numpy()
+        peak = c.peak.cpu().numpy()
+        mean = c.mean.cpu().numpy()
+        std = c.std.cpu().numpy()
+        py = c.peak_y.cpu().numpy()
+        px = c.peak_x.cpu().numpy()
+        off = ps.dir_off.cpu().numpy()
+        idx = ps.dir_idx.cpu().numpy()
+        P = ps.P
+        out = []
+        for b in range(ps.B):
+            if best[b] < 0:
+                continue
+            li, pi = divmod(int(best[b]), P)
+            g = (off[b] + li) * P + pi
+            gi = int(idx[off[b] + li])
+            ad = host.calculate_angular_distance(self.dir_rot[gi], self.dir_tilt[gi], 0.0,
+                                                 float(ps.rot[b]), float(ps.tilt[b]), 0.0)
+            z = (float(peak[b]) - mean[b]) / std[b] if std[b] > 0 else 0.0
+            yy, xx = int(py.reshape(-1)[g]), int(px.reshape(-1)[g])
+            out.append({
+                "line": int(ps.line_index[b]), "z": float(z), "l3": 1 if z > 3.0 else 0,
+                "accu": 1 if ad < 11.0 else 0, "rot": float(self.dir_rot[gi]), "tilt": float(self.dir_tilt[gi]),
+                "angdist": ad,
+                "xshift": round(float(self.crop // 2 - xx) / self.trans_scale),
+                "yshift": round(float(self.crop // 2 - yy) / self.trans_scale),
+                "psi_fake": float(ps.fake_psi[b, pi]), "psi_prior": float((ps.psi[b] + 360.0) % 360.0),
+                "rawcc": float(peak[b]), "best_local": li, "best_psi": pi, "peak_y": yy, "peak_x": xx,
+                "direction": gi, "mean": float(mean[b]), "std": float(std[b]),
+            })
+        return out

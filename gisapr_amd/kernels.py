@@ -1,0 +1,155 @@
+"""Typed wrappers of the C ABI for torch tensors on a HIP device.
+
+Each function validates shapes on the host, allocates outputs and workspace
+with the torch caching allocator, and launches on the current torch stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from ._lib import check, lib, ptr, require_hip, stream_of
+
+SUPPORTED_BOXES = (32, 64, 96, 128, 192, 256)
+
+
+def _dev(t):
+    return t.device
+
+
+def splat(xyz: torch.Tensor, numbers: torch.Tensor, box: int, width: int, apix: float, rp: float,
+          kn: float, volume: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Add the radial-exponential splat of the atoms into ``volume`` ([box]^3, zeroed if None)."""
+    require_hip(xyz, "xyz")
+    require_hip(numbers, "numbers")
+    if xyz.dtype != torch.float32 or numbers.dtype != torch.float32:
+        raise ValueError("xyz and numbers must be float32")
+    if xyz.ndim != 2 or xyz.shape[1] != 3 or numbers.shape != (xyz.shape[0],):
+        raise ValueError("xyz must be [n, 3] and numbers [n]")
+    if volume is None:
+        volume = torch.zeros((box, box, box), dtype=torch.float32, device=xyz.device)
+    require_hip(volume, "volume")
+    if tuple(volume.shape) != (box, box, box) or volume.dtype != torch.float32:
+        raise ValueError("volume must be float32 [box, box, box]")
+    check(lib().gis_splat(ptr(xyz), ptr(numbers), int(xyz.shape[0]), int(box), int(width), float(apix),
+                          float(rp), float(kn), ptr(volume), stream_of(xyz.device)), "gis_splat")
+    return volume
+
+
+def volume_dft(volume: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """[N]^3 float32 -> [2N, 2N, N+1] complex64 (pad, sinc^2, shift, rfftn)."""
+    require_hip(volume, "volume")
+    N = int(volume.shape[0])
+    if volume.ndim != 3 or len(set(volume.shape)) != 1 or volume.dtype != torch.float32:
+        raise ValueError("volume must be a cubic float32 tensor")
+    if out is None:
+        out = torch.empty((2 * N, 2 * N, N + 1), dtype=torch.complex64, device=volume.device)
+    check(lib().gis_volume_dft(ptr(volume), N, ptr(out), stream_of(volume.device)), "gis_volume_dft")
+    return out
+
+
+def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Optional[torch.Tensor],
+                    fsc_packed: Optional[torch.Tensor], with_norms: bool = False):
+    """Unit-norm packed template bank [M, n, n/2+1] complex64 (unshifted)."""
+    require_hip(dft, "dft")
+    N = int(dft.shape[0]) // 2
+    M = int(rts.shape[0])
+    require_hip(rts, "rts")
+    n = int(new_box)
+    bank = torch.empty((M, n, n // 2 + 1), dtype=torch.complex64, device=dft.device)
+    norms = torch.empty((M,), dtype=torch.float32, device=dft.device) if with_norms else None
+    if mask is not None:
+        require_hip(mask, "mask")
+        if tuple(mask.shape) != (N, N):
+            raise ValueError("mask must be [N, N]")
+    if fsc_packed is not None:
+        require_hip(fsc_packed, "fsc_packed")
+        if tuple(fsc_packed.shape) != (n, n // 2 + 1):
+            raise ValueError("fsc_packed must be [n, n/2+1]")
+    wsb = int(lib().gis_build_templates_workspace(N, M))
+    ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dft.device)
+    check(lib().gis_build_templates(ptr(dft), N, n, ptr(rts), M, ptr(mask), ptr(fsc_packed), ptr(bank),
+                                    ptr(norms), ptr(ws), wsb, stream_of(dft.device)), "gis_build_templates")
+    return (bank, norms) if with_norms else bank
+
+
+def prepare_particles(images: torch.Tensor, new_box: int, rotp: torch.Tensor, mask: Optional[torch.Tensor],
+                      ctf_tab: Optional[torch.Tensor], defocus: Optional[torch.Tensor], kconst: np.ndarray,
+                      out: Optional[torch.Tensor] = None, inv_norm: Optional[torch.Tensor] = None):
+    """Packed conjugated particle spectra [B, P, n, n/2+1] and 1/norms [B, P]."""
+    require_hip(images, "images")
+    B, N = int(images.shape[0]), int(images.shape[1])
+    P = int(rotp.shape[1])
+    n = int(new_box)
+    require_hip(rotp, "rotp")
+    if tuple(rotp.shape) != (B, P, 6):
+        raise ValueError("rotp must be [B, P, 6]")
+    if out is None:
+        out = torch.empty((B, P, n, n // 2 + 1), dtype=torch.complex64, device=images.device)
+    if inv_norm is None:
+        inv_norm = torch.empty((B, P), dtype=torch.float32, device=images.device)
+    kc = (C.c_float * 5)(*[float(v) for v in np.asarray(kconst, np.float32)])
+    wsb = int(lib().gis_prepare_particles_workspace(N, B * P))
+    ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=images.device)
+    check(lib().gis_prepare_particles(ptr(images), B, N, n, P, ptr(rotp), ptr(mask), ptr(ctf_tab),
+                                      ptr(defocus), C.cast(kc, C.c_void_p), ptr(out), ptr(inv_norm), ptr(ws),
+                                      wsb, stream_of(images.device)), "gis_prepare_particles")
+    return out, inv_norm
+
+
+@dataclass
+class CorrelationResult:
+    scores: torch.Tensor   # [sum M_i, P] float32
+    peak_y: torch.Tensor   # [sum M_i, P] int32
+    peak_x: torch.Tensor   # [sum M_i, P] int32
+    best: torch.Tensor     # [B] int32 (local_dir * P + psi, -1 if none)
+    peak: torch.Tensor     # [B] float32
+    mean: torch.Tensor     # [B] float64
+    std: torch.Tensor      # [B] float64
+
+
+def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torch.Tensor],
+              dir_off: torch.Tensor, dir_idx: torch.Tensor, crop: int, validate: bool = True) -> CorrelationResult:
+    require_hip(bank, "bank")
+    require_hip(spectra, "spectra")
+    B, P, n = int(spectra.shape[0]), int(spectra.shape[1]), int(spectra.shape[2])
+    if tuple(bank.shape[1:]) != (n, n // 2 + 1):
+        raise ValueError("bank and spectra layouts differ")
+    if dir_off.dtype != torch.int32 or dir_idx.dtype != torch.int32:
+        raise ValueError("dir_off / dir_idx must be int32")
+    if tuple(dir_off.shape) != (B + 1,):
+        raise ValueError("dir_off must be [B+1]")
+    total = int(dir_idx.shape[0])
+    if validate:
+        off = dir_off.cpu().numpy()
+        if off[0] != 0 or off[-1] != total or np.any(np.diff(off) < 0):
+            raise ValueError("dir_off must start at 0, be non-decreasing and end at len(dir_idx)")
+        if total and (int(dir_idx.min()) < 0 or int(dir_idx.max()) >= int(bank.shape[0])):
+            raise ValueError("dir_idx out of range of the template bank")
+    dev = spectra.device
+    scores = torch.empty((total, P), dtype=torch.float32, device=dev)
+    py = torch.empty((total, P), dtype=torch.int32, device=dev)
+    px = torch.empty((total, P), dtype=torch.int32, device=dev)
+    best = torch.empty((B,), dtype=torch.int32, device=dev)
+    peak = torch.empty((B,), dtype=torch.float32, device=dev)
+    mean = torch.empty((B,), dtype=torch.float64, device=dev)
+    std = torch.empty((B,), dtype=torch.float64, device=dev)
+    wsb = int(lib().gis_correlate_workspace(n))
+    ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dev)
+    check(lib().gis_correlate(ptr(bank), ptr(spectra), ptr(inv_norm), ptr(dir_off), ptr(dir_idx), B, P, n,
+                              int(crop), ptr(scores), ptr(py), ptr(px), ptr(best), ptr(peak), ptr(mean),
+                              ptr(std), ptr(ws), wsb, stream_of(dev)), "gis_correlate")
+    return CorrelationResult(scores, py, px, best, peak, mean, std)
+
+
+def score_sum(peak: torch.Tensor, best: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """sum exp(peak/1024) over particles with best >= 0, as a float64 device scalar."""
+    require_hip(peak, "peak")
+    out = torch.empty((1,), dtype=torch.float64, device=peak.device)
+    check(lib().gis_score_sum(ptr(peak), ptr(best), int(peak.shape[0]), ptr(out), stream_of(peak.device)),
+          "gis_score_sum")
+    return out

@@ -1,0 +1,130 @@
+/*
+ * gisapr_hip.h -- C ABI of libgisapr_hip.so, the gfx950 (MI355X) engine for
+ * GisAPR's candidate-scoring path.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers (hipMalloc / torch CUDA tensors)
+ *    unless stated; sizes are element counts.  Complex arrays are
+ *    interleaved float32 (re, im) = complex64.
+ *  - `stream` is a hipStream_t (NULL = default stream).  Every call is
+ *    asynchronous on that stream; nothing here synchronises.
+ *  - Return value: 0 on success, > 0 a hipError_t, < 0 an argument error
+ *    (-1 invalid, -2 unsupported size).  gis_last_error() gives the message.
+ *  - Box sizes compiled: N in {32, 64, 96, 128, 192, 256}.  The packed
+ *    Fourier layout of an n x n image is the rfft2 layout [n][n/2+1] with
+ *    rows in natural frequency order (v606:494-541).  Banks and particle
+ *    spectra are stored UNshifted; the reference's fftshift "pre-shift"
+ *    (v606:535-537) is folded into the correlation tables.
+ *
+ * Reference files cited: v606 = test1_with_isspa_weight_varingKK_search_
+ * translation_also_v606_torch_optimized_standalone.py; p2m =
+ * pdb2mrc_gpu_ver_fp32_v3.py; ZL = GisAPR-Lite-v1.3.2.zip/GisAPR-210-DE-v1.3.2.
+ */
+#ifndef GISAPR_HIP_H
+#define GISAPR_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failing call on this thread. */
+const char* gis_last_error(void);
+/* ABI version (1). */
+int gis_version(void);
+/* Makes `device` current for the calling thread. */
+int gis_init(int device);
+
+/*
+ * (a) Atom -> voxel splat.  Replaces ZL:gisapr_cuda_density.py:21-22
+ * deposit(xyz, numbers, volume, n, box, width, apix, rp, kn) and the
+ * index_add_ loop of density_from_coords (p2m:41-98).  ADDS into `volume`
+ * ([box][box][box] float32, [z][y][x]); the caller zeroes it or pre-fills it
+ * with a resident fixed-body density.  xyz [n][3] float32 (A), numbers [n]
+ * float32 atomic numbers (0 = skipped); width/rp/kn as p2m:53-58.
+ */
+int gis_splat(const float* xyz, const float* numbers, int n_atoms, int box, int width,
+              double apix, double rp, double kn, float* volume, void* stream);
+
+/*
+ * (b, prefix) Pad / sinc^2 / fftshift / rfftn of the model volume.
+ * Replaces IntegratedProjector.build_templates v606:1068-1073.
+ * volume [N][N][N] float32 -> dft [2N][2N][N+1] complex64 (natural order).
+ */
+int gis_volume_dft(const float* volume, int N, void* dft, void* stream);
+
+/* Workspace bytes gis_build_templates needs for M directions at box N. */
+size_t gis_build_templates_workspace(int N, int M);
+
+/*
+ * (b) Template bank.  Replaces the per-direction loop of
+ * IntegratedProjector.build_templates (v606:1077-1147): central-slice
+ * gather with Friedel conjugation, irfft2, crop, whitening
+ * (ProjectionWhitening batch==1 path v606:938-992), mask, fft2, crop to n,
+ * pack, x FSC, unit L2 norm.
+ *   dft  [2N][2N][N+1] complex64 from gis_volume_dft
+ *   rts  [M][3][3] float32: Euler(rot, tilt, 0)^T per direction (v606:214-215)
+ *   mask [N][N] float32 or NULL;  fsc [n][n/2+1] float32 or NULL (ones)
+ *   bank [M][n][n/2+1] complex64 out;  norms [M] float32 out or NULL
+ */
+int gis_build_templates(const void* dft, int N, int n, const float* rts, int M, const float* mask,
+                        const float* fsc, void* bank, float* norms, void* ws, size_t ws_bytes,
+                        void* stream);
+
+/* Workspace bytes gis_prepare_particles needs for `images` (= B*P) images. */
+size_t gis_prepare_particles_workspace(int N, int images);
+
+/*
+ * (b') Particle preparation.  Replaces ParticleFourierPreparer.prepare
+ * (v606:1340-1374) with PackedCTFTorch.batch (v606:778-829) fused in,
+ * batched over B particles x P psi angles.
+ *   images  [B][N][N] float32
+ *   rotp    [B][P][6] float32 psi sampling grids (c, -s, s, c, off_y, off_x),
+ *           source = M (y, x) + off, as PsiGridBank v606:1213-1239
+ *   mask    [N][N] float32 or NULL
+ *   ctf_tab with defocus != NULL: [4][n][n/2+1] float32 (x, y, u4, ncurve of
+ *           PackedCTFTorch.__init__), the CTF is evaluated per particle;
+ *           with defocus == NULL: [B][n][n/2+1] float32 explicit packed CTF
+ *           (e.g. PackedCTFNumPy output), or NULL for no CTF (ones)
+ *   defocus [B][3] float32 (dfU, dfV, angle deg) or NULL
+ *   kconst  HOST [5] float32: k1, k2, k3, k5, kk
+ *   spectra [B][P][n][n/2+1] complex64 out (x CTF, conjugated, unnormalised)
+ *   inv_norm [B][P] float32 out: 1 / ||spectrum||_2
+ */
+int gis_prepare_particles(const float* images, int B, int N, int n, int P, const float* rotp,
+                          const float* mask, const float* ctf_tab, const float* defocus,
+                          const float* kconst, void* spectra, float* inv_norm, void* ws,
+                          size_t ws_bytes, void* stream);
+
+/* Workspace bytes gis_correlate needs at newbox n. */
+size_t gis_correlate_workspace(int n);
+
+/*
+ * (d) FFT cross-correlation search + per-particle reduction.  Replaces
+ * correlate_templates_and_particle (v606:1418-1449) and the numerics of
+ * make_result_records (v606:1452-1491), batched over B particles.  Particle
+ * b is correlated against bank rows dir_idx[dir_off[b] .. dir_off[b+1]).
+ *   scores/peak_y/peak_x [dir_off[B]][P]: score = max(ccg crop) * 1024 n^2,
+ *                        (y, x) = divmod(argmax, C) in the C x C crop
+ *   best [B]: flat argmax over (local direction, psi) or -1 if none
+ *   peak [B] float32, mean/std [B] float64 (population std)
+ * C (--ccgCropSize) must be even and <= 16.
+ */
+int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, const int* dir_off,
+                  const int* dir_idx, int B, int P, int n, int C, float* scores, int* peak_y,
+                  int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
+                  size_t ws_bytes, void* stream);
+
+/*
+ * Cross-particle conformation score sum_b exp(peak_b / 1024) in float64 over
+ * particles with best_b >= 0 (best may be NULL).  Replaces
+ * new_method_to_fit_the_2nd_Gaussian_PEAK_v4.py:133-134.  out: 1 double.
+ */
+int gis_score_sum(const float* peak, const int* best, int B, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GISAPR_HIP_H */
