@@ -333,14 +333,20 @@ def make_workload(name_or_wl, n_particles=None, device="cpu"):
     dfv = rng.uniform(10000.0, 25000.0, B)
     dfa = rng.uniform(0.0, 180.0, B)
     shifts = np.random.default_rng(SEED_SHIFT).integers(-wl.max_shift, wl.max_shift + 1, (B, 2))
-    img = project_atoms(coords, numbers, rot, tilt, psi, shifts, wl.box, wl.apix, wl.res, device)
-    ctf = ctf_physical(wl.box, wl.apix, dfu, dfv, dfa, wl.voltage, wl.cs, 0.1, device)
-    img = torch.fft.irfft2(torch.fft.rfft2(img) * ctf, s=(wl.box, wl.box))
-    var = img.var(dim=(-2, -1), keepdim=True)
+    img = torch.empty((B, wl.box, wl.box), dtype=torch.float32, device=device)
     gen = torch.Generator(device="cpu").manual_seed(SEED_NOISE)
-    noise = torch.randn(img.shape, generator=gen, dtype=torch.float32).to(device)
-    img = img + noise * torch.sqrt(var / wl.snr)
-    img = whiten_batch(img)
+    chunk = max(1, min(B, (64 << 20) // (wl.box * wl.box * 8)))
+    for s in range(0, B, chunk):
+        e = min(B, s + chunk)
+        sl = slice(s, e)
+        im = project_atoms(coords, numbers, rot[sl], tilt[sl], psi[sl], shifts[sl], wl.box, wl.apix,
+                           wl.res, device)
+        ctf = ctf_physical(wl.box, wl.apix, dfu[sl], dfv[sl], dfa[sl], wl.voltage, wl.cs, 0.1, device)
+        im = torch.fft.irfft2(torch.fft.rfft2(im) * ctf, s=(wl.box, wl.box))
+        var = im.var(dim=(-2, -1), keepdim=True)
+        noise = torch.randn(im.shape, generator=gen, dtype=torch.float32).to(device)
+        im = im + noise * torch.sqrt(var / wl.snr)
+        img[sl] = whiten_batch(im)
     fsc = None if wl.fsc_resolution is None else fsc_logistic(wl.box, wl.apix, wl.fsc_resolution)
     return {
         "workload": wl, "coords": coords, "elements": elements, "chains": chains,

@@ -1,0 +1,83 @@
+"""Pins the oracle's packed layout, pre-shift and correlation operator
+(v606:494-541, 1418-1449) with independent derivations and brute force."""
+import numpy as np
+import pytest
+
+from oracle import fourier as ofr
+
+
+@pytest.mark.parametrize("n", [8, 16, 32, 96])
+def test_packed_layout_is_rfft2(n):
+    x = np.random.default_rng(n).standard_normal((n, n))
+    packed = ofr.centered_full_to_packed(np.fft.fftshift(np.fft.fft2(x)))
+    np.testing.assert_allclose(packed, np.fft.rfft2(x), atol=1e-10)
+
+
+def test_pre_shift_rolls():
+    n = 16
+    a = np.arange(n * (n // 2 + 1)).reshape(n, n // 2 + 1)
+    s = ofr.pre_shift(a)
+    assert s[0, 0] == a[n // 2, (n // 2 + 1) - (n // 2 + 1) // 2]  # rows by n/2, cols by (n/2+1)//2
+    np.testing.assert_array_equal(np.fft.ifftshift(s, axes=(-2, -1)), a)
+
+
+def test_crop_indices():
+    np.testing.assert_array_equal(ofr.crop_indices(16, 4), [14, 15, 0, 1])
+
+
+@pytest.mark.parametrize("n,C", [(16, 4), (16, 8), (32, 10)])
+def test_literal_operator_equals_unshifted_derivation(n, C):
+    """irfft2 of the pre-shifted product == (-1)^ly row sign and the rolled C2R
+    column weights applied to the UNshifted product (the form the HIP kernel uses)."""
+    rng = np.random.default_rng(n + C)
+    m = n // 2 + 1
+    T = rng.standard_normal((3, n, m)) + 1j * rng.standard_normal((3, n, m))
+    P = rng.standard_normal((2, n, m)) + 1j * rng.standard_normal((2, n, m))
+    sc, ys, xs = ofr.correlate(ofr.pre_shift(T), ofr.pre_shift(P), C, np.float64)
+    phi = ofr.correlation_functional_unshifted(n, C)
+    for a in range(3):
+        for b in range(2):
+            ccg = np.einsum("rc,rcij->ij", T[a] * P[b], phi).real
+            k = int(np.argmax(ccg))
+            assert sc[a, b] == pytest.approx(ccg.ravel()[k] * 1024 * n * n, rel=1e-10)
+            assert (ys[a, b], xs[a, b]) == divmod(k, C)
+
+
+def test_literal_operator_equals_direct_functional():
+    n, C = 16, 6
+    rng = np.random.default_rng(5)
+    m = n // 2 + 1
+    T = rng.standard_normal((2, n, m)) + 1j * rng.standard_normal((2, n, m))
+    P = rng.standard_normal((2, n, m)) + 1j * rng.standard_normal((2, n, m))
+    sc, _, _ = ofr.correlate(T, P, C, np.float64)
+    direct = ofr.correlate_direct(T, P, C)
+    np.testing.assert_allclose(sc, direct.reshape(2, 2, -1).max(-1) * 1024 * n * n, rtol=1e-10)
+
+
+def test_without_preshift_operator_is_textbook_cross_correlation():
+    """Unshifted operands reduce the operator to the circular cross-correlation
+    of the real images, sampled at lags [-C/2, C/2), times 1024 (scale check)."""
+    n, C = 16, 6
+    rng = np.random.default_rng(9)
+    a = rng.standard_normal((n, n))
+    b = np.roll(a, (2, -1), axis=(0, 1)) + 0.01 * rng.standard_normal((n, n))
+    T = np.fft.rfft2(a)[None]
+    Pc = np.conj(np.fft.rfft2(b))[None]
+    sc, ys, xs = ofr.correlate(T, Pc, C, np.float64)
+    # brute force: irfft2(A conj B)[d] = sum_y a(y + d) b(y)
+    lags = ofr.crop_indices(n, C)
+    cc = np.array([[np.sum(np.roll(a, (-dy, -dx), axis=(0, 1)) * b) for dx in lags] for dy in lags])
+    k = int(np.argmax(cc))
+    assert sc[0, 0] == pytest.approx(cc.ravel()[k] * 1024 * n * n, rel=1e-10)
+    assert (ys[0, 0], xs[0, 0]) == divmod(k, C)
+    # the peak sits at the lag that undoes the (2, -1) roll: dy = -2, dx = +1
+    assert (lags[ys[0, 0]] - n if lags[ys[0, 0]] >= n // 2 else lags[ys[0, 0]]) == -2
+    assert lags[xs[0, 0]] == 1
+
+
+def test_first_maximum_wins_ties():
+    n, C = 8, 4
+    T = np.zeros((1, n, n // 2 + 1), complex)
+    P = np.zeros((1, n, n // 2 + 1), complex)
+    sc, ys, xs = ofr.correlate(T, P, C, np.float64)
+    assert (ys[0, 0], xs[0, 0]) == (0, 0) and sc[0, 0] == 0.0
