@@ -21,8 +21,10 @@
 // leaves registers.  Q = T*P is formed on the fly from LDS-staged tiles.
 //
 // Work decomposition: one 256-thread workgroup per particle; a sweep covers
-// up to 32 (direction, psi) pairs, 8 per wave; each sweep streams the
-// spectra in (16-row x 16-column) tiles through LDS once for all its pairs.
+// up to 32 (direction, psi) pairs, 8 per wave.  The W and E tables live in
+// LDS for the whole workgroup; the spectra stream through LDS in
+// (16-row x 16-column) tiles, double-buffered: the global loads of chunk
+// r+1 are in flight (in registers) while the MFMAs consume chunk r.
 // The per-particle tail (make_result_records numerics, v606:1452-1491) runs
 // in the same workgroup: first-maximum argmax over (direction, psi), mean
 // and population std in float64.
@@ -35,8 +37,9 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWaves = 4;
 constexpr int kPairsPerWave = 8;
-constexpr int kRC = 16;  // rows per staged chunk
-constexpr int kTileElems = kRC * 16;
+constexpr int kRC = 16;                 // rows per staged chunk
+constexpr int kTileElems = kRC * 16;    // one float2 per thread per tile
+constexpr int kMaxTiles = 16;           // templates + psi images per sweep
 
 __device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -86,16 +89,25 @@ correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spe
                  float* __restrict__ scores, int* __restrict__ peak_y, int* __restrict__ peak_x,
                  int* __restrict__ best, float* __restrict__ peak, double* __restrict__ mean,
                  double* __restrict__ stdv) {
-  extern __shared__ float2 tiles[];  // [JT + PT][kRC][16]
+  extern __shared__ float2 smem[];
+  __shared__ const float2* tptr[kMaxTiles];
   const int b = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m = n / 2 + 1;
   const int nct = (m + 15) / 16;
   const int nrc = (n + kRC - 1) / kRC;
+  const int nrow_pad = nrc * kRC;
+  float2* Wl = smem;                          // [nrow_pad][16]
+  float2* El = Wl + nrow_pad * 16;            // [nct*16][16]
+  float2* tiles = El + nct * 16 * 16;         // [2][kMaxTiles][kTileElems]
   const int d0 = dir_off[b];
   const int Mi = dir_off[b + 1] - d0;
   const size_t plane = (size_t)n * m;
   const int lr = lane >> 4, lc = lane & 15;
+  const int srow = tid >> 4, scol = tid & 15;  // staging element of this thread
+
+  for (int i = tid; i < nrow_pad * 16; i += kThreads) Wl[i] = Wt[i];
+  for (int i = tid; i < nct * 256; i += kThreads) El[i] = Et[i];
 
   for (int j0 = 0; j0 < Mi; j0 += JT) {
     const int jt = min(JT, Mi - j0);
@@ -103,20 +115,40 @@ correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spe
       const int pt = min(PT, P - p0);
       const int sweep = jt * pt;
       const int ntile = jt + pt;
-      floatx4 outv[kPairsPerWave];
-#pragma unroll
-      for (int i = 0; i < kPairsPerWave; ++i) outv[i] = floatx4{0.f, 0.f, 0.f, 0.f};
-
-      // LDS offsets of each pair's template and psi tiles (fixed per sweep)
+      __syncthreads();
+      if (tid < ntile)
+        tptr[tid] = tid < jt ? bank + (size_t)dir_idx[d0 + j0 + tid] * plane
+                             : spectra + ((size_t)b * P + p0 + (tid - jt)) * plane;
+      __syncthreads();
+      // Pair slots beyond the sweep alias its last pair, so the MFMA loop has
+      // no per-pair branches (branches force accumulator copies between
+      // AGPRs and VGPRs); their results are never written.
       int toff[kPairsPerWave], poff[kPairsPerWave];
 #pragma unroll
       for (int i = 0; i < kPairsPerWave; ++i) {
-        const int k = wave + kWaves * i;
+        const int k = min(wave + kWaves * i, sweep - 1);
         const int j = k / pt, p = k - j * pt;
         toff[i] = j * kTileElems;
         poff[i] = (jt + p) * kTileElems;
       }
+      floatx4 outv[kPairsPerWave];
+#pragma unroll
+      for (int i = 0; i < kPairsPerWave; ++i) outv[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+
       for (int ct = 0; ct < nct; ++ct) {
+        const int c = ct * 16 + scol;
+        float2 pre[kMaxTiles];
+        // prologue: chunk 0 -> buffer 0
+        {
+          const bool ok = srow < n && c < m;
+#pragma unroll
+          for (int t = 0; t < kMaxTiles; ++t)
+            pre[t] = (t < ntile && ok) ? tptr[t][(size_t)srow * m + c] : make_float2(0.f, 0.f);
+#pragma unroll
+          for (int t = 0; t < kMaxTiles; ++t)
+            if (t < ntile) tiles[t * kTileElems + tid] = pre[t];
+        }
+        __syncthreads();
         floatx4 gr[kPairsPerWave], gi[kPairsPerWave];
 #pragma unroll
         for (int i = 0; i < kPairsPerWave; ++i) {
@@ -124,52 +156,55 @@ correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spe
           gi[i] = floatx4{0.f, 0.f, 0.f, 0.f};
         }
         for (int rc = 0; rc < nrc; ++rc) {
-          __syncthreads();
-          for (int e = tid; e < ntile * kTileElems; e += kThreads) {
-            const int tile = e / kTileElems, w = e - tile * kTileElems;
-            const int row = w >> 4, col = w & 15;
-            const int r = rc * kRC + row, c = ct * 16 + col;
-            float2 v = make_float2(0.f, 0.f);
-            if (r < n && c < m) {
-              const float2* src = tile < jt ? bank + (size_t)dir_idx[d0 + j0 + tile] * plane
-                                            : spectra + ((size_t)b * P + p0 + (tile - jt)) * plane;
-              v = src[(size_t)r * m + c];
-            }
-            tiles[e] = v;
+          const int cur = rc & 1;
+          const bool more = rc + 1 < nrc;
+          if (more) {  // issue the next chunk's global loads; consumed after the MFMAs
+            const int r = (rc + 1) * kRC + srow;
+            const bool ok = r < n && c < m;
+#pragma unroll
+            for (int t = 0; t < kMaxTiles; ++t)
+              pre[t] = (t < ntile && ok) ? tptr[t][(size_t)r * m + c] : make_float2(0.f, 0.f);
           }
-          __syncthreads();
+          const float2* tb = tiles + cur * (kMaxTiles * kTileElems);
 #pragma unroll 1
           for (int ks = 0; ks < kRC / 4; ++ks) {
             const int base = (ks * 4 + lr) * 16 + lc;
-            const float2 wv = Wt[(size_t)(rc * kRC) * 16 + base];
+            const float2 wv = Wl[(rc * kRC) * 16 + base];
             const float nwi = -wv.y;
+            float qr[kPairsPerWave], qi[kPairsPerWave];
 #pragma unroll
             for (int i = 0; i < kPairsPerWave; ++i) {
-              if (wave + kWaves * i < sweep) {
-                const float2 t = tiles[toff[i] + base];
-                const float2 pp = tiles[poff[i] + base];
-                const float qr = t.x * pp.x - t.y * pp.y;
-                const float qi = t.x * pp.y + t.y * pp.x;
-                gr[i] = mfma(qr, wv.x, gr[i]);
-                gr[i] = mfma(qi, nwi, gr[i]);
-                gi[i] = mfma(qr, wv.y, gi[i]);
-                gi[i] = mfma(qi, wv.x, gi[i]);
-              }
+              const float2 t = tb[toff[i] + base];
+              const float2 pp = tb[poff[i] + base];
+              qr[i] = t.x * pp.x - t.y * pp.y;
+              qi[i] = t.x * pp.y + t.y * pp.x;
             }
+#pragma unroll
+            for (int i = 0; i < kPairsPerWave; ++i) gr[i] = mfma(qr[i], wv.x, gr[i]);
+#pragma unroll
+            for (int i = 0; i < kPairsPerWave; ++i) gi[i] = mfma(qr[i], wv.y, gi[i]);
+#pragma unroll
+            for (int i = 0; i < kPairsPerWave; ++i) gr[i] = mfma(qi[i], nwi, gr[i]);
+#pragma unroll
+            for (int i = 0; i < kPairsPerWave; ++i) gi[i] = mfma(qi[i], wv.x, gi[i]);
           }
+          if (more) {
+            float2* nb = tiles + (cur ^ 1) * (kMaxTiles * kTileElems);
+#pragma unroll
+            for (int t = 0; t < kMaxTiles; ++t)
+              if (t < ntile) nb[t * kTileElems + tid] = pre[t];
+          }
+          __syncthreads();
         }
         // column-to-lag step: out^T += Re(E^T G^T) for this 16-column tile
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const float2 ev = Et[(size_t)(ct * 16 + 4 * lr + v) * 16 + lc];
+          const float2 ev = El[(ct * 16 + 4 * lr + v) * 16 + lc];
           const float ner = ev.x, nei = -ev.y;
 #pragma unroll
           for (int i = 0; i < kPairsPerWave; ++i) {
-            const int k = wave + kWaves * i;
-            if (k < sweep) {
-              outv[i] = mfma(ner, gr[i][v], outv[i]);
-              outv[i] = mfma(nei, gi[i][v], outv[i]);
-            }
+            outv[i] = mfma(ner, gr[i][v], outv[i]);
+            outv[i] = mfma(nei, gi[i][v], outv[i]);
           }
         }
       }
@@ -263,6 +298,10 @@ __global__ void score_sum_kernel(const float* __restrict__ peak, const int* __re
 inline int row_pad(int n) { return ((n + kRC - 1) / kRC) * kRC; }
 inline int col_pad(int n) { return ((n / 2 + 1 + 15) / 16) * 16; }
 
+size_t corr_lds_bytes(int n) {
+  return (size_t)(row_pad(n) * 16 + col_pad(n) * 16 + 2 * kMaxTiles * kTileElems) * sizeof(float2);
+}
+
 }  // namespace
 
 GIS_API size_t gis_correlate_workspace(int n) {
@@ -283,7 +322,7 @@ GIS_API int gis_correlate(const void* bank, const void* spectra, const float* in
                           int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
                           size_t ws_bytes, void* stream) {
   GIS_REQUIRE(B >= 0 && P >= 1, "gis_correlate: B=%d P=%d", B, P);
-  GIS_REQUIRE(n >= 4 && n % 2 == 0, "gis_correlate: newbox %d must be even and >= 4", n);
+  GIS_REQUIRE(n >= 4 && n % 2 == 0 && n <= 256, "gis_correlate: newbox %d must be even, in [4, 256]", n);
   if (C > 16 || C < 2 || C % 2 != 0 || C > n) {
     gis_set_error("gis_correlate: crop %d must be even, in [2, 16] and <= newbox", C);
     return GIS_EUNSUPPORTED;
@@ -298,9 +337,20 @@ GIS_API int gis_correlate(const void* bank, const void* spectra, const float* in
   float2* Et = Wt + (size_t)row_pad(n) * 16;
   hipLaunchKernelGGL(build_corr_tables, dim3(64), dim3(256), 0, st, n, C, row_pad(n), col_pad(n), Wt, Et);
   GIS_LAUNCH_CHECK("gis_correlate tables");
-  const int PT = std::min(P, 16);
-  const int JT = std::max(1, std::min(16, (kWaves * kPairsPerWave) / PT));
-  const size_t lds = (size_t)(JT + PT) * kTileElems * sizeof(float2);
+  const int PT = std::min(P, 8);
+  const int JT = std::max(1, std::min(kMaxTiles - PT, (kWaves * kPairsPerWave) / PT));
+  const size_t lds = corr_lds_bytes(n);
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)correlate_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)corr_lds_bytes(256));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      gis_set_error("gis_correlate: cannot reserve %zu bytes of LDS: %s", corr_lds_bytes(256), hipGetErrorString(e));
+      return (int)e;
+    }
+    attr_set = true;
+  }
   hipLaunchKernelGGL(correlate_kernel, dim3(B), dim3(kThreads), lds, st, (const float2*)bank,
                      (const float2*)spectra, inv_norm, dir_off, dir_idx, P, n, C, JT, PT, Wt, Et, scores,
                      peak_y, peak_x, best, peak, mean, stdv);
