@@ -13,9 +13,11 @@
 // ([B][P][n][n/2+1]); inv_norm[b][p] = 1 / ||spectrum|| is applied by the
 // correlation kernel (the reference divides before correlating).
 //
-// gfx950 mapping: one 256-thread workgroup per (particle, psi) image on a
-// persistent grid; the row pass samples two rotated rows per complex FFT;
-// the column pass runs only over the n/2+1 columns that survive the crop.
+// gfx950 mapping: one 256-thread workgroup per particle (persistent grid).
+// Row pass: for each psi, two rotated rows per complex LDS FFT, written to
+// the workgroup's scratch slot.  Column pass: per 8-column tile the CTF is
+// evaluated once into LDS and applied to all psi images of the particle.
+// Tile widths are compile-time powers of two (no runtime divisions).
 #include "common.h"
 
 #pragma clang fp contract(off)
@@ -23,21 +25,34 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kLdsFloat2 = 8192;
+constexpr int kMaxPsiGroup = 8;  // psi images per pass (scratch slots per workgroup)
+constexpr int kTK = 8;           // columns per column tile
+constexpr int kRB = 8;           // complex rows (= 2 real rows) per row batch
 
+// Scratch layout of one row-transformed image: tile-major [NT][N][kTK] with
+// NT = ceil((N/2+1)/kTK), so a column tile is one contiguous 16 KiB block.
 template <int N>
 struct PrepLayout {
   static constexpr int MH = N / 2 + 1;
-  static constexpr int FIXED = N + 16;
-  static constexpr int CAP = (kLdsFloat2 - FIXED) / 2;
-  static constexpr int RB = CAP / N;          // complex rows per batch (2 real rows each)
-  static constexpr int TK = CAP / (N + 1);    // columns per tile
-  static constexpr size_t SLOT = (size_t)N * MH;  // float2
+  static constexpr int SP = N + 1;                 // padded column stride in LDS
+  static constexpr int NT = (MH + kTK - 1) / kTK;  // column tiles
+  static constexpr size_t IMG = (size_t)NT * N * kTK;            // float2 per image
+  static constexpr size_t SLOT = (size_t)kMaxPsiGroup * IMG;     // float2 per workgroup
 };
 
-// ctf_tab: [4][n][n/2+1] = x, y, u4, ncurve (float32, PackedCTFTorch.__init__)
-// defocus: [B][3] = dfU, dfV, angle (deg) or NULL (no CTF columns -> all ones)
-// rotp:    [B][P][6] = c, -s, s, c, off_y, off_x of the psi sampling grid
+typedef float f2a4 __attribute__((ext_vector_type(2), aligned(4)));
+
+__device__ __forceinline__ float ctf_weight(float x, float y, float u4, float nc, float axx, float axy, float ayy,
+                                            float k1, float k2, float k3, float k5, float kk) {
+  const float gamma = k1 * ((axx * x * x + 2.0f * axy * x * y) + ayy * y * y) + k2 * u4 - k5 - k3;
+  float cv = -sinf(gamma);
+  if (fabsf(cv) < 1e-8f) cv = cv > 0.f ? 1e-8f : (cv < 0.f ? -1e-8f : 0.f);
+  return cv * sqrtf(1.0f / (nc + kk * cv * cv));
+}
+
+// ctf_tab with defocus:  [4][n][n/2+1] = x, y, u4, ncurve (PackedCTFTorch.__init__)
+// ctf_tab without:       [B][n][n/2+1] explicit packed CTF, or NULL (ones)
+// rotp:                  [B][P][6] = c, -s, s, c, off_y, off_x of the psi sampling grid
 template <int N>
 __global__ void __launch_bounds__(kThreads)
 prepare_particles_kernel(const float* __restrict__ images, int B, int P, int n,
@@ -47,127 +62,163 @@ prepare_particles_kernel(const float* __restrict__ images, int B, int P, int n,
                          float2* __restrict__ spectra, float* __restrict__ inv_norm,
                          float2* __restrict__ scratch) {
   using T = PrepLayout<N>;
-  constexpr int MH = T::MH;
-  __shared__ float2 lds[kLdsFloat2];
-  float2* tw = lds;
-  double* red = (double*)(tw + N);
-  float2* bufA = lds + T::FIXED;
-  float2* bufB = bufA + T::CAP;
-  const int tid = threadIdx.x, nthr = blockDim.x;
+  constexpr int MH = T::MH, SP = T::SP;
+  __shared__ float2 tw[N];
+  __shared__ float2 bufA[kTK * SP > kRB * N ? kTK * SP : kRB * N];
+  __shared__ float2 bufB[kTK * SP > kRB * N ? kTK * SP : kRB * N];
+  __shared__ float ctf_l[256 * kTK];  // [row][q], rows < n <= N <= 256
+  __shared__ double red[kMaxPsiGroup * (kThreads / 64)];
+  const int tid = threadIdx.x;
+  constexpr int nthr = kThreads;  // launched with exactly kThreads threads
   fill_twiddles(tw, N, tid, nthr);
-  float2* W = scratch + (size_t)blockIdx.x * T::SLOT;
+  float2* Wbase = scratch + (size_t)blockIdx.x * T::SLOT;
   const int mh_n = n / 2 + 1;
   const size_t plane_n = (size_t)n * mh_n;
+  const bool has_ctf = defocus != nullptr;
+  const bool explicit_ctf = !has_ctf && ctf_tab != nullptr;
 
-  for (int img = blockIdx.x; img < B * P; img += gridDim.x) {
-    const int b = img / P;
+  for (int b = blockIdx.x; b < B; b += gridDim.x) {
     const float* src = images + (size_t)b * N * N;
-    const float* rp = rotp + (size_t)img * 6;
-    const float m00 = rp[0], m01 = rp[1], m10 = rp[2], m11 = rp[3], oy = rp[4], ox = rp[5];
-    __syncthreads();
+    float axx = 0.f, ayy = 0.f, axy = 0.f;
+    if (has_ctf) {
+      const float dfu = defocus[3 * b + 0], dfv = defocus[3 * b + 1];
+      const float ang = defocus[3 * b + 2] * 0.017453292519943295f;
+      const float sa = sinf(ang), ca = cosf(ang);
+      axx = (-dfu * ca) * ca - (dfv * sa) * sa;
+      ayy = (-dfu * sa) * sa - (dfv * ca) * ca;
+      axy = (-dfu * sa) * ca + (dfv * sa) * ca;
+    }
+    for (int pg = 0; pg < P; pg += kMaxPsiGroup) {
+      const int np = min(kMaxPsiGroup, P - pg);
 
-    // ---- rows: bilinear rotation (zeros padding), mask, forward FFT -------
-    for (int y0 = 0; y0 < N; y0 += 2 * T::RB) {
-      const int nseq = min(T::RB, (N - y0 + 1) / 2);
-      for (int t = tid; t < nseq * N; t += nthr) {
-        const int q = t / N, X = t - q * N;
-        float v2[2];
+      // ---- rows: bilinear rotation (zeros padding), mask, forward FFT ---------
+      for (int pi = 0; pi < np; ++pi) {
+        const float* rp = rotp + ((size_t)b * P + pg + pi) * 6;
+        const float m00 = rp[0], m01 = rp[1], m10 = rp[2], m11 = rp[3], oy = rp[4], ox = rp[5];
+        float2* W = Wbase + (size_t)pi * T::IMG;
+        for (int y0 = 0; y0 < N; y0 += 2 * kRB) {
+          __syncthreads();
+          // Each wave samples 8x8 output blocks, so one load instruction's
+          // addresses stay within a compact rotated source patch; the two
+          // horizontal taps of a row come from one 8-byte load.
+          // Branch-free: the 2x2 footprint is read at clamped coordinates (always
+          // in bounds) and taps outside the image are zeroed by selects, so
+          // all ITER iterations' loads can be in flight together.
+          constexpr int BPR = N / 8;
+          constexpr int ITER = 2 * BPR / (kThreads / 64);
+          float* bufA_f = (float*)bufA;
+          const int lane = tid & 63;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int Y = y0 + 2 * q + h;
-          float v = 0.0f;
-          if (Y < N) {
+          for (int it = 0; it < ITER; ++it) {
+            const int blk = (tid >> 6) + it * (kThreads / 64);
+            const int band = blk / BPR, bc = blk - band * BPR;
+            const int yl = band * 8 + (lane >> 3);
+            const int Y = y0 + yl, X = bc * 8 + (lane & 7);
             const float sy = (float)Y * m00 + (float)X * m01 + oy;
             const float sx = (float)Y * m10 + (float)X * m11 + ox;
             const float fy0 = floorf(sy), fx0 = floorf(sx);
-            const int iy = (int)fy0, ix = (int)fx0;
             const float ty = sy - fy0, tx = sx - fx0;
-            float a00 = 0.f, a01 = 0.f, a10 = 0.f, a11 = 0.f;
+            // clamp before converting so far-outside samples stay finite ints
+            const int iy = (int)fminf(fmaxf(fy0, -2.0f), (float)N);
+            const int ix = (int)fminf(fmaxf(fx0, -2.0f), (float)N);
+            const int cy = min(max(iy, 0), N - 2), cx = min(max(ix, 0), N - 2);
+            const float* s0 = src + (size_t)cy * N + cx;
+            const f2a4 r0 = *(const f2a4*)s0;
+            const f2a4 r1 = *(const f2a4*)(s0 + N);
             const bool y0ok = iy >= 0 && iy < N, y1ok = iy + 1 >= 0 && iy + 1 < N;
             const bool x0ok = ix >= 0 && ix < N, x1ok = ix + 1 >= 0 && ix + 1 < N;
-            if (y0ok && x0ok) a00 = src[(size_t)iy * N + ix];
-            if (y0ok && x1ok) a01 = src[(size_t)iy * N + ix + 1];
-            if (y1ok && x0ok) a10 = src[(size_t)(iy + 1) * N + ix];
-            if (y1ok && x1ok) a11 = src[(size_t)(iy + 1) * N + ix + 1];
-            v = ((1.0f - ty) * (1.0f - tx)) * a00 + ((1.0f - ty) * tx) * a01 +
-                (ty * (1.0f - tx)) * a10 + (ty * tx) * a11;
+            // offsets of each valid tap inside the loaded 2x2 block are 0 or 1
+            const int dy0 = iy - cy, dy1 = iy + 1 - cy, dx0 = ix - cx, dx1 = ix + 1 - cx;
+            auto pick = [&](int dy, int dx) { return dy ? (dx ? r1.y : r1.x) : (dx ? r0.y : r0.x); };
+            const float a00 = (y0ok && x0ok) ? pick(dy0, dx0) : 0.f;
+            const float a01 = (y0ok && x1ok) ? pick(dy0, dx1) : 0.f;
+            const float a10 = (y1ok && x0ok) ? pick(dy1, dx0) : 0.f;
+            const float a11 = (y1ok && x1ok) ? pick(dy1, dx1) : 0.f;
+            float v = ((1.0f - ty) * (1.0f - tx)) * a00 + ((1.0f - ty) * tx) * a01 +
+                      (ty * (1.0f - tx)) * a10 + (ty * tx) * a11;
             if (mask) v *= mask[(size_t)Y * N + X];
+            bufA_f[2 * ((yl >> 1) * N + X) + (yl & 1)] = v;  // row pair -> (re, im)
           }
-          v2[h] = v;
+          __syncthreads();
+          float2* res = fft_lds<N, false>(bufA, bufB, kRB, N, tw, tid, nthr);
+          for (int t = tid; t < kRB * MH; t += nthr) {
+            const int q = t / MH, k = t - q * MH;
+            const float2 z = res[q * N + k];
+            const float2 zc = cconj(res[q * N + (N - k) % N]);
+            const int ya = y0 + 2 * q;
+            float2* Wk = W + ((size_t)(k / kTK) * N) * kTK + (k % kTK);
+            Wk[(size_t)ya * kTK] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+            Wk[(size_t)(ya + 1) * kTK] = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+          }
         }
-        bufA[q * N + X] = make_float2(v2[0], v2[1]);
       }
-      __syncthreads();
-      float2* res = fft_lds<N, false>(bufA, bufB, nseq, N, tw, tid, nthr);
-      for (int t = tid; t < nseq * MH; t += nthr) {
-        const int q = t / MH, k = t - q * MH;
-        const float2 z = res[q * N + k];
-        const float2 zc = cconj(res[q * N + (N - k) % N]);
-        const int ya = y0 + 2 * q;
-        W[(size_t)ya * MH + k] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
-        if (ya + 1 < N) W[(size_t)(ya + 1) * MH + k] = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
-      }
-      __syncthreads();
-    }
 
-    // ---- columns: forward FFT, crop, pack, CTF, conj ----------------------
-    float2* out = spectra + (size_t)img * plane_n;
-    float sin_az = 0.f, cos_az = 1.f, dfu = 0.f, dfv = 0.f;
-    const bool has_ctf = defocus != nullptr;
-    if (has_ctf) {
-      dfu = defocus[3 * b + 0];
-      dfv = defocus[3 * b + 1];
-      const float ang = defocus[3 * b + 2] * 0.017453292519943295f;
-      sin_az = sinf(ang);
-      cos_az = cosf(ang);
-    }
-    const float axx = (-dfu * cos_az) * cos_az - (dfv * sin_az) * sin_az;
-    const float ayy = (-dfu * sin_az) * sin_az - (dfv * cos_az) * cos_az;
-    const float axy = (-dfu * sin_az) * cos_az + (dfv * sin_az) * cos_az;
-    double nrm[1] = {0.0};
-    const int kmax = n / 2;
-    for (int k0 = 0; k0 <= kmax; k0 += T::TK) {
-      const int nk = min(T::TK, kmax + 1 - k0);
-      for (int t = tid; t < nk * N; t += nthr) {
-        const int Y = t / nk, q = t - Y * nk;
-        bufA[q * (N + 1) + Y] = W[(size_t)Y * MH + k0 + q];
-      }
-      __syncthreads();
-      float2* res = fft_lds<N, false>(bufA, bufB, nk, N + 1, tw, tid, nthr);
-      for (int t = tid; t < nk * n; t += nthr) {
-        const int r = t / nk, q = t - r * nk;
-        const int c = k0 + q;
-        const int fy = r < n / 2 ? r : r - n;
-        float2 v;
-        if (c < n / 2 || n == N) {
-          v = res[q * (N + 1) + (fy < 0 ? fy + N : fy)];
-        } else {
-          const int my = -fy;
-          v = cconj(res[q * (N + 1) + (my < 0 ? my + N : my)]);
+      // ---- columns: CTF tile once, then FFT/crop/pack/CTF/conj per psi ------
+      double nrm[kMaxPsiGroup];
+#pragma unroll
+      for (int i = 0; i < kMaxPsiGroup; ++i) nrm[i] = 0.0;
+      const int kmax = n / 2;
+      for (int k0 = 0; k0 <= kmax; k0 += kTK) {
+        __syncthreads();
+        for (int t = tid; t < n * kTK; t += nthr) {
+          const int r = t / kTK, q = t - r * kTK;
+          const int c = k0 + q;
+          float w = 1.0f;
+          if (c <= kmax) {
+            const size_t e = (size_t)r * mh_n + c;
+            if (has_ctf)
+              w = ctf_weight(ctf_tab[e], ctf_tab[plane_n + e], ctf_tab[2 * plane_n + e], ctf_tab[3 * plane_n + e],
+                             axx, axy, ayy, k1, k2, k3, k5, kk);
+            else if (explicit_ctf)
+              w = ctf_tab[(size_t)b * plane_n + e];
+          }
+          ctf_l[t] = w;
         }
-        if (has_ctf) {
-          const size_t e = (size_t)r * mh_n + c;
-          const float x = ctf_tab[e], y = ctf_tab[plane_n + e];
-          const float u4 = ctf_tab[2 * plane_n + e], nc = ctf_tab[3 * plane_n + e];
-          const float gamma = k1 * ((axx * x * x + 2.0f * axy * x * y) + ayy * y * y) + k2 * u4 - k5 - k3;
-          float cv = -sinf(gamma);
-          if (fabsf(cv) < 1e-8f) cv = cv > 0.f ? 1e-8f : (cv < 0.f ? -1e-8f : 0.f);
-          const float w = cv * sqrtf(1.0f / (nc + kk * cv * cv));
-          v.x *= w;
-          v.y *= w;
-        } else if (ctf_tab) {  // explicit per-particle packed CTF [B][n][n/2+1]
-          const float w = ctf_tab[(size_t)b * plane_n + (size_t)r * mh_n + c];
-          v.x *= w;
-          v.y *= w;
+#pragma unroll 1
+        for (int pi = 0; pi < np; ++pi) {
+          const float2* Wt = Wbase + (size_t)pi * T::IMG + (size_t)(k0 / kTK) * N * kTK;
+          for (int t = tid; t < kTK * N; t += nthr) {  // contiguous tile block
+            const int Y = t / kTK, q = t - Y * kTK;
+            const int c = k0 + q;
+            bufA[q * SP + Y] = c < MH ? Wt[t] : make_float2(0.f, 0.f);
+          }
+          __syncthreads();
+          float2* res = fft_lds<N, false>(bufA, bufB, kTK, SP, tw, tid, nthr);
+          float2* out = spectra + ((size_t)b * P + pg + pi) * plane_n;
+          double acc = 0.0;
+          for (int t = tid; t < n * kTK; t += nthr) {
+            const int r = t / kTK, q = t - r * kTK;
+            const int c = k0 + q;
+            if (c > kmax) continue;
+            const int fy = r < n / 2 ? r : r - n;
+            float2 v;
+            if (c < n / 2 || n == N) {
+              v = res[q * SP + (fy < 0 ? fy + N : fy)];
+            } else {  // packed column n/2 holds frequency -n/2 = conj of (+n/2) at -fy
+              const int my = -fy;
+              v = cconj(res[q * SP + (my < 0 ? my + N : my)]);
+            }
+            const float w = ctf_l[t];
+            v.x *= w;
+            v.y = -(v.y * w);  // conj
+            out[(size_t)r * mh_n + c] = v;
+            acc += (double)v.x * v.x + (double)v.y * v.y;
+          }
+#pragma unroll
+          for (int i = 0; i < kMaxPsiGroup; ++i)
+            if (i == pi) nrm[i] += acc;
+          __syncthreads();
         }
-        v.y = -v.y;  // conj
-        out[(size_t)r * mh_n + c] = v;
-        nrm[0] += (double)v.x * v.x + (double)v.y * v.y;
       }
-      __syncthreads();
+      block_sum_f64<kMaxPsiGroup>(nrm, red);
+      if (tid < np) {
+        double v = 0.0;
+#pragma unroll
+        for (int i = 0; i < kMaxPsiGroup; ++i)
+          if (i == tid) v = nrm[i];
+        inv_norm[(size_t)b * P + pg + tid] = 1.0f / (float)sqrt(v);
+      }
     }
-    block_sum_f64<1>(nrm, red);
-    if (tid == 0) inv_norm[img] = 1.0f / (float)sqrt(nrm[0]);
   }
 }
 
@@ -177,8 +228,8 @@ int launch_prepare(const float* images, int B, int P, int n, const float* rotp, 
                    float* inv_norm, void* ws, size_t ws_bytes, hipStream_t st) {
   using T = PrepLayout<N>;
   const size_t slot = T::SLOT * sizeof(float2);
-  int grid = (int)std::min<size_t>((size_t)B * P, ws_bytes / slot);
-  grid = std::min(grid, 8 * gis_cu_count());
+  int grid = (int)std::min<size_t>((size_t)B, ws_bytes / slot);
+  grid = std::min(grid, 4 * gis_cu_count());
   GIS_REQUIRE(grid >= 1, "gis_prepare_particles: workspace of %zu bytes is below one slot (%zu)", ws_bytes, slot);
   hipLaunchKernelGGL(prepare_particles_kernel<N>, dim3(grid), dim3(kThreads), 0, st, images, B, P, n, rotp,
                      mask, ctf_tab, defocus, k[0], k[1], k[2], k[3], k[4], spectra, inv_norm, (float2*)ws);
@@ -191,7 +242,8 @@ size_t prep_slot_bytes() { return PrepLayout<N>::SLOT * sizeof(float2); }
 
 }  // namespace
 
-GIS_API size_t gis_prepare_particles_workspace(int N, int images) {
+// Workspace bytes for gis_prepare_particles over `particles` particles.
+GIS_API size_t gis_prepare_particles_workspace(int N, int particles) {
   size_t s = 0;
   switch (N) {
     case 32: s = prep_slot_bytes<32>(); break;
@@ -202,8 +254,8 @@ GIS_API size_t gis_prepare_particles_workspace(int N, int images) {
     case 256: s = prep_slot_bytes<256>(); break;
     default: return 0;
   }
-  if (images <= 0) return 0;
-  return std::min<size_t>((size_t)images, (size_t)8 * gis_cu_count()) * s;
+  if (particles <= 0) return 0;
+  return std::min<size_t>((size_t)particles, (size_t)4 * gis_cu_count()) * s;
 }
 
 // Replaces ParticleFourierPreparer.prepare (v606:1340-1374) together with the
@@ -211,9 +263,10 @@ GIS_API size_t gis_prepare_particles_workspace(int N, int images) {
 //   images   [B][N][N] float32
 //   rotp     [B][P][6] float32 psi grids: (c, -s, s, c, off_y, off_x)
 //   mask     [N][N] float32 or NULL
-//   ctf_tab  [4][n][n/2+1] float32 (x, y, u4, ncurve) or NULL with defocus NULL
-//   defocus  [B][3] float32 (dfU, dfV, angle deg) or NULL (CTF = 1)
-//   kconst   [5] float32: k1, k2, k3, k5, kk
+//   ctf_tab  with defocus: [4][n][n/2+1] float32 (x, y, u4, ncurve);
+//            without: [B][n][n/2+1] explicit packed CTF, or NULL (CTF = 1)
+//   defocus  [B][3] float32 (dfU, dfV, angle deg) or NULL
+//   kconst   [5] float32 (host): k1, k2, k3, k5, kk
 //   spectra  [B][P][n][n/2+1] complex64 output (conjugated, unnormalised, unshifted)
 //   inv_norm [B][P] float32 output
 GIS_API int gis_prepare_particles(const float* images, int B, int N, int n, int P, const float* rotp,

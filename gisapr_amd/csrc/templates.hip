@@ -29,6 +29,8 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kLdsFloat2 = 8192;  // 64 KiB of LDS per workgroup
 
+typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
+
 __device__ __forceinline__ int ring_of(int fy, int kx) {
   return (int)rintf(sqrtf((float)(fy * fy + kx * kx)));
 }
@@ -65,7 +67,8 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
   double* red = (double*)(div + T::NR + (T::NR & 1));  // reduction scratch (16 doubles)
   float2* bufA = lds + T::FIXED;
   float2* bufB = bufA + T::CAP;
-  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int tid = threadIdx.x;
+  constexpr int nthr = kThreads;  // launched with exactly kThreads threads
   fill_twiddles(tw2, L2, tid, nthr);
   fill_twiddles(tw1, N, tid, nthr);
   float2* U = scratch + (size_t)blockIdx.x * T::SLOT;
@@ -82,60 +85,72 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     // ---- phase 1: gather + inverse column FFT (length 2N) ----------------
     for (int k0 = 0; k0 < M2; k0 += T::TK1) {
       const int nk = min(T::TK1, M2 - k0);
-      for (int t = tid; t < nk * L2; t += nthr) {
-        const int q = t / L2, ip = t - q * L2;
-        const int kx = k0 + q;
-        const int f = ip < N ? ip : ip - L2;
-        const float fy = (float)f * inv_l2;
-        const float fx = (float)kx * inv_l2;
-        float gx = r00 * fx + r01 * fy;
-        float gy = r10 * fx + r11 * fy;
-        float gz = r20 * fx + r21 * fy;
-        const bool cj = gx < 0.0f;
-        if (cj) { gx = -gx; gy = -gy; gz = -gz; }
-        float cz = gz * (float)L2 + (float)N;
-        float cy = gy * (float)L2 + (float)N;
-        float cx = gx * (float)L2;
-        cz = fminf(fmaxf(cz, 0.0f), (float)(L2 - 1));
-        cy = fminf(fmaxf(cy, 0.0f), (float)(L2 - 1));
-        cx = fminf(fmaxf(cx, 0.0f), (float)(M2 - 1));
-        const float z0f = floorf(cz), y0f = floorf(cy), x0f = floorf(cx);
-        const float tz = cz - z0f, ty = cy - y0f, tx = cx - x0f;
-        const int z0 = (int)z0f, y0 = (int)y0f, x0 = (int)x0f;
-        float2 acc = make_float2(0.f, 0.f);
-#pragma unroll
-        for (int dz = 0; dz < 2; ++dz) {
-          const int iz = z0 + dz;
-          if (iz > L2 - 1) continue;
-          const float wz = dz ? tz : 1.0f - tz;
-          const int kz = (iz + N) % L2;
-#pragma unroll
-          for (int dy = 0; dy < 2; ++dy) {
-            const int iy = y0 + dy;
-            if (iy > L2 - 1) continue;
-            const float wy = dy ? ty : 1.0f - ty;
-            const int ky = (iy + N) % L2;
-            const float2* row = D + ((size_t)kz * L2 + ky) * M2;
-#pragma unroll
-            for (int dx = 0; dx < 2; ++dx) {
-              const int ix = x0 + dx;
-              if (ix > M2 - 1) continue;
-              const float wx = dx ? tx : 1.0f - tx;
-              const float w = (wz * wy) * wx;
-              const float2 v = row[ix];
-              acc.x += w * v.x;
-              acc.y += w * v.y;
-            }
-          }
+      // Each wave samples blocks of 8 slice rows x (up to) 8 slice columns,
+      // so one load instruction touches a compact patch of the 3-D spectrum;
+      // the two x taps of a trilinear corner pair come from one 16-byte load.
+      // Branch-free: every tap index is clamped into the array; a clamped
+      // upper tap always carries weight 0 (the coordinate was clamped to the
+      // last index first), so all loads of an unrolled group are in flight.
+      const int ngroups = (nk + 7) / 8;
+      for (int cg = 0; cg < ngroups; ++cg) {
+#pragma unroll 4
+        for (int it = 0; it < L2 / 32; ++it) {
+          const int t = tid + it * kThreads;  // [0, 8 * L2)
+          const int lane = t & 63, rb = t >> 6;
+          const int q = cg * 8 + (lane >> 3), ip = rb * 8 + (lane & 7);
+          const bool active = q < nk;
+          const int kx = k0 + min(q, nk - 1);
+          const int f = ip < N ? ip : ip - L2;
+          const float fy = (float)f * inv_l2;
+          const float fx = (float)kx * inv_l2;
+          float gx = r00 * fx + r01 * fy;
+          float gy = r10 * fx + r11 * fy;
+          float gz = r20 * fx + r21 * fy;
+          const bool cj = gx < 0.0f;
+          if (cj) { gx = -gx; gy = -gy; gz = -gz; }
+          float cz = gz * (float)L2 + (float)N;
+          float cy = gy * (float)L2 + (float)N;
+          float cx = gx * (float)L2;
+          cz = fminf(fmaxf(cz, 0.0f), (float)(L2 - 1));
+          cy = fminf(fmaxf(cy, 0.0f), (float)(L2 - 1));
+          cx = fminf(fmaxf(cx, 0.0f), (float)(M2 - 1));
+          const float z0f = floorf(cz), y0f = floorf(cy), x0f = floorf(cx);
+          const float tz = cz - z0f, ty = cy - y0f, tx = cx - x0f;
+          const int z0 = (int)z0f, y0 = (int)y0f, x0 = (int)x0f;
+          const int kz0 = (z0 + N) % L2, kz1 = (min(z0 + 1, L2 - 1) + N) % L2;
+          const int ky0 = (y0 + N) % L2, ky1 = (min(y0 + 1, L2 - 1) + N) % L2;
+          const int xp = min(x0, M2 - 2);  // pair start; x0 = M2-1 takes the pair's second tap
+          const bool hi = x0 != xp;
+          const f4a8 v00 = *(const f4a8*)(D + ((size_t)kz0 * L2 + ky0) * M2 + xp);
+          const f4a8 v01 = *(const f4a8*)(D + ((size_t)kz0 * L2 + ky1) * M2 + xp);
+          const f4a8 v10 = *(const f4a8*)(D + ((size_t)kz1 * L2 + ky0) * M2 + xp);
+          const f4a8 v11 = *(const f4a8*)(D + ((size_t)kz1 * L2 + ky1) * M2 + xp);
+          const float wx0 = 1.0f - tx, wx1 = tx;
+          const float w00 = (1.0f - tz) * (1.0f - ty), w01 = (1.0f - tz) * ty;
+          const float w10 = tz * (1.0f - ty), w11 = tz * ty;
+          float2 acc = make_float2(0.f, 0.f);
+          auto tap = [&](const f4a8& v, float wzy) {
+            const float a0x = hi ? v.z : v.x, a0y = hi ? v.w : v.y;
+            const float a1x = hi ? 0.f : v.z, a1y = hi ? 0.f : v.w;
+            acc.x += (wzy * wx0) * a0x;
+            acc.y += (wzy * wx0) * a0y;
+            acc.x += (wzy * wx1) * a1x;
+            acc.y += (wzy * wx1) * a1y;
+          };
+          tap(v00, w00);
+          tap(v01, w01);
+          tap(v10, w10);
+          tap(v11, w11);
+          if (cj) acc.y = -acc.y;
+          if (active) bufA[q * (L2 + 1) + ip] = acc;
         }
-        if (cj) acc.y = -acc.y;
-        bufA[q * (L2 + 1) + ip] = acc;
       }
       __syncthreads();
       float2* res = fft_lds<L2, true>(bufA, bufB, nk, L2 + 1, tw2, tid, nthr);
       // keep spatial rows y in [3N/2, 2N) u [0, N/2) -> projection row Y
-      for (int t = tid; t < nk * N; t += nthr) {
-        const int Y = t / nk, q = t - Y * nk;
+      for (int t = tid; t < T::TK1 * N; t += nthr) {
+        const int Y = t / T::TK1, q = t - Y * T::TK1;
+        if (q >= nk) continue;
         const int y = (Y + 3 * N / 2) % L2;
         U[(size_t)Y * M2 + k0 + q] = res[q * (L2 + 1) + y];
       }
@@ -180,55 +195,66 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       __syncthreads();
     }
 
-    // ---- phase 3: forward column FFT (N) -> projection spectrum F ---------
+    // ---- phase 3: forward column FFT (N) -> projection spectrum F, and the
+    // ring sums of |F| over the full plane.  Thread `ring` owns one ring and
+    // scans the LDS-resident columns of each tile in a fixed order
+    // (deterministic); half-plane columns 1..N/2-1 stand for two full-plane
+    // elements of the same ring.
+    const int nr = ring_of(N / 2, N / 2) + 1;
+    float ring_sum = 0.0f;
+    int ring_cnt = 0;
     for (int k0 = 0; k0 < MH; k0 += T::TK3) {
       const int nk = min(T::TK3, MH - k0);
-      for (int t = tid; t < nk * N; t += nthr) {
-        const int Y = t / nk, q = t - Y * nk;
+      for (int t = tid; t < T::TK3 * N; t += nthr) {
+        const int Y = t / T::TK3, q = t - Y * T::TK3;
+        if (q >= nk) continue;
         bufA[q * (N + 1) + Y] = W[(size_t)Y * MH + k0 + q];
       }
       __syncthreads();
       float2* res = fft_lds<N, false>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
-      for (int t = tid; t < nk * N; t += nthr) {
-        const int ky = t / nk, q = t - ky * nk;
+      for (int t = tid; t < T::TK3 * N; t += nthr) {
+        const int ky = t / T::TK3, q = t - ky * T::TK3;
+        if (q >= nk) continue;
         W[(size_t)ky * MH + k0 + q] = res[q * (N + 1) + ky];
+      }
+      if (tid < nr) {
+        const int ring = tid;
+        const float lo = (float)ring - 0.5f, hi = (float)ring + 0.5f;
+        for (int q = 0; q < nk; ++q) {
+          const int kx = k0 + q;
+          const float h2 = hi * hi - (float)(kx * kx);
+          if (h2 < 0.0f) continue;
+          const float l2 = lo * lo - (float)(kx * kx);
+          const int ya = l2 > 0.0f ? max((int)floorf(sqrtf(l2)) - 1, 0) : 0;
+          const int yb = min((int)ceilf(sqrtf(h2)) + 1, N / 2);
+          const int wgt = (kx == 0 || kx == N / 2) ? 1 : 2;
+          for (int a = ya; a <= yb; ++a) {
+#pragma unroll
+            for (int sg = 0; sg < 2; ++sg) {
+              const int fy = sg ? -a : a;
+              if ((sg && a == 0) || fy >= N / 2 || fy < -N / 2) continue;
+              if (ring_of(fy, kx) != ring) continue;
+              const float2 v = res[q * (N + 1) + (fy < 0 ? fy + N : fy)];
+              const float m = sqrtf(v.x * v.x + v.y * v.y);
+              ring_sum += wgt == 2 ? m + m : m;
+              ring_cnt += wgt;
+            }
+          }
+        }
       }
       __syncthreads();
     }
 
-    // ---- phase 4: ring sums of |F| over the full plane (deterministic) ---
-    const int nr = ring_of(N / 2, N / 2) + 1;
-    for (int ring = tid; ring < nr; ring += nthr) {
-      float s = 0.0f;
-      int cnt = 0;
-      const float lo = (float)ring - 0.5f, hi = (float)ring + 0.5f;
-      for (int fy = -N / 2; fy < N / 2; ++fy) {
-        const float h2 = hi * hi - (float)(fy * fy);
-        if (h2 < 0.0f) continue;
-        const float l2 = lo * lo - (float)(fy * fy);
-        int ka = l2 > 0.0f ? (int)floorf(sqrtf(l2)) - 1 : 0;
-        int kb = (int)ceilf(sqrtf(h2)) + 1;
-        ka = max(ka, 0);
-        kb = min(kb, N / 2);
-        const int ky = fy < 0 ? fy + N : fy;
-        for (int kx = ka; kx <= kb; ++kx) {
-          if (ring_of(fy, kx) != ring) continue;
-          const int wgt = (kx == 0 || kx == N / 2) ? 1 : 2;
-          const float2 v = W[(size_t)ky * MH + kx];
-          const float a = sqrtf(v.x * v.x + v.y * v.y);
-          s += wgt == 2 ? a + a : a;
-          cnt += wgt;
-        }
-      }
-      div[ring] = (ring == 0 || cnt == 0) ? 1.0f : (float)cnt / s;
-    }
+    // ---- phase 4: ring divisors ------------------------------------------
+    if (tid < nr) div[tid] = (tid == 0 || ring_cnt == 0) ? 1.0f : (float)ring_cnt / ring_sum;
     __syncthreads();
 
     // ---- phase 5: scale by ring divisor, inverse column FFT (N) -----------
     for (int k0 = 0; k0 < MH; k0 += T::TK3) {
       const int nk = min(T::TK3, MH - k0);
-      for (int t = tid; t < nk * N; t += nthr) {
-        const int ky = t / nk, q = t - ky * nk;
+      for (int t = tid; t < T::TK3 * N; t += nthr) {
+        const int ky = t / T::TK3, q = t - ky * T::TK3;
+        if (q >= nk) continue;
         const int fy = ky < N / 2 ? ky : ky - N;
         const float g = div[ring_of(fy, k0 + q)];
         const float2 v = W[(size_t)ky * MH + k0 + q];
@@ -236,8 +262,9 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       }
       __syncthreads();
       float2* res = fft_lds<N, true>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
-      for (int t = tid; t < nk * N; t += nthr) {
-        const int Y = t / nk, q = t - Y * nk;
+      for (int t = tid; t < T::TK3 * N; t += nthr) {
+        const int Y = t / T::TK3, q = t - Y * T::TK3;
+        if (q >= nk) continue;
         W[(size_t)Y * MH + k0 + q] = res[q * (N + 1) + Y];
       }
       __syncthreads();
@@ -355,14 +382,16 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     const int kmax = n / 2;  // columns 0..n/2 are needed
     for (int k0 = 0; k0 <= kmax; k0 += T::TK3) {
       const int nk = min(T::TK3, kmax + 1 - k0);
-      for (int t = tid; t < nk * N; t += nthr) {
-        const int Y = t / nk, q = t - Y * nk;
+      for (int t = tid; t < T::TK3 * N; t += nthr) {
+        const int Y = t / T::TK3, q = t - Y * T::TK3;
+        if (q >= nk) continue;
         bufA[q * (N + 1) + Y] = W[(size_t)Y * MH + k0 + q];
       }
       __syncthreads();
       float2* res = fft_lds<N, false>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
-      for (int t = tid; t < nk * n; t += nthr) {
-        const int r = t / nk, q = t - r * nk;
+      for (int t = tid; t < T::TK3 * n; t += nthr) {
+        const int r = t / T::TK3, q = t - r * T::TK3;
+        if (q >= nk) continue;
         const int c = k0 + q;
         const int fy = r < n / 2 ? r : r - n;
         float2 v;

@@ -38,7 +38,8 @@ vdft_x_pass(const float* __restrict__ vol, float2* __restrict__ D) {
   float2* tw = lds;
   float2* bufA = tw + L;
   float2* bufB = bufA + RB * L;
-  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int tid = threadIdx.x;
+  constexpr int nthr = 256;  // launched with exactly 256 threads
   fill_twiddles(tw, L, tid, nthr);
   const int row_pair0 = blockIdx.x * RB;  // data row index = 2 * pair (+1)
   const int total_pairs = (N * N) / 2;
@@ -98,7 +99,8 @@ vdft_y_pass(float2* __restrict__ D) {
   float2* tw = lds;
   float2* bufA = tw + L;
   float2* bufB = bufA + TK * SP;
-  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int tid = threadIdx.x;
+  constexpr int nthr = 256;  // launched with exactly 256 threads
   fill_twiddles(tw, L, tid, nthr);
   const int vz = blockIdx.y;
   const int zs = (vz + N / 2 + N) % L;
@@ -131,7 +133,8 @@ vdft_z_pass(float2* __restrict__ D) {
   float2* tw = lds;
   float2* bufA = tw + L;
   float2* bufB = bufA + TK * SP;
-  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int tid = threadIdx.x;
+  constexpr int nthr = 256;  // launched with exactly 256 threads
   fill_twiddles(tw, L, tid, nthr);
   const int ky = blockIdx.y;
   const int k0 = blockIdx.x * TK;

@@ -72,8 +72,8 @@ int gis_build_templates(const void* dft, int N, int n, const float* rts, int M, 
                         const float* fsc, void* bank, float* norms, void* ws, size_t ws_bytes,
                         void* stream);
 
-/* Workspace bytes gis_prepare_particles needs for `images` (= B*P) images. */
-size_t gis_prepare_particles_workspace(int N, int images);
+/* Workspace bytes gis_prepare_particles needs for `particles` (= B) particles. */
+size_t gis_prepare_particles_workspace(int N, int particles);
 
 /*
  * (b') Particle preparation.  Replaces ParticleFourierPreparer.prepare
