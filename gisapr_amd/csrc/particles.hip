@@ -240,7 +240,42 @@ int launch_prepare(const float* images, int B, int P, int n, const float* rotp, 
 template <int N>
 size_t prep_slot_bytes() { return PrepLayout<N>::SLOT * sizeof(float2); }
 
+__global__ void __launch_bounds__(256)
+ctf_packed_kernel(const float* __restrict__ tab, const float* __restrict__ defocus, int B, int n, float k1,
+                  float k2, float k3, float k5, float kk, float* __restrict__ out) {
+  const size_t plane = (size_t)n * (n / 2 + 1);
+  const size_t total = plane * (size_t)B;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(t / plane);
+    const size_t e = t - (size_t)b * plane;
+    const float dfu = defocus[3 * b + 0], dfv = defocus[3 * b + 1];
+    const float ang = defocus[3 * b + 2] * 0.017453292519943295f;
+    const float sa = sinf(ang), ca = cosf(ang);
+    const float axx = (-dfu * ca) * ca - (dfv * sa) * sa;
+    const float ayy = (-dfu * sa) * sa - (dfv * ca) * ca;
+    const float axy = (-dfu * sa) * ca + (dfv * sa) * ca;
+    out[t] = ctf_weight(tab[e], tab[plane + e], tab[2 * plane + e], tab[3 * plane + e], axx, axy, ayy, k1, k2, k3,
+                        k5, kk);
+  }
+}
+
 }  // namespace
+
+// Packed CTF of B particles (PackedCTFTorch.batch, v606:778-829).
+//   ctf_tab [4][n][n/2+1] float32 (x, y, u4, ncurve); defocus [B][3] (dfU, dfV, angle deg)
+//   kconst  [5] float32 (host): k1, k2, k3, k5, kk;  out [B][n][n/2+1] float32
+GIS_API int gis_ctf_packed(const float* ctf_tab, const float* defocus, int B, int n, const float* kconst, float* out,
+                           void* stream) {
+  GIS_REQUIRE(B >= 0 && n >= 2 && n % 2 == 0, "gis_ctf_packed: B=%d n=%d", B, n);
+  if (B == 0) return GIS_OK;
+  GIS_REQUIRE(ctf_tab && defocus && kconst && out, "gis_ctf_packed: null pointer");
+  const size_t total = (size_t)B * n * (n / 2 + 1);
+  const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(ctf_packed_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, ctf_tab, defocus, B, n,
+                     kconst[0], kconst[1], kconst[2], kconst[3], kconst[4], out);
+  GIS_LAUNCH_CHECK("gis_ctf_packed");
+  return GIS_OK;
+}
 
 // Workspace bytes for gis_prepare_particles over `particles` particles.
 GIS_API size_t gis_prepare_particles_workspace(int N, int particles) {

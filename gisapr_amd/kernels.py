@@ -101,6 +101,18 @@ def prepare_particles(images: torch.Tensor, new_box: int, rotp: torch.Tensor, ma
     return out, inv_norm
 
 
+def ctf_packed(ctf_tab: torch.Tensor, defocus: torch.Tensor, n: int, kconst: np.ndarray) -> torch.Tensor:
+    """Packed weighted CTF [B, n, n/2+1] float32 of B particles."""
+    require_hip(ctf_tab, "ctf_tab")
+    require_hip(defocus, "defocus")
+    B = int(defocus.shape[0])
+    out = torch.empty((B, n, n // 2 + 1), dtype=torch.float32, device=defocus.device)
+    kc = (C.c_float * 5)(*[float(v) for v in np.asarray(kconst, np.float32)])
+    check(lib().gis_ctf_packed(ptr(ctf_tab), ptr(defocus), B, int(n), C.cast(kc, C.c_void_p), ptr(out),
+                               stream_of(defocus.device)), "gis_ctf_packed")
+    return out
+
+
 @dataclass
 class CorrelationResult:
     scores: torch.Tensor   # [sum M_i, P] float32

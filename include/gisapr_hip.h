@@ -97,6 +97,14 @@ int gis_prepare_particles(const float* images, int B, int N, int n, int P, const
                           const float* kconst, void* spectra, float* inv_norm, void* ws,
                           size_t ws_bytes, void* stream);
 
+/*
+ * Packed CTF of B particles.  Replaces PackedCTFTorch.batch (v606:778-829).
+ *   ctf_tab [4][n][n/2+1] float32 (x, y, u4, ncurve);  defocus [B][3]
+ *   kconst  HOST [5] float32 (k1, k2, k3, k5, kk);     out [B][n][n/2+1] float32
+ */
+int gis_ctf_packed(const float* ctf_tab, const float* defocus, int B, int n, const float* kconst, float* out,
+                   void* stream);
+
 /* Workspace bytes gis_correlate needs at newbox n. */
 size_t gis_correlate_workspace(int n);
 

@@ -1,0 +1,81 @@
+"""Product host-side setup (gisapr_amd.host) against the oracle restatement."""
+import numpy as np
+import pytest
+
+from gisapr_amd import host, synthetic
+from oracle import fourier as ofr
+from oracle import particles as opt
+from oracle import project as opj
+from oracle import search as osr
+
+
+def test_rotation_matrices_match_oracle():
+    rot, tilt = synthetic.healpix_nest_directions(2)
+    np.testing.assert_array_equal(host.rotation_matrices_t(rot, tilt), opj.rotation_matrices_t(rot, tilt))
+    e = host.euler_batch(rot[:20], tilt[:20])
+    for i in range(20):
+        np.testing.assert_allclose(e[i], opj.euler_angles_to_matrix(rot[i], tilt[i], 0.0), atol=1e-15)
+
+
+def test_local_orientation_map_matches_oracle():
+    drot, dtilt = synthetic.healpix_nest_directions(2)
+    rng = np.random.default_rng(3)
+    pr = rng.uniform(-180, 180, 60)
+    pt = np.degrees(np.arccos(rng.uniform(-1, 1, 60)))
+    got = host.local_orientation_map(pr, pt, drot, dtilt, 15.0)
+    for i in range(60):
+        assert got[i] == osr.local_orientations(pr[i], pt[i], drot, dtilt, 15.0)
+    assert sum(len(g) for g in got) > 0
+
+
+@pytest.mark.parametrize("psi,step,local,rng_", [(12.0, 7.5, True, 15.0), (359.9, 3.75, True, 10.0),
+                                                 (0.2, 15.0, False, 20.0), (181.875, 3.75, True, 10.0)])
+def test_psi_sampling_matches_oracle(psi, step, local, rng_):
+    assert host.psi_sampling_for_record(psi, step, local, rng_) == \
+           opt.psi_sampling_for_record(psi, step, local, rng_)
+
+
+def test_psi_grid_params_match_oracle_sampler():
+    N = 32
+    for a in (0.5, 7.5, -22.5, 183.75):
+        c, ms, s, c2, oy, ox = host.psi_grid_params(N, [a])[0]
+        yy, xx = np.indices((N, N)).astype(np.float64)
+        sy, sx = opt.rotation_source(N, a)
+        np.testing.assert_allclose(float(c) * yy + float(ms) * xx + float(oy), sy, atol=1e-5)
+        np.testing.assert_allclose(float(s) * yy + float(c2) * xx + float(ox), sx, atol=1e-5)
+
+
+def test_ctf_tables_reproduce_oracle_ctf():
+    n, px, kk = 64, 1.5, 3.0
+    tab = host.CTFTables(n, px, 2.7, 300.0, kk)
+    octf = opt.PackedCTF(n, px, 2.7, 300.0, kk)
+    x, y, u4, nc = tab.table
+    np.testing.assert_array_equal(x, octf.x)
+    np.testing.assert_array_equal(y, octf.y)
+    np.testing.assert_array_equal(u4, octf.u4)
+    np.testing.assert_array_equal(nc, octf.ncurve)
+    assert np.float32(tab.k1) == octf.k1 and np.float32(tab.k2) == octf.k2 and np.float32(tab.k3) == octf.k3
+
+
+def test_mask_and_fsc_match_oracle():
+    np.testing.assert_array_equal(host.generate_soft_mask((40, 40), 15, 4), opj.generate_soft_mask((40, 40), 15, 4))
+    vals = np.linspace(1, 0, 17).astype(np.float32)
+    want = ofr.centered_full_to_packed(opj.expand_radial_values(vals, 32))
+    np.testing.assert_array_equal(host.packed_fsc(vals, 32), want)
+
+
+def test_healpix_counts_and_poles():
+    for order in (0, 1, 2, 3):
+        rot, tilt = synthetic.healpix_nest_directions(order)
+        assert len(rot) == 12 * 4 ** order
+        assert np.all((rot >= -180) & (rot < 180)) and np.all((tilt >= 0) & (tilt <= 180))
+    rot, tilt = synthetic.healpix_nest_directions(0)
+    # base resolution: rings at cos(theta) = 2/3, 0, -2/3
+    np.testing.assert_allclose(np.sort(np.unique(np.round(np.cos(np.radians(tilt)), 6))), [-2 / 3, 0, 2 / 3],
+                               atol=1e-6)
+    # directions are distinct unit vectors, roughly uniform: mean vector ~ 0
+    rot, tilt = synthetic.healpix_nest_directions(3)
+    v = np.stack([np.sin(np.radians(tilt)) * np.cos(np.radians(rot)),
+                  np.sin(np.radians(tilt)) * np.sin(np.radians(rot)), np.cos(np.radians(tilt))], 1)
+    assert np.linalg.norm(v.mean(0)) < 1e-6
+    assert len(np.unique(np.round(v, 9), axis=0)) == len(v)
