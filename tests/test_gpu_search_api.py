@@ -43,7 +43,7 @@ def test_run_search_from_files(hip_device, tmp_path, name):
     assert set(metrics) >= {"search_s", "projection_s", "setup_s", "execution_s"}
     got = _parse(tmp_path / "out.txt")
     # the tensor-input form (worker's search_from_pdb) gives the same records
-    args2 = search.create_search_parser().parse_args(argv[:4] + [str(tmp_path / "out2.txt")] + argv[5:])
+    args2 = search.create_search_parser().parse_args(argv[:5] + [str(tmp_path / "out2.txt")] + argv[6:])
     search.run_search(args2, model_volume=torch.from_numpy(vol).to(hip_device))
     assert _parse(tmp_path / "out2.txt") == got
     parts = {"images": imgs, "rot": d["rot"], "tilt": d["tilt"], "psi": d["psi"],
