@@ -81,3 +81,38 @@ def test_first_maximum_wins_ties():
     P = np.zeros((1, n, n // 2 + 1), complex)
     sc, ys, xs = ofr.correlate(T, P, C, np.float64)
     assert (ys[0, 0], xs[0, 0]) == (0, 0) and sc[0, 0] == 0.0
+
+
+def test_precision_functional_equals_literal_operator_at_small_lags():
+    """Without truncation, the precision variant's Phi over lags [-3, 3]^2 gives
+    exactly the literal v606 ccg at those lags (x 1024 n^2)."""
+    from oracle import precision as opr
+    n, C = 16, 8
+    rng = np.random.default_rng(21)
+    m = n // 2 + 1
+    T = rng.standard_normal((2, n, m)) + 1j * rng.standard_normal((2, n, m))
+    P = rng.standard_normal((3, n, m)) + 1j * rng.standard_normal((3, n, m))
+    kidx = opr.disk_indices(n, 1e9)
+    assert len(kidx) == n * m
+    phi = opr.phi_table(n, kidx, 3)
+    s = opr.scores_f64(T.reshape(2, -1)[:, kidx], P.reshape(3, -1)[:, kidx], phi)
+    lags = ofr.crop_indices(n, C)  # ccg rows/cols [n-4..n-1, 0..3]
+    ly, lx = opr.lag_grid(3)
+    for a in range(2):
+        for b in range(3):
+            prod = ofr.pre_shift(T[a] * P[b])
+            ccg = np.fft.irfft2(prod, s=(n, n))
+            want = np.array([ccg[y % n, x % n] for y, x in zip(ly, lx)]) * 1024 * n * n
+            np.testing.assert_allclose(s[a, b], want, rtol=1e-10, atol=1e-9)
+    del lags
+
+
+def test_disk_indices_radius():
+    from oracle import precision as opr
+    n = 64
+    k = opr.disk_indices(n, 10.0)
+    m = n // 2 + 1
+    r, c = k // m, k % m
+    fy = np.where(r < n // 2, r, r - n)
+    assert np.all(fy ** 2 + c ** 2 <= 100) and len(k) == sum(
+        1 for y in range(-32, 32) for x in range(0, 33) if y * y + x * x <= 100)
