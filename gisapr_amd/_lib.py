@@ -30,6 +30,9 @@ SIGNATURES = {
     "gis_correlate": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                            _vp, _sz, _vp]),
     "gis_score_sum": (_i, [_vp, _vp, _i, _vp, _vp]),
+    "gis_gather_coefficients": (_i, [_vp, _sz, _i, _vp, _i, _i, _vp, _vp, _vp]),
+    "gis_correlate_mixed": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp,
+                                 _vp, _vp, _vp, _vp]),
 }
 
 

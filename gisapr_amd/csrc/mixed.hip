@@ -1,0 +1,221 @@
+// Small-shift correlation on split-BF16 MFMA (precision variant).
+//
+// The v606 score at an integer lag l = (ly, lx) is a fixed linear functional
+// of the packed product Q = T * P (correlate_templates_and_particle,
+// v606:1418-1449):  score(l) = 1024 * sum_k Re(Q_k * Phi[k][l]).  Keeping the
+// K coefficients inside a resolution radius and the lags |ly|, |lx| <= 3
+// turns the search of one particle into a dense product
+//     S[pairs][lags] = A[pairs][2K] * B[2K][lags],
+//     A[p][2k] = Re Q_pk, A[p][2k+1] = Im Q_pk;  B[2k][l] = Re Phi, B[2k+1][l] = -Im Phi
+// evaluated on v_mfma_f32_16x16x32_bf16 with both operands split into BF16
+// hi + lo parts (hi*hi + hi*lo + lo*hi, FP32 accumulation): ~16 significant
+// bits per product instead of BF16's 8.
+//
+// gfx950 mapping: one wave per (particle, tile of 16 direction x psi pairs).
+// Each lane forms 4 complex Q values per k-step from the FP32 template and
+// particle coefficients (two 32-byte loads each), splits them in registers
+// (v_cvt_pk_bf16_f32, round to nearest even) and feeds 4 lag tiles x 3
+// MFMAs; the B operand is pre-split on the host and laid out so each lane's
+// 8 BF16 values are one 16-byte load.  The pair tail (argmax over lags) runs
+// in the epilogue with 16-lane shuffles; the per-particle tail (argmax over
+// pairs, mean / std in FP64) is a separate one-wave-per-particle kernel.
+#include "common.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int kLagTiles = 4;  // 64 lag columns (49 used)
+
+__global__ void __launch_bounds__(256)
+gather_coeffs_kernel(const float2* __restrict__ src, size_t plane, int rows, const int* __restrict__ kidx, int K,
+                     int Kpad, const float* __restrict__ scale, float2* __restrict__ dst) {
+  const size_t total = (size_t)rows * Kpad;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t row = t / Kpad;
+    const int k = (int)(t - row * Kpad);
+    float2 v = make_float2(0.f, 0.f);
+    if (k < K) {
+      v = src[row * plane + kidx[k]];
+      if (scale) {
+        const float s = scale[row];
+        v.x *= s;
+        v.y *= s;
+      }
+    }
+    dst[t] = v;
+  }
+}
+
+__device__ __forceinline__ void split8(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const __bf16 h = (__bf16)x[e];
+    hi[e] = h;
+    lo[e] = (__bf16)(x[e] - (float)h);
+  }
+}
+
+__global__ void __launch_bounds__(64)
+mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__ Pk,
+                       const int* __restrict__ dir_off, const int* __restrict__ dir_idx,
+                       const int2* __restrict__ tiles, int P, int Kpad, const bf16x8* __restrict__ Bhi,
+                       const bf16x8* __restrict__ Blo, int half, float* __restrict__ scores,
+                       int* __restrict__ peak_y, int* __restrict__ peak_x) {
+  const int lane = threadIdx.x;
+  const int2 tile = tiles[blockIdx.x];
+  const int b = tile.x, pair0 = tile.y;
+  const int d0 = dir_off[b];
+  const int np = (dir_off[b + 1] - d0) * P;
+  const int i = lane & 15, g = lane >> 4;
+  const int pair = min(pair0 + i, np - 1);
+  const int j = pair / P, p = pair - j * P;
+  const float4* trow = (const float4*)(Tk + (size_t)dir_idx[d0 + j] * Kpad);
+  const float4* prow = (const float4*)(Pk + ((size_t)b * P + p) * Kpad);
+  const int ksteps = Kpad / 16;
+  floatx4 acc[kLagTiles];
+#pragma unroll
+  for (int nt = 0; nt < kLagTiles; ++nt) acc[nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int ks = 0; ks < ksteps; ++ks) {
+    const int q4 = (ks * 16 + 4 * g) / 2;  // float4 index of this lane's 4 coefficients
+    const float4 t0 = trow[q4], t1 = trow[q4 + 1];
+    const float4 p0 = prow[q4], p1 = prow[q4 + 1];
+    const float tr[4] = {t0.x, t0.z, t1.x, t1.z}, ti[4] = {t0.y, t0.w, t1.y, t1.w};
+    const float pr[4] = {p0.x, p0.z, p1.x, p1.z}, pim[4] = {p0.y, p0.w, p1.y, p1.w};
+    float x[8];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      x[2 * c] = tr[c] * pr[c] - ti[c] * pim[c];
+      x[2 * c + 1] = tr[c] * pim[c] + ti[c] * pr[c];
+    }
+    bf16x8 ahi, alo;
+    split8(x, ahi, alo);
+#pragma unroll
+    for (int nt = 0; nt < kLagTiles; ++nt) {
+      const size_t bi = ((size_t)nt * ksteps + ks) * 64 + lane;
+      const bf16x8 bh = Bhi[bi], bl = Blo[bi];
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bh, acc[nt], 0, 0, 0);
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bl, acc[nt], 0, 0, 0);
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bh, acc[nt], 0, 0, 0);
+    }
+  }
+
+  // epilogue: lane (g, i) holds S[pair row 4g+v][lag column nt*16+i]; first maximum over lags
+  const int side = 2 * half + 1, nlag = side * side;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    float bv = -INFINITY;
+    int bs = 0x7fffffff;
+#pragma unroll
+    for (int nt = 0; nt < kLagTiles; ++nt) {
+      const int s = nt * 16 + i;
+      if (s < nlag) {
+        const float val = acc[nt][v] * 1024.0f;
+        if (val > bv) { bv = val; bs = s; }  // s increases with nt: strict > keeps the first
+      }
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+      const float ov = __shfl_xor(bv, o, 16);
+      const int os = __shfl_xor(bs, o, 16);
+      if (ov > bv || (ov == bv && os < bs)) { bv = ov; bs = os; }
+    }
+    const int pr_row = pair0 + 4 * g + v;
+    if (i == 0 && pr_row < np) {
+      const size_t out = (size_t)d0 * P + pr_row;
+      scores[out] = bv;
+      peak_y[out] = bs / side;  // crop index: lag + half
+      peak_x[out] = bs % side;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64)
+particle_tail_kernel(const int* __restrict__ dir_off, int P, const float* __restrict__ scores,
+                     int* __restrict__ best, float* __restrict__ peak, double* __restrict__ mean,
+                     double* __restrict__ stdv) {
+  const int b = blockIdx.x, lane = threadIdx.x;
+  const int d0 = dir_off[b];
+  const int np = (dir_off[b + 1] - d0) * P;
+  const float* s = scores + (size_t)d0 * P;
+  float bv = -INFINITY;
+  int bi = 0x7fffffff;
+  double sum = 0.0;
+  for (int k = lane; k < np; k += 64) {
+    const float v = s[k];
+    sum += (double)v;
+    if (v > bv || (v == bv && k < bi)) { bv = v; bi = k; }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const float ov = __shfl_xor(bv, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    sum += __shfl_xor(sum, o, 64);
+  }
+  const double mu = np > 0 ? sum / (double)np : 0.0;
+  double ss = 0.0;
+  for (int k = lane; k < np; k += 64) {
+    const double dv = (double)s[k] - mu;
+    ss += dv * dv;
+  }
+  ss = wave_sum_f64(ss);
+  if (lane == 0) {
+    best[b] = np > 0 ? bi : -1;
+    peak[b] = np > 0 ? bv : 0.0f;
+    mean[b] = mu;
+    stdv[b] = np > 0 ? sqrt(ss / (double)np) : 0.0;
+  }
+}
+
+}  // namespace
+
+// Gathers the K listed packed coefficients of `rows` spectra into a dense
+// [rows][Kpad] block (zero padded), optionally scaled per row (1/norm).
+GIS_API int gis_gather_coefficients(const void* src, size_t plane, int rows, const int* kidx, int K, int Kpad,
+                                    const float* scale, void* dst, void* stream) {
+  GIS_REQUIRE(rows >= 0 && K >= 0 && Kpad >= K, "gis_gather_coefficients: rows=%d K=%d Kpad=%d", rows, K, Kpad);
+  if (rows == 0 || Kpad == 0) return GIS_OK;
+  GIS_REQUIRE(src && dst && (K == 0 || kidx), "gis_gather_coefficients: null pointer");
+  const size_t total = (size_t)rows * Kpad;
+  const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(gather_coeffs_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const float2*)src,
+                     plane, rows, kidx, K, Kpad, scale, (float2*)dst);
+  GIS_LAUNCH_CHECK("gis_gather_coefficients");
+  return GIS_OK;
+}
+
+// Precision variant of correlate_templates_and_particle (v606:1418-1449)
+// over lags [-half, half]^2 (half <= 3) and a truncated coefficient set.
+//   Tk [Mtot][Kpad], Pk [B*P][Kpad] complex64 (unit-norm, unshifted)
+//   tiles [ntiles] int2 (particle, first pair) covering every particle's pairs
+//   Bhi, Blo: split lag table, [4][Kpad/16][64] x 8 bf16 (host layout)
+//   scores / peak_y / peak_x [dir_off[B]][P] (peak indices are lag + half)
+//   best / peak / mean / std [B] as gis_correlate
+GIS_API int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx, int B, int P,
+                                int Kpad, const int* tiles, int ntiles, const void* Bhi, const void* Blo, int half,
+                                float* scores, int* peak_y, int* peak_x, int* best, float* peak, double* mean,
+                                double* stdv, void* stream) {
+  GIS_REQUIRE(B >= 0 && P >= 1 && ntiles >= 0, "gis_correlate_mixed: B=%d P=%d ntiles=%d", B, P, ntiles);
+  GIS_REQUIRE(Kpad % 16 == 0 && Kpad >= 16, "gis_correlate_mixed: Kpad %d must be a positive multiple of 16", Kpad);
+  if (half < 0 || (2 * half + 1) * (2 * half + 1) > 16 * kLagTiles) {
+    gis_set_error("gis_correlate_mixed: half %d gives more than %d lags", half, 16 * kLagTiles);
+    return GIS_EUNSUPPORTED;
+  }
+  if (B == 0) return GIS_OK;
+  GIS_REQUIRE(Tk && Pk && dir_off && dir_idx && (ntiles == 0 || tiles) && Bhi && Blo && scores && peak_y && peak_x &&
+                  best && peak && mean && stdv,
+              "gis_correlate_mixed: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (ntiles > 0) {
+    hipLaunchKernelGGL(mixed_correlate_kernel, dim3(ntiles), dim3(64), 0, st, (const float2*)Tk, (const float2*)Pk,
+                       dir_off, dir_idx, (const int2*)tiles, P, Kpad, (const bf16x8*)Bhi, (const bf16x8*)Blo, half,
+                       scores, peak_y, peak_x);
+    GIS_LAUNCH_CHECK("gis_correlate_mixed");
+  }
+  hipLaunchKernelGGL(particle_tail_kernel, dim3(B), dim3(64), 0, st, dir_off, P, scores, best, peak, mean, stdv);
+  GIS_LAUNCH_CHECK("gis_correlate_mixed tail");
+  return GIS_OK;
+}

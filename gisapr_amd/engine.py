@@ -35,6 +35,8 @@ class ParticleSet:
     dir_idx: torch.Tensor              # [sum M_i] int32
     spectra: Optional[torch.Tensor] = None   # [B, P, n, n/2+1] complex64
     inv_norm: Optional[torch.Tensor] = None  # [B, P] float32
+    coeffs: Optional[torch.Tensor] = None    # [B*P, Kpad] complex64, unit norm (small-shift search)
+    tiles: Optional[torch.Tensor] = None     # [ntiles, 2] int32 (small-shift search)
 
     @property
     def B(self) -> int:
@@ -49,7 +51,8 @@ class ParticleSet:
         return int(self.dir_idx.shape[0]) * self.P
 
     def nbytes_resident(self) -> int:
-        t = [self.images, self.rotp, self.dir_off, self.dir_idx, self.spectra, self.inv_norm, self.defocus]
+        t = [self.images, self.rotp, self.dir_off, self.dir_idx, self.spectra, self.inv_norm, self.defocus,
+             self.coeffs, self.tiles]
         return sum(x.numel() * x.element_size() for x in t if x is not None)
 
 
@@ -57,6 +60,7 @@ class ParticleSet:
 class SearchResult:
     corr: kernels.CorrelationResult
     score: torch.Tensor  # [1] float64: sum exp(rawCC / 1024)
+    center: int = 0      # peak index of the zero lag (crop // 2, or half for the small-shift search)
 
 
 class SearchEngine:
@@ -164,7 +168,51 @@ class SearchEngine:
         # dir_off / dir_idx were built on the host against this direction table
         corr = kernels.correlate(bank, ps.spectra, ps.inv_norm, ps.dir_off, ps.dir_idx, self.crop,
                                  validate=False)
-        return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best))
+        return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.crop // 2)
+
+    # ---------------------------------------------------------------- small-shift precision search
+    def enable_small_shift(self, resolution: float = 4.0, half: int = 3) -> host.SmallShiftTables:
+        """Set up the split-bf16 search over lags [-half, half]^2 and the
+        coefficients with |k| <= N apix / resolution (capped at n/2)."""
+        radius = min(self.N * self.apix / float(resolution), self.n / 2.0)
+        tab = host.SmallShiftTables(self.n, radius, half)
+        dev = self.device
+        self.small = tab
+        self.small_kidx = torch.from_numpy(tab.kidx).to(dev)
+        self.small_bhi = torch.from_numpy(tab.b_hi.view(np.int16)).to(dev)
+        self.small_blo = torch.from_numpy(tab.b_lo.view(np.int16)).to(dev)
+        return tab
+
+    def _require_small(self):
+        if getattr(self, "small", None) is None:
+            raise RuntimeError("call enable_small_shift() first")
+
+    def prepare_small_shift(self, ps: ParticleSet, drop_spectra: bool = False) -> ParticleSet:
+        """Gather the unit-norm in-disk coefficients of every psi spectrum (once per dataset)."""
+        self._require_small()
+        if ps.spectra is None:
+            self.prepare(ps)
+        ps.coeffs = kernels.gather_coefficients(ps.spectra, self.small_kidx, self.small.Kpad, ps.inv_norm)
+        ps.tiles = torch.from_numpy(host.small_shift_tiles(ps.dir_off.cpu().numpy(), ps.P)).to(self.device)
+        if drop_spectra:
+            ps.spectra = None
+        return ps
+
+    def template_coefficients(self, bank: torch.Tensor) -> torch.Tensor:
+        self._require_small()
+        return kernels.gather_coefficients(bank, self.small_kidx, self.small.Kpad)
+
+    def search_small_shift(self, bank: torch.Tensor, ps: ParticleSet) -> SearchResult:
+        """Best lag in [-half, half]^2 per (direction, psi) on split-bf16 MFMA."""
+        self._require_small()
+        if ps.coeffs is None:
+            self.prepare_small_shift(ps)
+        if int(bank.shape[0]) != len(self.dir_rot):
+            raise ValueError("template bank does not cover the engine's direction table")
+        tk = self.template_coefficients(bank)
+        corr = kernels.correlate_mixed(tk, ps.coeffs, ps.P, ps.dir_off, ps.dir_idx, ps.tiles, self.small_bhi,
+                                       self.small_blo, self.small.half, validate=False)
+        return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.small.half)
 
     def score_volume(self, volume: torch.Tensor, ps: ParticleSet) -> SearchResult:
         return self.search(self.templates(volume), ps)
@@ -197,8 +245,8 @@ class SearchEngine:
                 "line": int(ps.line_index[b]), "z": float(z), "l3": 1 if z > 3.0 else 0,
                 "accu": 1 if ad < 11.0 else 0, "rot": float(self.dir_rot[gi]), "tilt": float(self.dir_tilt[gi]),
                 "angdist": ad,
-                "xshift": round(float(self.crop // 2 - xx) / self.trans_scale),
-                "yshift": round(float(self.crop // 2 - yy) / self.trans_scale),
+                "xshift": round(float(res.center - xx) / self.trans_scale),
+                "yshift": round(float(res.center - yy) / self.trans_scale),
                 "psi_fake": float(ps.fake_psi[b, pi]), "psi_prior": float((ps.psi[b] + 360.0) % 360.0),
                 "rawcc": float(peak[b]), "best_local": li, "best_psi": pi, "peak_y": yy, "peak_x": xx,
                 "direction": gi, "mean": float(mean[b]), "std": float(std[b]),

@@ -210,3 +210,102 @@ class CTFTables:
     @property
     def kconst(self) -> np.ndarray:
         return np.asarray([self.k1, self.k2, self.k3, self.k5, self.kk], np.float32)
+
+
+class SmallShiftTables:
+    """Coefficient set and split lag table of the small-shift precision search.
+
+    The v606 score at integer lag (ly, lx) is a fixed linear functional of
+    the unshifted packed product Q = T * P (correlate_templates_and_particle,
+    v606:1418-1449 followed by the C2R of torch.fft.irfft2): with
+    pos = (c + (n/2+1)//2) mod (n/2+1) the column after the pre-shift roll,
+    w = 1 for pos in {0, n/2} and 2 otherwise (C2R drops the imaginary part
+    of those columns) and (-1)^ly from the row roll by n/2,
+
+        score(ly, lx) = 1024 * sum_{r,c} Re(Q[r][c] * w * (-1)^ly * exp(2 pi i (r ly + pos lx) / n)).
+
+    Only coefficients with |k| <= radius (frequency fy = r or r - n, fx = c,
+    with the packed Nyquist column at fx = -n/2) are kept and the lags run
+    over [-half, half]^2, row-major.  The real GEMM operand B[2k][s] =
+    Re Phi, B[2k+1][s] = -Im Phi is split into bf16 hi + lo and stored in the
+    v_mfma_f32_16x16x32_bf16 B-operand order [lag tile][k-step][lane][8]:
+    lane l of k-step ks holds rows 32 ks + 8 (l / 16) + 0..7 of column
+    16 tile + l % 16.
+    """
+
+    LAG_TILES = 4
+
+    def __init__(self, n: int, radius: float, half: int = 3):
+        n = int(n)
+        half = int(half)
+        side = 2 * half + 1
+        if side * side > 16 * self.LAG_TILES:
+            raise ValueError(f"half {half}: at most {16 * self.LAG_TILES} lags")
+        m = n // 2 + 1
+        self.n, self.half, self.radius = n, half, float(radius)
+        r = np.arange(n)
+        fy = np.where(r < n // 2, r, r - n)
+        c = np.arange(m)
+        fx = np.where(c < n // 2, c, -(n // 2))
+        inside = (fy[:, None] ** 2 + fx[None, :] ** 2) <= self.radius ** 2
+        self.kidx = np.flatnonzero(inside).astype(np.int32)
+        self.K = int(self.kidx.size)
+        self.Kpad = max(16, -(-self.K // 16) * 16)
+        rows, cols = self.kidx // m, self.kidx % m
+        pos = (cols + m // 2) % m
+        wt = np.where((pos == 0) | (pos == n // 2), 1.0, 2.0)
+        lags = np.arange(-half, half + 1)
+        ly = np.repeat(lags, side)
+        lx = np.tile(lags, side)
+        sgn = 1.0 - 2.0 * (np.abs(ly) % 2)
+        phase = (2.0 * np.pi / n) * (np.outer(rows, ly) + np.outer(pos, lx))
+        phi = (wt[:, None] * sgn[None, :]) * np.exp(1j * phase)          # [K, S]
+        S = side * side
+        breal = np.zeros((2 * self.Kpad, 16 * self.LAG_TILES), np.float64)
+        breal[0:2 * self.K:2, :S] = phi.real
+        breal[1:2 * self.K:2, :S] = -phi.imag
+        self.phi = phi
+        hi, lo = _split_bf16(breal)
+        self.b_hi = _mfma_b_layout(hi, self.Kpad, self.LAG_TILES)
+        self.b_lo = _mfma_b_layout(lo, self.Kpad, self.LAG_TILES)
+
+    @property
+    def n_lags(self) -> int:
+        return (2 * self.half + 1) ** 2
+
+
+def _bf16_bits(x32: np.ndarray) -> np.ndarray:
+    """float32 -> bfloat16 bit patterns, round to nearest even (finite inputs)."""
+    u = np.ascontiguousarray(x32, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def _bf16_value(bits: np.ndarray) -> np.ndarray:
+    return (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def _split_bf16(x: np.ndarray):
+    hi = _bf16_bits(x.astype(np.float32))
+    lo = _bf16_bits((x - _bf16_value(hi)).astype(np.float32))
+    return hi, lo
+
+
+def _mfma_b_layout(bits: np.ndarray, kpad: int, tiles: int) -> np.ndarray:
+    """[2 Kpad, 16 tiles] -> [tiles, Kpad/16, 64, 8] (lane l: rows 8 (l/16) + e, column l % 16)."""
+    ks = kpad // 16
+    b = bits.reshape(ks, 4, 8, tiles, 16)          # [ks][g][e][tile][i]
+    b = b.transpose(3, 0, 1, 4, 2)                 # [tile][ks][g][i][e]
+    return np.ascontiguousarray(b.reshape(tiles, ks, 64, 8))
+
+
+def small_shift_tiles(dir_off: np.ndarray, P: int, tile: int = 16) -> np.ndarray:
+    """[ntiles, 2] int32 (particle, first pair) covering each particle's M_i * P pairs."""
+    off = np.asarray(dir_off, np.int64)
+    npairs = np.diff(off) * int(P)
+    cnt = -(-npairs // tile)
+    b = np.repeat(np.arange(len(npairs), dtype=np.int64), cnt)
+    first = np.zeros(len(npairs) + 1, np.int64)
+    first[1:] = np.cumsum(cnt)
+    t = np.arange(int(first[-1]), dtype=np.int64) - first[b]
+    return np.ascontiguousarray(np.stack([b, t * tile], 1).astype(np.int32))

@@ -165,3 +165,77 @@ def score_sum(peak: torch.Tensor, best: Optional[torch.Tensor] = None) -> torch.
     check(lib().gis_score_sum(ptr(peak), ptr(best), int(peak.shape[0]), ptr(out), stream_of(peak.device)),
           "gis_score_sum")
     return out
+
+
+def gather_coefficients(src: torch.Tensor, kidx: torch.Tensor, kpad: int,
+                        scale: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """[rows, Kpad] complex64: the listed packed coefficients of each spectrum of
+    ``src`` ([..., n, n/2+1]), times ``scale`` per row if given, zero padded."""
+    require_hip(src, "src")
+    require_hip(kidx, "kidx")
+    if src.dtype != torch.complex64 or kidx.dtype != torch.int32:
+        raise ValueError("src must be complex64 and kidx int32")
+    plane = int(src.shape[-1]) * int(src.shape[-2])
+    rows = src.numel() // plane if plane else 0
+    K = int(kidx.numel())
+    if kpad < K or kpad % 16:
+        raise ValueError("kpad must be a multiple of 16 and >= len(kidx)")
+    if scale is not None:
+        require_hip(scale, "scale")
+        if scale.dtype != torch.float32 or scale.numel() != rows:
+            raise ValueError("scale must be float32 with one value per spectrum")
+    out = torch.empty((rows, kpad), dtype=torch.complex64, device=src.device)
+    check(lib().gis_gather_coefficients(ptr(src), plane, rows, ptr(kidx), K, int(kpad), ptr(scale), ptr(out),
+                                        stream_of(src.device)), "gis_gather_coefficients")
+    return out
+
+
+def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.Tensor, dir_idx: torch.Tensor,
+                    tiles: torch.Tensor, b_hi: torch.Tensor, b_lo: torch.Tensor, half: int,
+                    validate: bool = True) -> CorrelationResult:
+    """Split-bf16 small-shift search: best lag over [-half, half]^2 of every
+    (direction, psi) pair; peak_y / peak_x are lag + half.  ``validate``
+    checks dir_off / dir_idx / tiles against the operands on the host."""
+    for t, name in ((tk, "tk"), (pk, "pk"), (dir_off, "dir_off"), (dir_idx, "dir_idx"), (tiles, "tiles"),
+                    (b_hi, "b_hi"), (b_lo, "b_lo")):
+        require_hip(t, name)
+    kpad = int(tk.shape[1])
+    if int(pk.shape[1]) != kpad or pk.dtype != torch.complex64 or tk.dtype != torch.complex64:
+        raise ValueError("tk and pk must be complex64 with the same Kpad")
+    B = int(dir_off.shape[0]) - 1
+    if int(pk.shape[0]) != B * P:
+        raise ValueError("pk must hold P rows per particle")
+    if tuple(b_hi.shape) != tuple(b_lo.shape) or b_hi.numel() != 4 * (kpad // 16) * 64 * 8:
+        raise ValueError("lag table does not match Kpad")
+    if dir_off.dtype != torch.int32 or dir_idx.dtype != torch.int32 or tiles.dtype != torch.int32:
+        raise ValueError("dir_off / dir_idx / tiles must be int32")
+    if tiles.ndim != 2 or int(tiles.shape[1]) != 2:
+        raise ValueError("tiles must be [ntiles, 2]")
+    if validate:
+        off = dir_off.cpu().numpy().astype(np.int64)
+        total = int(dir_idx.shape[0])
+        if off[0] != 0 or off[-1] != total or np.any(np.diff(off) < 0):
+            raise ValueError("dir_off must start at 0, be non-decreasing and end at len(dir_idx)")
+        if total and (int(dir_idx.min()) < 0 or int(dir_idx.max()) >= int(tk.shape[0])):
+            raise ValueError("dir_idx out of range of the template coefficients")
+        t = tiles.cpu().numpy().astype(np.int64)
+        if len(t):
+            if t[:, 0].min() < 0 or t[:, 0].max() >= B:
+                raise ValueError("tile particle index out of range")
+            npairs = np.diff(off)[t[:, 0]] * P
+            if np.any(t[:, 1] < 0) or np.any(t[:, 1] >= npairs):
+                raise ValueError("tile first pair out of range")
+    dev = pk.device
+    total = int(dir_idx.shape[0])
+    scores = torch.empty((total, P), dtype=torch.float32, device=dev)
+    py = torch.empty((total, P), dtype=torch.int32, device=dev)
+    px = torch.empty((total, P), dtype=torch.int32, device=dev)
+    best = torch.empty((B,), dtype=torch.int32, device=dev)
+    peak = torch.empty((B,), dtype=torch.float32, device=dev)
+    mean = torch.empty((B,), dtype=torch.float64, device=dev)
+    std = torch.empty((B,), dtype=torch.float64, device=dev)
+    check(lib().gis_correlate_mixed(ptr(tk), ptr(pk), ptr(dir_off), ptr(dir_idx), B, int(P), kpad, ptr(tiles),
+                                    int(tiles.shape[0]), ptr(b_hi), ptr(b_lo), int(half), ptr(scores), ptr(py),
+                                    ptr(px), ptr(best), ptr(peak), ptr(mean), ptr(std), stream_of(dev)),
+          "gis_correlate_mixed")
+    return CorrelationResult(scores, py, px, best, peak, mean, std)

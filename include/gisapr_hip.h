@@ -131,6 +131,35 @@ int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, 
  */
 int gis_score_sum(const float* peak, const int* best, int B, double* out, void* stream);
 
+/*
+ * Gathers K listed coefficients (flat packed indices kidx) of `rows` spectra
+ * of `plane` complex64 values each into dst [rows][Kpad] complex64, zero
+ * padded, times scale[row] if scale is not NULL.  Feeds the small-shift
+ * search below; it takes the place of the full-plane products of
+ * correlate_templates_and_particle (v606:1418-1449) for coefficients outside
+ * the resolution disk.
+ */
+int gis_gather_coefficients(const void* src, size_t plane, int rows, const int* kidx, int K, int Kpad,
+                            const float* scale, void* dst, void* stream);
+
+/*
+ * Small-shift precision variant of correlate_templates_and_particle
+ * (v606:1418-1449) and the numerics of make_result_records (v606:1452-1491):
+ * the score at lag (ly, lx), |ly|, |lx| <= half <= 3, is evaluated as the
+ * linear functional 1024 * sum_k Re(T_k P_k Phi[k][lag]) over the gathered
+ * coefficients on split-bf16 MFMA (hi*hi + hi*lo + lo*hi, FP32 accumulation).
+ *   Tk [bank rows][Kpad], Pk [B*P][Kpad] complex64 (unit norm over the full plane)
+ *   tiles [ntiles] int2 (particle, first pair), 16 pairs per tile, covering every
+ *         particle's dir_off[b+1]-dir_off[b] times P pairs
+ *   Bhi / Blo [4][Kpad/16][64][8] bf16: split lag table in MFMA B-operand order
+ *   scores/peak_y/peak_x [dir_off[B]][P] (peak indices are lag + half),
+ *   best/peak/mean/std [B] as gis_correlate (FP64 mean/std).
+ */
+int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx, int B, int P,
+                        int Kpad, const int* tiles, int ntiles, const void* Bhi, const void* Blo, int half,
+                        float* scores, int* peak_y, int* peak_x, int* best, float* peak, double* mean,
+                        double* stdv, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

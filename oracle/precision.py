@@ -55,11 +55,25 @@ def scores_f64(T: np.ndarray, P: np.ndarray, phi: np.ndarray) -> np.ndarray:
     T [A, K], P [B, K] complex (unit-norm over the FULL packed plane, UNshifted)."""
     t = np.asarray(T, np.complex128)
     p = np.asarray(P, np.complex128)
-    q = t[:, None, :] * p[None, :, :]
-    return 1024.0 * np.einsum("abk,ks->abs", q, phi).real
+    q = (t[:, None, :] * p[None, :, :]).reshape(-1, t.shape[1])
+    return 1024.0 * (q @ np.asarray(phi, np.complex128)).real.reshape(t.shape[0], p.shape[0], -1)
 
 
 def best_over_lags(scores: np.ndarray):
     """First maximum over the lag axis: (score, index)."""
     k = np.argmax(scores, axis=-1)
     return np.take_along_axis(scores, k[..., None], -1)[..., 0], k
+
+
+def search_particle(T: np.ndarray, P: np.ndarray, phi: np.ndarray):
+    """One particle's small-shift search in FP64.
+
+    T [M, K] (its local directions), P [Psi, K].  Per (direction, psi) pair
+    the first maximum over lags; over the pairs (direction-major, as the
+    reference flattens scores, v606:1452-1491) the first maximum, and the
+    population mean / std of the pair maxima.
+    Returns (best pair, best lag, peak, mean, std, pair maxima [M*Psi], pair lags)."""
+    s = scores_f64(T, P, phi).reshape(T.shape[0] * P.shape[0], -1)
+    v, lag = best_over_lags(s)
+    b = int(np.argmax(v))
+    return b, int(lag[b]), float(v[b]), float(v.mean()), float(v.std()), v, lag

@@ -116,3 +116,65 @@ def test_disk_indices_radius():
     fy = np.where(r < n // 2, r, r - n)
     assert np.all(fy ** 2 + c ** 2 <= 100) and len(k) == sum(
         1 for y in range(-32, 32) for x in range(0, 33) if y * y + x * x <= 100)
+
+
+def test_small_shift_search_particle_brute_force_and_ties():
+    """Pair-major first maximum over (pair, lag), pinned by explicit loops,
+    with a duplicated direction planting exact ties."""
+    from oracle import precision as opr
+    n = 16
+    rng = np.random.default_rng(33)
+    kidx = opr.disk_indices(n, 5.0)
+    phi = opr.phi_table(n, kidx, 2)
+    K = len(kidx)
+    T = rng.standard_normal((3, K)) + 1j * rng.standard_normal((3, K))
+    T[2] = T[0]                                   # direction 2 ties direction 0
+    P = rng.standard_normal((2, K)) + 1j * rng.standard_normal((2, K))
+    b, lag, peak, mean, std, v, _ = opr.search_particle(T, P, phi)
+    best = None
+    allv = []
+    for j in range(3):
+        for p in range(2):
+            s = [1024 * sum((T[j, k] * P[p, k] * phi[k, l]).real for k in range(K)) for l in range(25)]
+            m = max(s)
+            allv.append(m)
+            if best is None or m > best[2]:
+                best = (j * 2 + p, s.index(m), m)
+    assert (b, lag) == best[:2] and b < 4
+    assert peak == pytest.approx(best[2], rel=1e-12)
+    assert mean == pytest.approx(np.mean(allv), rel=1e-12) and std == pytest.approx(np.std(allv), rel=1e-12)
+
+
+def test_small_shift_host_tables_match_oracle():
+    """The product's lag table (host side of the split-bf16 search): same
+    coefficient set and Phi as the oracle, hi + lo within 2^-16 relative of
+    Phi, and the MFMA B-operand layout inverts to the [2K x lags] operand."""
+    from gisapr_amd import host
+    from oracle import precision as opr
+    for n, R, half in [(32, 9.0, 3), (64, 20.5, 2), (256, 67.84, 3)]:
+        t = host.SmallShiftTables(n, R, half)
+        np.testing.assert_array_equal(t.kidx, opr.disk_indices(n, R))
+        phi = opr.phi_table(n, t.kidx.astype(np.int64), half)
+        np.testing.assert_allclose(t.phi, phi, atol=1e-12)
+        tiles, ks = t.b_hi.shape[:2]
+        def unlayout(b):
+            return host._bf16_value(b.reshape(tiles, ks, 4, 16, 8).transpose(1, 2, 4, 0, 3).reshape(ks * 32, -1))
+        B = unlayout(t.b_hi) + unlayout(t.b_lo)
+        S = (2 * half + 1) ** 2
+        assert np.abs(B[0:2 * t.K:2, :S] - phi.real).max() <= 2.0 ** -16 * 2
+        assert np.abs(B[1:2 * t.K:2, :S] + phi.imag).max() <= 2.0 ** -16 * 2
+        assert not B[2 * t.K:].any() and not B[:, S:].any()
+        hi_only = unlayout(t.b_hi)
+        assert np.abs(hi_only[0:2 * t.K:2, :S] - phi.real).max() > 1e-4  # lo carries real bits
+    K = host.SmallShiftTables(256, 256 * 1.06 / 4, 3).K   # 4 A at box 256, 1.06 A/px
+    assert 7000 < K < 7400
+
+
+def test_small_shift_tiles_cover_pairs():
+    from gisapr_amd import host
+    off = np.array([0, 0, 3, 10, 11])
+    t = host.small_shift_tiles(off, 5)
+    pairs = np.diff(off) * 5
+    for b in range(4):
+        mine = t[t[:, 0] == b, 1]
+        assert list(mine) == list(range(0, pairs[b], 16))
