@@ -1,14 +1,24 @@
 #!/usr/bin/env python
-"""Candidate-scoring throughput on one MI355X.
+"""Candidate-scoring throughput on one MI355X, FP32 FFT path and split-bf16 path.
 
-One step = one candidate evaluation on the box-256 workload (12,000-atom
-3-chain model, moving chain 3,000 atoms, 5,000 particles of 256x256, HEALPix
-order 3 local search +/-10 deg, psi step 3.75 deg, crop 16):
-move the chain (host), splat the full model, 3-D transform, 768-direction
-template bank, prepare all particle spectra (rotation, FFT, CTF), correlate
-every (particle, local direction, psi), per-particle reduction, and the
-float64 score sum.  Inputs (particle images, metadata) are resident in HBM
-before timing starts.
+Workload (box256): 12,000-atom 3-chain model, moving chain 3,000 atoms,
+5,000 particles of 256x256 at 1.06 A/px, HEALPix order 3 local search
++/-10 deg, psi step 3.75 deg (5 psi).  The particle stack is resident in
+HBM and its psi spectra are prepared once per dataset before timing (the
+time is reported as ``particle_prep_ms``).  One step = one candidate
+evaluation: move the chain (host), splat the full model, 3-D transform,
+768-direction template bank, search every (particle, local direction, psi)
+pair, per-particle reduction and the float64 score sum.
+
+Two legs run back to back on the same particles and candidate poses:
+  fp32   the v606 search: full-plane products, inverse FFT, max over the
+         16 x 16 lag crop (+/-8 px)
+  mixed  the small-shift search: coefficients inside 1/(4 A), 49 lags
+         (+/-3 px), split-bf16 MFMA with FP32 accumulation
+``value`` is the mixed leg's particle-orientation scores/s; the fp32 leg,
+their ratio, top-1 agreement between the legs, and the mixed leg's error
+against the same functional evaluated in float64 on a 256-particle subset
+are reported beside it.
 
 Prints ONE JSON line.  ``--gpus`` above 1 is not supported (one-GPU build).
 """
@@ -25,12 +35,19 @@ import numpy as np
 
 METRIC = "particle-orientation scores/s (mixed vs FP32 FFT, same top-1); p50/p99 ms"
 PEAK_FP32_MFMA_TFLOPS = 157.3
+PEAK_BF16_MFMA_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
 
 def correlation_flops_per_score(n: int) -> float:
-    """Algorithmic FP32 work of one score: conj-multiply + full C2R."""
+    """Algorithmic FP32 work of one v606 score: conj-multiply + full C2R."""
     return 6.0 * n * (n / 2 + 1) + 2.5 * n * n * math.log2(n * n)
+
+
+def mixed_flops_per_score(K: int, lags: int) -> float:
+    """Split-bf16 work of one small-shift score: 3 products x 4 flops per
+    complex coefficient and lag (real part only)."""
+    return 3.0 * 4.0 * K * lags
 
 
 def cpu_baseline(eng, ps, bank, sample: int):
@@ -42,11 +59,12 @@ def cpu_baseline(eng, ps, bank, sample: int):
     bank_pre = ofr.pre_shift(bank.cpu().numpy())
     mask = eng.mask.cpu().numpy() if eng.mask is not None else None
     ctf = opt.PackedCTF(eng.n, eng.new_apix, 2.7, 300.0, eng.ctf.kk)
+    sample = min(sample, ps.B)
     imgs = ps.images[:sample].cpu().numpy()
     dfo = ps.defocus[:sample].cpu().numpy()
     scores = 0
     t0 = time.perf_counter()
-    for b in range(min(sample, ps.B)):
+    for b in range(sample):
         idx = ps.local[b]
         if not idx:
             continue
@@ -58,9 +76,56 @@ def cpu_baseline(eng, ps, bank, sample: int):
         scores += sc.size
     dt = time.perf_counter() - t0
     return {"value": scores / dt, "unit": "scores/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (numpy float32, 1 thread) particle prep + correlation + record for the "
-                      f"first {sample} particles of the same workload, templates from the GPU bank: "
-                      f"{scores} scores in {dt:.1f} s"}
+            "sample": f"oracle (numpy float32, 1 thread) v606 FP32 path: particle prep + correlation + "
+                      f"record for the first {sample} particles of the same workload, templates from the "
+                      f"GPU bank: {scores} scores in {dt:.1f} s"}
+
+
+def top1(res, ps, center):
+    """Per particle (best flat pair, y shift, x shift); -1 pair for none."""
+    best = res.corr.best.cpu().numpy()
+    py = res.corr.peak_y.cpu().numpy().reshape(-1)
+    px = res.corr.peak_x.cpu().numpy().reshape(-1)
+    off = ps.dir_off.cpu().numpy()
+    out = np.full((ps.B, 3), -1, np.int64)
+    for b in range(ps.B):
+        if best[b] < 0:
+            continue
+        g = off[b] * ps.P + best[b]
+        out[b] = (best[b], center - py[g], center - px[g])
+    return out
+
+
+def precision_check(eng, ps, bank, res, subset: int, seed: int):
+    """Mixed-leg scores against the same truncated functional in float64
+    (torch complex128 on the device) on a seeded particle subset."""
+    import torch
+    tab = eng.small
+    dev = eng.device
+    rng = np.random.default_rng(seed)
+    pick = np.sort(rng.choice(ps.B, min(subset, ps.B), replace=False))
+    tk = eng.template_coefficients(bank)[:, :tab.K].to(torch.complex128)
+    pk = ps.coeffs[:, :tab.K].to(torch.complex128)
+    phi = torch.from_numpy(tab.phi).to(dev)
+    off = ps.dir_off.cpu().numpy()
+    idx = ps.dir_idx.cpu().numpy()
+    sc = res.corr.scores.cpu().numpy().reshape(-1)
+    best = res.corr.best.cpu().numpy()
+    P = ps.P
+    agree, worst, used = 0, 0.0, 0
+    for b in pick:
+        d = idx[off[b]:off[b + 1]]
+        if len(d) == 0:
+            continue
+        q = (tk[torch.from_numpy(d).to(dev)][:, None, :] * pk[b * P:(b + 1) * P][None]).reshape(-1, tab.K)
+        s = 1024.0 * (q @ phi).real                     # [pairs, lags]
+        v = s.max(dim=1).values.cpu().numpy()
+        got = sc[off[b] * P:off[b + 1] * P].astype(np.float64)
+        worst = max(worst, float(np.abs(got - v).max() / np.abs(v).max()))
+        agree += int(int(np.argmax(v)) == int(best[b]))
+        used += 1
+    return {"subset": int(used), "seed": seed, "top1_agree_fp64": agree / max(used, 1),
+            "max_rel_err_fp64": worst, "reference": "same functional in float64 (torch complex128)"}
 
 
 def main() -> int:
@@ -70,7 +135,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="box256")
     ap.add_argument("--particles", type=int, default=None)
-    ap.add_argument("--cpu-sample", type=int, default=100)
+    ap.add_argument("--path", choices=("both", "fp32", "mixed"), default="both")
+    ap.add_argument("--resolution", type=float, default=4.0, help="small-shift disk radius in A")
+    ap.add_argument("--precision-subset", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -91,9 +159,19 @@ def main() -> int:
                        local_range=wl.local_range, kk=wl.kk, voltage=wl.voltage, cs=wl.cs,
                        mask_radius=wl.mask_radius, mask_edge=wl.mask_edge, fsc_values=data["fsc"],
                        dir_rot=data["dir_rot"], dir_tilt=data["dir_tilt"], device=dev)
+    tab = eng.enable_small_shift(resolution=args.resolution, half=3)
     dfo = np.stack([data["dfu"], data["dfv"], data["dfa"]], 1)
     ps = eng.add_particles(data["images"], data["rot"], data["tilt"], data["psi"], defocus=dfo, prepare=False)
     del data["images"]
+    torch.cuda.synchronize()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record()
+    eng.prepare(ps)
+    e1.record()
+    eng.prepare_small_shift(ps)
+    e2.record()
+    torch.cuda.synchronize()
+    prep_ms, gather_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
     n_scores = ps.n_scores
     rng = np.random.default_rng(synthetic.SEED_CAND)
     poses = [synthetic.candidate_pose(seed=int(s), rot_deg=wl.cand_rot_deg, trans_A=wl.cand_trans_A)
@@ -101,70 +179,102 @@ def main() -> int:
     coords, numbers, chains = data["coords"], data["numbers"], data["chains"]
     setup_s = time.perf_counter() - t_setup
 
-    stages = ("density", "templates", "particles", "correlate")
-    ev_time = {k: 0.0 for k in stages}
+    def run_leg(search):
+        stages = ("density", "templates", "search")
+        ev_time = {k: 0.0 for k in stages}
 
-    def step(pose, timed):
-        xyz = move_chain(coords, chains, wl.moving_chain, pose)
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)] if timed else None
-        if timed:
-            ev[0].record()
-        vol = eng.density(xyz, numbers, wl.res)
-        if timed:
-            ev[1].record()
-        bank = eng.templates(vol)
-        if timed:
-            ev[2].record()
-        eng.prepare(ps)
-        if timed:
-            ev[3].record()
-        res = eng.search(bank, ps)
-        if timed:
-            ev[4].record()
-        return res, bank, ev
+        def step(pose, timed):
+            xyz = move_chain(coords, chains, wl.moving_chain, pose)
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)] if timed else None
+            if timed:
+                ev[0].record()
+            vol = eng.density(xyz, numbers, wl.res)
+            if timed:
+                ev[1].record()
+            bank = eng.templates(vol)
+            if timed:
+                ev[2].record()
+            res = search(bank, ps)
+            if timed:
+                ev[3].record()
+            return res, bank, ev
 
-    for i in range(args.warmup):
-        step(poses[i], False)
-    torch.cuda.synchronize()
-    evs = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        res, bank, ev = step(poses[args.warmup + i], True)
-        evs.append(ev)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    for ev in evs:
-        for k, name in enumerate(stages):
-            ev_time[name] += ev[k].elapsed_time(ev[k + 1])
-    stage_ms = {k: v / args.steps for k, v in ev_time.items()}
-    ms_per_step = dt * 1000.0 / args.steps
-    value = n_scores * args.steps / dt
-    score = float(res.score.cpu())
+        for i in range(args.warmup):
+            step(poses[i], False)
+        torch.cuda.synchronize()
+        evs = []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            res, bank, ev = step(poses[args.warmup + i], True)
+            evs.append(ev)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        for ev in evs:
+            for k, name in enumerate(stages):
+                ev_time[name] += ev[k].elapsed_time(ev[k + 1])
+        return {"scores_per_s": n_scores * args.steps / dt, "ms_per_step": dt * 1000.0 / args.steps,
+                "stage_ms": {k: v / args.steps for k, v in ev_time.items()},
+                "score": float(res.score.cpu())}, res, bank
 
-    corr_ms = stage_ms["correlate"]
-    flops = correlation_flops_per_score(eng.n) * n_scores
-    achieved = flops / (corr_ms * 1e-3) / 1e12
+    legs, results = {}, {}
+    if args.path in ("both", "fp32"):
+        legs["fp32"], results["fp32"], bank = run_leg(eng.search)
+    if args.path in ("both", "mixed"):
+        legs["mixed"], results["mixed"], bank = run_leg(eng.search_small_shift)
+
+    # roofline of each leg's search kernel (HIP events on the launch stream)
+    n = eng.n
+    if "fp32" in legs:
+        ms = legs["fp32"]["stage_ms"]["search"]
+        ach = correlation_flops_per_score(n) * n_scores / (ms * 1e-3) / 1e12
+        legs["fp32"]["roofline"] = {"bound": "mfma", "kernel": "correlate_kernel", "achieved": ach,
+                                    "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                                    "frac": ach / PEAK_FP32_MFMA_TFLOPS,
+                                    "flops_per_score": correlation_flops_per_score(n), "launch_ms": ms}
+    if "mixed" in legs:
+        ms = legs["mixed"]["stage_ms"]["search"]
+        fps = mixed_flops_per_score(tab.K, tab.n_lags)
+        ach = fps * n_scores / (ms * 1e-3) / 1e12
+        legs["mixed"]["roofline"] = {"bound": "mfma", "kernel": "mixed_correlate_kernel (+ template gather)",
+                                     "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
+                                     "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps,
+                                     "launch_ms": ms}
+    head = "mixed" if "mixed" in legs else "fp32"
+    main_leg = legs[head]
     out = {
-        "metric": METRIC, "value": value, "unit": "scores/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded generators; random-walk model)",
+        "metric": METRIC, "value": main_leg["scores_per_s"], "unit": "scores/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": main_leg["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3 (split hi+lo, fp32 accumulate)" if head == "mixed" else "fp32",
+        "data": "synthetic (seeded generators; random-walk model)",
         "config": {"workload": f"{wl.name}: {sum(wl.chain_sizes)}-atom model (moving chain "
                                f"{wl.chain_sizes[-1]}), {ps.B} particles {wl.box}x{wl.box} at {wl.apix} A/px, "
                                f"HEALPix order {wl.healpix_order} ({len(eng.dir_rot)} directions) local "
-                               f"+/-{wl.local_range} deg, psi step {wl.psi_step} ({ps.P} psi), crop {wl.crop}, "
-                               f"one candidate per step",
+                               f"+/-{wl.local_range} deg, psi step {wl.psi_step} ({ps.P} psi); one candidate "
+                               f"per step, particle spectra resident",
                    "box": wl.box, "particles": ps.B, "directions": len(eng.dir_rot), "psi": ps.P,
-                   "crop": wl.crop, "scores_per_step": n_scores, "path": "fp32-fft"},
-        "candidates_per_s": 1000.0 / ms_per_step,
-        "stage_ms": stage_ms,
-        "score": score,
+                   "scores_per_step": n_scores, "path": head,
+                   "fp32_crop": wl.crop, "mixed_lags": tab.n_lags, "mixed_K": tab.K,
+                   "mixed_resolution_A": args.resolution},
+        "candidates_per_s": 1000.0 / main_leg["ms_per_step"],
+        "particle_prep_ms": prep_ms, "coefficient_gather_ms": gather_ms,
+        "legs": legs,
         "setup_s": setup_s,
-        "roofline": {"bound": "mfma", "kernel": "correlate_kernel", "achieved": achieved,
-                     "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_MFMA_TFLOPS,
-                     "traffic": None,
-                     "flops_per_score": correlation_flops_per_score(eng.n), "launch_ms": corr_ms},
+        "roofline": dict(main_leg["roofline"], traffic=None),
         "cpu_baseline": None,
     }
+    if "fp32" in legs and "mixed" in legs:
+        out["speedup_mixed_vs_fp32"] = legs["mixed"]["scores_per_s"] / legs["fp32"]["scores_per_s"]
+        a = top1(results["fp32"], ps, eng.crop // 2)
+        b = top1(results["mixed"], ps, 3)
+        valid = a[:, 0] >= 0
+        out["top1_same_as_fp32"] = {
+            "orientation": float(np.mean((a[valid, 0] == b[valid, 0]))),
+            "orientation_and_shift": float(np.mean(np.all(a[valid] == b[valid], axis=1))),
+            "particles": int(valid.sum())}
+    if "mixed" in legs and args.precision_subset > 0:
+        out["precision"] = precision_check(eng, ps, bank, results["mixed"],
+                                           args.precision_subset, synthetic.SEED_CAND)
     if not args.no_cpu_baseline and args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(eng, ps, bank, args.cpu_sample)
     print(json.dumps(out), flush=True)

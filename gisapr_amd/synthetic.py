@@ -299,8 +299,9 @@ WORKLOADS: Dict[str, Workload] = {
     "box96": Workload("box96", 96, 1.5, 3.0, [1500, 600], "B", 200, 0.1, 2, 15.0, 7.5, 10,
                       mask_radius=44, confine=40.0),
     # Plain FP32 FFT search: box 256, 1.06 A/px, 12,000 atoms, 5,000 particles.
+    # Same image generator as box96 (shifts +/-3 px); the search covers +/-8 px (crop 16).
     "box256": Workload("box256", 256, 1.06, 2.12, [4500, 4500, 3000], "C", 5000, 0.05, 3, 10.0,
-                       3.75, 16, fsc_resolution=4.0, confine=80.0, max_shift=8),
+                       3.75, 16, fsc_resolution=4.0, confine=80.0),
     # Small fast cases for tests.
     "box64": Workload("box64", 64, 2.0, 4.0, [500, 200], "B", 24, 0.1, 2, 15.0, 7.5, 8,
                       mask_radius=28, confine=28.0),
