@@ -140,6 +140,9 @@ def main() -> int:
     ap.add_argument("--precision-subset", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="after the legs, run one device-to-device copy of this many MiB (a known-size kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
@@ -239,6 +242,13 @@ def main() -> int:
                                      "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
                                      "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps,
                                      "launch_ms": ms}
+    if args.copy_calibration_mb > 0:
+        src = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     head = "mixed" if "mixed" in legs else "fp32"
     main_leg = legs[head]
     out = {
