@@ -20,7 +20,9 @@ SIGNATURES = {
     "gis_version": (_i, []),
     "gis_init": (_i, [_i]),
     "gis_splat": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _d, _vp, _vp]),
-    "gis_volume_dft": (_i, [_vp, _i, _vp, _vp]),
+    "gis_volume_dft_bytes": (_sz, [_i]),
+    "gis_volume_dft_workspace": (_sz, [_i]),
+    "gis_volume_dft": (_i, [_vp, _i, _vp, _vp, _sz, _vp]),
     "gis_build_templates_workspace": (_sz, [_i, _i]),
     "gis_build_templates": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gis_prepare_particles_workspace": (_sz, [_i, _i]),
@@ -31,8 +33,8 @@ SIGNATURES = {
                            _vp, _sz, _vp]),
     "gis_score_sum": (_i, [_vp, _vp, _i, _vp, _vp]),
     "gis_gather_coefficients": (_i, [_vp, _sz, _i, _vp, _i, _i, _vp, _vp, _vp]),
-    "gis_correlate_mixed": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp,
-                                 _vp, _vp, _vp, _vp]),
+    "gis_correlate_mixed": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp,
+                                 _vp, _vp, _vp]),
 }
 
 

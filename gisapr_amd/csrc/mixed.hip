@@ -57,77 +57,99 @@ __device__ __forceinline__ void split8(const float (&x)[8], bf16x8& hi, bf16x8& 
   }
 }
 
-__global__ void __launch_bounds__(64)
+// One wave per (group of 16 direction rows, chunk of PT psi values).  Row r
+// is the r-th entry of dir_idx (particle b with dir_off[b] <= r <
+// dir_off[b+1]); M-tile p holds the pairs (row, psi p0 + p).  The template
+// coefficients of a row are loaded once per k-step and multiplied by the PT
+// particle spectra, and every lag-table fragment feeds PT M-tiles.
+template <int PT>
+__global__ void __launch_bounds__(256)
 mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__ Pk,
-                       const int* __restrict__ dir_off, const int* __restrict__ dir_idx,
-                       const int2* __restrict__ tiles, int P, int Kpad, const bf16x8* __restrict__ Bhi,
-                       const bf16x8* __restrict__ Blo, int half, float* __restrict__ scores,
-                       int* __restrict__ peak_y, int* __restrict__ peak_x) {
-  const int lane = threadIdx.x;
-  const int2 tile = tiles[blockIdx.x];
-  const int b = tile.x, pair0 = tile.y;
-  const int d0 = dir_off[b];
-  const int np = (dir_off[b + 1] - d0) * P;
+                       const int* __restrict__ dir_off, const int* __restrict__ dir_idx, int B, int nrows,
+                       int P, int Kpad, const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo, int half,
+                       float* __restrict__ scores, int* __restrict__ peak_y, int* __restrict__ peak_x) {
+  const int lane = threadIdx.x & 63;
+  const int group = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (group * 16 >= nrows) return;  // whole wave
+  const int p0 = blockIdx.y * PT;
   const int i = lane & 15, g = lane >> 4;
-  const int pair = min(pair0 + i, np - 1);
-  const int j = pair / P, p = pair - j * P;
-  const float4* trow = (const float4*)(Tk + (size_t)dir_idx[d0 + j] * Kpad);
-  const float4* prow = (const float4*)(Pk + ((size_t)b * P + p) * Kpad);
-  const int ksteps = Kpad / 16;
-  floatx4 acc[kLagTiles];
+  const int r = min(group * 16 + i, nrows - 1);
+  // particle of row r: last b with dir_off[b] <= r (empty particles share offsets)
+  int lo = 0, hi = B;  // invariant dir_off[lo] <= r < dir_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (dir_off[mid] <= r) lo = mid; else hi = mid;
+  }
+  const float4* trow = (const float4*)(Tk + (size_t)dir_idx[r] * Kpad);
+  const float4* prow[PT];
 #pragma unroll
-  for (int nt = 0; nt < kLagTiles; ++nt) acc[nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int p = 0; p < PT; ++p) prow[p] = (const float4*)(Pk + ((size_t)lo * P + min(p0 + p, P - 1)) * Kpad);
+  const int ksteps = Kpad / 16;
+  floatx4 acc[PT][kLagTiles];
+#pragma unroll
+  for (int p = 0; p < PT; ++p)
+#pragma unroll
+    for (int nt = 0; nt < kLagTiles; ++nt) acc[p][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   for (int ks = 0; ks < ksteps; ++ks) {
-    const int q4 = (ks * 16 + 4 * g) / 2;  // float4 index of this lane's 4 coefficients
+    const int q4 = ks * 8 + 2 * g;  // float4 index of this lane's 4 coefficients
     const float4 t0 = trow[q4], t1 = trow[q4 + 1];
-    const float4 p0 = prow[q4], p1 = prow[q4 + 1];
     const float tr[4] = {t0.x, t0.z, t1.x, t1.z}, ti[4] = {t0.y, t0.w, t1.y, t1.w};
-    const float pr[4] = {p0.x, p0.z, p1.x, p1.z}, pim[4] = {p0.y, p0.w, p1.y, p1.w};
-    float x[8];
+    bf16x8 ahi[PT], alo[PT];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      x[2 * c] = tr[c] * pr[c] - ti[c] * pim[c];
-      x[2 * c + 1] = tr[c] * pim[c] + ti[c] * pr[c];
+    for (int p = 0; p < PT; ++p) {
+      const float4 u0 = prow[p][q4], u1 = prow[p][q4 + 1];
+      const float pr[4] = {u0.x, u0.z, u1.x, u1.z}, pim[4] = {u0.y, u0.w, u1.y, u1.w};
+      float x[8];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        x[2 * c] = tr[c] * pr[c] - ti[c] * pim[c];
+        x[2 * c + 1] = tr[c] * pim[c] + ti[c] * pr[c];
+      }
+      split8(x, ahi[p], alo[p]);
     }
-    bf16x8 ahi, alo;
-    split8(x, ahi, alo);
 #pragma unroll
     for (int nt = 0; nt < kLagTiles; ++nt) {
       const size_t bi = ((size_t)nt * ksteps + ks) * 64 + lane;
       const bf16x8 bh = Bhi[bi], bl = Blo[bi];
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bh, acc[nt], 0, 0, 0);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bl, acc[nt], 0, 0, 0);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bh, acc[nt], 0, 0, 0);
+#pragma unroll
+      for (int p = 0; p < PT; ++p) {
+        acc[p][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[p], bh, acc[p][nt], 0, 0, 0);
+        acc[p][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[p], bl, acc[p][nt], 0, 0, 0);
+        acc[p][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[p], bh, acc[p][nt], 0, 0, 0);
+      }
     }
   }
 
-  // epilogue: lane (g, i) holds S[pair row 4g+v][lag column nt*16+i]; first maximum over lags
+  // epilogue: lane (g, i) holds S[row 4g+v][lag column nt*16+i] of each psi tile; first maximum over lags
   const int side = 2 * half + 1, nlag = side * side;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    float bv = -INFINITY;
-    int bs = 0x7fffffff;
+  for (int p = 0; p < PT; ++p) {
 #pragma unroll
-    for (int nt = 0; nt < kLagTiles; ++nt) {
-      const int s = nt * 16 + i;
-      if (s < nlag) {
-        const float val = acc[nt][v] * 1024.0f;
-        if (val > bv) { bv = val; bs = s; }  // s increases with nt: strict > keeps the first
+    for (int v = 0; v < 4; ++v) {
+      float bv = -INFINITY;
+      int bs = 0x7fffffff;
+#pragma unroll
+      for (int nt = 0; nt < kLagTiles; ++nt) {
+        const int s = nt * 16 + i;
+        if (s < nlag) {
+          const float val = acc[p][nt][v] * 1024.0f;
+          if (val > bv) { bv = val; bs = s; }  // s increases with nt: strict > keeps the first
+        }
       }
-    }
 #pragma unroll
-    for (int o = 8; o > 0; o >>= 1) {
-      const float ov = __shfl_xor(bv, o, 16);
-      const int os = __shfl_xor(bs, o, 16);
-      if (ov > bv || (ov == bv && os < bs)) { bv = ov; bs = os; }
-    }
-    const int pr_row = pair0 + 4 * g + v;
-    if (i == 0 && pr_row < np) {
-      const size_t out = (size_t)d0 * P + pr_row;
-      scores[out] = bv;
-      peak_y[out] = bs / side;  // crop index: lag + half
-      peak_x[out] = bs % side;
+      for (int o = 8; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(bv, o, 16);
+        const int os = __shfl_xor(bs, o, 16);
+        if (ov > bv || (ov == bv && os < bs)) { bv = ov; bs = os; }
+      }
+      const int row = group * 16 + 4 * g + v;
+      if (i == 0 && row < nrows && p0 + p < P) {
+        const size_t out = (size_t)row * P + p0 + p;
+        scores[out] = bv;
+        peak_y[out] = bs / side;  // crop index: lag + half
+        peak_x[out] = bs % side;
+      }
     }
   }
 }
@@ -189,30 +211,41 @@ GIS_API int gis_gather_coefficients(const void* src, size_t plane, int rows, con
 
 // Precision variant of correlate_templates_and_particle (v606:1418-1449)
 // over lags [-half, half]^2 (half <= 3) and a truncated coefficient set.
-//   Tk [Mtot][Kpad], Pk [B*P][Kpad] complex64 (unit-norm, unshifted)
-//   tiles [ntiles] int2 (particle, first pair) covering every particle's pairs
+//   Tk [bank rows][Kpad], Pk [B*P][Kpad] complex64 (unit-norm, unshifted)
+//   nrows = dir_off[B] = number of dir_idx entries (passed so no device read is needed)
 //   Bhi, Blo: split lag table, [4][Kpad/16][64] x 8 bf16 (host layout)
 //   scores / peak_y / peak_x [dir_off[B]][P] (peak indices are lag + half)
 //   best / peak / mean / std [B] as gis_correlate
 GIS_API int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx, int B, int P,
-                                int Kpad, const int* tiles, int ntiles, const void* Bhi, const void* Blo, int half,
-                                float* scores, int* peak_y, int* peak_x, int* best, float* peak, double* mean,
-                                double* stdv, void* stream) {
-  GIS_REQUIRE(B >= 0 && P >= 1 && ntiles >= 0, "gis_correlate_mixed: B=%d P=%d ntiles=%d", B, P, ntiles);
+                                int nrows, int Kpad, const void* Bhi, const void* Blo, int half, float* scores,
+                                int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv,
+                                void* stream) {
+  GIS_REQUIRE(B >= 0 && P >= 1 && nrows >= 0, "gis_correlate_mixed: B=%d P=%d nrows=%d", B, P, nrows);
   GIS_REQUIRE(Kpad % 16 == 0 && Kpad >= 16, "gis_correlate_mixed: Kpad %d must be a positive multiple of 16", Kpad);
   if (half < 0 || (2 * half + 1) * (2 * half + 1) > 16 * kLagTiles) {
     gis_set_error("gis_correlate_mixed: half %d gives more than %d lags", half, 16 * kLagTiles);
     return GIS_EUNSUPPORTED;
   }
   if (B == 0) return GIS_OK;
-  GIS_REQUIRE(Tk && Pk && dir_off && dir_idx && (ntiles == 0 || tiles) && Bhi && Blo && scores && peak_y && peak_x &&
-                  best && peak && mean && stdv,
+  GIS_REQUIRE(Tk && Pk && dir_off && dir_idx && Bhi && Blo && scores && peak_y && peak_x && best && peak && mean &&
+                  stdv,
               "gis_correlate_mixed: null pointer");
   hipStream_t st = (hipStream_t)stream;
-  if (ntiles > 0) {
-    hipLaunchKernelGGL(mixed_correlate_kernel, dim3(ntiles), dim3(64), 0, st, (const float2*)Tk, (const float2*)Pk,
-                       dir_off, dir_idx, (const int2*)tiles, P, Kpad, (const bf16x8*)Bhi, (const bf16x8*)Blo, half,
-                       scores, peak_y, peak_x);
+  if (nrows > 0) {
+    const int pt = std::min(P, 5);
+    const dim3 grid((unsigned)((nrows + 63) / 64), (unsigned)((P + pt - 1) / pt));
+#define GIS_MIXED_LAUNCH(PTV)                                                                                   \
+  hipLaunchKernelGGL(mixed_correlate_kernel<PTV>, grid, dim3(256), 0, st, (const float2*)Tk, (const float2*)Pk, \
+                     dir_off, dir_idx, B, nrows, P, Kpad, (const bf16x8*)Bhi, (const bf16x8*)Blo, half, scores,  \
+                     peak_y, peak_x)
+    switch (pt) {
+      case 1: GIS_MIXED_LAUNCH(1); break;
+      case 2: GIS_MIXED_LAUNCH(2); break;
+      case 3: GIS_MIXED_LAUNCH(3); break;
+      case 4: GIS_MIXED_LAUNCH(4); break;
+      default: GIS_MIXED_LAUNCH(5); break;
+    }
+#undef GIS_MIXED_LAUNCH
     GIS_LAUNCH_CHECK("gis_correlate_mixed");
   }
   hipLaunchKernelGGL(particle_tail_kernel, dim3(B), dim3(64), 0, st, dir_off, P, scores, best, peak, mean, stdv);

@@ -3,8 +3,9 @@
 // For every direction d (rotation R_d = Euler(rot, tilt, 0)^T) this computes
 // the unit-norm packed template of IntegratedProjector.build_templates
 // (v606:1077-1147):
-//   1. trilinear central-slice gather from the 3-D half spectrum with the
-//      Friedel conjugation where the rotated x < 0 (v606:856-896, 1099-1113)
+//   1. trilinear central-slice gather from the 3-D half spectrum (bricked
+//      layout of gis_volume_dft) with the Friedel conjugation where the
+//      rotated x < 0 (v606:856-896, 1099-1113)
 //   2. ifftshift rows, irfft2 (complex-to-real rule: the imaginary parts of
 //      columns 0 and N are dropped after the row transform), ifftshift, crop
 //      the padding (v606:1122-1130)
@@ -21,6 +22,7 @@
 // per-workgroup scratch slot in HBM (mostly L2/MALL resident).  Two real
 // rows travel in one complex FFT (real + i*imag) in the row passes.
 #include "common.h"
+#include "vdft_layout.h"
 
 #pragma clang fp contract(off)
 
@@ -28,8 +30,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kLdsFloat2 = 8192;  // 64 KiB of LDS per workgroup
-
-typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
 
 __device__ __forceinline__ int ring_of(int fy, int kx) {
   return (int)rintf(sqrtf((float)(fy * fy + kx * kx)));
@@ -86,11 +86,11 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     for (int k0 = 0; k0 < M2; k0 += T::TK1) {
       const int nk = min(T::TK1, M2 - k0);
       // Each wave samples blocks of 8 slice rows x (up to) 8 slice columns,
-      // so one load instruction touches a compact patch of the 3-D spectrum;
-      // the two x taps of a trilinear corner pair come from one 16-byte load.
-      // Branch-free: every tap index is clamped into the array; a clamped
-      // upper tap always carries weight 0 (the coordinate was clamped to the
-      // last index first), so all loads of an unrolled group are in flight.
+      // so one load instruction touches a compact patch of the 3-D spectrum,
+      // which is stored in 2x2x4 bricks (one cache line each).  Branch-free:
+      // every tap index is clamped into the array; a clamped upper tap
+      // always carries weight 0 (the coordinate was clamped to the last
+      // index first), so all eight loads of a sample are in flight together.
       const int ngroups = (nk + 7) / 8;
       for (int cg = 0; cg < ngroups; ++cg) {
 #pragma unroll 4
@@ -117,30 +117,27 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
           const float z0f = floorf(cz), y0f = floorf(cy), x0f = floorf(cx);
           const float tz = cz - z0f, ty = cy - y0f, tx = cx - x0f;
           const int z0 = (int)z0f, y0 = (int)y0f, x0 = (int)x0f;
-          const int kz0 = (z0 + N) % L2, kz1 = (min(z0 + 1, L2 - 1) + N) % L2;
-          const int ky0 = (y0 + N) % L2, ky1 = (min(y0 + 1, L2 - 1) + N) % L2;
-          const int xp = min(x0, M2 - 2);  // pair start; x0 = M2-1 takes the pair's second tap
-          const bool hi = x0 != xp;
-          const f4a8 v00 = *(const f4a8*)(D + ((size_t)kz0 * L2 + ky0) * M2 + xp);
-          const f4a8 v01 = *(const f4a8*)(D + ((size_t)kz0 * L2 + ky1) * M2 + xp);
-          const f4a8 v10 = *(const f4a8*)(D + ((size_t)kz1 * L2 + ky0) * M2 + xp);
-          const f4a8 v11 = *(const f4a8*)(D + ((size_t)kz1 * L2 + ky1) * M2 + xp);
+          const size_t pz0 = vdft_brick_z<N>((z0 + N) % L2), pz1 = vdft_brick_z<N>((min(z0 + 1, L2 - 1) + N) % L2);
+          const size_t py0 = vdft_brick_y<N>((y0 + N) % L2), py1 = vdft_brick_y<N>((min(y0 + 1, L2 - 1) + N) % L2);
+          const size_t px0 = vdft_brick_x(x0), px1 = vdft_brick_x(min(x0 + 1, M2 - 1));
+          const float2 v000 = D[pz0 + py0 + px0], v001 = D[pz0 + py0 + px1];
+          const float2 v010 = D[pz0 + py1 + px0], v011 = D[pz0 + py1 + px1];
+          const float2 v100 = D[pz1 + py0 + px0], v101 = D[pz1 + py0 + px1];
+          const float2 v110 = D[pz1 + py1 + px0], v111 = D[pz1 + py1 + px1];
           const float wx0 = 1.0f - tx, wx1 = tx;
           const float w00 = (1.0f - tz) * (1.0f - ty), w01 = (1.0f - tz) * ty;
           const float w10 = tz * (1.0f - ty), w11 = tz * ty;
           float2 acc = make_float2(0.f, 0.f);
-          auto tap = [&](const f4a8& v, float wzy) {
-            const float a0x = hi ? v.z : v.x, a0y = hi ? v.w : v.y;
-            const float a1x = hi ? 0.f : v.z, a1y = hi ? 0.f : v.w;
-            acc.x += (wzy * wx0) * a0x;
-            acc.y += (wzy * wx0) * a0y;
-            acc.x += (wzy * wx1) * a1x;
-            acc.y += (wzy * wx1) * a1y;
+          auto tap = [&](const float2& a0, const float2& a1, float wzy) {
+            acc.x += (wzy * wx0) * a0.x;
+            acc.y += (wzy * wx0) * a0.y;
+            acc.x += (wzy * wx1) * a1.x;
+            acc.y += (wzy * wx1) * a1.y;
           };
-          tap(v00, w00);
-          tap(v01, w01);
-          tap(v10, w10);
-          tap(v11, w11);
+          tap(v000, v001, w00);
+          tap(v010, v011, w01);
+          tap(v100, v101, w10);
+          tap(v110, v111, w11);
           if (cj) acc.y = -acc.y;
           if (active) bufA[q * (L2 + 1) + ip] = acc;
         }

@@ -12,7 +12,16 @@
 // that hold data, the y pass only the N data planes and synthesises the zero
 // rows, and the z pass synthesises the zero planes.  Traffic per transform
 // is ~ 4N^3 + 2*8*N^2(N+1) + 2*8*2N^2(N+1) bytes.
+//
+// Output layout (bricked): the x and y passes work on a natural-order
+// workspace [2N][2N][N+1]; the z pass writes the result in bricks of
+// 2 (z) x 2 (y) x 4 (x) complex values = one 128-byte cache line, so the 2x2x2
+// trilinear cell of a central-slice sample spans ~2.8 lines instead of four
+// 16-element x rows, whatever the slice orientation:
+//   D[((((kz/2) * N + ky/2) * XB + kx/4) * 2 + kz%2) * 2 + ky%2) * 4 + kx%4],
+//   XB = N/4 + 1 (x padded to N+4; the padding is never read).
 #include "common.h"
+#include "vdft_layout.h"
 
 // Row x' of the shifted padded volume holds padded column (x'+N) mod 2N,
 // which is data column (x'+N) mod 2N - N/2 when that lies in [0, N).
@@ -121,10 +130,11 @@ vdft_y_pass(float2* __restrict__ D) {
   }
 }
 
-// z pass: column FFTs along z (stride 2N*M); zero planes are synthesised.
+// z pass: column FFTs along z (stride 2N*M) of the natural-order workspace W;
+// zero planes are synthesised; the result goes to D in the bricked layout.
 template <int N>
 __global__ void __launch_bounds__(256)
-vdft_z_pass(float2* __restrict__ D) {
+vdft_z_pass(const float2* __restrict__ W, float2* __restrict__ D) {
   constexpr int L = 2 * N;
   constexpr int M = N + 1;
   constexpr int TK = (8192 - L) / (2 * (L + 1));
@@ -143,53 +153,71 @@ vdft_z_pass(float2* __restrict__ D) {
   for (int t = tid; t < L * TK; t += nthr) {
     const int zs = t / TK, q = t - zs * TK;
     float2 v = make_float2(0.f, 0.f);
-    if (q < nk && shifted_to_data(zs, N) >= 0) v = D[zs * plane + (size_t)ky * M + k0 + q];
+    if (q < nk && shifted_to_data(zs, N) >= 0) v = W[zs * plane + (size_t)ky * M + k0 + q];
     bufA[q * SP + zs] = v;
   }
   __syncthreads();
   float2* res = fft_lds<L, false>(bufA, bufB, nk, SP, tw, tid, nthr);
+  const size_t ypart = vdft_brick_y<N>(ky);
   for (int t = tid; t < L * TK; t += nthr) {
     const int kz = t / TK, q = t - kz * TK;
-    if (q < nk) D[kz * plane + (size_t)ky * M + k0 + q] = res[q * SP + kz];
+    if (q < nk) D[vdft_brick_z<N>(kz) + ypart + vdft_brick_x(k0 + q)] = res[q * SP + kz];
   }
 }
 
 template <int N>
-static int launch_volume_dft(const float* vol, float2* D, hipStream_t st) {
+static int launch_volume_dft(const float* vol, float2* D, float2* Wk, hipStream_t st) {
   constexpr int L = 2 * N;
   constexpr int M = N + 1;
   constexpr int RB = (8192 - L) / (2 * L);
   constexpr int TK = (8192 - L) / (2 * (L + 1));
   const int pairs = N * N / 2;
   const size_t lx = (size_t)(L + 2 * RB * L) * sizeof(float2);
-  hipLaunchKernelGGL(vdft_x_pass<N>, dim3((pairs + RB - 1) / RB), dim3(256), lx, st, vol, D);
+  hipLaunchKernelGGL(vdft_x_pass<N>, dim3((pairs + RB - 1) / RB), dim3(256), lx, st, vol, Wk);
   GIS_LAUNCH_CHECK("gis_volume_dft x");
   const size_t lyz = (size_t)(L + 2 * TK * (L + 1)) * sizeof(float2);
   const int ktiles = (M + TK - 1) / TK;
-  hipLaunchKernelGGL(vdft_y_pass<N>, dim3(ktiles, N), dim3(256), lyz, st, D);
+  hipLaunchKernelGGL(vdft_y_pass<N>, dim3(ktiles, N), dim3(256), lyz, st, Wk);
   GIS_LAUNCH_CHECK("gis_volume_dft y");
-  hipLaunchKernelGGL(vdft_z_pass<N>, dim3(ktiles, L), dim3(256), lyz, st, D);
+  hipLaunchKernelGGL(vdft_z_pass<N>, dim3(ktiles, L), dim3(256), lyz, st, (const float2*)Wk, D);
   GIS_LAUNCH_CHECK("gis_volume_dft z");
   return GIS_OK;
 }
 
+static bool vdft_box_supported(int N) {
+  return N == 32 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256;
+}
+
+// Bytes of the bricked spectrum and of the natural-order workspace at box N.
+GIS_API size_t gis_volume_dft_bytes(int N) {
+  return N > 0 ? (size_t)4 * N * N * (N / 4 + 1) * 4 * sizeof(float2) : 0;
+}
+GIS_API size_t gis_volume_dft_workspace(int N) {
+  return N > 0 ? (size_t)4 * N * N * (N + 1) * sizeof(float2) : 0;
+}
+
 // Replaces the pad / sinc^2 / fftshift / rfftn prefix of
 // IntegratedProjector.build_templates (v606:1068-1073).
-// `volume` is [N][N][N] float32 ([z][y][x]); `dft` receives [2N][2N][N+1]
-// complex64 in natural frequency order.
-GIS_API int gis_volume_dft(const float* volume, int N, void* dft, void* stream) {
-  GIS_REQUIRE(volume != nullptr && dft != nullptr, "gis_volume_dft: null pointer");
+// `volume` is [N][N][N] float32 ([z][y][x]); `dft` receives the [2N][2N][N+1]
+// complex64 half spectrum in natural frequency order, stored in 2x2x4 bricks
+// (gis_volume_dft_bytes(N) bytes); `ws` holds gis_volume_dft_workspace(N) bytes.
+GIS_API int gis_volume_dft(const float* volume, int N, void* dft, void* ws, size_t ws_bytes, void* stream) {
+  GIS_REQUIRE(volume != nullptr && dft != nullptr && ws != nullptr, "gis_volume_dft: null pointer");
+  if (!vdft_box_supported(N)) {
+    gis_set_error("gis_volume_dft: box %d not in {32, 64, 96, 128, 192, 256}", N);
+    return GIS_EUNSUPPORTED;
+  }
+  GIS_REQUIRE(ws_bytes >= gis_volume_dft_workspace(N), "gis_volume_dft: workspace %zu < %zu bytes", ws_bytes,
+              gis_volume_dft_workspace(N));
   hipStream_t st = (hipStream_t)stream;
   float2* D = (float2*)dft;
+  float2* W = (float2*)ws;
   switch (N) {
-    case 32: return launch_volume_dft<32>(volume, D, st);
-    case 64: return launch_volume_dft<64>(volume, D, st);
-    case 96: return launch_volume_dft<96>(volume, D, st);
-    case 128: return launch_volume_dft<128>(volume, D, st);
-    case 192: return launch_volume_dft<192>(volume, D, st);
-    case 256: return launch_volume_dft<256>(volume, D, st);
-    default:
-      gis_set_error("gis_volume_dft: box %d not in {32, 64, 96, 128, 192, 256}", N);
-      return GIS_EUNSUPPORTED;
+    case 32: return launch_volume_dft<32>(volume, D, W, st);
+    case 64: return launch_volume_dft<64>(volume, D, W, st);
+    case 96: return launch_volume_dft<96>(volume, D, W, st);
+    case 128: return launch_volume_dft<128>(volume, D, W, st);
+    case 192: return launch_volume_dft<192>(volume, D, W, st);
+    default: return launch_volume_dft<256>(volume, D, W, st);
   }
 }

@@ -36,7 +36,6 @@ class ParticleSet:
     spectra: Optional[torch.Tensor] = None   # [B, P, n, n/2+1] complex64
     inv_norm: Optional[torch.Tensor] = None  # [B, P] float32
     coeffs: Optional[torch.Tensor] = None    # [B*P, Kpad] complex64, unit norm (small-shift search)
-    tiles: Optional[torch.Tensor] = None     # [ntiles, 2] int32 (small-shift search)
 
     @property
     def B(self) -> int:
@@ -52,7 +51,7 @@ class ParticleSet:
 
     def nbytes_resident(self) -> int:
         t = [self.images, self.rotp, self.dir_off, self.dir_idx, self.spectra, self.inv_norm, self.defocus,
-             self.coeffs, self.tiles]
+             self.coeffs]
         return sum(x.numel() * x.element_size() for x in t if x is not None)
 
 
@@ -193,7 +192,6 @@ class SearchEngine:
         if ps.spectra is None:
             self.prepare(ps)
         ps.coeffs = kernels.gather_coefficients(ps.spectra, self.small_kidx, self.small.Kpad, ps.inv_norm)
-        ps.tiles = torch.from_numpy(host.small_shift_tiles(ps.dir_off.cpu().numpy(), ps.P)).to(self.device)
         if drop_spectra:
             ps.spectra = None
         return ps
@@ -210,7 +208,7 @@ class SearchEngine:
         if int(bank.shape[0]) != len(self.dir_rot):
             raise ValueError("template bank does not cover the engine's direction table")
         tk = self.template_coefficients(bank)
-        corr = kernels.correlate_mixed(tk, ps.coeffs, ps.P, ps.dir_off, ps.dir_idx, ps.tiles, self.small_bhi,
+        corr = kernels.correlate_mixed(tk, ps.coeffs, ps.P, ps.dir_off, ps.dir_idx, self.small_bhi,
                                        self.small_blo, self.small.half, validate=False)
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.small.half)
 

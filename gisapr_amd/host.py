@@ -298,14 +298,3 @@ def _mfma_b_layout(bits: np.ndarray, kpad: int, tiles: int) -> np.ndarray:
     b = b.transpose(3, 0, 1, 4, 2)                 # [tile][ks][g][i][e]
     return np.ascontiguousarray(b.reshape(tiles, ks, 64, 8))
 
-
-def small_shift_tiles(dir_off: np.ndarray, P: int, tile: int = 16) -> np.ndarray:
-    """[ntiles, 2] int32 (particle, first pair) covering each particle's M_i * P pairs."""
-    off = np.asarray(dir_off, np.int64)
-    npairs = np.diff(off) * int(P)
-    cnt = -(-npairs // tile)
-    b = np.repeat(np.arange(len(npairs), dtype=np.int64), cnt)
-    first = np.zeros(len(npairs) + 1, np.int64)
-    first[1:] = np.cumsum(cnt)
-    t = np.arange(int(first[-1]), dtype=np.int64) - first[b]
-    return np.ascontiguousarray(np.stack([b, t * tile], 1).astype(np.int32))

@@ -41,22 +41,38 @@ def splat(xyz: torch.Tensor, numbers: torch.Tensor, box: int, width: int, apix: 
 
 
 def volume_dft(volume: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
-    """[N]^3 float32 -> [2N, 2N, N+1] complex64 (pad, sinc^2, shift, rfftn)."""
+    """[N]^3 float32 -> bricked half spectrum of [2N, 2N, N+1] complex64
+    (pad, sinc^2, shift, rfftn); ``volume_dft_natural`` unpacks it."""
     require_hip(volume, "volume")
     N = int(volume.shape[0])
     if volume.ndim != 3 or len(set(volume.shape)) != 1 or volume.dtype != torch.float32:
         raise ValueError("volume must be a cubic float32 tensor")
+    nb = int(lib().gis_volume_dft_bytes(N)) // 8
     if out is None:
-        out = torch.empty((2 * N, 2 * N, N + 1), dtype=torch.complex64, device=volume.device)
-    check(lib().gis_volume_dft(ptr(volume), N, ptr(out), stream_of(volume.device)), "gis_volume_dft")
+        out = torch.empty((nb,), dtype=torch.complex64, device=volume.device)
+    elif out.numel() != nb or out.dtype != torch.complex64:
+        raise ValueError(f"out must be {nb} complex64 values")
+    wsb = int(lib().gis_volume_dft_workspace(N))
+    ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=volume.device)
+    check(lib().gis_volume_dft(ptr(volume), N, ptr(out), ptr(ws), wsb, stream_of(volume.device)),
+          "gis_volume_dft")
     return out
+
+
+def volume_dft_natural(dft: torch.Tensor, N: int) -> torch.Tensor:
+    """Unpack the bricked spectrum to natural order [2N, 2N, N+1] (for inspection and tests)."""
+    xb = N // 4 + 1
+    b = dft.view(N, N, xb, 2, 2, 4).permute(0, 3, 1, 4, 2, 5).reshape(2 * N, 2 * N, 4 * xb)
+    return b[..., :N + 1].contiguous()
 
 
 def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Optional[torch.Tensor],
                     fsc_packed: Optional[torch.Tensor], with_norms: bool = False):
     """Unit-norm packed template bank [M, n, n/2+1] complex64 (unshifted)."""
     require_hip(dft, "dft")
-    N = int(dft.shape[0]) // 2
+    N = next((b for b in SUPPORTED_BOXES if int(lib().gis_volume_dft_bytes(b)) == dft.numel() * 8), 0)
+    if not N or dft.dtype != torch.complex64:
+        raise ValueError("dft must be a bricked spectrum from volume_dft")
     M = int(rts.shape[0])
     require_hip(rts, "rts")
     n = int(new_box)
@@ -191,13 +207,12 @@ def gather_coefficients(src: torch.Tensor, kidx: torch.Tensor, kpad: int,
 
 
 def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.Tensor, dir_idx: torch.Tensor,
-                    tiles: torch.Tensor, b_hi: torch.Tensor, b_lo: torch.Tensor, half: int,
-                    validate: bool = True) -> CorrelationResult:
+                    b_hi: torch.Tensor, b_lo: torch.Tensor, half: int, validate: bool = True) -> CorrelationResult:
     """Split-bf16 small-shift search: best lag over [-half, half]^2 of every
     (direction, psi) pair; peak_y / peak_x are lag + half.  ``validate``
-    checks dir_off / dir_idx / tiles against the operands on the host."""
-    for t, name in ((tk, "tk"), (pk, "pk"), (dir_off, "dir_off"), (dir_idx, "dir_idx"), (tiles, "tiles"),
-                    (b_hi, "b_hi"), (b_lo, "b_lo")):
+    checks dir_off / dir_idx against the operands on the host."""
+    for t, name in ((tk, "tk"), (pk, "pk"), (dir_off, "dir_off"), (dir_idx, "dir_idx"), (b_hi, "b_hi"),
+                    (b_lo, "b_lo")):
         require_hip(t, name)
     kpad = int(tk.shape[1])
     if int(pk.shape[1]) != kpad or pk.dtype != torch.complex64 or tk.dtype != torch.complex64:
@@ -207,26 +222,16 @@ def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.T
         raise ValueError("pk must hold P rows per particle")
     if tuple(b_hi.shape) != tuple(b_lo.shape) or b_hi.numel() != 4 * (kpad // 16) * 64 * 8:
         raise ValueError("lag table does not match Kpad")
-    if dir_off.dtype != torch.int32 or dir_idx.dtype != torch.int32 or tiles.dtype != torch.int32:
-        raise ValueError("dir_off / dir_idx / tiles must be int32")
-    if tiles.ndim != 2 or int(tiles.shape[1]) != 2:
-        raise ValueError("tiles must be [ntiles, 2]")
+    if dir_off.dtype != torch.int32 or dir_idx.dtype != torch.int32:
+        raise ValueError("dir_off / dir_idx must be int32")
+    total = int(dir_idx.shape[0])
     if validate:
         off = dir_off.cpu().numpy().astype(np.int64)
-        total = int(dir_idx.shape[0])
         if off[0] != 0 or off[-1] != total or np.any(np.diff(off) < 0):
             raise ValueError("dir_off must start at 0, be non-decreasing and end at len(dir_idx)")
         if total and (int(dir_idx.min()) < 0 or int(dir_idx.max()) >= int(tk.shape[0])):
             raise ValueError("dir_idx out of range of the template coefficients")
-        t = tiles.cpu().numpy().astype(np.int64)
-        if len(t):
-            if t[:, 0].min() < 0 or t[:, 0].max() >= B:
-                raise ValueError("tile particle index out of range")
-            npairs = np.diff(off)[t[:, 0]] * P
-            if np.any(t[:, 1] < 0) or np.any(t[:, 1] >= npairs):
-                raise ValueError("tile first pair out of range")
     dev = pk.device
-    total = int(dir_idx.shape[0])
     scores = torch.empty((total, P), dtype=torch.float32, device=dev)
     py = torch.empty((total, P), dtype=torch.int32, device=dev)
     px = torch.empty((total, P), dtype=torch.int32, device=dev)
@@ -234,8 +239,8 @@ def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.T
     peak = torch.empty((B,), dtype=torch.float32, device=dev)
     mean = torch.empty((B,), dtype=torch.float64, device=dev)
     std = torch.empty((B,), dtype=torch.float64, device=dev)
-    check(lib().gis_correlate_mixed(ptr(tk), ptr(pk), ptr(dir_off), ptr(dir_idx), B, int(P), kpad, ptr(tiles),
-                                    int(tiles.shape[0]), ptr(b_hi), ptr(b_lo), int(half), ptr(scores), ptr(py),
-                                    ptr(px), ptr(best), ptr(peak), ptr(mean), ptr(std), stream_of(dev)),
+    check(lib().gis_correlate_mixed(ptr(tk), ptr(pk), ptr(dir_off), ptr(dir_idx), B, int(P), total, kpad,
+                                    ptr(b_hi), ptr(b_lo), int(half), ptr(scores), ptr(py), ptr(px), ptr(best),
+                                    ptr(peak), ptr(mean), ptr(std), stream_of(dev)),
           "gis_correlate_mixed")
     return CorrelationResult(scores, py, px, best, peak, mean, std)

@@ -50,9 +50,16 @@ int gis_splat(const float* xyz, const float* numbers, int n_atoms, int box, int 
 /*
  * (b, prefix) Pad / sinc^2 / fftshift / rfftn of the model volume.
  * Replaces IntegratedProjector.build_templates v606:1068-1073.
- * volume [N][N][N] float32 -> dft [2N][2N][N+1] complex64 (natural order).
+ * volume [N][N][N] float32 -> dft: the [2N][2N][N+1] complex64 half spectrum
+ * in natural frequency order, stored in bricks of 2 (z) x 2 (y) x 4 (x)
+ * values: element (kz, ky, kx) is at
+ *   ((((kz/2) * N + ky/2) * XB + kx/4) * 2 + kz%2) * 2 + ky%2) * 4 + kx%4,
+ * XB = N/4 + 1 (gis_volume_dft_bytes(N) bytes).  ws: gis_volume_dft_workspace(N)
+ * bytes of scratch (natural-order intermediate).
  */
-int gis_volume_dft(const float* volume, int N, void* dft, void* stream);
+size_t gis_volume_dft_bytes(int N);
+size_t gis_volume_dft_workspace(int N);
+int gis_volume_dft(const float* volume, int N, void* dft, void* ws, size_t ws_bytes, void* stream);
 
 /* Workspace bytes gis_build_templates needs for M directions at box N. */
 size_t gis_build_templates_workspace(int N, int M);
@@ -63,7 +70,7 @@ size_t gis_build_templates_workspace(int N, int M);
  * gather with Friedel conjugation, irfft2, crop, whitening
  * (ProjectionWhitening batch==1 path v606:938-992), mask, fft2, crop to n,
  * pack, x FSC, unit L2 norm.
- *   dft  [2N][2N][N+1] complex64 from gis_volume_dft
+ *   dft  bricked half spectrum from gis_volume_dft
  *   rts  [M][3][3] float32: Euler(rot, tilt, 0)^T per direction (v606:214-215)
  *   mask [N][N] float32 or NULL;  fsc [n][n/2+1] float32 or NULL (ones)
  *   bank [M][n][n/2+1] complex64 out;  norms [M] float32 out or NULL
@@ -149,16 +156,15 @@ int gis_gather_coefficients(const void* src, size_t plane, int rows, const int* 
  * linear functional 1024 * sum_k Re(T_k P_k Phi[k][lag]) over the gathered
  * coefficients on split-bf16 MFMA (hi*hi + hi*lo + lo*hi, FP32 accumulation).
  *   Tk [bank rows][Kpad], Pk [B*P][Kpad] complex64 (unit norm over the full plane)
- *   tiles [ntiles] int2 (particle, first pair), 16 pairs per tile, covering every
- *         particle's dir_off[b+1]-dir_off[b] times P pairs
+ *   nrows = dir_off[B] (the number of dir_idx entries)
  *   Bhi / Blo [4][Kpad/16][64][8] bf16: split lag table in MFMA B-operand order
  *   scores/peak_y/peak_x [dir_off[B]][P] (peak indices are lag + half),
  *   best/peak/mean/std [B] as gis_correlate (FP64 mean/std).
  */
 int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx, int B, int P,
-                        int Kpad, const int* tiles, int ntiles, const void* Bhi, const void* Blo, int half,
-                        float* scores, int* peak_y, int* peak_x, int* best, float* peak, double* mean,
-                        double* stdv, void* stream);
+                        int nrows, int Kpad, const void* Bhi, const void* Blo, int half, float* scores,
+                        int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv,
+                        void* stream);
 
 #ifdef __cplusplus
 }

@@ -48,5 +48,7 @@ def test_version_and_argument_errors_without_gpu(library):
     rc = library.gis_correlate(None, None, None, None, None, 1, 5, 64, 18, None, None, None, None, None,
                                None, None, None, 0, None)
     assert rc == -2 and b"crop" in library.gis_last_error()
-    rc = library.gis_volume_dft(None, 32, None, None)
+    rc = library.gis_volume_dft(None, 32, None, None, 0, None)
     assert rc == -1
+    assert library.gis_volume_dft_bytes(256) == 16 * 256 * 256 * 65 * 8
+    assert library.gis_volume_dft_workspace(256) == 4 * 256 * 256 * 257 * 8

@@ -79,3 +79,17 @@ def test_healpix_counts_and_poles():
                   np.sin(np.radians(tilt)) * np.sin(np.radians(rot)), np.cos(np.radians(tilt))], 1)
     assert np.linalg.norm(v.mean(0)) < 1e-6
     assert len(np.unique(np.round(v, 9), axis=0)) == len(v)
+
+
+def test_volume_dft_brick_unpack_is_the_documented_index():
+    """kernels.volume_dft_natural inverts the bricked index of gis_volume_dft."""
+    import torch
+    from gisapr_amd import kernels
+    N = 32
+    xb = N // 4 + 1
+    nat = torch.arange(2 * N * 2 * N * (N + 1), dtype=torch.float32).reshape(2 * N, 2 * N, N + 1)
+    bricked = torch.zeros(16 * N * N * xb, dtype=torch.complex64)
+    kz, ky, kx = np.meshgrid(np.arange(2 * N), np.arange(2 * N), np.arange(N + 1), indexing="ij")
+    idx = ((((kz // 2) * N + ky // 2) * xb + kx // 4) * 2 + kz % 2) * 2 * 4 + (ky % 2) * 4 + kx % 4
+    bricked[torch.from_numpy(idx.ravel())] = nat.reshape(-1).to(torch.complex64)
+    np.testing.assert_array_equal(kernels.volume_dft_natural(bricked, N).real.numpy(), nat.numpy())

@@ -169,12 +169,3 @@ def test_small_shift_host_tables_match_oracle():
     K = host.SmallShiftTables(256, 256 * 1.06 / 4, 3).K   # 4 A at box 256, 1.06 A/px
     assert 7000 < K < 7400
 
-
-def test_small_shift_tiles_cover_pairs():
-    from gisapr_amd import host
-    off = np.array([0, 0, 3, 10, 11])
-    t = host.small_shift_tiles(off, 5)
-    pairs = np.diff(off) * 5
-    for b in range(4):
-        mine = t[t[:, 0] == b, 1]
-        assert list(mine) == list(range(0, pairs[b], 16))
