@@ -29,7 +29,7 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kLdsFloat2 = 8192;  // 64 KiB of LDS per workgroup
+constexpr int kLdsFloat2 = 6600;  // 52.8 KB of LDS: three workgroups per CU (a 768-direction bank in one round)
 
 __device__ __forceinline__ int ring_of(int fy, int kx) {
   return (int)rintf(sqrtf((float)(fy * fy + kx * kx)));
@@ -193,13 +193,16 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     }
 
     // ---- phase 3: forward column FFT (N) -> projection spectrum F, and the
-    // ring sums of |F| over the full plane.  Thread `ring` owns one ring and
-    // scans the LDS-resident columns of each tile in a fixed order
-    // (deterministic); half-plane columns 1..N/2-1 stand for two full-plane
-    // elements of the same ring.
+    // ring sums of |F| over the full plane.  Along one half column (fy >= 0
+    // or fy < 0 at fixed kx) the ring index never decreases with |fy| and
+    // steps by at most one, so a wave runs a segmented scan over |fy| and
+    // stores one partial (sum, count) per ring present in that half column;
+    // phase 4 adds the partials of a ring in a fixed order (deterministic).
+    // Half-plane columns 1..N/2-1 stand for two full-plane elements.
     const int nr = ring_of(N / 2, N / 2) + 1;
-    float ring_sum = 0.0f;
-    int ring_cnt = 0;
+    constexpr int RS = N / 2 + 1;  // rings per half column, upper bound
+    float* rsum = (float*)U;       // [MH][2][RS] (U is free after phase 2)
+    int* rcnt = (int*)(rsum + (size_t)MH * 2 * RS);
     for (int k0 = 0; k0 < MH; k0 += T::TK3) {
       const int nk = min(T::TK3, MH - k0);
       for (int t = tid; t < T::TK3 * N; t += nthr) {
@@ -214,36 +217,67 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         if (q >= nk) continue;
         W[(size_t)ky * MH + k0 + q] = res[q * (N + 1) + ky];
       }
-      if (tid < nr) {
-        const int ring = tid;
-        const float lo = (float)ring - 0.5f, hi = (float)ring + 0.5f;
-        for (int q = 0; q < nk; ++q) {
-          const int kx = k0 + q;
-          const float h2 = hi * hi - (float)(kx * kx);
-          if (h2 < 0.0f) continue;
-          const float l2 = lo * lo - (float)(kx * kx);
-          const int ya = l2 > 0.0f ? max((int)floorf(sqrtf(l2)) - 1, 0) : 0;
-          const int yb = min((int)ceilf(sqrtf(h2)) + 1, N / 2);
-          const int wgt = (kx == 0 || kx == N / 2) ? 1 : 2;
-          for (int a = ya; a <= yb; ++a) {
-#pragma unroll
-            for (int sg = 0; sg < 2; ++sg) {
-              const int fy = sg ? -a : a;
-              if ((sg && a == 0) || fy >= N / 2 || fy < -N / 2) continue;
-              if (ring_of(fy, kx) != ring) continue;
-              const float2 v = res[q * (N + 1) + (fy < 0 ? fy + N : fy)];
-              const float m = sqrtf(v.x * v.x + v.y * v.y);
-              ring_sum += wgt == 2 ? m + m : m;
-              ring_cnt += wgt;
-            }
+      const int lane = tid & 63;
+      for (int task = tid >> 6; task < 2 * nk; task += nthr / 64) {
+        const int q = task >> 1, h = task & 1;
+        const int kx = k0 + q;
+        const float wgt = (kx == 0 || kx == N / 2) ? 1.0f : 2.0f;
+        const int iw = (kx == 0 || kx == N / 2) ? 1 : 2;
+        float* ps = rsum + (size_t)(kx * 2 + h) * RS - kx;  // indexed by ring (>= kx)
+        int* pc = rcnt + (size_t)(kx * 2 + h) * RS - kx;
+        int carry_key = -1;
+        float carry_val = 0.0f;
+        int carry_cnt = 0;
+        for (int c0 = 0; c0 < N / 2; c0 += 64) {
+          const int a = c0 + lane;  // |fy| - h
+          const bool ok = a < N / 2;
+          const int fy = h ? -(a + 1) : a;
+          int key = -2 - lane;  // unique sentinel above the valid lanes
+          float val = 0.0f;
+          int cnt = 0;
+          if (ok) {
+            const float2 v = res[q * (N + 1) + (fy < 0 ? fy + N : fy)];
+            val = sqrtf(v.x * v.x + v.y * v.y) * wgt;
+            cnt = iw;
+            key = ring_of(fy, kx);
           }
+#pragma unroll
+          for (int d = 1; d < 64; d <<= 1) {
+            const float vu = __shfl_up(val, d, 64);
+            const int cu = __shfl_up(cnt, d, 64);
+            const int ku = __shfl_up(key, d, 64);
+            if (lane >= d && ku == key) { val += vu; cnt += cu; }
+          }
+          if (key == carry_key) { val += carry_val; cnt += carry_cnt; }
+          const int fyn = h ? fy - 1 : fy + 1;
+          const bool more = a + 1 < N / 2;
+          const bool last = ok && (!more || ring_of(fyn, kx) != key);
+          if (last) { ps[key] = val; pc[key] = cnt; }
+          carry_key = __shfl(key, 63, 64);
+          carry_val = __shfl(val, 63, 64);
+          carry_cnt = __shfl(cnt, 63, 64);
         }
       }
       __syncthreads();
     }
 
     // ---- phase 4: ring divisors ------------------------------------------
-    if (tid < nr) div[tid] = (tid == 0 || ring_cnt == 0) ? 1.0f : (float)ring_cnt / ring_sum;
+    if (tid < nr) {
+      const int ring = tid;
+      float ring_sum = 0.0f;
+      int ring_cnt = 0;
+      for (int kx = 0; kx <= min(ring, N / 2); ++kx) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int lo = ring_of(h ? -1 : 0, kx), hi = ring_of(h ? -N / 2 : N / 2 - 1, kx);
+          if (ring >= lo && ring <= hi) {
+            ring_sum += rsum[(size_t)(kx * 2 + h) * RS + ring - kx];
+            ring_cnt += rcnt[(size_t)(kx * 2 + h) * RS + ring - kx];
+          }
+        }
+      }
+      div[ring] = (ring == 0 || ring_cnt == 0) ? 1.0f : (float)ring_cnt / ring_sum;
+    }
     __syncthreads();
 
     // ---- phase 5: scale by ring divisor, inverse column FFT (N) -----------
