@@ -29,12 +29,13 @@ SIGNATURES = {
     "gis_prepare_particles": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gis_ctf_packed": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "gis_correlate_workspace": (_sz, [_i]),
-    "gis_correlate": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                           _vp, _sz, _vp]),
+    "gis_correlate": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
+                           _vp, _vp, _sz, _vp]),
+    "gis_score_sum_segments": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
     "gis_score_sum": (_i, [_vp, _vp, _i, _vp, _vp]),
     "gis_gather_coefficients": (_i, [_vp, _sz, _i, _vp, _i, _i, _vp, _vp, _vp]),
-    "gis_correlate_mixed": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp,
-                                 _vp, _vp, _vp]),
+    "gis_correlate_mixed": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _vp, _vp, _vp,
+                                 _vp, _vp, _vp, _vp, _vp]),
 }
 
 

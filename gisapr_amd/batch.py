@@ -1,0 +1,440 @@
+"""Batched multi-candidate evaluation: one optimizer step on the device.
+
+A PSO swarm, a simplex or a grid step evaluates many rigid-body candidates of
+one moving chain against the same particle stack.  ``CandidateBatch`` keeps
+everything of such a step resident in HBM and runs it as a fixed sequence of
+launches that can be captured once as a HIP graph and replayed per step:
+
+  per candidate slot c:  fixed-body density (copy)  ->  splat of the moving
+                         chain  ->  3-D transform  ->  768-direction template
+                         bank  ->  (small-shift path) in-disk coefficients
+  all slots at once:     one search launch over the virtual particles
+                         (slot c, particle b in the slot's subset), reading the
+                         slot's bank rows c*M + direction and the shared
+                         particle spectra through a particle map
+                         one segmented float64 score sum per slot.
+
+The only host work per step is the rigid-body transform and 8.3 rounding of
+the moving chain (ZL:gisapr_batch_density.py:41-49) and one host-to-device
+copy of the coordinates.
+
+``RigidDensityCache`` and ``BatchSearchEngine`` keep the call surface of
+ZL:gisapr_batch_density.py:18-117 and ZL:gisapr_batch_search.py:142-364.
+``Autotuner`` times the search kernel's tile variants on the real operands
+once per shape and keeps the fastest (in memory and, optionally, in a JSON
+file).
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from dataclasses import dataclass
+from typing import Callable, Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import kernels
+from .density import atoms_to_atomic_numbers_and_masses, density_from_coords, splat_parameters, yflip_vectorized
+from .engine import ParticleSet, SearchEngine
+from .pdb_text import quantize_8_3, read_pdb_cached, transform_coordinates
+
+
+# ---------------------------------------------------------------------------
+# Autotuner
+# ---------------------------------------------------------------------------
+
+class Autotuner:
+    """Chooses the fastest tile variant of a launch on its real operands.
+
+    ``choose(key, variants, run)`` times ``run(variant)`` for each variant
+    (one untimed call, then ``reps`` timed calls with HIP events on the
+    current stream) and returns the fastest; the choice is remembered per
+    ``key`` and written to ``path`` (JSON) when one is given.
+    """
+
+    def __init__(self, path: Optional[str] = None, reps: int = 2):
+        self.path = path
+        self.reps = int(reps)
+        self.table: Dict[str, dict] = {}
+        self.timings: Dict[str, List[tuple]] = {}
+        if path and os.path.exists(path):
+            with open(path, "r", encoding="utf-8") as fh:
+                self.table = json.load(fh)
+
+    @staticmethod
+    def key(*parts) -> str:
+        return "/".join(str(p) for p in parts)
+
+    def choose(self, key: str, variants: Sequence[dict], run: Callable[[dict], object]) -> dict:
+        if key in self.table:
+            return dict(self.table[key])
+        results = []
+        for v in variants:
+            run(v)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(self.reps):
+                run(v)
+            e1.record()
+            torch.cuda.synchronize()
+            results.append((e0.elapsed_time(e1) / self.reps, v))
+        best = min(results, key=lambda r: r[0])[1]
+        self.table[key] = dict(best)
+        self.timings[key] = [(ms, dict(v)) for ms, v in results]
+        if self.path:
+            tmp = self.path + ".tmp"
+            with open(tmp, "w", encoding="utf-8") as fh:
+                json.dump(self.table, fh, indent=1, sort_keys=True)
+            os.replace(tmp, self.path)
+        return dict(best)
+
+
+MIXED_VARIANTS = [dict(pt=5, wpb=4), dict(pt=5, wpb=2), dict(pt=5, wpb=1), dict(pt=3, wpb=4), dict(pt=1, wpb=4)]
+FP32_VARIANTS = [dict(jt=0), dict(jt=2), dict(jt=4), dict(jt=6)]
+
+
+def _variants_for(path: str, P: int) -> List[dict]:
+    if path == "mixed":
+        out = []
+        for v in MIXED_VARIANTS:
+            v = dict(v, pt=min(v["pt"], P))
+            if v not in out:
+                out.append(v)
+        return out
+    jt_max = 16 - min(P, 8)
+    return [v for v in FP32_VARIANTS if v["jt"] <= jt_max]
+
+
+# ---------------------------------------------------------------------------
+# Search plan over candidate slots
+# ---------------------------------------------------------------------------
+
+@dataclass
+class SlotPlan:
+    """Virtual particles (slot c, particle b) of one batched search launch."""
+    slots: int
+    subsets: List[np.ndarray]        # particle indices per slot (ascending)
+    dir_off: torch.Tensor            # [V+1] int32
+    dir_idx: torch.Tensor            # [sum] int32: c * M + direction
+    pmap: torch.Tensor               # [V] int32: particle of each virtual particle
+    slot_off: torch.Tensor           # [slots+1] int32 virtual-particle offsets
+    slot_off_host: np.ndarray
+    n_pairs: int                     # (direction, psi) pairs searched per step
+
+    @property
+    def virtual(self) -> int:
+        return int(self.slot_off_host[-1])
+
+
+def build_slot_plan(ps: ParticleSet, M: int, slots: int, subsets: Optional[Sequence] = None,
+                    device=None) -> SlotPlan:
+    """Plan the batched search of ``slots`` candidates; slot c searches the
+    particles ``subsets[c]`` (all particles when None)."""
+    off = ps.dir_off.cpu().numpy().astype(np.int64)
+    idx = ps.dir_idx.cpu().numpy().astype(np.int64)
+    B = ps.B
+    if subsets is None:
+        subsets = [np.arange(B)] * slots
+    if len(subsets) != slots:
+        raise ValueError("one particle subset per slot is required")
+    subs, v_off, v_idx, v_map, s_off = [], [0], [], [], [0]
+    for c, sub in enumerate(subsets):
+        sub = np.unique(np.asarray(sub, np.int64))
+        if len(sub) and (sub[0] < 0 or sub[-1] >= B):
+            raise ValueError("particle subset index out of range")
+        subs.append(sub)
+        counts = off[sub + 1] - off[sub]
+        starts = off[sub]
+        rows = np.repeat(starts, counts) + (np.arange(counts.sum()) - np.repeat(np.cumsum(counts) - counts, counts))
+        v_idx.append(idx[rows] + c * M)
+        v_off.append(counts)
+        v_map.append(sub)
+        s_off.append(len(sub))
+    d_off = np.concatenate([[0], np.cumsum(np.concatenate(v_off[1:]) if len(v_off) > 1 else [])]).astype(np.int32)
+    d_idx = np.concatenate(v_idx).astype(np.int32) if v_idx else np.zeros(0, np.int32)
+    pm = np.concatenate(v_map).astype(np.int32) if v_map else np.zeros(0, np.int32)
+    so = np.cumsum(s_off).astype(np.int32)
+    dev = ps.dir_off.device if device is None else device
+    return SlotPlan(slots=slots, subsets=subs, dir_off=torch.from_numpy(d_off).to(dev),
+                    dir_idx=torch.from_numpy(d_idx).to(dev), pmap=torch.from_numpy(pm).to(dev),
+                    slot_off=torch.from_numpy(so).to(dev), slot_off_host=so, n_pairs=int(len(d_idx)) * ps.P)
+
+
+# ---------------------------------------------------------------------------
+# Fixed body + moving chain
+# ---------------------------------------------------------------------------
+
+@dataclass
+class RigidBody:
+    """A model split into a fixed body and one moving chain (first-model rows)."""
+    fixed_coords: np.ndarray         # [F, 3] float32
+    fixed_numbers: np.ndarray        # [F]
+    moving_coords: np.ndarray        # [A, 3] float32, first-model coordinates of the moving chain
+    moving_numbers: np.ndarray       # [A]
+
+    @classmethod
+    def from_arrays(cls, coords, numbers, chains, moving_chain):
+        chains = np.asarray(chains)
+        sel = chains == str(moving_chain)
+        if not np.any(sel):
+            raise ValueError(f"chain {moving_chain!r} has no atoms")
+        coords = np.asarray(coords, np.float32)
+        numbers = np.asarray(numbers, np.float32)
+        return cls(coords[~sel], numbers[~sel], coords[sel], numbers[sel])
+
+    def moved(self, pose) -> np.ndarray:
+        """Moving-chain coordinates after ``pose`` (rotvec deg, translation A) about
+        the chain's mean, 8.3-rounded (pdb_text.py:123-142)."""
+        return quantize_8_3(transform_coordinates(self.moving_coords, pose))
+
+
+# ---------------------------------------------------------------------------
+# The batched step
+# ---------------------------------------------------------------------------
+
+class CandidateBatch:
+    """``slots`` candidate poses of one moving chain, searched against one
+    resident particle set, on the FP32 ("fp32") or split-bf16 small-shift
+    ("mixed") path.
+
+    ``evaluate(poses)`` runs one step eagerly; ``capture()`` records the
+    device part as a HIP graph and later ``evaluate`` calls replay it.
+    Returns the float64 score sum_b exp(rawCC_b / 1024) per candidate.
+    """
+
+    def __init__(self, engine: SearchEngine, ps: ParticleSet, body: RigidBody, res: float, slots: int, *,
+                 path: str = "mixed", subsets=None, autotuner: Optional[Autotuner] = None):
+        if path not in ("mixed", "fp32"):
+            raise ValueError("path must be 'mixed' or 'fp32'")
+        self.engine, self.ps, self.body, self.path = engine, ps, body, path
+        self.slots = int(slots)
+        dev = engine.device
+        self.device = dev
+        N, n = engine.N, engine.n
+        M = len(engine.dir_rot)
+        self.M = M
+        self.res = float(res)
+        self.width, self.rp, self.kn = splat_parameters(engine.apix, res)
+        if path == "mixed":
+            if getattr(engine, "small", None) is None:
+                engine.enable_small_shift()
+            if ps.coeffs is None:
+                engine.prepare_small_shift(ps)
+        elif ps.spectra is None:
+            engine.prepare(ps)
+        self.plan = build_slot_plan(ps, M, self.slots, subsets, dev)
+        # resident buffers
+        self.fixed = density_from_coords(body.fixed_coords, body.fixed_numbers, N, engine.apix, res, device=dev)
+        self.moving_numbers = torch.as_tensor(body.moving_numbers, dtype=torch.float32).to(dev).contiguous()
+        A = len(body.moving_numbers)
+        self.coords_host = torch.empty((self.slots, A, 3), dtype=torch.float32).pin_memory()
+        self.coords_dev = torch.zeros((self.slots, A, 3), dtype=torch.float32, device=dev)
+        self.volume = torch.empty((N, N, N), dtype=torch.float32, device=dev)
+        self.dft = torch.empty((int(kernels.lib().gis_volume_dft_bytes(N)) // 8,), dtype=torch.complex64, device=dev)
+        self.dft_ws = torch.empty((int(kernels.lib().gis_volume_dft_workspace(N)),), dtype=torch.uint8, device=dev)
+        if path == "mixed":
+            self.bank = torch.empty((M, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+            self.tk = torch.empty((self.slots * M, engine.small.Kpad), dtype=torch.complex64, device=dev)
+        else:
+            self.bank = torch.empty((self.slots * M, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+        self.autotuner = autotuner
+        self.tile: dict = {}
+        self.graph = None
+        self.static_out = None
+
+    # -- device part ---------------------------------------------------------
+    def _templates(self, c: int):
+        eng = self.engine
+        self.volume.copy_(self.fixed)
+        kernels.splat(self.coords_dev[c], self.moving_numbers, eng.N, self.width, eng.apix, self.rp, self.kn,
+                      self.volume)
+        kernels.volume_dft(self.volume, out=self.dft, workspace=self.dft_ws)
+        if self.path == "mixed":
+            kernels.build_templates(self.dft, eng.n, eng.rts, eng.mask, eng.fsc_packed, out=self.bank)
+            kernels.gather_coefficients(self.bank, eng.small_kidx, eng.small.Kpad,
+                                        out=self.tk[c * self.M:(c + 1) * self.M])
+        else:
+            kernels.build_templates(self.dft, eng.n, eng.rts, eng.mask, eng.fsc_packed,
+                                    out=self.bank[c * self.M:(c + 1) * self.M])
+
+    def _search(self, tile: dict):
+        eng, ps, pl = self.engine, self.ps, self.plan
+        if self.path == "mixed":
+            return kernels.correlate_mixed(self.tk, ps.coeffs, ps.P, pl.dir_off, pl.dir_idx, eng.small_bhi,
+                                           eng.small_blo, eng.small.half, validate=False, pmap=pl.pmap,
+                                           pt=tile.get("pt", 0), wpb=tile.get("wpb", 0))
+        return kernels.correlate(self.bank, ps.spectra, ps.inv_norm, pl.dir_off, pl.dir_idx, eng.crop,
+                                 validate=False, pmap=pl.pmap, jt=tile.get("jt", 0))
+
+    def _device_step(self):
+        for c in range(self.slots):
+            self._templates(c)
+        corr = self._search(self.tile)
+        scores = kernels.score_sum_segments(corr.peak, corr.best, self.plan.slot_off)
+        return scores, corr
+
+    # -- host part -----------------------------------------------------------
+    def _stage(self, poses):
+        poses = np.asarray(poses, np.float64).reshape(-1, 6)
+        if len(poses) > self.slots:
+            raise ValueError(f"{len(poses)} poses for {self.slots} slots")
+        for c in range(self.slots):
+            pose = poses[min(c, len(poses) - 1)]  # unused slots repeat the last pose
+            self.coords_host[c].copy_(torch.from_numpy(self.body.moved(pose)))
+        self.coords_dev.copy_(self.coords_host, non_blocking=True)
+        return len(poses)
+
+    def tune(self, poses):
+        """Pick the search kernel's tile variant for this plan on the banks of ``poses``."""
+        if self.autotuner is None:
+            return self.tile
+        self._stage(poses)
+        for c in range(self.slots):
+            self._templates(c)
+        ps, pl = self.ps, self.plan
+        if self.path == "mixed":
+            key = Autotuner.key("mixed", self.engine.n, ps.P, self.engine.small.Kpad, _bucket(int(pl.dir_idx.numel())))
+        else:
+            key = Autotuner.key("fp32", self.engine.n, ps.P, _bucket(int(pl.dir_idx.numel())))
+        self.tile = self.autotuner.choose(key, _variants_for(self.path, ps.P), self._search)
+        return self.tile
+
+    def capture(self, poses=None):
+        """Record the device part of a step as a HIP graph (after one eager warm-up step)."""
+        if poses is not None:
+            self._stage(poses)
+        torch.cuda.synchronize()
+        self._device_step()  # warm-up: library set-up calls happen outside the capture
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.static_out = self._device_step()
+        self.graph = g
+        return g
+
+    def evaluate(self, poses, sync: bool = True):
+        """Score ``poses`` ([k, 6], k <= slots); returns float64 [k] (or the device
+        tensor and the CorrelationResult when sync is False)."""
+        k = self._stage(poses)
+        if self.graph is not None:
+            self.graph.replay()
+            scores, corr = self.static_out
+        else:
+            scores, corr = self._device_step()
+        if not sync:
+            return scores, corr
+        return scores.cpu().numpy()[:k].copy()
+
+    def slot_result(self, corr, c: int):
+        """Per-particle view (best, peak, mean, std) of slot c's virtual particles."""
+        a, b = int(self.plan.slot_off_host[c]), int(self.plan.slot_off_host[c + 1])
+        return dict(particles=self.plan.subsets[c], best=corr.best[a:b], peak=corr.peak[a:b], mean=corr.mean[a:b],
+                    std=corr.std[a:b])
+
+
+def _bucket(x: int) -> int:
+    return 1 << max(0, int(x - 1).bit_length())
+
+
+# ---------------------------------------------------------------------------
+# Reference-shaped front ends
+# ---------------------------------------------------------------------------
+
+class RigidDensityCache:
+    """Fixed-body density reuse (ZL:gisapr_batch_density.py:18-117).
+
+    The fixed atoms (all density rows outside the moving chain's first-model
+    lines) are splatted once and kept on the device; each ``generate`` clones
+    them and splats only the moving chain after the rigid-body transform and
+    8.3 rounding.  Atomic accumulation order differs from a single splat of
+    all atoms: equal up to float32 rounding, not bitwise."""
+
+    def __init__(self):
+        self.key = None
+        self.fixed = None
+        self.last_metrics = {}
+
+    def clear(self):
+        self.key = self.fixed = None
+        self.moving_coords = self.moving_numbers = self.transform_rows = None
+
+    @staticmethod
+    def source_key(kwargs, max_mb, mode, backend):
+        source = kwargs.get("source_pdb")
+        box = int(kwargs["boxsize"])
+        if mode != "fixed" or not source or kwargs.get("do_center", False) or box ** 3 * 4 > max_mb * 1024 ** 2:
+            return None
+        st = os.stat(source)
+        return (os.path.realpath(source), st.st_size, st.st_mtime_ns, str(kwargs["chain_id"]), box,
+                float(kwargs["apix"]), float(kwargs["res"]), str(kwargs.get("device", "cuda")),
+                bool(kwargs.get("include_hetatm", False)), backend)
+
+    def _coordinates(self, pose, key):
+        t0 = time.perf_counter()
+        moved = transform_coordinates(self.moving_coords, pose)
+        t1 = time.perf_counter()
+        q = quantize_8_3(moved)[self.transform_rows]
+        return dict(key=key, pose=tuple(pose), coordinates=q, transform_s=t1 - t0,
+                    quantize_s=time.perf_counter() - t1)
+
+    def coordinate_job(self, kwargs, max_mb=128.0, mode="fixed", backend="auto"):
+        key = self.source_key(kwargs, max_mb, mode, backend)
+        if key is None or key != self.key or self.fixed is None:
+            return None
+        pose = tuple(kwargs["pose"])
+        return lambda: self._coordinates(pose, key)
+
+    def generate(self, kwargs, max_mb=128.0, mode="fixed", backend="auto", prepared=None):
+        from .density import generate_volume
+        started = time.perf_counter()
+        self.last_metrics = dict(fixed_density_hits=0, fixed_density_cached_atoms=0)
+        key = self.source_key(kwargs, max_mb, mode, backend)
+        if key is None:
+            self.clear()
+            args = {k: v for k, v in kwargs.items() if k in ("pdb_path", "boxsize", "apix", "res", "device",
+                                                                "do_center", "do_yflip", "include_hetatm",
+                                                                "source_pdb", "chain_id", "pose")}
+            return generate_volume(**args)
+        source, chain = kwargs["source_pdb"], str(kwargs["chain_id"])
+        box = int(kwargs["boxsize"])
+        include = bool(kwargs.get("include_hetatm", False))
+        device = kwargs.get("device", "cuda")
+        hit = key == self.key and self.fixed is not None
+        if not hit:
+            self.clear()
+            first = read_pdb_cached(source)
+            dens = read_pdb_cached(source, include_hetatm=include, first_model_only=False)
+            sel = first.chains == chain
+            if not sel.any():
+                raise ValueError(f"Chain {chain!r} has no atoms in the first PDB model")
+            moving_lines = first.atom_line_indices[sel]
+            self.moving_coords = first.coords[sel]
+            positions = {int(line): i for i, line in enumerate(moving_lines)}
+            moving = np.isin(dens.atom_line_indices, moving_lines)
+            self.transform_rows = np.asarray([positions[int(line)] for line in dens.atom_line_indices[moving]],
+                                             dtype=np.int64)
+            numbers, _ = atoms_to_atomic_numbers_and_masses(dens.elements)
+            self.moving_numbers = numbers[moving]
+            self.fixed_atom_count = int((~moving).sum())
+            self.fixed = density_from_coords(dens.coords[~moving], numbers[~moving], box, kwargs["apix"],
+                                             kwargs["res"], device=device)
+            self.key = key
+        prepared_at = time.perf_counter()
+        pose = tuple(kwargs["pose"])
+        prefetched = prepared is not None and prepared.get("key") == key and prepared.get("pose") == pose
+        if not prefetched:
+            prepared = self._coordinates(pose, key)
+        t0 = time.perf_counter()
+        volume = density_from_coords(prepared["coordinates"], self.moving_numbers, box, kwargs["apix"],
+                                     kwargs["res"], device=device, _initial_volume=self.fixed)
+        if kwargs.get("do_yflip", False):
+            volume = yflip_vectorized(volume)
+        self.last_metrics.update(fixed_density_hits=int(hit), fixed_density_cached_atoms=self.fixed_atom_count,
+                                 density_moving_atoms=len(self.moving_numbers),
+                                 density_prepare_s=prepared_at - started, density_transform_s=prepared["transform_s"],
+                                 density_quantize_s=prepared["quantize_s"], density_prefetch_hits=int(prefetched),
+                                 density_deposit_s=time.perf_counter() - t0)
+        return volume

@@ -84,7 +84,7 @@ __global__ void build_corr_tables(int n, int C, int nrow_pad, int mpad, float2* 
 __global__ void __launch_bounds__(kThreads)
 correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spectra,
                  const float* __restrict__ inv_norm, const int* __restrict__ dir_off,
-                 const int* __restrict__ dir_idx, int P, int n, int C, int JT, int PT,
+                 const int* __restrict__ dir_idx, const int* __restrict__ pmap, int P, int n, int C, int JT, int PT,
                  const float2* __restrict__ Wt, const float2* __restrict__ Et,
                  float* __restrict__ scores, int* __restrict__ peak_y, int* __restrict__ peak_x,
                  int* __restrict__ best, float* __restrict__ peak, double* __restrict__ mean,
@@ -92,6 +92,7 @@ correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spe
   extern __shared__ float2 smem[];
   __shared__ const float2* tptr[kMaxTiles];
   const int b = blockIdx.x;
+  const int pb = pmap ? pmap[b] : b;  // particle row of spectra / inv_norm
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m = n / 2 + 1;
   const int nct = (m + 15) / 16;
@@ -118,7 +119,7 @@ correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spe
       __syncthreads();
       if (tid < ntile)
         tptr[tid] = tid < jt ? bank + (size_t)dir_idx[d0 + j0 + tid] * plane
-                             : spectra + ((size_t)b * P + p0 + (tid - jt)) * plane;
+                             : spectra + ((size_t)pb * P + p0 + (tid - jt)) * plane;
       __syncthreads();
       // Pair slots beyond the sweep alias its last pair, so the MFMA loop has
       // no per-pair branches (branches force accumulator copies between
@@ -215,7 +216,7 @@ correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spe
         const int k = wave + kWaves * i;
         if (k >= sweep) continue;
         const int j = k / pt, p = k - j * pt;
-        const float sc = 1024.0f * (inv_norm ? inv_norm[(size_t)b * P + p0 + p] : 1.0f);
+        const float sc = 1024.0f * (inv_norm ? inv_norm[(size_t)pb * P + p0 + p] : 1.0f);
         float bv = -INFINITY;
         int bi = 0x7fffffff;
 #pragma unroll
@@ -295,6 +296,26 @@ __global__ void score_sum_kernel(const float* __restrict__ peak, const int* __re
   }
 }
 
+// One block per segment: out[s] = sum over i in [off[s], off[s+1]) with
+// best_i >= 0 of exp(peak_i / 1024), float64, fixed reduction order.
+__global__ void score_sum_segments_kernel(const float* __restrict__ peak, const int* __restrict__ best,
+                                          const int* __restrict__ off, double* __restrict__ out) {
+  __shared__ double red[kThreads / 64];
+  const int sgm = blockIdx.x;
+  const int i0 = off[sgm], i1 = off[sgm + 1];
+  double s = 0.0;
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+    if (best == nullptr || best[i] >= 0) s += exp((double)peak[i] / 1024.0);
+  s = wave_sum_f64(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    out[sgm] = t;
+  }
+}
+
 inline int row_pad(int n) { return ((n + kRC - 1) / kRC) * kRC; }
 inline int col_pad(int n) { return ((n / 2 + 1 + 15) / 16) * 16; }
 
@@ -315,11 +336,14 @@ GIS_API size_t gis_correlate_workspace(int n) {
 //   spectra   [B][P][n][n/2+1] complex64 conjugated particle spectra (UNshifted)
 //   inv_norm  [B][P] float32 per-spectrum 1/L2 norm, or NULL (already unit norm)
 //   dir_off   [B+1] int32 offsets into dir_idx; dir_idx global template indices
+//   pmap      [B] int32 spectra row of each searched particle, or NULL (identity);
+//             lets one launch search several candidates' banks against shared spectra
+//   jt        template tiles per sweep (0 = automatic)
 //   scores/peak_y/peak_x  [dir_off[B]][P] outputs (score = max * 1024 n^2)
 //   best [B] flat argmax (local_dir * P + psi) or -1; peak [B]; mean/std [B] float64
 GIS_API int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, const int* dir_off,
-                          const int* dir_idx, int B, int P, int n, int C, float* scores, int* peak_y,
-                          int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
+                          const int* dir_idx, const int* pmap, int B, int P, int n, int C, int jt, float* scores,
+                          int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
                           size_t ws_bytes, void* stream) {
   GIS_REQUIRE(B >= 0 && P >= 1, "gis_correlate: B=%d P=%d", B, P);
   GIS_REQUIRE(n >= 4 && n % 2 == 0 && n <= 256, "gis_correlate: newbox %d must be even, in [4, 256]", n);
@@ -338,7 +362,9 @@ GIS_API int gis_correlate(const void* bank, const void* spectra, const float* in
   hipLaunchKernelGGL(build_corr_tables, dim3(64), dim3(256), 0, st, n, C, row_pad(n), col_pad(n), Wt, Et);
   GIS_LAUNCH_CHECK("gis_correlate tables");
   const int PT = std::min(P, 8);
-  const int JT = std::max(1, std::min(kMaxTiles - PT, (kWaves * kPairsPerWave) / PT));
+  const int jt_max = kMaxTiles - PT;
+  GIS_REQUIRE(jt >= 0 && jt <= jt_max, "gis_correlate: jt %d not in [0, %d]", jt, jt_max);
+  const int JT = jt > 0 ? jt : std::max(1, std::min(jt_max, (kWaves * kPairsPerWave) / PT));
   const size_t lds = corr_lds_bytes(n);
   static bool attr_set = false;
   if (!attr_set) {
@@ -352,7 +378,7 @@ GIS_API int gis_correlate(const void* bank, const void* spectra, const float* in
     attr_set = true;
   }
   hipLaunchKernelGGL(correlate_kernel, dim3(B), dim3(kThreads), lds, st, (const float2*)bank,
-                     (const float2*)spectra, inv_norm, dir_off, dir_idx, P, n, C, JT, PT, Wt, Et, scores,
+                     (const float2*)spectra, inv_norm, dir_off, dir_idx, pmap, P, n, C, JT, PT, Wt, Et, scores,
                      peak_y, peak_x, best, peak, mean, stdv);
   GIS_LAUNCH_CHECK("gis_correlate");
   return GIS_OK;
@@ -365,5 +391,20 @@ GIS_API int gis_score_sum(const float* peak, const int* best, int B, double* out
   GIS_REQUIRE(B == 0 || peak, "gis_score_sum: null peak");
   hipLaunchKernelGGL(score_sum_kernel, dim3(1), dim3(kThreads), 0, (hipStream_t)stream, peak, best, B, out);
   GIS_LAUNCH_CHECK("gis_score_sum");
+  return GIS_OK;
+}
+
+// Per-candidate conformation scores in one launch: out[s] = sum over particles
+// i in [off[s], off[s+1]) with best_i >= 0 of exp(peak_i / 1024), float64
+// (new_method_to_fit_the_2nd_Gaussian_PEAK_v4.py:133-134 per candidate;
+// ZL:gisapr_batch_search.py:361).  off [nseg+1] device int32.
+GIS_API int gis_score_sum_segments(const float* peak, const int* best, const int* off, int nseg, double* out,
+                                   void* stream) {
+  GIS_REQUIRE(nseg >= 0, "gis_score_sum_segments: nseg %d", nseg);
+  if (nseg == 0) return GIS_OK;
+  GIS_REQUIRE(peak && off && out, "gis_score_sum_segments: null pointer");
+  hipLaunchKernelGGL(score_sum_segments_kernel, dim3(nseg), dim3(kThreads), 0, (hipStream_t)stream, peak, best,
+                     off, out);
+  GIS_LAUNCH_CHECK("gis_score_sum_segments");
   return GIS_OK;
 }

@@ -62,14 +62,15 @@ __device__ __forceinline__ void split8(const float (&x)[8], bf16x8& hi, bf16x8& 
 // dir_off[b+1]); M-tile p holds the pairs (row, psi p0 + p).  The template
 // coefficients of a row are loaded once per k-step and multiplied by the PT
 // particle spectra, and every lag-table fragment feeds PT M-tiles.
-template <int PT>
-__global__ void __launch_bounds__(256)
+template <int PT, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
 mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__ Pk,
-                       const int* __restrict__ dir_off, const int* __restrict__ dir_idx, int B, int nrows,
+                       const int* __restrict__ dir_off, const int* __restrict__ dir_idx,
+                       const int* __restrict__ pmap, int B, int nrows,
                        int P, int Kpad, const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo, int half,
                        float* __restrict__ scores, int* __restrict__ peak_y, int* __restrict__ peak_x) {
   const int lane = threadIdx.x & 63;
-  const int group = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int group = blockIdx.x * WPB + (threadIdx.x >> 6);
   if (group * 16 >= nrows) return;  // whole wave
   const int p0 = blockIdx.y * PT;
   const int i = lane & 15, g = lane >> 4;
@@ -80,10 +81,11 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
     const int mid = (lo + hi) >> 1;
     if (dir_off[mid] <= r) lo = mid; else hi = mid;
   }
+  const int pb = pmap ? pmap[lo] : lo;  // particle row of Pk
   const float4* trow = (const float4*)(Tk + (size_t)dir_idx[r] * Kpad);
   const float4* prow[PT];
 #pragma unroll
-  for (int p = 0; p < PT; ++p) prow[p] = (const float4*)(Pk + ((size_t)lo * P + min(p0 + p, P - 1)) * Kpad);
+  for (int p = 0; p < PT; ++p) prow[p] = (const float4*)(Pk + ((size_t)pb * P + min(p0 + p, P - 1)) * Kpad);
   const int ksteps = Kpad / 16;
   floatx4 acc[PT][kLagTiles];
 #pragma unroll
@@ -211,15 +213,17 @@ GIS_API int gis_gather_coefficients(const void* src, size_t plane, int rows, con
 
 // Precision variant of correlate_templates_and_particle (v606:1418-1449)
 // over lags [-half, half]^2 (half <= 3) and a truncated coefficient set.
-//   Tk [bank rows][Kpad], Pk [B*P][Kpad] complex64 (unit-norm, unshifted)
+//   Tk [bank rows][Kpad], Pk [particles*P][Kpad] complex64 (unit-norm, unshifted)
+//   pmap [B] particle row of each searched particle in Pk, or NULL (identity)
 //   nrows = dir_off[B] = number of dir_idx entries (passed so no device read is needed)
+//   pt = psi values per wave (1..5, 0 = min(P, 5)); wpb = waves per block (1, 2, 4; 0 = 4)
 //   Bhi, Blo: split lag table, [4][Kpad/16][64] x 8 bf16 (host layout)
 //   scores / peak_y / peak_x [dir_off[B]][P] (peak indices are lag + half)
 //   best / peak / mean / std [B] as gis_correlate
-GIS_API int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx, int B, int P,
-                                int nrows, int Kpad, const void* Bhi, const void* Blo, int half, float* scores,
-                                int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv,
-                                void* stream) {
+GIS_API int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx,
+                                const int* pmap, int B, int P, int nrows, int Kpad, const void* Bhi, const void* Blo,
+                                int half, int pt, int wpb, float* scores, int* peak_y, int* peak_x, int* best,
+                                float* peak, double* mean, double* stdv, void* stream) {
   GIS_REQUIRE(B >= 0 && P >= 1 && nrows >= 0, "gis_correlate_mixed: B=%d P=%d nrows=%d", B, P, nrows);
   GIS_REQUIRE(Kpad % 16 == 0 && Kpad >= 16, "gis_correlate_mixed: Kpad %d must be a positive multiple of 16", Kpad);
   if (half < 0 || (2 * half + 1) * (2 * half + 1) > 16 * kLagTiles) {
@@ -231,20 +235,33 @@ GIS_API int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_o
                   stdv,
               "gis_correlate_mixed: null pointer");
   hipStream_t st = (hipStream_t)stream;
+  GIS_REQUIRE(pt >= 0 && pt <= 5, "gis_correlate_mixed: pt %d not in [0, 5]", pt);
+  GIS_REQUIRE(wpb == 0 || wpb == 1 || wpb == 2 || wpb == 4, "gis_correlate_mixed: wpb %d not in {0, 1, 2, 4}", wpb);
   if (nrows > 0) {
-    const int pt = std::min(P, 5);
-    const dim3 grid((unsigned)((nrows + 63) / 64), (unsigned)((P + pt - 1) / pt));
-#define GIS_MIXED_LAUNCH(PTV)                                                                                   \
-  hipLaunchKernelGGL(mixed_correlate_kernel<PTV>, grid, dim3(256), 0, st, (const float2*)Tk, (const float2*)Pk, \
-                     dir_off, dir_idx, B, nrows, P, Kpad, (const bf16x8*)Bhi, (const bf16x8*)Blo, half, scores,  \
-                     peak_y, peak_x)
-    switch (pt) {
-      case 1: GIS_MIXED_LAUNCH(1); break;
-      case 2: GIS_MIXED_LAUNCH(2); break;
-      case 3: GIS_MIXED_LAUNCH(3); break;
-      case 4: GIS_MIXED_LAUNCH(4); break;
-      default: GIS_MIXED_LAUNCH(5); break;
+    if (pt == 0) pt = std::min(P, 5);
+    if (wpb == 0) wpb = 4;
+    const int rows_per_block = 16 * wpb;
+    const dim3 grid((unsigned)((nrows + rows_per_block - 1) / rows_per_block), (unsigned)((P + pt - 1) / pt));
+#define GIS_MIXED_LAUNCH(PTV, WV)                                                                              \
+  hipLaunchKernelGGL((mixed_correlate_kernel<PTV, WV>), grid, dim3(64 * WV), 0, st, (const float2*)Tk,         \
+                     (const float2*)Pk, dir_off, dir_idx, pmap, B, nrows, P, Kpad, (const bf16x8*)Bhi,          \
+                     (const bf16x8*)Blo, half, scores, peak_y, peak_x)
+#define GIS_MIXED_PT(WV)                      \
+  switch (pt) {                               \
+    case 1: GIS_MIXED_LAUNCH(1, WV); break;   \
+    case 2: GIS_MIXED_LAUNCH(2, WV); break;   \
+    case 3: GIS_MIXED_LAUNCH(3, WV); break;   \
+    case 4: GIS_MIXED_LAUNCH(4, WV); break;   \
+    default: GIS_MIXED_LAUNCH(5, WV); break;  \
+  }
+    if (wpb == 1) {
+      GIS_MIXED_PT(1)
+    } else if (wpb == 2) {
+      GIS_MIXED_PT(2)
+    } else {
+      GIS_MIXED_PT(4)
     }
+#undef GIS_MIXED_PT
 #undef GIS_MIXED_LAUNCH
     GIS_LAUNCH_CHECK("gis_correlate_mixed");
   }

@@ -40,7 +40,8 @@ def splat(xyz: torch.Tensor, numbers: torch.Tensor, box: int, width: int, apix: 
     return volume
 
 
-def volume_dft(volume: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+def volume_dft(volume: torch.Tensor, out: Optional[torch.Tensor] = None,
+               workspace: Optional[torch.Tensor] = None) -> torch.Tensor:
     """[N]^3 float32 -> bricked half spectrum of [2N, 2N, N+1] complex64
     (pad, sinc^2, shift, rfftn); ``volume_dft_natural`` unpacks it."""
     require_hip(volume, "volume")
@@ -53,7 +54,13 @@ def volume_dft(volume: torch.Tensor, out: Optional[torch.Tensor] = None) -> torc
     elif out.numel() != nb or out.dtype != torch.complex64:
         raise ValueError(f"out must be {nb} complex64 values")
     wsb = int(lib().gis_volume_dft_workspace(N))
-    ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=volume.device)
+    if workspace is not None:
+        require_hip(workspace, "workspace")
+        if workspace.numel() * workspace.element_size() < wsb:
+            raise ValueError(f"workspace must hold {wsb} bytes")
+        ws = workspace
+    else:
+        ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=volume.device)
     check(lib().gis_volume_dft(ptr(volume), N, ptr(out), ptr(ws), wsb, stream_of(volume.device)),
           "gis_volume_dft")
     return out
@@ -67,8 +74,10 @@ def volume_dft_natural(dft: torch.Tensor, N: int) -> torch.Tensor:
 
 
 def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Optional[torch.Tensor],
-                    fsc_packed: Optional[torch.Tensor], with_norms: bool = False):
-    """Unit-norm packed template bank [M, n, n/2+1] complex64 (unshifted)."""
+                    fsc_packed: Optional[torch.Tensor], with_norms: bool = False,
+                    out: Optional[torch.Tensor] = None):
+    """Unit-norm packed template bank [M, n, n/2+1] complex64 (unshifted),
+    written into ``out`` when given."""
     require_hip(dft, "dft")
     N = next((b for b in SUPPORTED_BOXES if int(lib().gis_volume_dft_bytes(b)) == dft.numel() * 8), 0)
     if not N or dft.dtype != torch.complex64:
@@ -76,7 +85,13 @@ def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Op
     M = int(rts.shape[0])
     require_hip(rts, "rts")
     n = int(new_box)
-    bank = torch.empty((M, n, n // 2 + 1), dtype=torch.complex64, device=dft.device)
+    if out is None:
+        bank = torch.empty((M, n, n // 2 + 1), dtype=torch.complex64, device=dft.device)
+    else:
+        require_hip(out, "out")
+        if tuple(out.shape) != (M, n, n // 2 + 1) or out.dtype != torch.complex64:
+            raise ValueError("out must be complex64 [M, n, n/2+1]")
+        bank = out
     norms = torch.empty((M,), dtype=torch.float32, device=dft.device) if with_norms else None
     if mask is not None:
         require_hip(mask, "mask")
@@ -140,11 +155,25 @@ class CorrelationResult:
     std: torch.Tensor      # [B] float64
 
 
+def _check_pmap(pmap, B, rows):
+    if pmap is None:
+        return
+    require_hip(pmap, "pmap")
+    if pmap.dtype != torch.int32 or tuple(pmap.shape) != (B,):
+        raise ValueError("pmap must be int32 [B]")
+    if B and (int(pmap.min()) < 0 or int(pmap.max()) >= rows):
+        raise ValueError("pmap out of range of the particle rows")
+
+
 def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torch.Tensor],
-              dir_off: torch.Tensor, dir_idx: torch.Tensor, crop: int, validate: bool = True) -> CorrelationResult:
+              dir_off: torch.Tensor, dir_idx: torch.Tensor, crop: int, validate: bool = True,
+              pmap: Optional[torch.Tensor] = None, jt: int = 0) -> CorrelationResult:
+    """FP32 v606 search.  With ``pmap`` the searched particles are
+    B = len(dir_off) - 1 entries each mapped to a row of ``spectra``."""
     require_hip(bank, "bank")
     require_hip(spectra, "spectra")
-    B, P, n = int(spectra.shape[0]), int(spectra.shape[1]), int(spectra.shape[2])
+    P, n = int(spectra.shape[1]), int(spectra.shape[2])
+    B = int(dir_off.shape[0]) - 1 if pmap is not None else int(spectra.shape[0])
     if tuple(bank.shape[1:]) != (n, n // 2 + 1):
         raise ValueError("bank and spectra layouts differ")
     if dir_off.dtype != torch.int32 or dir_idx.dtype != torch.int32:
@@ -158,6 +187,7 @@ def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torc
             raise ValueError("dir_off must start at 0, be non-decreasing and end at len(dir_idx)")
         if total and (int(dir_idx.min()) < 0 or int(dir_idx.max()) >= int(bank.shape[0])):
             raise ValueError("dir_idx out of range of the template bank")
+        _check_pmap(pmap, B, int(spectra.shape[0]))
     dev = spectra.device
     scores = torch.empty((total, P), dtype=torch.float32, device=dev)
     py = torch.empty((total, P), dtype=torch.int32, device=dev)
@@ -168,9 +198,9 @@ def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torc
     std = torch.empty((B,), dtype=torch.float64, device=dev)
     wsb = int(lib().gis_correlate_workspace(n))
     ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dev)
-    check(lib().gis_correlate(ptr(bank), ptr(spectra), ptr(inv_norm), ptr(dir_off), ptr(dir_idx), B, P, n,
-                              int(crop), ptr(scores), ptr(py), ptr(px), ptr(best), ptr(peak), ptr(mean),
-                              ptr(std), ptr(ws), wsb, stream_of(dev)), "gis_correlate")
+    check(lib().gis_correlate(ptr(bank), ptr(spectra), ptr(inv_norm), ptr(dir_off), ptr(dir_idx), ptr(pmap), B,
+                              P, n, int(crop), int(jt), ptr(scores), ptr(py), ptr(px), ptr(best), ptr(peak),
+                              ptr(mean), ptr(std), ptr(ws), wsb, stream_of(dev)), "gis_correlate")
     return CorrelationResult(scores, py, px, best, peak, mean, std)
 
 
@@ -183,8 +213,21 @@ def score_sum(peak: torch.Tensor, best: Optional[torch.Tensor] = None) -> torch.
     return out
 
 
+def score_sum_segments(peak: torch.Tensor, best: Optional[torch.Tensor], off: torch.Tensor) -> torch.Tensor:
+    """Per-segment sum exp(peak/1024) (segments [off[s], off[s+1])) as float64 [nseg]."""
+    require_hip(peak, "peak")
+    require_hip(off, "off")
+    if off.dtype != torch.int32:
+        raise ValueError("off must be int32")
+    nseg = int(off.shape[0]) - 1
+    out = torch.empty((max(nseg, 0),), dtype=torch.float64, device=peak.device)
+    check(lib().gis_score_sum_segments(ptr(peak), ptr(best), ptr(off), nseg, ptr(out), stream_of(peak.device)),
+          "gis_score_sum_segments")
+    return out
+
+
 def gather_coefficients(src: torch.Tensor, kidx: torch.Tensor, kpad: int,
-                        scale: Optional[torch.Tensor] = None) -> torch.Tensor:
+                        scale: Optional[torch.Tensor] = None, out: Optional[torch.Tensor] = None) -> torch.Tensor:
     """[rows, Kpad] complex64: the listed packed coefficients of each spectrum of
     ``src`` ([..., n, n/2+1]), times ``scale`` per row if given, zero padded."""
     require_hip(src, "src")
@@ -200,14 +243,20 @@ def gather_coefficients(src: torch.Tensor, kidx: torch.Tensor, kpad: int,
         require_hip(scale, "scale")
         if scale.dtype != torch.float32 or scale.numel() != rows:
             raise ValueError("scale must be float32 with one value per spectrum")
-    out = torch.empty((rows, kpad), dtype=torch.complex64, device=src.device)
+    if out is None:
+        out = torch.empty((rows, kpad), dtype=torch.complex64, device=src.device)
+    else:
+        require_hip(out, "out")
+        if tuple(out.shape) != (rows, kpad) or out.dtype != torch.complex64:
+            raise ValueError("out must be complex64 [rows, kpad]")
     check(lib().gis_gather_coefficients(ptr(src), plane, rows, ptr(kidx), K, int(kpad), ptr(scale), ptr(out),
                                         stream_of(src.device)), "gis_gather_coefficients")
     return out
 
 
 def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.Tensor, dir_idx: torch.Tensor,
-                    b_hi: torch.Tensor, b_lo: torch.Tensor, half: int, validate: bool = True) -> CorrelationResult:
+                    b_hi: torch.Tensor, b_lo: torch.Tensor, half: int, validate: bool = True,
+                    pmap: Optional[torch.Tensor] = None, pt: int = 0, wpb: int = 0) -> CorrelationResult:
     """Split-bf16 small-shift search: best lag over [-half, half]^2 of every
     (direction, psi) pair; peak_y / peak_x are lag + half.  ``validate``
     checks dir_off / dir_idx against the operands on the host."""
@@ -218,7 +267,9 @@ def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.T
     if int(pk.shape[1]) != kpad or pk.dtype != torch.complex64 or tk.dtype != torch.complex64:
         raise ValueError("tk and pk must be complex64 with the same Kpad")
     B = int(dir_off.shape[0]) - 1
-    if int(pk.shape[0]) != B * P:
+    if pmap is None and int(pk.shape[0]) != B * P:
+        raise ValueError("pk must hold P rows per particle")
+    if int(pk.shape[0]) % P:
         raise ValueError("pk must hold P rows per particle")
     if tuple(b_hi.shape) != tuple(b_lo.shape) or b_hi.numel() != 4 * (kpad // 16) * 64 * 8:
         raise ValueError("lag table does not match Kpad")
@@ -231,6 +282,7 @@ def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.T
             raise ValueError("dir_off must start at 0, be non-decreasing and end at len(dir_idx)")
         if total and (int(dir_idx.min()) < 0 or int(dir_idx.max()) >= int(tk.shape[0])):
             raise ValueError("dir_idx out of range of the template coefficients")
+        _check_pmap(pmap, B, int(pk.shape[0]) // P)
     dev = pk.device
     scores = torch.empty((total, P), dtype=torch.float32, device=dev)
     py = torch.empty((total, P), dtype=torch.int32, device=dev)
@@ -239,8 +291,8 @@ def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.T
     peak = torch.empty((B,), dtype=torch.float32, device=dev)
     mean = torch.empty((B,), dtype=torch.float64, device=dev)
     std = torch.empty((B,), dtype=torch.float64, device=dev)
-    check(lib().gis_correlate_mixed(ptr(tk), ptr(pk), ptr(dir_off), ptr(dir_idx), B, int(P), total, kpad,
-                                    ptr(b_hi), ptr(b_lo), int(half), ptr(scores), ptr(py), ptr(px), ptr(best),
-                                    ptr(peak), ptr(mean), ptr(std), stream_of(dev)),
+    check(lib().gis_correlate_mixed(ptr(tk), ptr(pk), ptr(dir_off), ptr(dir_idx), ptr(pmap), B, int(P), total,
+                                    kpad, ptr(b_hi), ptr(b_lo), int(half), int(pt), int(wpb), ptr(scores), ptr(py),
+                                    ptr(px), ptr(best), ptr(peak), ptr(mean), ptr(std), stream_of(dev)),
           "gis_correlate_mixed")
     return CorrelationResult(scores, py, px, best, peak, mean, std)

@@ -135,9 +135,27 @@ def coordinate_fields(xyz):
 
 
 def quantize_8_3(xyz) -> np.ndarray:
-    """float(f"{x:8.3f}") of every coordinate (the write/re-read round trip), float32."""
+    """float(f"{x:8.3f}") of every coordinate (the write/re-read round trip), float32.
+
+    Vectorised: k = round-half-even(1000 x) on the exact binary value of x,
+    then k / 1000 (correctly rounded, as float() of the 8.3 text is).  The
+    few values whose 1000 x lies within 1e-6 of a tie are decided exactly
+    from the formatted text."""
     xyz = np.asarray(xyz, np.float64)
-    return np.asarray([[float(f) for f in coordinate_fields(r)] for r in xyz], np.float32).reshape(-1, 3)
+    if xyz.size and not np.all(np.isfinite(xyz)):
+        raise ValueError("PDB coordinates must be finite")
+    t = xyz * 1000.0
+    k = np.rint(t)
+    frac = np.abs(t - np.trunc(t))
+    near = np.abs(frac - 0.5) < 1e-6
+    if np.any(near):
+        idx = np.nonzero(near)
+        k[idx] = [round(float(f"{v:.3f}") * 1000.0) for v in xyz[idx]]
+    out = k / 1000.0
+    if out.size and (out.max() >= 9999.9995 or out.min() <= -999.9995):
+        for r in xyz.reshape(-1, 3):
+            coordinate_fields(r)  # raises with the reference's message
+    return out.astype(np.float32).reshape(-1, 3)
 
 
 def transformed_density_atoms(path, chain_id, pose, include_hetatm=False):

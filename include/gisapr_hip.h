@@ -120,6 +120,10 @@ size_t gis_correlate_workspace(int n);
  * correlate_templates_and_particle (v606:1418-1449) and the numerics of
  * make_result_records (v606:1452-1491), batched over B particles.  Particle
  * b is correlated against bank rows dir_idx[dir_off[b] .. dir_off[b+1]).
+ *   pmap [B] spectra row of searched particle b, or NULL (identity): one
+ *        launch can search several candidates' banks (rows c*M + direction)
+ *        against one resident particle stack (ZL:gisapr_batch_search.py:318-340)
+ *   jt   template tiles per sweep, 0 = automatic (autotuner knob)
  *   scores/peak_y/peak_x [dir_off[B]][P]: score = max(ccg crop) * 1024 n^2,
  *                        (y, x) = divmod(argmax, C) in the C x C crop
  *   best [B]: flat argmax over (local direction, psi) or -1 if none
@@ -127,8 +131,8 @@ size_t gis_correlate_workspace(int n);
  * C (--ccgCropSize) must be even and <= 16.
  */
 int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, const int* dir_off,
-                  const int* dir_idx, int B, int P, int n, int C, float* scores, int* peak_y,
-                  int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
+                  const int* dir_idx, const int* pmap, int B, int P, int n, int C, int jt, float* scores,
+                  int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
                   size_t ws_bytes, void* stream);
 
 /*
@@ -137,6 +141,14 @@ int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, 
  * new_method_to_fit_the_2nd_Gaussian_PEAK_v4.py:133-134.  out: 1 double.
  */
 int gis_score_sum(const float* peak, const int* best, int B, double* out, void* stream);
+
+/*
+ * Per-candidate conformation scores in one launch (ZL:gisapr_batch_search.py:361):
+ * out[s] = sum over i in [off[s], off[s+1]) with best_i >= 0 of exp(peak_i / 1024),
+ * float64.  off [nseg+1] device int32; out [nseg].
+ */
+int gis_score_sum_segments(const float* peak, const int* best, const int* off, int nseg, double* out,
+                           void* stream);
 
 /*
  * Gathers K listed coefficients (flat packed indices kidx) of `rows` spectra
@@ -155,16 +167,18 @@ int gis_gather_coefficients(const void* src, size_t plane, int rows, const int* 
  * the score at lag (ly, lx), |ly|, |lx| <= half <= 3, is evaluated as the
  * linear functional 1024 * sum_k Re(T_k P_k Phi[k][lag]) over the gathered
  * coefficients on split-bf16 MFMA (hi*hi + hi*lo + lo*hi, FP32 accumulation).
- *   Tk [bank rows][Kpad], Pk [B*P][Kpad] complex64 (unit norm over the full plane)
+ *   Tk [bank rows][Kpad], Pk [particles*P][Kpad] complex64 (unit norm over the full plane)
+ *   pmap [B] Pk particle row of searched particle b, or NULL (identity)
  *   nrows = dir_off[B] (the number of dir_idx entries)
  *   Bhi / Blo [4][Kpad/16][64][8] bf16: split lag table in MFMA B-operand order
+ *   pt  psi values per wave (1..5, 0 = min(P, 5)); wpb waves per block (1, 2, 4; 0 = 4)
  *   scores/peak_y/peak_x [dir_off[B]][P] (peak indices are lag + half),
  *   best/peak/mean/std [B] as gis_correlate (FP64 mean/std).
  */
-int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx, int B, int P,
-                        int nrows, int Kpad, const void* Bhi, const void* Blo, int half, float* scores,
-                        int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv,
-                        void* stream);
+int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx, const int* pmap,
+                        int B, int P, int nrows, int Kpad, const void* Bhi, const void* Blo, int half, int pt,
+                        int wpb, float* scores, int* peak_y, int* peak_x, int* best, float* peak, double* mean,
+                        double* stdv, void* stream);
 
 #ifdef __cplusplus
 }

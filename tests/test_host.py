@@ -93,3 +93,19 @@ def test_volume_dft_brick_unpack_is_the_documented_index():
     idx = ((((kz // 2) * N + ky // 2) * xb + kx // 4) * 2 + kz % 2) * 2 * 4 + (ky % 2) * 4 + kx % 4
     bricked[torch.from_numpy(idx.ravel())] = nat.reshape(-1).to(torch.complex64)
     np.testing.assert_array_equal(kernels.volume_dft_natural(bricked, N).real.numpy(), nat.numpy())
+
+
+def test_quantize_8_3_matches_text_round_trip():
+    """The vectorised 8.3 quantiser equals float(f"{x:8.3f}") on random values,
+    exact decimal ties (x.xxx5) and negative zero."""
+    from gisapr_amd.pdb_text import quantize_8_3
+    rng = np.random.default_rng(7)
+    vals = np.concatenate([rng.uniform(-999, 9999, 30000), rng.uniform(-1, 1, 3000),
+                           np.arange(-5000, 5000) / 2000.0 + 0.0005, [-0.0001, -0.0004, 0.0005, -0.0005, 2.0005]])
+    vals = vals[: (len(vals) // 3) * 3].reshape(-1, 3)
+    want = np.asarray([[float(f"{v:8.3f}") for v in r] for r in vals], np.float32)
+    got = quantize_8_3(vals)
+    np.testing.assert_array_equal(got, want)
+    assert np.array_equal(np.signbit(got), np.signbit(want))
+    with pytest.raises(ValueError):
+        quantize_8_3([[10000.0, 0.0, 0.0]])
