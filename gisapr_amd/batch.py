@@ -122,7 +122,14 @@ class SlotPlan:
     pmap: torch.Tensor               # [V] int32: particle of each virtual particle
     slot_off: torch.Tensor           # [slots+1] int32 virtual-particle offsets
     slot_off_host: np.ndarray
+    row_off_host: np.ndarray         # [slots+1] dir_idx offsets at slot boundaries
     n_pairs: int                     # (direction, psi) pairs searched per step
+
+    def prefix(self, k: int):
+        """Operands of a search over the first k slots only (views, no copies)."""
+        v = int(self.slot_off_host[k])
+        r = int(self.row_off_host[k])
+        return self.dir_off[:v + 1], self.dir_idx[:r], self.pmap[:v], self.slot_off[:k + 1]
 
     @property
     def virtual(self) -> int:
@@ -160,7 +167,8 @@ def build_slot_plan(ps: ParticleSet, M: int, slots: int, subsets: Optional[Seque
     dev = ps.dir_off.device if device is None else device
     return SlotPlan(slots=slots, subsets=subs, dir_off=torch.from_numpy(d_off).to(dev),
                     dir_idx=torch.from_numpy(d_idx).to(dev), pmap=torch.from_numpy(pm).to(dev),
-                    slot_off=torch.from_numpy(so).to(dev), slot_off_host=so, n_pairs=int(len(d_idx)) * ps.P)
+                    slot_off=torch.from_numpy(so).to(dev), slot_off_host=so, row_off_host=d_off[so].copy(),
+                    n_pairs=int(len(d_idx)) * ps.P)
 
 
 # ---------------------------------------------------------------------------
@@ -240,6 +248,7 @@ class CandidateBatch:
             self.tk = torch.empty((self.slots * M, engine.small.Kpad), dtype=torch.complex64, device=dev)
         else:
             self.bank = torch.empty((self.slots * M, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+        self._copied = torch.cuda.Event()
         self.autotuner = autotuner
         self.tile: dict = {}
         self.graph = None
@@ -260,20 +269,22 @@ class CandidateBatch:
             kernels.build_templates(self.dft, eng.n, eng.rts, eng.mask, eng.fsc_packed,
                                     out=self.bank[c * self.M:(c + 1) * self.M])
 
-    def _search(self, tile: dict):
-        eng, ps, pl = self.engine, self.ps, self.plan
+    def _search(self, tile: dict, k: Optional[int] = None):
+        eng, ps = self.engine, self.ps
+        dir_off, dir_idx, pmap, _ = self.plan.prefix(self.slots if k is None else k)
         if self.path == "mixed":
-            return kernels.correlate_mixed(self.tk, ps.coeffs, ps.P, pl.dir_off, pl.dir_idx, eng.small_bhi,
-                                           eng.small_blo, eng.small.half, validate=False, pmap=pl.pmap,
+            return kernels.correlate_mixed(self.tk, ps.coeffs, ps.P, dir_off, dir_idx, eng.small_bhi,
+                                           eng.small_blo, eng.small.half, validate=False, pmap=pmap,
                                            pt=tile.get("pt", 0), wpb=tile.get("wpb", 0))
-        return kernels.correlate(self.bank, ps.spectra, ps.inv_norm, pl.dir_off, pl.dir_idx, eng.crop,
-                                 validate=False, pmap=pl.pmap, jt=tile.get("jt", 0))
+        return kernels.correlate(self.bank, ps.spectra, ps.inv_norm, dir_off, dir_idx, eng.crop,
+                                 validate=False, pmap=pmap, jt=tile.get("jt", 0))
 
-    def _device_step(self):
-        for c in range(self.slots):
+    def _device_step(self, k: Optional[int] = None):
+        k = self.slots if k is None else k
+        for c in range(k):
             self._templates(c)
-        corr = self._search(self.tile)
-        scores = kernels.score_sum_segments(corr.peak, corr.best, self.plan.slot_off)
+        corr = self._search(self.tile, k)
+        scores = kernels.score_sum_segments(corr.peak, corr.best, self.plan.prefix(k)[3])
         return scores, corr
 
     # -- host part -----------------------------------------------------------
@@ -281,11 +292,14 @@ class CandidateBatch:
         poses = np.asarray(poses, np.float64).reshape(-1, 6)
         if len(poses) > self.slots:
             raise ValueError(f"{len(poses)} poses for {self.slots} slots")
-        for c in range(self.slots):
-            pose = poses[min(c, len(poses) - 1)]  # unused slots repeat the last pose
-            self.coords_host[c].copy_(torch.from_numpy(self.body.moved(pose)))
-        self.coords_dev.copy_(self.coords_host, non_blocking=True)
-        return len(poses)
+        k = len(poses)
+        fill = self.slots if self.graph is not None else k  # a graph runs every slot: repeat the last pose
+        self._copied.synchronize()  # the previous step's copy out of the pinned buffer has run
+        for c in range(fill):
+            self.coords_host[c].copy_(torch.from_numpy(self.body.moved(poses[min(c, k - 1)])))
+        self.coords_dev[:fill].copy_(self.coords_host[:fill], non_blocking=True)
+        self._copied.record()
+        return k
 
     def tune(self, poses):
         """Pick the search kernel's tile variant for this plan on the banks of ``poses``."""
@@ -317,13 +331,14 @@ class CandidateBatch:
 
     def evaluate(self, poses, sync: bool = True):
         """Score ``poses`` ([k, 6], k <= slots); returns float64 [k] (or the device
-        tensor and the CorrelationResult when sync is False)."""
+        tensor and the CorrelationResult when sync is False).  Eager steps run
+        the first k slots only; a captured graph always runs every slot."""
         k = self._stage(poses)
         if self.graph is not None:
             self.graph.replay()
             scores, corr = self.static_out
         else:
-            scores, corr = self._device_step()
+            scores, corr = self._device_step(k)
         if not sync:
             return scores, corr
         return scores.cpu().numpy()[:k].copy()
