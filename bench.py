@@ -1,24 +1,32 @@
 #!/usr/bin/env python
-"""Candidate-scoring throughput on one MI355X, FP32 FFT path and split-bf16 path.
+"""Candidate-scoring throughput and serving latency on one MI355X.
 
-Workload (box256): 12,000-atom 3-chain model, moving chain 3,000 atoms,
+Main workload (box256): 12,000-atom 3-chain model, moving chain 3,000 atoms,
 5,000 particles of 256x256 at 1.06 A/px, HEALPix order 3 local search
-+/-10 deg, psi step 3.75 deg (5 psi).  The particle stack is resident in
-HBM and its psi spectra are prepared once per dataset before timing (the
-time is reported as ``particle_prep_ms``).  One step = one candidate
-evaluation: move the chain (host), splat the full model, 3-D transform,
-768-direction template bank, search every (particle, local direction, psi)
-pair, per-particle reduction and the float64 score sum.
++/-10 deg, psi step 3.75 deg (5 psi).  The particle stack is resident in HBM
+and its psi spectra are prepared once per dataset before timing (reported
+as ``particle_prep_ms``).  A candidate evaluation = move the chain (host),
+splat, 3-D transform, 768-direction template bank, search of every
+(particle, local direction, psi) pair, per-particle reduction and the
+float64 score sum.  Legs, on the same particles and poses:
 
-Two legs run back to back on the same particles and candidate poses:
-  fp32   the v606 search: full-plane products, inverse FFT, max over the
-         16 x 16 lag crop (+/-8 px)
-  mixed  the small-shift search: coefficients inside 1/(4 A), 49 lags
-         (+/-3 px), split-bf16 MFMA with FP32 accumulation
-``value`` is the mixed leg's particle-orientation scores/s; the fp32 leg,
-their ratio, top-1 agreement between the legs, and the mixed leg's error
-against the same functional evaluated in float64 on a 256-particle subset
-are reported beside it.
+  fp32           one candidate per step, the v606 FP32 FFT search (+/-8 px)
+  mixed          one candidate per step, the small-shift split-bf16 search
+                 (coefficients inside 1/(4 A), 49 lags, +/-3 px)
+  batched_fp32   ``--slots`` candidates per step, one search launch over all
+  batched_mixed  of them, replayed as a captured HIP graph, tile variant
+                 from the autotuner
+
+``value`` is batched_mixed particle-orientation scores/s; the ratio to the
+plain fp32 leg, top-1 agreement between the mixed and fp32 searches, and the
+mixed search's error against the same functional in float64 on a
+256-particle subset are reported beside it.
+
+Serving leg (``--serve-requests``, 0 to skip): three resident datasets (box
+128 / 2,000, box 192 / 10,000, box 256 / 20,000 particles, 3,000-atom moving
+chain), a seeded Poisson stream of requests at ``--serve-rate``/s with 1-64
+candidates each (geometric, mean 8), continuous batching on the mixed path;
+p50/p99 request latency and throughput.
 
 Prints ONE JSON line.  ``--gpus`` above 1 is not supported (one-GPU build).
 """
@@ -128,29 +136,82 @@ def precision_check(eng, ps, bank, res, subset: int, seed: int):
             "max_rel_err_fp64": worst, "reference": "same functional in float64 (torch complex128)"}
 
 
+def make_engine(wl, data, dev, resolution):
+    from gisapr_amd.engine import SearchEngine
+    eng = SearchEngine(box=wl.box, apix=wl.apix, crop=wl.crop, psi_step=wl.psi_step,
+                       local_range=wl.local_range, kk=wl.kk, voltage=wl.voltage, cs=wl.cs,
+                       mask_radius=wl.mask_radius, mask_edge=wl.mask_edge, fsc_values=data["fsc"],
+                       dir_rot=data["dir_rot"], dir_tilt=data["dir_tilt"], device=dev)
+    eng.enable_small_shift(resolution=resolution, half=3)
+    return eng
+
+
+def serve_leg(args, dev, log):
+    """Serving leg: three resident datasets, seeded Poisson request stream."""
+    import torch
+    from gisapr_amd import synthetic
+    from gisapr_amd.batch import Autotuner, RigidBody
+    from gisapr_amd.serve import SearchServer, poisson_requests
+    t0 = time.perf_counter()
+    server = SearchServer(dev, path="mixed", slots=args.serve_slots, autotuner=Autotuner())
+    names = ["serve128", "serve192", "serve256"]
+    sizes = {}
+    for name in names:
+        wl = synthetic.WORKLOADS[name]
+        data = synthetic.make_workload(name, device=dev)
+        eng = make_engine(wl, data, dev, args.resolution)
+        dfo = np.stack([data["dfu"], data["dfv"], data["dfa"]], 1)
+        ps = eng.add_particles(data["images"], data["rot"], data["tilt"], data["psi"], defocus=dfo)
+        del data["images"]
+        body = RigidBody.from_arrays(data["coords"], data["numbers"], data["chains"], wl.moving_chain)
+        ds = server.add_dataset(name, eng, ps, body, wl.res)
+        sizes[name] = {"box": wl.box, "particles": ps.B, "resident_GiB": ds.resident_bytes / 2 ** 30}
+        log(f"serving dataset {name} resident ({ds.resident_bytes / 2**30:.1f} GiB)")
+    setup = time.perf_counter() - t0
+    # warm every executor once (library set-up, autotuning) before the stream
+    for name in names:
+        for ex in server.datasets[name].executors:
+            ex.tune(np.zeros((server.slots, 6)))
+            ex.evaluate(np.zeros((1, 6)))
+    torch.cuda.synchronize()
+    reqs = poisson_requests(args.serve_requests, args.serve_rate, names, seed=7)
+    stats = server.serve(reqs)
+    stats.update(datasets=sizes, setup_s=setup, rate_per_s=args.serve_rate, slots=server.slots, path="mixed",
+                 offered_candidates_per_s=args.serve_rate * float(np.mean([r.size for r in reqs])))
+    return stats
+
+
 def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__)
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="box256")
     ap.add_argument("--particles", type=int, default=None)
-    ap.add_argument("--path", choices=("both", "fp32", "mixed"), default="both")
+    ap.add_argument("--slots", type=int, default=8, help="candidates per batched step")
     ap.add_argument("--resolution", type=float, default=4.0, help="small-shift disk radius in A")
     ap.add_argument("--precision-subset", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the legs, run one device-to-device copy of this many MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         return 2
 
+    def log(*a):
+        if args.verbose:
+            print("[bench]", *a, file=sys.stderr, flush=True)
+
     import torch
     from gisapr_amd import synthetic
-    from gisapr_amd.engine import SearchEngine
+    from gisapr_amd.batch import Autotuner, CandidateBatch, RigidBody
     from gisapr_amd.pdb_text import move_chain
 
     dev = torch.device("cuda", 0)
@@ -158,11 +219,8 @@ def main() -> int:
     wl = synthetic.WORKLOADS[args.workload]
     t_setup = time.perf_counter()
     data = synthetic.make_workload(args.workload, n_particles=args.particles, device=dev)
-    eng = SearchEngine(box=wl.box, apix=wl.apix, crop=wl.crop, psi_step=wl.psi_step,
-                       local_range=wl.local_range, kk=wl.kk, voltage=wl.voltage, cs=wl.cs,
-                       mask_radius=wl.mask_radius, mask_edge=wl.mask_edge, fsc_values=data["fsc"],
-                       dir_rot=data["dir_rot"], dir_tilt=data["dir_tilt"], device=dev)
-    tab = eng.enable_small_shift(resolution=args.resolution, half=3)
+    eng = make_engine(wl, data, dev, args.resolution)
+    tab = eng.small
     dfo = np.stack([data["dfu"], data["dfv"], data["dfa"]], 1)
     ps = eng.add_particles(data["images"], data["rot"], data["tilt"], data["psi"], defocus=dfo, prepare=False)
     del data["images"]
@@ -177,12 +235,14 @@ def main() -> int:
     prep_ms, gather_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
     n_scores = ps.n_scores
     rng = np.random.default_rng(synthetic.SEED_CAND)
-    poses = [synthetic.candidate_pose(seed=int(s), rot_deg=wl.cand_rot_deg, trans_A=wl.cand_trans_A)
-             for s in rng.integers(0, 2 ** 31, args.warmup + args.steps)]
+    npose = max(args.warmup + args.steps, 1) * max(args.slots, 1)
+    poses = np.stack([synthetic.candidate_pose(seed=int(s), rot_deg=wl.cand_rot_deg, trans_A=wl.cand_trans_A)
+                      for s in rng.integers(0, 2 ** 31, npose)])
     coords, numbers, chains = data["coords"], data["numbers"], data["chains"]
     setup_s = time.perf_counter() - t_setup
+    log("setup", setup_s)
 
-    def run_leg(search):
+    def run_single(search):
         stages = ("density", "templates", "search")
         ev_time = {k: 0.0 for k in stages}
 
@@ -215,33 +275,55 @@ def main() -> int:
         for ev in evs:
             for k, name in enumerate(stages):
                 ev_time[name] += ev[k].elapsed_time(ev[k + 1])
-        return {"scores_per_s": n_scores * args.steps / dt, "ms_per_step": dt * 1000.0 / args.steps,
+        return {"scores_per_s": n_scores * args.steps / dt, "ms_per_candidate": dt * 1000.0 / args.steps,
                 "stage_ms": {k: v / args.steps for k, v in ev_time.items()},
                 "score": float(res.score.cpu())}, res, bank
 
     legs, results = {}, {}
-    if args.path in ("both", "fp32"):
-        legs["fp32"], results["fp32"], bank = run_leg(eng.search)
-    if args.path in ("both", "mixed"):
-        legs["mixed"], results["mixed"], bank = run_leg(eng.search_small_shift)
+    legs["fp32"], results["fp32"], _ = run_single(eng.search)
+    legs["mixed"], results["mixed"], bank = run_single(eng.search_small_shift)
+    log("single legs", {k: v["ms_per_candidate"] for k, v in legs.items()})
 
-    # roofline of each leg's search kernel (HIP events on the launch stream)
+    # batched legs: ``slots`` candidates per step, graph-captured, autotuned search tile
+    body = RigidBody.from_arrays(coords, numbers, chains, wl.moving_chain)
+    tuner = Autotuner()
+    for path in ("fp32", "mixed"):
+        batch = CandidateBatch(eng, ps, body, wl.res, args.slots, path=path, autotuner=tuner)
+        batch.tune(poses[:args.slots])
+        batch.capture(poses[:args.slots])
+        for i in range(args.warmup):
+            batch.evaluate(poses[i * args.slots:(i + 1) * args.slots])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            j = (args.warmup + i) * args.slots
+            out = batch.evaluate(poses[j:j + args.slots])
+        dt = time.perf_counter() - t0
+        pairs = batch.plan.n_pairs
+        legs["batched_" + path] = {"scores_per_s": pairs * args.steps / dt,
+                                   "ms_per_step": dt * 1000.0 / args.steps,
+                                   "ms_per_candidate": dt * 1000.0 / (args.steps * args.slots),
+                                   "slots": args.slots, "tile": dict(batch.tile), "graph": True,
+                                   "last_scores": [float(x) for x in out[:2]]}
+        del batch
+        torch.cuda.empty_cache()
+    log("batched legs", {k: v["ms_per_candidate"] for k, v in legs.items()})
+
+    # roofline of each single leg's search stage (HIP events on the launch stream)
     n = eng.n
-    if "fp32" in legs:
-        ms = legs["fp32"]["stage_ms"]["search"]
-        ach = correlation_flops_per_score(n) * n_scores / (ms * 1e-3) / 1e12
-        legs["fp32"]["roofline"] = {"bound": "mfma", "kernel": "correlate_kernel", "achieved": ach,
-                                    "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                                    "frac": ach / PEAK_FP32_MFMA_TFLOPS,
-                                    "flops_per_score": correlation_flops_per_score(n), "launch_ms": ms}
-    if "mixed" in legs:
-        ms = legs["mixed"]["stage_ms"]["search"]
-        fps = mixed_flops_per_score(tab.K, tab.n_lags)
-        ach = fps * n_scores / (ms * 1e-3) / 1e12
-        legs["mixed"]["roofline"] = {"bound": "mfma", "kernel": "mixed_correlate_kernel (+ template gather)",
-                                     "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
-                                     "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps,
-                                     "launch_ms": ms}
+    ms = legs["fp32"]["stage_ms"]["search"]
+    ach = correlation_flops_per_score(n) * n_scores / (ms * 1e-3) / 1e12
+    legs["fp32"]["roofline"] = {"bound": "mfma", "kernel": "correlate_kernel", "achieved": ach,
+                                "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                                "frac": ach / PEAK_FP32_MFMA_TFLOPS,
+                                "flops_per_score": correlation_flops_per_score(n), "launch_ms": ms}
+    ms = legs["mixed"]["stage_ms"]["search"]
+    fps = mixed_flops_per_score(tab.K, tab.n_lags)
+    ach = fps * n_scores / (ms * 1e-3) / 1e12
+    legs["mixed"]["roofline"] = {"bound": "mfma", "kernel": "mixed_correlate_kernel (+ template gather, tail)",
+                                 "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
+                                 "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps, "launch_ms": ms}
+
     if args.copy_calibration_mb > 0:
         src = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -249,44 +331,50 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    head = "mixed" if "mixed" in legs else "fp32"
-    main_leg = legs[head]
+
+    head = legs["batched_mixed"]
     out = {
-        "metric": METRIC, "value": main_leg["scores_per_s"], "unit": "scores/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": main_leg["ms_per_step"],
+        "metric": METRIC, "value": head["scores_per_s"], "unit": "scores/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3 (split hi+lo, fp32 accumulate)" if head == "mixed" else "fp32",
+        "dtype": "bf16x3 (split hi+lo, fp32 accumulate)",
         "data": "synthetic (seeded generators; random-walk model)",
         "config": {"workload": f"{wl.name}: {sum(wl.chain_sizes)}-atom model (moving chain "
                                f"{wl.chain_sizes[-1]}), {ps.B} particles {wl.box}x{wl.box} at {wl.apix} A/px, "
                                f"HEALPix order {wl.healpix_order} ({len(eng.dir_rot)} directions) local "
-                               f"+/-{wl.local_range} deg, psi step {wl.psi_step} ({ps.P} psi); one candidate "
-                               f"per step, particle spectra resident",
+                               f"+/-{wl.local_range} deg, psi step {wl.psi_step} ({ps.P} psi); {args.slots} "
+                               f"candidates per batched step, particle spectra resident",
                    "box": wl.box, "particles": ps.B, "directions": len(eng.dir_rot), "psi": ps.P,
-                   "scores_per_step": n_scores, "path": head,
+                   "scores_per_candidate": n_scores, "path": "batched_mixed", "slots": args.slots,
                    "fp32_crop": wl.crop, "mixed_lags": tab.n_lags, "mixed_K": tab.K,
                    "mixed_resolution_A": args.resolution},
-        "candidates_per_s": 1000.0 / main_leg["ms_per_step"],
+        "candidates_per_s": 1000.0 / head["ms_per_candidate"],
+        "speedup_batched_mixed_vs_fp32": head["scores_per_s"] / legs["fp32"]["scores_per_s"],
+        "speedup_mixed_vs_fp32": legs["mixed"]["scores_per_s"] / legs["fp32"]["scores_per_s"],
         "particle_prep_ms": prep_ms, "coefficient_gather_ms": gather_ms,
         "legs": legs,
         "setup_s": setup_s,
-        "roofline": dict(main_leg["roofline"], traffic=None),
+        "roofline": dict(legs["mixed"]["roofline"], traffic=None),
         "cpu_baseline": None,
     }
-    if "fp32" in legs and "mixed" in legs:
-        out["speedup_mixed_vs_fp32"] = legs["mixed"]["scores_per_s"] / legs["fp32"]["scores_per_s"]
-        a = top1(results["fp32"], ps, eng.crop // 2)
-        b = top1(results["mixed"], ps, 3)
-        valid = a[:, 0] >= 0
-        out["top1_same_as_fp32"] = {
-            "orientation": float(np.mean((a[valid, 0] == b[valid, 0]))),
-            "orientation_and_shift": float(np.mean(np.all(a[valid] == b[valid], axis=1))),
-            "particles": int(valid.sum())}
-    if "mixed" in legs and args.precision_subset > 0:
-        out["precision"] = precision_check(eng, ps, bank, results["mixed"],
-                                           args.precision_subset, synthetic.SEED_CAND)
+    a = top1(results["fp32"], ps, eng.crop // 2)
+    b = top1(results["mixed"], ps, 3)
+    valid = a[:, 0] >= 0
+    out["top1_same_as_fp32"] = {
+        "orientation": float(np.mean((a[valid, 0] == b[valid, 0]))),
+        "orientation_and_shift": float(np.mean(np.all(a[valid] == b[valid], axis=1))),
+        "particles": int(valid.sum())}
+    if args.precision_subset > 0:
+        out["precision"] = precision_check(eng, ps, bank, results["mixed"], args.precision_subset,
+                                           synthetic.SEED_CAND)
     if not args.no_cpu_baseline and args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(eng, ps, bank, args.cpu_sample)
+    del ps, eng, bank, results
+    torch.cuda.empty_cache()
+    if args.serve_requests > 0:
+        out["serving"] = serve_leg(args, dev, log)
+        out["p50_ms"] = out["serving"]["p50_ms"]
+        out["p99_ms"] = out["serving"]["p99_ms"]
     print(json.dumps(out), flush=True)
     return 0
 

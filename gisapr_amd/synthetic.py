@@ -302,6 +302,16 @@ WORKLOADS: Dict[str, Workload] = {
     # Same image generator as box96 (shifts +/-3 px); the search covers +/-8 px (crop 16).
     "box256": Workload("box256", 256, 1.06, 2.12, [4500, 4500, 3000], "C", 5000, 0.05, 3, 10.0,
                        3.75, 16, fsc_resolution=4.0, confine=80.0),
+    # Batched PSO step: box 192, 1.3 A/px, 20,000 atoms (moving chain 4,500), 20,000 particles.
+    "box192": Workload("box192", 192, 1.3, 2.6, [7750, 7750, 4500], "C", 20000, 0.05, 3, 10.0,
+                       3.75, 16, fsc_resolution=4.0, confine=90.0, mask_radius=84),
+    # Serving datasets: 12,000-atom complex with a 3,000-atom moving chain at three boxes.
+    "serve128": Workload("serve128", 128, 2.0, 4.0, [4500, 4500, 3000], "C", 2000, 0.05, 3, 10.0,
+                         3.75, 16, fsc_resolution=8.0, confine=80.0, mask_radius=56),
+    "serve192": Workload("serve192", 192, 1.3, 2.6, [4500, 4500, 3000], "C", 10000, 0.05, 3, 10.0,
+                         3.75, 16, fsc_resolution=4.0, confine=80.0, mask_radius=84),
+    "serve256": Workload("serve256", 256, 1.06, 2.12, [4500, 4500, 3000], "C", 20000, 0.05, 3, 10.0,
+                         3.75, 16, fsc_resolution=4.0, confine=80.0),
     # Small fast cases for tests.
     "box64": Workload("box64", 64, 2.0, 4.0, [500, 200], "B", 24, 0.1, 2, 15.0, 7.5, 8,
                       mask_radius=28, confine=28.0),
