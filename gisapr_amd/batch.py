@@ -453,3 +453,151 @@ class RigidDensityCache:
                                  density_quantize_s=prepared["quantize_s"], density_prefetch_hits=int(prefetched),
                                  density_deposit_s=time.perf_counter() - t0)
         return volume
+
+
+def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torch.Tensor], path: str = "fp32",
+                   tile: Optional[dict] = None):
+    """Banks of several model volumes, then ONE search launch over all of them
+    (ZL:gisapr_batch_search.py:251-364, the candidate x direction dimension
+    batched).  Returns (per-model float64 scores, CorrelationResult, SlotPlan)."""
+    tile = tile or {}
+    C, M = len(volumes), len(engine.dir_rot)
+    plan = build_slot_plan(ps, M, C)
+    if path == "mixed":
+        if ps.coeffs is None:
+            engine.prepare_small_shift(ps)
+        tk = torch.empty((C * M, engine.small.Kpad), dtype=torch.complex64, device=engine.device)
+        for c, vol in enumerate(volumes):
+            bank = engine.templates(vol)
+            kernels.gather_coefficients(bank, engine.small_kidx, engine.small.Kpad, out=tk[c * M:(c + 1) * M])
+        corr = kernels.correlate_mixed(tk, ps.coeffs, ps.P, plan.dir_off, plan.dir_idx, engine.small_bhi,
+                                       engine.small_blo, engine.small.half, validate=False, pmap=plan.pmap,
+                                       pt=tile.get("pt", 0), wpb=tile.get("wpb", 0))
+    else:
+        if ps.spectra is None:
+            engine.prepare(ps)
+        n = engine.n
+        bank = torch.empty((C * M, n, n // 2 + 1), dtype=torch.complex64, device=engine.device)
+        for c, vol in enumerate(volumes):
+            dft = kernels.volume_dft(vol)
+            kernels.build_templates(dft, n, engine.rts, engine.mask, engine.fsc_packed, out=bank[c * M:(c + 1) * M])
+        corr = kernels.correlate(bank, ps.spectra, ps.inv_norm, plan.dir_off, plan.dir_idx, engine.crop,
+                                 validate=False, pmap=plan.pmap, jt=tile.get("jt", 0))
+    scores = kernels.score_sum_segments(corr.peak, corr.best, plan.slot_off)
+    return scores, corr, plan
+
+
+class BatchSearchEngine:
+    """Group evaluation of several models against one particle selection
+    (call surface of ZL:gisapr_batch_search.py:142-364).
+
+    ``run(args, models)`` takes the v606 argument namespace and a list of
+    ``generate_volume`` keyword dicts (``source_pdb``, ``chain_id``,
+    ``pose``, ``boxsize``, ``apix``, ``res``, ...); each model's density
+    comes from ``RigidDensityCache``, the particles are read and prepared
+    once per context, and every group of models is searched in one launch.
+    Returns one dict per model: ``score`` (float64 sum exp(rawCC/1024)),
+    ``search_text`` (the three v606 records per particle), ``metrics``.
+    ``options``: ``batch_candidates`` (group size), ``path`` ("fp32", the
+    reference's search, or "mixed"), ``fixed_density_cache_mb``,
+    ``batch_density_mode`` ("fixed" reuses the fixed body).  The fixed-bank
+    calibration mode (``orientation_rows`` / ``calibrate``) is not provided.
+    """
+
+    def __init__(self, search_globals=None, device=None, options=None):
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.options = dict(batch_candidates=8, path="fp32", fixed_density_cache_mb=512.0,
+                            batch_density_mode="fixed")
+        self.options.update(options or {})
+        self.density = RigidDensityCache()
+        self.context = None
+
+    def _context(self, args):
+        from . import host
+        from .search import ParticleMRCReader, read_model_angles, read_particle_table, select_particle_records
+        runtime = {"o", "mrc", "i", "gpuid", "verbose"}
+        key = tuple(sorted((k, str(v)) for k, v in vars(args).items() if k not in runtime))
+        if self.context is not None and self.context["key"] == key:
+            return self.context
+        N, n = int(args.oriboxsize), int(args.newboxsize)
+        angles = read_model_angles(args.ang)
+        table = read_particle_table(args.p)
+        records = select_particle_records(table, args.start, args.end)
+        if not records:
+            raise ValueError("No particles selected for batch evaluation")
+        engine = SearchEngine(box=N, new_box=n, apix=float(args.apix), crop=int(args.ccgCropSize),
+                              psi_step=float(args.psiStep), local_range=float(args.localRange),
+                              do_local_search=bool(args.doLocalSearch), kk=float(args.kk),
+                              voltage=float(args.voltage), cs=float(args.cs), mask_radius=int(args.maskRadius),
+                              mask_edge=int(args.maskEdge), discard_mask=bool(args.discardMask),
+                              fsc_values=host.read_fsc_values(args.FSC, N, bool(args.ignoreFSC)),
+                              dir_rot=angles.rot, dir_tilt=angles.tilt, device=self.device)
+        reader = ParticleMRCReader(N, self.device)
+        try:
+            images = reader.read_many([r.image_name for r in records])
+        finally:
+            reader.close()
+        dfo = None
+        if table.has_ctf:
+            dfo = np.asarray([[r.defocus_u, r.defocus_v, r.defocus_angle] for r in records], np.float32)
+        ps = engine.add_particles(images, [r.rot for r in records], [r.tilt for r in records],
+                                  [r.psi for r in records], defocus=dfo, line_index=[r.line_index for r in records])
+        if self.options["path"] == "mixed":
+            engine.enable_small_shift()
+            engine.prepare_small_shift(ps)
+        self.context = dict(key=key, args=args, engine=engine, ps=ps, records=records, angles=angles)
+        return self.context
+
+    def run(self, args, models, orientation_rows=None, calibrate=False):
+        from .search import _records
+        if orientation_rows is not None or calibrate:
+            raise NotImplementedError("the fixed-bank calibration mode is not provided by this engine")
+        t0 = time.perf_counter()
+        ctx = self._context(args)
+        setup_s = time.perf_counter() - t0
+        engine, ps, records, angles = ctx["engine"], ctx["ps"], ctx["records"], ctx["angles"]
+        path = self.options["path"]
+        center = engine.crop // 2 if path == "fp32" else engine.small.half
+        P, M = ps.P, len(angles.rot)
+        results = []
+        cap = max(1, int(self.options["batch_candidates"]))
+        for g0 in range(0, len(models), cap):
+            group = models[g0:g0 + cap]
+            t1 = time.perf_counter()
+            volumes = [self.density.generate(dict(m, device=str(self.device)),
+                                             self.options["fixed_density_cache_mb"],
+                                             self.options["batch_density_mode"]) for m in group]
+            torch.cuda.synchronize(self.device)
+            t2 = time.perf_counter()
+            scores, corr, plan = search_volumes(engine, ps, volumes, path)
+            sc = scores.cpu().numpy()
+            torch.cuda.synchronize(self.device)
+            t3 = time.perf_counter()
+            best = corr.best.cpu().numpy()
+            peak = corr.peak.cpu().numpy()
+            mean = corr.mean.cpu().numpy()
+            std = corr.std.cpu().numpy()
+            py = corr.peak_y.cpu().numpy().reshape(-1)
+            px = corr.peak_x.cpu().numpy().reshape(-1)
+            d_off = plan.dir_off.cpu().numpy()
+            d_idx = plan.dir_idx.cpu().numpy()
+            for c in range(len(group)):
+                lines = []
+                for j in range(int(plan.slot_off_host[c]), int(plan.slot_off_host[c + 1])):
+                    b = j - int(plan.slot_off_host[c])
+                    if best[j] < 0:
+                        continue
+                    li, pi = divmod(int(best[j]), P)
+                    g = (d_off[j] + li) * P + pi
+                    gi = int(d_idx[d_off[j] + li]) - c * M
+                    for item in _records(records[b], ps.fake_psi[b], angles.rot, angles.tilt, gi, pi,
+                                         float(peak[j]), int(py[g]), int(px[g]), float(mean[j]), float(std[j]),
+                                         2 * center, engine.trans_scale):
+                        lines.append(str(item) + "\n")
+                n_g = len(group)
+                metrics = dict(density_s=(t2 - t1) / n_g, search_s=(t3 - t2) / n_g, setup_s=setup_s / len(models),
+                               **self.density.last_metrics)
+                metrics["execution_s"] = metrics["density_s"] + metrics["search_s"] + metrics["setup_s"]
+                results.append(dict(score=float(sc[c]), orientation_rows=None, search_text="".join(lines),
+                                    metrics=metrics))
+        return results

@@ -106,3 +106,50 @@ def test_autotuner_variants_agree(setup, tmp_path):
             np.testing.assert_array_equal(b, outs[0][1])
     again = Autotuner(str(tmp_path / "tune.json"))
     assert again.table == tuner.table and len(tuner.table) == 2
+
+
+def test_batch_search_engine_matches_run_search(hip_device, tmp_path):
+    """ZL-shaped BatchSearchEngine.run: each model's records equal run_search
+    on that model's density (same direction, psi and shift; rawCC within
+    1e-5: the fixed body and the moving chain are splatted in two passes)."""
+    import ast
+    import torch
+    from gisapr_amd import density, mrc, search, star, synthetic
+    from gisapr_amd.batch import BatchSearchEngine
+    say("BatchSearchEngine")
+    wl = synthetic.WORKLOADS["box32"]
+    d = synthetic.make_workload("box32")
+    mrc.write_mrc(tmp_path / "parts.mrcs", d["images"].numpy())
+    star.write_particle_star(tmp_path / "parts.star", str(tmp_path / "parts.mrcs"), d["rot"], d["tilt"], d["psi"],
+                             d["dfu"], d["dfv"], d["dfa"])
+    star.write_direction_star(tmp_path / "dirs.star", d["dir_rot"], d["dir_tilt"])
+    (tmp_path / "model.pdb").write_text(synthetic.pdb_text(d["coords"], d["elements"], d["chains"]))
+    argv = ["--i", "x", "--p", str(tmp_path / "parts.star"), "--o", str(tmp_path / "out.txt"),
+            "--ang", str(tmp_path / "dirs.star"), "--mrc", "unused.mrc",
+            "--oriboxsize", str(wl.box), "--newboxsize", str(wl.box),
+            "--apix", str(wl.apix), "--kk", str(wl.kk), "--psiStep", str(wl.psi_step), "--doLocalSearch",
+            "--localRange", str(wl.local_range), "--ccgCropSize", str(wl.crop),
+            "--maskRadius", str(wl.mask_radius), "--maskEdge", str(wl.mask_edge)]
+    args = search.create_search_parser().parse_args(argv)
+    models = [dict(pdb_path=str(tmp_path / "model.pdb"), source_pdb=str(tmp_path / "model.pdb"),
+                   chain_id=wl.moving_chain, pose=list(p), boxsize=wl.box, apix=wl.apix, res=wl.res)
+              for p in ([0, 0, 0, 0, 0, 0], [3.0, -2.0, 1.0, 0.5, 0.0, -0.5], [-4.0, 1.0, 2.0, 0.0, 1.0, 0.0])]
+    eng = BatchSearchEngine(None, hip_device, dict(batch_candidates=2))
+    out = eng.run(args, models)
+    assert len(out) == 3
+    for m, r in zip(models, out):
+        vol = density.generate_volume(**{k: v for k, v in m.items()}, device=hip_device)
+        search.run_search(args, model_volume=vol)
+        want = [ast.literal_eval(x) for x in open(tmp_path / "out.txt")]
+        got = [ast.literal_eval(x) for x in r["search_text"].splitlines()]
+        assert len(got) == len(want) > 0
+        for g, w in zip(got, want):
+            assert g[:2] == w[:2]
+            if g[0] == "cc: ":
+                assert g[5:7] == w[5:7] and g[8:10] == w[8:10]
+            elif g[0] == "psi: ":
+                assert g[2] == w[2]
+            else:
+                assert g[2] == pytest.approx(w[2], rel=1e-5)
+        raw = [w[2] for w in want if w[0] == "rawCC: "]
+        assert r["score"] == pytest.approx(float(np.sum(np.exp(np.asarray(raw) / 1024.0))), rel=1e-5)
