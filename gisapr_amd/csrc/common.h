@@ -169,6 +169,50 @@ __device__ float2* fft_lds(float2* a, float2* b, int nseq, int sp, const float2*
   return src;
 }
 
+// Orders LDS accesses of the lanes of one wave (a wave executes its LDS
+// operations in order; this keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Same transform as fft_lds, but each wave owns whole sequences (q = wave,
+// wave + nwaves, ...) and runs all their stages with wave-level ordering
+// only: no workgroup barrier between stages.  Ends with one __syncthreads.
+template <int L, bool INV>
+__device__ float2* fft_lds_waves(float2* a, float2* b, int nseq, int sp, const float2* tw, int tid, int nthr) {
+  using S = FftShape<L>;
+  const int wave = tid >> 6, lane = tid & 63, nw = nthr >> 6;
+  constexpr int parity = S::stages & 1;
+  for (int q = wave; q < nseq; q += nw) {
+    float2* src = a + q * sp;
+    float2* dst = b + q * sp;
+    int Ns = 1;
+#pragma unroll
+    for (int s = 0; s < S::n4; ++s) {
+      stockham_stage<L, 4, INV>(src, dst, Ns, 1, 0, tw, lane, 64);
+      wave_lds_sync();
+      float2* t = src; src = dst; dst = t;
+      Ns *= 4;
+    }
+    if constexpr (S::n2) {
+      stockham_stage<L, 2, INV>(src, dst, Ns, 1, 0, tw, lane, 64);
+      wave_lds_sync();
+      float2* t = src; src = dst; dst = t;
+      Ns *= 2;
+    }
+    if constexpr (S::three) {
+      stockham_stage<L, 3, INV>(src, dst, Ns, 1, 0, tw, lane, 64);
+      wave_lds_sync();
+      float2* t = src; src = dst; dst = t;
+      Ns *= 3;
+    }
+  }
+  __syncthreads();
+  return parity ? b : a;
+}
+
 // ---------------------------------------------------------------------------
 // Block-wide reductions (blockDim.x a multiple of 64)
 // ---------------------------------------------------------------------------

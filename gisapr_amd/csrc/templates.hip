@@ -143,7 +143,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         }
       }
       __syncthreads();
-      float2* res = fft_lds<L2, true>(bufA, bufB, nk, L2 + 1, tw2, tid, nthr);
+      float2* res = fft_lds_waves<L2, true>(bufA, bufB, nk, L2 + 1, tw2, tid, nthr);
       // keep spatial rows y in [3N/2, 2N) u [0, N/2) -> projection row Y
       for (int t = tid; t < T::TK1 * N; t += nthr) {
         const int Y = t / T::TK1, q = t - Y * T::TK1;
@@ -170,7 +170,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         bufA[q * L2 + k] = make_float2(a.x - b.y, a.y + b.x);  // a + i*b
       }
       __syncthreads();
-      float2* res = fft_lds<L2, true>(bufA, bufB, nseq, L2, tw2, tid, nthr);
+      float2* res = fft_lds_waves<L2, true>(bufA, bufB, nseq, L2, tw2, tid, nthr);
       float2* oth = res == bufA ? bufB : bufA;
       const float sc = 1.0f / ((float)L2 * (float)L2);
       for (int t = tid; t < nseq * N; t += nthr) {
@@ -179,7 +179,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         oth[q * N + X] = make_float2(v.x * sc, v.y * sc);  // projection rows ya (re), yb (im)
       }
       __syncthreads();
-      float2* spec = fft_lds<N, false>(oth, res, nseq, N, tw1, tid, nthr);
+      float2* spec = fft_lds_waves<N, false>(oth, res, nseq, N, tw1, tid, nthr);
       for (int t = tid; t < nseq * MH; t += nthr) {
         const int q = t / MH, k = t - q * MH;
         const float2 z = spec[q * N + k];
@@ -211,7 +211,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         bufA[q * (N + 1) + Y] = W[(size_t)Y * MH + k0 + q];
       }
       __syncthreads();
-      float2* res = fft_lds<N, false>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
+      float2* res = fft_lds_waves<N, false>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
       for (int t = tid; t < T::TK3 * N; t += nthr) {
         const int ky = t / T::TK3, q = t - ky * T::TK3;
         if (q >= nk) continue;
@@ -292,7 +292,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         bufA[q * (N + 1) + ky] = make_float2(v.x * g, v.y * g);
       }
       __syncthreads();
-      float2* res = fft_lds<N, true>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
+      float2* res = fft_lds_waves<N, true>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
       for (int t = tid; t < T::TK3 * N; t += nthr) {
         const int Y = t / T::TK3, q = t - Y * T::TK3;
         if (q >= nk) continue;
@@ -317,7 +317,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         bufA[q * N + k] = make_float2(a.x - b.y, a.y + b.x);
       }
       __syncthreads();
-      float2* res = fft_lds<N, true>(bufA, bufB, nseq, N, tw1, tid, nthr);
+      float2* res = fft_lds_waves<N, true>(bufA, bufB, nseq, N, tw1, tid, nthr);
       const float sc = 1.0f / ((float)N * (float)N);
       for (int t = tid; t < nseq * N; t += nthr) {
         const int q = t / N, X = t - q * N;
@@ -394,7 +394,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         bufA[q * N + X] = make_float2(va, vb);
       }
       __syncthreads();
-      float2* res = fft_lds<N, false>(bufA, bufB, nseq, N, tw1, tid, nthr);
+      float2* res = fft_lds_waves<N, false>(bufA, bufB, nseq, N, tw1, tid, nthr);
       for (int t = tid; t < nseq * MH; t += nthr) {
         const int q = t / MH, k = t - q * MH;
         const float2 z = res[q * N + k];
@@ -419,7 +419,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         bufA[q * (N + 1) + Y] = W[(size_t)Y * MH + k0 + q];
       }
       __syncthreads();
-      float2* res = fft_lds<N, false>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
+      float2* res = fft_lds_waves<N, false>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
       for (int t = tid; t < T::TK3 * n; t += nthr) {
         const int r = t / T::TK3, q = t - r * T::TK3;
         if (q >= nk) continue;
