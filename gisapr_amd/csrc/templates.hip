@@ -43,13 +43,21 @@ struct TplLayout {
   static constexpr int NR = (int)(0.7072 * N) + 3;  // rings (upper bound)
   static constexpr int FIXED = L2 + N + NR / 2 + 1 + 32;  // twiddles, ring divisors, reductions
   static constexpr int CAP = (kLdsFloat2 - FIXED) / 2;    // per ping-pong buffer
-  static constexpr int TK1 = CAP / (L2 + 1);              // phase 1 columns per tile
-  static constexpr int RB2 = CAP / L2;                    // phase 2 complex rows per batch
+  // sequence counts per tile are rounded down to a multiple of the 4 waves
+  // (each wave owns whole sequences of a tile)
+  static constexpr int round4(int x) { return x >= 4 ? x & ~3 : x; }
+  static constexpr int TK1 = round4(CAP / (L2 + 1));      // phase 1 columns per tile
+  static constexpr int RB2 = round4(CAP / L2);            // phase 2 complex rows per batch
   static constexpr int TK3 = CAP / (N + 1);               // box-length column tiles
   static constexpr int RB6 = CAP / N;                     // box-length complex rows
   static_assert(TK1 >= 1 && RB2 >= 1, "LDS budget");
-  static constexpr size_t U_ELEMS = (size_t)N * M2;  // float2
-  static constexpr size_t W_ELEMS = (size_t)N * MH;  // float2
+  // Scratch images are stored tile-major (the columns of one tile of TK
+  // columns form one contiguous [rows][TK] block), so the column passes
+  // read and write contiguous blocks and the row passes full 32-byte runs.
+  static constexpr int UT = (M2 + TK1 - 1) / TK1;  // column tiles of U
+  static constexpr int WT = (MH + TK3 - 1) / TK3;  // column tiles of W
+  static constexpr size_t U_ELEMS = (size_t)N * UT * TK1;  // float2
+  static constexpr size_t W_ELEMS = (size_t)N * WT * TK3;  // float2
   static constexpr size_t SLOT = U_ELEMS + W_ELEMS;
 };
 
@@ -73,6 +81,12 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
   fill_twiddles(tw1, N, tid, nthr);
   float2* U = scratch + (size_t)blockIdx.x * T::SLOT;
   float2* W = U + T::U_ELEMS;
+  auto uidx = [](int row, int col) -> size_t {
+    return ((size_t)(col / T::TK1) * N + row) * T::TK1 + col % T::TK1;
+  };
+  auto widx = [](int row, int col) -> size_t {
+    return ((size_t)(col / T::TK3) * N + row) * T::TK3 + col % T::TK3;
+  };
   float* Z = (float*)U;  // real image, aliases U once U is consumed
   const int mh_n = n / 2 + 1;
   const float inv_l2 = 1.0f / (float)L2;
@@ -85,19 +99,21 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     // ---- phase 1: gather + inverse column FFT (length 2N) ----------------
     for (int k0 = 0; k0 < M2; k0 += T::TK1) {
       const int nk = min(T::TK1, M2 - k0);
-      // Each wave samples blocks of 8 slice rows x (up to) 8 slice columns,
+      // Each wave samples blocks of (64/CW) slice rows x CW slice columns,
       // so one load instruction touches a compact patch of the 3-D spectrum,
       // which is stored in 2x2x4 bricks (one cache line each).  Branch-free:
       // every tap index is clamped into the array; a clamped upper tap
       // always carries weight 0 (the coordinate was clamped to the last
       // index first), so all eight loads of a sample are in flight together.
-      const int ngroups = (nk + 7) / 8;
+      constexpr int CW = T::TK1 <= 4 ? 4 : 8;  // columns per lane patch
+      constexpr int RW = 64 / CW;              // rows per lane patch
+      const int ngroups = (nk + CW - 1) / CW;
       for (int cg = 0; cg < ngroups; ++cg) {
 #pragma unroll 4
-        for (int it = 0; it < L2 / 32; ++it) {
-          const int t = tid + it * kThreads;  // [0, 8 * L2)
+        for (int it = 0; it < CW * L2 / kThreads; ++it) {
+          const int t = tid + it * kThreads;  // [0, CW * L2)
           const int lane = t & 63, rb = t >> 6;
-          const int q = cg * 8 + (lane >> 3), ip = rb * 8 + (lane & 7);
+          const int q = cg * CW + lane / RW, ip = rb * RW + lane % RW;
           const bool active = q < nk;
           const int kx = k0 + min(q, nk - 1);
           const int f = ip < N ? ip : ip - L2;
@@ -149,7 +165,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         const int Y = t / T::TK1, q = t - Y * T::TK1;
         if (q >= nk) continue;
         const int y = (Y + 3 * N / 2) % L2;
-        U[(size_t)Y * M2 + k0 + q] = res[q * (L2 + 1) + y];
+        U[uidx(Y, k0 + q)] = res[q * (L2 + 1) + y];
       }
       __syncthreads();
     }
@@ -163,8 +179,8 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         const int ya = y0 + 2 * q, yb = ya + 1;
         // Hermitian extension of the half spectrum, Im of columns 0 and N dropped
         const int ks = k <= N ? k : L2 - k;
-        float2 a = U[(size_t)ya * M2 + ks];
-        float2 b = yb < y0 + nrow ? U[(size_t)yb * M2 + ks] : make_float2(0.f, 0.f);
+        float2 a = U[uidx(ya, ks)];
+        float2 b = yb < y0 + nrow ? U[uidx(yb, ks)] : make_float2(0.f, 0.f);
         if (ks == 0 || ks == N) { a.y = 0.f; b.y = 0.f; }
         if (k > N) { a.y = -a.y; b.y = -b.y; }
         bufA[q * L2 + k] = make_float2(a.x - b.y, a.y + b.x);  // a + i*b
@@ -185,9 +201,9 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         const float2 z = spec[q * N + k];
         const float2 zc = cconj(spec[q * N + (N - k) % N]);
         const int ya = y0 + 2 * q;
-        W[(size_t)ya * MH + k] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+        W[widx(ya, k)] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
         if (ya + 1 < y0 + nrow)
-          W[(size_t)(ya + 1) * MH + k] = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+          W[widx(ya + 1, k)] = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
       }
       __syncthreads();
     }
@@ -208,14 +224,14 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       for (int t = tid; t < T::TK3 * N; t += nthr) {
         const int Y = t / T::TK3, q = t - Y * T::TK3;
         if (q >= nk) continue;
-        bufA[q * (N + 1) + Y] = W[(size_t)Y * MH + k0 + q];
+        bufA[q * (N + 1) + Y] = W[widx(Y, k0 + q)];
       }
       __syncthreads();
       float2* res = fft_lds_waves<N, false>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
       for (int t = tid; t < T::TK3 * N; t += nthr) {
         const int ky = t / T::TK3, q = t - ky * T::TK3;
         if (q >= nk) continue;
-        W[(size_t)ky * MH + k0 + q] = res[q * (N + 1) + ky];
+        W[widx(ky, k0 + q)] = res[q * (N + 1) + ky];
       }
       const int lane = tid & 63;
       for (int task = tid >> 6; task < 2 * nk; task += nthr / 64) {
@@ -288,7 +304,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         if (q >= nk) continue;
         const int fy = ky < N / 2 ? ky : ky - N;
         const float g = div[ring_of(fy, k0 + q)];
-        const float2 v = W[(size_t)ky * MH + k0 + q];
+        const float2 v = W[widx(ky, k0 + q)];
         bufA[q * (N + 1) + ky] = make_float2(v.x * g, v.y * g);
       }
       __syncthreads();
@@ -296,7 +312,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       for (int t = tid; t < T::TK3 * N; t += nthr) {
         const int Y = t / T::TK3, q = t - Y * T::TK3;
         if (q >= nk) continue;
-        W[(size_t)Y * MH + k0 + q] = res[q * (N + 1) + Y];
+        W[widx(Y, k0 + q)] = res[q * (N + 1) + Y];
       }
       __syncthreads();
     }
@@ -310,8 +326,8 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         const int q = t / N, k = t - q * N;
         const int ya = y0 + 2 * q, yb = ya + 1;
         const int ks = k <= N / 2 ? k : N - k;
-        float2 a = W[(size_t)ya * MH + ks];
-        float2 b = yb < y0 + nrow ? W[(size_t)yb * MH + ks] : make_float2(0.f, 0.f);
+        float2 a = W[widx(ya, ks)];
+        float2 b = yb < y0 + nrow ? W[widx(yb, ks)] : make_float2(0.f, 0.f);
         if (ks == 0 || ks == N / 2) { a.y = 0.f; b.y = 0.f; }
         if (k > N / 2) { a.y = -a.y; b.y = -b.y; }
         bufA[q * N + k] = make_float2(a.x - b.y, a.y + b.x);
@@ -400,9 +416,9 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         const float2 z = res[q * N + k];
         const float2 zc = cconj(res[q * N + (N - k) % N]);
         const int ya = y0 + 2 * q;
-        W[(size_t)ya * MH + k] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+        W[widx(ya, k)] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
         if (ya + 1 < y0 + nrow)
-          W[(size_t)(ya + 1) * MH + k] = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+          W[widx(ya + 1, k)] = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
       }
       __syncthreads();
     }
@@ -416,7 +432,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       for (int t = tid; t < T::TK3 * N; t += nthr) {
         const int Y = t / T::TK3, q = t - Y * T::TK3;
         if (q >= nk) continue;
-        bufA[q * (N + 1) + Y] = W[(size_t)Y * MH + k0 + q];
+        bufA[q * (N + 1) + Y] = W[widx(Y, k0 + q)];
       }
       __syncthreads();
       float2* res = fft_lds_waves<N, false>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
