@@ -23,6 +23,12 @@
 #include "common.h"
 #include "vdft_layout.h"
 
+// LDS per workgroup (float2): 52.8 KB lets three workgroups share a CU.
+// Sequences per workgroup are a multiple of the 4 waves (each wave owns
+// whole sequences; see fft_lds_waves).
+constexpr int kVdftLds = 6600;
+constexpr int vdft_round4(int x) { return x >= 4 ? x & ~3 : x; }
+
 // Row x' of the shifted padded volume holds padded column (x'+N) mod 2N,
 // which is data column (x'+N) mod 2N - N/2 when that lies in [0, N).
 __device__ __forceinline__ int shifted_to_data(int xs, int N) {
@@ -42,7 +48,7 @@ __global__ void __launch_bounds__(256)
 vdft_x_pass(const float* __restrict__ vol, float2* __restrict__ D) {
   constexpr int L = 2 * N;
   constexpr int M = N + 1;
-  constexpr int RB = (8192 - L) / (2 * L);  // complex sequences per block (LDS <= 64 KiB)
+  constexpr int RB = vdft_round4((kVdftLds - L) / (2 * L));  // complex sequences per block
   extern __shared__ float2 lds[];
   float2* tw = lds;
   float2* bufA = tw + L;
@@ -76,7 +82,7 @@ vdft_x_pass(const float* __restrict__ vol, float2* __restrict__ D) {
     bufA[q * L + xs] = make_float2(v2[0], v2[1]);
   }
   __syncthreads();
-  float2* res = fft_lds<L, false>(bufA, bufB, nseq, L, tw, tid, nthr);
+  float2* res = fft_lds_waves<L, false>(bufA, bufB, nseq, L, tw, tid, nthr);
   // split: A[k] = (Z[k] + conj Z[L-k]) / 2 ; B[k] = (Z[k] - conj Z[L-k]) / (2i)
   for (int t = tid; t < nseq * M; t += nthr) {
     const int q = t / M, k = t - q * M;
@@ -102,7 +108,7 @@ __global__ void __launch_bounds__(256)
 vdft_y_pass(float2* __restrict__ D) {
   constexpr int L = 2 * N;
   constexpr int M = N + 1;
-  constexpr int TK = (8192 - L) / (2 * (L + 1));
+  constexpr int TK = vdft_round4((kVdftLds - L) / (2 * (L + 1)));
   constexpr int SP = L + 1;
   extern __shared__ float2 lds[];
   float2* tw = lds;
@@ -123,7 +129,7 @@ vdft_y_pass(float2* __restrict__ D) {
     bufA[q * SP + ys] = v;
   }
   __syncthreads();
-  float2* res = fft_lds<L, false>(bufA, bufB, nk, SP, tw, tid, nthr);
+  float2* res = fft_lds_waves<L, false>(bufA, bufB, nk, SP, tw, tid, nthr);
   for (int t = tid; t < L * TK; t += nthr) {
     const int ky = t / TK, q = t - ky * TK;
     if (q < nk) P[(size_t)ky * M + k0 + q] = res[q * SP + ky];
@@ -137,7 +143,7 @@ __global__ void __launch_bounds__(256)
 vdft_z_pass(const float2* __restrict__ W, float2* __restrict__ D) {
   constexpr int L = 2 * N;
   constexpr int M = N + 1;
-  constexpr int TK = (8192 - L) / (2 * (L + 1));
+  constexpr int TK = vdft_round4((kVdftLds - L) / (2 * (L + 1)));
   constexpr int SP = L + 1;
   extern __shared__ float2 lds[];
   float2* tw = lds;
@@ -157,7 +163,7 @@ vdft_z_pass(const float2* __restrict__ W, float2* __restrict__ D) {
     bufA[q * SP + zs] = v;
   }
   __syncthreads();
-  float2* res = fft_lds<L, false>(bufA, bufB, nk, SP, tw, tid, nthr);
+  float2* res = fft_lds_waves<L, false>(bufA, bufB, nk, SP, tw, tid, nthr);
   const size_t ypart = vdft_brick_y<N>(ky);
   for (int t = tid; t < L * TK; t += nthr) {
     const int kz = t / TK, q = t - kz * TK;
@@ -169,8 +175,8 @@ template <int N>
 static int launch_volume_dft(const float* vol, float2* D, float2* Wk, hipStream_t st) {
   constexpr int L = 2 * N;
   constexpr int M = N + 1;
-  constexpr int RB = (8192 - L) / (2 * L);
-  constexpr int TK = (8192 - L) / (2 * (L + 1));
+  constexpr int RB = vdft_round4((kVdftLds - L) / (2 * L));
+  constexpr int TK = vdft_round4((kVdftLds - L) / (2 * (L + 1)));
   const int pairs = N * N / 2;
   const size_t lx = (size_t)(L + 2 * RB * L) * sizeof(float2);
   hipLaunchKernelGGL(vdft_x_pass<N>, dim3((pairs + RB - 1) / RB), dim3(256), lx, st, vol, Wk);
