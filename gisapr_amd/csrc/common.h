@@ -70,8 +70,9 @@ __device__ __forceinline__ void fill_twiddles(float2* tw, int L, int tid, int nt
 }
 
 // ---------------------------------------------------------------------------
-// LDS Stockham FFT, L = 2^a * 3^b (b <= 1): radix-4 stages, one radix-2 stage
-// if a is odd, one radix-3 stage if 3 | L.  Natural-order input and output.
+// LDS Stockham FFT, L = 2^a * 3^b (b <= 1): radix-8 stages, then one radix-4
+// or radix-2 stage for the rest of 2^a, one radix-3 stage if 3 | L.
+// Natural-order input and output.
 //
 // nseq sequences live in LDS at buf[q * sp + i]; the result ends in one of
 // the two ping-pong buffers, whose address is returned.  The caller must
@@ -104,6 +105,28 @@ __device__ __forceinline__ void stockham_stage(const float2* __restrict__ src, f
       float2 a = v[0], b = v[1];
       v[0] = cadd(a, b);
       v[1] = csub(a, b);
+    } else if constexpr (R == 8) {
+      // two 4-point DFTs (even / odd inputs) and the twiddles w^m, w = e^{-+2 pi i / 8}
+      float2 a = cadd(v[0], v[4]), b = csub(v[0], v[4]);
+      float2 c = cadd(v[2], v[6]), d = mul_mi<INV>(csub(v[2], v[6]));
+      const float2 A0 = cadd(a, c), A2 = csub(a, c), A1 = cadd(b, d), A3 = csub(b, d);
+      a = cadd(v[1], v[5]); b = csub(v[1], v[5]);
+      c = cadd(v[3], v[7]); d = mul_mi<INV>(csub(v[3], v[7]));
+      const float2 B0 = cadd(a, c), B2 = csub(a, c), B1 = cadd(b, d), B3 = csub(b, d);
+      constexpr float rh = 0.70710678118654752440f;
+      const float2 t1 = INV ? make_float2((B1.x - B1.y) * rh, (B1.x + B1.y) * rh)
+                            : make_float2((B1.x + B1.y) * rh, (B1.y - B1.x) * rh);
+      const float2 t2 = mul_mi<INV>(B2);
+      const float2 t3 = INV ? make_float2((-B3.x - B3.y) * rh, (B3.x - B3.y) * rh)
+                            : make_float2((B3.y - B3.x) * rh, (-B3.x - B3.y) * rh);
+      v[0] = cadd(A0, B0);
+      v[4] = csub(A0, B0);
+      v[1] = cadd(A1, t1);
+      v[5] = csub(A1, t1);
+      v[2] = cadd(A2, t2);
+      v[6] = csub(A2, t2);
+      v[3] = cadd(A3, t3);
+      v[7] = csub(A3, t3);
     } else if constexpr (R == 4) {
       float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
       float2 c = cadd(v[1], v[3]), d = mul_mi<INV>(csub(v[1], v[3]));
@@ -136,9 +159,10 @@ struct FftShape {
   static constexpr int lg = (p2 >= 512) ? 9 : (p2 >= 256) ? 8 : (p2 >= 128) ? 7 : (p2 >= 64) ? 6
                           : (p2 >= 32) ? 5 : (p2 >= 16) ? 4 : (p2 >= 8) ? 3 : (p2 >= 4) ? 2 : 1;
   static_assert((1 << lg) == p2, "FFT length must be 2^a or 3*2^a");
-  static constexpr int n4 = lg / 2;
-  static constexpr int n2 = lg % 2;
-  static constexpr int stages = n4 + n2 + three;
+  static constexpr int n8 = lg / 3;                  // radix-8 stages
+  static constexpr int n4 = (lg % 3 == 2) ? 1 : 0;   // then one radix-4 or one radix-2 stage
+  static constexpr int n2 = (lg % 3 == 1) ? 1 : 0;
+  static constexpr int stages = n8 + n4 + n2 + three;
 };
 
 template <int L, bool INV>
@@ -148,7 +172,13 @@ __device__ float2* fft_lds(float2* a, float2* b, int nseq, int sp, const float2*
   float2* dst = b;
   int Ns = 1;
 #pragma unroll
-  for (int s = 0; s < S::n4; ++s) {
+  for (int s = 0; s < S::n8; ++s) {
+    stockham_stage<L, 8, INV>(src, dst, Ns, nseq, sp, tw, tid, nthr);
+    __syncthreads();
+    float2* t = src; src = dst; dst = t;
+    Ns *= 8;
+  }
+  if constexpr (S::n4) {
     stockham_stage<L, 4, INV>(src, dst, Ns, nseq, sp, tw, tid, nthr);
     __syncthreads();
     float2* t = src; src = dst; dst = t;
@@ -190,7 +220,13 @@ __device__ float2* fft_lds_waves(float2* a, float2* b, int nseq, int sp, const f
     float2* dst = b + q * sp;
     int Ns = 1;
 #pragma unroll
-    for (int s = 0; s < S::n4; ++s) {
+    for (int s = 0; s < S::n8; ++s) {
+      stockham_stage<L, 8, INV>(src, dst, Ns, 1, 0, tw, lane, 64);
+      wave_lds_sync();
+      float2* t = src; src = dst; dst = t;
+      Ns *= 8;
+    }
+    if constexpr (S::n4) {
       stockham_stage<L, 4, INV>(src, dst, Ns, 1, 0, tw, lane, 64);
       wave_lds_sync();
       float2* t = src; src = dst; dst = t;
