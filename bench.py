@@ -136,6 +136,21 @@ def precision_check(eng, ps, bank, res, subset: int, seed: int):
             "max_rel_err_fp64": worst, "reference": "same functional in float64 (torch complex128)"}
 
 
+def profiled_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of ``kernel`` on ``workload`` from the committed
+    rocprofv3 PMC summary profiles/<workload>_<kernel>_traffic.md (its
+    ``traffic_bytes_per_launch:`` line), or None when there is none."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"{workload}_{kernel}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path, "r", encoding="utf-8") as fh:
+        for line in fh:
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1].strip())
+    return None
+
+
 def make_engine(wl, data, dev, resolution):
     from gisapr_amd.engine import SearchEngine
     eng = SearchEngine(box=wl.box, apix=wl.apix, crop=wl.crop, psi_step=wl.psi_step,
@@ -199,6 +214,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the legs, run one device-to-device copy of this many MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--legs", default="fp32,mixed,batched_fp32,batched_mixed",
+                    help="comma list of legs to run (profiling runs select one)")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -279,15 +296,21 @@ def main() -> int:
                 "stage_ms": {k: v / args.steps for k, v in ev_time.items()},
                 "score": float(res.score.cpu())}, res, bank
 
+    want = set(args.legs.split(","))
     legs, results = {}, {}
-    legs["fp32"], results["fp32"], _ = run_single(eng.search)
-    legs["mixed"], results["mixed"], bank = run_single(eng.search_small_shift)
+    bank = None
+    if "fp32" in want:
+        legs["fp32"], results["fp32"], bank = run_single(eng.search)
+    if "mixed" in want:
+        legs["mixed"], results["mixed"], bank = run_single(eng.search_small_shift)
     log("single legs", {k: v["ms_per_candidate"] for k, v in legs.items()})
 
     # batched legs: ``slots`` candidates per step, graph-captured, autotuned search tile
     body = RigidBody.from_arrays(coords, numbers, chains, wl.moving_chain)
     tuner = Autotuner()
     for path in ("fp32", "mixed"):
+        if "batched_" + path not in want:
+            continue
         batch = CandidateBatch(eng, ps, body, wl.res, args.slots, path=path, autotuner=tuner)
         batch.tune(poses[:args.slots])
         batch.capture(poses[:args.slots])
@@ -311,18 +334,23 @@ def main() -> int:
 
     # roofline of each single leg's search stage (HIP events on the launch stream)
     n = eng.n
-    ms = legs["fp32"]["stage_ms"]["search"]
-    ach = correlation_flops_per_score(n) * n_scores / (ms * 1e-3) / 1e12
-    legs["fp32"]["roofline"] = {"bound": "mfma", "kernel": "correlate_kernel", "achieved": ach,
-                                "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                                "frac": ach / PEAK_FP32_MFMA_TFLOPS,
-                                "flops_per_score": correlation_flops_per_score(n), "launch_ms": ms}
-    ms = legs["mixed"]["stage_ms"]["search"]
-    fps = mixed_flops_per_score(tab.K, tab.n_lags)
-    ach = fps * n_scores / (ms * 1e-3) / 1e12
-    legs["mixed"]["roofline"] = {"bound": "mfma", "kernel": "mixed_correlate_kernel (+ template gather, tail)",
-                                 "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
-                                 "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps, "launch_ms": ms}
+    if "fp32" in legs:
+        ms = legs["fp32"]["stage_ms"]["search"]
+        ach = correlation_flops_per_score(n) * n_scores / (ms * 1e-3) / 1e12
+        legs["fp32"]["roofline"] = {"bound": "mfma", "kernel": "correlate_kernel", "achieved": ach,
+                                    "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                                    "frac": ach / PEAK_FP32_MFMA_TFLOPS,
+                                    "flops_per_score": correlation_flops_per_score(n), "launch_ms": ms,
+                                    "traffic": profiled_traffic("correlate_kernel", wl.name)}
+    if "mixed" in legs:
+        ms = legs["mixed"]["stage_ms"]["search"]
+        fps = mixed_flops_per_score(tab.K, tab.n_lags)
+        ach = fps * n_scores / (ms * 1e-3) / 1e12
+        legs["mixed"]["roofline"] = {"bound": "mfma",
+                                     "kernel": "mixed_correlate_kernel (+ template gather, tail)",
+                                     "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
+                                     "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps,
+                                     "launch_ms": ms, "traffic": profiled_traffic("mixed_correlate_kernel", wl.name)}
 
     if args.copy_calibration_mb > 0:
         src = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device=dev)
@@ -332,12 +360,15 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
 
-    head = legs["batched_mixed"]
+    head_name = next(k for k in ("batched_mixed", "mixed", "batched_fp32", "fp32") if k in legs)
+    head = legs[head_name]
+    single = "mixed" if "mixed" in legs else "fp32"
     out = {
         "metric": METRIC, "value": head["scores_per_s"], "unit": "scores/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head.get("ms_per_step", head["ms_per_candidate"]),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3 (split hi+lo, fp32 accumulate)",
+        "dtype": "bf16x3 (split hi+lo, fp32 accumulate)" if "mixed" in head_name else "fp32",
         "data": "synthetic (seeded generators; random-walk model)",
         "config": {"workload": f"{wl.name}: {sum(wl.chain_sizes)}-atom model (moving chain "
                                f"{wl.chain_sizes[-1]}), {ps.B} particles {wl.box}x{wl.box} at {wl.apix} A/px, "
@@ -345,29 +376,30 @@ def main() -> int:
                                f"+/-{wl.local_range} deg, psi step {wl.psi_step} ({ps.P} psi); {args.slots} "
                                f"candidates per batched step, particle spectra resident",
                    "box": wl.box, "particles": ps.B, "directions": len(eng.dir_rot), "psi": ps.P,
-                   "scores_per_candidate": n_scores, "path": "batched_mixed", "slots": args.slots,
+                   "scores_per_candidate": n_scores, "path": head_name, "slots": args.slots,
                    "fp32_crop": wl.crop, "mixed_lags": tab.n_lags, "mixed_K": tab.K,
                    "mixed_resolution_A": args.resolution},
         "candidates_per_s": 1000.0 / head["ms_per_candidate"],
-        "speedup_batched_mixed_vs_fp32": head["scores_per_s"] / legs["fp32"]["scores_per_s"],
-        "speedup_mixed_vs_fp32": legs["mixed"]["scores_per_s"] / legs["fp32"]["scores_per_s"],
         "particle_prep_ms": prep_ms, "coefficient_gather_ms": gather_ms,
         "legs": legs,
         "setup_s": setup_s,
-        "roofline": dict(legs["mixed"]["roofline"], traffic=None),
+        "roofline": dict(legs[single].get("roofline", {})),
         "cpu_baseline": None,
     }
-    a = top1(results["fp32"], ps, eng.crop // 2)
-    b = top1(results["mixed"], ps, 3)
-    valid = a[:, 0] >= 0
-    out["top1_same_as_fp32"] = {
-        "orientation": float(np.mean((a[valid, 0] == b[valid, 0]))),
-        "orientation_and_shift": float(np.mean(np.all(a[valid] == b[valid], axis=1))),
-        "particles": int(valid.sum())}
-    if args.precision_subset > 0:
+    if "fp32" in legs:
+        out["speedup_vs_fp32"] = {k: v["scores_per_s"] / legs["fp32"]["scores_per_s"] for k, v in legs.items()}
+    if "fp32" in results and "mixed" in results:
+        a = top1(results["fp32"], ps, eng.crop // 2)
+        b = top1(results["mixed"], ps, 3)
+        valid = a[:, 0] >= 0
+        out["top1_same_as_fp32"] = {
+            "orientation": float(np.mean((a[valid, 0] == b[valid, 0]))),
+            "orientation_and_shift": float(np.mean(np.all(a[valid] == b[valid], axis=1))),
+            "particles": int(valid.sum())}
+    if args.precision_subset > 0 and "mixed" in results:
         out["precision"] = precision_check(eng, ps, bank, results["mixed"], args.precision_subset,
                                            synthetic.SEED_CAND)
-    if not args.no_cpu_baseline and args.cpu_sample > 0:
+    if not args.no_cpu_baseline and args.cpu_sample > 0 and bank is not None:
         out["cpu_baseline"] = cpu_baseline(eng, ps, bank, args.cpu_sample)
     del ps, eng, bank, results
     torch.cuda.empty_cache()
