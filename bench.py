@@ -60,11 +60,13 @@ def mixed_flops_per_score(K: int, lags: int) -> float:
 
 def cpu_baseline(eng, ps, bank, sample: int):
     """The oracle (float32 NumPy port, one core) on the first ``sample`` particles:
-    particle preparation + correlation + per-particle record, templates given."""
+    particle preparation + correlation + per-particle record, templates given
+    (``bank`` = (rows, 1/norms) from the GPU)."""
     from oracle import fourier as ofr
     from oracle import particles as opt
     from oracle import search as osr
-    bank_pre = ofr.pre_shift(bank.cpu().numpy())
+    raw, tinv = bank
+    bank_pre = ofr.pre_shift((raw * tinv[:, None, None]).cpu().numpy())  # unit-norm rows
     mask = eng.mask.cpu().numpy() if eng.mask is not None else None
     ctf = opt.PackedCTF(eng.n, eng.new_apix, 2.7, 300.0, eng.ctf.kk)
     sample = min(sample, ps.B)
@@ -106,13 +108,14 @@ def top1(res, ps, center):
 
 def precision_check(eng, ps, bank, res, subset: int, seed: int):
     """Mixed-leg scores against the same truncated functional in float64
-    (torch complex128 on the device) on a seeded particle subset."""
+    (torch complex128 on the device) on a seeded particle subset.  ``bank``
+    is (rows, 1/norms) of the candidate the mixed leg searched last."""
     import torch
     tab = eng.small
     dev = eng.device
     rng = np.random.default_rng(seed)
     pick = np.sort(rng.choice(ps.B, min(subset, ps.B), replace=False))
-    tk = eng.template_coefficients(bank)[:, :tab.K].to(torch.complex128)
+    tk = eng.template_coefficients(*bank)[:, :tab.K].to(torch.complex128)
     pk = ps.coeffs[:, :tab.K].to(torch.complex128)
     phi = torch.from_numpy(tab.phi).to(dev)
     off = ps.dir_off.cpu().numpy()
@@ -271,13 +274,13 @@ def main() -> int:
             vol = eng.density(xyz, numbers, wl.res)
             if timed:
                 ev[1].record()
-            bank = eng.templates(vol)
+            bank, tinv = eng.templates(vol, normalize=False)
             if timed:
                 ev[2].record()
-            res = search(bank, ps)
+            res = search(bank, ps, tinv)
             if timed:
                 ev[3].record()
-            return res, bank, ev
+            return res, (bank, tinv), ev
 
         for i in range(args.warmup):
             step(poses[i], False)

@@ -24,13 +24,13 @@ SIGNATURES = {
     "gis_volume_dft_workspace": (_sz, [_i]),
     "gis_volume_dft": (_i, [_vp, _i, _vp, _vp, _sz, _vp]),
     "gis_build_templates_workspace": (_sz, [_i, _i]),
-    "gis_build_templates": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "gis_build_templates": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _sz, _vp]),
     "gis_prepare_particles_workspace": (_sz, [_i, _i]),
     "gis_prepare_particles": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gis_ctf_packed": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "gis_correlate_workspace": (_sz, [_i]),
-    "gis_correlate": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
-                           _vp, _vp, _sz, _vp]),
+    "gis_correlate": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp,
+                           _vp, _vp, _vp, _sz, _vp]),
     "gis_score_sum_segments": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
     "gis_score_sum": (_i, [_vp, _vp, _i, _vp, _vp]),
     "gis_gather_coefficients": (_i, [_vp, _sz, _i, _vp, _i, _i, _vp, _vp, _vp]),

@@ -248,6 +248,7 @@ class CandidateBatch:
             self.tk = torch.empty((self.slots * M, engine.small.Kpad), dtype=torch.complex64, device=dev)
         else:
             self.bank = torch.empty((self.slots * M, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+        self.tinv = torch.empty((self.slots * M,), dtype=torch.float32, device=dev)  # 1/norm per bank row
         self._copied = torch.cuda.Event()
         self.autotuner = autotuner
         self.tile: dict = {}
@@ -261,13 +262,14 @@ class CandidateBatch:
         kernels.splat(self.coords_dev[c], self.moving_numbers, eng.N, self.width, eng.apix, self.rp, self.kn,
                       self.volume)
         kernels.volume_dft(self.volume, out=self.dft, workspace=self.dft_ws)
+        rows = slice(c * self.M, (c + 1) * self.M)
+        bank = self.bank if self.path == "mixed" else self.bank[rows]
+        _, norms = kernels.build_templates(self.dft, eng.n, eng.rts, eng.mask, eng.fsc_packed, out=bank,
+                                           normalize=False)
+        torch.reciprocal(norms, out=self.tinv[rows])
         if self.path == "mixed":
-            kernels.build_templates(self.dft, eng.n, eng.rts, eng.mask, eng.fsc_packed, out=self.bank)
-            kernels.gather_coefficients(self.bank, eng.small_kidx, eng.small.Kpad,
-                                        out=self.tk[c * self.M:(c + 1) * self.M])
-        else:
-            kernels.build_templates(self.dft, eng.n, eng.rts, eng.mask, eng.fsc_packed,
-                                    out=self.bank[c * self.M:(c + 1) * self.M])
+            kernels.gather_coefficients(self.bank, eng.small_kidx, eng.small.Kpad, self.tinv[rows],
+                                        out=self.tk[rows])
 
     def _search(self, tile: dict, k: Optional[int] = None):
         eng, ps = self.engine, self.ps
@@ -277,7 +279,7 @@ class CandidateBatch:
                                            eng.small_blo, eng.small.half, validate=False, pmap=pmap,
                                            pt=tile.get("pt", 0), wpb=tile.get("wpb", 0))
         return kernels.correlate(self.bank, ps.spectra, ps.inv_norm, dir_off, dir_idx, eng.crop,
-                                 validate=False, pmap=pmap, jt=tile.get("jt", 0))
+                                 validate=False, pmap=pmap, jt=tile.get("jt", 0), tinv=self.tinv)
 
     def _device_step(self, k: Optional[int] = None):
         k = self.slots if k is None else k
@@ -468,8 +470,8 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
             engine.prepare_small_shift(ps)
         tk = torch.empty((C * M, engine.small.Kpad), dtype=torch.complex64, device=engine.device)
         for c, vol in enumerate(volumes):
-            bank = engine.templates(vol)
-            kernels.gather_coefficients(bank, engine.small_kidx, engine.small.Kpad, out=tk[c * M:(c + 1) * M])
+            bank, tinv = engine.templates(vol, normalize=False)
+            kernels.gather_coefficients(bank, engine.small_kidx, engine.small.Kpad, tinv, out=tk[c * M:(c + 1) * M])
         corr = kernels.correlate_mixed(tk, ps.coeffs, ps.P, plan.dir_off, plan.dir_idx, engine.small_bhi,
                                        engine.small_blo, engine.small.half, validate=False, pmap=plan.pmap,
                                        pt=tile.get("pt", 0), wpb=tile.get("wpb", 0))
@@ -478,11 +480,14 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
             engine.prepare(ps)
         n = engine.n
         bank = torch.empty((C * M, n, n // 2 + 1), dtype=torch.complex64, device=engine.device)
+        tinv = torch.empty((C * M,), dtype=torch.float32, device=engine.device)
         for c, vol in enumerate(volumes):
             dft = kernels.volume_dft(vol)
-            kernels.build_templates(dft, n, engine.rts, engine.mask, engine.fsc_packed, out=bank[c * M:(c + 1) * M])
+            _, norms = kernels.build_templates(dft, n, engine.rts, engine.mask, engine.fsc_packed,
+                                               out=bank[c * M:(c + 1) * M], normalize=False)
+            torch.reciprocal(norms, out=tinv[c * M:(c + 1) * M])
         corr = kernels.correlate(bank, ps.spectra, ps.inv_norm, plan.dir_off, plan.dir_idx, engine.crop,
-                                 validate=False, pmap=plan.pmap, jt=tile.get("jt", 0))
+                                 validate=False, pmap=plan.pmap, jt=tile.get("jt", 0), tinv=tinv)
     scores = kernels.score_sum_segments(corr.peak, corr.best, plan.slot_off)
     return scores, corr, plan
 

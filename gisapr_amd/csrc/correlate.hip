@@ -83,8 +83,9 @@ __global__ void build_corr_tables(int n, int C, int nrow_pad, int mpad, float2* 
 
 __global__ void __launch_bounds__(kThreads)
 correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spectra,
-                 const float* __restrict__ inv_norm, const int* __restrict__ dir_off,
-                 const int* __restrict__ dir_idx, const int* __restrict__ pmap, int P, int n, int C, int JT, int PT,
+                 const float* __restrict__ inv_norm, const float* __restrict__ tinv,
+                 const int* __restrict__ dir_off, const int* __restrict__ dir_idx, const int* __restrict__ pmap,
+                 int P, int n, int C, int JT, int PT,
                  const float2* __restrict__ Wt, const float2* __restrict__ Et,
                  float* __restrict__ scores, int* __restrict__ peak_y, int* __restrict__ peak_x,
                  int* __restrict__ best, float* __restrict__ peak, double* __restrict__ mean,
@@ -216,7 +217,8 @@ correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spe
         const int k = wave + kWaves * i;
         if (k >= sweep) continue;
         const int j = k / pt, p = k - j * pt;
-        const float sc = 1024.0f * (inv_norm ? inv_norm[(size_t)pb * P + p0 + p] : 1.0f);
+        const float sc = 1024.0f * (inv_norm ? inv_norm[(size_t)pb * P + p0 + p] : 1.0f) *
+                         (tinv ? tinv[dir_idx[d0 + j0 + j]] : 1.0f);
         float bv = -INFINITY;
         int bi = 0x7fffffff;
 #pragma unroll
@@ -335,14 +337,16 @@ GIS_API size_t gis_correlate_workspace(int n) {
 //   bank      [Mtot][n][n/2+1] complex64, unit-norm templates (UNshifted)
 //   spectra   [B][P][n][n/2+1] complex64 conjugated particle spectra (UNshifted)
 //   inv_norm  [B][P] float32 per-spectrum 1/L2 norm, or NULL (already unit norm)
+//   tinv      [Mtot] float32 per-template 1/L2 norm, or NULL (bank already unit norm)
 //   dir_off   [B+1] int32 offsets into dir_idx; dir_idx global template indices
 //   pmap      [B] int32 spectra row of each searched particle, or NULL (identity);
 //             lets one launch search several candidates' banks against shared spectra
 //   jt        template tiles per sweep (0 = automatic)
 //   scores/peak_y/peak_x  [dir_off[B]][P] outputs (score = max * 1024 n^2)
 //   best [B] flat argmax (local_dir * P + psi) or -1; peak [B]; mean/std [B] float64
-GIS_API int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, const int* dir_off,
-                          const int* dir_idx, const int* pmap, int B, int P, int n, int C, int jt, float* scores,
+GIS_API int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
+                          const int* dir_off, const int* dir_idx, const int* pmap, int B, int P, int n, int C,
+                          int jt, float* scores,
                           int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
                           size_t ws_bytes, void* stream) {
   GIS_REQUIRE(B >= 0 && P >= 1, "gis_correlate: B=%d P=%d", B, P);
@@ -378,7 +382,7 @@ GIS_API int gis_correlate(const void* bank, const void* spectra, const float* in
     attr_set = true;
   }
   hipLaunchKernelGGL(correlate_kernel, dim3(B), dim3(kThreads), lds, st, (const float2*)bank,
-                     (const float2*)spectra, inv_norm, dir_off, dir_idx, pmap, P, n, C, JT, PT, Wt, Et, scores,
+                     (const float2*)spectra, inv_norm, tinv, dir_off, dir_idx, pmap, P, n, C, JT, PT, Wt, Et, scores,
                      peak_y, peak_x, best, peak, mean, stdv);
   GIS_LAUNCH_CHECK("gis_correlate");
   return GIS_OK;

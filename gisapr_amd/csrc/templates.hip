@@ -28,8 +28,17 @@
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kLdsFloat2 = 6600;  // 52.8 KB of LDS: three workgroups per CU (a 768-direction bank in one round)
+// Workgroup shape (build-time tunables): 256 threads and 52.8 KB of LDS let
+// three workgroups share a CU, so a 768-direction bank runs in one round.
+#ifndef GIS_TPL_THREADS
+#define GIS_TPL_THREADS 256
+#endif
+#ifndef GIS_TPL_LDS
+#define GIS_TPL_LDS 6600
+#endif
+constexpr int kThreads = GIS_TPL_THREADS;
+constexpr int kWavesT = kThreads / 64;
+constexpr int kLdsFloat2 = GIS_TPL_LDS;
 
 __device__ __forceinline__ int ring_of(int fy, int kx) {
   return (int)rintf(sqrtf((float)(fy * fy + kx * kx)));
@@ -43,11 +52,11 @@ struct TplLayout {
   static constexpr int NR = (int)(0.7072 * N) + 3;  // rings (upper bound)
   static constexpr int FIXED = L2 + N + NR / 2 + 1 + 32;  // twiddles, ring divisors, reductions
   static constexpr int CAP = (kLdsFloat2 - FIXED) / 2;    // per ping-pong buffer
-  // sequence counts per tile are rounded down to a multiple of the 4 waves
+  // sequence counts per tile are rounded down to a multiple of the waves
   // (each wave owns whole sequences of a tile)
-  static constexpr int round4(int x) { return x >= 4 ? x & ~3 : x; }
-  static constexpr int TK1 = round4(CAP / (L2 + 1));      // phase 1 columns per tile
-  static constexpr int RB2 = round4(CAP / L2);            // phase 2 complex rows per batch
+  static constexpr int roundw(int x) { return x >= kWavesT ? x - x % kWavesT : x; }
+  static constexpr int TK1 = roundw(CAP / (L2 + 1));      // phase 1 columns per tile
+  static constexpr int RB2 = roundw(CAP / L2);            // phase 2 complex rows per batch
   static constexpr int TK3 = CAP / (N + 1);               // box-length column tiles
   static constexpr int RB6 = CAP / N;                     // box-length complex rows
   static_assert(TK1 >= 1 && RB2 >= 1, "LDS budget");
@@ -65,10 +74,11 @@ template <int N>
 __global__ void __launch_bounds__(kThreads)
 build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ rts, int M, int n,
                        const float* __restrict__ mask, const float* __restrict__ fsc,
-                       float2* __restrict__ bank, float* __restrict__ norms, float2* __restrict__ scratch) {
+                       float2* __restrict__ bank, float* __restrict__ norms, float2* __restrict__ scratch,
+                       int normalize) {
   using T = TplLayout<N>;
   constexpr int L2 = T::L2, M2 = T::M2, MH = T::MH;
-  __shared__ float2 lds[kLdsFloat2];
+  extern __shared__ float2 lds[];  // kLdsFloat2 values
   float2* tw2 = lds;                       // L2 twiddles
   float2* tw1 = tw2 + L2;                  // N twiddles
   float* div = (float*)(tw1 + N);          // NR ring divisors
@@ -110,8 +120,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       const int ngroups = (nk + CW - 1) / CW;
       for (int cg = 0; cg < ngroups; ++cg) {
 #pragma unroll 4
-        for (int it = 0; it < CW * L2 / kThreads; ++it) {
-          const int t = tid + it * kThreads;  // [0, CW * L2)
+        for (int t = tid; t < CW * L2; t += kThreads) {
           const int lane = t & 63, rb = t >> 6;
           const int q = cg * CW + lane / RW, ip = rb * RW + lane % RW;
           const bool active = q < nk;
@@ -457,28 +466,42 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       __syncthreads();
     }
 
-    // ---- phase 10: unit L2 norm over the packed half plane ---------------
+    // ---- phase 10: unit L2 norm over the packed half plane (or, when the
+    // caller applies it later, only the norm) ------------------------------
     block_sum_f64<1>(nrm, red);
     const float norm = (float)sqrt(nrm[0]);
-    for (int t = tid; t < n * mh_n; t += nthr) {
-      float2 v = out[t];
-      out[t] = make_float2(v.x / norm, v.y / norm);
-    }
+    if (normalize)
+      for (int t = tid; t < n * mh_n; t += nthr) {
+        float2 v = out[t];
+        out[t] = make_float2(v.x / norm, v.y / norm);
+      }
     if (norms && tid == 0) norms[d] = norm;
   }
 }
 
 template <int N>
 int launch_build_templates(const float2* D, const float* rts, int M, int n, const float* mask,
-                           const float* fsc, float2* bank, float* norms, void* ws, size_t ws_bytes,
+                           const float* fsc, float2* bank, float* norms, int normalize, void* ws, size_t ws_bytes,
                            hipStream_t st) {
   using T = TplLayout<N>;
   const size_t slot = T::SLOT * sizeof(float2);
   int grid = (int)std::min<size_t>((size_t)M, ws_bytes / slot);
   grid = std::min(grid, 4 * gis_cu_count());
   GIS_REQUIRE(grid >= 1, "gis_build_templates: workspace of %zu bytes is below one slot (%zu)", ws_bytes, slot);
-  hipLaunchKernelGGL(build_templates_kernel<N>, dim3(grid), dim3(kThreads), 0, st, D, rts, M, n, mask, fsc,
-                     bank, norms, (float2*)ws);
+  const size_t lds = (size_t)kLdsFloat2 * sizeof(float2);
+  static bool attr_set = false;
+  if (lds > 65536 && !attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)build_templates_kernel<N>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      gis_set_error("gis_build_templates: cannot reserve %zu bytes of LDS: %s", lds, hipGetErrorString(e));
+      return (int)e;
+    }
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(build_templates_kernel<N>, dim3(grid), dim3(kThreads), lds, st, D, rts, M, n, mask, fsc,
+                     bank, norms, (float2*)ws, normalize);
   GIS_LAUNCH_CHECK("gis_build_templates");
   return GIS_OK;
 }
@@ -518,9 +541,10 @@ GIS_API size_t gis_build_templates_workspace(int N, int M) {
 //   bank  [M][n][n/2+1] complex64 output (unit L2 norm, UNshifted rfft2 layout)
 //   norms [M] float32 output of the pre-normalisation norms, or NULL
 GIS_API int gis_build_templates(const void* dft, int N, int n, const float* rts, int M, const float* mask,
-                                const float* fsc, void* bank, float* norms, void* ws, size_t ws_bytes,
-                                void* stream) {
+                                const float* fsc, void* bank, float* norms, int normalize, void* ws,
+                                size_t ws_bytes, void* stream) {
   GIS_REQUIRE(dft && rts && bank && ws, "gis_build_templates: null pointer");
+  GIS_REQUIRE(normalize || norms, "gis_build_templates: an unnormalised bank needs the norms output");
   GIS_REQUIRE(M >= 0, "gis_build_templates: M < 0");
   GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0, "gis_build_templates: newbox %d must be even and <= box %d", n, N);
   if (M == 0) return GIS_OK;
@@ -528,12 +552,12 @@ GIS_API int gis_build_templates(const void* dft, int N, int n, const float* rts,
   const float2* D = (const float2*)dft;
   float2* B = (float2*)bank;
   switch (N) {
-    case 32: return launch_build_templates<32>(D, rts, M, n, mask, fsc, B, norms, ws, ws_bytes, st);
-    case 64: return launch_build_templates<64>(D, rts, M, n, mask, fsc, B, norms, ws, ws_bytes, st);
-    case 96: return launch_build_templates<96>(D, rts, M, n, mask, fsc, B, norms, ws, ws_bytes, st);
-    case 128: return launch_build_templates<128>(D, rts, M, n, mask, fsc, B, norms, ws, ws_bytes, st);
-    case 192: return launch_build_templates<192>(D, rts, M, n, mask, fsc, B, norms, ws, ws_bytes, st);
-    case 256: return launch_build_templates<256>(D, rts, M, n, mask, fsc, B, norms, ws, ws_bytes, st);
+    case 32: return launch_build_templates<32>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st);
+    case 64: return launch_build_templates<64>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st);
+    case 96: return launch_build_templates<96>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st);
+    case 128: return launch_build_templates<128>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st);
+    case 192: return launch_build_templates<192>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st);
+    case 256: return launch_build_templates<256>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st);
     default:
       gis_set_error("gis_build_templates: box %d not in {32, 64, 96, 128, 192, 256}", N);
       return GIS_EUNSUPPORTED;

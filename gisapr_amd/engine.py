@@ -154,19 +154,26 @@ class SearchEngine:
         vol = base.clone() if base is not None else None
         return kernels.splat(xyz, zn, self.N, width, self.apix, rp, kn, vol)
 
-    def templates(self, volume: torch.Tensor, with_norms=False):
+    def templates(self, volume: torch.Tensor, with_norms=False, normalize=True):
+        """Unit-norm bank (and norms); with ``normalize=False`` returns
+        (bank, 1/norms) with unnormalised rows, for ``search(..., tinv=)``."""
         dft = kernels.volume_dft(volume)
-        return kernels.build_templates(dft, self.n, self.rts, self.mask, self.fsc_packed, with_norms)
+        out = kernels.build_templates(dft, self.n, self.rts, self.mask, self.fsc_packed, with_norms,
+                                      normalize=normalize)
+        if normalize:
+            return out
+        bank, norms = out
+        return bank, 1.0 / norms
 
     # ---------------------------------------------------------------- search
-    def search(self, bank: torch.Tensor, ps: ParticleSet) -> SearchResult:
+    def search(self, bank: torch.Tensor, ps: ParticleSet, tinv: Optional[torch.Tensor] = None) -> SearchResult:
         if ps.spectra is None:
             self.prepare(ps)
         if int(bank.shape[0]) != len(self.dir_rot):
             raise ValueError("template bank does not cover the engine's direction table")
         # dir_off / dir_idx were built on the host against this direction table
         corr = kernels.correlate(bank, ps.spectra, ps.inv_norm, ps.dir_off, ps.dir_idx, self.crop,
-                                 validate=False)
+                                 validate=False, tinv=tinv)
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.crop // 2)
 
     # ---------------------------------------------------------------- small-shift precision search
@@ -196,18 +203,19 @@ class SearchEngine:
             ps.spectra = None
         return ps
 
-    def template_coefficients(self, bank: torch.Tensor) -> torch.Tensor:
+    def template_coefficients(self, bank: torch.Tensor, tinv: Optional[torch.Tensor] = None) -> torch.Tensor:
         self._require_small()
-        return kernels.gather_coefficients(bank, self.small_kidx, self.small.Kpad)
+        return kernels.gather_coefficients(bank, self.small_kidx, self.small.Kpad, tinv)
 
-    def search_small_shift(self, bank: torch.Tensor, ps: ParticleSet) -> SearchResult:
+    def search_small_shift(self, bank: torch.Tensor, ps: ParticleSet,
+                           tinv: Optional[torch.Tensor] = None) -> SearchResult:
         """Best lag in [-half, half]^2 per (direction, psi) on split-bf16 MFMA."""
         self._require_small()
         if ps.coeffs is None:
             self.prepare_small_shift(ps)
         if int(bank.shape[0]) != len(self.dir_rot):
             raise ValueError("template bank does not cover the engine's direction table")
-        tk = self.template_coefficients(bank)
+        tk = self.template_coefficients(bank, tinv)
         corr = kernels.correlate_mixed(tk, ps.coeffs, ps.P, ps.dir_off, ps.dir_idx, self.small_bhi,
                                        self.small_blo, self.small.half, validate=False)
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.small.half)

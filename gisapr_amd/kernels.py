@@ -75,9 +75,12 @@ def volume_dft_natural(dft: torch.Tensor, N: int) -> torch.Tensor:
 
 def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Optional[torch.Tensor],
                     fsc_packed: Optional[torch.Tensor], with_norms: bool = False,
-                    out: Optional[torch.Tensor] = None):
-    """Unit-norm packed template bank [M, n, n/2+1] complex64 (unshifted),
-    written into ``out`` when given."""
+                    out: Optional[torch.Tensor] = None, normalize: bool = True):
+    """Packed template bank [M, n, n/2+1] complex64 (unshifted), written into
+    ``out`` when given.  Unit-norm rows (the reference's bank) unless
+    ``normalize`` is False: then the rows keep their scale and the norms are
+    returned (callers fold 1/norm into the search)."""
+    with_norms = with_norms or not normalize
     require_hip(dft, "dft")
     N = next((b for b in SUPPORTED_BOXES if int(lib().gis_volume_dft_bytes(b)) == dft.numel() * 8), 0)
     if not N or dft.dtype != torch.complex64:
@@ -104,7 +107,8 @@ def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Op
     wsb = int(lib().gis_build_templates_workspace(N, M))
     ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dft.device)
     check(lib().gis_build_templates(ptr(dft), N, n, ptr(rts), M, ptr(mask), ptr(fsc_packed), ptr(bank),
-                                    ptr(norms), ptr(ws), wsb, stream_of(dft.device)), "gis_build_templates")
+                                    ptr(norms), int(bool(normalize)), ptr(ws), wsb, stream_of(dft.device)),
+          "gis_build_templates")
     return (bank, norms) if with_norms else bank
 
 
@@ -167,9 +171,11 @@ def _check_pmap(pmap, B, rows):
 
 def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torch.Tensor],
               dir_off: torch.Tensor, dir_idx: torch.Tensor, crop: int, validate: bool = True,
-              pmap: Optional[torch.Tensor] = None, jt: int = 0) -> CorrelationResult:
+              pmap: Optional[torch.Tensor] = None, jt: int = 0,
+              tinv: Optional[torch.Tensor] = None) -> CorrelationResult:
     """FP32 v606 search.  With ``pmap`` the searched particles are
-    B = len(dir_off) - 1 entries each mapped to a row of ``spectra``."""
+    B = len(dir_off) - 1 entries each mapped to a row of ``spectra``;
+    ``tinv`` gives 1/norm per bank row for an unnormalised bank."""
     require_hip(bank, "bank")
     require_hip(spectra, "spectra")
     P, n = int(spectra.shape[1]), int(spectra.shape[2])
@@ -188,6 +194,10 @@ def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torc
         if total and (int(dir_idx.min()) < 0 or int(dir_idx.max()) >= int(bank.shape[0])):
             raise ValueError("dir_idx out of range of the template bank")
         _check_pmap(pmap, B, int(spectra.shape[0]))
+    if tinv is not None:
+        require_hip(tinv, "tinv")
+        if tinv.dtype != torch.float32 or tinv.numel() != int(bank.shape[0]):
+            raise ValueError("tinv must be float32 with one value per bank row")
     dev = spectra.device
     scores = torch.empty((total, P), dtype=torch.float32, device=dev)
     py = torch.empty((total, P), dtype=torch.int32, device=dev)
@@ -198,9 +208,9 @@ def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torc
     std = torch.empty((B,), dtype=torch.float64, device=dev)
     wsb = int(lib().gis_correlate_workspace(n))
     ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dev)
-    check(lib().gis_correlate(ptr(bank), ptr(spectra), ptr(inv_norm), ptr(dir_off), ptr(dir_idx), ptr(pmap), B,
-                              P, n, int(crop), int(jt), ptr(scores), ptr(py), ptr(px), ptr(best), ptr(peak),
-                              ptr(mean), ptr(std), ptr(ws), wsb, stream_of(dev)), "gis_correlate")
+    check(lib().gis_correlate(ptr(bank), ptr(spectra), ptr(inv_norm), ptr(tinv), ptr(dir_off), ptr(dir_idx),
+                              ptr(pmap), B, P, n, int(crop), int(jt), ptr(scores), ptr(py), ptr(px), ptr(best),
+                              ptr(peak), ptr(mean), ptr(std), ptr(ws), wsb, stream_of(dev)), "gis_correlate")
     return CorrelationResult(scores, py, px, best, peak, mean, std)
 
 

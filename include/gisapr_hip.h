@@ -74,9 +74,13 @@ size_t gis_build_templates_workspace(int N, int M);
  *   rts  [M][3][3] float32: Euler(rot, tilt, 0)^T per direction (v606:214-215)
  *   mask [N][N] float32 or NULL;  fsc [n][n/2+1] float32 or NULL (ones)
  *   bank [M][n][n/2+1] complex64 out;  norms [M] float32 out or NULL
+ *   normalize 1: bank rows unit norm (the reference's bank); 0: rows keep
+ *             their scale and norms (required) carries it, for callers that
+ *             fold 1/norm into a later step (gis_correlate tinv, the
+ *             coefficient gather's scale) and skip a pass over the bank
  */
 int gis_build_templates(const void* dft, int N, int n, const float* rts, int M, const float* mask,
-                        const float* fsc, void* bank, float* norms, void* ws, size_t ws_bytes,
+                        const float* fsc, void* bank, float* norms, int normalize, void* ws, size_t ws_bytes,
                         void* stream);
 
 /* Workspace bytes gis_prepare_particles needs for `particles` (= B) particles. */
@@ -120,6 +124,8 @@ size_t gis_correlate_workspace(int n);
  * correlate_templates_and_particle (v606:1418-1449) and the numerics of
  * make_result_records (v606:1452-1491), batched over B particles.  Particle
  * b is correlated against bank rows dir_idx[dir_off[b] .. dir_off[b+1]).
+ *   inv_norm [particles][P] 1/norm of each spectrum or NULL; tinv [bank rows]
+ *        1/norm of each template row or NULL (bank already unit norm)
  *   pmap [B] spectra row of searched particle b, or NULL (identity): one
  *        launch can search several candidates' banks (rows c*M + direction)
  *        against one resident particle stack (ZL:gisapr_batch_search.py:318-340)
@@ -130,8 +136,9 @@ size_t gis_correlate_workspace(int n);
  *   peak [B] float32, mean/std [B] float64 (population std)
  * C (--ccgCropSize) must be even and <= 16.
  */
-int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, const int* dir_off,
-                  const int* dir_idx, const int* pmap, int B, int P, int n, int C, int jt, float* scores,
+int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
+                  const int* dir_off, const int* dir_idx, const int* pmap, int B, int P, int n, int C, int jt,
+                  float* scores,
                   int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
                   size_t ws_bytes, void* stream);
 

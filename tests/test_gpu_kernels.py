@@ -93,6 +93,12 @@ def test_templates_match_oracle(hip_device, N, n):
     for d in range(len(rot)):
         e = rel_l2(got[d], want[d])
         assert e < 1e-4, (d, e)
+    # unnormalised rows + norms (the scale is folded into the search) describe the same bank
+    raw, norms = kernels.build_templates(dft, n, rts, torch.from_numpy(mask).to(dev),
+                                         torch.from_numpy(fscp).to(dev), normalize=False)
+    raw, norms = raw.cpu().numpy(), norms.cpu().numpy()
+    np.testing.assert_allclose(np.linalg.norm(raw.reshape(len(rot), -1), axis=1), norms, rtol=1e-5)
+    np.testing.assert_allclose(raw / norms[:, None, None], bank, rtol=1e-5, atol=1e-7)
 
 
 @pytest.mark.parametrize("N,n,with_ctf", [(32, 32, True), (64, 64, True), (64, 48, True), (64, 64, False)])
@@ -197,3 +203,27 @@ def test_end_to_end_matches_oracle(hip_device, name):
         assert abs(r["rawcc"] - w["rawcc"]) <= 1e-4 * abs(w["rawcc"]) + 1e-3
     s_want = osr.score_sum([w["rawcc"] for w in want])
     assert abs(float(res.score.cpu()) - s_want) <= 1e-4 * s_want
+
+
+def test_correlate_with_template_norms_equals_unit_bank(hip_device):
+    """gis_correlate on an unnormalised bank with tinv = 1/norm equals the
+    search on the unit-norm bank (scores to float32 rounding, same peaks)."""
+    import torch
+    from gisapr_amd import kernels
+    rng = np.random.default_rng(4)
+    n, C, M, B, P = 64, 10, 7, 5, 3
+    m = n // 2 + 1
+    d = hip_device
+    raw = (rng.standard_normal((M, n, m)) + 1j * rng.standard_normal((M, n, m))).astype(np.complex64) * 37.0
+    norms = np.linalg.norm(raw.reshape(M, -1), axis=1).astype(np.float32)
+    unit = (raw / norms[:, None, None]).astype(np.complex64)
+    spec = (rng.standard_normal((B, P, n, m)) + 1j * rng.standard_normal((B, P, n, m))).astype(np.complex64)
+    off = torch.tensor([0, 2, 2, 5, 6, 7], dtype=torch.int32, device=d)
+    idx = torch.tensor([0, 3, 1, 2, 6, 5, 4], dtype=torch.int32, device=d)
+    inv = torch.from_numpy((1.0 / np.linalg.norm(spec.reshape(B, P, -1), axis=2)).astype(np.float32)).to(d)
+    sp = torch.from_numpy(spec).to(d)
+    a = kernels.correlate(torch.from_numpy(unit).to(d), sp, inv, off, idx, C)
+    b = kernels.correlate(torch.from_numpy(raw).to(d), sp, inv, off, idx, C,
+                          tinv=torch.from_numpy(1.0 / norms).to(d))
+    np.testing.assert_allclose(b.scores.cpu().numpy(), a.scores.cpu().numpy(), rtol=2e-5, atol=1e-4)
+    np.testing.assert_array_equal(b.best.cpu().numpy(), a.best.cpu().numpy())
