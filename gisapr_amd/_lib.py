@@ -34,8 +34,8 @@ SIGNATURES = {
     "gis_score_sum_segments": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
     "gis_score_sum": (_i, [_vp, _vp, _i, _vp, _vp]),
     "gis_gather_coefficients": (_i, [_vp, _sz, _i, _vp, _i, _i, _vp, _vp, _vp]),
-    "gis_correlate_mixed": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _vp, _vp, _vp,
-                                 _vp, _vp, _vp, _vp, _vp]),
+    "gis_correlate_mixed": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _vp,
+                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 

@@ -92,7 +92,7 @@ class Autotuner:
         return dict(best)
 
 
-MIXED_VARIANTS = [dict(pt=5, wpb=4), dict(pt=5, wpb=2), dict(pt=5, wpb=1), dict(pt=3, wpb=4), dict(pt=1, wpb=4)]
+MIXED_VARIANTS = [dict(pt=5, wpb=4), dict(pt=5, wpb=2), dict(pt=3, wpb=4), dict(pt=1, wpb=4)]
 FP32_VARIANTS = [dict(jt=0), dict(jt=2), dict(jt=4), dict(jt=6)]
 
 
@@ -125,15 +125,45 @@ class SlotPlan:
     row_off_host: np.ndarray         # [slots+1] dir_idx offsets at slot boundaries
     n_pairs: int                     # (direction, psi) pairs searched per step
 
+    row_particle: np.ndarray = None  # [rows] particle of each dir_idx row
+    _orders: dict = None
+
     def prefix(self, k: int):
         """Operands of a search over the first k slots only (views, no copies)."""
         v = int(self.slot_off_host[k])
         r = int(self.row_off_host[k])
         return self.dir_off[:v + 1], self.dir_idx[:r], self.pmap[:v], self.slot_off[:k + 1]
 
+    particle_rank: np.ndarray = None  # [particles] processing rank of each particle
+
+    def particle_major(self, k: int) -> torch.Tensor:
+        """Processing order of the first k slots' rows: each particle's rows of all
+        slots adjacent (they share the particle's coefficients in cache), particles
+        in ``particle_rank`` order (neighbours in orientation share templates)."""
+        if self._orders is None:
+            self._orders = {}
+        if k not in self._orders:
+            r = int(self.row_off_host[k])
+            key = self.particle_rank[self.row_particle[:r]]
+            order = np.argsort(key, kind="stable").astype(np.int32)
+            self._orders[k] = torch.from_numpy(order).to(self.dir_idx.device)
+        return self._orders[k]
+
     @property
     def virtual(self) -> int:
         return int(self.slot_off_host[-1])
+
+
+def orientation_rank(dir_off: np.ndarray, dir_idx: np.ndarray) -> np.ndarray:
+    """Rank of each particle when sorted by its first local direction (particles
+    whose priors are close share most of their local directions)."""
+    off = np.asarray(dir_off, np.int64)
+    idx = np.asarray(dir_idx, np.int64)
+    first = np.where(off[1:] > off[:-1], idx[np.minimum(off[:-1], max(len(idx) - 1, 0))], -1)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty(len(first), np.int64)
+    rank[order] = np.arange(len(first))
+    return rank
 
 
 def build_slot_plan(ps: ParticleSet, M: int, slots: int, subsets: Optional[Sequence] = None,
@@ -161,6 +191,8 @@ def build_slot_plan(ps: ParticleSet, M: int, slots: int, subsets: Optional[Seque
         v_map.append(sub)
         s_off.append(len(sub))
     d_off = np.concatenate([[0], np.cumsum(np.concatenate(v_off[1:]) if len(v_off) > 1 else [])]).astype(np.int32)
+    row_particle = np.repeat(np.concatenate(v_map) if v_map else np.zeros(0, np.int64),
+                             np.concatenate(v_off[1:]) if len(v_off) > 1 else np.zeros(0, np.int64))
     d_idx = np.concatenate(v_idx).astype(np.int32) if v_idx else np.zeros(0, np.int32)
     pm = np.concatenate(v_map).astype(np.int32) if v_map else np.zeros(0, np.int32)
     so = np.cumsum(s_off).astype(np.int32)
@@ -168,7 +200,8 @@ def build_slot_plan(ps: ParticleSet, M: int, slots: int, subsets: Optional[Seque
     return SlotPlan(slots=slots, subsets=subs, dir_off=torch.from_numpy(d_off).to(dev),
                     dir_idx=torch.from_numpy(d_idx).to(dev), pmap=torch.from_numpy(pm).to(dev),
                     slot_off=torch.from_numpy(so).to(dev), slot_off_host=so, row_off_host=d_off[so].copy(),
-                    n_pairs=int(len(d_idx)) * ps.P)
+                    n_pairs=int(len(d_idx)) * ps.P, row_particle=row_particle,
+                    particle_rank=orientation_rank(off, idx))
 
 
 # ---------------------------------------------------------------------------
@@ -277,7 +310,8 @@ class CandidateBatch:
         if self.path == "mixed":
             return kernels.correlate_mixed(self.tk, ps.coeffs, ps.P, dir_off, dir_idx, eng.small_bhi,
                                            eng.small_blo, eng.small.half, validate=False, pmap=pmap,
-                                           pt=tile.get("pt", 0), wpb=tile.get("wpb", 0))
+                                           pt=tile.get("pt", 0), wpb=tile.get("wpb", 0), plast=eng.small_plast,
+                                           rows=self.plan.particle_major(self.slots if k is None else k))
         return kernels.correlate(self.bank, ps.spectra, ps.inv_norm, dir_off, dir_idx, eng.crop,
                                  validate=False, pmap=pmap, jt=tile.get("jt", 0), tinv=self.tinv)
 
@@ -474,7 +508,7 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
             kernels.gather_coefficients(bank, engine.small_kidx, engine.small.Kpad, tinv, out=tk[c * M:(c + 1) * M])
         corr = kernels.correlate_mixed(tk, ps.coeffs, ps.P, plan.dir_off, plan.dir_idx, engine.small_bhi,
                                        engine.small_blo, engine.small.half, validate=False, pmap=plan.pmap,
-                                       pt=tile.get("pt", 0), wpb=tile.get("wpb", 0))
+                                       pt=tile.get("pt", 0), wpb=tile.get("wpb", 0), plast=engine.small_plast)
     else:
         if ps.spectra is None:
             engine.prepare(ps)

@@ -61,20 +61,32 @@ __device__ __forceinline__ void split8(const float (&x)[8], bf16x8& hi, bf16x8& 
 // is the r-th entry of dir_idx (particle b with dir_off[b] <= r <
 // dir_off[b+1]); M-tile p holds the pairs (row, psi p0 + p).  The template
 // coefficients of a row are loaded once per k-step and multiplied by the PT
-// particle spectra, and every lag-table fragment feeds PT M-tiles.
-template <int PT, int WPB>
+// particle spectra, and every lag-table fragment feeds PT M-tiles.  The lag
+// table streams through LDS in chunks of KC k-steps shared by the WPB waves
+// of a block (register-staged, double-buffered).  NT lag tiles of 16 go to
+// the MFMAs; with VL the last lag (49 = 3 x 16 + 1) is a float32 dot product
+// on the VALU instead of a fourth, almost empty, tile.
+template <int PT, int WPB, int NT, bool VL>
 __global__ void __launch_bounds__(64 * WPB)
 mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__ Pk,
                        const int* __restrict__ dir_off, const int* __restrict__ dir_idx,
-                       const int* __restrict__ pmap, int B, int nrows,
-                       int P, int Kpad, const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo, int half,
+                       const int* __restrict__ pmap, const int* __restrict__ rows, int B, int nrows,
+                       int P, int Kpad, const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo,
+                       const float2* __restrict__ Plast, int half,
                        float* __restrict__ scores, int* __restrict__ peak_y, int* __restrict__ peak_x) {
-  const int lane = threadIdx.x & 63;
-  const int group = blockIdx.x * WPB + (threadIdx.x >> 6);
-  if (group * 16 >= nrows) return;  // whole wave
+  constexpr int KC = 4;               // k-steps per LDS chunk
+  constexpr int FRAG = NT * 2 * 64;   // lag-table fragments per k-step (hi and lo)
+  constexpr int NTH = 64 * WPB;
+  constexpr int SPT = (KC * FRAG + NTH - 1) / NTH;  // staged fragments per thread
+  __shared__ bf16x8 blds[2][KC * FRAG];
+  const int tid = threadIdx.x, lane = tid & 63;
+  int group = blockIdx.x * WPB + (tid >> 6);
+  const bool live = group * 16 < nrows;
+  if (!live) group = (nrows - 1) / 16;  // keeps the block's barriers uniform; writes nothing
   const int p0 = blockIdx.y * PT;
   const int i = lane & 15, g = lane >> 4;
-  const int r = min(group * 16 + i, nrows - 1);
+  const int rslot = min(group * 16 + i, nrows - 1);
+  const int r = rows ? rows[rslot] : rslot;  // processing order -> CSR row
   // particle of row r: last b with dir_off[b] <= r (empty particles share offsets)
   int lo = 0, hi = B;  // invariant dir_off[lo] <= r < dir_off[hi]
   while (hi - lo > 1) {
@@ -86,55 +98,106 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
   const float4* prow[PT];
 #pragma unroll
   for (int p = 0; p < PT; ++p) prow[p] = (const float4*)(Pk + ((size_t)pb * P + min(p0 + p, P - 1)) * Kpad);
+  const float4* plast = (const float4*)Plast;
   const int ksteps = Kpad / 16;
-  floatx4 acc[PT][kLagTiles];
+  const int nchunk = (ksteps + KC - 1) / KC;
+  floatx4 acc[PT][NT];
+  float accl[PT];
 #pragma unroll
-  for (int p = 0; p < PT; ++p)
+  for (int p = 0; p < PT; ++p) {
+    accl[p] = 0.f;
 #pragma unroll
-    for (int nt = 0; nt < kLagTiles; ++nt) acc[p][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int nt = 0; nt < NT; ++nt) acc[p][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+  }
 
-  for (int ks = 0; ks < ksteps; ++ks) {
-    const int q4 = ks * 8 + 2 * g;  // float4 index of this lane's 4 coefficients
-    const float4 t0 = trow[q4], t1 = trow[q4 + 1];
-    const float tr[4] = {t0.x, t0.z, t1.x, t1.z}, ti[4] = {t0.y, t0.w, t1.y, t1.w};
-    bf16x8 ahi[PT], alo[PT];
+  bf16x8 stg[SPT];
+  auto fetch = [&](int ch) {
 #pragma unroll
-    for (int p = 0; p < PT; ++p) {
-      const float4 u0 = prow[p][q4], u1 = prow[p][q4 + 1];
-      const float pr[4] = {u0.x, u0.z, u1.x, u1.z}, pim[4] = {u0.y, u0.w, u1.y, u1.w};
-      float x[8];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        x[2 * c] = tr[c] * pr[c] - ti[c] * pim[c];
-        x[2 * c + 1] = tr[c] * pim[c] + ti[c] * pr[c];
-      }
-      split8(x, ahi[p], alo[p]);
+    for (int s = 0; s < SPT; ++s) {
+      const int e = tid + s * NTH;
+      const int kk = e / FRAG, f = e - kk * FRAG;
+      const int ks = ch * KC + kk;
+      const int nt = f >> 7, h = (f >> 6) & 1, ln = f & 63;
+      bf16x8 v = {};
+      if (e < KC * FRAG && ks < ksteps) v = (h ? Blo : Bhi)[((size_t)nt * ksteps + ks) * 64 + ln];
+      stg[s] = v;
     }
+  };
+  auto store = [&](int buf) {
 #pragma unroll
-    for (int nt = 0; nt < kLagTiles; ++nt) {
-      const size_t bi = ((size_t)nt * ksteps + ks) * 64 + lane;
-      const bf16x8 bh = Bhi[bi], bl = Blo[bi];
+    for (int s = 0; s < SPT; ++s) {
+      const int e = tid + s * NTH;
+      if (e < KC * FRAG) blds[buf][e] = stg[s];
+    }
+  };
+  fetch(0);
+  store(0);
+  __syncthreads();
+
+  for (int ch = 0; ch < nchunk; ++ch) {
+    const int cur = ch & 1;
+    if (ch + 1 < nchunk) fetch(ch + 1);  // in flight during this chunk's MFMAs
+    const int kend = min(KC, ksteps - ch * KC);
+    for (int kk = 0; kk < kend; ++kk) {
+      const int ks = ch * KC + kk;
+      const int q4 = ks * 8 + 2 * g;  // float4 index of this lane's 4 coefficients
+      const float4 t0 = trow[q4], t1 = trow[q4 + 1];
+      float4 uc[PT][2];
+#pragma unroll
+      for (int p = 0; p < PT; ++p) { uc[p][0] = prow[p][q4]; uc[p][1] = prow[p][q4 + 1]; }
+      float4 f0 = {}, f1 = {};
+      if (VL) { f0 = plast[q4]; f1 = plast[q4 + 1]; }
+      const float tr[4] = {t0.x, t0.z, t1.x, t1.z}, ti[4] = {t0.y, t0.w, t1.y, t1.w};
+      bf16x8 ahi[PT], alo[PT];
 #pragma unroll
       for (int p = 0; p < PT; ++p) {
-        acc[p][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[p], bh, acc[p][nt], 0, 0, 0);
-        acc[p][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[p], bl, acc[p][nt], 0, 0, 0);
-        acc[p][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[p], bh, acc[p][nt], 0, 0, 0);
+        const float4 u0 = uc[p][0], u1 = uc[p][1];
+        const float pr[4] = {u0.x, u0.z, u1.x, u1.z}, pim[4] = {u0.y, u0.w, u1.y, u1.w};
+        float x[8];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          x[2 * c] = tr[c] * pr[c] - ti[c] * pim[c];
+          x[2 * c + 1] = tr[c] * pim[c] + ti[c] * pr[c];
+        }
+        if (VL)
+          accl[p] += ((x[0] * f0.x + x[1] * f0.y) + (x[2] * f0.z + x[3] * f0.w)) +
+                     ((x[4] * f1.x + x[5] * f1.y) + (x[6] * f1.z + x[7] * f1.w));
+        split8(x, ahi[p], alo[p]);
+      }
+      const bf16x8* fr = blds[cur] + kk * FRAG + lane;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const bf16x8 bh = fr[(nt * 2) * 64], bl = fr[(nt * 2 + 1) * 64];
+#pragma unroll
+        for (int p = 0; p < PT; ++p) {
+          acc[p][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[p], bh, acc[p][nt], 0, 0, 0);
+          acc[p][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[p], bl, acc[p][nt], 0, 0, 0);
+          acc[p][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[p], bh, acc[p][nt], 0, 0, 0);
+        }
       }
     }
+    if (ch + 1 < nchunk) store(cur ^ 1);
+    __syncthreads();
   }
 
   // epilogue: lane (g, i) holds S[row 4g+v][lag column nt*16+i] of each psi tile; first maximum over lags
   const int side = 2 * half + 1, nlag = side * side;
 #pragma unroll
   for (int p = 0; p < PT; ++p) {
+    float tl = 0.f;
+    if (VL) {  // row i's float32 last-lag sum, over the 4 k-groups of lanes i, i+16, i+32, i+48
+      tl = accl[p];
+      tl += __shfl_xor(tl, 16, 64);
+      tl += __shfl_xor(tl, 32, 64);
+    }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       float bv = -INFINITY;
       int bs = 0x7fffffff;
 #pragma unroll
-      for (int nt = 0; nt < kLagTiles; ++nt) {
+      for (int nt = 0; nt < NT; ++nt) {
         const int s = nt * 16 + i;
-        if (s < nlag) {
+        if (s < (VL ? nlag - 1 : nlag)) {
           const float val = acc[p][nt][v] * 1024.0f;
           if (val > bv) { bv = val; bs = s; }  // s increases with nt: strict > keeps the first
         }
@@ -145,8 +208,13 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
         const int os = __shfl_xor(bs, o, 16);
         if (ov > bv || (ov == bv && os < bs)) { bv = ov; bs = os; }
       }
-      const int row = group * 16 + 4 * g + v;
-      if (i == 0 && row < nrows && p0 + p < P) {
+      const int slot = group * 16 + 4 * g + v;
+      const int row = slot < nrows ? (rows ? rows[slot] : slot) : nrows;
+      if (VL) {
+        const float vl = __shfl(tl, 4 * g + v, 64) * 1024.0f;  // lane 4g+v holds that row's sum
+        if (vl > bv) { bv = vl; bs = nlag - 1; }  // the last lag: strict > keeps earlier ties
+      }
+      if (live && i == 0 && row < nrows && p0 + p < P) {
         const size_t out = (size_t)row * P + p0 + p;
         scores[out] = bv;
         peak_y[out] = bs / side;  // crop index: lag + half
@@ -215,14 +283,20 @@ GIS_API int gis_gather_coefficients(const void* src, size_t plane, int rows, con
 // over lags [-half, half]^2 (half <= 3) and a truncated coefficient set.
 //   Tk [bank rows][Kpad], Pk [particles*P][Kpad] complex64 (unit-norm, unshifted)
 //   pmap [B] particle row of each searched particle in Pk, or NULL (identity)
+//   rows [nrows] processing order of the dir_idx rows (a permutation), or NULL:
+//        rows of the same particle processed together share its Pk rows in
+//        cache (several candidates' rows of one particle in a batched launch)
 //   nrows = dir_off[B] = number of dir_idx entries (passed so no device read is needed)
-//   pt = psi values per wave (1..5, 0 = min(P, 5)); wpb = waves per block (1, 2, 4; 0 = 4)
+//   pt = psi values per wave (1..5, 0 = min(P, 5)); wpb = waves per block (2 or 4; 0 = 4)
 //   Bhi, Blo: split lag table, [4][Kpad/16][64] x 8 bf16 (host layout)
+//   Plast: float2 [Kpad] (Re Phi, -Im Phi) of the last lag, used when the lag
+//          count is 16 t + 1 (t tiles on MFMA, the last lag on the VALU); else NULL
 //   scores / peak_y / peak_x [dir_off[B]][P] (peak indices are lag + half)
 //   best / peak / mean / std [B] as gis_correlate
 GIS_API int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx,
-                                const int* pmap, int B, int P, int nrows, int Kpad, const void* Bhi, const void* Blo,
-                                int half, int pt, int wpb, float* scores, int* peak_y, int* peak_x, int* best,
+                                const int* pmap, const int* rows, int B, int P, int nrows, int Kpad, const void* Bhi, const void* Blo,
+                                const void* Plast, int half, int pt, int wpb, float* scores, int* peak_y,
+                                int* peak_x, int* best,
                                 float* peak, double* mean, double* stdv, void* stream) {
   GIS_REQUIRE(B >= 0 && P >= 1 && nrows >= 0, "gis_correlate_mixed: B=%d P=%d nrows=%d", B, P, nrows);
   GIS_REQUIRE(Kpad % 16 == 0 && Kpad >= 16, "gis_correlate_mixed: Kpad %d must be a positive multiple of 16", Kpad);
@@ -236,32 +310,42 @@ GIS_API int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_o
               "gis_correlate_mixed: null pointer");
   hipStream_t st = (hipStream_t)stream;
   GIS_REQUIRE(pt >= 0 && pt <= 5, "gis_correlate_mixed: pt %d not in [0, 5]", pt);
-  GIS_REQUIRE(wpb == 0 || wpb == 1 || wpb == 2 || wpb == 4, "gis_correlate_mixed: wpb %d not in {0, 1, 2, 4}", wpb);
+  const int nlag = (2 * half + 1) * (2 * half + 1);
+  const bool vl = nlag % 16 == 1 && nlag > 1;
+  const int nt = vl ? nlag / 16 : (nlag + 15) / 16;
+  GIS_REQUIRE(!vl || Plast, "gis_correlate_mixed: %d lags need the last-lag table", nlag);
   if (nrows > 0) {
     if (pt == 0) pt = std::min(P, 5);
     if (wpb == 0) wpb = 4;
+    GIS_REQUIRE(wpb == 2 || wpb == 4, "gis_correlate_mixed: wpb %d not in {0, 2, 4}", wpb);
     const int rows_per_block = 16 * wpb;
     const dim3 grid((unsigned)((nrows + rows_per_block - 1) / rows_per_block), (unsigned)((P + pt - 1) / pt));
-#define GIS_MIXED_LAUNCH(PTV, WV)                                                                              \
-  hipLaunchKernelGGL((mixed_correlate_kernel<PTV, WV>), grid, dim3(64 * WV), 0, st, (const float2*)Tk,         \
-                     (const float2*)Pk, dir_off, dir_idx, pmap, B, nrows, P, Kpad, (const bf16x8*)Bhi,          \
-                     (const bf16x8*)Blo, half, scores, peak_y, peak_x)
-#define GIS_MIXED_PT(WV)                      \
-  switch (pt) {                               \
-    case 1: GIS_MIXED_LAUNCH(1, WV); break;   \
-    case 2: GIS_MIXED_LAUNCH(2, WV); break;   \
-    case 3: GIS_MIXED_LAUNCH(3, WV); break;   \
-    case 4: GIS_MIXED_LAUNCH(4, WV); break;   \
-    default: GIS_MIXED_LAUNCH(5, WV); break;  \
+    const float2* pl = (const float2*)Plast;
+#define GIS_MIXED_LAUNCH(PTV, WV, NTV, VLV)                                                                   \
+  hipLaunchKernelGGL((mixed_correlate_kernel<PTV, WV, NTV, VLV>), grid, dim3(64 * WV), 0, st,                  \
+                     (const float2*)Tk, (const float2*)Pk, dir_off, dir_idx, pmap, rows, B, nrows, P, Kpad,            \
+                     (const bf16x8*)Bhi, (const bf16x8*)Blo, pl, half, scores, peak_y, peak_x)
+#define GIS_MIXED_NT(PTV, WV)                                 \
+  if (vl && nt == 3) GIS_MIXED_LAUNCH(PTV, WV, 3, true);      \
+  else if (nt == 1) GIS_MIXED_LAUNCH(PTV, WV, 1, false);      \
+  else if (nt == 2) GIS_MIXED_LAUNCH(PTV, WV, 2, false);      \
+  else if (nt == 3) GIS_MIXED_LAUNCH(PTV, WV, 3, false);      \
+  else GIS_MIXED_LAUNCH(PTV, WV, 4, false);
+#define GIS_MIXED_PT(WV)                     \
+  switch (pt) {                              \
+    case 1: { GIS_MIXED_NT(1, WV) } break;   \
+    case 2: { GIS_MIXED_NT(2, WV) } break;   \
+    case 3: { GIS_MIXED_NT(3, WV) } break;   \
+    case 4: { GIS_MIXED_NT(4, WV) } break;   \
+    default: { GIS_MIXED_NT(5, WV) } break;  \
   }
-    if (wpb == 1) {
-      GIS_MIXED_PT(1)
-    } else if (wpb == 2) {
+    if (wpb == 2) {
       GIS_MIXED_PT(2)
     } else {
       GIS_MIXED_PT(4)
     }
 #undef GIS_MIXED_PT
+#undef GIS_MIXED_NT
 #undef GIS_MIXED_LAUNCH
     GIS_LAUNCH_CHECK("gis_correlate_mixed");
   }

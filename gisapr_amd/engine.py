@@ -36,6 +36,7 @@ class ParticleSet:
     spectra: Optional[torch.Tensor] = None   # [B, P, n, n/2+1] complex64
     inv_norm: Optional[torch.Tensor] = None  # [B, P] float32
     coeffs: Optional[torch.Tensor] = None    # [B*P, Kpad] complex64, unit norm (small-shift search)
+    row_order: Optional[torch.Tensor] = None # [sum M_i] int32 processing order of the small-shift search
 
     @property
     def B(self) -> int:
@@ -187,6 +188,7 @@ class SearchEngine:
         self.small_kidx = torch.from_numpy(tab.kidx).to(dev)
         self.small_bhi = torch.from_numpy(tab.b_hi.view(np.int16)).to(dev)
         self.small_blo = torch.from_numpy(tab.b_lo.view(np.int16)).to(dev)
+        self.small_plast = torch.from_numpy(tab.phi_last).to(dev)
         return tab
 
     def _require_small(self):
@@ -216,8 +218,16 @@ class SearchEngine:
         if int(bank.shape[0]) != len(self.dir_rot):
             raise ValueError("template bank does not cover the engine's direction table")
         tk = self.template_coefficients(bank, tinv)
+        if getattr(ps, "row_order", None) is None:
+            from .batch import orientation_rank
+            off = ps.dir_off.cpu().numpy()
+            rank = orientation_rank(off, ps.dir_idx.cpu().numpy())
+            row_particle = np.repeat(np.arange(ps.B), np.diff(off))
+            ps.row_order = torch.from_numpy(np.argsort(rank[row_particle], kind="stable").astype(np.int32)).to(
+                self.device)
         corr = kernels.correlate_mixed(tk, ps.coeffs, ps.P, ps.dir_off, ps.dir_idx, self.small_bhi,
-                                       self.small_blo, self.small.half, validate=False)
+                                       self.small_blo, self.small.half, validate=False, plast=self.small_plast,
+                                       rows=ps.row_order)
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.small.half)
 
     def score_volume(self, volume: torch.Tensor, ps: ParticleSet) -> SearchResult:

@@ -265,6 +265,11 @@ class SmallShiftTables:
         breal[0:2 * self.K:2, :S] = phi.real
         breal[1:2 * self.K:2, :S] = -phi.imag
         self.phi = phi
+        # last lag as float32 (Re Phi, -Im Phi) pairs: 49 = 3 x 16 + 1 lags run 3
+        # tiles on the MFMAs and this one on the VALU
+        self.phi_last = np.zeros((self.Kpad, 2), np.float32)
+        self.phi_last[:self.K, 0] = phi[:, S - 1].real
+        self.phi_last[:self.K, 1] = -phi[:, S - 1].imag
         hi, lo = _split_bf16(breal)
         self.b_hi = _mfma_b_layout(hi, self.Kpad, self.LAG_TILES)
         self.b_lo = _mfma_b_layout(lo, self.Kpad, self.LAG_TILES)

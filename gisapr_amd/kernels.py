@@ -266,7 +266,8 @@ def gather_coefficients(src: torch.Tensor, kidx: torch.Tensor, kpad: int,
 
 def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.Tensor, dir_idx: torch.Tensor,
                     b_hi: torch.Tensor, b_lo: torch.Tensor, half: int, validate: bool = True,
-                    pmap: Optional[torch.Tensor] = None, pt: int = 0, wpb: int = 0) -> CorrelationResult:
+                    pmap: Optional[torch.Tensor] = None, pt: int = 0, wpb: int = 0,
+                    plast: Optional[torch.Tensor] = None, rows: Optional[torch.Tensor] = None) -> CorrelationResult:
     """Split-bf16 small-shift search: best lag over [-half, half]^2 of every
     (direction, psi) pair; peak_y / peak_x are lag + half.  ``validate``
     checks dir_off / dir_idx against the operands on the host."""
@@ -283,6 +284,13 @@ def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.T
         raise ValueError("pk must hold P rows per particle")
     if tuple(b_hi.shape) != tuple(b_lo.shape) or b_hi.numel() != 4 * (kpad // 16) * 64 * 8:
         raise ValueError("lag table does not match Kpad")
+    nlag = (2 * int(half) + 1) ** 2
+    if nlag % 16 == 1 and nlag > 1:
+        if plast is None:
+            raise ValueError(f"{nlag} lags need the last-lag table (plast)")
+        require_hip(plast, "plast")
+        if plast.dtype != torch.float32 or plast.numel() != 2 * kpad:
+            raise ValueError("plast must be float32 [Kpad, 2]")
     if dir_off.dtype != torch.int32 or dir_idx.dtype != torch.int32:
         raise ValueError("dir_off / dir_idx must be int32")
     total = int(dir_idx.shape[0])
@@ -293,6 +301,13 @@ def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.T
         if total and (int(dir_idx.min()) < 0 or int(dir_idx.max()) >= int(tk.shape[0])):
             raise ValueError("dir_idx out of range of the template coefficients")
         _check_pmap(pmap, B, int(pk.shape[0]) // P)
+        if rows is not None and not torch.equal(torch.sort(rows.long()).values,
+                                                torch.arange(total, device=rows.device)):
+            raise ValueError("rows must be a permutation of the dir_idx rows")
+    if rows is not None:
+        require_hip(rows, "rows")
+        if rows.dtype != torch.int32 or rows.numel() != total:
+            raise ValueError("rows must be int32 [len(dir_idx)]")
     dev = pk.device
     scores = torch.empty((total, P), dtype=torch.float32, device=dev)
     py = torch.empty((total, P), dtype=torch.int32, device=dev)
@@ -301,8 +316,10 @@ def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.T
     peak = torch.empty((B,), dtype=torch.float32, device=dev)
     mean = torch.empty((B,), dtype=torch.float64, device=dev)
     std = torch.empty((B,), dtype=torch.float64, device=dev)
-    check(lib().gis_correlate_mixed(ptr(tk), ptr(pk), ptr(dir_off), ptr(dir_idx), ptr(pmap), B, int(P), total,
-                                    kpad, ptr(b_hi), ptr(b_lo), int(half), int(pt), int(wpb), ptr(scores), ptr(py),
-                                    ptr(px), ptr(best), ptr(peak), ptr(mean), ptr(std), stream_of(dev)),
+    check(lib().gis_correlate_mixed(ptr(tk), ptr(pk), ptr(dir_off), ptr(dir_idx), ptr(pmap), ptr(rows), B, int(P),
+                                    total,
+                                    kpad, ptr(b_hi), ptr(b_lo), ptr(plast), int(half), int(pt), int(wpb),
+                                    ptr(scores), ptr(py), ptr(px), ptr(best), ptr(peak), ptr(mean), ptr(std),
+                                    stream_of(dev)),
           "gis_correlate_mixed")
     return CorrelationResult(scores, py, px, best, peak, mean, std)

@@ -176,16 +176,22 @@ int gis_gather_coefficients(const void* src, size_t plane, int rows, const int* 
  * coefficients on split-bf16 MFMA (hi*hi + hi*lo + lo*hi, FP32 accumulation).
  *   Tk [bank rows][Kpad], Pk [particles*P][Kpad] complex64 (unit norm over the full plane)
  *   pmap [B] Pk particle row of searched particle b, or NULL (identity)
+ *   rows [nrows] processing order of the dir_idx rows (a permutation) or NULL;
+ *        grouping one particle's rows of several candidates lets them share
+ *        its Pk rows in cache
  *   nrows = dir_off[B] (the number of dir_idx entries)
  *   Bhi / Blo [4][Kpad/16][64][8] bf16: split lag table in MFMA B-operand order
- *   pt  psi values per wave (1..5, 0 = min(P, 5)); wpb waves per block (1, 2, 4; 0 = 4)
+ *   Plast [Kpad] float2 (Re Phi, -Im Phi) of the last lag: when the lag count
+ *         is 16 t + 1 (49 for half = 3) t tiles run on MFMA and the last lag
+ *         as a float32 dot product; NULL otherwise
+ *   pt  psi values per wave (1..5, 0 = min(P, 5)); wpb waves per block (2 or 4; 0 = 4)
  *   scores/peak_y/peak_x [dir_off[B]][P] (peak indices are lag + half),
  *   best/peak/mean/std [B] as gis_correlate (FP64 mean/std).
  */
 int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx, const int* pmap,
-                        int B, int P, int nrows, int Kpad, const void* Bhi, const void* Blo, int half, int pt,
-                        int wpb, float* scores, int* peak_y, int* peak_x, int* best, float* peak, double* mean,
-                        double* stdv, void* stream);
+                        const int* rows, int B, int P, int nrows, int Kpad, const void* Bhi, const void* Blo, const void* Plast,
+                        int half, int pt, int wpb, float* scores, int* peak_y, int* peak_x, int* best, float* peak,
+                        double* mean, double* stdv, void* stream);
 
 #ifdef __cplusplus
 }

@@ -139,7 +139,17 @@ def test_mixed_correlate_random(hip_device, n, radius, half):
     corr = kernels.correlate_mixed(padded(T), padded(Pk), P, torch.from_numpy(off).to(d),
                                    torch.from_numpy(idx).to(d),
                                    torch.from_numpy(tab.b_hi.view(np.int16)).to(d),
-                                   torch.from_numpy(tab.b_lo.view(np.int16)).to(d), half)
+                                   torch.from_numpy(tab.b_lo.view(np.int16)).to(d), half,
+                                   plast=torch.from_numpy(tab.phi_last).to(d))
+    # any processing order of the rows gives bit-identical results
+    perm = torch.from_numpy(rng.permutation(len(idx)).astype(np.int32)).to(d)
+    corr2 = kernels.correlate_mixed(padded(T), padded(Pk), P, torch.from_numpy(off).to(d),
+                                    torch.from_numpy(idx).to(d),
+                                    torch.from_numpy(tab.b_hi.view(np.int16)).to(d),
+                                    torch.from_numpy(tab.b_lo.view(np.int16)).to(d), half,
+                                    plast=torch.from_numpy(tab.phi_last).to(d), rows=perm)
+    for a, b in zip(corr.__dict__.values(), corr2.__dict__.values()):
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
     torch.cuda.synchronize()
     phi = opr.phi_table(n, tab.kidx.astype(np.int64), half)
     T32 = T.astype(np.complex64).astype(np.complex128)
@@ -160,12 +170,15 @@ def test_mixed_correlate_rejects_bad_arguments(hip_device):
     bl = torch.from_numpy(tab.b_lo.view(np.int16)).to(d)
     pk = torch.zeros((2, tab.Kpad), dtype=torch.complex64, device=d)
     with pytest.raises(ValueError):
+        kernels.correlate_mixed(tk, pk, 2, off, idx, bh, bl, 3)           # 49 lags without the last-lag table
+    pl = torch.from_numpy(tab.phi_last).to(d)
+    with pytest.raises(ValueError):
         kernels.correlate_mixed(tk, pk, 2, off, idx, bh, bl, 4)   # 81 lags > 64
     with pytest.raises(ValueError):
-        kernels.correlate_mixed(tk, pk[:1], 2, off, idx, bh, bl, 3)
+        kernels.correlate_mixed(tk, pk[:1], 2, off, idx, bh, bl, 3, plast=pl)
     with pytest.raises(ValueError):
         bad = torch.tensor([0, 4], dtype=torch.int32, device=d)
-        kernels.correlate_mixed(tk, pk, 2, off, bad, bh, bl, 3)           # direction 4 of a 4-row bank
+        kernels.correlate_mixed(tk, pk, 2, off, bad, bh, bl, 3, plast=pl)  # direction 4 of a 4-row bank
 
 
 def _oracle_inputs(data, wl, tab):
