@@ -22,6 +22,11 @@ plain fp32 leg, top-1 agreement between the mixed and fp32 searches, and the
 mixed search's error against the same functional in float64 on a
 256-particle subset are reported beside it.
 
+Batched PSO leg (``--pso-iters``, 0 to skip): box-192 workload (20,000-atom
+model, moving chain 4,500, 20,000 particles at 1.3 A/px), a swarm of 64
+candidates x 20 iterations with seeded poses, each member scoring its own
+2,000-20,000-particle subset, one graph-captured batched step per iteration.
+
 Serving leg (``--serve-requests``, 0 to skip): three resident datasets (box
 128 / 2,000, box 192 / 10,000, box 256 / 20,000 particles, 3,000-atom moving
 chain), a seeded Poisson stream of requests at ``--serve-rate``/s with 1-64
@@ -164,6 +169,61 @@ def make_engine(wl, data, dev, resolution):
     return eng
 
 
+def pso_leg(args, dev, log):
+    """Batched PSO step (box192 workload): a swarm of ``--pso-slots`` candidates x
+    ``--pso-iters`` iterations, poses from a seeded normal distribution (rotation
+    capped at 5 deg, translation at 2 A), each swarm member scoring its own
+    seeded particle subset of 2,000-20,000 particles; one graph-captured
+    batched step per iteration on the mixed path."""
+    import torch
+    from gisapr_amd import synthetic
+    from gisapr_amd.batch import Autotuner, CandidateBatch, RigidBody
+    t0 = time.perf_counter()
+    wl = synthetic.WORKLOADS["box192"]
+    data = synthetic.make_workload("box192", n_particles=args.pso_particles, device=dev)
+    eng = make_engine(wl, data, dev, args.resolution)
+    dfo = np.stack([data["dfu"], data["dfv"], data["dfa"]], 1)
+    ps = eng.add_particles(data["images"], data["rot"], data["tilt"], data["psi"], defocus=dfo, prepare=False)
+    del data["images"]
+    eng.prepare_small_shift(ps, drop_spectra=True)
+    rng = np.random.default_rng(synthetic.SEED_CAND + 3)
+    S = args.pso_slots
+    lo_n, hi_n = min(2000, ps.B), ps.B
+    subsets = [np.sort(rng.choice(ps.B, int(rng.integers(lo_n, hi_n + 1)), replace=False)) for _ in range(S)]
+    body = RigidBody.from_arrays(data["coords"], data["numbers"], data["chains"], wl.moving_chain)
+    batch = CandidateBatch(eng, ps, body, wl.res, S, path="mixed", subsets=subsets, autotuner=Autotuner())
+
+    def swarm():
+        rot = rng.normal(0.0, 2.0, (S, 3))
+        nr = np.linalg.norm(rot, axis=1, keepdims=True)
+        rot = np.where(nr > 5.0, rot * 5.0 / np.maximum(nr, 1e-12), rot)
+        tr = rng.normal(0.0, 1.0, (S, 3))
+        nt = np.linalg.norm(tr, axis=1, keepdims=True)
+        tr = np.where(nt > 2.0, tr * 2.0 / np.maximum(nt, 1e-12), tr)
+        return np.concatenate([rot, tr], 1)
+
+    first = swarm()
+    batch.tune(first)
+    batch.capture(first)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    log("pso setup", setup)
+    best = -np.inf
+    t1 = time.perf_counter()
+    for it in range(args.pso_iters):
+        scores = batch.evaluate(swarm())
+        best = max(best, float(scores.max()))
+    dt = time.perf_counter() - t1
+    pairs = batch.plan.n_pairs * args.pso_iters
+    return {"workload": f"box192: {sum(wl.chain_sizes)}-atom model (moving chain {wl.chain_sizes[-1]}), "
+                        f"{ps.B} particles 192x192 at {wl.apix} A/px, {len(eng.dir_rot)} directions, {ps.P} psi; "
+                        f"{S} candidates x {args.pso_iters} iterations, subsets {lo_n}-{hi_n} particles",
+            "scores_per_s": pairs / dt, "candidates_per_s": S * args.pso_iters / dt,
+            "ms_per_iteration": dt * 1000.0 / args.pso_iters, "pairs_per_iteration": batch.plan.n_pairs,
+            "mean_subset": float(np.mean([len(x) for x in subsets])), "tile": dict(batch.tile), "graph": True,
+            "path": "mixed", "setup_s": setup, "best_score": best}
+
+
 def serve_leg(args, dev, log):
     """Serving leg: three resident datasets, seeded Poisson request stream."""
     import torch
@@ -211,6 +271,9 @@ def main() -> int:
     ap.add_argument("--precision-subset", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pso-iters", type=int, default=20, help="batched PSO leg iterations (0 to skip)")
+    ap.add_argument("--pso-slots", type=int, default=64)
+    ap.add_argument("--pso-particles", type=int, default=20000)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-slots", type=int, default=16)
@@ -406,6 +469,9 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(eng, ps, bank, args.cpu_sample)
     del ps, eng, bank, results
     torch.cuda.empty_cache()
+    if args.pso_iters > 0:
+        out["pso"] = pso_leg(args, dev, log)
+        torch.cuda.empty_cache()
     if args.serve_requests > 0:
         out["serving"] = serve_leg(args, dev, log)
         out["p50_ms"] = out["serving"]["p50_ms"]
