@@ -79,6 +79,59 @@ __device__ __forceinline__ void fill_twiddles(float2* tw, int L, int tid, int nt
 // __syncthreads() before the call (data ready); the function ends with a
 // barrier.  Forward: exp(-2 pi i / L); inverse: exp(+...), unscaled.
 // ---------------------------------------------------------------------------
+// In-register R-point DFT (R in {2, 3, 4, 8}); forward e^{-2 pi i/R}, inverse e^{+...}.
+template <int R, bool INV>
+__device__ __forceinline__ void dft_radix(float2 (&v)[R]) {
+  if constexpr (R == 2) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 8) {
+    // two 4-point DFTs (even / odd inputs) and the twiddles w^m, w = e^{-+2 pi i / 8}
+    float2 a = cadd(v[0], v[4]), b = csub(v[0], v[4]);
+    float2 c = cadd(v[2], v[6]), d = mul_mi<INV>(csub(v[2], v[6]));
+    const float2 A0 = cadd(a, c), A2 = csub(a, c), A1 = cadd(b, d), A3 = csub(b, d);
+    a = cadd(v[1], v[5]); b = csub(v[1], v[5]);
+    c = cadd(v[3], v[7]); d = mul_mi<INV>(csub(v[3], v[7]));
+    const float2 B0 = cadd(a, c), B2 = csub(a, c), B1 = cadd(b, d), B3 = csub(b, d);
+    constexpr float rh = 0.70710678118654752440f;
+    const float2 t1 = INV ? make_float2((B1.x - B1.y) * rh, (B1.x + B1.y) * rh)
+                          : make_float2((B1.x + B1.y) * rh, (B1.y - B1.x) * rh);
+    const float2 t2 = mul_mi<INV>(B2);
+    const float2 t3 = INV ? make_float2((-B3.x - B3.y) * rh, (B3.x - B3.y) * rh)
+                          : make_float2((B3.y - B3.x) * rh, (-B3.x - B3.y) * rh);
+    v[0] = cadd(A0, B0);
+    v[4] = csub(A0, B0);
+    v[1] = cadd(A1, t1);
+    v[5] = csub(A1, t1);
+    v[2] = cadd(A2, t2);
+    v[6] = csub(A2, t2);
+    v[3] = cadd(A3, t3);
+    v[7] = csub(A3, t3);
+  } else if constexpr (R == 4) {
+    float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    float2 c = cadd(v[1], v[3]), d = mul_mi<INV>(csub(v[1], v[3]));
+    v[0] = cadd(a, c);
+    v[2] = csub(a, c);
+    v[1] = cadd(b, d);
+    v[3] = csub(b, d);
+  } else if constexpr (R == 3) {
+    const float c1 = -0.5f;
+    const float s1 = INV ? 0.86602540378443864676f : -0.86602540378443864676f;
+    float2 a = v[0], b = v[1], c = v[2];
+    float2 t1 = cadd(b, c);
+    float2 t2 = make_float2(a.x + c1 * t1.x, a.y + c1 * t1.y);
+    float2 t3 = csub(b, c);  // (b - c) * i*s1
+    float2 t4 = make_float2(-s1 * t3.y, s1 * t3.x);
+    v[0] = cadd(a, t1);
+    v[1] = cadd(t2, t4);
+    v[2] = csub(t2, t4);
+  }
+}
+
+// One Stockham radix-R stage: butterfly j (of L/R) reads src[j + r*L/R],
+// applies the twiddles w^(r*k), k = j mod Ns, and writes
+// dst[(j/Ns)*Ns*R + k + r*Ns].
 template <int L, int R, bool INV>
 __device__ __forceinline__ void stockham_stage(const float2* __restrict__ src, float2* __restrict__ dst,
                                                int Ns, int nseq, int sp, const float2* __restrict__ tw,
@@ -95,57 +148,14 @@ __device__ __forceinline__ void stockham_stage(const float2* __restrict__ src, f
     float2 v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = S[j + r * LR];
+    if (Ns > 1)  // the first stage's twiddles are all 1
 #pragma unroll
-    for (int r = 1; r < R; ++r) {
-      float2 w = tw[k * r * step];
-      if (INV) w.y = -w.y;
-      v[r] = cmul(v[r], w);
-    }
-    if constexpr (R == 2) {
-      float2 a = v[0], b = v[1];
-      v[0] = cadd(a, b);
-      v[1] = csub(a, b);
-    } else if constexpr (R == 8) {
-      // two 4-point DFTs (even / odd inputs) and the twiddles w^m, w = e^{-+2 pi i / 8}
-      float2 a = cadd(v[0], v[4]), b = csub(v[0], v[4]);
-      float2 c = cadd(v[2], v[6]), d = mul_mi<INV>(csub(v[2], v[6]));
-      const float2 A0 = cadd(a, c), A2 = csub(a, c), A1 = cadd(b, d), A3 = csub(b, d);
-      a = cadd(v[1], v[5]); b = csub(v[1], v[5]);
-      c = cadd(v[3], v[7]); d = mul_mi<INV>(csub(v[3], v[7]));
-      const float2 B0 = cadd(a, c), B2 = csub(a, c), B1 = cadd(b, d), B3 = csub(b, d);
-      constexpr float rh = 0.70710678118654752440f;
-      const float2 t1 = INV ? make_float2((B1.x - B1.y) * rh, (B1.x + B1.y) * rh)
-                            : make_float2((B1.x + B1.y) * rh, (B1.y - B1.x) * rh);
-      const float2 t2 = mul_mi<INV>(B2);
-      const float2 t3 = INV ? make_float2((-B3.x - B3.y) * rh, (B3.x - B3.y) * rh)
-                            : make_float2((B3.y - B3.x) * rh, (-B3.x - B3.y) * rh);
-      v[0] = cadd(A0, B0);
-      v[4] = csub(A0, B0);
-      v[1] = cadd(A1, t1);
-      v[5] = csub(A1, t1);
-      v[2] = cadd(A2, t2);
-      v[6] = csub(A2, t2);
-      v[3] = cadd(A3, t3);
-      v[7] = csub(A3, t3);
-    } else if constexpr (R == 4) {
-      float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
-      float2 c = cadd(v[1], v[3]), d = mul_mi<INV>(csub(v[1], v[3]));
-      v[0] = cadd(a, c);
-      v[2] = csub(a, c);
-      v[1] = cadd(b, d);
-      v[3] = csub(b, d);
-    } else if constexpr (R == 3) {
-      const float c1 = -0.5f;
-      const float s1 = INV ? 0.86602540378443864676f : -0.86602540378443864676f;
-      float2 a = v[0], b = v[1], c = v[2];
-      float2 t1 = cadd(b, c);
-      float2 t2 = make_float2(a.x + c1 * t1.x, a.y + c1 * t1.y);
-      float2 t3 = csub(b, c);  // (b - c) * i*s1
-      float2 t4 = make_float2(-s1 * t3.y, s1 * t3.x);
-      v[0] = cadd(a, t1);
-      v[1] = cadd(t2, t4);
-      v[2] = csub(t2, t4);
-    }
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[k * r * step];
+        if (INV) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    dft_radix<R, INV>(v);
     const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
     for (int r = 0; r < R; ++r) D[base + r * Ns] = v[r];
@@ -247,6 +257,86 @@ __device__ float2* fft_lds_waves(float2* a, float2* b, int nseq, int sp, const f
   }
   __syncthreads();
   return parity ? b : a;
+}
+
+// In-place variant for one sequence owned by one wave: every stage first
+// loads all of its butterflies' inputs into registers (a lane holds
+// ceil(L/R/64) butterflies), orders the wave's LDS accesses, then writes the
+// outputs over the same buffer.  Same arithmetic, in the same order, as
+// fft_lds / fft_lds_waves with half the LDS.  Element i of the sequence
+// lives at s[lds_pad<PAD>(i)]: PAD > 0 inserts one unused slot after every
+// 2^PAD elements, which spreads the strided stores of the early stages over
+// the LDS banks.  The caller orders the wave's (or the block's) accesses
+// before and after.
+template <int PAD>
+__device__ __forceinline__ int lds_pad(int i) { return PAD > 0 ? i + (i >> PAD) : i; }
+template <int L, int PAD>
+constexpr int lds_padded_len() { return PAD > 0 ? L + (L >> PAD) : L; }
+
+template <int L, int R, bool INV, int PAD>
+__device__ __forceinline__ void stockham_stage_inplace(float2* s, int Ns, const float2* __restrict__ tw, int lane) {
+  constexpr int LR = L / R;
+  constexpr int NB = (LR + 63) / 64;
+  const int step = L / (Ns * R);
+  float2 v[NB][R];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = lane + 64 * b;
+    if (LR % 64 == 0 || j < LR)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = s[lds_pad<PAD>(j + r * LR)];
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = lane + 64 * b;
+    if (LR % 64 == 0 || j < LR) {
+      const int k = j % Ns;
+      if (Ns > 1)  // the first stage's twiddles are all 1
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[k * r * step];
+          if (INV) w.y = -w.y;
+          v[b][r] = cmul(v[b][r], w);
+        }
+      dft_radix<R, INV>(v[b]);
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[lds_pad<PAD>(base + r * Ns)] = v[b][r];
+    }
+  }
+  wave_lds_sync();
+}
+
+template <int L, bool INV, int PAD = 0>
+__device__ __forceinline__ void fft_wave_inplace(float2* s, const float2* __restrict__ tw, int lane) {
+  using S = FftShape<L>;
+  int Ns = 1;
+#pragma unroll
+  for (int st = 0; st < S::n8; ++st) {
+    stockham_stage_inplace<L, 8, INV, PAD>(s, Ns, tw, lane);
+    Ns *= 8;
+  }
+  if constexpr (S::n4) {
+    stockham_stage_inplace<L, 4, INV, PAD>(s, Ns, tw, lane);
+    Ns *= 4;
+  }
+  if constexpr (S::n2) {
+    stockham_stage_inplace<L, 2, INV, PAD>(s, Ns, tw, lane);
+    Ns *= 2;
+  }
+  if constexpr (S::three) stockham_stage_inplace<L, 3, INV, PAD>(s, Ns, tw, lane);
+}
+
+// nseq sequences at buf[q * sp + lds_pad<PAD>(i)], each transformed in place
+// by one wave (q = wave, wave + nwaves, ...).  The caller must
+// __syncthreads() before the call; the function ends with a barrier.
+template <int L, bool INV, int PAD>
+__device__ __forceinline__ void fft_lds_inplace(float2* buf, int nseq, int sp, const float2* __restrict__ tw,
+                                                int tid, int nthr) {
+  const int wave = tid >> 6, lane = tid & 63, nw = nthr >> 6;
+  for (int q = wave; q < nseq; q += nw) fft_wave_inplace<L, INV, PAD>(buf + q * sp, tw, lane);
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------

@@ -44,6 +44,12 @@ __device__ __forceinline__ int ring_of(int fy, int kx) {
   return (int)rintf(sqrtf((float)(fy * fy + kx * kx)));
 }
 
+// Sequences live in LDS at buf[q * SP + lds_pad<kPad>(i)] and are
+// transformed in place, one wave per sequence (one buffer, no ping-pong).
+constexpr int kPad = 3;
+template <int L>
+constexpr int tpl_sp() { return lds_padded_len<L, kPad>() + 1; }
+
 template <int N>
 struct TplLayout {
   static constexpr int L2 = 2 * N;       // padded length
@@ -51,14 +57,16 @@ struct TplLayout {
   static constexpr int MH = N / 2 + 1;   // box half-spectrum width
   static constexpr int NR = (int)(0.7072 * N) + 3;  // rings (upper bound)
   static constexpr int FIXED = L2 + N + NR / 2 + 1 + 32;  // twiddles, ring divisors, reductions
-  static constexpr int CAP = (kLdsFloat2 - FIXED) / 2;    // per ping-pong buffer
+  static constexpr int CAP = kLdsFloat2 - FIXED;           // sequence buffer
+  static constexpr int SP2 = tpl_sp<L2>();                 // stride of a 2N sequence
+  static constexpr int SP1 = tpl_sp<N>();                  // stride of an N sequence
   // sequence counts per tile are rounded down to a multiple of the waves
   // (each wave owns whole sequences of a tile)
   static constexpr int roundw(int x) { return x >= kWavesT ? x - x % kWavesT : x; }
-  static constexpr int TK1 = roundw(CAP / (L2 + 1));      // phase 1 columns per tile
-  static constexpr int RB2 = roundw(CAP / L2);            // phase 2 complex rows per batch
-  static constexpr int TK3 = CAP / (N + 1);               // box-length column tiles
-  static constexpr int RB6 = CAP / N;                     // box-length complex rows
+  static constexpr int TK1 = roundw(CAP / SP2);           // phase 1 columns per tile
+  static constexpr int RB2 = roundw(CAP / SP2);           // phase 2 complex rows per batch
+  static constexpr int TK3 = CAP / SP1;                   // box-length column tiles
+  static constexpr int RB6 = CAP / SP1;                   // box-length complex rows
   static_assert(TK1 >= 1 && RB2 >= 1, "LDS budget");
   // Scratch images are stored tile-major (the columns of one tile of TK
   // columns form one contiguous [rows][TK] block), so the column passes
@@ -71,7 +79,7 @@ struct TplLayout {
 };
 
 template <int N>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
 build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ rts, int M, int n,
                        const float* __restrict__ mask, const float* __restrict__ fsc,
                        float2* __restrict__ bank, float* __restrict__ norms, float2* __restrict__ scratch,
@@ -83,8 +91,9 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
   float2* tw1 = tw2 + L2;                  // N twiddles
   float* div = (float*)(tw1 + N);          // NR ring divisors
   double* red = (double*)(div + T::NR + (T::NR & 1));  // reduction scratch (16 doubles)
-  float2* bufA = lds + T::FIXED;
-  float2* bufB = bufA + T::CAP;
+  float2* buf = lds + T::FIXED;
+  constexpr int SP2 = T::SP2, SP1 = T::SP1;
+  auto pd = [](int i) { return lds_pad<kPad>(i); };
   const int tid = threadIdx.x;
   constexpr int nthr = kThreads;  // launched with exactly kThreads threads
   fill_twiddles(tw2, L2, tid, nthr);
@@ -117,64 +126,92 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       // index first), so all eight loads of a sample are in flight together.
       constexpr int CW = T::TK1 <= 4 ? 4 : 8;  // columns per lane patch
       constexpr int RW = 64 / CW;              // rows per lane patch
+      static_assert((CW * L2) % (2 * kThreads) == 0, "two samples per thread and step");
       const int ngroups = (nk + CW - 1) / CW;
+      // (z0 + N) mod 2N for z0 in [0, 2N): the fftshift of the spectrum's z / y axes
+      auto wrap = [](unsigned z) -> unsigned { return z >= (unsigned)N ? z - N : z + N; };
       for (int cg = 0; cg < ngroups; ++cg) {
-#pragma unroll 4
-        for (int t = tid; t < CW * L2; t += kThreads) {
-          const int lane = t & 63, rb = t >> 6;
-          const int q = cg * CW + lane / RW, ip = rb * RW + lane % RW;
-          const bool active = q < nk;
-          const int kx = k0 + min(q, nk - 1);
-          const int f = ip < N ? ip : ip - L2;
-          const float fy = (float)f * inv_l2;
-          const float fx = (float)kx * inv_l2;
-          float gx = r00 * fx + r01 * fy;
-          float gy = r10 * fx + r11 * fy;
-          float gz = r20 * fx + r21 * fy;
-          const bool cj = gx < 0.0f;
-          if (cj) { gx = -gx; gy = -gy; gz = -gz; }
-          float cz = gz * (float)L2 + (float)N;
-          float cy = gy * (float)L2 + (float)N;
-          float cx = gx * (float)L2;
-          cz = fminf(fmaxf(cz, 0.0f), (float)(L2 - 1));
-          cy = fminf(fmaxf(cy, 0.0f), (float)(L2 - 1));
-          cx = fminf(fmaxf(cx, 0.0f), (float)(M2 - 1));
-          const float z0f = floorf(cz), y0f = floorf(cy), x0f = floorf(cx);
-          const float tz = cz - z0f, ty = cy - y0f, tx = cx - x0f;
-          const int z0 = (int)z0f, y0 = (int)y0f, x0 = (int)x0f;
-          const size_t pz0 = vdft_brick_z<N>((z0 + N) % L2), pz1 = vdft_brick_z<N>((min(z0 + 1, L2 - 1) + N) % L2);
-          const size_t py0 = vdft_brick_y<N>((y0 + N) % L2), py1 = vdft_brick_y<N>((min(y0 + 1, L2 - 1) + N) % L2);
-          const size_t px0 = vdft_brick_x(x0), px1 = vdft_brick_x(min(x0 + 1, M2 - 1));
-          const float2 v000 = D[pz0 + py0 + px0], v001 = D[pz0 + py0 + px1];
-          const float2 v010 = D[pz0 + py1 + px0], v011 = D[pz0 + py1 + px1];
-          const float2 v100 = D[pz1 + py0 + px0], v101 = D[pz1 + py0 + px1];
-          const float2 v110 = D[pz1 + py1 + px0], v111 = D[pz1 + py1 + px1];
-          const float wx0 = 1.0f - tx, wx1 = tx;
-          const float w00 = (1.0f - tz) * (1.0f - ty), w01 = (1.0f - tz) * ty;
-          const float w10 = tz * (1.0f - ty), w11 = tz * ty;
-          float2 acc = make_float2(0.f, 0.f);
-          auto tap = [&](const float2& a0, const float2& a1, float wzy) {
-            acc.x += (wzy * wx0) * a0.x;
-            acc.y += (wzy * wx0) * a0.y;
-            acc.x += (wzy * wx1) * a1.x;
-            acc.y += (wzy * wx1) * a1.y;
-          };
-          tap(v000, v001, w00);
-          tap(v010, v011, w01);
-          tap(v100, v101, w10);
-          tap(v110, v111, w11);
-          if (cj) acc.y = -acc.y;
-          if (active) bufA[q * (L2 + 1) + ip] = acc;
+        for (int t0 = tid; t0 < CW * L2; t0 += 2 * kThreads) {
+          // two samples per step: all 16 taps are issued before any is used
+          unsigned ad[2][8];
+          float wt[2][7];  // tz, ty, tx, then the conjugation flag and the store slot
+          int slot[2];
+          bool cjs[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int t = t0 + u * kThreads;
+            const int lane = t & 63, rb = t >> 6;
+            const int q = cg * CW + lane / RW, ip = rb * RW + lane % RW;
+            slot[u] = q < nk ? q * SP2 + pd(ip) : -1;
+            const int kx = k0 + min(q, nk - 1);
+            const int f = ip < N ? ip : ip - L2;
+            const float fy = (float)f * inv_l2;
+            const float fx = (float)kx * inv_l2;
+            float gx = r00 * fx + r01 * fy;
+            float gy = r10 * fx + r11 * fy;
+            float gz = r20 * fx + r21 * fy;
+            const bool cj = gx < 0.0f;
+            if (cj) { gx = -gx; gy = -gy; gz = -gz; }
+            cjs[u] = cj;
+            float cz = gz * (float)L2 + (float)N;
+            float cy = gy * (float)L2 + (float)N;
+            float cx = gx * (float)L2;
+            cz = fminf(fmaxf(cz, 0.0f), (float)(L2 - 1));
+            cy = fminf(fmaxf(cy, 0.0f), (float)(L2 - 1));
+            cx = fminf(fmaxf(cx, 0.0f), (float)(M2 - 1));
+            const float z0f = floorf(cz), y0f = floorf(cy), x0f = floorf(cx);
+            wt[u][0] = cz - z0f;
+            wt[u][1] = cy - y0f;
+            wt[u][2] = cx - x0f;
+            const unsigned z0 = (unsigned)(int)z0f, y0 = (unsigned)(int)y0f, x0 = (unsigned)(int)x0f;
+            // Branch-free: a clamped upper tap always carries weight 0 (the
+            // coordinate was clamped to the last index first)
+            const unsigned pz0 = vdft_brick_z32<N>(wrap(z0)), pz1 = vdft_brick_z32<N>(wrap(min(z0 + 1, (unsigned)L2 - 1)));
+            const unsigned py0 = vdft_brick_y32<N>(wrap(y0)), py1 = vdft_brick_y32<N>(wrap(min(y0 + 1, (unsigned)L2 - 1)));
+            const unsigned px0 = vdft_brick_x32(x0), px1 = vdft_brick_x32(min(x0 + 1, (unsigned)M2 - 1));
+            ad[u][0] = pz0 + py0 + px0; ad[u][1] = pz0 + py0 + px1;
+            ad[u][2] = pz0 + py1 + px0; ad[u][3] = pz0 + py1 + px1;
+            ad[u][4] = pz1 + py0 + px0; ad[u][5] = pz1 + py0 + px1;
+            ad[u][6] = pz1 + py1 + px0; ad[u][7] = pz1 + py1 + px1;
+          }
+          // 32-bit byte offsets from the (scalar) base: the spectrum is < 4 GiB
+          const char* Db = reinterpret_cast<const char*>(D);
+          float2 v[2][8];
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[u][i] = *reinterpret_cast<const float2*>(Db + (size_t)(ad[u][i] * 8u));
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const float tz = wt[u][0], ty = wt[u][1], tx = wt[u][2];
+            const float wx0 = 1.0f - tx, wx1 = tx;
+            const float w00 = (1.0f - tz) * (1.0f - ty), w01 = (1.0f - tz) * ty;
+            const float w10 = tz * (1.0f - ty), w11 = tz * ty;
+            float2 acc = make_float2(0.f, 0.f);
+            auto tap = [&](const float2& a0, const float2& a1, float wzy) {
+              acc.x += (wzy * wx0) * a0.x;
+              acc.y += (wzy * wx0) * a0.y;
+              acc.x += (wzy * wx1) * a1.x;
+              acc.y += (wzy * wx1) * a1.y;
+            };
+            tap(v[u][0], v[u][1], w00);
+            tap(v[u][2], v[u][3], w01);
+            tap(v[u][4], v[u][5], w10);
+            tap(v[u][6], v[u][7], w11);
+            if (cjs[u]) acc.y = -acc.y;
+            if (slot[u] >= 0) buf[slot[u]] = acc;
+          }
         }
       }
       __syncthreads();
-      float2* res = fft_lds_waves<L2, true>(bufA, bufB, nk, L2 + 1, tw2, tid, nthr);
+      fft_lds_inplace<L2, true, kPad>(buf, nk, SP2, tw2, tid, nthr);
       // keep spatial rows y in [3N/2, 2N) u [0, N/2) -> projection row Y
       for (int t = tid; t < T::TK1 * N; t += nthr) {
         const int Y = t / T::TK1, q = t - Y * T::TK1;
         if (q >= nk) continue;
         const int y = (Y + 3 * N / 2) % L2;
-        U[uidx(Y, k0 + q)] = res[q * (L2 + 1) + y];
+        U[uidx(Y, k0 + q)] = buf[q * SP2 + pd(y)];
       }
       __syncthreads();
     }
@@ -192,23 +229,39 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         float2 b = yb < y0 + nrow ? U[uidx(yb, ks)] : make_float2(0.f, 0.f);
         if (ks == 0 || ks == N) { a.y = 0.f; b.y = 0.f; }
         if (k > N) { a.y = -a.y; b.y = -b.y; }
-        bufA[q * L2 + k] = make_float2(a.x - b.y, a.y + b.x);  // a + i*b
+        buf[q * SP2 + pd(k)] = make_float2(a.x - b.y, a.y + b.x);  // a + i*b
       }
       __syncthreads();
-      float2* res = fft_lds_waves<L2, true>(bufA, bufB, nseq, L2, tw2, tid, nthr);
-      float2* oth = res == bufA ? bufB : bufA;
-      const float sc = 1.0f / ((float)L2 * (float)L2);
-      for (int t = tid; t < nseq * N; t += nthr) {
-        const int q = t / N, X = t - q * N;
-        const float2 v = res[q * L2 + (X + 3 * N / 2) % L2];
-        oth[q * N + X] = make_float2(v.x * sc, v.y * sc);  // projection rows ya (re), yb (im)
+      // inverse 2N transform; each wave then crops the centre N (scaled) of
+      // its own sequences to the front of their slots; forward N transform
+      fft_lds_inplace<L2, true, kPad>(buf, nseq, SP2, tw2, tid, nthr);
+      {
+        const int wave = tid >> 6, lane = tid & 63;
+        const float sc = 1.0f / ((float)L2 * (float)L2);
+        constexpr int KP = (N + 63) / 64;
+        for (int q = wave; q < nseq; q += kWavesT) {
+          float2* sq = buf + q * SP2;
+          float2 keep[KP];
+#pragma unroll
+          for (int i = 0; i < KP; ++i) {
+            const int X = lane + 64 * i;
+            if (X < N) {
+              const float2 v = sq[pd((X + 3 * N / 2) % L2)];
+              keep[i] = make_float2(v.x * sc, v.y * sc);  // projection rows ya (re), yb (im)
+            }
+          }
+          wave_lds_sync();
+#pragma unroll
+          for (int i = 0; i < KP; ++i)
+            if (lane + 64 * i < N) sq[pd(lane + 64 * i)] = keep[i];
+        }
+        __syncthreads();
       }
-      __syncthreads();
-      float2* spec = fft_lds_waves<N, false>(oth, res, nseq, N, tw1, tid, nthr);
+      fft_lds_inplace<N, false, kPad>(buf, nseq, SP2, tw1, tid, nthr);
       for (int t = tid; t < nseq * MH; t += nthr) {
         const int q = t / MH, k = t - q * MH;
-        const float2 z = spec[q * N + k];
-        const float2 zc = cconj(spec[q * N + (N - k) % N]);
+        const float2 z = buf[q * SP2 + pd(k)];
+        const float2 zc = cconj(buf[q * SP2 + pd((N - k) % N)]);
         const int ya = y0 + 2 * q;
         W[widx(ya, k)] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
         if (ya + 1 < y0 + nrow)
@@ -233,14 +286,15 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       for (int t = tid; t < T::TK3 * N; t += nthr) {
         const int Y = t / T::TK3, q = t - Y * T::TK3;
         if (q >= nk) continue;
-        bufA[q * (N + 1) + Y] = W[widx(Y, k0 + q)];
+        buf[q * SP1 + pd(Y)] = W[widx(Y, k0 + q)];
       }
       __syncthreads();
-      float2* res = fft_lds_waves<N, false>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
+      fft_lds_inplace<N, false, kPad>(buf, nk, SP1, tw1, tid, nthr);
+      const float2* res = buf;
       for (int t = tid; t < T::TK3 * N; t += nthr) {
         const int ky = t / T::TK3, q = t - ky * T::TK3;
         if (q >= nk) continue;
-        W[widx(ky, k0 + q)] = res[q * (N + 1) + ky];
+        W[widx(ky, k0 + q)] = res[q * SP1 + pd(ky)];
       }
       const int lane = tid & 63;
       for (int task = tid >> 6; task < 2 * nk; task += nthr / 64) {
@@ -261,7 +315,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
           float val = 0.0f;
           int cnt = 0;
           if (ok) {
-            const float2 v = res[q * (N + 1) + (fy < 0 ? fy + N : fy)];
+            const float2 v = res[q * SP1 + pd(fy < 0 ? fy + N : fy)];
             val = sqrtf(v.x * v.x + v.y * v.y) * wgt;
             cnt = iw;
             key = ring_of(fy, kx);
@@ -314,14 +368,14 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         const int fy = ky < N / 2 ? ky : ky - N;
         const float g = div[ring_of(fy, k0 + q)];
         const float2 v = W[widx(ky, k0 + q)];
-        bufA[q * (N + 1) + ky] = make_float2(v.x * g, v.y * g);
+        buf[q * SP1 + pd(ky)] = make_float2(v.x * g, v.y * g);
       }
       __syncthreads();
-      float2* res = fft_lds_waves<N, true>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
+      fft_lds_inplace<N, true, kPad>(buf, nk, SP1, tw1, tid, nthr);
       for (int t = tid; t < T::TK3 * N; t += nthr) {
         const int Y = t / T::TK3, q = t - Y * T::TK3;
         if (q >= nk) continue;
-        W[widx(Y, k0 + q)] = res[q * (N + 1) + Y];
+        W[widx(Y, k0 + q)] = buf[q * SP1 + pd(Y)];
       }
       __syncthreads();
     }
@@ -339,14 +393,14 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         float2 b = yb < y0 + nrow ? W[widx(yb, ks)] : make_float2(0.f, 0.f);
         if (ks == 0 || ks == N / 2) { a.y = 0.f; b.y = 0.f; }
         if (k > N / 2) { a.y = -a.y; b.y = -b.y; }
-        bufA[q * N + k] = make_float2(a.x - b.y, a.y + b.x);
+        buf[q * SP1 + pd(k)] = make_float2(a.x - b.y, a.y + b.x);
       }
       __syncthreads();
-      float2* res = fft_lds_waves<N, true>(bufA, bufB, nseq, N, tw1, tid, nthr);
+      fft_lds_inplace<N, true, kPad>(buf, nseq, SP1, tw1, tid, nthr);
       const float sc = 1.0f / ((float)N * (float)N);
       for (int t = tid; t < nseq * N; t += nthr) {
         const int q = t / N, X = t - q * N;
-        const float2 v = res[q * N + X];
+        const float2 v = buf[q * SP1 + pd(X)];
         const int ya = y0 + 2 * q;
         const float za = v.x * sc;
         Z[(size_t)ya * N + X] = za;
@@ -416,14 +470,14 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
           vb = ((vb - ((fa * (float)yb + fb * (float)X) + fc)) - fmean) * fis;
           if (mask) vb *= mask[(size_t)yb * N + X];
         }
-        bufA[q * N + X] = make_float2(va, vb);
+        buf[q * SP1 + pd(X)] = make_float2(va, vb);
       }
       __syncthreads();
-      float2* res = fft_lds_waves<N, false>(bufA, bufB, nseq, N, tw1, tid, nthr);
+      fft_lds_inplace<N, false, kPad>(buf, nseq, SP1, tw1, tid, nthr);
       for (int t = tid; t < nseq * MH; t += nthr) {
         const int q = t / MH, k = t - q * MH;
-        const float2 z = res[q * N + k];
-        const float2 zc = cconj(res[q * N + (N - k) % N]);
+        const float2 z = buf[q * SP1 + pd(k)];
+        const float2 zc = cconj(buf[q * SP1 + pd((N - k) % N)]);
         const int ya = y0 + 2 * q;
         W[widx(ya, k)] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
         if (ya + 1 < y0 + nrow)
@@ -441,10 +495,11 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       for (int t = tid; t < T::TK3 * N; t += nthr) {
         const int Y = t / T::TK3, q = t - Y * T::TK3;
         if (q >= nk) continue;
-        bufA[q * (N + 1) + Y] = W[widx(Y, k0 + q)];
+        buf[q * SP1 + pd(Y)] = W[widx(Y, k0 + q)];
       }
       __syncthreads();
-      float2* res = fft_lds_waves<N, false>(bufA, bufB, nk, N + 1, tw1, tid, nthr);
+      fft_lds_inplace<N, false, kPad>(buf, nk, SP1, tw1, tid, nthr);
+      const float2* res = buf;
       for (int t = tid; t < T::TK3 * n; t += nthr) {
         const int r = t / T::TK3, q = t - r * T::TK3;
         if (q >= nk) continue;
@@ -452,10 +507,10 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         const int fy = r < n / 2 ? r : r - n;
         float2 v;
         if (c < n / 2 || n == N) {
-          v = res[q * (N + 1) + (fy < 0 ? fy + N : fy)];
+          v = res[q * SP1 + pd(fy < 0 ? fy + N : fy)];
         } else {  // packed column n/2 holds frequency -n/2 = conj of (+n/2) at -fy
           const int my = -fy;
-          v = cconj(res[q * (N + 1) + (my < 0 ? my + N : my)]);
+          v = cconj(res[q * SP1 + pd(my < 0 ? my + N : my)]);
         }
         const float g = fsc ? fsc[(size_t)r * mh_n + c] : 1.0f;
         v.x *= g;

@@ -23,3 +23,17 @@ __device__ __forceinline__ size_t vdft_brick_y(int ky) {
   return (size_t)(ky >> 1) * VdftBricks<N>::YSTRIDE + (size_t)(ky & 1) * 4;
 }
 __device__ __forceinline__ size_t vdft_brick_x(int kx) { return (size_t)(kx >> 2) * 16 + (kx & 3); }
+
+// 32-bit forms of the same offsets (the spectrum holds < 2^32 values for every
+// supported box), so a gather can address D with a scalar base and a 32-bit
+// vector offset.
+template <int N>
+__device__ __forceinline__ unsigned vdft_brick_z32(unsigned kz) {
+  return (kz >> 1) * (unsigned)VdftBricks<N>::ZSTRIDE + (kz & 1u) * 8u;
+}
+template <int N>
+__device__ __forceinline__ unsigned vdft_brick_y32(unsigned ky) {
+  return (ky >> 1) * (unsigned)VdftBricks<N>::YSTRIDE + (ky & 1u) * 4u;
+}
+__device__ __forceinline__ unsigned vdft_brick_x32(unsigned kx) { return (kx >> 2) * 16u + (kx & 3u); }
+static_assert(VdftBricks<256>::ELEMS < (1ull << 32), "32-bit brick offsets");

@@ -10,11 +10,14 @@
 // Three LDS Stockham passes (x: real-to-complex, then y, then z).  The zero
 // padding is never read or written: the x pass touches only the N^2 rows
 // that hold data, the y pass only the N data planes and synthesises the zero
-// rows, and the z pass synthesises the zero planes.  Traffic per transform
-// is ~ 4N^3 + 2*8*N^2(N+1) + 2*8*2N^2(N+1) bytes.
+// rows, and the z pass synthesises the zero planes.  Every pass moves whole
+// 128-byte lines: a block transforms 16 columns (two groups of 8 kx, or a
+// pair of ky rows x 8 kx) in place in one LDS buffer.  Traffic per
+// transform ~ 4N^3 + 8*N^2*G*16*(1/2 + 1/2 + 1 + 1) + 8*4N^2(N+4) bytes
+// (G = ceil((N+1)/8)): 1.44 GB at box 256.
 //
-// Output layout (bricked): the x and y passes work on a natural-order
-// workspace [2N][2N][N+1]; the z pass writes the result in bricks of
+// Output layout (bricked): the x and y passes work on a workspace of the N
+// data planes (see VdftWork); the z pass writes the result in bricks of
 // 2 (z) x 2 (y) x 4 (x) complex values = one 128-byte cache line, so the 2x2x2
 // trilinear cell of a central-slice sample spans ~2.8 lines instead of four
 // 16-element x rows, whatever the slice orientation:
@@ -22,12 +25,6 @@
 //   XB = N/4 + 1 (x padded to N+4; the padding is never read).
 #include "common.h"
 #include "vdft_layout.h"
-
-// LDS per workgroup (float2): 52.8 KB lets three workgroups share a CU.
-// Sequences per workgroup are a multiple of the 4 waves (each wave owns
-// whole sequences; see fft_lds_waves).
-constexpr int kVdftLds = 6600;
-constexpr int vdft_round4(int x) { return x >= 4 ? x & ~3 : x; }
 
 // Row x' of the shifted padded volume holds padded column (x'+N) mod 2N,
 // which is data column (x'+N) mod 2N - N/2 when that lies in [0, N).
@@ -42,150 +39,205 @@ __device__ __forceinline__ float sinc2_axis_freq(int padded_index, int N) {
   return (float)((double)(padded_index - N) / (double)(2 * N));
 }
 
+// Workspace of the x and y passes: the N data planes (vz) of the
+// natural-order spectrum, all 2N rows (ky, shifted order), kx in groups of
+// eight; a pair of rows of one group is one 128-byte line:
+//   W[(((vz * N + ky/2) * G + kx/8) * 2 + ky%2) * 8 + kx%8],  G = ceil((N+1)/8).
+template <int N>
+struct VdftWork {
+  static constexpr int G = (N + 8) / 8;
+  static constexpr size_t ELEMS = (size_t)N * N * G * 16;
+};
+
+// Every pass: 512 threads (8 waves), 16 sequences of length 2N per block in
+// one LDS buffer (each wave transforms its sequences in place, registers
+// holding a stage's butterflies), ~70 KB at box 256 so two blocks share a CU.
+constexpr int kVdftThreads = 512;
+constexpr int kVdftSeq = 16;
+constexpr int kVdftPad = 3;  // one LDS slot of padding per 8 values
+static_assert((2 * 32) % 32 == 0 && kVdftThreads == 512, "trip counts below assume 512 threads and 2N % 32 == 0");
+template <int N>
+constexpr int vdft_sp() { return lds_padded_len<2 * N, kVdftPad>() + 1; }
+template <int N>
+constexpr size_t vdft_lds_bytes() { return (size_t)(2 * N + kVdftSeq * vdft_sp<N>()) * sizeof(float2); }
+
 // x pass: two data rows per complex FFT (real a + i*b), split afterwards.
 template <int N>
-__global__ void __launch_bounds__(256)
-vdft_x_pass(const float* __restrict__ vol, float2* __restrict__ D) {
+__global__ void __launch_bounds__(kVdftThreads)
+vdft_x_pass(const float* __restrict__ vol, float2* __restrict__ Wk) {
   constexpr int L = 2 * N;
-  constexpr int M = N + 1;
-  constexpr int RB = vdft_round4((kVdftLds - L) / (2 * L));  // complex sequences per block
+  constexpr int SP = vdft_sp<N>();
+  constexpr int G = VdftWork<N>::G;
   extern __shared__ float2 lds[];
   float2* tw = lds;
-  float2* bufA = tw + L;
-  float2* bufB = bufA + RB * L;
-  const int tid = threadIdx.x;
-  constexpr int nthr = 256;  // launched with exactly 256 threads
+  float2* buf = tw + L;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int nthr = kVdftThreads;
   fill_twiddles(tw, L, tid, nthr);
-  const int row_pair0 = blockIdx.x * RB;  // data row index = 2 * pair (+1)
-  const int total_pairs = (N * N) / 2;
-  const int nseq = min(RB, total_pairs - row_pair0);
-  for (int t = tid; t < nseq * L; t += nthr) {
-    const int q = t / L, xs = t - q * L;
-    float v2[2];
+  const int pair0 = blockIdx.x * kVdftSeq;  // data row index = 2 * pair (+1)
+  constexpr int nseq = kVdftSeq;             // N^2 / 2 is a multiple of 16
+  static_assert((N * N / 2) % kVdftSeq == 0, "row pairs per block");
+  // data columns (q, h, vx), vx fastest: unconditional loads, unrolled, so
+  // all of a thread's loads are in flight together
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int row = 2 * (row_pair0 + q) + h;  // row = vz * N + vy
-      const int vz = row / N, vy = row - vz * N;
-      const int vx = shifted_to_data(xs, N);
-      float v = 0.0f;
-      if (vx >= 0) {
-        const float fz = sinc2_axis_freq(vz + N / 2, N);
-        const float fy = sinc2_axis_freq(vy + N / 2, N);
-        const float fx = sinc2_axis_freq(vx + N / 2, N);
-        const float r = sqrtf((fz * fz + fy * fy) + fx * fx);
-        float s = 1.0f;
-        if (r != 0.0f) s = sinpif(r) / (3.14159265358979323846f * r);
-        v = vol[((size_t)vz * N + vy) * N + vx] * (s * s);
-      }
-      v2[h] = v;
-    }
-    bufA[q * L + xs] = make_float2(v2[0], v2[1]);
+  for (int it = 0; it < kVdftSeq * 2 * N / nthr; ++it) {
+    const int t = tid + it * nthr;
+    const int vx = t % N, qh = t / N, q = qh >> 1, h = qh & 1;
+    const int row = 2 * (pair0 + q) + h;  // row = vz * N + vy
+    const int vz = row / N, vy = row - vz * N;
+    const float raw = vol[((size_t)vz * N + vy) * N + vx];
+    const float fz = sinc2_axis_freq(vz + N / 2, N);
+    const float fy = sinc2_axis_freq(vy + N / 2, N);
+    const float fx = sinc2_axis_freq(vx + N / 2, N);
+    const float r = sqrtf((fz * fz + fy * fy) + fx * fx);
+    float sn = 1.0f;
+    if (r != 0.0f) sn = sinpif(r) / (3.14159265358979323846f * r);
+    const int xs = (vx + N / 2 + N) % L;  // shifted column of data column vx
+    float* dst = (float*)&buf[q * SP + lds_pad<kVdftPad>(xs)];
+    dst[h] = raw * (sn * sn);
+  }
+  // zero columns: shifted xs in [N/2, 3N/2)
+  for (int t = tid; t < kVdftSeq * N; t += nthr) {
+    const int q = t / N, xs = N / 2 + (t - q * N);
+    buf[q * SP + lds_pad<kVdftPad>(xs)] = make_float2(0.f, 0.f);
   }
   __syncthreads();
-  float2* res = fft_lds_waves<L, false>(bufA, bufB, nseq, L, tw, tid, nthr);
-  // split: A[k] = (Z[k] + conj Z[L-k]) / 2 ; B[k] = (Z[k] - conj Z[L-k]) / (2i)
-  for (int t = tid; t < nseq * M; t += nthr) {
-    const int q = t / M, k = t - q * M;
-    const float2 z = res[q * L + k];
-    const float2 zc = cconj(res[q * L + ((L - k) % L)]);
-    const float2 a = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
-    const float2 d = make_float2(0.5f * (z.x - zc.x), 0.5f * (z.y - zc.y));
-    const float2 b = make_float2(d.y, -d.x);  // d / i
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int row = 2 * (row_pair0 + q) + h;
-      const int vz = row / N, vy = row - vz * N;
-      const int zs = (vz + N / 2 + N) % L;  // shifted plane index
-      const int ys = (vy + N / 2 + N) % L;
-      D[((size_t)zs * L + ys) * M + k] = h ? b : a;
+  for (int q = wave; q < nseq; q += nthr / 64) fft_wave_inplace<L, false, kVdftPad>(buf + q * SP, tw, lane);
+  __syncthreads();
+  // split: A[k] = (Z[k] + conj Z[L-k]) / 2 ; B[k] = (Z[k] - conj Z[L-k]) / (2i).
+  // Rows 2p and 2p+1 are workspace rows ys and ys+1 (ys even): 16
+  // consecutive threads fill one line (row parity h, 8 kx).
+  for (int t = tid; t < nseq * G * 16; t += nthr) {
+    const int q = t / (G * 16), r = t - q * (G * 16);
+    const int g = r >> 4, h = (r >> 3) & 1, k = g * 8 + (r & 7);
+    if (k > N) continue;
+    const float2 z = buf[q * SP + lds_pad<kVdftPad>(k)];
+    const float2 zc = cconj(buf[q * SP + lds_pad<kVdftPad>((L - k) % L)]);
+    float2 val;
+    if (h == 0) {
+      val = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+    } else {
+      const float2 d = make_float2(0.5f * (z.x - zc.x), 0.5f * (z.y - zc.y));
+      val = make_float2(d.y, -d.x);  // d / i
     }
+    const int row = 2 * (pair0 + q);
+    const int vz = row / N, vy = row - vz * N;
+    const int ys = (vy + N / 2 + N) % L;
+    Wk[(((size_t)vz * N + ys / 2) * G + g) * 16 + h * 8 + (k & 7)] = val;
   }
 }
 
-// y pass over the N data planes: column FFTs along y (stride M).
+// y pass: column FFTs along ky over the N data rows of one data plane, 16
+// kx columns (two groups) per block; the zero rows are synthesised.
 template <int N>
-__global__ void __launch_bounds__(256)
-vdft_y_pass(float2* __restrict__ D) {
+__global__ void __launch_bounds__(kVdftThreads)
+vdft_y_pass(float2* __restrict__ Wk) {
   constexpr int L = 2 * N;
-  constexpr int M = N + 1;
-  constexpr int TK = vdft_round4((kVdftLds - L) / (2 * (L + 1)));
-  constexpr int SP = L + 1;
+  constexpr int SP = vdft_sp<N>();
+  constexpr int G = VdftWork<N>::G;
   extern __shared__ float2 lds[];
   float2* tw = lds;
-  float2* bufA = tw + L;
-  float2* bufB = bufA + TK * SP;
-  const int tid = threadIdx.x;
-  constexpr int nthr = 256;  // launched with exactly 256 threads
+  float2* buf = tw + L;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int nthr = kVdftThreads;
   fill_twiddles(tw, L, tid, nthr);
   const int vz = blockIdx.y;
-  const int zs = (vz + N / 2 + N) % L;
-  const int k0 = blockIdx.x * TK;
-  const int nk = min(TK, M - k0);
-  float2* P = D + (size_t)zs * L * M;
-  for (int t = tid; t < L * TK; t += nthr) {
-    const int ys = t / TK, q = t - ys * TK;
+  const int g0 = blockIdx.x * 2;
+  const int ng = min(2, G - g0);
+  const int ncol = min(kVdftSeq, N + 1 - g0 * 8);  // valid kx columns of the tile
+  float2* P = Wk + (size_t)vz * N * G * 16;
+  // 32 threads per row pair: (group, row parity, kx % 8) = two adjacent lines
+#pragma unroll
+  for (int it = 0; it < (L / 2) * 32 / nthr; ++it) {
+    const int t = tid + it * nthr;
+    const int p = t >> 5, r = t & 31;
+    const int gl = r >> 4, h = (r >> 3) & 1, c = gl * 8 + (r & 7);
+    const int ys = 2 * p + h;
     float2 v = make_float2(0.f, 0.f);
-    if (q < nk && shifted_to_data(ys, N) >= 0) v = P[(size_t)ys * M + k0 + q];
-    bufA[q * SP + ys] = v;
+    if (gl < ng && shifted_to_data(ys, N) >= 0) v = P[((size_t)p * G + g0 + gl) * 16 + (r & 15)];
+    buf[c * SP + lds_pad<kVdftPad>(ys)] = v;
   }
   __syncthreads();
-  float2* res = fft_lds_waves<L, false>(bufA, bufB, nk, SP, tw, tid, nthr);
-  for (int t = tid; t < L * TK; t += nthr) {
-    const int ky = t / TK, q = t - ky * TK;
-    if (q < nk) P[(size_t)ky * M + k0 + q] = res[q * SP + ky];
+  for (int q = wave; q < ncol; q += nthr / 64) fft_wave_inplace<L, false, kVdftPad>(buf + q * SP, tw, lane);
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < (L / 2) * 32 / nthr; ++it) {
+    const int t = tid + it * nthr;
+    const int p = t >> 5, r = t & 31;
+    const int gl = r >> 4, h = (r >> 3) & 1, c = gl * 8 + (r & 7);
+    if (c < ncol) P[((size_t)p * G + g0 + gl) * 16 + (r & 15)] = buf[c * SP + lds_pad<kVdftPad>(2 * p + h)];
   }
 }
 
-// z pass: column FFTs along z (stride 2N*M) of the natural-order workspace W;
-// zero planes are synthesised; the result goes to D in the bricked layout.
+// z pass: column FFTs along kz for one pair of ky rows and one group of 8 kx
+// (16 columns); the zero planes are synthesised; the result goes to D in
+// the bricked layout, two whole bricks per kz pair.
 template <int N>
-__global__ void __launch_bounds__(256)
-vdft_z_pass(const float2* __restrict__ W, float2* __restrict__ D) {
+__global__ void __launch_bounds__(kVdftThreads)
+vdft_z_pass(const float2* __restrict__ Wk, float2* __restrict__ D) {
   constexpr int L = 2 * N;
-  constexpr int M = N + 1;
-  constexpr int TK = vdft_round4((kVdftLds - L) / (2 * (L + 1)));
-  constexpr int SP = L + 1;
+  constexpr int SP = vdft_sp<N>();
+  constexpr int G = VdftWork<N>::G;
+  using BR = VdftBricks<N>;
   extern __shared__ float2 lds[];
   float2* tw = lds;
-  float2* bufA = tw + L;
-  float2* bufB = bufA + TK * SP;
-  const int tid = threadIdx.x;
-  constexpr int nthr = 256;  // launched with exactly 256 threads
+  float2* buf = tw + L;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int nthr = kVdftThreads;
   fill_twiddles(tw, L, tid, nthr);
-  const int ky = blockIdx.y;
-  const int k0 = blockIdx.x * TK;
-  const int nk = min(TK, M - k0);
-  const size_t plane = (size_t)L * M;
-  for (int t = tid; t < L * TK; t += nthr) {
-    const int zs = t / TK, q = t - zs * TK;
+  const int g = blockIdx.x, p = blockIdx.y;  // kx group, ky pair
+  const int nx = min(8, N + 1 - g * 8);      // valid kx of the group
+  // column c = (ky % 2) * 8 + kx % 8
+#pragma unroll
+  for (int it = 0; it < L * 16 / nthr; ++it) {
+    const int t = tid + it * nthr;
+    const int zs = t >> 4, c = t & 15;
+    const int vz = shifted_to_data(zs, N);
     float2 v = make_float2(0.f, 0.f);
-    if (q < nk && shifted_to_data(zs, N) >= 0) v = W[zs * plane + (size_t)ky * M + k0 + q];
-    bufA[q * SP + zs] = v;
+    if (vz >= 0 && (c & 7) < nx) v = Wk[(((size_t)vz * N + p) * G + g) * 16 + c];
+    buf[c * SP + lds_pad<kVdftPad>(zs)] = v;
   }
   __syncthreads();
-  float2* res = fft_lds_waves<L, false>(bufA, bufB, nk, SP, tw, tid, nthr);
-  const size_t ypart = vdft_brick_y<N>(ky);
-  for (int t = tid; t < L * TK; t += nthr) {
-    const int kz = t / TK, q = t - kz * TK;
-    if (q < nk) D[vdft_brick_z<N>(kz) + ypart + vdft_brick_x(k0 + q)] = res[q * SP + kz];
+  for (int q = wave; q < 16; q += nthr / 64)
+    if ((q & 7) < nx) fft_wave_inplace<L, false, kVdftPad>(buf + q * SP, tw, lane);
+  __syncthreads();
+  // 32 threads per kz pair: (x brick 2g or 2g+1, kz % 2, ky % 2, kx % 4)
+#pragma unroll
+  for (int it = 0; it < (L / 2) * 32 / nthr; ++it) {
+    const int t = tid + it * nthr;
+    const int kp = t >> 5, r = t & 31;
+    const int xb = r >> 4, kzp = (r >> 3) & 1, h = (r >> 2) & 1, x4 = r & 3;
+    const int x8 = xb * 4 + x4;
+    if (x8 >= nx) continue;
+    D[(size_t)kp * BR::ZSTRIDE + (size_t)p * BR::YSTRIDE + (size_t)(2 * g + xb) * 16 + (r & 15)] =
+        buf[(h * 8 + x8) * SP + lds_pad<kVdftPad>(2 * kp + kzp)];
   }
 }
 
 template <int N>
 static int launch_volume_dft(const float* vol, float2* D, float2* Wk, hipStream_t st) {
-  constexpr int L = 2 * N;
-  constexpr int M = N + 1;
-  constexpr int RB = vdft_round4((kVdftLds - L) / (2 * L));
-  constexpr int TK = vdft_round4((kVdftLds - L) / (2 * (L + 1)));
+  constexpr int G = VdftWork<N>::G;
+  const size_t lds = vdft_lds_bytes<N>();
+  static bool attr_set = false;
+  if (lds > 65536 && !attr_set) {
+    const void* fns[3] = {(const void*)vdft_x_pass<N>, (const void*)vdft_y_pass<N>, (const void*)vdft_z_pass<N>};
+    for (const void* f : fns) {
+      hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        gis_set_error("gis_volume_dft: cannot reserve %zu bytes of LDS: %s", lds, hipGetErrorString(e));
+        return (int)e;
+      }
+    }
+    attr_set = true;
+  }
   const int pairs = N * N / 2;
-  const size_t lx = (size_t)(L + 2 * RB * L) * sizeof(float2);
-  hipLaunchKernelGGL(vdft_x_pass<N>, dim3((pairs + RB - 1) / RB), dim3(256), lx, st, vol, Wk);
+  hipLaunchKernelGGL(vdft_x_pass<N>, dim3((pairs + kVdftSeq - 1) / kVdftSeq), dim3(kVdftThreads), lds, st, vol,
+                     Wk);
   GIS_LAUNCH_CHECK("gis_volume_dft x");
-  const size_t lyz = (size_t)(L + 2 * TK * (L + 1)) * sizeof(float2);
-  const int ktiles = (M + TK - 1) / TK;
-  hipLaunchKernelGGL(vdft_y_pass<N>, dim3(ktiles, N), dim3(256), lyz, st, Wk);
+  hipLaunchKernelGGL(vdft_y_pass<N>, dim3((G + 1) / 2, N), dim3(kVdftThreads), lds, st, Wk);
   GIS_LAUNCH_CHECK("gis_volume_dft y");
-  hipLaunchKernelGGL(vdft_z_pass<N>, dim3(ktiles, L), dim3(256), lyz, st, (const float2*)Wk, D);
+  hipLaunchKernelGGL(vdft_z_pass<N>, dim3(G, N), dim3(kVdftThreads), lds, st, (const float2*)Wk, D);
   GIS_LAUNCH_CHECK("gis_volume_dft z");
   return GIS_OK;
 }
@@ -199,7 +251,7 @@ GIS_API size_t gis_volume_dft_bytes(int N) {
   return N > 0 ? (size_t)4 * N * N * (N / 4 + 1) * 4 * sizeof(float2) : 0;
 }
 GIS_API size_t gis_volume_dft_workspace(int N) {
-  return N > 0 ? (size_t)4 * N * N * (N + 1) * sizeof(float2) : 0;
+  return N > 0 ? (size_t)N * N * ((N + 8) / 8) * 16 * sizeof(float2) : 0;
 }
 
 // Replaces the pad / sinc^2 / fftshift / rfftn prefix of

@@ -55,7 +55,8 @@ int gis_splat(const float* xyz, const float* numbers, int n_atoms, int box, int 
  * values: element (kz, ky, kx) is at
  *   ((((kz/2) * N + ky/2) * XB + kx/4) * 2 + kz%2) * 2 + ky%2) * 4 + kx%4,
  * XB = N/4 + 1 (gis_volume_dft_bytes(N) bytes).  ws: gis_volume_dft_workspace(N)
- * bytes of scratch (natural-order intermediate).
+ * bytes of scratch (the x / y intermediate of the N data planes,
+ * N * N * ceil((N+1)/8) * 16 complex64 values).
  */
 size_t gis_volume_dft_bytes(int N);
 size_t gis_volume_dft_workspace(int N);

@@ -51,4 +51,5 @@ def test_version_and_argument_errors_without_gpu(library):
     rc = library.gis_volume_dft(None, 32, None, None, 0, None)
     assert rc == -1
     assert library.gis_volume_dft_bytes(256) == 16 * 256 * 256 * 65 * 8
-    assert library.gis_volume_dft_workspace(256) == 4 * 256 * 256 * 257 * 8
+    # N data planes x 2N rows x 33 groups of 8 kx (257 padded to 264)
+    assert library.gis_volume_dft_workspace(256) == 256 * 512 * 264 * 8

@@ -53,7 +53,7 @@ def test_splat_matches_oracle(hip_device, box, apix, res):
     assert np.all(got[-1] == 0) and np.all(got[:, -1] == 0) and np.all(got[:, :, -1] == 0)
 
 
-@pytest.mark.parametrize("N", [32, 64, 96])
+@pytest.mark.parametrize("N", [32, 64, 96, 128, 192])
 def test_volume_dft_matches_oracle(hip_device, N):
     import torch
     from gisapr_amd import kernels
@@ -64,6 +64,37 @@ def test_volume_dft_matches_oracle(hip_device, N):
     want = np.fft.ifftshift(opj.volume_dft(vol.astype(np.float64), np.float64), axes=(0, 1))
     assert got.shape == want.shape
     assert rel_l2(got, want) < 2e-6
+
+
+def test_volume_dft_impulse_box256(hip_device):
+    """Full-size check of the box-256 transform (a float64 oracle of the 512^3
+    padded volume is too slow here): a single voxel at data position
+    (a, b, c) has the closed-form spectrum
+        D[k] = s^2 * exp(-2 pi i (kz*Pz + ky*Py + kx*Px) / 2N),
+    P = (position + N/2 + N) mod 2N (pad, then fftshift), s = sinc(|f|) of
+    the voxel's padded frequency, which pins every index of the three passes
+    and the bricked store."""
+    import torch
+    from gisapr_amd import kernels
+    N, L = 256, 512
+    a, b, c = 37, 200, 131
+    say("volume_dft impulse", N, (a, b, c))
+    vol = torch.zeros((N, N, N), dtype=torch.float32, device=hip_device)
+    vol[a, b, c] = 1.0
+    got = kernels.volume_dft_natural(kernels.volume_dft(vol), N).cpu().numpy()
+    assert got.shape == (L, L, N + 1)
+    f = (np.array([a, b, c], np.float64) + N / 2 - N) / (2 * N)
+    r = np.sqrt(np.sum(f * f))
+    s2 = (np.sin(np.pi * r) / (np.pi * r)) ** 2
+    P = (np.array([a, b, c]) + N // 2 + N) % L
+    ez = np.exp(-2j * np.pi * np.arange(L) * P[0] / L)
+    ey = np.exp(-2j * np.pi * np.arange(L) * P[1] / L)
+    ex = np.exp(-2j * np.pi * np.arange(N + 1) * P[2] / L)
+    worst = 0.0
+    for z0 in range(0, L, 64):
+        want = s2 * ez[z0:z0 + 64, None, None] * ey[None, :, None] * ex[None, None, :]
+        worst = max(worst, float(np.abs(got[z0:z0 + 64] - want).max()))
+    assert worst < 2e-6 * s2, worst
 
 
 def _angles():
