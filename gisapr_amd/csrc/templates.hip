@@ -50,6 +50,26 @@ constexpr int kPad = 3;
 template <int L>
 constexpr int tpl_sp() { return lds_padded_len<L, kPad>() + 1; }
 
+// A strided tile loop of (at least) IT iterations whose loads are issued in
+// batches of UB before any of them is used: load(it) returns the iteration's
+// values and must stay in bounds for every it < round_up(IT, UB) (callers
+// clamp indices); store(it, v) consumes them and drops iterations past the
+// tile.  A scheduling barrier between a batch's loads and its uses keeps a
+// batch's loads in flight together whatever the register budget.
+template <int IT, int UB, typename LoadF, typename StoreF>
+__device__ __forceinline__ void batched_tile_loop(LoadF load, StoreF store) {
+  constexpr int ITP = (IT + UB - 1) / UB * UB;
+#pragma unroll 1
+  for (int i0 = 0; i0 < ITP; i0 += UB) {
+    decltype(load(0)) v[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) v[u] = load(i0 + u);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < UB; ++u) store(i0 + u, v[u]);
+  }
+}
+
 template <int N>
 struct TplLayout {
   static constexpr int L2 = 2 * N;       // padded length
@@ -76,6 +96,7 @@ struct TplLayout {
   static constexpr size_t U_ELEMS = (size_t)N * UT * TK1;  // float2
   static constexpr size_t W_ELEMS = (size_t)N * WT * TK3;  // float2
   static constexpr size_t SLOT = U_ELEMS + W_ELEMS;
+  static_assert(SLOT < (1u << 31), "32-bit scratch offsets");
 };
 
 template <int N>
@@ -100,11 +121,14 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
   fill_twiddles(tw1, N, tid, nthr);
   float2* U = scratch + (size_t)blockIdx.x * T::SLOT;
   float2* W = U + T::U_ELEMS;
-  auto uidx = [](int row, int col) -> size_t {
-    return ((size_t)(col / T::TK1) * N + row) * T::TK1 + col % T::TK1;
+  // 32-bit element offsets within the workgroup's scratch slot
+  auto uidx = [](int row, int col) -> unsigned {
+    const unsigned c = (unsigned)col;
+    return ((c / T::TK1) * N + (unsigned)row) * T::TK1 + c % T::TK1;
   };
-  auto widx = [](int row, int col) -> size_t {
-    return ((size_t)(col / T::TK3) * N + row) * T::TK3 + col % T::TK3;
+  auto widx = [](int row, int col) -> unsigned {
+    const unsigned c = (unsigned)col;
+    return ((c / T::TK3) * N + (unsigned)row) * T::TK3 + c % T::TK3;
   };
   float* Z = (float*)U;  // real image, aliases U once U is consumed
   const int mh_n = n / 2 + 1;
@@ -220,17 +244,29 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     for (int y0 = 0; y0 < N; y0 += 2 * T::RB2) {
       const int nrow = min(2 * T::RB2, N - y0);
       const int nseq = (nrow + 1) / 2;
-      for (int t = tid; t < nseq * L2; t += nthr) {
-        const int q = t / L2, k = t - q * L2;
-        const int ya = y0 + 2 * q, yb = ya + 1;
-        // Hermitian extension of the half spectrum, Im of columns 0 and N dropped
-        const int ks = k <= N ? k : L2 - k;
-        float2 a = U[uidx(ya, ks)];
-        float2 b = yb < y0 + nrow ? U[uidx(yb, ks)] : make_float2(0.f, 0.f);
-        if (ks == 0 || ks == N) { a.y = 0.f; b.y = 0.f; }
-        if (k > N) { a.y = -a.y; b.y = -b.y; }
-        buf[q * SP2 + pd(k)] = make_float2(a.x - b.y, a.y + b.x);  // a + i*b
-      }
+      // Tile loads: unconditional (rows clamped into the image, unused
+      // values dropped at the LDS store) and batched (batched_tile_loop).
+      constexpr int IT2 = (T::RB2 * L2 + kThreads - 1) / kThreads;
+      batched_tile_loop<IT2, 4>(
+          [&](int it) {
+            const int t = tid + it * kThreads;
+            const int q = t / L2, k = t - q * L2;
+            const int ya = min(y0 + 2 * q, N - 1), yb = min(y0 + 2 * q + 1, N - 1);
+            const int ks = k <= N ? k : L2 - k;
+            const float2 a = U[uidx(ya, ks)], b = U[uidx(yb, ks)];
+            return make_float4(a.x, a.y, b.x, b.y);
+          },
+          [&](int it, float4 ab) {
+            const int t = tid + it * kThreads;
+            const int q = t / L2, k = t - q * L2;
+            // Hermitian extension of the half spectrum, Im of columns 0 and N dropped
+            const int ks = k <= N ? k : L2 - k;
+            float2 a = make_float2(ab.x, ab.y), b = make_float2(ab.z, ab.w);
+            if (2 * q + 1 >= nrow) b = make_float2(0.f, 0.f);
+            if (ks == 0 || ks == N) { a.y = 0.f; b.y = 0.f; }
+            if (k > N) { a.y = -a.y; b.y = -b.y; }
+            if (q < nseq) buf[q * SP2 + pd(k)] = make_float2(a.x - b.y, a.y + b.x);  // a + i*b
+          });
       __syncthreads();
       // inverse 2N transform; each wave then crops the centre N (scaled) of
       // its own sequences to the front of their slots; forward N transform
@@ -283,11 +319,19 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     int* rcnt = (int*)(rsum + (size_t)MH * 2 * RS);
     for (int k0 = 0; k0 < MH; k0 += T::TK3) {
       const int nk = min(T::TK3, MH - k0);
-      for (int t = tid; t < T::TK3 * N; t += nthr) {
-        const int Y = t / T::TK3, q = t - Y * T::TK3;
-        if (q >= nk) continue;
-        buf[q * SP1 + pd(Y)] = W[widx(Y, k0 + q)];
-      }
+      constexpr int IT3 = (T::TK3 * N + kThreads - 1) / kThreads;
+      batched_tile_loop<IT3, 4>(
+          [&](int it) {
+            const int t = tid + it * kThreads;
+            const int Y = t / T::TK3, q = t - Y * T::TK3;
+            // W's last column tile is padded to TK3 columns: the load is in bounds
+            return W[widx(min(Y, N - 1), k0 + q)];
+          },
+          [&](int it, float2 v) {
+            const int t = tid + it * kThreads;
+            const int Y = t / T::TK3, q = t - Y * T::TK3;
+            if (Y < N && q < nk) buf[q * SP1 + pd(Y)] = v;
+          });
       __syncthreads();
       fft_lds_inplace<N, false, kPad>(buf, nk, SP1, tw1, tid, nthr);
       const float2* res = buf;
@@ -362,14 +406,22 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     // ---- phase 5: scale by ring divisor, inverse column FFT (N) -----------
     for (int k0 = 0; k0 < MH; k0 += T::TK3) {
       const int nk = min(T::TK3, MH - k0);
-      for (int t = tid; t < T::TK3 * N; t += nthr) {
-        const int ky = t / T::TK3, q = t - ky * T::TK3;
-        if (q >= nk) continue;
-        const int fy = ky < N / 2 ? ky : ky - N;
-        const float g = div[ring_of(fy, k0 + q)];
-        const float2 v = W[widx(ky, k0 + q)];
-        buf[q * SP1 + pd(ky)] = make_float2(v.x * g, v.y * g);
-      }
+      constexpr int IT5 = (T::TK3 * N + kThreads - 1) / kThreads;
+      batched_tile_loop<IT5, 4>(
+          [&](int it) {
+            const int t = tid + it * kThreads;
+            const int ky = min(t / T::TK3, N - 1), q = t - (t / T::TK3) * T::TK3;
+            return W[widx(ky, k0 + q)];
+          },
+          [&](int it, float2 v) {
+            const int t = tid + it * kThreads;
+            const int ky = t / T::TK3, q = t - ky * T::TK3;
+            if (ky < N && q < nk) {
+              const int fy = ky < N / 2 ? ky : ky - N;
+              const float g = div[ring_of(fy, k0 + q)];
+              buf[q * SP1 + pd(ky)] = make_float2(v.x * g, v.y * g);
+            }
+          });
       __syncthreads();
       fft_lds_inplace<N, true, kPad>(buf, nk, SP1, tw1, tid, nthr);
       for (int t = tid; t < T::TK3 * N; t += nthr) {
@@ -385,16 +437,26 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     for (int y0 = 0; y0 < N; y0 += 2 * T::RB6) {
       const int nrow = min(2 * T::RB6, N - y0);
       const int nseq = (nrow + 1) / 2;
-      for (int t = tid; t < nseq * N; t += nthr) {
-        const int q = t / N, k = t - q * N;
-        const int ya = y0 + 2 * q, yb = ya + 1;
-        const int ks = k <= N / 2 ? k : N - k;
-        float2 a = W[widx(ya, ks)];
-        float2 b = yb < y0 + nrow ? W[widx(yb, ks)] : make_float2(0.f, 0.f);
-        if (ks == 0 || ks == N / 2) { a.y = 0.f; b.y = 0.f; }
-        if (k > N / 2) { a.y = -a.y; b.y = -b.y; }
-        buf[q * SP1 + pd(k)] = make_float2(a.x - b.y, a.y + b.x);
-      }
+      constexpr int IT6 = (T::RB6 * N + kThreads - 1) / kThreads;
+      batched_tile_loop<IT6, 4>(
+          [&](int it) {
+            const int t = tid + it * kThreads;
+            const int q = t / N, k = t - q * N;
+            const int ya = min(y0 + 2 * q, N - 1), yb = min(y0 + 2 * q + 1, N - 1);
+            const int ks = k <= N / 2 ? k : N - k;
+            const float2 a = W[widx(ya, ks)], b = W[widx(yb, ks)];
+            return make_float4(a.x, a.y, b.x, b.y);
+          },
+          [&](int it, float4 ab) {
+            const int t = tid + it * kThreads;
+            const int q = t / N, k = t - q * N;
+            const int ks = k <= N / 2 ? k : N - k;
+            float2 a = make_float2(ab.x, ab.y), b = make_float2(ab.z, ab.w);
+            if (2 * q + 1 >= nrow) b = make_float2(0.f, 0.f);
+            if (ks == 0 || ks == N / 2) { a.y = 0.f; b.y = 0.f; }
+            if (k > N / 2) { a.y = -a.y; b.y = -b.y; }
+            if (q < nseq) buf[q * SP1 + pd(k)] = make_float2(a.x - b.y, a.y + b.x);
+          });
       __syncthreads();
       fft_lds_inplace<N, true, kPad>(buf, nseq, SP1, tw1, tid, nthr);
       const float sc = 1.0f / ((float)N * (float)N);
@@ -458,20 +520,28 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     for (int y0 = 0; y0 < N; y0 += 2 * T::RB6) {
       const int nrow = min(2 * T::RB6, N - y0);
       const int nseq = (nrow + 1) / 2;
-      for (int t = tid; t < nseq * N; t += nthr) {
-        const int q = t / N, X = t - q * N;
-        const int ya = y0 + 2 * q, yb = ya + 1;
-        float va = Z[(size_t)ya * N + X];
-        va = ((va - ((fa * (float)ya + fb * (float)X) + fc)) - fmean) * fis;
-        if (mask) va *= mask[(size_t)ya * N + X];
-        float vb = 0.f;
-        if (yb < y0 + nrow) {
-          vb = Z[(size_t)yb * N + X];
-          vb = ((vb - ((fa * (float)yb + fb * (float)X) + fc)) - fmean) * fis;
-          if (mask) vb *= mask[(size_t)yb * N + X];
-        }
-        buf[q * SP1 + pd(X)] = make_float2(va, vb);
-      }
+      constexpr int IT8 = (T::RB6 * N + kThreads - 1) / kThreads;
+      batched_tile_loop<IT8, 4>(
+          [&](int it) {
+            const int t = tid + it * kThreads;
+            const int q = t / N, X = t - q * N;
+            const int ya = min(y0 + 2 * q, N - 1), yb = min(y0 + 2 * q + 1, N - 1);
+            return make_float4(Z[ya * N + X], Z[yb * N + X], mask ? mask[ya * N + X] : 1.0f,
+                               mask ? mask[yb * N + X] : 1.0f);
+          },
+          [&](int it, float4 zm) {
+            const int t = tid + it * kThreads;
+            const int q = t / N, X = t - q * N;
+            const int ya = y0 + 2 * q, yb = ya + 1;
+            float va = ((zm.x - ((fa * (float)ya + fb * (float)X) + fc)) - fmean) * fis;
+            if (mask) va *= zm.z;
+            float vb = 0.f;
+            if (2 * q + 1 < nrow) {
+              vb = ((zm.y - ((fa * (float)yb + fb * (float)X) + fc)) - fmean) * fis;
+              if (mask) vb *= zm.w;
+            }
+            if (q < nseq) buf[q * SP1 + pd(X)] = make_float2(va, vb);
+          });
       __syncthreads();
       fft_lds_inplace<N, false, kPad>(buf, nseq, SP1, tw1, tid, nthr);
       for (int t = tid; t < nseq * MH; t += nthr) {
@@ -492,11 +562,18 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     const int kmax = n / 2;  // columns 0..n/2 are needed
     for (int k0 = 0; k0 <= kmax; k0 += T::TK3) {
       const int nk = min(T::TK3, kmax + 1 - k0);
-      for (int t = tid; t < T::TK3 * N; t += nthr) {
-        const int Y = t / T::TK3, q = t - Y * T::TK3;
-        if (q >= nk) continue;
-        buf[q * SP1 + pd(Y)] = W[widx(Y, k0 + q)];
-      }
+      constexpr int IT9 = (T::TK3 * N + kThreads - 1) / kThreads;
+      batched_tile_loop<IT9, 4>(
+          [&](int it) {
+            const int t = tid + it * kThreads;
+            const int Y = t / T::TK3, q = t - Y * T::TK3;
+            return W[widx(min(Y, N - 1), k0 + q)];
+          },
+          [&](int it, float2 v) {
+            const int t = tid + it * kThreads;
+            const int Y = t / T::TK3, q = t - Y * T::TK3;
+            if (Y < N && q < nk) buf[q * SP1 + pd(Y)] = v;
+          });
       __syncthreads();
       fft_lds_inplace<N, false, kPad>(buf, nk, SP1, tw1, tid, nthr);
       const float2* res = buf;
