@@ -111,16 +111,16 @@ def top1(res, ps, center):
     return out
 
 
-def precision_check(eng, ps, bank, res, subset: int, seed: int):
+def precision_check(eng, ps, tk, res, subset: int, seed: int):
     """Mixed-leg scores against the same truncated functional in float64
-    (torch complex128 on the device) on a seeded particle subset.  ``bank``
-    is (rows, 1/norms) of the candidate the mixed leg searched last."""
+    (torch complex128 on the device) on a seeded particle subset.  ``tk`` are
+    the template coefficients of the candidate the mixed leg searched last."""
     import torch
     tab = eng.small
     dev = eng.device
     rng = np.random.default_rng(seed)
     pick = np.sort(rng.choice(ps.B, min(subset, ps.B), replace=False))
-    tk = eng.template_coefficients(*bank)[:, :tab.K].to(torch.complex128)
+    tk = tk[:, :tab.K].to(torch.complex128)
     pk = ps.coeffs[:, :tab.K].to(torch.complex128)
     phi = torch.from_numpy(tab.phi).to(dev)
     off = ps.dir_off.cpu().numpy()
@@ -325,7 +325,7 @@ def main() -> int:
     setup_s = time.perf_counter() - t_setup
     log("setup", setup_s)
 
-    def run_single(search):
+    def run_single(make_templates, search):
         stages = ("density", "templates", "search")
         ev_time = {k: 0.0 for k in stages}
 
@@ -337,13 +337,13 @@ def main() -> int:
             vol = eng.density(xyz, numbers, wl.res)
             if timed:
                 ev[1].record()
-            bank, tinv = eng.templates(vol, normalize=False)
+            tpl = make_templates(vol)
             if timed:
                 ev[2].record()
-            res = search(bank, ps, tinv)
+            res = search(tpl, ps)
             if timed:
                 ev[3].record()
-            return res, (bank, tinv), ev
+            return res, tpl, ev
 
         for i in range(args.warmup):
             step(poses[i], False)
@@ -351,7 +351,7 @@ def main() -> int:
         evs = []
         t0 = time.perf_counter()
         for i in range(args.steps):
-            res, bank, ev = step(poses[args.warmup + i], True)
+            res, tpl, ev = step(poses[args.warmup + i], True)
             evs.append(ev)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
@@ -360,15 +360,19 @@ def main() -> int:
                 ev_time[name] += ev[k].elapsed_time(ev[k + 1])
         return {"scores_per_s": n_scores * args.steps / dt, "ms_per_candidate": dt * 1000.0 / args.steps,
                 "stage_ms": {k: v / args.steps for k, v in ev_time.items()},
-                "score": float(res.score.cpu())}, res, bank
+                "score": float(res.score.cpu())}, res, tpl
 
     want = set(args.legs.split(","))
     legs, results = {}, {}
-    bank = None
+    bank = tk_last = None  # the last candidate's (bank, 1/norms) / template coefficients
     if "fp32" in want:
-        legs["fp32"], results["fp32"], bank = run_single(eng.search)
+        # FP32 path: full unnormalised bank (1/norm folded into the search)
+        legs["fp32"], results["fp32"], bank = run_single(lambda v: eng.templates(v, normalize=False),
+                                                         lambda t, p: eng.search(t[0], p, t[1]))
     if "mixed" in want:
-        legs["mixed"], results["mixed"], bank = run_single(eng.search_small_shift)
+        # split-bf16 path: the template kernel writes the in-disk coefficients directly
+        legs["mixed"], results["mixed"], tk_last = run_single(eng.small_templates,
+                                                              lambda t, p: eng.search_small_shift(None, p, tk=t))
     log("single legs", {k: v["ms_per_candidate"] for k, v in legs.items()})
 
     # batched legs: ``slots`` candidates per step, graph-captured, autotuned search tile
@@ -463,11 +467,11 @@ def main() -> int:
             "orientation_and_shift": float(np.mean(np.all(a[valid] == b[valid], axis=1))),
             "particles": int(valid.sum())}
     if args.precision_subset > 0 and "mixed" in results:
-        out["precision"] = precision_check(eng, ps, bank, results["mixed"], args.precision_subset,
+        out["precision"] = precision_check(eng, ps, tk_last, results["mixed"], args.precision_subset,
                                            synthetic.SEED_CAND)
     if not args.no_cpu_baseline and args.cpu_sample > 0 and bank is not None:
         out["cpu_baseline"] = cpu_baseline(eng, ps, bank, args.cpu_sample)
-    del ps, eng, bank, results
+    del ps, eng, bank, tk_last, results
     torch.cuda.empty_cache()
     if args.pso_iters > 0:
         out["pso"] = pso_leg(args, dev, log)

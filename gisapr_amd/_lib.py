@@ -25,6 +25,8 @@ SIGNATURES = {
     "gis_volume_dft": (_i, [_vp, _i, _vp, _vp, _sz, _vp]),
     "gis_build_templates_workspace": (_sz, [_i, _i]),
     "gis_build_templates": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _sz, _vp]),
+    "gis_build_template_coefficients": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _sz,
+                                             _vp]),
     "gis_prepare_particles_workspace": (_sz, [_i, _i]),
     "gis_prepare_particles": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gis_ctf_packed": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),

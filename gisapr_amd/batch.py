@@ -277,11 +277,15 @@ class CandidateBatch:
         self.dft = torch.empty((int(kernels.lib().gis_volume_dft_bytes(N)) // 8,), dtype=torch.complex64, device=dev)
         self.dft_ws = torch.empty((int(kernels.lib().gis_volume_dft_workspace(N)),), dtype=torch.uint8, device=dev)
         if path == "mixed":
-            self.bank = torch.empty((M, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+            # template coefficients written straight from the template kernel (no bank)
+            self.bank = None
             self.tk = torch.empty((self.slots * M, engine.small.Kpad), dtype=torch.complex64, device=dev)
+            self.norms = torch.empty((self.slots * M,), dtype=torch.float32, device=dev)
         else:
             self.bank = torch.empty((self.slots * M, n, n // 2 + 1), dtype=torch.complex64, device=dev)
         self.tinv = torch.empty((self.slots * M,), dtype=torch.float32, device=dev)  # 1/norm per bank row
+        self.tpl_ws = torch.empty((max(int(kernels.lib().gis_build_templates_workspace(N, M)), 1),),
+                                  dtype=torch.uint8, device=dev)
         self._copied = torch.cuda.Event()
         self.autotuner = autotuner
         self.tile: dict = {}
@@ -296,13 +300,14 @@ class CandidateBatch:
                       self.volume)
         kernels.volume_dft(self.volume, out=self.dft, workspace=self.dft_ws)
         rows = slice(c * self.M, (c + 1) * self.M)
-        bank = self.bank if self.path == "mixed" else self.bank[rows]
-        _, norms = kernels.build_templates(self.dft, eng.n, eng.rts, eng.mask, eng.fsc_packed, out=bank,
+        if self.path == "mixed":
+            kernels.build_template_coefficients(self.dft, eng.n, eng.rts, eng.mask, eng.fsc_packed, eng.small_kslot,
+                                                eng.small.K, eng.small.Kpad, out=self.tk[rows],
+                                                norms=self.norms[rows], workspace=self.tpl_ws)
+            return
+        _, norms = kernels.build_templates(self.dft, eng.n, eng.rts, eng.mask, eng.fsc_packed, out=self.bank[rows],
                                            normalize=False)
         torch.reciprocal(norms, out=self.tinv[rows])
-        if self.path == "mixed":
-            kernels.gather_coefficients(self.bank, eng.small_kidx, eng.small.Kpad, self.tinv[rows],
-                                        out=self.tk[rows])
 
     def _search(self, tile: dict, k: Optional[int] = None):
         eng, ps = self.engine, self.ps
@@ -504,8 +509,7 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
             engine.prepare_small_shift(ps)
         tk = torch.empty((C * M, engine.small.Kpad), dtype=torch.complex64, device=engine.device)
         for c, vol in enumerate(volumes):
-            bank, tinv = engine.templates(vol, normalize=False)
-            kernels.gather_coefficients(bank, engine.small_kidx, engine.small.Kpad, tinv, out=tk[c * M:(c + 1) * M])
+            engine.small_templates(vol, out=tk[c * M:(c + 1) * M])
         corr = kernels.correlate_mixed(tk, ps.coeffs, ps.P, plan.dir_off, plan.dir_idx, engine.small_bhi,
                                        engine.small_blo, engine.small.half, validate=False, pmap=plan.pmap,
                                        pt=tile.get("pt", 0), wpb=tile.get("wpb", 0), plast=engine.small_plast)

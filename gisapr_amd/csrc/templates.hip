@@ -70,6 +70,22 @@ __device__ __forceinline__ void batched_tile_loop(LoadF load, StoreF store) {
   }
 }
 
+constexpr int largest_divisor_upto(int n, int cap) {
+  int d = cap < n ? cap : n;
+  while (d > 1 && n % d != 0) --d;
+  return d < 1 ? 1 : d;
+}
+
+// Optional coefficient output: instead of the packed bank row, direction d
+// writes the listed coefficients, coeffs[d * Kpad + kslot[packed index]] for
+// the packed indices with kslot >= 0 (K of them, slots [0, K)), scaled by
+// 1/norm once the norm is known, and zeros in slots [K, Kpad).
+struct CoefOut {
+  const int* kslot;  // [n * (n/2+1)], -1 = not kept; null = write the bank
+  int K, Kpad;
+  float2* coeffs;    // [M][Kpad]
+};
+
 template <int N>
 struct TplLayout {
   static constexpr int L2 = 2 * N;       // padded length
@@ -104,7 +120,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
 build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ rts, int M, int n,
                        const float* __restrict__ mask, const float* __restrict__ fsc,
                        float2* __restrict__ bank, float* __restrict__ norms, float2* __restrict__ scratch,
-                       int normalize) {
+                       int normalize, CoefOut co) {
   using T = TplLayout<N>;
   constexpr int L2 = T::L2, M2 = T::M2, MH = T::MH;
   extern __shared__ float2 lds[];  // kLdsFloat2 values
@@ -150,19 +166,24 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       // index first), so all eight loads of a sample are in flight together.
       constexpr int CW = T::TK1 <= 4 ? 4 : 8;  // columns per lane patch
       constexpr int RW = 64 / CW;              // rows per lane patch
-      static_assert((CW * L2) % (2 * kThreads) == 0, "two samples per thread and step");
+#ifndef GIS_TPL_SPS
+#define GIS_TPL_SPS 2
+#endif
+      // samples per thread and step (all their taps are issued together)
+      constexpr int SPS = largest_divisor_upto((CW * L2) / kThreads, GIS_TPL_SPS);
+      static_assert((CW * L2) % (SPS * kThreads) == 0, "whole steps per column group");
       const int ngroups = (nk + CW - 1) / CW;
       // (z0 + N) mod 2N for z0 in [0, 2N): the fftshift of the spectrum's z / y axes
       auto wrap = [](unsigned z) -> unsigned { return z >= (unsigned)N ? z - N : z + N; };
       for (int cg = 0; cg < ngroups; ++cg) {
-        for (int t0 = tid; t0 < CW * L2; t0 += 2 * kThreads) {
-          // two samples per step: all 16 taps are issued before any is used
-          unsigned ad[2][8];
-          float wt[2][7];  // tz, ty, tx, then the conjugation flag and the store slot
-          int slot[2];
-          bool cjs[2];
+        for (int t0 = tid; t0 < CW * L2; t0 += SPS * kThreads) {
+          // SPS samples per step: all their taps are issued before any is used
+          unsigned ad[SPS][8];
+          float wt[SPS][3];  // tz, ty, tx
+          int slot[SPS];
+          bool cjs[SPS];
 #pragma unroll
-          for (int u = 0; u < 2; ++u) {
+          for (int u = 0; u < SPS; ++u) {
             const int t = t0 + u * kThreads;
             const int lane = t & 63, rb = t >> 6;
             const int q = cg * CW + lane / RW, ip = rb * RW + lane % RW;
@@ -200,14 +221,14 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
           }
           // 32-bit byte offsets from the (scalar) base: the spectrum is < 4 GiB
           const char* Db = reinterpret_cast<const char*>(D);
-          float2 v[2][8];
+          float2 v[SPS][8];
 #pragma unroll
-          for (int u = 0; u < 2; ++u)
+          for (int u = 0; u < SPS; ++u)
 #pragma unroll
             for (int i = 0; i < 8; ++i) v[u][i] = *reinterpret_cast<const float2*>(Db + (size_t)(ad[u][i] * 8u));
           __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-          for (int u = 0; u < 2; ++u) {
+          for (int u = 0; u < SPS; ++u) {
             const float tz = wt[u][0], ty = wt[u][1], tx = wt[u][2];
             const float wx0 = 1.0f - tx, wx1 = tx;
             const float w00 = (1.0f - tz) * (1.0f - ty), w01 = (1.0f - tz) * ty;
@@ -557,7 +578,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     }
 
     // ---- phase 9: forward column FFT (N), crop to n, pack, FSC -----------
-    float2* out = bank + (size_t)d * n * mh_n;
+    float2* out = co.kslot ? nullptr : bank + (size_t)d * n * mh_n;
     double nrm[1] = {0.0};
     const int kmax = n / 2;  // columns 0..n/2 are needed
     for (int k0 = 0; k0 <= kmax; k0 += T::TK3) {
@@ -592,7 +613,12 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         const float g = fsc ? fsc[(size_t)r * mh_n + c] : 1.0f;
         v.x *= g;
         v.y *= g;
-        out[(size_t)r * mh_n + c] = v;
+        if (co.kslot) {
+          const int j = co.kslot[r * mh_n + c];
+          if (j >= 0) co.coeffs[(size_t)d * co.Kpad + j] = v;
+        } else {
+          out[(size_t)r * mh_n + c] = v;
+        }
         nrm[0] += (double)v.x * v.x + (double)v.y * v.y;
       }
       __syncthreads();
@@ -602,11 +628,25 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     // caller applies it later, only the norm) ------------------------------
     block_sum_f64<1>(nrm, red);
     const float norm = (float)sqrt(nrm[0]);
-    if (normalize)
+    if (co.kslot) {
+      // the phase-9 stores of every thread are visible after block_sum_f64's
+      // barriers; 1/norm rounded once, as the reciprocal a gather would apply
+      const float sc = 1.0f / norm;
+      float2* cr = co.coeffs + (size_t)d * co.Kpad;
+      for (int j = tid; j < co.Kpad; j += nthr) {
+        float2 v = make_float2(0.f, 0.f);
+        if (j < co.K) {
+          v = cr[j];
+          v = make_float2(v.x * sc, v.y * sc);
+        }
+        cr[j] = v;
+      }
+    } else if (normalize) {
       for (int t = tid; t < n * mh_n; t += nthr) {
         float2 v = out[t];
         out[t] = make_float2(v.x / norm, v.y / norm);
       }
+    }
     if (norms && tid == 0) norms[d] = norm;
   }
 }
@@ -614,7 +654,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
 template <int N>
 int launch_build_templates(const float2* D, const float* rts, int M, int n, const float* mask,
                            const float* fsc, float2* bank, float* norms, int normalize, void* ws, size_t ws_bytes,
-                           hipStream_t st) {
+                           hipStream_t st, CoefOut co) {
   using T = TplLayout<N>;
   const size_t slot = T::SLOT * sizeof(float2);
   int grid = (int)std::min<size_t>((size_t)M, ws_bytes / slot);
@@ -633,7 +673,7 @@ int launch_build_templates(const float2* D, const float* rts, int M, int n, cons
     attr_set = true;
   }
   hipLaunchKernelGGL(build_templates_kernel<N>, dim3(grid), dim3(kThreads), lds, st, D, rts, M, n, mask, fsc,
-                     bank, norms, (float2*)ws, normalize);
+                     bank, norms, (float2*)ws, normalize, co);
   GIS_LAUNCH_CHECK("gis_build_templates");
   return GIS_OK;
 }
@@ -672,6 +712,25 @@ GIS_API size_t gis_build_templates_workspace(int N, int M) {
 //   fsc   [n][n/2+1] float32 packed FSC weights, or NULL (all ones)
 //   bank  [M][n][n/2+1] complex64 output (unit L2 norm, UNshifted rfft2 layout)
 //   norms [M] float32 output of the pre-normalisation norms, or NULL
+static int build_templates_dispatch(const void* dft, int N, int n, const float* rts, int M, const float* mask,
+                                    const float* fsc, void* bank, float* norms, int normalize, void* ws,
+                                    size_t ws_bytes, void* stream, CoefOut co) {
+  hipStream_t st = (hipStream_t)stream;
+  const float2* D = (const float2*)dft;
+  float2* B = (float2*)bank;
+  switch (N) {
+    case 32: return launch_build_templates<32>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, co);
+    case 64: return launch_build_templates<64>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, co);
+    case 96: return launch_build_templates<96>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, co);
+    case 128: return launch_build_templates<128>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, co);
+    case 192: return launch_build_templates<192>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, co);
+    case 256: return launch_build_templates<256>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, co);
+    default:
+      gis_set_error("gis_build_templates: box %d not in {32, 64, 96, 128, 192, 256}", N);
+      return GIS_EUNSUPPORTED;
+  }
+}
+
 GIS_API int gis_build_templates(const void* dft, int N, int n, const float* rts, int M, const float* mask,
                                 const float* fsc, void* bank, float* norms, int normalize, void* ws,
                                 size_t ws_bytes, void* stream) {
@@ -680,18 +739,25 @@ GIS_API int gis_build_templates(const void* dft, int N, int n, const float* rts,
   GIS_REQUIRE(M >= 0, "gis_build_templates: M < 0");
   GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0, "gis_build_templates: newbox %d must be even and <= box %d", n, N);
   if (M == 0) return GIS_OK;
-  hipStream_t st = (hipStream_t)stream;
-  const float2* D = (const float2*)dft;
-  float2* B = (float2*)bank;
-  switch (N) {
-    case 32: return launch_build_templates<32>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st);
-    case 64: return launch_build_templates<64>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st);
-    case 96: return launch_build_templates<96>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st);
-    case 128: return launch_build_templates<128>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st);
-    case 192: return launch_build_templates<192>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st);
-    case 256: return launch_build_templates<256>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st);
-    default:
-      gis_set_error("gis_build_templates: box %d not in {32, 64, 96, 128, 192, 256}", N);
-      return GIS_EUNSUPPORTED;
-  }
+  return build_templates_dispatch(dft, N, n, rts, M, mask, fsc, bank, norms, normalize, ws, ws_bytes, stream,
+                                  CoefOut{nullptr, 0, 0, nullptr});
+}
+
+// The template bank reduced to the listed coefficients of every direction
+// (the small-shift search's template operand): same transform as
+// gis_build_templates, but direction d writes only
+//   coeffs[d * Kpad + kslot[r * (n/2+1) + c]] = bank[d][r][c] / norm_d
+// for the packed entries with kslot >= 0 (slots [0, K)), zeros in [K, Kpad),
+// and norms[d] (optional), without storing the bank.
+GIS_API int gis_build_template_coefficients(const void* dft, int N, int n, const float* rts, int M,
+                                            const float* mask, const float* fsc, const int* kslot, int K, int Kpad,
+                                            void* coeffs, float* norms, void* ws, size_t ws_bytes, void* stream) {
+  GIS_REQUIRE(dft && rts && kslot && coeffs && ws, "gis_build_template_coefficients: null pointer");
+  GIS_REQUIRE(M >= 0, "gis_build_template_coefficients: M < 0");
+  GIS_REQUIRE(K >= 0 && Kpad >= K, "gis_build_template_coefficients: K=%d Kpad=%d", K, Kpad);
+  GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0, "gis_build_template_coefficients: newbox %d must be even and <= box %d",
+              n, N);
+  if (M == 0) return GIS_OK;
+  return build_templates_dispatch(dft, N, n, rts, M, mask, fsc, nullptr, norms, 0, ws, ws_bytes, stream,
+                                  CoefOut{kslot, K, Kpad, (float2*)coeffs});
 }

@@ -186,6 +186,7 @@ class SearchEngine:
         dev = self.device
         self.small = tab
         self.small_kidx = torch.from_numpy(tab.kidx).to(dev)
+        self.small_kslot = torch.from_numpy(tab.kslot).to(dev)
         self.small_bhi = torch.from_numpy(tab.b_hi.view(np.int16)).to(dev)
         self.small_blo = torch.from_numpy(tab.b_lo.view(np.int16)).to(dev)
         self.small_plast = torch.from_numpy(tab.phi_last).to(dev)
@@ -206,18 +207,32 @@ class SearchEngine:
         return ps
 
     def template_coefficients(self, bank: torch.Tensor, tinv: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """The in-disk coefficients of a bank (times 1/norm per row when given)."""
         self._require_small()
         return kernels.gather_coefficients(bank, self.small_kidx, self.small.Kpad, tinv)
 
-    def search_small_shift(self, bank: torch.Tensor, ps: ParticleSet,
-                           tinv: Optional[torch.Tensor] = None) -> SearchResult:
-        """Best lag in [-half, half]^2 per (direction, psi) on split-bf16 MFMA."""
+    def small_templates(self, volume: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """[M, Kpad] unit-norm template coefficients of a model volume, built
+        without storing the bank (the small-shift search's template operand)."""
+        self._require_small()
+        dft = kernels.volume_dft(volume)
+        tk, _ = kernels.build_template_coefficients(dft, self.n, self.rts, self.mask, self.fsc_packed,
+                                                    self.small_kslot, self.small.K, self.small.Kpad, out=out)
+        return tk
+
+    def search_small_shift(self, bank: Optional[torch.Tensor], ps: ParticleSet,
+                           tinv: Optional[torch.Tensor] = None, tk: Optional[torch.Tensor] = None) -> SearchResult:
+        """Best lag in [-half, half]^2 per (direction, psi) on split-bf16 MFMA,
+        from a bank (gathered here) or from template coefficients ``tk``."""
         self._require_small()
         if ps.coeffs is None:
             self.prepare_small_shift(ps)
-        if int(bank.shape[0]) != len(self.dir_rot):
-            raise ValueError("template bank does not cover the engine's direction table")
-        tk = self.template_coefficients(bank, tinv)
+        if tk is None:
+            if int(bank.shape[0]) != len(self.dir_rot):
+                raise ValueError("template bank does not cover the engine's direction table")
+            tk = self.template_coefficients(bank, tinv)
+        elif int(tk.shape[0]) != len(self.dir_rot):
+            raise ValueError("template coefficients do not cover the engine's direction table")
         if getattr(ps, "row_order", None) is None:
             from .batch import orientation_rank
             off = ps.dir_off.cpu().numpy()

@@ -251,6 +251,10 @@ class SmallShiftTables:
         self.kidx = np.flatnonzero(inside).astype(np.int32)
         self.K = int(self.kidx.size)
         self.Kpad = max(16, -(-self.K // 16) * 16)
+        # inverse map: slot of each packed entry (-1 = not kept), [n][n/2+1]
+        self.kslot = np.full(n * m, -1, np.int32)
+        self.kslot[self.kidx] = np.arange(self.K, dtype=np.int32)
+        self.kslot = self.kslot.reshape(n, m)
         rows, cols = self.kidx // m, self.kidx % m
         pos = (cols + m // 2) % m
         wt = np.where((pos == 0) | (pos == n // 2), 1.0, 2.0)

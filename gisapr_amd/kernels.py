@@ -112,6 +112,57 @@ def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Op
     return (bank, norms) if with_norms else bank
 
 
+def build_template_coefficients(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Optional[torch.Tensor],
+                                fsc_packed: Optional[torch.Tensor], kslot: torch.Tensor, K: int, kpad: int,
+                                out: Optional[torch.Tensor] = None, norms: Optional[torch.Tensor] = None,
+                                workspace: Optional[torch.Tensor] = None):
+    """[M, kpad] complex64: the unit-norm bank's coefficients listed by ``kslot``
+    ([n, n/2+1] int32, slot or -1), zero padded, without storing the bank;
+    returns (coefficients, norms)."""
+    require_hip(dft, "dft")
+    N = next((b for b in SUPPORTED_BOXES if int(lib().gis_volume_dft_bytes(b)) == dft.numel() * 8), 0)
+    if not N or dft.dtype != torch.complex64:
+        raise ValueError("dft must be a bricked spectrum from volume_dft")
+    M = int(rts.shape[0])
+    require_hip(rts, "rts")
+    require_hip(kslot, "kslot")
+    n = int(new_box)
+    if tuple(kslot.shape) != (n, n // 2 + 1) or kslot.dtype != torch.int32:
+        raise ValueError("kslot must be int32 [n, n/2+1]")
+    if not 0 <= int(K) <= int(kpad):
+        raise ValueError("need 0 <= K <= kpad")
+    if out is None:
+        out = torch.empty((M, int(kpad)), dtype=torch.complex64, device=dft.device)
+    else:
+        require_hip(out, "out")
+        if tuple(out.shape) != (M, int(kpad)) or out.dtype != torch.complex64 or not out.is_contiguous():
+            raise ValueError("out must be contiguous complex64 [M, kpad]")
+    if norms is None:
+        norms = torch.empty((M,), dtype=torch.float32, device=dft.device)
+    elif norms.numel() != M or norms.dtype != torch.float32:
+        raise ValueError("norms must be float32 [M]")
+    if mask is not None:
+        require_hip(mask, "mask")
+        if tuple(mask.shape) != (N, N):
+            raise ValueError("mask must be [N, N]")
+    if fsc_packed is not None:
+        require_hip(fsc_packed, "fsc_packed")
+        if tuple(fsc_packed.shape) != (n, n // 2 + 1):
+            raise ValueError("fsc_packed must be [n, n/2+1]")
+    wsb = int(lib().gis_build_templates_workspace(N, M))
+    if workspace is not None:
+        require_hip(workspace, "workspace")
+        if workspace.numel() * workspace.element_size() < wsb:
+            raise ValueError(f"workspace must hold {wsb} bytes")
+        ws = workspace
+    else:
+        ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dft.device)
+    check(lib().gis_build_template_coefficients(ptr(dft), N, n, ptr(rts), M, ptr(mask), ptr(fsc_packed), ptr(kslot),
+                                                int(K), int(kpad), ptr(out), ptr(norms), ptr(ws), wsb,
+                                                stream_of(dft.device)), "gis_build_template_coefficients")
+    return out, norms
+
+
 def prepare_particles(images: torch.Tensor, new_box: int, rotp: torch.Tensor, mask: Optional[torch.Tensor],
                       ctf_tab: Optional[torch.Tensor], defocus: Optional[torch.Tensor], kconst: np.ndarray,
                       out: Optional[torch.Tensor] = None, inv_norm: Optional[torch.Tensor] = None):

@@ -84,6 +84,23 @@ int gis_build_templates(const void* dft, int N, int n, const float* rts, int M, 
                         const float* fsc, void* bank, float* norms, int normalize, void* ws, size_t ws_bytes,
                         void* stream);
 
+/*
+ * (c -> e boundary) The template bank reduced to the small-shift search's
+ * operand: the transform of gis_build_templates (same arguments and
+ * workspace), but direction d stores only the listed packed coefficients,
+ *   coeffs[d * Kpad + kslot[r * (n/2+1) + c]] = bank[d][r][c] / norm_d
+ * for the entries with kslot >= 0 (slots [0, K)), zeros in [K, Kpad), and
+ * norms[d] (or NULL) -- the bank itself is never written.  Equals
+ * gis_gather_coefficients(bank with normalize 0, scale 1/norms) bit for bit.
+ * Replaces the template side of correlate_templates_and_particle's inputs
+ * (v606:1131-1146 bank rows, restricted to the kept coefficients).
+ *   kslot [n][n/2+1] int32: slot of each packed entry, -1 if not kept
+ *   coeffs [M][Kpad] complex64 out
+ */
+int gis_build_template_coefficients(const void* dft, int N, int n, const float* rts, int M, const float* mask,
+                                    const float* fsc, const int* kslot, int K, int Kpad, void* coeffs, float* norms,
+                                    void* ws, size_t ws_bytes, void* stream);
+
 /* Workspace bytes gis_prepare_particles needs for `particles` (= B) particles. */
 size_t gis_prepare_particles_workspace(int N, int particles);
 

@@ -27,7 +27,8 @@ def library():
 
 def test_header_declares_the_path():
     names = declared_functions()
-    for n in ("gis_splat", "gis_volume_dft", "gis_build_templates", "gis_prepare_particles",
+    for n in ("gis_splat", "gis_volume_dft", "gis_build_templates", "gis_build_template_coefficients",
+              "gis_prepare_particles",
               "gis_correlate", "gis_score_sum", "gis_last_error"):
         assert n in names
 
@@ -50,6 +51,9 @@ def test_version_and_argument_errors_without_gpu(library):
     assert rc == -2 and b"crop" in library.gis_last_error()
     rc = library.gis_volume_dft(None, 32, None, None, 0, None)
     assert rc == -1
+    rc = library.gis_build_template_coefficients(None, 32, 32, None, 1, None, None, None, 0, 16, None, None, None,
+                                                 0, None)
+    assert rc == -1 and b"null" in library.gis_last_error()
     assert library.gis_volume_dft_bytes(256) == 16 * 256 * 256 * 65 * 8
     # N data planes x 2N rows x 33 groups of 8 kx (257 padded to 264)
     assert library.gis_volume_dft_workspace(256) == 256 * 512 * 264 * 8

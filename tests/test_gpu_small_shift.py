@@ -237,6 +237,42 @@ def test_small_shift_end_to_end_matches_oracle(hip_device, name, resolution):
     assert abs(float(res.score.cpu()) - s_want) <= TOL * s_want
 
 
+@pytest.mark.parametrize("name", ["box32", "box64", "box256"])
+def test_template_coefficients_equal_gathered_bank(hip_device, name):
+    """The coefficient output of the template kernel equals the gather of the
+    unnormalised bank scaled by 1/norm, bit for bit, with zero padding; at
+    box 256 over a 96-direction subset of the 768."""
+    import torch
+    from gisapr_amd import kernels, synthetic
+    from gisapr_amd.engine import SearchEngine
+    wl = synthetic.WORKLOADS[name]
+    data = synthetic.make_workload(name, n_particles=4)
+    say("template coefficients", name)
+    eng = SearchEngine(box=wl.box, apix=wl.apix, crop=wl.crop, psi_step=wl.psi_step, local_range=wl.local_range,
+                       kk=wl.kk, mask_radius=wl.mask_radius, mask_edge=wl.mask_edge, fsc_values=data["fsc"],
+                       dir_rot=data["dir_rot"], dir_tilt=data["dir_tilt"], device=hip_device)
+    tab = eng.enable_small_shift(resolution=4.0 if wl.box == 256 else 8.0, half=3)
+    vol = eng.density(data["coords"], data["numbers"], wl.res)
+    rts = eng.rts[::8].contiguous() if wl.box == 256 else eng.rts
+    dft = kernels.volume_dft(vol)
+    bank, norms = kernels.build_templates(dft, eng.n, rts, eng.mask, eng.fsc_packed, normalize=False)
+    want = kernels.gather_coefficients(bank, eng.small_kidx, tab.Kpad, 1.0 / norms)
+    got, gnorms = kernels.build_template_coefficients(dft, eng.n, rts, eng.mask, eng.fsc_packed, eng.small_kslot,
+                                                      tab.K, tab.Kpad)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(gnorms.cpu().numpy(), norms.cpu().numpy())
+    g = got.cpu().numpy()
+    np.testing.assert_array_equal(g, want.cpu().numpy())
+    assert not g[:, tab.K:].any()
+    # written into a strided slice of a larger buffer (the batched layout)
+    big = torch.full((2 * rts.shape[0], tab.Kpad), 7.0 + 7.0j, dtype=torch.complex64, device=hip_device)
+    kernels.build_template_coefficients(dft, eng.n, rts, eng.mask, eng.fsc_packed, eng.small_kslot, tab.K,
+                                        tab.Kpad, out=big[rts.shape[0]:])
+    b = big.cpu().numpy()
+    np.testing.assert_array_equal(b[rts.shape[0]:], g)
+    assert np.all(b[:rts.shape[0]] == 7.0 + 7.0j)
+
+
 def test_small_shift_box256_subset_precision(hip_device):
     """Box 256 at 1.06 A/px, 4 A disk (K ~ 7.2k), 49 lags, seeded 256-particle
     subset: the split-bf16 path against the same functional in FP64 on the

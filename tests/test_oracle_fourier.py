@@ -154,6 +154,10 @@ def test_small_shift_host_tables_match_oracle():
     for n, R, half in [(32, 9.0, 3), (64, 20.5, 2), (256, 67.84, 3)]:
         t = host.SmallShiftTables(n, R, half)
         np.testing.assert_array_equal(t.kidx, opr.disk_indices(n, R))
+        # kslot is the inverse of kidx on the packed plane, -1 elsewhere
+        assert t.kslot.shape == (n, n // 2 + 1) and t.kslot.dtype == np.int32
+        np.testing.assert_array_equal(t.kslot.reshape(-1)[t.kidx], np.arange(t.K))
+        assert (t.kslot >= 0).sum() == t.K
         phi = opr.phi_table(n, t.kidx.astype(np.int64), half)
         np.testing.assert_allclose(t.phi, phi, atol=1e-12)
         tiles, ks = t.b_hi.shape[:2]
