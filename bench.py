@@ -417,7 +417,7 @@ def main() -> int:
         fps = mixed_flops_per_score(tab.K, tab.n_lags)
         ach = fps * n_scores / (ms * 1e-3) / 1e12
         legs["mixed"]["roofline"] = {"bound": "mfma",
-                                     "kernel": "mixed_correlate_kernel (+ template gather, tail)",
+                                     "kernel": "mixed_correlate_kernel (+ particle tail)",
                                      "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
                                      "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps,
                                      "launch_ms": ms, "traffic": profiled_traffic("mixed_correlate_kernel", wl.name)}
