@@ -150,7 +150,17 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
   const int mh_n = n / 2 + 1;
   const float inv_l2 = 1.0f / (float)L2;
 
-  for (int d = blockIdx.x; d < M; d += gridDim.x) {
+#ifndef GIS_TPL_XCD_ORDER
+#define GIS_TPL_XCD_ORDER 1
+#endif
+  // Workgroups are dispatched to the 8 XCDs round robin; with the XCD order
+  // the first round gives each XCD a contiguous range of directions.
+  int d_first = blockIdx.x;
+  if (GIS_TPL_XCD_ORDER) {
+    const int g8 = gridDim.x / 8;
+    if ((int)blockIdx.x < 8 * g8) d_first = (blockIdx.x % 8) * g8 + blockIdx.x / 8;
+  }
+  for (int d = d_first; d < M; d += gridDim.x) {
     const float* R = rts + 9 * d;
     const float r00 = R[0], r01 = R[1], r10 = R[3], r11 = R[4], r20 = R[6], r21 = R[7];
     __syncthreads();
@@ -406,17 +416,41 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     }
 
     // ---- phase 4: ring divisors ------------------------------------------
+    // One thread per ring adds the ring's partials in a fixed order (kx
+    // ascending, then the half column).  The partials are in scratch, so the
+    // loads go out unconditionally in batches of KB columns (offsets clamped
+    // into the table, unused values masked afterwards) rather than as one
+    // dependent load per step.
     if (tid < nr) {
       const int ring = tid;
       float ring_sum = 0.0f;
       int ring_cnt = 0;
-      for (int kx = 0; kx <= min(ring, N / 2); ++kx) {
+      constexpr int KB = 8;
+      const int kxmax = min(ring, N / 2);
+      for (int kx0 = 0; kx0 <= kxmax; kx0 += KB) {
+        float sv[KB][2];
+        int cv[KB][2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int lo = ring_of(h ? -1 : 0, kx), hi = ring_of(h ? -N / 2 : N / 2 - 1, kx);
-          if (ring >= lo && ring <= hi) {
-            ring_sum += rsum[(size_t)(kx * 2 + h) * RS + ring - kx];
-            ring_cnt += rcnt[(size_t)(kx * 2 + h) * RS + ring - kx];
+        for (int u = 0; u < KB; ++u) {
+          const int kx = min(kx0 + u, N / 2);
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int off = (kx * 2 + h) * RS + min(max(ring - kx, 0), RS - 1);
+            sv[u][h] = rsum[off];
+            cv[u][h] = rcnt[off];
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < KB; ++u) {
+          const int kx = kx0 + u;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int lo = ring_of(h ? -1 : 0, kx), hi = ring_of(h ? -N / 2 : N / 2 - 1, kx);
+            if (kx <= kxmax && ring >= lo && ring <= hi) {
+              ring_sum += sv[u][h];
+              ring_cnt += cv[u][h];
+            }
           }
         }
       }
