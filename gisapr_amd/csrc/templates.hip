@@ -47,8 +47,16 @@ __device__ __forceinline__ int ring_of(int fy, int kx) {
 // Sequences live in LDS at buf[q * SP + lds_pad<kPad>(i)] and are
 // transformed in place, one wave per sequence (one buffer, no ping-pong).
 constexpr int kPad = 3;
+// Sequence strides in LDS (float2): the padded length plus a skew chosen so
+// the transposing tile loops and the gather's LDS stores spread over banks.
+#ifndef GIS_TPL_SP2X
+#define GIS_TPL_SP2X 1
+#endif
+#ifndef GIS_TPL_SP1X
+#define GIS_TPL_SP1X 1
+#endif
 template <int L>
-constexpr int tpl_sp() { return lds_padded_len<L, kPad>() + 1; }
+constexpr int tpl_sp(int skew) { return lds_padded_len<L, kPad>() + skew; }
 
 // A strided tile loop of (at least) IT iterations whose loads are issued in
 // batches of UB before any of them is used: load(it) returns the iteration's
@@ -94,8 +102,8 @@ struct TplLayout {
   static constexpr int NR = (int)(0.7072 * N) + 3;  // rings (upper bound)
   static constexpr int FIXED = L2 + N + NR / 2 + 1 + 32;  // twiddles, ring divisors, reductions
   static constexpr int CAP = kLdsFloat2 - FIXED;           // sequence buffer
-  static constexpr int SP2 = tpl_sp<L2>();                 // stride of a 2N sequence
-  static constexpr int SP1 = tpl_sp<N>();                  // stride of an N sequence
+  static constexpr int SP2 = tpl_sp<L2>(GIS_TPL_SP2X);     // stride of a 2N sequence
+  static constexpr int SP1 = tpl_sp<N>(GIS_TPL_SP1X);      // stride of an N sequence
   // sequence counts per tile are rounded down to a multiple of the waves
   // (each wave owns whole sequences of a tile)
   static constexpr int roundw(int x) { return x >= kWavesT ? x - x % kWavesT : x; }
@@ -632,29 +640,40 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       __syncthreads();
       fft_lds_inplace<N, false, kPad>(buf, nk, SP1, tw1, tid, nthr);
       const float2* res = buf;
-      for (int t = tid; t < T::TK3 * n; t += nthr) {
-        const int r = t / T::TK3, q = t - r * T::TK3;
-        if (q >= nk) continue;
-        const int c = k0 + q;
-        const int fy = r < n / 2 ? r : r - n;
-        float2 v;
-        if (c < n / 2 || n == N) {
-          v = res[q * SP1 + pd(fy < 0 ? fy + N : fy)];
-        } else {  // packed column n/2 holds frequency -n/2 = conj of (+n/2) at -fy
-          const int my = -fy;
-          v = cconj(res[q * SP1 + pd(my < 0 ? my + N : my)]);
-        }
-        const float g = fsc ? fsc[(size_t)r * mh_n + c] : 1.0f;
-        v.x *= g;
-        v.y *= g;
-        if (co.kslot) {
-          const int j = co.kslot[r * mh_n + c];
-          if (j >= 0) co.coeffs[(size_t)d * co.Kpad + j] = v;
-        } else {
-          out[(size_t)r * mh_n + c] = v;
-        }
-        nrm[0] += (double)v.x * v.x + (double)v.y * v.y;
-      }
+      // the FSC weight (and, for coefficient output, the slot) of each packed
+      // entry is read ahead in batches; the per-thread norm adds in t order
+      constexpr int IT9S = (T::TK3 * N + kThreads - 1) / kThreads;  // covers TK3 * n (n <= N)
+      batched_tile_loop<IT9S, 4>(
+          [&](int it) {
+            const int t = tid + it * kThreads;
+            const int r = min(t / T::TK3, n - 1), c = min(k0 + t % T::TK3, mh_n - 1);
+            const float g = fsc ? fsc[r * mh_n + c] : 1.0f;
+            const int j = co.kslot ? co.kslot[r * mh_n + c] : -1;
+            return make_float2(g, __int_as_float(j));
+          },
+          [&](int it, float2 gj) {
+            const int t = tid + it * kThreads;
+            const int r = t / T::TK3, q = t - r * T::TK3;
+            if (t >= T::TK3 * n || q >= nk) return;
+            const int c = k0 + q;
+            const int fy = r < n / 2 ? r : r - n;
+            float2 v;
+            if (c < n / 2 || n == N) {
+              v = res[q * SP1 + pd(fy < 0 ? fy + N : fy)];
+            } else {  // packed column n/2 holds frequency -n/2 = conj of (+n/2) at -fy
+              const int my = -fy;
+              v = cconj(res[q * SP1 + pd(my < 0 ? my + N : my)]);
+            }
+            v.x *= gj.x;
+            v.y *= gj.x;
+            if (co.kslot) {
+              const int j = __float_as_int(gj.y);
+              if (j >= 0) co.coeffs[(size_t)d * co.Kpad + j] = v;
+            } else {
+              out[(size_t)r * mh_n + c] = v;
+            }
+            nrm[0] += (double)v.x * v.x + (double)v.y * v.y;
+          });
       __syncthreads();
     }
 
@@ -667,13 +686,16 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       // barriers; 1/norm rounded once, as the reciprocal a gather would apply
       const float sc = 1.0f / norm;
       float2* cr = co.coeffs + (size_t)d * co.Kpad;
-      for (int j = tid; j < co.Kpad; j += nthr) {
-        float2 v = make_float2(0.f, 0.f);
-        if (j < co.K) {
-          v = cr[j];
-          v = make_float2(v.x * sc, v.y * sc);
+      for (int j0 = tid; j0 < co.Kpad; j0 += 4 * nthr) {  // loads in batches of 4, clamped into the row
+        float2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = cr[co.K > 0 ? min(j0 + u * nthr, co.K - 1) : 0];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = j0 + u * nthr;
+          if (j < co.Kpad) cr[j] = j < co.K ? make_float2(v[u].x * sc, v[u].y * sc) : make_float2(0.f, 0.f);
         }
-        cr[j] = v;
       }
     } else if (normalize) {
       for (int t = tid; t < n * mh_n; t += nthr) {
