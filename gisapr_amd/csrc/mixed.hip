@@ -22,6 +22,8 @@
 #include "common.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 namespace {
@@ -48,12 +50,19 @@ gather_coeffs_kernel(const float2* __restrict__ src, size_t plane, int rows, con
   }
 }
 
+// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi), both round to nearest
+// even; pairs of values go through one packed conversion each way.
 __device__ __forceinline__ void split8(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const __bf16 h = (__bf16)x[e];
-    hi[e] = h;
-    lo[e] = (__bf16)(x[e] - (float)h);
+  for (int e = 0; e < 8; e += 2) {
+    const f32x2 v = {x[e], x[e + 1]};
+    const bf16x2 h = __builtin_convertvector(v, bf16x2);
+    const f32x2 hf = __builtin_convertvector(h, f32x2);
+    const bf16x2 l = __builtin_convertvector(v - hf, bf16x2);
+    hi[e] = h[0];
+    hi[e + 1] = h[1];
+    lo[e] = l[0];
+    lo[e + 1] = l[1];
   }
 }
 
@@ -134,19 +143,51 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
   store(0);
   __syncthreads();
 
+#ifndef GIS_MIXED_PREFETCH
+#define GIS_MIXED_PREFETCH 1
+#endif
+  // This lane's template / particle / last-lag coefficients of one k-step.
+  struct KStep {
+    float4 t0, t1, f0, f1;
+    float4 u[PT][2];
+  };
+  auto load_ks = [&](int ks, KStep& o) {
+    const int q4 = ks * 8 + 2 * g;  // float4 index of this lane's 4 coefficients
+    o.t0 = trow[q4];
+    o.t1 = trow[q4 + 1];
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+      o.u[p][0] = prow[p][q4];
+      o.u[p][1] = prow[p][q4 + 1];
+    }
+    if (VL) {
+      o.f0 = plast[q4];
+      o.f1 = plast[q4 + 1];
+    } else {
+      o.f0 = o.f1 = float4{};
+    }
+  };
+  // software pipeline: the next k-step's loads are in flight while this one computes
+  KStep nxt;
+  if (GIS_MIXED_PREFETCH) load_ks(0, nxt);
+
   for (int ch = 0; ch < nchunk; ++ch) {
     const int cur = ch & 1;
     if (ch + 1 < nchunk) fetch(ch + 1);  // in flight during this chunk's MFMAs
     const int kend = min(KC, ksteps - ch * KC);
     for (int kk = 0; kk < kend; ++kk) {
       const int ks = ch * KC + kk;
-      const int q4 = ks * 8 + 2 * g;  // float4 index of this lane's 4 coefficients
-      const float4 t0 = trow[q4], t1 = trow[q4 + 1];
+      KStep now;
+      if (GIS_MIXED_PREFETCH) {
+        now = nxt;
+        load_ks(min(ks + 1, ksteps - 1), nxt);
+      } else {
+        load_ks(ks, now);
+      }
+      const float4 t0 = now.t0, t1 = now.t1, f0 = now.f0, f1 = now.f1;
       float4 uc[PT][2];
 #pragma unroll
-      for (int p = 0; p < PT; ++p) { uc[p][0] = prow[p][q4]; uc[p][1] = prow[p][q4 + 1]; }
-      float4 f0 = {}, f1 = {};
-      if (VL) { f0 = plast[q4]; f1 = plast[q4 + 1]; }
+      for (int p = 0; p < PT; ++p) { uc[p][0] = now.u[p][0]; uc[p][1] = now.u[p][1]; }
       const float tr[4] = {t0.x, t0.z, t1.x, t1.z}, ti[4] = {t0.y, t0.w, t1.y, t1.w};
       bf16x8 ahi[PT], alo[PT];
 #pragma unroll
