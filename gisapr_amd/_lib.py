@@ -13,6 +13,7 @@ _LIB_PATH = Path(__file__).resolve().parent / "libgisapr_hip.so"
 _lib = None
 
 _vp, _i, _d, _sz, _fp, _ip, _dp = C.c_void_p, C.c_int, C.c_double, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p
+_f = C.c_float
 
 # name -> (restype, argtypes); mirrors include/gisapr_hip.h
 SIGNATURES = {
@@ -22,6 +23,14 @@ SIGNATURES = {
     "gis_splat": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _d, _vp, _vp]),
     "gis_volume_dft_bytes": (_sz, [_i]),
     "gis_volume_dft_workspace": (_sz, [_i]),
+    "gis_volume_dft_bytes_ex": (_sz, [_i, _i]),
+    "gis_volume_dft_workspace_ex": (_sz, [_i, _i]),
+    "gis_volume_dft_ex": (_i, [_vp, _i, _i, _vp, _vp, _sz, _vp]),
+    "gis_build_templates_workspace_ex": (_sz, [_i, _i, _i]),
+    "gis_project_slices": (_i, [_vp, _i, _i, _vp, _i, _i, _i, _f, _vp, _vp, _sz, _vp]),
+    "gis_whiten_images": (_i, [_vp, _i, _i, _i, _i, _f, _vp, _vp, _sz, _vp]),
+    "gis_extract_slices_workspace": (_sz, [_i, _i]),
+    "gis_extract_slices": (_i, [_vp, _i, _vp, _i, _vp, _vp, _sz, _vp]),
     "gis_volume_dft": (_i, [_vp, _i, _vp, _vp, _sz, _vp]),
     "gis_build_templates_workspace": (_sz, [_i, _i]),
     "gis_build_templates": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _sz, _vp]),

@@ -94,10 +94,23 @@ struct CoefOut {
   float2* coeffs;    // [M][Kpad]
 };
 
-template <int N>
+// Real-space modes (the standalone projector's path): `img` replaces the
+// central slices by [M][N][N] real images; `proj` receives [M][N][N] real
+// images instead of the bank -- the cropped projection, or after the ring
+// division (whiten) and / or the plane fit + (x - mean) / std (normz, std
+// threshold eps).
+struct TplIO {
+  const float* img;
+  float* proj;
+  int whiten, normz;
+  float eps;
+  float2* slices;  // [M][L][L/2+1] sampled central slices (rows fftshifted) instead of any transform, or null
+};
+
+template <int N, int L>
 struct TplLayout {
-  static constexpr int L2 = 2 * N;       // padded length
-  static constexpr int M2 = N + 1;       // padded half-spectrum width
+  static constexpr int L2 = L;           // slice transform length (2N padded, N unpadded)
+  static constexpr int M2 = L / 2 + 1;   // slice half-spectrum width
   static constexpr int MH = N / 2 + 1;   // box half-spectrum width
   static constexpr int NR = (int)(0.7072 * N) + 3;  // rings (upper bound)
   static constexpr int FIXED = L2 + N + NR / 2 + 1 + 32;  // twiddles, ring divisors, reductions
@@ -117,19 +130,23 @@ struct TplLayout {
   // read and write contiguous blocks and the row passes full 32-byte runs.
   static constexpr int UT = (M2 + TK1 - 1) / TK1;  // column tiles of U
   static constexpr int WT = (MH + TK3 - 1) / TK3;  // column tiles of W
-  static constexpr size_t U_ELEMS = (size_t)N * UT * TK1;  // float2
+  // U also holds the ring partials ([MH][2][N/2+1] sums + counts) and the
+  // real image (N x N floats) once its columns are consumed
+  static constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
+  static constexpr size_t U_ELEMS = cmax(cmax((size_t)N * UT * TK1, (size_t)MH * 2 * (N / 2 + 1)),
+                                         ((size_t)N * N + 1) / 2);  // float2
   static constexpr size_t W_ELEMS = (size_t)N * WT * TK3;  // float2
   static constexpr size_t SLOT = U_ELEMS + W_ELEMS;
   static_assert(SLOT < (1u << 31), "32-bit scratch offsets");
 };
 
-template <int N>
+template <int N, int L>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
 build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ rts, int M, int n,
                        const float* __restrict__ mask, const float* __restrict__ fsc,
                        float2* __restrict__ bank, float* __restrict__ norms, float2* __restrict__ scratch,
-                       int normalize, CoefOut co) {
-  using T = TplLayout<N>;
+                       int normalize, CoefOut co, TplIO io) {
+  using T = TplLayout<N, L>;
   constexpr int L2 = T::L2, M2 = T::M2, MH = T::MH;
   extern __shared__ float2 lds[];  // kLdsFloat2 values
   float2* tw2 = lds;                       // L2 twiddles
@@ -169,12 +186,15 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     if ((int)blockIdx.x < 8 * g8) d_first = (blockIdx.x % 8) * g8 + blockIdx.x / 8;
   }
   for (int d = d_first; d < M; d += gridDim.x) {
-    const float* R = rts + 9 * d;
-    const float r00 = R[0], r01 = R[1], r10 = R[3], r11 = R[4], r20 = R[6], r21 = R[7];
+    float r00 = 0.f, r01 = 0.f, r10 = 0.f, r11 = 0.f, r20 = 0.f, r21 = 0.f;
+    if (!io.img) {  // image mode has no rotations
+      const float* R = rts + 9 * d;
+      r00 = R[0], r01 = R[1], r10 = R[3], r11 = R[4], r20 = R[6], r21 = R[7];
+    }
     __syncthreads();
 
-    // ---- phase 1: gather + inverse column FFT (length 2N) ----------------
-    for (int k0 = 0; k0 < M2; k0 += T::TK1) {
+    // ---- phase 1: gather + inverse column FFT (length L) -----------------
+    for (int k0 = 0; k0 < (io.img ? 0 : M2); k0 += T::TK1) {
       const int nk = min(T::TK1, M2 - k0);
       // Each wave samples blocks of (64/CW) slice rows x CW slice columns,
       // so one load instruction touches a compact patch of the 3-D spectrum,
@@ -191,8 +211,8 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       constexpr int SPS = largest_divisor_upto((CW * L2) / kThreads, GIS_TPL_SPS);
       static_assert((CW * L2) % (SPS * kThreads) == 0, "whole steps per column group");
       const int ngroups = (nk + CW - 1) / CW;
-      // (z0 + N) mod 2N for z0 in [0, 2N): the fftshift of the spectrum's z / y axes
-      auto wrap = [](unsigned z) -> unsigned { return z >= (unsigned)N ? z - N : z + N; };
+      // (z0 + L/2) mod L for z0 in [0, L): the fftshift of the spectrum's z / y axes
+      auto wrap = [](unsigned z) -> unsigned { return z >= (unsigned)(L2 / 2) ? z - L2 / 2 : z + L2 / 2; };
       for (int cg = 0; cg < ngroups; ++cg) {
         for (int t0 = tid; t0 < CW * L2; t0 += SPS * kThreads) {
           // SPS samples per step: all their taps are issued before any is used
@@ -207,7 +227,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
             const int q = cg * CW + lane / RW, ip = rb * RW + lane % RW;
             slot[u] = q < nk ? q * SP2 + pd(ip) : -1;
             const int kx = k0 + min(q, nk - 1);
-            const int f = ip < N ? ip : ip - L2;
+            const int f = ip < L2 / 2 ? ip : ip - L2;
             const float fy = (float)f * inv_l2;
             const float fx = (float)kx * inv_l2;
             float gx = r00 * fx + r01 * fy;
@@ -216,8 +236,8 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
             const bool cj = gx < 0.0f;
             if (cj) { gx = -gx; gy = -gy; gz = -gz; }
             cjs[u] = cj;
-            float cz = gz * (float)L2 + (float)N;
-            float cy = gy * (float)L2 + (float)N;
+            float cz = gz * (float)L2 + (float)(L2 / 2);
+            float cy = gy * (float)L2 + (float)(L2 / 2);
             float cx = gx * (float)L2;
             cz = fminf(fmaxf(cz, 0.0f), (float)(L2 - 1));
             cy = fminf(fmaxf(cy, 0.0f), (float)(L2 - 1));
@@ -229,8 +249,8 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
             const unsigned z0 = (unsigned)(int)z0f, y0 = (unsigned)(int)y0f, x0 = (unsigned)(int)x0f;
             // Branch-free: a clamped upper tap always carries weight 0 (the
             // coordinate was clamped to the last index first)
-            const unsigned pz0 = vdft_brick_z32<N>(wrap(z0)), pz1 = vdft_brick_z32<N>(wrap(min(z0 + 1, (unsigned)L2 - 1)));
-            const unsigned py0 = vdft_brick_y32<N>(wrap(y0)), py1 = vdft_brick_y32<N>(wrap(min(y0 + 1, (unsigned)L2 - 1)));
+            const unsigned pz0 = vdft_brick_z32<L2>(wrap(z0)), pz1 = vdft_brick_z32<L2>(wrap(min(z0 + 1, (unsigned)L2 - 1)));
+            const unsigned py0 = vdft_brick_y32<L2>(wrap(y0)), py1 = vdft_brick_y32<L2>(wrap(min(y0 + 1, (unsigned)L2 - 1)));
             const unsigned px0 = vdft_brick_x32(x0), px1 = vdft_brick_x32(min(x0 + 1, (unsigned)M2 - 1));
             ad[u][0] = pz0 + py0 + px0; ad[u][1] = pz0 + py0 + px1;
             ad[u][2] = pz0 + py1 + px0; ad[u][3] = pz0 + py1 + px1;
@@ -268,21 +288,51 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         }
       }
       __syncthreads();
+      if (io.slices) {
+        // the sampled slice itself: row (ip + L/2) mod L (fftshifted order), column kx
+        for (int t = tid; t < nk * L2; t += nthr) {
+          const int q = t / L2, ip = t - q * L2;
+          io.slices[((size_t)d * L2 + (ip + L2 / 2) % L2) * M2 + k0 + q] = buf[q * SP2 + pd(ip)];
+        }
+        __syncthreads();
+        continue;
+      }
       fft_lds_inplace<L2, true, kPad>(buf, nk, SP2, tw2, tid, nthr);
-      // keep spatial rows y in [3N/2, 2N) u [0, N/2) -> projection row Y
+      // ifftshift + crop: projection row Y is spatial row (Y + (L-N)/2 + L/2) mod L
       for (int t = tid; t < T::TK1 * N; t += nthr) {
         const int Y = t / T::TK1, q = t - Y * T::TK1;
         if (q >= nk) continue;
-        const int y = (Y + 3 * N / 2) % L2;
+        const int y = (Y + (L2 - N) / 2 + L2 / 2) % L2;
         U[uidx(Y, k0 + q)] = buf[q * SP2 + pd(y)];
       }
       __syncthreads();
     }
 
+    if (io.slices) continue;  // slices only: done with this direction
+
     // ---- phase 2: complex-to-real rows (2N) -> crop -> forward FFT (N) ----
     for (int y0 = 0; y0 < N; y0 += 2 * T::RB2) {
       const int nrow = min(2 * T::RB2, N - y0);
       const int nseq = (nrow + 1) / 2;
+      if (io.img) {
+        // image mode: rows ya (re) and yb (im) of the input image to the slot fronts
+        const float* im = io.img + (size_t)d * N * N;
+        constexpr int IT2I = (T::RB2 * N + kThreads - 1) / kThreads;
+        batched_tile_loop<IT2I, 4>(
+            [&](int it) {
+              const int t = tid + it * kThreads;
+              const int q = t / N, X = t - q * N;
+              const int ya = min(y0 + 2 * q, N - 1), yb = min(y0 + 2 * q + 1, N - 1);
+              return make_float2(im[ya * N + X], im[yb * N + X]);
+            },
+            [&](int it, float2 ab) {
+              const int t = tid + it * kThreads;
+              const int q = t / N, X = t - q * N;
+              if (2 * q + 1 >= nrow) ab.y = 0.f;
+              if (q < nseq) buf[q * SP2 + pd(X)] = ab;
+            });
+        __syncthreads();
+      } else {
       // Tile loads: unconditional (rows clamped into the image, unused
       // values dropped at the LDS store) and batched (batched_tile_loop).
       constexpr int IT2 = (T::RB2 * L2 + kThreads - 1) / kThreads;
@@ -291,19 +341,19 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
             const int t = tid + it * kThreads;
             const int q = t / L2, k = t - q * L2;
             const int ya = min(y0 + 2 * q, N - 1), yb = min(y0 + 2 * q + 1, N - 1);
-            const int ks = k <= N ? k : L2 - k;
+            const int ks = k <= L2 / 2 ? k : L2 - k;
             const float2 a = U[uidx(ya, ks)], b = U[uidx(yb, ks)];
             return make_float4(a.x, a.y, b.x, b.y);
           },
           [&](int it, float4 ab) {
             const int t = tid + it * kThreads;
             const int q = t / L2, k = t - q * L2;
-            // Hermitian extension of the half spectrum, Im of columns 0 and N dropped
-            const int ks = k <= N ? k : L2 - k;
+            // Hermitian extension of the half spectrum, Im of columns 0 and L/2 dropped
+            const int ks = k <= L2 / 2 ? k : L2 - k;
             float2 a = make_float2(ab.x, ab.y), b = make_float2(ab.z, ab.w);
             if (2 * q + 1 >= nrow) b = make_float2(0.f, 0.f);
-            if (ks == 0 || ks == N) { a.y = 0.f; b.y = 0.f; }
-            if (k > N) { a.y = -a.y; b.y = -b.y; }
+            if (ks == 0 || ks == L2 / 2) { a.y = 0.f; b.y = 0.f; }
+            if (k > L2 / 2) { a.y = -a.y; b.y = -b.y; }
             if (q < nseq) buf[q * SP2 + pd(k)] = make_float2(a.x - b.y, a.y + b.x);  // a + i*b
           });
       __syncthreads();
@@ -321,7 +371,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
           for (int i = 0; i < KP; ++i) {
             const int X = lane + 64 * i;
             if (X < N) {
-              const float2 v = sq[pd((X + 3 * N / 2) % L2)];
+              const float2 v = sq[pd((X + (L2 - N) / 2 + L2 / 2) % L2)];
               keep[i] = make_float2(v.x * sc, v.y * sc);  // projection rows ya (re), yb (im)
             }
           }
@@ -331,6 +381,19 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
             if (lane + 64 * i < N) sq[pd(lane + 64 * i)] = keep[i];
         }
         __syncthreads();
+      }
+      }
+      if (io.proj && !io.whiten && !io.normz) {
+        // the cropped projection itself: rows ya (re) and yb (im) of each slot front
+        for (int t = tid; t < nseq * N; t += nthr) {
+          const int q = t / N, X = t - q * N;
+          const float2 v = buf[q * SP2 + pd(X)];
+          const int ya = y0 + 2 * q;
+          io.proj[((size_t)d * N + ya) * N + X] = v.x;
+          if (ya + 1 < y0 + nrow) io.proj[((size_t)d * N + ya + 1) * N + X] = v.y;
+        }
+        __syncthreads();
+        continue;
       }
       fft_lds_inplace<N, false, kPad>(buf, nseq, SP2, tw1, tid, nthr);
       for (int t = tid; t < nseq * MH; t += nthr) {
@@ -345,6 +408,9 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       __syncthreads();
     }
 
+    if (io.proj && !io.whiten && !io.normz) continue;  // raw projection: done with this direction
+    const bool ring_divide = io.whiten || !io.proj;   // the bank is always whitened
+
     // ---- phase 3: forward column FFT (N) -> projection spectrum F, and the
     // ring sums of |F| over the full plane.  Along one half column (fy >= 0
     // or fy < 0 at fixed kx) the ring index never decreases with |fy| and
@@ -352,6 +418,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     // stores one partial (sum, count) per ring present in that half column;
     // phase 4 adds the partials of a ring in a fixed order (deterministic).
     // Half-plane columns 1..N/2-1 stand for two full-plane elements.
+    if (ring_divide) {
     const int nr = ring_of(N / 2, N / 2) + 1;
     constexpr int RS = N / 2 + 1;  // rings per half column, upper bound
     float* rsum = (float*)U;       // [MH][2][RS] (U is free after phase 2)
@@ -494,6 +561,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       }
       __syncthreads();
     }
+    }
 
     // ---- phase 6: complex-to-real rows (N) -> real image + plane sums ----
     double acc[4] = {0.0, 0.0, 0.0, 0.0};  // sum z, sum z^2, sum Y*z, sum X*z
@@ -574,10 +642,21 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       double var = sum_res2 / cnt - mean * mean;
       var = var > 0.0 ? var : 0.0;
       const double sd = sqrt(var);
-      inv_std = sd > 1e-12 ? 1.0 / sd : 1.0;
+      inv_std = sd > (double)io.eps ? 1.0 / sd : 1.0;
     }
     const float fa = (float)pa, fb = (float)pb, fc = (float)pc, fmean = (float)mean;
     const float fis = (float)inv_std;
+
+    if (io.proj) {
+      // real-space output: Z, or (Z - plane - mean) / std
+      float* po = io.proj + (size_t)d * N * N;
+      for (int t = tid; t < N * N; t += nthr) {
+        const int y = t / N, x = t - y * N;
+        const float z = Z[t];
+        po[t] = io.normz ? ((z - ((fa * (float)y + fb * (float)x) + fc)) - fmean) * fis : z;
+      }
+      continue;
+    }
 
     // ---- phase 8: normalise, mask, forward row FFT (N) -------------------
     for (int y0 = 0; y0 < N; y0 += 2 * T::RB6) {
@@ -707,11 +786,11 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
   }
 }
 
-template <int N>
+template <int N, int L>
 int launch_build_templates(const float2* D, const float* rts, int M, int n, const float* mask,
                            const float* fsc, float2* bank, float* norms, int normalize, void* ws, size_t ws_bytes,
-                           hipStream_t st, CoefOut co) {
-  using T = TplLayout<N>;
+                           hipStream_t st, CoefOut co, TplIO io) {
+  using T = TplLayout<N, L>;
   const size_t slot = T::SLOT * sizeof(float2);
   int grid = (int)std::min<size_t>((size_t)M, ws_bytes / slot);
   grid = std::min(grid, 4 * gis_cu_count());
@@ -719,7 +798,7 @@ int launch_build_templates(const float2* D, const float* rts, int M, int n, cons
   const size_t lds = (size_t)kLdsFloat2 * sizeof(float2);
   static bool attr_set = false;
   if (lds > 65536 && !attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)build_templates_kernel<N>,
+    hipError_t e = hipFuncSetAttribute((const void*)build_templates_kernel<N, L>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {
       (void)hipGetLastError();
@@ -728,37 +807,42 @@ int launch_build_templates(const float2* D, const float* rts, int M, int n, cons
     }
     attr_set = true;
   }
-  hipLaunchKernelGGL(build_templates_kernel<N>, dim3(grid), dim3(kThreads), lds, st, D, rts, M, n, mask, fsc,
-                     bank, norms, (float2*)ws, normalize, co);
+  hipLaunchKernelGGL((build_templates_kernel<N, L>), dim3(grid), dim3(kThreads), lds, st, D, rts, M, n, mask, fsc,
+                     bank, norms, (float2*)ws, normalize, co, io);
   GIS_LAUNCH_CHECK("gis_build_templates");
   return GIS_OK;
 }
 
-template <int N>
-size_t tpl_slot_bytes() { return TplLayout<N>::SLOT * sizeof(float2); }
+template <int N, int L>
+size_t tpl_slot_bytes() { return TplLayout<N, L>::SLOT * sizeof(float2); }
 
 }  // namespace
 
-static size_t slot_bytes_for(int N) {
+static size_t slot_bytes_for(int N, int pad) {
+#define GIS_TPL_SLOT(NV) \
+  case NV: return pad ? tpl_slot_bytes<NV, 2 * NV>() : tpl_slot_bytes<NV, NV>();
   switch (N) {
-    case 32: return tpl_slot_bytes<32>();
-    case 64: return tpl_slot_bytes<64>();
-    case 96: return tpl_slot_bytes<96>();
-    case 128: return tpl_slot_bytes<128>();
-    case 192: return tpl_slot_bytes<192>();
-    case 256: return tpl_slot_bytes<256>();
+    GIS_TPL_SLOT(32)
+    GIS_TPL_SLOT(64)
+    GIS_TPL_SLOT(96)
+    GIS_TPL_SLOT(128)
+    GIS_TPL_SLOT(192)
+    GIS_TPL_SLOT(256)
     default: return 0;
   }
+#undef GIS_TPL_SLOT
 }
 
-// Workspace bytes that let gis_build_templates run `M` directions with full
-// parallelism (one scratch slot per resident workgroup).
-GIS_API size_t gis_build_templates_workspace(int N, int M) {
-  const size_t s = slot_bytes_for(N);
+// Workspace bytes that let the template kernel run `M` directions (or images)
+// with full parallelism (one scratch slot per resident workgroup), for the
+// padded (pad 1) or unpadded (pad 0) slice transform.
+GIS_API size_t gis_build_templates_workspace_ex(int N, int M, int pad) {
+  const size_t s = slot_bytes_for(N, pad);
   if (s == 0 || M <= 0) return 0;
   const size_t g = std::min<size_t>((size_t)M, (size_t)4 * gis_cu_count());
   return g * s;
 }
+GIS_API size_t gis_build_templates_workspace(int N, int M) { return gis_build_templates_workspace_ex(N, M, 1); }
 
 // Replaces the per-direction loop of IntegratedProjector.build_templates
 // (v606:1077-1147).
@@ -768,24 +852,33 @@ GIS_API size_t gis_build_templates_workspace(int N, int M) {
 //   fsc   [n][n/2+1] float32 packed FSC weights, or NULL (all ones)
 //   bank  [M][n][n/2+1] complex64 output (unit L2 norm, UNshifted rfft2 layout)
 //   norms [M] float32 output of the pre-normalisation norms, or NULL
-static int build_templates_dispatch(const void* dft, int N, int n, const float* rts, int M, const float* mask,
-                                    const float* fsc, void* bank, float* norms, int normalize, void* ws,
-                                    size_t ws_bytes, void* stream, CoefOut co) {
+static int build_templates_dispatch(const void* dft, int N, int pad, int n, const float* rts, int M,
+                                    const float* mask, const float* fsc, void* bank, float* norms, int normalize,
+                                    void* ws, size_t ws_bytes, void* stream, CoefOut co, TplIO io) {
   hipStream_t st = (hipStream_t)stream;
   const float2* D = (const float2*)dft;
   float2* B = (float2*)bank;
+#define GIS_TPL_CASE(NV)                                                                                        \
+  case NV:                                                                                                      \
+    return pad ? launch_build_templates<NV, 2 * NV>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, \
+                                                    st, co, io)                                                 \
+               : launch_build_templates<NV, NV>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, \
+                                                co, io);
   switch (N) {
-    case 32: return launch_build_templates<32>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, co);
-    case 64: return launch_build_templates<64>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, co);
-    case 96: return launch_build_templates<96>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, co);
-    case 128: return launch_build_templates<128>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, co);
-    case 192: return launch_build_templates<192>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, co);
-    case 256: return launch_build_templates<256>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, co);
+    GIS_TPL_CASE(32)
+    GIS_TPL_CASE(64)
+    GIS_TPL_CASE(96)
+    GIS_TPL_CASE(128)
+    GIS_TPL_CASE(192)
+    GIS_TPL_CASE(256)
     default:
       gis_set_error("gis_build_templates: box %d not in {32, 64, 96, 128, 192, 256}", N);
       return GIS_EUNSUPPORTED;
   }
+#undef GIS_TPL_CASE
 }
+
+static const TplIO kBankIO = {nullptr, nullptr, 1, 1, 1e-12f, nullptr};
 
 GIS_API int gis_build_templates(const void* dft, int N, int n, const float* rts, int M, const float* mask,
                                 const float* fsc, void* bank, float* norms, int normalize, void* ws,
@@ -795,8 +888,8 @@ GIS_API int gis_build_templates(const void* dft, int N, int n, const float* rts,
   GIS_REQUIRE(M >= 0, "gis_build_templates: M < 0");
   GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0, "gis_build_templates: newbox %d must be even and <= box %d", n, N);
   if (M == 0) return GIS_OK;
-  return build_templates_dispatch(dft, N, n, rts, M, mask, fsc, bank, norms, normalize, ws, ws_bytes, stream,
-                                  CoefOut{nullptr, 0, 0, nullptr});
+  return build_templates_dispatch(dft, N, 1, n, rts, M, mask, fsc, bank, norms, normalize, ws, ws_bytes, stream,
+                                  CoefOut{nullptr, 0, 0, nullptr}, kBankIO);
 }
 
 // The template bank reduced to the listed coefficients of every direction
@@ -814,6 +907,75 @@ GIS_API int gis_build_template_coefficients(const void* dft, int N, int n, const
   GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0, "gis_build_template_coefficients: newbox %d must be even and <= box %d",
               n, N);
   if (M == 0) return GIS_OK;
-  return build_templates_dispatch(dft, N, n, rts, M, mask, fsc, nullptr, norms, 0, ws, ws_bytes, stream,
-                                  CoefOut{kslot, K, Kpad, (float2*)coeffs});
+  return build_templates_dispatch(dft, N, 1, n, rts, M, mask, fsc, nullptr, norms, 0, ws, ws_bytes, stream,
+                                  CoefOut{kslot, K, Kpad, (float2*)coeffs}, kBankIO);
+}
+
+// Real-space central-slice projections of a volume (the standalone
+// projector, project3d_and_whiten_cuda.py:119-147): for every direction the
+// cropped projection [N][N]; with `whiten` the ring division of its spectrum
+// (getSpectrum_divideBySpectrum_torch, :196-230) and with `normz` the plane
+// fit + (x - mean) / std (normalize_torch, :231-291, std threshold eps).
+//   dft  [L][L][L/2+1] from gis_volume_dft_ex(.., pad, ..), L = pad ? 2N : N
+//   proj [M][N][N] float32 out;  ws: gis_build_templates_workspace_ex(N, M, pad)
+GIS_API int gis_project_slices(const void* dft, int N, int pad, const float* rts, int M, int whiten, int normz,
+                               float eps, float* proj, void* ws, size_t ws_bytes, void* stream) {
+  GIS_REQUIRE(dft && rts && proj && ws, "gis_project_slices: null pointer");
+  GIS_REQUIRE(M >= 0, "gis_project_slices: M < 0");
+  GIS_REQUIRE(pad == 0 || pad == 1, "gis_project_slices: pad %d not 0 or 1", pad);
+  if (M == 0) return GIS_OK;
+  return build_templates_dispatch(dft, N, pad, N, rts, M, nullptr, nullptr, nullptr, nullptr, 0, ws, ws_bytes,
+                                  stream, CoefOut{nullptr, 0, 0, nullptr}, TplIO{nullptr, proj, whiten, normz, eps, nullptr});
+}
+
+// The same ring division and / or plane normalisation applied to M real
+// [N][N] images (getSpectrum_divideBySpectrum_torch and normalize_torch,
+// project3d_and_whiten_cuda.py:196-291).  At least one of whiten / normz.
+//   img, out [M][N][N] float32;  ws: gis_build_templates_workspace(N, M)
+GIS_API int gis_whiten_images(const float* img, int N, int M, int whiten, int normz, float eps, float* out,
+                              void* ws, size_t ws_bytes, void* stream) {
+  GIS_REQUIRE(img && out && ws, "gis_whiten_images: null pointer");
+  GIS_REQUIRE(M >= 0, "gis_whiten_images: M < 0");
+  GIS_REQUIRE(whiten || normz, "gis_whiten_images: nothing to do (whiten and normz both 0)");
+  if (M == 0) return GIS_OK;
+  return build_templates_dispatch(nullptr, N, 1, N, nullptr, M, nullptr, nullptr, nullptr, nullptr, 0, ws, ws_bytes,
+                                  stream, CoefOut{nullptr, 0, 0, nullptr}, TplIO{img, out, whiten, normz, eps, nullptr});
+}
+
+// The (N, pad) instantiation that runs slices of transform length L.
+static bool slice_config(int L, int* N, int* pad) {
+  const int boxes[6] = {32, 64, 96, 128, 192, 256};
+  for (int b : boxes)
+    if (2 * b == L) { *N = b; *pad = 1; return true; }
+  for (int b : boxes)
+    if (b == L) { *N = b; *pad = 0; return true; }
+  return false;
+}
+
+GIS_API size_t gis_extract_slices_workspace(int L, int M) {
+  int N = 0, pad = 0;
+  return slice_config(L, &N, &pad) ? gis_build_templates_workspace_ex(N, M, pad) : 0;
+}
+
+// extract_central_slices_rfft (project3d_and_whiten_cuda.py:397-435; the
+// gather of v606:1099-1113): trilinear, border-clamped samples of the half
+// spectrum at M rotated central slices, conjugated where the rotated x < 0.
+//   dft    [L][L][L/2+1] bricked (gis_volume_dft_ex, or a natural-order
+//          spectrum rearranged into the same bricks), L in {32..512}
+//   rts    [M][3][3] float32, rotation applied to (x, y, z) frequency vectors
+//   slices [M][L][L/2+1] complex64 out, rows in fftshifted order (the
+//          reference's output layout);  ws: gis_extract_slices_workspace(L, M)
+GIS_API int gis_extract_slices(const void* dft, int L, const float* rts, int M, void* slices, void* ws,
+                               size_t ws_bytes, void* stream) {
+  GIS_REQUIRE(dft && rts && slices && ws, "gis_extract_slices: null pointer");
+  GIS_REQUIRE(M >= 0, "gis_extract_slices: M < 0");
+  int N = 0, pad = 0;
+  if (!slice_config(L, &N, &pad)) {
+    gis_set_error("gis_extract_slices: transform length %d not a supported box or twice one", L);
+    return GIS_EUNSUPPORTED;
+  }
+  if (M == 0) return GIS_OK;
+  return build_templates_dispatch(dft, N, pad, N, rts, M, nullptr, nullptr, nullptr, nullptr, 0, ws, ws_bytes, stream,
+                                  CoefOut{nullptr, 0, 0, nullptr},
+                                  TplIO{nullptr, nullptr, 0, 0, 0.f, (float2*)slices});
 }

@@ -1,11 +1,13 @@
 // 3-D transform of the model volume for central-slice projection.
 //
-// Computes, for a box-N volume, the half-spectrum
-//     D = rfftn( fftshift( pad_{N/2}(vol) * sinc^2(|f|) ) )     [2N][2N][N+1]
-// in natural frequency order (v606:1068-1073: pad N/2 zeros per side,
-// apply_sinc2_filter_inplace_3d v606:837-853, fftshift, rfftn).  The final
-// fftshift over (z, y) of v606:1073 is not materialised: the slice gather
-// indexes D with the shift folded in.
+// Computes, for a box-N volume and transform length L (L = 2N: the volume
+// padded with N/2 zeros per side; L = N: unpadded), the half spectrum
+//     D = rfftn( fftshift( pad(vol) * sinc^2(|f|) ) )     [L][L][L/2+1]
+// in natural frequency order, f = fftshift(fftfreq(L)) per axis
+// (v606:1068-1073: pad N/2, apply_sinc2_filter_inplace_3d v606:837-853,
+// fftshift, rfftn; project3d_and_whiten_cuda.py:103-117, whose --skip_padding
+// gives L = N).  The final fftshift over (z, y) is not materialised: the
+// slice gather indexes D with the shift folded in.
 //
 // Three LDS Stockham passes (x: real-to-complex, then y, then z).  The zero
 // padding is never read or written: the x pass touches only the N^2 rows
@@ -13,61 +15,65 @@
 // rows, and the z pass synthesises the zero planes.  Every pass moves whole
 // 128-byte lines: a block transforms 16 columns (two groups of 8 kx, or a
 // pair of ky rows x 8 kx) in place in one LDS buffer.  Traffic per
-// transform ~ 4N^3 + 8*N^2*G*16*(1/2 + 1/2 + 1 + 1) + 8*4N^2(N+4) bytes
-// (G = ceil((N+1)/8)): 1.44 GB at box 256.
+// transform ~ 4N^3 + 8*N*(L/2)*G*16*(N/L + N/L + 1 + 1) + 8*L*L*(L/2+4) bytes
+// (G = ceil((L/2+1)/8)): 1.44 GB at box 256, L = 512.
 //
-// Output layout (bricked): the x and y passes work on a workspace of the N
-// data planes (see VdftWork); the z pass writes the result in bricks of
-// 2 (z) x 2 (y) x 4 (x) complex values = one 128-byte cache line, so the 2x2x2
-// trilinear cell of a central-slice sample spans ~2.8 lines instead of four
-// 16-element x rows, whatever the slice orientation:
-//   D[((((kz/2) * N + ky/2) * XB + kx/4) * 2 + kz%2) * 2 + ky%2) * 4 + kx%4],
-//   XB = N/4 + 1 (x padded to N+4; the padding is never read).
+// Output layout (bricked, vdft_layout.h): the x and y passes work on a
+// workspace of the N data planes (see VdftWork); the z pass writes the
+// result in bricks of 2 (z) x 2 (y) x 4 (x) complex values = one 128-byte
+// cache line, so the 2x2x2 trilinear cell of a central-slice sample spans
+// ~2.8 lines instead of four 16-element x rows, whatever the orientation:
+//   D[((((kz/2) * L/2 + ky/2) * XB + kx/4) * 2 + kz%2) * 2 + ky%2) * 4 + kx%4],
+//   XB = L/8 + 1 (x padded to L/2+4; the padding is never read).
 #include "common.h"
 #include "vdft_layout.h"
 
-// Row x' of the shifted padded volume holds padded column (x'+N) mod 2N,
-// which is data column (x'+N) mod 2N - N/2 when that lies in [0, N).
-__device__ __forceinline__ int shifted_to_data(int xs, int N) {
-  const int p = (xs + N) % (2 * N);
-  const int v = p - N / 2;
+// Position xs of the fftshifted length-L axis holds padded position
+// (xs + L/2) mod L, which is data position (xs + L/2) mod L - (L-N)/2 when
+// that lies in [0, N).
+template <int N, int L>
+__device__ __forceinline__ int shifted_to_data(int xs) {
+  const int v = (xs + L / 2) % L - (L - N) / 2;
   return (v >= 0 && v < N) ? v : -1;
 }
+// Shifted position of data position v.
+template <int N, int L>
+__device__ __forceinline__ int data_to_shifted(int v) { return (v + (L - N) / 2 + L / 2) % L; }
 
-__device__ __forceinline__ float sinc2_axis_freq(int padded_index, int N) {
-  // fftshift(fftfreq(2N))[p] = (p - N) / (2N)
-  return (float)((double)(padded_index - N) / (double)(2 * N));
+template <int N, int L>
+__device__ __forceinline__ float sinc2_axis_freq(int v) {
+  // fftshift(fftfreq(L))[p] = (p - L/2) / L at padded position p = v + (L-N)/2
+  return (float)((double)(v + (L - N) / 2 - L / 2) / (double)L);
 }
 
 // Workspace of the x and y passes: the N data planes (vz) of the
-// natural-order spectrum, all 2N rows (ky, shifted order), kx in groups of
+// natural-order spectrum, all L rows (ky, shifted order), kx in groups of
 // eight; a pair of rows of one group is one 128-byte line:
-//   W[(((vz * N + ky/2) * G + kx/8) * 2 + ky%2) * 8 + kx%8],  G = ceil((N+1)/8).
-template <int N>
+//   W[(((vz * L/2 + ky/2) * G + kx/8) * 2 + ky%2) * 8 + kx%8],  G = ceil((L/2+1)/8).
+template <int N, int L>
 struct VdftWork {
-  static constexpr int G = (N + 8) / 8;
-  static constexpr size_t ELEMS = (size_t)N * N * G * 16;
+  static constexpr int G = (L / 2 + 8) / 8;
+  static constexpr size_t ELEMS = (size_t)N * (L / 2) * G * 16;
 };
 
-// Every pass: 512 threads (8 waves), 16 sequences of length 2N per block in
+// Every pass: 512 threads (8 waves), 16 sequences of length L per block in
 // one LDS buffer (each wave transforms its sequences in place, registers
-// holding a stage's butterflies), ~70 KB at box 256 so two blocks share a CU.
+// holding a stage's butterflies), ~78 KB at L = 512 so two blocks share a CU.
 constexpr int kVdftThreads = 512;
 constexpr int kVdftSeq = 16;
 constexpr int kVdftPad = 3;  // one LDS slot of padding per 8 values
-static_assert((2 * 32) % 32 == 0 && kVdftThreads == 512, "trip counts below assume 512 threads and 2N % 32 == 0");
-template <int N>
-constexpr int vdft_sp() { return lds_padded_len<2 * N, kVdftPad>() + 1; }
-template <int N>
-constexpr size_t vdft_lds_bytes() { return (size_t)(2 * N + kVdftSeq * vdft_sp<N>()) * sizeof(float2); }
+template <int L>
+constexpr int vdft_sp() { return lds_padded_len<L, kVdftPad>() + 1; }
+template <int L>
+constexpr size_t vdft_lds_bytes() { return (size_t)(L + kVdftSeq * vdft_sp<L>()) * sizeof(float2); }
 
 // x pass: two data rows per complex FFT (real a + i*b), split afterwards.
-template <int N>
+template <int N, int L>
 __global__ void __launch_bounds__(kVdftThreads)
 vdft_x_pass(const float* __restrict__ vol, float2* __restrict__ Wk) {
-  constexpr int L = 2 * N;
-  constexpr int SP = vdft_sp<N>();
-  constexpr int G = VdftWork<N>::G;
+  constexpr int SP = vdft_sp<L>();
+  constexpr int G = VdftWork<N, L>::G;
+  static_assert(L % 32 == 0 && N % 32 == 0 && (L == N || L == 2 * N), "transform length");
   extern __shared__ float2 lds[];
   float2* tw = lds;
   float2* buf = tw + L;
@@ -86,20 +92,23 @@ vdft_x_pass(const float* __restrict__ vol, float2* __restrict__ Wk) {
     const int row = 2 * (pair0 + q) + h;  // row = vz * N + vy
     const int vz = row / N, vy = row - vz * N;
     const float raw = vol[((size_t)vz * N + vy) * N + vx];
-    const float fz = sinc2_axis_freq(vz + N / 2, N);
-    const float fy = sinc2_axis_freq(vy + N / 2, N);
-    const float fx = sinc2_axis_freq(vx + N / 2, N);
+    const float fz = sinc2_axis_freq<N, L>(vz);
+    const float fy = sinc2_axis_freq<N, L>(vy);
+    const float fx = sinc2_axis_freq<N, L>(vx);
     const float r = sqrtf((fz * fz + fy * fy) + fx * fx);
     float sn = 1.0f;
     if (r != 0.0f) sn = sinpif(r) / (3.14159265358979323846f * r);
-    const int xs = (vx + N / 2 + N) % L;  // shifted column of data column vx
-    float* dst = (float*)&buf[q * SP + lds_pad<kVdftPad>(xs)];
+    float* dst = (float*)&buf[q * SP + lds_pad<kVdftPad>(data_to_shifted<N, L>(vx))];
     dst[h] = raw * (sn * sn);
   }
-  // zero columns: shifted xs in [N/2, 3N/2)
-  for (int t = tid; t < kVdftSeq * N; t += nthr) {
-    const int q = t / N, xs = N / 2 + (t - q * N);
-    buf[q * SP + lds_pad<kVdftPad>(xs)] = make_float2(0.f, 0.f);
+  // zero columns: the L - N padded positions outside the data
+  if constexpr (L > N) {
+    constexpr int OFF = (L - N) / 2;
+    for (int t = tid; t < kVdftSeq * (L - N); t += nthr) {
+      const int q = t / (L - N), j = t - q * (L - N);
+      const int pp = j < OFF ? j : j + N;
+      buf[q * SP + lds_pad<kVdftPad>((pp + L / 2) % L)] = make_float2(0.f, 0.f);
+    }
   }
   __syncthreads();
   for (int q = wave; q < nseq; q += nthr / 64) fft_wave_inplace<L, false, kVdftPad>(buf + q * SP, tw, lane);
@@ -110,7 +119,7 @@ vdft_x_pass(const float* __restrict__ vol, float2* __restrict__ Wk) {
   for (int t = tid; t < nseq * G * 16; t += nthr) {
     const int q = t / (G * 16), r = t - q * (G * 16);
     const int g = r >> 4, h = (r >> 3) & 1, k = g * 8 + (r & 7);
-    if (k > N) continue;
+    if (k > L / 2) continue;
     const float2 z = buf[q * SP + lds_pad<kVdftPad>(k)];
     const float2 zc = cconj(buf[q * SP + lds_pad<kVdftPad>((L - k) % L)]);
     float2 val;
@@ -122,19 +131,18 @@ vdft_x_pass(const float* __restrict__ vol, float2* __restrict__ Wk) {
     }
     const int row = 2 * (pair0 + q);
     const int vz = row / N, vy = row - vz * N;
-    const int ys = (vy + N / 2 + N) % L;
-    Wk[(((size_t)vz * N + ys / 2) * G + g) * 16 + h * 8 + (k & 7)] = val;
+    const int ys = data_to_shifted<N, L>(vy);
+    Wk[(((size_t)vz * (L / 2) + ys / 2) * G + g) * 16 + h * 8 + (k & 7)] = val;
   }
 }
 
 // y pass: column FFTs along ky over the N data rows of one data plane, 16
 // kx columns (two groups) per block; the zero rows are synthesised.
-template <int N>
+template <int N, int L>
 __global__ void __launch_bounds__(kVdftThreads)
 vdft_y_pass(float2* __restrict__ Wk) {
-  constexpr int L = 2 * N;
-  constexpr int SP = vdft_sp<N>();
-  constexpr int G = VdftWork<N>::G;
+  constexpr int SP = vdft_sp<L>();
+  constexpr int G = VdftWork<N, L>::G;
   extern __shared__ float2 lds[];
   float2* tw = lds;
   float2* buf = tw + L;
@@ -144,8 +152,8 @@ vdft_y_pass(float2* __restrict__ Wk) {
   const int vz = blockIdx.y;
   const int g0 = blockIdx.x * 2;
   const int ng = min(2, G - g0);
-  const int ncol = min(kVdftSeq, N + 1 - g0 * 8);  // valid kx columns of the tile
-  float2* P = Wk + (size_t)vz * N * G * 16;
+  const int ncol = min(kVdftSeq, L / 2 + 1 - g0 * 8);  // valid kx columns of the tile
+  float2* P = Wk + (size_t)vz * (L / 2) * G * 16;
   // 32 threads per row pair: (group, row parity, kx % 8) = two adjacent lines
 #pragma unroll
   for (int it = 0; it < (L / 2) * 32 / nthr; ++it) {
@@ -154,7 +162,7 @@ vdft_y_pass(float2* __restrict__ Wk) {
     const int gl = r >> 4, h = (r >> 3) & 1, c = gl * 8 + (r & 7);
     const int ys = 2 * p + h;
     float2 v = make_float2(0.f, 0.f);
-    if (gl < ng && shifted_to_data(ys, N) >= 0) v = P[((size_t)p * G + g0 + gl) * 16 + (r & 15)];
+    if (gl < ng && shifted_to_data<N, L>(ys) >= 0) v = P[((size_t)p * G + g0 + gl) * 16 + (r & 15)];
     buf[c * SP + lds_pad<kVdftPad>(ys)] = v;
   }
   __syncthreads();
@@ -172,13 +180,12 @@ vdft_y_pass(float2* __restrict__ Wk) {
 // z pass: column FFTs along kz for one pair of ky rows and one group of 8 kx
 // (16 columns); the zero planes are synthesised; the result goes to D in
 // the bricked layout, two whole bricks per kz pair.
-template <int N>
+template <int N, int L>
 __global__ void __launch_bounds__(kVdftThreads)
 vdft_z_pass(const float2* __restrict__ Wk, float2* __restrict__ D) {
-  constexpr int L = 2 * N;
-  constexpr int SP = vdft_sp<N>();
-  constexpr int G = VdftWork<N>::G;
-  using BR = VdftBricks<N>;
+  constexpr int SP = vdft_sp<L>();
+  constexpr int G = VdftWork<N, L>::G;
+  using BR = VdftBricks<L>;
   extern __shared__ float2 lds[];
   float2* tw = lds;
   float2* buf = tw + L;
@@ -186,15 +193,15 @@ vdft_z_pass(const float2* __restrict__ Wk, float2* __restrict__ D) {
   constexpr int nthr = kVdftThreads;
   fill_twiddles(tw, L, tid, nthr);
   const int g = blockIdx.x, p = blockIdx.y;  // kx group, ky pair
-  const int nx = min(8, N + 1 - g * 8);      // valid kx of the group
+  const int nx = min(8, L / 2 + 1 - g * 8);  // valid kx of the group
   // column c = (ky % 2) * 8 + kx % 8
 #pragma unroll
   for (int it = 0; it < L * 16 / nthr; ++it) {
     const int t = tid + it * nthr;
     const int zs = t >> 4, c = t & 15;
-    const int vz = shifted_to_data(zs, N);
+    const int vz = shifted_to_data<N, L>(zs);
     float2 v = make_float2(0.f, 0.f);
-    if (vz >= 0 && (c & 7) < nx) v = Wk[(((size_t)vz * N + p) * G + g) * 16 + c];
+    if (vz >= 0 && (c & 7) < nx) v = Wk[(((size_t)vz * (L / 2) + p) * G + g) * 16 + c];
     buf[c * SP + lds_pad<kVdftPad>(zs)] = v;
   }
   __syncthreads();
@@ -214,13 +221,14 @@ vdft_z_pass(const float2* __restrict__ Wk, float2* __restrict__ D) {
   }
 }
 
-template <int N>
+template <int N, int L>
 static int launch_volume_dft(const float* vol, float2* D, float2* Wk, hipStream_t st) {
-  constexpr int G = VdftWork<N>::G;
-  const size_t lds = vdft_lds_bytes<N>();
+  constexpr int G = VdftWork<N, L>::G;
+  const size_t lds = vdft_lds_bytes<L>();
   static bool attr_set = false;
   if (lds > 65536 && !attr_set) {
-    const void* fns[3] = {(const void*)vdft_x_pass<N>, (const void*)vdft_y_pass<N>, (const void*)vdft_z_pass<N>};
+    const void* fns[3] = {(const void*)vdft_x_pass<N, L>, (const void*)vdft_y_pass<N, L>,
+                          (const void*)vdft_z_pass<N, L>};
     for (const void* f : fns) {
       hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (e != hipSuccess) {
@@ -232,12 +240,12 @@ static int launch_volume_dft(const float* vol, float2* D, float2* Wk, hipStream_
     attr_set = true;
   }
   const int pairs = N * N / 2;
-  hipLaunchKernelGGL(vdft_x_pass<N>, dim3((pairs + kVdftSeq - 1) / kVdftSeq), dim3(kVdftThreads), lds, st, vol,
-                     Wk);
+  hipLaunchKernelGGL((vdft_x_pass<N, L>), dim3((pairs + kVdftSeq - 1) / kVdftSeq), dim3(kVdftThreads), lds, st,
+                     vol, Wk);
   GIS_LAUNCH_CHECK("gis_volume_dft x");
-  hipLaunchKernelGGL(vdft_y_pass<N>, dim3((G + 1) / 2, N), dim3(kVdftThreads), lds, st, Wk);
+  hipLaunchKernelGGL((vdft_y_pass<N, L>), dim3((G + 1) / 2, N), dim3(kVdftThreads), lds, st, Wk);
   GIS_LAUNCH_CHECK("gis_volume_dft y");
-  hipLaunchKernelGGL(vdft_z_pass<N>, dim3(G, N), dim3(kVdftThreads), lds, st, (const float2*)Wk, D);
+  hipLaunchKernelGGL((vdft_z_pass<N, L>), dim3(G, L / 2), dim3(kVdftThreads), lds, st, (const float2*)Wk, D);
   GIS_LAUNCH_CHECK("gis_volume_dft z");
   return GIS_OK;
 }
@@ -246,36 +254,54 @@ static bool vdft_box_supported(int N) {
   return N == 32 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256;
 }
 
-// Bytes of the bricked spectrum and of the natural-order workspace at box N.
-GIS_API size_t gis_volume_dft_bytes(int N) {
-  return N > 0 ? (size_t)4 * N * N * (N / 4 + 1) * 4 * sizeof(float2) : 0;
-}
-GIS_API size_t gis_volume_dft_workspace(int N) {
-  return N > 0 ? (size_t)N * N * ((N + 8) / 8) * 16 * sizeof(float2) : 0;
-}
+static size_t vdft_len(int N, int pad) { return pad ? (size_t)2 * N : (size_t)N; }
 
-// Replaces the pad / sinc^2 / fftshift / rfftn prefix of
-// IntegratedProjector.build_templates (v606:1068-1073).
-// `volume` is [N][N][N] float32 ([z][y][x]); `dft` receives the [2N][2N][N+1]
-// complex64 half spectrum in natural frequency order, stored in 2x2x4 bricks
-// (gis_volume_dft_bytes(N) bytes); `ws` holds gis_volume_dft_workspace(N) bytes.
-GIS_API int gis_volume_dft(const float* volume, int N, void* dft, void* ws, size_t ws_bytes, void* stream) {
+// Bytes of the bricked spectrum and of the workspace at box N, padded (pad 1,
+// L = 2N) or not (pad 0, L = N).
+GIS_API size_t gis_volume_dft_bytes_ex(int N, int pad) {
+  const size_t L = vdft_len(N, pad);
+  return N > 0 ? L * L * (L / 8 + 1) * 4 * sizeof(float2) : 0;
+}
+GIS_API size_t gis_volume_dft_workspace_ex(int N, int pad) {
+  const size_t L = vdft_len(N, pad);
+  return N > 0 ? (size_t)N * (L / 2) * ((L / 2 + 8) / 8) * 16 * sizeof(float2) : 0;
+}
+GIS_API size_t gis_volume_dft_bytes(int N) { return gis_volume_dft_bytes_ex(N, 1); }
+GIS_API size_t gis_volume_dft_workspace(int N) { return gis_volume_dft_workspace_ex(N, 1); }
+
+// The pad / sinc^2 / fftshift / rfftn prefix of the central-slice projector
+// (IntegratedProjector.build_templates v606:1068-1073; project3d_and_whiten_cuda.py
+// :103-117, pad 0 = its --skip_padding).  `volume` is [N][N][N] float32
+// ([z][y][x]); `dft` receives the [L][L][L/2+1] complex64 half spectrum in
+// natural frequency order, stored in 2x2x4 bricks (gis_volume_dft_bytes_ex
+// bytes); `ws` holds gis_volume_dft_workspace_ex bytes.
+GIS_API int gis_volume_dft_ex(const float* volume, int N, int pad, void* dft, void* ws, size_t ws_bytes,
+                              void* stream) {
   GIS_REQUIRE(volume != nullptr && dft != nullptr && ws != nullptr, "gis_volume_dft: null pointer");
   if (!vdft_box_supported(N)) {
     gis_set_error("gis_volume_dft: box %d not in {32, 64, 96, 128, 192, 256}", N);
     return GIS_EUNSUPPORTED;
   }
-  GIS_REQUIRE(ws_bytes >= gis_volume_dft_workspace(N), "gis_volume_dft: workspace %zu < %zu bytes", ws_bytes,
-              gis_volume_dft_workspace(N));
+  GIS_REQUIRE(pad == 0 || pad == 1, "gis_volume_dft: pad %d not 0 or 1", pad);
+  GIS_REQUIRE(ws_bytes >= gis_volume_dft_workspace_ex(N, pad), "gis_volume_dft: workspace %zu < %zu bytes",
+              ws_bytes, gis_volume_dft_workspace_ex(N, pad));
   hipStream_t st = (hipStream_t)stream;
   float2* D = (float2*)dft;
   float2* W = (float2*)ws;
+#define GIS_VDFT_CASE(NV)                                                                       \
+  case NV:                                                                                      \
+    return pad ? launch_volume_dft<NV, 2 * NV>(volume, D, W, st) : launch_volume_dft<NV, NV>(volume, D, W, st);
   switch (N) {
-    case 32: return launch_volume_dft<32>(volume, D, W, st);
-    case 64: return launch_volume_dft<64>(volume, D, W, st);
-    case 96: return launch_volume_dft<96>(volume, D, W, st);
-    case 128: return launch_volume_dft<128>(volume, D, W, st);
-    case 192: return launch_volume_dft<192>(volume, D, W, st);
-    default: return launch_volume_dft<256>(volume, D, W, st);
+    GIS_VDFT_CASE(32)
+    GIS_VDFT_CASE(64)
+    GIS_VDFT_CASE(96)
+    GIS_VDFT_CASE(128)
+    GIS_VDFT_CASE(192)
+    default: return pad ? launch_volume_dft<256, 512>(volume, D, W, st) : launch_volume_dft<256, 256>(volume, D, W, st);
   }
+#undef GIS_VDFT_CASE
+}
+
+GIS_API int gis_volume_dft(const float* volume, int N, void* dft, void* ws, size_t ws_bytes, void* stream) {
+  return gis_volume_dft_ex(volume, N, 1, dft, ws, ws_bytes, stream);
 }

@@ -41,19 +41,20 @@ def splat(xyz: torch.Tensor, numbers: torch.Tensor, box: int, width: int, apix: 
 
 
 def volume_dft(volume: torch.Tensor, out: Optional[torch.Tensor] = None,
-               workspace: Optional[torch.Tensor] = None) -> torch.Tensor:
-    """[N]^3 float32 -> bricked half spectrum of [2N, 2N, N+1] complex64
-    (pad, sinc^2, shift, rfftn); ``volume_dft_natural`` unpacks it."""
+               workspace: Optional[torch.Tensor] = None, pad: bool = True) -> torch.Tensor:
+    """[N]^3 float32 -> bricked half spectrum of [L, L, L/2+1] complex64
+    (pad to L = 2N or not, sinc^2, shift, rfftn); ``volume_dft_natural`` unpacks it."""
     require_hip(volume, "volume")
     N = int(volume.shape[0])
     if volume.ndim != 3 or len(set(volume.shape)) != 1 or volume.dtype != torch.float32:
         raise ValueError("volume must be a cubic float32 tensor")
-    nb = int(lib().gis_volume_dft_bytes(N)) // 8
+    pad = int(bool(pad))
+    nb = int(lib().gis_volume_dft_bytes_ex(N, pad)) // 8
     if out is None:
         out = torch.empty((nb,), dtype=torch.complex64, device=volume.device)
     elif out.numel() != nb or out.dtype != torch.complex64:
         raise ValueError(f"out must be {nb} complex64 values")
-    wsb = int(lib().gis_volume_dft_workspace(N))
+    wsb = int(lib().gis_volume_dft_workspace_ex(N, pad))
     if workspace is not None:
         require_hip(workspace, "workspace")
         if workspace.numel() * workspace.element_size() < wsb:
@@ -61,16 +62,109 @@ def volume_dft(volume: torch.Tensor, out: Optional[torch.Tensor] = None,
         ws = workspace
     else:
         ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=volume.device)
-    check(lib().gis_volume_dft(ptr(volume), N, ptr(out), ptr(ws), wsb, stream_of(volume.device)),
+    check(lib().gis_volume_dft_ex(ptr(volume), N, pad, ptr(out), ptr(ws), wsb, stream_of(volume.device)),
           "gis_volume_dft")
     return out
 
 
-def volume_dft_natural(dft: torch.Tensor, N: int) -> torch.Tensor:
-    """Unpack the bricked spectrum to natural order [2N, 2N, N+1] (for inspection and tests)."""
-    xb = N // 4 + 1
-    b = dft.view(N, N, xb, 2, 2, 4).permute(0, 3, 1, 4, 2, 5).reshape(2 * N, 2 * N, 4 * xb)
-    return b[..., :N + 1].contiguous()
+def volume_dft_natural(dft: torch.Tensor, N: int, pad: bool = True) -> torch.Tensor:
+    """Unpack the bricked spectrum to natural order [L, L, L/2+1] (for inspection and tests)."""
+    L = 2 * N if pad else N
+    xb = L // 8 + 1
+    b = dft.view(L // 2, L // 2, xb, 2, 2, 4).permute(0, 3, 1, 4, 2, 5).reshape(L, L, 4 * xb)
+    return b[..., :L // 2 + 1].contiguous()
+
+
+def _spectrum_box(dft: torch.Tensor, pad: bool) -> int:
+    N = next((b for b in SUPPORTED_BOXES if int(lib().gis_volume_dft_bytes_ex(b, int(pad))) == dft.numel() * 8), 0)
+    if not N or dft.dtype != torch.complex64:
+        raise ValueError("dft must be a bricked spectrum from volume_dft (with the same pad)")
+    return N
+
+
+def project_slices(dft: torch.Tensor, rts: torch.Tensor, whiten: bool = False, normalize: bool = False,
+                   eps: float = 1e-12, pad: bool = True, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """[M, N, N] float32 real-space central-slice projections of the volume
+    whose spectrum is ``dft``; optionally ring-divided (whiten) and plane
+    normalised (normalize)."""
+    require_hip(dft, "dft")
+    require_hip(rts, "rts")
+    N = _spectrum_box(dft, pad)
+    M = int(rts.shape[0])
+    if tuple(rts.shape[1:]) != (3, 3) or rts.dtype != torch.float32 or not rts.is_contiguous():
+        raise ValueError("rts must be contiguous float32 [M, 3, 3]")
+    if out is None:
+        out = torch.empty((M, N, N), dtype=torch.float32, device=dft.device)
+    elif tuple(out.shape) != (M, N, N) or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError("out must be contiguous float32 [M, N, N]")
+    wsb = int(lib().gis_build_templates_workspace_ex(N, M, int(bool(pad))))
+    ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dft.device)
+    check(lib().gis_project_slices(ptr(dft), N, int(bool(pad)), ptr(rts), M, int(bool(whiten)), int(bool(normalize)),
+                                   float(eps), ptr(out), ptr(ws), wsb, stream_of(dft.device)), "gis_project_slices")
+    return out
+
+
+def natural_to_bricked(dft: torch.Tensor) -> torch.Tensor:
+    """A natural-order [L, L, L/2+1] complex64 half spectrum (rows and planes
+    in natural frequency order) rearranged into the bricked layout."""
+    L = int(dft.shape[0])
+    if tuple(dft.shape) != (L, L, L // 2 + 1) or L % 8 or dft.dtype != torch.complex64:
+        raise ValueError("dft must be complex64 [L, L, L/2+1], L a multiple of 8")
+    xb = L // 8 + 1
+    padded = torch.zeros((L, L, 4 * xb), dtype=dft.dtype, device=dft.device)
+    padded[..., :L // 2 + 1] = dft
+    return padded.view(L // 2, 2, L // 2, 2, xb, 4).permute(0, 2, 4, 1, 3, 5).contiguous().view(-1)
+
+
+def extract_slices(dft_bricked: torch.Tensor, L: int, rts: torch.Tensor) -> torch.Tensor:
+    """[M, L, L/2+1] complex64 central slices (rows fftshifted) sampled from a
+    bricked half spectrum of transform length L."""
+    require_hip(dft_bricked, "dft")
+    require_hip(rts, "rts")
+    L = int(L)
+    if dft_bricked.dtype != torch.complex64 or dft_bricked.numel() != L * L * (L // 8 + 1) * 4:
+        raise ValueError("dft must be a bricked complex64 spectrum of length L")
+    M = int(rts.shape[0])
+    if tuple(rts.shape[1:]) != (3, 3) or rts.dtype != torch.float32 or not rts.is_contiguous():
+        raise ValueError("rts must be contiguous float32 [M, 3, 3]")
+    wsb = int(lib().gis_extract_slices_workspace(L, max(M, 1)))
+    if wsb == 0:
+        raise ValueError(f"transform length {L} is not a supported box or twice one")
+    out = torch.empty((M, L, L // 2 + 1), dtype=torch.complex64, device=dft_bricked.device)
+    ws = torch.empty((wsb,), dtype=torch.uint8, device=dft_bricked.device)
+    check(lib().gis_extract_slices(ptr(dft_bricked), L, ptr(rts), M, ptr(out), ptr(ws), wsb,
+                                   stream_of(dft_bricked.device)), "gis_extract_slices")
+    return out
+
+
+def whiten_images(images: torch.Tensor, whiten: bool = True, normalize: bool = True, eps: float = 1e-12,
+                  out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Ring division of the spectrum (whiten) and / or least-squares plane +
+    (x - mean) / std (normalize) of [M, N, N] (or [N, N]) float32 images."""
+    require_hip(images, "images")
+    squeeze = images.ndim == 2
+    img = images[None] if squeeze else images
+    if img.ndim != 3 or img.shape[1] != img.shape[2] or img.dtype != torch.float32:
+        raise ValueError("images must be float32 [M, N, N] or [N, N]")
+    N, M = int(img.shape[1]), int(img.shape[0])
+    if N not in SUPPORTED_BOXES:
+        raise ValueError(f"image size {N} not in {SUPPORTED_BOXES}")
+    if not (whiten or normalize):
+        raise ValueError("nothing to do: whiten and normalize are both off")
+    img = img.contiguous()
+    if out is None:
+        res = torch.empty_like(img)
+    else:
+        res = out[None] if out.ndim == 2 else out
+        if tuple(res.shape) != tuple(img.shape) or res.dtype != torch.float32 or not res.is_contiguous():
+            raise ValueError("out must be contiguous float32 of the images' shape")
+        if res.data_ptr() == img.data_ptr():
+            raise ValueError("out may not alias the input")
+    wsb = int(lib().gis_build_templates_workspace(N, M))
+    ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=img.device)
+    check(lib().gis_whiten_images(ptr(img), N, M, int(bool(whiten)), int(bool(normalize)), float(eps), ptr(res),
+                                  ptr(ws), wsb, stream_of(img.device)), "gis_whiten_images")
+    return res[0] if squeeze else res
 
 
 def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Optional[torch.Tensor],

@@ -1,0 +1,188 @@
+"""Standalone projector: real-space central-slice projections of a model
+volume at the orientations of a RELION STAR file, optionally whitened
+(project3d_and_whiten_cuda.py, same CLI and library names).
+
+Every projection comes from the template kernel's real-space modes
+(``kernels.project_slices``): one launch for all orientations, with the pad /
+sinc^2 / rfftn prefix of ``kernels.volume_dft`` (unpadded for
+``--skip_padding``).  ``getSpectrum_divideBySpectrum_torch`` and
+``normalize_torch`` run the same ring division and plane normalisation on
+given images (``kernels.whiten_images``); ``extract_central_slices_rfft``
+samples central slices of a natural-layout spectrum (``kernels.extract_slices``).
+There is no CPU path: tensors must live on the HIP device.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import time
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import host, kernels, mrc, star
+
+__all__ = ["create_project3d_parser", "Euler_angles2matrix", "build_radius_map_torch", "build_xy_grid_torch",
+           "getSpectrum_divideBySpectrum_torch", "normalize_torch", "extract_central_slices_rfft",
+           "run_project3d", "extract_filename", "main"]
+
+
+def create_project3d_parser() -> argparse.ArgumentParser:
+    """Flags of project3d_and_whiten_cuda.py:15-23."""
+    parser = argparse.ArgumentParser(description="Project models from relion angle star then whitening them.")
+    parser.add_argument("--i", type=str, required=True, help="Input model in mrc form.")
+    parser.add_argument("--ang", type=str, required=True, help="The RELION angle starfile.")
+    parser.add_argument("--o", type=str, required=True, help="The output starfile name.")
+    parser.add_argument("--skip_padding", action="store_true", help="Skip the padding process. default = False")
+    parser.add_argument("--normal_background_powerspectrum", action="store_true",
+                        help="Apply whitening filter to output projections. default = False")
+    parser.add_argument("--gpuid", type=str, default=None,
+                        help="The gpuid, only one gpuid should be given. If not given, the current HIP device.")
+    return parser
+
+
+def Euler_angles2matrix(alpha, beta, gamma) -> List[List[float]]:
+    """RELION ZYZ Euler matrix in degrees with pi written 3.14159265359
+    (project3d_and_whiten_cuda.py:700-722)."""
+    return host.euler_angles_to_matrix(float(alpha), float(beta), float(gamma))
+
+
+def build_radius_map_torch(ysize: int, xsize: int, device) -> torch.Tensor:
+    """round(sqrt((y - ysize//2)^2 + (x - xsize//2)^2)) as int64 (:24-31)."""
+    y = torch.arange(ysize, device=device, dtype=torch.float32)
+    x = torch.arange(xsize, device=device, dtype=torch.float32)
+    yy, xx = torch.meshgrid(y, x, indexing="ij")
+    return torch.round(torch.sqrt((yy - ysize // 2) ** 2 + (xx - xsize // 2) ** 2)).long()
+
+
+def build_xy_grid_torch(ysize: int, xsize: int, device):
+    """(row index grid, column index grid) in float32 (:33-37)."""
+    y = torch.arange(ysize, device=device, dtype=torch.float32)
+    x = torch.arange(xsize, device=device, dtype=torch.float32)
+    yy, xx = torch.meshgrid(y, x, indexing="ij")
+    return yy, xx
+
+
+def _image_2d(img: torch.Tensor, name: str) -> torch.Tensor:
+    if not isinstance(img, torch.Tensor) or img.ndim != 2:
+        raise ValueError(f"{name} must be 2D")
+    if img.shape[0] != img.shape[1]:
+        raise ValueError(f"{name}: only square images are supported (got {tuple(img.shape)})")
+    kernels.require_hip(img, name)
+    return img.to(torch.float32).contiguous()
+
+
+def getSpectrum_divideBySpectrum_torch(img: torch.Tensor, radius_map: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Ring-mean |F| whitening of one real image (:196-230): divide the
+    centred spectrum by its ring mean (DC ring untouched), inverse, real part.
+    ``radius_map``, when given, must be the default ring map."""
+    im = _image_2d(img, "img")
+    if radius_map is not None:
+        want = build_radius_map_torch(im.shape[0], im.shape[1], im.device)
+        if tuple(radius_map.shape) != tuple(want.shape) or not torch.equal(radius_map.to(want.device).long(), want):
+            raise ValueError("only the default ring map round(sqrt((y-H//2)^2 + (x-W//2)^2)) is supported")
+    return kernels.whiten_images(im, whiten=True, normalize=False)
+
+
+def normalize_torch(img: torch.Tensor, xx: Optional[torch.Tensor] = None, yy: Optional[torch.Tensor] = None,
+                    eps: float = 1e-12) -> torch.Tensor:
+    """Least-squares plane removal then (x - mean) / std, or x - mean when
+    std <= eps (:231-291).  ``xx`` / ``yy``, when given, must be the index
+    grids of the image (in either role: a plane fitted over the whole grid
+    does not depend on which grid is called x)."""
+    im = _image_2d(img, "img")
+    if (xx is None) != (yy is None):
+        raise ValueError("give both xx and yy, or neither")
+    if xx is not None:
+        gy, gx = build_xy_grid_torch(im.shape[0], im.shape[1], im.device)
+        a, b = xx.to(im.device, torch.float32), yy.to(im.device, torch.float32)
+        if not ((torch.equal(a, gx) and torch.equal(b, gy)) or (torch.equal(a, gy) and torch.equal(b, gx))):
+            raise ValueError("xx / yy must be the image's row and column index grids")
+    return kernels.whiten_images(im, whiten=False, normalize=True, eps=eps)
+
+
+def extract_central_slices_rfft(dft: torch.Tensor, image_shape, rotation_matrices: torch.Tensor,
+                                rotation_matrix_zyx: bool) -> torch.Tensor:
+    """Central slices of an fftshifted rfft (:397-435): ``dft`` [L, L, L/2+1]
+    with planes and rows fftshifted, ``image_shape`` the real (L, L, L)
+    shape, ``rotation_matrices`` [..., 3, 3] acting on xyz vectors (or zyx
+    when ``rotation_matrix_zyx``).  Returns [..., L, L/2+1] complex64 with
+    rows fftshifted."""
+    kernels.require_hip(dft, "dft")
+    L = int(dft.shape[0])
+    if tuple(int(v) for v in image_shape) != (L, L, L) or tuple(dft.shape) != (L, L, L // 2 + 1):
+        raise ValueError("dft must be the [L, L, L/2+1] rfft of a cubic image of shape (L, L, L)")
+    R = torch.as_tensor(rotation_matrices).to(dft.device, torch.float32)
+    lead = tuple(R.shape[:-2])
+    R = R.reshape(-1, 3, 3)
+    if rotation_matrix_zyx:
+        R = R.flip(-1).flip(-2)  # the same rotation acting on xyz vectors
+    natural = torch.fft.ifftshift(dft.to(torch.complex64), dim=(0, 1))
+    out = kernels.extract_slices(kernels.natural_to_bricked(natural.contiguous()), L, R.contiguous())
+    return out.reshape(*lead, L, L // 2 + 1)
+
+
+def extract_filename(file_path: str) -> str:
+    """Base name without its last extension (:724-729)."""
+    base = os.path.basename(file_path)
+    parts = base.split(".")
+    return ".".join(parts[:-1]) if len(parts) > 1 else base
+
+
+def _device(gpuid) -> torch.device:
+    if gpuid is not None:
+        dev = torch.device("cuda", int(gpuid))
+        torch.cuda.set_device(dev)
+        return dev
+    if not torch.cuda.is_available():
+        raise RuntimeError("the projector runs on the HIP device only; no GPU is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def run_project3d(args: argparse.Namespace, *, volume: Optional[np.ndarray] = None) -> Dict[str, float]:
+    """The main of project3d_and_whiten_cuda.py (:39-190): projections of
+    ``args.i`` (or ``volume``) at the STAR file's (rot, tilt), psi ignored,
+    written to ``<o>.mrcs`` with ``<o>.star`` = the input header and rows,
+    each _rlnImageName replaced by ``<row>@<o>.mrcs`` (tab-separated).
+    Rows after the particle loop are not copied."""
+    lines = star.read_text_lines(args.ang)
+    loop = star.find_star_loop(lines, ("_rlnImageName", "_rlnAngleRot", "_rlnAngleTilt", "_rlnAnglePsi"))
+    c_img, c_rot, c_tilt = (loop.labels[k] for k in ("_rlnImageName", "_rlnAngleRot", "_rlnAngleTilt"))
+    rows = loop.data_line_indices
+    rot = np.array([float(loop.tokens(r)[c_rot]) for r in rows])
+    tilt = np.array([float(loop.tokens(r)[c_tilt]) for r in rows])
+    vol = np.asarray(volume if volume is not None else mrc.read(args.i), np.float32)
+    if vol.ndim != 3 or len(set(vol.shape)) != 1:
+        raise ValueError(f"the model must be a cubic volume (got {vol.shape})")
+    N = int(vol.shape[0])
+    dev = _device(getattr(args, "gpuid", None))
+    pad = not args.skip_padding
+    whiten = bool(args.normal_background_powerspectrum)
+    out_base = extract_filename(args.o)
+    out_mrcs, out_star = out_base + ".mrcs", out_base + ".star"
+    t0 = time.time()
+    v = torch.from_numpy(np.ascontiguousarray(vol)).to(dev)
+    dft = kernels.volume_dft(v, pad=pad)
+    rts = torch.from_numpy(host.rotation_matrices_t(rot, tilt).astype(np.float32)).to(dev).contiguous()
+    proj = kernels.project_slices(dft, rts, whiten=whiten, normalize=whiten, pad=pad)
+    stack = proj.cpu().numpy()
+    t1 = time.time()
+    mrc.write_mrc(out_mrcs, stack, voxel_size=0.0)
+    with open(out_star, "w") as fh:
+        fh.writelines(lines[:loop.data_start])
+        for r in rows:
+            tok = loop.tokens(r)
+            tok[c_img] = f"{r - loop.data_start + 1}@{out_mrcs}"
+            fh.write("\t".join(tok) + "\n")
+    t2 = time.time()
+    print("Project3D, total execution time = ", round(t2 - t0, 4), " seconds")
+    return {"projections": float(len(rows)), "box": float(N), "gpu_s": t1 - t0, "total_s": t2 - t0}
+
+
+def main(argv: Optional[Sequence[str]] = None) -> None:
+    run_project3d(create_project3d_parser().parse_args(argv))
+
+
+if __name__ == "__main__":
+    main()

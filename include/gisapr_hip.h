@@ -62,8 +62,21 @@ size_t gis_volume_dft_bytes(int N);
 size_t gis_volume_dft_workspace(int N);
 int gis_volume_dft(const float* volume, int N, void* dft, void* ws, size_t ws_bytes, void* stream);
 
+/*
+ * The same transform padded (pad 1, L = 2N, as above) or unpadded (pad 0,
+ * L = N: project3d_and_whiten_cuda.py --skip_padding, :103-117): the
+ * [L][L][L/2+1] half spectrum in bricks of 2 (z) x 2 (y) x 4 (x) values,
+ *   ((((kz/2) * L/2 + ky/2) * XB + kx/4) * 2 + kz%2) * 2 + ky%2) * 4 + kx%4,
+ * XB = L/8 + 1, with sinc^2 over fftshift(fftfreq(L)).
+ */
+size_t gis_volume_dft_bytes_ex(int N, int pad);
+size_t gis_volume_dft_workspace_ex(int N, int pad);
+int gis_volume_dft_ex(const float* volume, int N, int pad, void* dft, void* ws, size_t ws_bytes, void* stream);
+
 /* Workspace bytes gis_build_templates needs for M directions at box N. */
 size_t gis_build_templates_workspace(int N, int M);
+/* The same for the padded (pad 1) or unpadded (pad 0) slice transform. */
+size_t gis_build_templates_workspace_ex(int N, int M, int pad);
 
 /*
  * (b) Template bank.  Replaces the per-direction loop of
@@ -100,6 +113,42 @@ int gis_build_templates(const void* dft, int N, int n, const float* rts, int M, 
 int gis_build_template_coefficients(const void* dft, int N, int n, const float* rts, int M, const float* mask,
                                     const float* fsc, const int* kslot, int K, int Kpad, void* coeffs, float* norms,
                                     void* ws, size_t ws_bytes, void* stream);
+
+/*
+ * Standalone projector (project3d_and_whiten_cuda.py): real-space central
+ * slice projections.  Replaces the per-angle loop of its main (:119-147:
+ * extract_central_slices_rfft, ifftshift, irfftn, ifftshift, crop) and, with
+ * --normal_background_powerspectrum, getSpectrum_divideBySpectrum_torch
+ * (:196-230, whiten) then normalize_torch (:231-291, normz: least-squares
+ * plane, then (x - mean) / std when std > eps, else x - mean).
+ *   dft  [L][L][L/2+1] bricked, from gis_volume_dft_ex(volume, N, pad, ..)
+ *   rts  [M][3][3] float32 R = Euler(rot, tilt, 0)^T
+ *   proj [M][N][N] float32 out;  ws: gis_build_templates_workspace_ex(N, M, pad)
+ */
+int gis_project_slices(const void* dft, int N, int pad, const float* rts, int M, int whiten, int normz, float eps,
+                       float* proj, void* ws, size_t ws_bytes, void* stream);
+
+/*
+ * getSpectrum_divideBySpectrum_torch (whiten) and / or normalize_torch
+ * (normz) of project3d_and_whiten_cuda.py:196-291 on M real [N][N] images
+ * (at least one of the two).  img, out [M][N][N] float32 (out may not alias
+ * img); ws: gis_build_templates_workspace(N, M).
+ */
+int gis_whiten_images(const float* img, int N, int M, int whiten, int normz, float eps, float* out, void* ws,
+                      size_t ws_bytes, void* stream);
+
+/*
+ * extract_central_slices_rfft (project3d_and_whiten_cuda.py:397-435; the
+ * sampling of v606:1099-1113): trilinear, border-clamped samples of a
+ * bricked [L][L][L/2+1] half spectrum at M rotated central slices,
+ * conjugated where the rotated x < 0.  rts [M][3][3] act on (x, y, z)
+ * frequency vectors; slices [M][L][L/2+1] complex64 out with rows in
+ * fftshifted order; L a supported box or twice one;
+ * ws: gis_extract_slices_workspace(L, M).
+ */
+size_t gis_extract_slices_workspace(int L, int M);
+int gis_extract_slices(const void* dft, int L, const float* rts, int M, void* slices, void* ws, size_t ws_bytes,
+                       void* stream);
 
 /* Workspace bytes gis_prepare_particles needs for `particles` (= B) particles. */
 size_t gis_prepare_particles_workspace(int N, int particles);

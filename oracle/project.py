@@ -65,11 +65,13 @@ def sinc2_weights(n2: int, dtype=np.float32) -> np.ndarray:
     return (np.sinc(r) ** 2).astype(dtype)
 
 
-def volume_dft(volume, dtype=np.float32) -> np.ndarray:
-    """v606:1068-1075: pad N/2, sinc^2, fftshift, rfftn, fftshift(z, y)."""
+def volume_dft(volume, dtype=np.float32, pad=True) -> np.ndarray:
+    """v606:1068-1075 / project3d_and_whiten_cuda.py:103-117: pad N/2 (unless
+    --skip_padding), sinc^2 over the (padded) shape, fftshift, rfftn,
+    fftshift(z, y)."""
     n = volume.shape[0]
-    vol = np.pad(np.asarray(volume, dtype), n // 2)
-    vol = vol * sinc2_weights(2 * n, dtype)
+    vol = np.pad(np.asarray(volume, dtype), n // 2) if pad else np.asarray(volume, dtype)
+    vol = vol * sinc2_weights(vol.shape[0], dtype)
     dft = np.fft.rfftn(np.fft.fftshift(vol))
     return np.fft.fftshift(dft, axes=(0, 1))
 
@@ -123,14 +125,16 @@ def central_slice(dft: np.ndarray, rt: np.ndarray, dtype=np.float32) -> np.ndarr
     return s.astype(np.complex64 if dtype == np.float32 else np.complex128)
 
 
-def slice_to_projection(s: np.ndarray, dtype=np.float32) -> np.ndarray:
-    """v606:1122-1130: ifftshift rows, irfftn, ifftshift, crop the padding."""
+def slice_to_projection(s: np.ndarray, dtype=np.float32, box=None) -> np.ndarray:
+    """v606:1122-1130 / project3d_and_whiten_cuda.py:127-132: ifftshift rows,
+    irfftn, ifftshift, crop the padding (box = the unpadded size; default
+    half the slice length)."""
     n2 = s.shape[0]
-    n = n2 // 2
-    p = np.fft.irfft2(np.fft.ifftshift(s, axes=0), axes=(0, 1))
+    n = n2 // 2 if box is None else box
+    p = np.fft.irfft2(np.fft.ifftshift(s, axes=0), s=(n2, n2), axes=(0, 1))
     p = np.fft.ifftshift(p, axes=(0, 1))
-    pad = n // 2
-    return p[pad:pad + n, pad:pad + n].astype(dtype)
+    off = (n2 - n) // 2
+    return p[off:off + n, off:off + n].astype(dtype)
 
 
 def ring_map(n: int) -> np.ndarray:
@@ -139,12 +143,10 @@ def ring_map(n: int) -> np.ndarray:
     return np.round(np.sqrt((y - n // 2) ** 2 + (x - n // 2) ** 2)).astype(np.int64)
 
 
-def whiten(img: np.ndarray, dtype=np.float32) -> np.ndarray:
-    """v606:938-992 (batch == 1 path) / p3d:196-291.
-
-    Ring mean of |F| -> divide (DC ring untouched) -> inverse; least-squares
-    plane with x = row index and y = column index; then (x-mean)/std.
-    """
+def divide_by_spectrum(img: np.ndarray, dtype=np.float32) -> np.ndarray:
+    """getSpectrum_divideBySpectrum_torch (project3d_and_whiten_cuda.py:196-230;
+    the first half of v606:938-992): ring mean of |F| -> divide (DC ring
+    untouched, empty rings 1) -> inverse, real part."""
     n = img.shape[0]
     cdt = np.complex64 if dtype == np.float32 else np.complex128
     F = np.fft.fftshift(np.fft.fft2(img.astype(dtype))).astype(cdt)
@@ -158,9 +160,19 @@ def whiten(img: np.ndarray, dtype=np.float32) -> np.ndarray:
     div[nz] = cnt[nz] / spec[nz]
     div[0] = 1.0
     F = F * div[rm]
-    im = np.fft.ifft2(np.fft.ifftshift(F)).real.astype(dtype)
+    return np.fft.ifft2(np.fft.ifftshift(F)).real.astype(dtype)
+
+
+def normalize_plane(img: np.ndarray, dtype=np.float32, eps=1e-12) -> np.ndarray:
+    """normalize_torch (project3d_and_whiten_cuda.py:231-291; the second half
+    of v606:938-992): least-squares plane with x = row index and y = column
+    index (the call-site swap of v606:916-918 / p3d:84-85; a plane fitted
+    over the whole grid is the same either way), subtract, then
+    (x - mean) / std (population) if std > eps, else x - mean."""
+    n = img.shape[0]
+    im = img.astype(dtype)
     yy, xx = np.indices((n, n)).astype(dtype)
-    cx, cy = yy, xx  # the call-site swap kept by v606:916-918
+    cx, cy = yy, xx
     x, y, z = cx.ravel(), cy.ravel(), im.ravel()
     A = np.array([[np.sum(x * x), np.sum(x * y), np.sum(x)],
                   [np.sum(x * y), np.sum(y * y), np.sum(y)],
@@ -170,11 +182,17 @@ def whiten(img: np.ndarray, dtype=np.float32) -> np.ndarray:
     im = im - (coef[0] * cx + coef[1] * cy + coef[2])
     mean = im.mean(dtype=np.float64)
     std = im.std(dtype=np.float64)
-    if std > 1e-12:
+    if std > eps:
         im = (im - mean) / std
     else:
         im = im - mean
     return im.astype(dtype)
+
+
+def whiten(img: np.ndarray, dtype=np.float32) -> np.ndarray:
+    """v606:938-992 (batch == 1 path) / p3d:196-291: divide_by_spectrum then
+    normalize_plane."""
+    return normalize_plane(divide_by_spectrum(img, dtype), dtype)
 
 
 def template_from_projection(proj, new_box, mask, packed_fsc, dtype=np.float32):
@@ -207,3 +225,19 @@ def build_templates(volume, rot, tilt, new_box, mask, packed_fsc, dtype=np.float
         out.append(template_from_projection(proj, new_box, mask, packed_fsc, dtype))
     bank = np.stack(out)
     return (bank, np.stack(projs)) if return_projections else bank
+
+
+def project3d(volume, rot, tilt, whitening=False, pad=True, dtype=np.float32) -> np.ndarray:
+    """The projection loop of project3d_and_whiten_cuda.py main (:103-147):
+    [M, N, N] real projections at Euler(rot, tilt, 0), optionally ring-divided
+    and plane-normalised (--normal_background_powerspectrum); pad=False is
+    --skip_padding."""
+    n = volume.shape[0]
+    dft = volume_dft(volume, dtype, pad=pad)
+    out = []
+    for rt in rotation_matrices_t(rot, tilt):
+        p = slice_to_projection(central_slice(dft, rt, dtype), dtype, box=n)
+        if whitening:
+            p = normalize_plane(divide_by_spectrum(p, dtype), dtype)
+        out.append(p)
+    return np.stack(out)

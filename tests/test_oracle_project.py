@@ -108,6 +108,84 @@ def test_whitening_invariants():
     np.testing.assert_allclose(ring_mean[1:], 1.0, rtol=1e-12)
 
 
+def test_volume_dft_unpadded_brute_force_tiny():
+    """--skip_padding (project3d_and_whiten_cuda.py:103-117): no zeros, sinc^2
+    over fftshift(fftfreq(N)) of the N grid, then fftshift / rfftn / fftshift."""
+    N = 6
+    rng = np.random.default_rng(7)
+    vol = rng.standard_normal((N, N, N))
+    got = opj.volume_dft(vol, np.float64, pad=False)
+    f = (np.arange(N) - N // 2) / N
+    r = np.sqrt(f[:, None, None] ** 2 + f[None, :, None] ** 2 + f[None, None, :] ** 2)
+    src = np.fft.fftshift(vol * np.sinc(r) ** 2)
+    idx = np.arange(N)
+    zz, yy, xx = np.meshgrid(idx, idx, idx, indexing="ij")
+    want = np.zeros((N, N, N // 2 + 1), complex)
+    for kz in range(N):
+        for ky in range(N):
+            for kx in range(N // 2 + 1):
+                want[kz, ky, kx] = np.sum(src * np.exp(-2j * np.pi * (kz * zz + ky * yy + kx * xx) / N))
+    np.testing.assert_allclose(got, np.fft.fftshift(want, axes=(0, 1)), atol=1e-9)
+
+
+@pytest.mark.parametrize("pad", [True, False])
+def test_project3d_identity_orientation_is_weighted_z_sum(pad):
+    """project3d at rot = tilt = 0 samples grid nodes only: the image is the
+    z-sum of the sinc^2-weighted (padded) volume, cropped to N."""
+    N = 16
+    rng = np.random.default_rng(8)
+    vol = rng.standard_normal((N, N, N))
+    got = opj.project3d(vol, [0.0], [0.0], pad=pad, dtype=np.float64)[0]
+    L = 2 * N if pad else N
+    o = (L - N) // 2
+    full = np.pad(vol, o) * opj.sinc2_weights(L, np.float64)
+    np.testing.assert_allclose(got, full.sum(0)[o:o + N, o:o + N], atol=1e-9)
+
+
+def test_project3d_unpadded_tilt_270_is_weighted_x_sum():
+    """Unpadded, tilt 270 projects along x on grid nodes: the image is the
+    x-sum of the weighted volume with rows y and columns z (band-limited
+    blobs keep the Nyquist border clamp irrelevant)."""
+    N = 32
+    zz, yy, xx = np.indices((N, N, N)).astype(float)
+    vol = np.zeros((N, N, N))
+    for (cz, cy, cx), a in (((13, 18, 14), 1.0), ((19, 12, 16), 0.7)):
+        vol += a * np.exp(-((zz - cz) ** 2 + (yy - cy) ** 2 + (xx - cx) ** 2) / (2 * 2.5 ** 2))
+    got = opj.project3d(vol, [0.0], [270.0], pad=False, dtype=np.float64)[0]
+    want = (vol * opj.sinc2_weights(N, np.float64)).sum(2).T
+    np.testing.assert_allclose(got, want, atol=1e-6 * np.abs(want).max())
+
+
+def test_divide_by_spectrum_flattens_every_ring():
+    """getSpectrum_divideBySpectrum_torch: the output's own spectrum has ring
+    mean |F| = 1 on every ring but DC (whose mean is left as it was)."""
+    rng = np.random.default_rng(9)
+    n = 32
+    img = rng.standard_normal((n, n)).cumsum(0).cumsum(1)
+    out = opj.divide_by_spectrum(img, np.float64)
+    F = np.fft.fftshift(np.fft.fft2(out))
+    rm = opj.ring_map(n)
+    cnt = np.bincount(rm.ravel())
+    ring_mean = np.bincount(rm.ravel(), np.abs(F).ravel()) / cnt
+    np.testing.assert_allclose(ring_mean[1:], 1.0, rtol=1e-10)
+    F0 = np.fft.fftshift(np.fft.fft2(img))
+    assert abs(ring_mean[0] - np.abs(F0[rm == 0]).mean()) < 1e-9 * abs(ring_mean[0])
+
+
+def test_normalize_plane_invariants_and_pure_plane():
+    """normalize_torch: zero mean, unit population std, no residual plane;
+    a pure plane leaves std = 0 < eps and maps to zero."""
+    rng = np.random.default_rng(10)
+    n = 32
+    yy, xx = np.indices((n, n)).astype(float)
+    img = 3.0 * yy - 2.0 * xx + 5.0 + rng.standard_normal((n, n))
+    out = opj.normalize_plane(img, np.float64)
+    assert abs(out.mean()) < 1e-10 and abs(out.std() - 1.0) < 1e-10
+    assert abs((out * yy).sum()) < 1e-7 and abs((out * xx).sum()) < 1e-7
+    flat = opj.normalize_plane(3.0 * yy - 2.0 * xx + 5.0, np.float64)
+    assert np.abs(flat).max() < 1e-9
+
+
 def test_template_is_unit_norm_and_preshifted():
     N = 16
     rng = np.random.default_rng(6)
