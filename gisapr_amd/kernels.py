@@ -137,6 +137,47 @@ def extract_slices(dft_bricked: torch.Tensor, L: int, rts: torch.Tensor) -> torc
     return out
 
 
+def sinc2_filter_(volume: torch.Tensor) -> torch.Tensor:
+    """volume *= sinc^2(|f|) in place (f = fftshift(fftfreq) per axis) for a
+    contiguous float32 [D, H, W] volume; returns the volume."""
+    require_hip(volume, "volume")
+    if volume.ndim != 3:
+        raise ValueError("volume must be 3-D")
+    if volume.dtype != torch.float32 or not volume.is_contiguous():
+        raise ValueError("volume must be a contiguous float32 tensor")
+    if volume.numel():
+        D, H, W = (int(v) for v in volume.shape)
+        check(lib().gis_sinc2_filter(ptr(volume), D, H, W, stream_of(volume.device)), "gis_sinc2_filter")
+    return volume
+
+
+def slice_sample_grid(base_zyx: torch.Tensor, rts: torch.Tensor, image_shape, dft_shape):
+    """grid_sample coordinates [M, *base, 3] (x, y, z order, align_corners=True)
+    of M rotations of the base frequency points ``base_zyx`` [..., 3] (z, y, x)
+    in an rfft volume of shape ``dft_shape`` taken from a real image of shape
+    ``image_shape``, and the conjugation mask [M, *base] (bool)."""
+    require_hip(base_zyx, "base_zyx")
+    require_hip(rts, "rotation_matrices")
+    if base_zyx.dtype != torch.float32 or base_zyx.shape[-1] != 3:
+        raise ValueError("base grid must be float32 [..., 3]")
+    if rts.dtype != torch.float32 or rts.ndim != 3 or tuple(rts.shape[1:]) != (3, 3):
+        raise ValueError("rotation matrices must be float32 [M, 3, 3]")
+    img = tuple(int(v) for v in image_shape)
+    dsh = tuple(int(v) for v in dft_shape)
+    if len(img) != 3 or len(dsh) != 3:
+        raise ValueError("image_shape and dft_shape must have three dimensions")
+    lead = tuple(base_zyx.shape[:-1])
+    M, P = int(rts.shape[0]), int(base_zyx.numel() // 3)
+    grid = torch.empty((M, *lead, 3), dtype=torch.float32, device=base_zyx.device)
+    conj = torch.empty((M, *lead), dtype=torch.bool, device=base_zyx.device)
+    if M and P:
+        base = base_zyx.contiguous()
+        R = rts.contiguous()
+        check(lib().gis_slice_sample_grid(ptr(base), P, ptr(R), M, *img, *dsh, ptr(grid), ptr(conj),
+                                          stream_of(base.device)), "gis_slice_sample_grid")
+    return grid, conj
+
+
 def whiten_images(images: torch.Tensor, whiten: bool = True, normalize: bool = True, eps: float = 1e-12,
                   out: Optional[torch.Tensor] = None) -> torch.Tensor:
     """Ring division of the spectrum (whiten) and / or least-squares plane +

@@ -9,6 +9,8 @@ sinc^2 / rfftn prefix of ``kernels.volume_dft`` (unpadded for
 ``normalize_torch`` run the same ring division and plane normalisation on
 given images (``kernels.whiten_images``); ``extract_central_slices_rfft``
 samples central slices of a natural-layout spectrum (``kernels.extract_slices``).
+``run_projection`` and ``apply_sinc2_filter_inplace_3d`` are the library form
+of project3d_and_whiten_cuda_nowrite.py.
 There is no CPU path: tensors must live on the HIP device.
 """
 from __future__ import annotations
@@ -22,14 +24,16 @@ import numpy as np
 import torch
 
 from . import host, kernels, mrc, star
+from .projection import apply_sinc2_filter_inplace_3d  # noqa: F401  (project3d_and_whiten_cuda_nowrite.py:42-81)
 
 __all__ = ["create_project3d_parser", "Euler_angles2matrix", "build_radius_map_torch", "build_xy_grid_torch",
            "getSpectrum_divideBySpectrum_torch", "normalize_torch", "extract_central_slices_rfft",
-           "run_project3d", "extract_filename", "main"]
+           "apply_sinc2_filter_inplace_3d", "run_project3d", "run_projection", "extract_filename", "main"]
 
 
 def create_project3d_parser() -> argparse.ArgumentParser:
-    """Flags of project3d_and_whiten_cuda.py:15-23."""
+    """Flags of project3d_and_whiten_cuda.py:15-23 and the --return_array
+    flag of project3d_and_whiten_cuda_nowrite.py:25."""
     parser = argparse.ArgumentParser(description="Project models from relion angle star then whitening them.")
     parser.add_argument("--i", type=str, required=True, help="Input model in mrc form.")
     parser.add_argument("--ang", type=str, required=True, help="The RELION angle starfile.")
@@ -39,6 +43,8 @@ def create_project3d_parser() -> argparse.ArgumentParser:
                         help="Apply whitening filter to output projections. default = False")
     parser.add_argument("--gpuid", type=str, default=None,
                         help="The gpuid, only one gpuid should be given. If not given, the current HIP device.")
+    parser.add_argument("--return_array", action="store_true",
+                        help="Return numpy array instead of writing the stack. default = False")
     return parser
 
 
@@ -140,44 +146,89 @@ def _device(gpuid) -> torch.device:
     return torch.device("cuda", torch.cuda.current_device())
 
 
+def _read_angles(particle_star: str):
+    lines = star.read_text_lines(particle_star)
+    loop = star.find_star_loop(lines, ("_rlnImageName", "_rlnAngleRot", "_rlnAngleTilt", "_rlnAnglePsi"))
+    c_rot, c_tilt = loop.labels["_rlnAngleRot"], loop.labels["_rlnAngleTilt"]
+    rows = loop.data_line_indices
+    rot = np.array([float(loop.tokens(r)[c_rot]) for r in rows])
+    tilt = np.array([float(loop.tokens(r)[c_tilt]) for r in rows])
+    return lines, loop, rot, tilt
+
+
+def _load_volume(mrc_volume) -> np.ndarray:
+    vol = mrc_volume if isinstance(mrc_volume, np.ndarray) else mrc.read(mrc_volume)
+    vol = np.ascontiguousarray(vol, np.float32)
+    if vol.ndim != 3 or len(set(vol.shape)) != 1:
+        raise ValueError(f"the model must be a cubic volume (got {vol.shape})")
+    return vol
+
+
+def _project(vol: np.ndarray, rot, tilt, pad: bool, whiten: bool, dev: torch.device) -> np.ndarray:
+    """[M, N, N] float32 projections: one volume transform, one template
+    kernel launch for every orientation (whitened in the same launch)."""
+    if len(rot) == 0:
+        return np.zeros((0,) + vol.shape[1:], np.float32)
+    dft = kernels.volume_dft(torch.from_numpy(vol).to(dev), pad=pad)
+    rts = torch.from_numpy(host.rotation_matrices_t(rot, tilt).astype(np.float32)).to(dev).contiguous()
+    return kernels.project_slices(dft, rts, whiten=whiten, normalize=whiten, pad=pad).cpu().numpy()
+
+
+def _write_star(path: str, lines, loop, mrcs_name: str) -> None:
+    """Input header, then each particle row with _rlnImageName replaced by
+    '<row>@<mrcs_name>', tab-separated (project3d_and_whiten_cuda.py:169-190)."""
+    c_img = loop.labels["_rlnImageName"]
+    with open(path, "w") as fh:
+        fh.writelines(lines[:loop.data_start])
+        for r in loop.data_line_indices:
+            tok = loop.tokens(r)
+            tok[c_img] = f"{r - loop.data_start + 1}@{mrcs_name}"
+            fh.write("\t".join(tok) + "\n")
+
+
 def run_project3d(args: argparse.Namespace, *, volume: Optional[np.ndarray] = None) -> Dict[str, float]:
     """The main of project3d_and_whiten_cuda.py (:39-190): projections of
     ``args.i`` (or ``volume``) at the STAR file's (rot, tilt), psi ignored,
     written to ``<o>.mrcs`` with ``<o>.star`` = the input header and rows,
-    each _rlnImageName replaced by ``<row>@<o>.mrcs`` (tab-separated).
-    Rows after the particle loop are not copied."""
-    lines = star.read_text_lines(args.ang)
-    loop = star.find_star_loop(lines, ("_rlnImageName", "_rlnAngleRot", "_rlnAngleTilt", "_rlnAnglePsi"))
-    c_img, c_rot, c_tilt = (loop.labels[k] for k in ("_rlnImageName", "_rlnAngleRot", "_rlnAngleTilt"))
-    rows = loop.data_line_indices
-    rot = np.array([float(loop.tokens(r)[c_rot]) for r in rows])
-    tilt = np.array([float(loop.tokens(r)[c_tilt]) for r in rows])
-    vol = np.asarray(volume if volume is not None else mrc.read(args.i), np.float32)
-    if vol.ndim != 3 or len(set(vol.shape)) != 1:
-        raise ValueError(f"the model must be a cubic volume (got {vol.shape})")
-    N = int(vol.shape[0])
+    each _rlnImageName replaced by ``<row>@<o>.mrcs`` (tab-separated), both
+    in the working directory (``<o>`` = the base name of --o without its
+    extension).  With --return_array the stack is not written.  Rows after
+    the particle loop are not copied."""
+    lines, loop, rot, tilt = _read_angles(args.ang)
+    vol = _load_volume(volume if volume is not None else args.i)
     dev = _device(getattr(args, "gpuid", None))
-    pad = not args.skip_padding
-    whiten = bool(args.normal_background_powerspectrum)
     out_base = extract_filename(args.o)
     out_mrcs, out_star = out_base + ".mrcs", out_base + ".star"
     t0 = time.time()
-    v = torch.from_numpy(np.ascontiguousarray(vol)).to(dev)
-    dft = kernels.volume_dft(v, pad=pad)
-    rts = torch.from_numpy(host.rotation_matrices_t(rot, tilt).astype(np.float32)).to(dev).contiguous()
-    proj = kernels.project_slices(dft, rts, whiten=whiten, normalize=whiten, pad=pad)
-    stack = proj.cpu().numpy()
+    stack = _project(vol, rot, tilt, not args.skip_padding, bool(args.normal_background_powerspectrum), dev)
     t1 = time.time()
-    mrc.write_mrc(out_mrcs, stack, voxel_size=0.0)
-    with open(out_star, "w") as fh:
-        fh.writelines(lines[:loop.data_start])
-        for r in rows:
-            tok = loop.tokens(r)
-            tok[c_img] = f"{r - loop.data_start + 1}@{out_mrcs}"
-            fh.write("\t".join(tok) + "\n")
+    if not getattr(args, "return_array", False):
+        mrc.write_mrc(out_mrcs, stack, voxel_size=0.0)
+    _write_star(out_star, lines, loop, out_mrcs)
     t2 = time.time()
     print("Project3D, total execution time = ", round(t2 - t0, 4), " seconds")
-    return {"projections": float(len(rows)), "box": float(N), "gpu_s": t1 - t0, "total_s": t2 - t0}
+    return {"projections": float(len(rot)), "box": float(vol.shape[0]), "gpu_s": t1 - t0, "total_s": t2 - t0}
+
+
+def run_projection(particle_star, mrc_volume, whitening, do_skip_padding, do_return_array, gpuid,
+                   output_filename) -> Optional[np.ndarray]:
+    """Library form of the projector (project3d_and_whiten_cuda_nowrite.py:82-238):
+    the STAR file goes to ``output_filename`` as given and the stack to
+    ``output_filename + '.mrcs'``, which the STAR rows name; with
+    ``do_return_array`` the stack is returned as float32 [M, N, N] instead
+    of written (the STAR file is written either way).  ``mrc_volume`` is an
+    MRC path or a cubic array; ``gpuid`` None means the current HIP device."""
+    lines, loop, rot, tilt = _read_angles(particle_star)
+    vol = _load_volume(mrc_volume)
+    dev = _device(gpuid)
+    out_mrcs = str(output_filename) + ".mrcs"
+    t0 = time.time()
+    stack = _project(vol, rot, tilt, not do_skip_padding, bool(whitening), dev)
+    if not do_return_array:
+        mrc.write_mrc(out_mrcs, stack, voxel_size=0.0)
+    _write_star(str(output_filename), lines, loop, out_mrcs)
+    print("Project3D, total execution time = ", round(time.time() - t0, 4), " seconds")
+    return stack if do_return_array else None
 
 
 def main(argv: Optional[Sequence[str]] = None) -> None:

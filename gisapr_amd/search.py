@@ -16,7 +16,9 @@ test1_with_isspa_weight_varingKK_search_translation_also_v606_torch_optimized_st
   psi_sampling_for_record, PsiGridBank, ParticleFourierPreparer,
   PackedCTFTorch, PackedCTFNumPy, ParticleMRCReader, CorrelationWorkspace,
   correlate_templates_and_particle, make_result_records,
-  calculate_angular_distance, euler_angles_to_matrix.
+  calculate_angular_distance, euler_angles_to_matrix, and the grid_sample
+  projector's pieces apply_sinc2_filter_inplace_3d, build_projection_base_grid,
+  rotated_grid_to_grid_sample, ProjectionWhitening.
 
 All array work runs in the HIP kernels (libgisapr_hip.so); there is no CPU
 path -- on a host without a HIP device ``choose_device`` raises.
@@ -34,6 +36,8 @@ import torch
 
 from . import host, kernels, mrc
 from .engine import SearchEngine
+from .projection import (ProjectionWhitening, apply_sinc2_filter_inplace_3d,  # noqa: F401
+                         build_projection_base_grid, rotated_grid_to_grid_sample)
 from .host import (calculate_angular_distance, euler_angles_to_matrix, expand_radial_values,  # noqa: F401
                    generate_soft_mask, psi_sampling_for_record)
 from .star import (ModelAngles, ParticleRecord, ParticleTable, find_star_loop, read_model_angles,  # noqa: F401
@@ -155,6 +159,7 @@ class IntegratedProjector:
         self.real_space_mask = real_space_mask
         self.packed_fsc = packed_fsc
         self.batch_size = max(1, int(projection_batch_size))
+        self.last_orientation_count = self.last_slice_count = 0
 
     def _volume(self, mrc_volume) -> torch.Tensor:
         if isinstance(mrc_volume, torch.Tensor):
@@ -169,18 +174,34 @@ class IntegratedProjector:
             raise ValueError(f"Model volume box {vol.shape[0]} does not match --oriboxsize {self.original_box}")
         return vol.contiguous()
 
-    def build_bank(self, mrc_volume, angles: ModelAngles) -> torch.Tensor:
-        """Unit-norm templates in the UNshifted packed layout (the engine's form)."""
+    def _selected(self, angles: ModelAngles, orientation_indices) -> Optional[List[int]]:
+        if orientation_indices is None:
+            return None
+        sel = sorted(set(int(i) for i in orientation_indices))
+        if any(i < 0 or i >= len(angles.rot) for i in sel):
+            raise ValueError("Projection orientation index is outside the angle table")
+        return sel
+
+    def build_bank(self, mrc_volume, angles: ModelAngles, orientation_indices=None) -> torch.Tensor:
+        """Unit-norm templates in the UNshifted packed layout (the engine's
+        form), one row per orientation, or per sorted unique index of
+        ``orientation_indices`` (the zip release's restricted projection,
+        ZL v7:906-966; only those directions are sampled)."""
+        sel = self._selected(angles, orientation_indices)
+        mats = angles.rotation_matrices_t if sel is None else angles.rotation_matrices_t[sel]
+        self.last_orientation_count = self.last_slice_count = int(len(mats))
         vol = self._volume(mrc_volume)
-        rts = torch.from_numpy(np.ascontiguousarray(angles.rotation_matrices_t)).to(self.device)
+        if len(mats) == 0:
+            return torch.empty((0, self.new_box, self.new_box // 2 + 1), dtype=torch.complex64, device=self.device)
+        rts = torch.from_numpy(np.ascontiguousarray(mats, np.float32)).to(self.device)
         dft = kernels.volume_dft(vol)
         fsc = self.packed_fsc.to(torch.float32).contiguous()
         mask = None if self.real_space_mask is None else self.real_space_mask.to(torch.float32).contiguous()
         return kernels.build_templates(dft, self.new_box, rts, mask, fsc)
 
-    def build_templates(self, mrc_volume, angles: ModelAngles) -> torch.Tensor:
+    def build_templates(self, mrc_volume, angles: ModelAngles, orientation_indices=None) -> torch.Tensor:
         """Pre-shifted bank [M, n, n/2+1] complex64, the v606 contract."""
-        return self.layout.pre_shift_for_correlation(self.build_bank(mrc_volume, angles))
+        return self.layout.pre_shift_for_correlation(self.build_bank(mrc_volume, angles, orientation_indices))
 
 
 class PsiGridBank:

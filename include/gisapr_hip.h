@@ -150,6 +150,25 @@ size_t gis_extract_slices_workspace(int L, int M);
 int gis_extract_slices(const void* dft, int L, const float* rts, int M, void* slices, void* ws, size_t ws_bytes,
                        void* stream);
 
+/*
+ * apply_sinc2_filter_inplace_3d (v606:837-853;
+ * project3d_and_whiten_cuda_nowrite.py:42-81): vol[z][y][x] *= sinc^2(|f|)
+ * in place, f = fftshift(fftfreq(n)) per axis, vol [D][H][W] float32.
+ */
+int gis_sinc2_filter(float* vol, int D, int H, int W, void* stream);
+
+/*
+ * rotated_grid_to_grid_sample (v606:866-897): for M rotations R [M][3][3]
+ * (acting on xyz vectors) of P base frequency points base_zyx [P][3] (z, y, x),
+ * the align_corners=True grid_sample coordinates grid_xyz [M][P][3] (x, y, z
+ * order) of an rfft volume of shape (dft_d, dft_h, dft_w) sampled as a real
+ * image of shape (img_d, img_h, img_w), and conj [M][P] (1 where the rotated
+ * x frequency is negative: the point was mirrored and its sample must be
+ * conjugated).
+ */
+int gis_slice_sample_grid(const float* base_zyx, int P, const float* R, int M, int img_d, int img_h, int img_w,
+                          int dft_d, int dft_h, int dft_w, float* grid_xyz, unsigned char* conj, void* stream);
+
 /* Workspace bytes gis_prepare_particles needs for `particles` (= B) particles. */
 size_t gis_prepare_particles_workspace(int N, int particles);
 

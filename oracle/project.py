@@ -65,6 +65,40 @@ def sinc2_weights(n2: int, dtype=np.float32) -> np.ndarray:
     return (np.sinc(r) ** 2).astype(dtype)
 
 
+def sinc2_filter(volume, dtype=np.float64) -> np.ndarray:
+    """apply_sinc2_filter_inplace_3d (v606:837-853;
+    project3d_and_whiten_cuda_nowrite.py:42-81) on any [D, H, W] shape:
+    volume * sinc(|f|)^2 with f = fftshift(fftfreq(n)) per axis (out of place)."""
+    D, H, W = volume.shape
+    fz, fy, fx = (np.fft.fftshift(np.fft.fftfreq(n)) for n in (D, H, W))
+    r = np.sqrt(fz[:, None, None] ** 2 + fy[None, :, None] ** 2 + fx[None, None, :] ** 2)
+    return (np.asarray(volume, dtype) * np.sinc(r) ** 2).astype(dtype)
+
+
+def projection_base_grid(image_shape) -> np.ndarray:
+    """build_projection_base_grid (v606:856-863): [H, W//2+1, 3] (z, y, x)
+    frequencies, z = 0, y = fftshift(fftfreq(H)), x = rfftfreq(W)."""
+    _, H, W = image_shape
+    yy, xx = np.meshgrid(np.fft.fftshift(np.fft.fftfreq(H)), np.fft.rfftfreq(W), indexing="ij")
+    return np.stack((np.zeros_like(yy), yy, xx), -1)
+
+
+def grid_sample_coordinates(base_zyx: np.ndarray, rts: np.ndarray, image_shape, dft_shape):
+    """rotated_grid_to_grid_sample (v606:866-897) in float64: rotate the
+    (x, y, z) base vectors by each R, mirror where the rotated x < 0, array
+    coordinates (z, y centred at size//2, x from 0), align_corners=True
+    normalisation by (dft_shape/2 - 1/2), returned in (x, y, z) order with the
+    mirror mask."""
+    base_xyz = np.asarray(base_zyx, np.float64)[..., ::-1]
+    rot = np.einsum("mij,...j->m...i", np.asarray(rts, np.float64), base_xyz)
+    conj = rot[..., 0] < 0
+    rot = np.where(conj[..., None], -rot, rot)
+    D, H, W = image_shape
+    coords = np.stack((rot[..., 2] * D + D // 2, rot[..., 1] * H + H // 2, rot[..., 0] * W), -1)
+    norm = coords / (0.5 * np.asarray(dft_shape, np.float64) - 0.5) - 1.0
+    return norm[..., ::-1].copy(), conj
+
+
 def volume_dft(volume, dtype=np.float32, pad=True) -> np.ndarray:
     """v606:1068-1075 / project3d_and_whiten_cuda.py:103-117: pad N/2 (unless
     --skip_padding), sinc^2 over the (padded) shape, fftshift, rfftn,

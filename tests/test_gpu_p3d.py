@@ -172,3 +172,36 @@ def test_project3d_cli_end_to_end(hip_device, tmp_path, monkeypatch, skip_paddin
     for i, (line, src) in enumerate(zip(body, rows)):
         tok, stok = line.rstrip("\n").split("\t"), src.split()
         assert tok[0] == f"{i + 1}@out.mrcs" and tok[1:] == stok[1:]
+
+
+@pytest.mark.parametrize("return_array", [True, False])
+def test_run_projection_library_form(hip_device, tmp_path, monkeypatch, return_array):
+    """project3d_and_whiten_cuda_nowrite.run_projection: the STAR file goes to
+    output_filename as given, the stack to output_filename + '.mrcs' unless
+    it is returned."""
+    from gisapr_amd import mrc, p3d
+    say("run_projection", "return_array", return_array)
+    N = 32
+    vol = blob_volume(N, 10)
+    mrc.write_mrc(tmp_path / "model.mrc", vol)
+    header = ["data_\n", "loop_\n", "_rlnImageName #1\n", "_rlnAngleRot #2\n", "_rlnAngleTilt #3\n",
+              "_rlnAnglePsi #4\n"]
+    rows = [f"{i + 1}@x.mrcs {r:.3f} {t:.3f} 10.0\n" for i, (r, t) in enumerate(zip(ROT, TILT))]
+    (tmp_path / "ang.star").write_text("".join(header + rows))
+    monkeypatch.chdir(tmp_path)
+    got = p3d.run_projection(particle_star="ang.star", mrc_volume="model.mrc", whitening=True,
+                             do_skip_padding=False, do_return_array=return_array, gpuid=0,
+                             output_filename="proj")
+    want = opj.project3d(vol.astype(np.float64), ROT, TILT, whitening=True, pad=True, dtype=np.float64)
+    if return_array:
+        assert not (tmp_path / "proj.mrcs").exists()
+        stack = got
+        assert stack.dtype == np.float32
+    else:
+        assert got is None
+        stack = mrc.read(tmp_path / "proj.mrcs")
+    for m in range(len(ROT)):
+        assert rel_l2(stack[m], want[m]) < 5e-5, m
+    out = (tmp_path / "proj").read_text().splitlines(keepends=True)
+    assert out[:len(header)] == header
+    assert [ln.split("\t")[0] for ln in out[len(header):]] == [f"{i + 1}@proj.mrcs" for i in range(len(ROT))]

@@ -208,3 +208,73 @@ def test_soft_mask_and_radial_expansion_closed_form():
     # truncated radius; shells beyond the table are 0
     assert e[4, 4] == 1.0 and e[4, 5] == 2.0 and e[5, 5] == 2.0 and e[4, 6] == 3.0 and e[4, 7] == 0.0
     assert e[6, 6] == 3.0 and e[0, 0] == 0.0  # radius 2.83 -> 2
+
+
+def test_sinc2_filter_closed_form_on_non_cubic_shapes():
+    """Window values at hand-computed frequencies (odd and even axes), and the
+    cubic window of the volume transform."""
+    import math
+    D, H, W = 5, 8, 7
+    got = opj.sinc2_filter(np.ones((D, H, W)))
+    assert got[2, 4, 3] == 1.0  # zero frequency at n // 2 on every axis
+
+    def s2(r):
+        return (math.sin(math.pi * r) / (math.pi * r)) ** 2
+
+    for k in (1, 3):
+        assert abs(got[2, 4, 3 + k] - s2(k / 7)) < 1e-15
+    r = math.sqrt((2 / 5) ** 2 + (4 / 8) ** 2 + (3 / 7) ** 2)  # corner: -2/5, -1/2, -3/7
+    assert abs(got[0, 0, 0] - s2(r)) < 1e-15
+    np.testing.assert_allclose(opj.sinc2_filter(np.ones((12, 12, 12))), opj.sinc2_weights(12, np.float64),
+                               rtol=0, atol=1e-15)
+
+
+def test_grid_sample_coordinates_identity_lands_on_the_central_plane():
+    """R = I: the align_corners grid indexes voxel (L/2, row j, column kx)
+    exactly; no point is mirrored."""
+    L = 16
+    img, dsh = (L, L, L), (L, L, L // 2 + 1)
+    base = opj.projection_base_grid(img)
+    assert base.shape == (L, L // 2 + 1, 3) and not base[..., 0].any()
+    g, conj = opj.grid_sample_coordinates(base, np.eye(3)[None], img, dsh)
+    idx = (g[0] + 1.0) / 2.0 * (np.array(dsh[::-1], np.float64) - 1.0)  # x, y, z voxel indices
+    jj, kk = np.meshgrid(np.arange(L), np.arange(L // 2 + 1), indexing="ij")
+    np.testing.assert_allclose(idx[..., 0], kk, atol=1e-12)
+    np.testing.assert_allclose(idx[..., 1], jj, atol=1e-12)
+    np.testing.assert_allclose(idx[..., 2], L // 2, atol=1e-12)
+    assert not conj.any()
+
+
+def test_grid_sample_coordinates_half_turn_mirrors_positive_x():
+    """R = diag(-1, -1, 1): every point with fx > 0 is mirrored back onto
+    itself; the fx = 0 column keeps -0 (not mirrored) and lands on row L - j."""
+    L = 16
+    img, dsh = (L, L, L), (L, L, L // 2 + 1)
+    base = opj.projection_base_grid(img)
+    g0, _ = opj.grid_sample_coordinates(base, np.eye(3)[None], img, dsh)
+    g, conj = opj.grid_sample_coordinates(base, np.diag([-1.0, -1.0, 1.0])[None], img, dsh)
+    assert conj[0, :, 1:].all() and not conj[0, :, 0].any()
+    np.testing.assert_allclose(g[0, :, 1:], g0[0, :, 1:], atol=1e-12)
+    yidx = (g[0, :, 0, 1] + 1.0) / 2.0 * (L - 1)
+    np.testing.assert_allclose(yidx, (L - np.arange(L)), atol=1e-12)
+
+
+def test_grid_sample_coordinates_drive_grid_sample_to_the_central_slice():
+    """torch's grid_sample (trilinear, border, align_corners) on the oracle
+    grid, conjugated by the mask, equals the brute-force-pinned trilinear
+    central slice for random rotations."""
+    import torch
+    import torch.nn.functional as F
+    L = 16
+    rng = np.random.default_rng(4)
+    dft = rng.standard_normal((L, L, L // 2 + 1)) + 1j * rng.standard_normal((L, L, L // 2 + 1))
+    R = np.stack([opj.euler_angles_to_matrix(*rng.uniform(-180, 180, 3)) for _ in range(3)]).astype(np.float64)
+    img, dsh = (L, L, L), dft.shape
+    g, conj = opj.grid_sample_coordinates(opj.projection_base_grid(img), R, img, dsh)
+    inp = torch.from_numpy(np.stack((dft.real, dft.imag))[None])
+    out = F.grid_sample(inp, torch.from_numpy(g)[None], mode="bilinear", padding_mode="border",
+                        align_corners=True)[0].numpy()
+    s = out[0] + 1j * out[1]
+    s = np.where(conj, np.conj(s), s)
+    for m in range(len(R)):
+        np.testing.assert_allclose(s[m], opj.central_slice(dft, R[m], np.float64), rtol=0, atol=1e-10)
