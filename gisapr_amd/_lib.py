@@ -27,6 +27,7 @@ SIGNATURES = {
     "gis_volume_dft_workspace_ex": (_sz, [_i, _i]),
     "gis_volume_dft_ex": (_i, [_vp, _i, _i, _vp, _vp, _sz, _vp]),
     "gis_build_templates_workspace_ex": (_sz, [_i, _i, _i]),
+    "gis_build_templates_resident": (_i, [_i, _i]),
     "gis_project_slices": (_i, [_vp, _i, _i, _vp, _i, _i, _i, _f, _vp, _vp, _sz, _vp]),
     "gis_whiten_images": (_i, [_vp, _i, _i, _i, _i, _f, _vp, _vp, _sz, _vp]),
     "gis_extract_slices_workspace": (_sz, [_i, _i]),
@@ -38,6 +39,9 @@ SIGNATURES = {
     "gis_build_templates": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _sz, _vp]),
     "gis_build_template_coefficients": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _sz,
                                              _vp]),
+    "gis_build_templates_batched": (_i, [_vp, _sz, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _sz, _vp]),
+    "gis_build_template_coefficients_batched": (_i, [_vp, _sz, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _vp, _vp,
+                                                     _vp, _sz, _vp]),
     "gis_prepare_particles_workspace": (_sz, [_i, _i]),
     "gis_prepare_particles": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gis_ctf_packed": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),

@@ -244,14 +244,27 @@ class CandidateBatch:
     ``evaluate(poses)`` runs one step eagerly; ``capture()`` records the
     device part as a HIP graph and later ``evaluate`` calls replay it.
     Returns the float64 score sum_b exp(rawCC_b / 1024) per candidate.
+
+    The banks of ``template_group`` candidates are built by one template
+    launch over their spectra (each spectrum stays resident until then: one
+    3-D half spectrum per grouped candidate).  The default groups just
+    enough candidates to give every resident template workgroup a direction
+    (one candidate when the direction table alone fills the device).
     """
 
     def __init__(self, engine: SearchEngine, ps: ParticleSet, body: RigidBody, res: float, slots: int, *,
-                 path: str = "mixed", subsets=None, autotuner: Optional[Autotuner] = None):
+                 path: str = "mixed", subsets=None, autotuner: Optional[Autotuner] = None,
+                 template_group: Optional[int] = None):
         if path not in ("mixed", "fp32"):
             raise ValueError("path must be 'mixed' or 'fp32'")
         self.engine, self.ps, self.body, self.path = engine, ps, body, path
         self.slots = int(slots)
+        if template_group is None:
+            resident = int(kernels.lib().gis_build_templates_resident(engine.N, 1))
+            template_group = min(8, max(1, -(-resident // max(1, len(engine.dir_rot)))))
+        if int(template_group) < 1:
+            raise ValueError("template_group must be >= 1")
+        self.group = min(int(template_group), self.slots)
         dev = engine.device
         self.device = dev
         N, n = engine.N, engine.n
@@ -274,7 +287,8 @@ class CandidateBatch:
         self.coords_host = torch.empty((self.slots, A, 3), dtype=torch.float32).pin_memory()
         self.coords_dev = torch.zeros((self.slots, A, 3), dtype=torch.float32, device=dev)
         self.volume = torch.empty((N, N, N), dtype=torch.float32, device=dev)
-        self.dft = torch.empty((int(kernels.lib().gis_volume_dft_bytes(N)) // 8,), dtype=torch.complex64, device=dev)
+        self.dft = torch.empty((self.group, int(kernels.lib().gis_volume_dft_bytes(N)) // 8), dtype=torch.complex64,
+                               device=dev)
         self.dft_ws = torch.empty((int(kernels.lib().gis_volume_dft_workspace(N)),), dtype=torch.uint8, device=dev)
         if path == "mixed":
             # template coefficients written straight from the template kernel (no bank)
@@ -284,7 +298,7 @@ class CandidateBatch:
         else:
             self.bank = torch.empty((self.slots * M, n, n // 2 + 1), dtype=torch.complex64, device=dev)
         self.tinv = torch.empty((self.slots * M,), dtype=torch.float32, device=dev)  # 1/norm per bank row
-        self.tpl_ws = torch.empty((max(int(kernels.lib().gis_build_templates_workspace(N, M)), 1),),
+        self.tpl_ws = torch.empty((max(int(kernels.lib().gis_build_templates_workspace(N, self.group * M)), 1),),
                                   dtype=torch.uint8, device=dev)
         self._copied = torch.cuda.Event()
         self.autotuner = autotuner
@@ -293,21 +307,34 @@ class CandidateBatch:
         self.static_out = None
 
     # -- device part ---------------------------------------------------------
-    def _templates(self, c: int):
+    def _spectrum(self, c: int, g: int):
+        """Slot c's density (resident fixed body + splatted moving chain) and
+        its 3-D half spectrum into group position g."""
         eng = self.engine
         self.volume.copy_(self.fixed)
         kernels.splat(self.coords_dev[c], self.moving_numbers, eng.N, self.width, eng.apix, self.rp, self.kn,
                       self.volume)
-        kernels.volume_dft(self.volume, out=self.dft, workspace=self.dft_ws)
-        rows = slice(c * self.M, (c + 1) * self.M)
+        kernels.volume_dft(self.volume, out=self.dft[g], workspace=self.dft_ws)
+
+    def _templates(self, c0: int, c1: int):
+        """Banks (or in-disk coefficients) of slots [c0, c1), one template launch."""
+        eng = self.engine
+        for c in range(c0, c1):
+            self._spectrum(c, c - c0)
+        dft = self.dft[:c1 - c0]
+        rows = slice(c0 * self.M, c1 * self.M)
         if self.path == "mixed":
-            kernels.build_template_coefficients(self.dft, eng.n, eng.rts, eng.mask, eng.fsc_packed, eng.small_kslot,
+            kernels.build_template_coefficients(dft, eng.n, eng.rts, eng.mask, eng.fsc_packed, eng.small_kslot,
                                                 eng.small.K, eng.small.Kpad, out=self.tk[rows],
                                                 norms=self.norms[rows], workspace=self.tpl_ws)
             return
-        _, norms = kernels.build_templates(self.dft, eng.n, eng.rts, eng.mask, eng.fsc_packed, out=self.bank[rows],
-                                           normalize=False)
+        _, norms = kernels.build_templates(dft, eng.n, eng.rts, eng.mask, eng.fsc_packed, out=self.bank[rows],
+                                           normalize=False, workspace=self.tpl_ws)
         torch.reciprocal(norms, out=self.tinv[rows])
+
+    def _all_templates(self, k: int):
+        for c0 in range(0, k, self.group):
+            self._templates(c0, min(c0 + self.group, k))
 
     def _search(self, tile: dict, k: Optional[int] = None):
         eng, ps = self.engine, self.ps
@@ -322,8 +349,7 @@ class CandidateBatch:
 
     def _device_step(self, k: Optional[int] = None):
         k = self.slots if k is None else k
-        for c in range(k):
-            self._templates(c)
+        self._all_templates(k)
         corr = self._search(self.tile, k)
         scores = kernels.score_sum_segments(corr.peak, corr.best, self.plan.prefix(k)[3])
         return scores, corr
@@ -347,8 +373,7 @@ class CandidateBatch:
         if self.autotuner is None:
             return self.tile
         self._stage(poses)
-        for c in range(self.slots):
-            self._templates(c)
+        self._all_templates(self.slots)
         ps, pl = self.ps, self.plan
         if self.path == "mixed":
             key = Autotuner.key("mixed", self.engine.n, ps.P, self.engine.small.Kpad, _bucket(int(pl.dir_idx.numel())))

@@ -140,9 +140,16 @@ struct TplLayout {
   static_assert(SLOT < (1u << 31), "32-bit scratch offsets");
 };
 
+// C spectra at D + c * stride (c < C, stride in complex64 values): work item
+// w = c * M + d is direction d of spectrum c, and writes output row w.
+struct VolSet {
+  int C;
+  size_t stride;
+};
+
 template <int N, int L>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
-build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ rts, int M, int n,
+build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __restrict__ rts, int M, int n,
                        const float* __restrict__ mask, const float* __restrict__ fsc,
                        float2* __restrict__ bank, float* __restrict__ norms, float2* __restrict__ scratch,
                        int normalize, CoefOut co, TplIO io) {
@@ -179,13 +186,16 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
 #define GIS_TPL_XCD_ORDER 1
 #endif
   // Workgroups are dispatched to the 8 XCDs round robin; with the XCD order
-  // the first round gives each XCD a contiguous range of directions.
-  int d_first = blockIdx.x;
+  // each round gives each XCD a contiguous range of work items.
+  int w_first = blockIdx.x;
   if (GIS_TPL_XCD_ORDER) {
     const int g8 = gridDim.x / 8;
-    if ((int)blockIdx.x < 8 * g8) d_first = (blockIdx.x % 8) * g8 + blockIdx.x / 8;
+    if ((int)blockIdx.x < 8 * g8) w_first = (blockIdx.x % 8) * g8 + blockIdx.x / 8;
   }
-  for (int d = d_first; d < M; d += gridDim.x) {
+  const int W_items = vs.C * M;
+  for (int w = w_first; w < W_items; w += gridDim.x) {
+    const int cand = w / M, d = w - cand * M;
+    const float2* __restrict__ D = D0 + (size_t)cand * vs.stride;
     float r00 = 0.f, r01 = 0.f, r10 = 0.f, r11 = 0.f, r20 = 0.f, r21 = 0.f;
     if (!io.img) {  // image mode has no rotations
       const float* R = rts + 9 * d;
@@ -699,7 +709,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
     }
 
     // ---- phase 9: forward column FFT (N), crop to n, pack, FSC -----------
-    float2* out = co.kslot ? nullptr : bank + (size_t)d * n * mh_n;
+    float2* out = co.kslot ? nullptr : bank + (size_t)w * n * mh_n;
     double nrm[1] = {0.0};
     const int kmax = n / 2;  // columns 0..n/2 are needed
     for (int k0 = 0; k0 <= kmax; k0 += T::TK3) {
@@ -747,7 +757,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
             v.y *= gj.x;
             if (co.kslot) {
               const int j = __float_as_int(gj.y);
-              if (j >= 0) co.coeffs[(size_t)d * co.Kpad + j] = v;
+              if (j >= 0) co.coeffs[(size_t)w * co.Kpad + j] = v;
             } else {
               out[(size_t)r * mh_n + c] = v;
             }
@@ -764,7 +774,7 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
       // the phase-9 stores of every thread are visible after block_sum_f64's
       // barriers; 1/norm rounded once, as the reciprocal a gather would apply
       const float sc = 1.0f / norm;
-      float2* cr = co.coeffs + (size_t)d * co.Kpad;
+      float2* cr = co.coeffs + (size_t)w * co.Kpad;
       for (int j0 = tid; j0 < co.Kpad; j0 += 4 * nthr) {  // loads in batches of 4, clamped into the row
         float2 v[4];
 #pragma unroll
@@ -782,20 +792,44 @@ build_templates_kernel(const float2* __restrict__ D, const float* __restrict__ r
         out[t] = make_float2(v.x / norm, v.y / norm);
       }
     }
-    if (norms && tid == 0) norms[d] = norm;
+    if (norms && tid == 0) norms[w] = norm;
   }
 }
 
+// Workgroups of build_templates_kernel<N, L> resident on the current device
+// at once (occupancy per CU x CUs; cached per device).
 template <int N, int L>
-int launch_build_templates(const float2* D, const float* rts, int M, int n, const float* mask,
+int resident_workgroups() {
+  static int per_cu[64] = {0};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  int* pc = (dev >= 0 && dev < 64) ? &per_cu[dev] : nullptr;
+  if (pc && *pc == 0) {
+    int nb = 0;
+    const size_t lds = (size_t)kLdsFloat2 * sizeof(float2);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, build_templates_kernel<N, L>, kThreads, lds) != hipSuccess ||
+        nb <= 0) {
+      (void)hipGetLastError();
+      nb = 4;
+    }
+    *pc = nb;
+  }
+  return (pc ? *pc : 4) * gis_cu_count();
+}
+
+template <int N, int L>
+int launch_build_templates(const float2* D, VolSet vs, const float* rts, int M, int n, const float* mask,
                            const float* fsc, float2* bank, float* norms, int normalize, void* ws, size_t ws_bytes,
                            hipStream_t st, CoefOut co, TplIO io) {
   using T = TplLayout<N, L>;
   const size_t slot = T::SLOT * sizeof(float2);
-  int grid = (int)std::min<size_t>((size_t)M, ws_bytes / slot);
-  grid = std::min(grid, 4 * gis_cu_count());
-  GIS_REQUIRE(grid >= 1, "gis_build_templates: workspace of %zu bytes is below one slot (%zu)", ws_bytes, slot);
   const size_t lds = (size_t)kLdsFloat2 * sizeof(float2);
+  // a persistent grid of exactly the resident workgroups: a workgroup beyond
+  // them would start a full direction late and finish a round after the rest
+  const int resident = resident_workgroups<N, L>();
+  int grid = (int)std::min<size_t>((size_t)vs.C * (size_t)M, ws_bytes / slot);
+  grid = std::min(grid, resident);
+  GIS_REQUIRE(grid >= 1, "gis_build_templates: workspace of %zu bytes is below one slot (%zu)", ws_bytes, slot);
   static bool attr_set = false;
   if (lds > 65536 && !attr_set) {
     hipError_t e = hipFuncSetAttribute((const void*)build_templates_kernel<N, L>,
@@ -807,8 +841,8 @@ int launch_build_templates(const float2* D, const float* rts, int M, int n, cons
     }
     attr_set = true;
   }
-  hipLaunchKernelGGL((build_templates_kernel<N, L>), dim3(grid), dim3(kThreads), lds, st, D, rts, M, n, mask, fsc,
-                     bank, norms, (float2*)ws, normalize, co, io);
+  hipLaunchKernelGGL((build_templates_kernel<N, L>), dim3(grid), dim3(kThreads), lds, st, D, vs, rts, M, n, mask,
+                     fsc, bank, norms, (float2*)ws, normalize, co, io);
   GIS_LAUNCH_CHECK("gis_build_templates");
   return GIS_OK;
 }
@@ -844,6 +878,25 @@ GIS_API size_t gis_build_templates_workspace_ex(int N, int M, int pad) {
 }
 GIS_API size_t gis_build_templates_workspace(int N, int M) { return gis_build_templates_workspace_ex(N, M, 1); }
 
+// Template workgroups the current device runs at once for box N (pad 1: the
+// padded transform): a launch with fewer directions (C * M) leaves CUs idle.
+GIS_API int gis_build_templates_resident(int N, int pad) {
+#define GIS_TPL_RES(NV) \
+  case NV: return pad ? resident_workgroups<NV, 2 * NV>() : resident_workgroups<NV, NV>();
+  switch (N) {
+    GIS_TPL_RES(32)
+    GIS_TPL_RES(64)
+    GIS_TPL_RES(96)
+    GIS_TPL_RES(128)
+    GIS_TPL_RES(192)
+    GIS_TPL_RES(256)
+    default:
+      gis_set_error("gis_build_templates_resident: box %d not in {32, 64, 96, 128, 192, 256}", N);
+      return GIS_EUNSUPPORTED;
+  }
+#undef GIS_TPL_RES
+}
+
 // Replaces the per-direction loop of IntegratedProjector.build_templates
 // (v606:1077-1147).
 //   dft   [2N][2N][N+1] complex64 from gis_volume_dft
@@ -852,18 +905,18 @@ GIS_API size_t gis_build_templates_workspace(int N, int M) { return gis_build_te
 //   fsc   [n][n/2+1] float32 packed FSC weights, or NULL (all ones)
 //   bank  [M][n][n/2+1] complex64 output (unit L2 norm, UNshifted rfft2 layout)
 //   norms [M] float32 output of the pre-normalisation norms, or NULL
-static int build_templates_dispatch(const void* dft, int N, int pad, int n, const float* rts, int M,
+static int build_templates_dispatch(const void* dft, VolSet vs, int N, int pad, int n, const float* rts, int M,
                                     const float* mask, const float* fsc, void* bank, float* norms, int normalize,
                                     void* ws, size_t ws_bytes, void* stream, CoefOut co, TplIO io) {
   hipStream_t st = (hipStream_t)stream;
   const float2* D = (const float2*)dft;
   float2* B = (float2*)bank;
-#define GIS_TPL_CASE(NV)                                                                                        \
-  case NV:                                                                                                      \
-    return pad ? launch_build_templates<NV, 2 * NV>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, \
-                                                    st, co, io)                                                 \
-               : launch_build_templates<NV, NV>(D, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, st, \
-                                                co, io);
+#define GIS_TPL_CASE(NV)                                                                                         \
+  case NV:                                                                                                       \
+    return pad ? launch_build_templates<NV, 2 * NV>(D, vs, rts, M, n, mask, fsc, B, norms, normalize, ws,        \
+                                                    ws_bytes, st, co, io)                                        \
+               : launch_build_templates<NV, NV>(D, vs, rts, M, n, mask, fsc, B, norms, normalize, ws, ws_bytes, \
+                                                st, co, io);
   switch (N) {
     GIS_TPL_CASE(32)
     GIS_TPL_CASE(64)
@@ -879,6 +932,7 @@ static int build_templates_dispatch(const void* dft, int N, int pad, int n, cons
 }
 
 static const TplIO kBankIO = {nullptr, nullptr, 1, 1, 1e-12f, nullptr};
+static const VolSet kOneVolume = {1, 0};
 
 GIS_API int gis_build_templates(const void* dft, int N, int n, const float* rts, int M, const float* mask,
                                 const float* fsc, void* bank, float* norms, int normalize, void* ws,
@@ -888,7 +942,7 @@ GIS_API int gis_build_templates(const void* dft, int N, int n, const float* rts,
   GIS_REQUIRE(M >= 0, "gis_build_templates: M < 0");
   GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0, "gis_build_templates: newbox %d must be even and <= box %d", n, N);
   if (M == 0) return GIS_OK;
-  return build_templates_dispatch(dft, N, 1, n, rts, M, mask, fsc, bank, norms, normalize, ws, ws_bytes, stream,
+  return build_templates_dispatch(dft, kOneVolume, N, 1, n, rts, M, mask, fsc, bank, norms, normalize, ws, ws_bytes, stream,
                                   CoefOut{nullptr, 0, 0, nullptr}, kBankIO);
 }
 
@@ -907,8 +961,45 @@ GIS_API int gis_build_template_coefficients(const void* dft, int N, int n, const
   GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0, "gis_build_template_coefficients: newbox %d must be even and <= box %d",
               n, N);
   if (M == 0) return GIS_OK;
-  return build_templates_dispatch(dft, N, 1, n, rts, M, mask, fsc, nullptr, norms, 0, ws, ws_bytes, stream,
+  return build_templates_dispatch(dft, kOneVolume, N, 1, n, rts, M, mask, fsc, nullptr, norms, 0, ws, ws_bytes, stream,
                                   CoefOut{kslot, K, Kpad, (float2*)coeffs}, kBankIO);
+}
+
+// Several candidates' banks in one launch: C spectra at dft + c * dft_stride
+// (complex64 values, each as from gis_volume_dft), the same M rotations, mask
+// and FSC; bank row c * M + d (and norms[c * M + d]) for direction d of
+// spectrum c.  One launch keeps every workgroup busy across the candidates
+// (no per-launch tail) and lets one workgroup's slice gather overlap another's
+// transforms.  ws: gis_build_templates_workspace(N, C * M).
+GIS_API int gis_build_templates_batched(const void* dft, size_t dft_stride, int C, int N, int n, const float* rts,
+                                        int M, const float* mask, const float* fsc, void* bank, float* norms,
+                                        int normalize, void* ws, size_t ws_bytes, void* stream) {
+  GIS_REQUIRE(dft && rts && bank && ws, "gis_build_templates_batched: null pointer");
+  GIS_REQUIRE(normalize || norms, "gis_build_templates_batched: an unnormalised bank needs the norms output");
+  GIS_REQUIRE(M >= 0 && C >= 0, "gis_build_templates_batched: M = %d, C = %d", M, C);
+  GIS_REQUIRE((size_t)C * (size_t)M < (1u << 31), "gis_build_templates_batched: C * M too large");
+  GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0, "gis_build_templates_batched: newbox %d must be even and <= box %d", n,
+              N);
+  if (M == 0 || C == 0) return GIS_OK;
+  return build_templates_dispatch(dft, VolSet{C, dft_stride}, N, 1, n, rts, M, mask, fsc, bank, norms, normalize, ws,
+                                  ws_bytes, stream, CoefOut{nullptr, 0, 0, nullptr}, kBankIO);
+}
+
+// gis_build_template_coefficients over C spectra in one launch (layout as
+// gis_build_templates_batched): coeffs row c * M + d.
+GIS_API int gis_build_template_coefficients_batched(const void* dft, size_t dft_stride, int C, int N, int n,
+                                                    const float* rts, int M, const float* mask, const float* fsc,
+                                                    const int* kslot, int K, int Kpad, void* coeffs, float* norms,
+                                                    void* ws, size_t ws_bytes, void* stream) {
+  GIS_REQUIRE(dft && rts && kslot && coeffs && ws, "gis_build_template_coefficients_batched: null pointer");
+  GIS_REQUIRE(M >= 0 && C >= 0, "gis_build_template_coefficients_batched: M = %d, C = %d", M, C);
+  GIS_REQUIRE((size_t)C * (size_t)M < (1u << 31), "gis_build_template_coefficients_batched: C * M too large");
+  GIS_REQUIRE(K >= 0 && Kpad >= K, "gis_build_template_coefficients_batched: K=%d Kpad=%d", K, Kpad);
+  GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0,
+              "gis_build_template_coefficients_batched: newbox %d must be even and <= box %d", n, N);
+  if (M == 0 || C == 0) return GIS_OK;
+  return build_templates_dispatch(dft, VolSet{C, dft_stride}, N, 1, n, rts, M, mask, fsc, nullptr, norms, 0, ws,
+                                  ws_bytes, stream, CoefOut{kslot, K, Kpad, (float2*)coeffs}, kBankIO);
 }
 
 // Real-space central-slice projections of a volume (the standalone
@@ -924,7 +1015,7 @@ GIS_API int gis_project_slices(const void* dft, int N, int pad, const float* rts
   GIS_REQUIRE(M >= 0, "gis_project_slices: M < 0");
   GIS_REQUIRE(pad == 0 || pad == 1, "gis_project_slices: pad %d not 0 or 1", pad);
   if (M == 0) return GIS_OK;
-  return build_templates_dispatch(dft, N, pad, N, rts, M, nullptr, nullptr, nullptr, nullptr, 0, ws, ws_bytes,
+  return build_templates_dispatch(dft, kOneVolume, N, pad, N, rts, M, nullptr, nullptr, nullptr, nullptr, 0, ws, ws_bytes,
                                   stream, CoefOut{nullptr, 0, 0, nullptr}, TplIO{nullptr, proj, whiten, normz, eps, nullptr});
 }
 
@@ -938,7 +1029,7 @@ GIS_API int gis_whiten_images(const float* img, int N, int M, int whiten, int no
   GIS_REQUIRE(M >= 0, "gis_whiten_images: M < 0");
   GIS_REQUIRE(whiten || normz, "gis_whiten_images: nothing to do (whiten and normz both 0)");
   if (M == 0) return GIS_OK;
-  return build_templates_dispatch(nullptr, N, 1, N, nullptr, M, nullptr, nullptr, nullptr, nullptr, 0, ws, ws_bytes,
+  return build_templates_dispatch(nullptr, kOneVolume, N, 1, N, nullptr, M, nullptr, nullptr, nullptr, nullptr, 0, ws, ws_bytes,
                                   stream, CoefOut{nullptr, 0, 0, nullptr}, TplIO{img, out, whiten, normz, eps, nullptr});
 }
 
@@ -975,7 +1066,7 @@ GIS_API int gis_extract_slices(const void* dft, int L, const float* rts, int M, 
     return GIS_EUNSUPPORTED;
   }
   if (M == 0) return GIS_OK;
-  return build_templates_dispatch(dft, N, pad, N, rts, M, nullptr, nullptr, nullptr, nullptr, 0, ws, ws_bytes, stream,
+  return build_templates_dispatch(dft, kOneVolume, N, pad, N, rts, M, nullptr, nullptr, nullptr, nullptr, 0, ws, ws_bytes, stream,
                                   CoefOut{nullptr, 0, 0, nullptr},
                                   TplIO{nullptr, nullptr, 0, 0, 0.f, (float2*)slices});
 }

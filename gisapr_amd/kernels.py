@@ -208,29 +208,41 @@ def whiten_images(images: torch.Tensor, whiten: bool = True, normalize: bool = T
     return res[0] if squeeze else res
 
 
+def _spectra(dft: torch.Tensor):
+    """(box N, candidates C) of one bricked spectrum ([E]) or C of them ([C, E])."""
+    require_hip(dft, "dft")
+    if dft.dtype != torch.complex64 or dft.ndim not in (1, 2):
+        raise ValueError("dft must be a bricked complex64 spectrum [E] from volume_dft, or [C, E]")
+    E = int(dft.shape[-1])
+    N = next((b for b in SUPPORTED_BOXES if int(lib().gis_volume_dft_bytes(b)) == E * 8), 0)
+    if not N:
+        raise ValueError("dft must be a bricked spectrum from volume_dft")
+    return N, (int(dft.shape[0]) if dft.ndim == 2 else 1)
+
+
 def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Optional[torch.Tensor],
                     fsc_packed: Optional[torch.Tensor], with_norms: bool = False,
-                    out: Optional[torch.Tensor] = None, normalize: bool = True):
+                    out: Optional[torch.Tensor] = None, normalize: bool = True,
+                    workspace: Optional[torch.Tensor] = None):
     """Packed template bank [M, n, n/2+1] complex64 (unshifted), written into
     ``out`` when given.  Unit-norm rows (the reference's bank) unless
     ``normalize`` is False: then the rows keep their scale and the norms are
-    returned (callers fold 1/norm into the search)."""
+    returned (callers fold 1/norm into the search).  ``dft`` [C, E] (C
+    spectra) gives the C banks in one launch, rows c * M + d."""
     with_norms = with_norms or not normalize
-    require_hip(dft, "dft")
-    N = next((b for b in SUPPORTED_BOXES if int(lib().gis_volume_dft_bytes(b)) == dft.numel() * 8), 0)
-    if not N or dft.dtype != torch.complex64:
-        raise ValueError("dft must be a bricked spectrum from volume_dft")
+    N, C = _spectra(dft)
     M = int(rts.shape[0])
     require_hip(rts, "rts")
     n = int(new_box)
+    rows = C * M
     if out is None:
-        bank = torch.empty((M, n, n // 2 + 1), dtype=torch.complex64, device=dft.device)
+        bank = torch.empty((rows, n, n // 2 + 1), dtype=torch.complex64, device=dft.device)
     else:
         require_hip(out, "out")
-        if tuple(out.shape) != (M, n, n // 2 + 1) or out.dtype != torch.complex64:
-            raise ValueError("out must be complex64 [M, n, n/2+1]")
+        if tuple(out.shape) != (rows, n, n // 2 + 1) or out.dtype != torch.complex64:
+            raise ValueError("out must be complex64 [C * M, n, n/2+1]")
         bank = out
-    norms = torch.empty((M,), dtype=torch.float32, device=dft.device) if with_norms else None
+    norms = torch.empty((rows,), dtype=torch.float32, device=dft.device) if with_norms else None
     if mask is not None:
         require_hip(mask, "mask")
         if tuple(mask.shape) != (N, N):
@@ -239,12 +251,26 @@ def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Op
         require_hip(fsc_packed, "fsc_packed")
         if tuple(fsc_packed.shape) != (n, n // 2 + 1):
             raise ValueError("fsc_packed must be [n, n/2+1]")
-    wsb = int(lib().gis_build_templates_workspace(N, M))
-    ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dft.device)
-    check(lib().gis_build_templates(ptr(dft), N, n, ptr(rts), M, ptr(mask), ptr(fsc_packed), ptr(bank),
-                                    ptr(norms), int(bool(normalize)), ptr(ws), wsb, stream_of(dft.device)),
-          "gis_build_templates")
+    ws, wsb = _template_workspace(N, rows, dft.device, workspace)
+    if dft.ndim == 1:
+        check(lib().gis_build_templates(ptr(dft), N, n, ptr(rts), M, ptr(mask), ptr(fsc_packed), ptr(bank),
+                                        ptr(norms), int(bool(normalize)), ptr(ws), wsb, stream_of(dft.device)),
+              "gis_build_templates")
+    else:
+        check(lib().gis_build_templates_batched(ptr(dft), int(dft.shape[1]), C, N, n, ptr(rts), M, ptr(mask),
+                                                ptr(fsc_packed), ptr(bank), ptr(norms), int(bool(normalize)),
+                                                ptr(ws), wsb, stream_of(dft.device)), "gis_build_templates_batched")
     return (bank, norms) if with_norms else bank
+
+
+def _template_workspace(N: int, rows: int, device, workspace: Optional[torch.Tensor]):
+    wsb = int(lib().gis_build_templates_workspace(N, rows))
+    if workspace is None:
+        return torch.empty((max(wsb, 1),), dtype=torch.uint8, device=device), wsb
+    require_hip(workspace, "workspace")
+    if workspace.numel() * workspace.element_size() < wsb:
+        raise ValueError(f"workspace must hold {wsb} bytes")
+    return workspace, wsb
 
 
 def build_template_coefficients(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Optional[torch.Tensor],
@@ -253,12 +279,11 @@ def build_template_coefficients(dft: torch.Tensor, new_box: int, rts: torch.Tens
                                 workspace: Optional[torch.Tensor] = None):
     """[M, kpad] complex64: the unit-norm bank's coefficients listed by ``kslot``
     ([n, n/2+1] int32, slot or -1), zero padded, without storing the bank;
-    returns (coefficients, norms)."""
-    require_hip(dft, "dft")
-    N = next((b for b in SUPPORTED_BOXES if int(lib().gis_volume_dft_bytes(b)) == dft.numel() * 8), 0)
-    if not N or dft.dtype != torch.complex64:
-        raise ValueError("dft must be a bricked spectrum from volume_dft")
+    returns (coefficients, norms).  ``dft`` [C, E] (C spectra) gives
+    [C * M, kpad] in one launch, rows c * M + d."""
+    N, C = _spectra(dft)
     M = int(rts.shape[0])
+    rows = C * M
     require_hip(rts, "rts")
     require_hip(kslot, "kslot")
     n = int(new_box)
@@ -267,15 +292,15 @@ def build_template_coefficients(dft: torch.Tensor, new_box: int, rts: torch.Tens
     if not 0 <= int(K) <= int(kpad):
         raise ValueError("need 0 <= K <= kpad")
     if out is None:
-        out = torch.empty((M, int(kpad)), dtype=torch.complex64, device=dft.device)
+        out = torch.empty((rows, int(kpad)), dtype=torch.complex64, device=dft.device)
     else:
         require_hip(out, "out")
-        if tuple(out.shape) != (M, int(kpad)) or out.dtype != torch.complex64 or not out.is_contiguous():
-            raise ValueError("out must be contiguous complex64 [M, kpad]")
+        if tuple(out.shape) != (rows, int(kpad)) or out.dtype != torch.complex64 or not out.is_contiguous():
+            raise ValueError("out must be contiguous complex64 [C * M, kpad]")
     if norms is None:
-        norms = torch.empty((M,), dtype=torch.float32, device=dft.device)
-    elif norms.numel() != M or norms.dtype != torch.float32:
-        raise ValueError("norms must be float32 [M]")
+        norms = torch.empty((rows,), dtype=torch.float32, device=dft.device)
+    elif norms.numel() != rows or norms.dtype != torch.float32 or not norms.is_contiguous():
+        raise ValueError("norms must be contiguous float32 [C * M]")
     if mask is not None:
         require_hip(mask, "mask")
         if tuple(mask.shape) != (N, N):
@@ -284,17 +309,17 @@ def build_template_coefficients(dft: torch.Tensor, new_box: int, rts: torch.Tens
         require_hip(fsc_packed, "fsc_packed")
         if tuple(fsc_packed.shape) != (n, n // 2 + 1):
             raise ValueError("fsc_packed must be [n, n/2+1]")
-    wsb = int(lib().gis_build_templates_workspace(N, M))
-    if workspace is not None:
-        require_hip(workspace, "workspace")
-        if workspace.numel() * workspace.element_size() < wsb:
-            raise ValueError(f"workspace must hold {wsb} bytes")
-        ws = workspace
+    ws, wsb = _template_workspace(N, rows, dft.device, workspace)
+    if dft.ndim == 1:
+        check(lib().gis_build_template_coefficients(ptr(dft), N, n, ptr(rts), M, ptr(mask), ptr(fsc_packed),
+                                                    ptr(kslot), int(K), int(kpad), ptr(out), ptr(norms), ptr(ws), wsb,
+                                                    stream_of(dft.device)), "gis_build_template_coefficients")
     else:
-        ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dft.device)
-    check(lib().gis_build_template_coefficients(ptr(dft), N, n, ptr(rts), M, ptr(mask), ptr(fsc_packed), ptr(kslot),
-                                                int(K), int(kpad), ptr(out), ptr(norms), ptr(ws), wsb,
-                                                stream_of(dft.device)), "gis_build_template_coefficients")
+        check(lib().gis_build_template_coefficients_batched(ptr(dft), int(dft.shape[1]), C, N, n, ptr(rts), M,
+                                                            ptr(mask), ptr(fsc_packed), ptr(kslot), int(K),
+                                                            int(kpad), ptr(out), ptr(norms), ptr(ws), wsb,
+                                                            stream_of(dft.device)),
+              "gis_build_template_coefficients_batched")
     return out, norms
 
 

@@ -77,6 +77,10 @@ int gis_volume_dft_ex(const float* volume, int N, int pad, void* dft, void* ws, 
 size_t gis_build_templates_workspace(int N, int M);
 /* The same for the padded (pad 1) or unpadded (pad 0) slice transform. */
 size_t gis_build_templates_workspace_ex(int N, int M, int pad);
+/* Template workgroups resident on the current device at once (box N, pad as
+ * above); launches of fewer directions than this leave compute units idle.
+ * No reference counterpart: sizes the batched builders' candidate groups. */
+int gis_build_templates_resident(int N, int pad);
 
 /*
  * (b) Template bank.  Replaces the per-direction loop of
@@ -113,6 +117,22 @@ int gis_build_templates(const void* dft, int N, int n, const float* rts, int M, 
 int gis_build_template_coefficients(const void* dft, int N, int n, const float* rts, int M, const float* mask,
                                     const float* fsc, const int* kslot, int K, int Kpad, void* coeffs, float* norms,
                                     void* ws, size_t ws_bytes, void* stream);
+
+/*
+ * Both builders over C candidates' spectra in one launch (the batched
+ * engine's template stage, ZL gisapr_batch_search.py:286-291 building one
+ * bank per candidate model): spectrum c at dft + c * dft_stride (complex64
+ * values), the same rotations, mask and FSC; output row c * M + d (bank,
+ * coeffs and norms) for direction d of candidate c.  Results equal C single
+ * calls bit for bit.  ws: gis_build_templates_workspace(N, C * M).
+ */
+int gis_build_templates_batched(const void* dft, size_t dft_stride, int C, int N, int n, const float* rts, int M,
+                                const float* mask, const float* fsc, void* bank, float* norms, int normalize, void* ws,
+                                size_t ws_bytes, void* stream);
+int gis_build_template_coefficients_batched(const void* dft, size_t dft_stride, int C, int N, int n, const float* rts,
+                                            int M, const float* mask, const float* fsc, const int* kslot, int K,
+                                            int Kpad, void* coeffs, float* norms, void* ws, size_t ws_bytes,
+                                            void* stream);
 
 /*
  * Standalone projector (project3d_and_whiten_cuda.py): real-space central

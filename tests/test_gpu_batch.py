@@ -46,14 +46,14 @@ def single(eng, ps, wl, data, pose, path):
     return res.corr.best.cpu().numpy(), res.corr.peak.cpu().numpy()
 
 
-@pytest.mark.parametrize("path", ["mixed", "fp32"])
-def test_batch_slots_match_single_candidates(setup, path):
+@pytest.mark.parametrize("path,group", [("mixed", 8), ("fp32", 8), ("mixed", 2), ("fp32", 2)])
+def test_batch_slots_match_single_candidates(setup, path, group):
     from gisapr_amd.batch import CandidateBatch, RigidBody
     wl, data, eng, ps, poses = setup
-    say("batch slots", path)
+    say("batch slots", path, "template group", group)
     body = RigidBody.from_arrays(data["coords"], data["numbers"], data["chains"], wl.moving_chain)
     subsets = [np.arange(ps.B), np.arange(0, ps.B, 2), np.arange(5, 17)]
-    batch = CandidateBatch(eng, ps, body, wl.res, slots=3, path=path, subsets=subsets)
+    batch = CandidateBatch(eng, ps, body, wl.res, slots=3, path=path, subsets=subsets, template_group=group)
     got = batch.evaluate(poses[:3])
     _, corr = batch.evaluate(poses[:3], sync=False)
     for c in range(3):

@@ -132,6 +132,38 @@ def test_templates_match_oracle(hip_device, N, n):
     np.testing.assert_allclose(raw / norms[:, None, None], bank, rtol=1e-5, atol=1e-7)
 
 
+@pytest.mark.parametrize("N,n,M", [(64, 48, 300), (128, 128, 37)])
+def test_batched_template_builds_equal_single_calls(hip_device, N, n, M):
+    """C spectra in one template launch give exactly the rows of C single
+    launches (bank and coefficient modes, norms included); 3 x 300 work
+    items exceed the resident workgroups, so workgroups loop over several."""
+    import torch
+    from gisapr_amd import host, kernels
+    say("batched template builds", N, n)
+    dev = hip_device
+    rng = np.random.default_rng(7 + N)
+    C = 3
+    rot, tilt = rng.uniform(-180, 180, M), rng.uniform(0, 180, M)
+    rts = torch.from_numpy(np.ascontiguousarray(host.rotation_matrices_t(rot, tilt))).to(dev)
+    mask = torch.from_numpy(host.generate_soft_mask((N, N), N // 2 - 6, 4)).to(dev)
+    fsc = torch.from_numpy(host.packed_fsc(np.linspace(1.0, 0.3, N).astype(np.float32), n)).to(dev)
+    dfts = torch.stack([kernels.volume_dft(torch.from_numpy(rng.standard_normal((N, N, N)).astype(np.float32))
+                                           .to(dev)) for _ in range(C)])
+    bank, norms = kernels.build_templates(dfts, n, rts, mask, fsc, normalize=False)
+    assert bank.shape == (C * M, n, n // 2 + 1)
+    kslot = torch.full((n, n // 2 + 1), -1, dtype=torch.int32)
+    keep = torch.from_numpy(rng.permutation(n * (n // 2 + 1))[:101])
+    kslot.view(-1)[keep] = torch.arange(101, dtype=torch.int32)
+    kslot = kslot.to(dev)
+    coef, cn = kernels.build_template_coefficients(dfts, n, rts, mask, fsc, kslot, 101, 112)
+    for c in range(C):
+        rows = slice(c * M, (c + 1) * M)
+        b1, n1 = kernels.build_templates(dfts[c], n, rts, mask, fsc, normalize=False)
+        assert torch.equal(bank[rows], b1) and torch.equal(norms[rows], n1), c
+        k1, kn1 = kernels.build_template_coefficients(dfts[c], n, rts, mask, fsc, kslot, 101, 112)
+        assert torch.equal(coef[rows], k1) and torch.equal(cn[rows], kn1), c
+
+
 @pytest.mark.parametrize("N,n,with_ctf", [(32, 32, True), (64, 64, True), (64, 48, True), (64, 64, False)])
 def test_particles_match_oracle(hip_device, N, n, with_ctf):
     import torch
