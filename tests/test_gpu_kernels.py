@@ -234,8 +234,11 @@ def test_correlate_matches_oracle(hip_device, n, C):
         np.testing.assert_allclose(float(c.std[b]), s.std(), rtol=1e-4, atol=1e-6)
 
 
-@pytest.mark.parametrize("name", ["box32", "box64"])
+@pytest.mark.parametrize("name", ["box32", "box64", "box96"])
 def test_end_to_end_matches_oracle(hip_device, name):
+    """Splat -> bank -> particle prep -> search -> records against the float64
+    oracle; box96 is the 200-particle, 192-direction CPU-runnable workload
+    (radix-3 transforms of length 96 / 192)."""
     import torch
     from gisapr_amd import synthetic
     from gisapr_amd.engine import SearchEngine

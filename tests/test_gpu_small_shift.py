@@ -207,7 +207,7 @@ def _oracle_inputs(data, wl, tab):
     return T, np.concatenate(rows), local, P
 
 
-@pytest.mark.parametrize("name,resolution", [("box32", 15.0), ("box64", 6.0)])
+@pytest.mark.parametrize("name,resolution", [("box32", 15.0), ("box64", 6.0), ("box96", 6.0)])
 def test_small_shift_end_to_end_matches_oracle(hip_device, name, resolution):
     from gisapr_amd import synthetic
     from gisapr_amd.engine import SearchEngine
