@@ -568,8 +568,11 @@ class BatchSearchEngine:
     ``search_text`` (the three v606 records per particle), ``metrics``.
     ``options``: ``batch_candidates`` (group size), ``path`` ("fp32", the
     reference's search, or "mixed"), ``fixed_density_cache_mb``,
-    ``batch_density_mode`` ("fixed" reuses the fixed body).  The fixed-bank
-    calibration mode (``orientation_rows`` / ``calibrate``) is not provided.
+    ``batch_density_mode`` ("fixed" reuses the fixed body).
+    ``orientation_rows`` (a calibration bank's rows) restricts every
+    particle to its bank directions; ``calibrate`` returns each model's
+    per-particle Top-K rows as ``orientation_rows`` (ZL gisapr_batch_search.py
+    :207-364 with orientation_bank_v1.py).
     """
 
     def __init__(self, search_globals=None, device=None, options=None):
@@ -616,14 +619,33 @@ class BatchSearchEngine:
         self.context = dict(key=key, args=args, engine=engine, ps=ps, records=records, angles=angles)
         return self.context
 
+    def _restricted(self, ctx, orientation_rows, keep: int) -> ParticleSet:
+        """The context's particle set with each particle's directions cut to
+        its bank row (spectra shared)."""
+        from dataclasses import replace
+        from . import orientation_bank
+        ps, angles = ctx["ps"], ctx["angles"]
+        local = [orientation_bank.restricted_indices(orientation_rows.get(str(rec.line_index)), rec, ps.local[b],
+                                                     angles, keep)
+                 for b, rec in enumerate(ctx["records"])]
+        off = np.zeros(ps.B + 1, np.int32)
+        off[1:] = np.cumsum([len(l) for l in local])
+        idx = np.asarray([k for l in local for k in l], np.int32)
+        return replace(ps, local=local, dir_off=torch.from_numpy(off).to(ps.dir_off.device),
+                       dir_idx=torch.from_numpy(idx).to(ps.dir_off.device), row_order=None)
+
     def run(self, args, models, orientation_rows=None, calibrate=False):
+        from . import orientation_bank
         from .search import _records
-        if orientation_rows is not None or calibrate:
-            raise NotImplementedError("the fixed-bank calibration mode is not provided by this engine")
+        if calibrate and orientation_rows is not None:
+            raise ValueError("Cannot calibrate from restricted directions")
+        keep = int(getattr(args, "keep_orientations", 3))
         t0 = time.perf_counter()
         ctx = self._context(args)
         setup_s = time.perf_counter() - t0
         engine, ps, records, angles = ctx["engine"], ctx["ps"], ctx["records"], ctx["angles"]
+        if orientation_rows is not None:
+            ps = self._restricted(ctx, orientation_rows, keep)
         path = self.options["path"]
         center = engine.crop // 2 if path == "fp32" else engine.small.half
         P, M = ps.P, len(angles.rot)
@@ -649,10 +671,15 @@ class BatchSearchEngine:
             px = corr.peak_x.cpu().numpy().reshape(-1)
             d_off = plan.dir_off.cpu().numpy()
             d_idx = plan.dir_idx.cpu().numpy()
+            all_scores = corr.scores.reshape(-1, P).cpu().numpy() if calibrate else None
             for c in range(len(group)):
                 lines = []
+                ranked = {} if calibrate else None
                 for j in range(int(plan.slot_off_host[c]), int(plan.slot_off_host[c + 1])):
                     b = j - int(plan.slot_off_host[c])
+                    if calibrate:
+                        ranked[str(records[b].line_index)] = orientation_bank.ranked_row(
+                            records[b], ps.local[b], angles, all_scores[d_off[j]:d_off[j + 1]], keep)
                     if best[j] < 0:
                         continue
                     li, pi = divmod(int(best[j]), P)
@@ -666,6 +693,6 @@ class BatchSearchEngine:
                 metrics = dict(density_s=(t2 - t1) / n_g, search_s=(t3 - t2) / n_g, setup_s=setup_s / len(models),
                                **self.density.last_metrics)
                 metrics["execution_s"] = metrics["density_s"] + metrics["search_s"] + metrics["setup_s"]
-                results.append(dict(score=float(sc[c]), orientation_rows=None, search_text="".join(lines),
+                results.append(dict(score=float(sc[c]), orientation_rows=ranked, search_text="".join(lines),
                                     metrics=metrics))
         return results

@@ -103,18 +103,30 @@ class SearchEngine:
         self.ctf_tab = torch.from_numpy(self.ctf.table).to(dev)
 
     # ---------------------------------------------------------------- particles
+    def local_directions(self, rot, tilt) -> List[List[int]]:
+        """Each particle's searched directions: the local map (v606:351-378)
+        with --doLocalSearch, else every direction."""
+        rot, tilt = np.asarray(rot, np.float64).reshape(-1), np.asarray(tilt, np.float64).reshape(-1)
+        if self.do_local_search:
+            return host.local_orientation_map(rot, tilt, self.dir_rot, self.dir_tilt, self.local_range)
+        return [list(range(len(self.dir_rot))) for _ in range(len(rot))]
+
     def add_particles(self, images, rot, tilt, psi, defocus=None, line_index=None,
-                      prepare=True) -> ParticleSet:
+                      prepare=True, local: Optional[Sequence[Sequence[int]]] = None) -> ParticleSet:
+        """``local`` overrides the searched directions per particle (ascending
+        global indices), e.g. a calibration bank's Top-K rows."""
         dev = self.device
         images = torch.as_tensor(images, dtype=torch.float32).to(dev).contiguous()
         B = int(images.shape[0])
         if tuple(images.shape[1:]) != (self.N, self.N):
             raise ValueError(f"particle images must be [B, {self.N}, {self.N}]")
         rot, tilt, psi = (np.asarray(v, np.float64).reshape(B) for v in (rot, tilt, psi))
-        if self.do_local_search:
-            local = host.local_orientation_map(rot, tilt, self.dir_rot, self.dir_tilt, self.local_range)
+        if local is None:
+            local = self.local_directions(rot, tilt)
         else:
-            local = [list(range(len(self.dir_rot))) for _ in range(B)]
+            local = [sorted(int(i) for i in l) for l in local]
+            if len(local) != B or any(i < 0 or i >= len(self.dir_rot) for l in local for i in l):
+                raise ValueError("local must list valid direction indices for every particle")
         fakes, effs = [], []
         for p in psi:
             f, e = host.psi_sampling_for_record(float(p), self.psi_step, self.do_local_search, self.local_range)

@@ -26,15 +26,17 @@ path -- on a host without a HIP device ``choose_device`` raises.
 from __future__ import annotations
 
 import argparse
+import json
 import os
 import sys
 import time
+from pathlib import Path
 from typing import Dict, Iterable, List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
 
-from . import host, kernels, mrc
+from . import host, kernels, mrc, orientation_bank
 from .engine import SearchEngine
 from .projection import (ProjectionWhitening, apply_sinc2_filter_inplace_3d,  # noqa: F401
                          build_projection_base_grid, rotated_grid_to_grid_sample)
@@ -402,6 +404,16 @@ def create_search_parser() -> argparse.ArgumentParser:
     p.add_argument("--outputBufferParticles", type=int, default=96)
     p.add_argument("--torchThreads", type=int, default=4)
     p.add_argument("--profileStages", action="store_true")
+    # zip release: projection of the searched directions only, Top-K calibration banks
+    p.add_argument("--projection_mode", choices=("needed", "full"), default="needed",
+                   help="needed: project only the union of the searched directions; full: every direction")
+    p.add_argument("--fixed_density_cache_mb", type=_nonnegative_mb, default=128.0)
+    p.add_argument("--keep-orientations", dest="keep_orientations", type=orientation_bank.positive_int, default=3)
+    g = p.add_mutually_exclusive_group()
+    g.add_argument("--orientations-in", dest="orientations_in", metavar="JSON",
+                   help="Score only the directions saved in this calibration bank.")
+    g.add_argument("--orientations-out", dest="orientations_out", metavar="JSON",
+                   help="Run calibration and export per-particle Top-K directions.")
     p.add_argument("--verbose", action="store_true")
     p.add_argument("--ctf_cache_mb", type=_nonnegative_mb, default=256.0)
     p.add_argument("--search_cache_mb", type=_nonnegative_mb, default=64.0)
@@ -427,9 +439,17 @@ def chunks(sequence, size) -> Iterable:
         yield sequence[start:start + size]
 
 
-def run_search(args: argparse.Namespace, *, model_volume=None, session_cache=None) -> Dict[str, float]:
+def run_search(args: argparse.Namespace, *, model_volume=None, session_cache=None, orientation_rows=None,
+               calibrate=False) -> Dict[str, object]:
     """v606 run_search: writes ``args.o`` (three records per particle) and returns
-    timing metrics (search_s, projection_s, setup_s, execution_s, ctf_hits)."""
+    timing metrics (search_s, projection_s, setup_s, execution_s, ctf_hits).
+
+    The zip release's additions (ZL v7:1483-1790): ``orientation_rows`` (a
+    calibration bank's rows, or --orientations-in) restricts each particle to
+    its bank directions; ``calibrate`` (or --orientations-out, which also
+    writes the bank) ranks each particle's directions and returns them as
+    ``orientation_rows``; with --projection_mode needed only the union of the
+    searched directions is projected (``template_directions``)."""
     total_start = time.time()
     verbose = bool(getattr(args, "verbose", False))
 
@@ -462,6 +482,19 @@ def run_search(args: argparse.Namespace, *, model_volume=None, session_cache=Non
     table = meta(("particles", fkey(args.p)), lambda: read_particle_table(args.p))
     selected = select_particle_records(table, args.start, args.end)
     log("Model orientation count:", len(angles.rot), "Selected particle count:", len(selected))
+    keep = int(getattr(args, "keep_orientations", 3))
+    bank_in, bank_out = getattr(args, "orientations_in", None), getattr(args, "orientations_out", None)
+    context = None
+    if bank_in or bank_out:
+        context = orientation_bank.settings(args.p, args.ang, keep, args.doLocalSearch, args.localRange,
+                                            args.psiStep)
+    if bank_in:
+        saved = json.loads(Path(bank_in).read_text(encoding="utf-8"))
+        orientation_bank.check_bank(saved, context)
+        orientation_rows = saved["rows"]
+    calibrate = bool(calibrate) or bool(bank_out)
+    if orientation_rows is not None and calibrate:
+        raise ValueError("Cannot calibrate from already restricted directions")
     fsc_values = host.read_fsc_values(args.FSC, N, bool(args.ignoreFSC))
     engine = SearchEngine(box=N, new_box=n, apix=float(args.apix), crop=int(args.ccgCropSize),
                           psi_step=float(args.psiStep), local_range=float(args.localRange),
@@ -470,27 +503,56 @@ def run_search(args: argparse.Namespace, *, model_volume=None, session_cache=Non
                           discard_mask=bool(args.discardMask), fsc_values=fsc_values, dir_rot=angles.rot,
                           dir_tilt=angles.tilt, device=device)
 
+    # searched directions per particle: the local map, restricted to a bank's rows when given
+    local_all = engine.local_directions([r.rot for r in selected], [r.tilt for r in selected])
+    if orientation_rows is not None:
+        local_all = [orientation_bank.restricted_indices(orientation_rows.get(str(rec.line_index)), rec, allowed,
+                                                         angles, keep)
+                     for rec, allowed in zip(selected, local_all)]
+    needed = None
+    if getattr(args, "projection_mode", "needed") == "needed":
+        needed = sorted({i for l in local_all for i in l})
+
     projection_start = time.time()
     projector = IntegratedProjector(device, N, n, layout, engine.mask, engine.fsc_packed)
-    bank = projector.build_bank(args.mrc if model_volume is None else model_volume, angles)
+    volume_src = args.mrc if model_volume is None else model_volume
+    if needed is None:
+        bank = projector.build_bank(volume_src, angles)
+    else:  # rows of unsearched directions stay unset: no particle reads them
+        bank = torch.empty((len(angles.rot), n, n // 2 + 1), dtype=torch.complex64, device=device)
+        if needed:
+            rows = projector.build_bank(volume_src, angles, needed)
+            bank.index_copy_(0, torch.as_tensor(needed, dtype=torch.long, device=device), rows)
+        else:
+            projector.last_orientation_count = projector.last_slice_count = 0
     sync_if_cuda(device)
     projection_seconds = time.time() - projection_start
+    ranked: Dict[str, dict] = {}
 
     search_start = time.time()
     reader = ParticleMRCReader(N, device)
     output_mode = "a" if args.doContinue else "w"
     try:
         with open(args.o, output_mode, encoding="utf-8") as out:
-            for group in chunks(selected, max(1, int(args.ctfBatchSize)) * 8):
+            group_size = max(1, int(args.ctfBatchSize)) * 8
+            for g0 in range(0, len(selected), group_size):
+                group = selected[g0:g0 + group_size]
+                local = local_all[g0:g0 + group_size]
                 images = reader.read_many([r.image_name for r in group])
                 dfo = None
                 if table.has_ctf:
                     dfo = np.asarray([[r.defocus_u, r.defocus_v, r.defocus_angle] for r in group], np.float32)
                 ps = engine.add_particles(images, [r.rot for r in group], [r.tilt for r in group],
                                           [r.psi for r in group], defocus=dfo,
-                                          line_index=[r.line_index for r in group])
+                                          line_index=[r.line_index for r in group], local=local)
                 res = engine.search(bank, ps)
                 recs = {r["line"]: r for r in engine.records(res, ps)}
+                if calibrate:
+                    scores = res.corr.scores.reshape(-1, ps.P).cpu().numpy()
+                    off = ps.dir_off.cpu().numpy()
+                    for b, rec in enumerate(group):
+                        ranked[str(rec.line_index)] = orientation_bank.ranked_row(
+                            rec, local[b], angles, scores[off[b]:off[b + 1]], keep)
                 lines: List[str] = []
                 for b, rec in enumerate(group):
                     r = recs.get(rec.line_index)
@@ -508,8 +570,15 @@ def run_search(args: argparse.Namespace, *, model_volume=None, session_cache=Non
     search_seconds = time.time() - search_start
     total = time.time() - total_start
     log("Total search time =", round(search_seconds, 4), "seconds")
-    return dict(search_s=search_seconds, projection_s=projection_seconds,
-                setup_s=max(0.0, total - search_seconds - projection_seconds), execution_s=total, ctf_hits=0)
+    if bank_out:
+        if any(row.get("invalid_rawcc") for row in ranked.values()):
+            raise ValueError("NaN rawCC in calibration: no usable orientation bank can be exported")
+        orientation_bank.atomic_json(bank_out, orientation_bank.pack_bank(context, ranked,
+                                                                          mrc=str(Path(args.mrc).resolve())))
+    return dict(orientation_rows=ranked if calibrate else None, search_s=search_seconds,
+                projection_s=projection_seconds, setup_s=max(0.0, total - search_seconds - projection_seconds),
+                execution_s=total, ctf_hits=0, template_directions=projector.last_orientation_count,
+                sampled_directions=projector.last_slice_count)
 
 
 def main() -> None:

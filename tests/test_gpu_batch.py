@@ -153,3 +153,53 @@ def test_batch_search_engine_matches_run_search(hip_device, tmp_path):
                 assert g[2] == pytest.approx(w[2], rel=1e-5)
         raw = [w[2] for w in want if w[0] == "rawCC: "]
         assert r["score"] == pytest.approx(float(np.sum(np.exp(np.asarray(raw) / 1024.0))), rel=1e-5)
+
+
+def test_batch_search_engine_calibration_rows(hip_device, tmp_path):
+    """BatchSearchEngine.run(calibrate=True) returns each model's Top-K rows;
+    running a model restricted to its own rows keeps its best direction, psi,
+    shift and rawCC; calibrating restricted directions is refused."""
+    import ast
+    from gisapr_amd import search, star, synthetic
+    from gisapr_amd.batch import BatchSearchEngine
+    say("BatchSearchEngine calibration")
+    wl = synthetic.WORKLOADS["box32"]
+    d = synthetic.make_workload("box32")
+    from gisapr_amd import mrc
+    mrc.write_mrc(tmp_path / "parts.mrcs", d["images"].numpy())
+    star.write_particle_star(tmp_path / "parts.star", str(tmp_path / "parts.mrcs"), d["rot"], d["tilt"], d["psi"],
+                             d["dfu"], d["dfv"], d["dfa"])
+    star.write_direction_star(tmp_path / "dirs.star", d["dir_rot"], d["dir_tilt"])
+    (tmp_path / "model.pdb").write_text(synthetic.pdb_text(d["coords"], d["elements"], d["chains"]))
+    argv = ["--i", "x", "--p", str(tmp_path / "parts.star"), "--o", str(tmp_path / "out.txt"),
+            "--ang", str(tmp_path / "dirs.star"), "--mrc", "unused.mrc",
+            "--oriboxsize", str(wl.box), "--newboxsize", str(wl.box),
+            "--apix", str(wl.apix), "--kk", str(wl.kk), "--psiStep", str(wl.psi_step), "--doLocalSearch",
+            "--localRange", str(wl.local_range), "--ccgCropSize", str(wl.crop),
+            "--maskRadius", str(wl.mask_radius), "--maskEdge", str(wl.mask_edge), "--keep-orientations", "2"]
+    args = search.create_search_parser().parse_args(argv)
+    models = [dict(pdb_path=str(tmp_path / "model.pdb"), source_pdb=str(tmp_path / "model.pdb"),
+                   chain_id=wl.moving_chain, pose=list(p), boxsize=wl.box, apix=wl.apix, res=wl.res)
+              for p in ([0, 0, 0, 0, 0, 0], [3.0, -2.0, 1.0, 0.5, 0.0, -0.5])]
+    eng = BatchSearchEngine(None, hip_device, dict(batch_candidates=2))
+    cal = eng.run(args, models, calibrate=True)
+    lines = {str(r.line_index) for r in star.read_particle_table(str(tmp_path / "parts.star")).records}
+    for r in cal:
+        assert set(r["orientation_rows"]) == lines
+        assert all(len(row["directions"]) <= 2 for row in r["orientation_rows"].values())
+    assert eng.run(args, models)[0]["orientation_rows"] is None
+    for i in range(2):
+        res = eng.run(args, [models[i]], orientation_rows=cal[i]["orientation_rows"])[0]
+        got = [ast.literal_eval(x) for x in res["search_text"].splitlines()]
+        want = [ast.literal_eval(x) for x in cal[i]["search_text"].splitlines()]
+        assert len(got) == len(want) > 0
+        for g, w in zip(got, want):
+            assert g[:2] == w[:2]
+            if g[0] == "cc: ":
+                assert g[5:7] == w[5:7] and g[8:10] == w[8:10]
+            elif g[0] == "psi: ":
+                assert g[2:] == w[2:]
+            else:  # the density's float atomics may add in another order between runs
+                assert g[2] == pytest.approx(w[2], rel=1e-5)
+    with pytest.raises(ValueError):
+        eng.run(args, models[:1], orientation_rows=cal[0]["orientation_rows"], calibrate=True)

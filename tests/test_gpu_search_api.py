@@ -112,3 +112,62 @@ def test_v606_namespace_mirrors(hip_device):
     recs3 = search.make_result_records(search.ParticleRecord(5, "", 10.0, 30.0, 3.0), [0.0, 7.5, 15.0],
                                        angles.rot, angles.tilt, [0, 1], sc, ys, xs, C, 1.0)
     assert [r[0] for r in recs3] == ["cc: ", "psi: ", "rawCC: "] and recs3[2][2] == pytest.approx(float(s2.max()), rel=1e-3)
+
+
+def test_calibration_bank_round_trip(hip_device, tmp_path):
+    """--orientations-out ranks each particle's directions (best over psi,
+    Top-K); --orientations-in then scores only those and finds the same best
+    direction, psi, shift and rawCC; both projection modes give the same
+    records; a restricted bank cannot be recalibrated."""
+    import json
+    import torch
+    from gisapr_amd import mrc, orientation_bank, search, star, synthetic
+    wl = synthetic.WORKLOADS["box32"]
+    d = synthetic.make_workload("box32")
+    print("[case] calibration bank", flush=True)
+    mrc.write_mrc(tmp_path / "parts.mrcs", d["images"].numpy())
+    star.write_particle_star(tmp_path / "parts.star", str(tmp_path / "parts.mrcs"), d["rot"], d["tilt"], d["psi"],
+                             d["dfu"], d["dfv"], d["dfa"])
+    star.write_direction_star(tmp_path / "dirs.star", d["dir_rot"], d["dir_tilt"])
+    vol = osp.density_from_coords(d["coords"], d["numbers"], wl.box, wl.apix, wl.res)
+    mrc.write_mrc(tmp_path / "model.mrc", vol)
+    base = ["--i", "x", "--p", str(tmp_path / "parts.star"), "--ang", str(tmp_path / "dirs.star"),
+            "--mrc", str(tmp_path / "model.mrc"), "--oriboxsize", str(wl.box), "--newboxsize", str(wl.box),
+            "--apix", str(wl.apix), "--kk", str(wl.kk), "--psiStep", str(wl.psi_step), "--doLocalSearch",
+            "--localRange", str(wl.local_range), "--ccgCropSize", str(wl.crop), "--maskRadius", str(wl.mask_radius),
+            "--maskEdge", str(wl.mask_edge), "--keep-orientations", "2"]
+    parse = search.create_search_parser().parse_args
+    full = search.run_search(parse(base + ["--o", str(tmp_path / "full.txt"), "--projection_mode", "full"]))
+    m = search.run_search(parse(base + ["--o", str(tmp_path / "cal.txt"), "--orientations-out",
+                                        str(tmp_path / "bank.json")]))
+    assert _parse(tmp_path / "cal.txt") == _parse(tmp_path / "full.txt")
+    assert full["template_directions"] == len(d["dir_rot"]) and m["template_directions"] <= len(d["dir_rot"])
+    bank = json.loads((tmp_path / "bank.json").read_text())
+    table = star.read_particle_table(str(tmp_path / "parts.star"))
+    ctx = orientation_bank.settings(tmp_path / "parts.star", tmp_path / "dirs.star", 2, True, wl.local_range,
+                                    wl.psi_step)
+    orientation_bank.check_bank(bank, ctx)
+    assert bank["rows"] == m["orientation_rows"] and set(bank["rows"]) == {str(r.line_index) for r in table.records}
+    cal = _parse(tmp_path / "cal.txt")
+    for line, row in bank["rows"].items():
+        dirs = row["directions"]
+        assert len(dirs) <= 2 and [x[3] for x in dirs] == sorted((x[3] for x in dirs), reverse=True)
+        if int(line) in cal:  # the top-ranked direction is the record's best direction, its score the rawCC
+            cc, raw = cal[int(line)]["cc: "], cal[int(line)]["rawCC: "]
+            assert (dirs[0][1], dirs[0][2]) == (cc[5], cc[6]) and dirs[0][3] == raw[2]
+    r = search.run_search(parse(base + ["--o", str(tmp_path / "res.txt"), "--orientations-in",
+                                        str(tmp_path / "bank.json")]))
+    assert r["orientation_rows"] is None
+    assert r["template_directions"] == len({x[0] for row in bank["rows"].values() for x in row["directions"]})
+    got = _parse(tmp_path / "res.txt")
+    assert sorted(got) == sorted(cal)
+    for line, g in got.items():
+        c = cal[line]
+        assert g["cc: "][5:7] == c["cc: "][5:7] and g["cc: "][8:10] == c["cc: "][8:10]
+        assert g["psi: "] == c["psi: "] and g["rawCC: "] == c["rawCC: "]
+    with pytest.raises(ValueError):
+        search.run_search(parse(base + ["--o", str(tmp_path / "bad.txt")]), orientation_rows=bank["rows"],
+                          calibrate=True)
+    with pytest.raises(ValueError):  # a bank for other settings is refused
+        search.run_search(parse(base[:-1] + ["3", "--o", str(tmp_path / "bad.txt"), "--orientations-in",
+                                             str(tmp_path / "bank.json")]))
