@@ -439,6 +439,20 @@ def chunks(sequence, size) -> Iterable:
         yield sequence[start:start + size]
 
 
+def _session_get(session_cache, key, factory):
+    """Parsed STAR tables and angle lists kept across requests: the worker's
+    SearchCache (its ``metadata.get_or_build``, Z3 gisapr_cache.py:110-117,
+    120-123), a plain dict, or nothing (always parse)."""
+    if session_cache is None:
+        return factory()
+    store = getattr(session_cache, "metadata", None)
+    if store is not None and hasattr(store, "get_or_build"):
+        return store.get_or_build(key, factory)
+    if key not in session_cache:
+        session_cache[key] = factory()
+    return session_cache[key]
+
+
 def run_search(args: argparse.Namespace, *, model_volume=None, session_cache=None, orientation_rows=None,
                calibrate=False) -> Dict[str, object]:
     """v606 run_search: writes ``args.o`` (three records per particle) and returns
@@ -458,11 +472,7 @@ def run_search(args: argparse.Namespace, *, model_volume=None, session_cache=Non
             print(*v, **kw)
 
     def meta(key, factory):
-        if session_cache is None:
-            return factory()
-        if key not in session_cache:
-            session_cache[key] = factory()
-        return session_cache[key]
+        return _session_get(session_cache, key, factory)
 
     device = choose_device(args.gpuid)
     N, n = int(args.oriboxsize), int(args.newboxsize)

@@ -109,3 +109,33 @@ def test_quantize_8_3_matches_text_round_trip():
     assert np.array_equal(np.signbit(got), np.signbit(want))
     with pytest.raises(ValueError):
         quantize_8_3([[10000.0, 0.0, 0.0]])
+
+
+def test_search_session_cache_forms():
+    """run_search's metadata cache accepts the worker's SearchCache shape
+    (``metadata.get_or_build``), a dict, or None."""
+    from gisapr_amd.search import _session_get
+    calls = []
+
+    def factory():
+        calls.append(1)
+        return len(calls)
+
+    class Store:
+        def __init__(self):
+            self.d = {}
+
+        def get_or_build(self, key, f):
+            if key not in self.d:
+                self.d[key] = f()
+            return self.d[key]
+
+    class SearchCacheLike:
+        def __init__(self):
+            self.metadata = Store()
+
+    sc = SearchCacheLike()
+    assert _session_get(sc, ("a", 1), factory) == 1 and _session_get(sc, ("a", 1), factory) == 1
+    d = {}
+    assert _session_get(d, "k", factory) == 2 and _session_get(d, "k", factory) == 2 and d == {"k": 2}
+    assert _session_get(None, "k", factory) == 3 and _session_get(None, "k", factory) == 4

@@ -35,3 +35,4 @@ def test_extract_filename_and_index_maps():
     np.testing.assert_array_equal(r, np.round(np.sqrt((y - 4) ** 2 + (x - 4) ** 2)).astype(np.int64))
     yy, xx = p3d.build_xy_grid_torch(4, 6, "cpu")
     assert torch.equal(yy, torch.arange(4.0)[:, None].expand(4, 6)) and torch.equal(xx, torch.arange(6.0).expand(4, 6))
+
