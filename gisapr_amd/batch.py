@@ -555,6 +555,17 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
     return scores, corr, plan
 
 
+class _ByteCount:
+    """Read-only ``bytes`` of what an engine keeps resident."""
+
+    def __init__(self, count: Callable[[], int]):
+        self._count = count
+
+    @property
+    def bytes(self) -> int:
+        return int(self._count())
+
+
 class BatchSearchEngine:
     """Group evaluation of several models against one particle selection
     (call surface of ZL:gisapr_batch_search.py:142-364).
@@ -577,11 +588,19 @@ class BatchSearchEngine:
 
     def __init__(self, search_globals=None, device=None, options=None):
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
-        self.options = dict(batch_candidates=8, path="fp32", fixed_density_cache_mb=512.0,
-                            batch_density_mode="fixed")
-        self.options.update(options or {})
+        # the caller's dict as given (the worker compares it to decide whether to
+        # keep this engine); defaults fill what it leaves out
+        self.options = dict(options or {})
+        self._defaults = dict(batch_candidates=8, path="fp32", fixed_density_cache_mb=512.0,
+                              batch_density_mode="fixed")
+        # resident bytes the worker's ping reports (ZL gisapr_worker_v2.py:181-183)
+        self.particles = _ByteCount(lambda: self.context["ps"].nbytes_resident() if self.context else 0)
+        self.volumes = _ByteCount(lambda: 0)  # model volumes are not retained between requests
         self.density = RigidDensityCache()
         self.context = None
+
+    def _opt(self, key):
+        return self.options.get(key, self._defaults[key])
 
     def _context(self, args):
         from . import host
@@ -613,7 +632,7 @@ class BatchSearchEngine:
             dfo = np.asarray([[r.defocus_u, r.defocus_v, r.defocus_angle] for r in records], np.float32)
         ps = engine.add_particles(images, [r.rot for r in records], [r.tilt for r in records],
                                   [r.psi for r in records], defocus=dfo, line_index=[r.line_index for r in records])
-        if self.options["path"] == "mixed":
+        if self._opt("path") == "mixed":
             engine.enable_small_shift()
             engine.prepare_small_shift(ps)
         self.context = dict(key=key, args=args, engine=engine, ps=ps, records=records, angles=angles)
@@ -646,17 +665,17 @@ class BatchSearchEngine:
         engine, ps, records, angles = ctx["engine"], ctx["ps"], ctx["records"], ctx["angles"]
         if orientation_rows is not None:
             ps = self._restricted(ctx, orientation_rows, keep)
-        path = self.options["path"]
+        path = self._opt("path")
         center = engine.crop // 2 if path == "fp32" else engine.small.half
         P, M = ps.P, len(angles.rot)
         results = []
-        cap = max(1, int(self.options["batch_candidates"]))
+        cap = max(1, int(self._opt("batch_candidates")))
         for g0 in range(0, len(models), cap):
             group = models[g0:g0 + cap]
             t1 = time.perf_counter()
             volumes = [self.density.generate(dict(m, device=str(self.device)),
-                                             self.options["fixed_density_cache_mb"],
-                                             self.options["batch_density_mode"]) for m in group]
+                                             self._opt("fixed_density_cache_mb"),
+                                             self._opt("batch_density_mode")) for m in group]
             torch.cuda.synchronize(self.device)
             t2 = time.perf_counter()
             scores, corr, plan = search_volumes(engine, ps, volumes, path)

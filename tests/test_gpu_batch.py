@@ -182,7 +182,9 @@ def test_batch_search_engine_calibration_rows(hip_device, tmp_path):
                    chain_id=wl.moving_chain, pose=list(p), boxsize=wl.box, apix=wl.apix, res=wl.res)
               for p in ([0, 0, 0, 0, 0, 0], [3.0, -2.0, 1.0, 0.5, 0.0, -0.5])]
     eng = BatchSearchEngine(None, hip_device, dict(batch_candidates=2))
+    assert eng.options == dict(batch_candidates=2)  # kept as given: the worker compares it
     cal = eng.run(args, models, calibrate=True)
+    assert eng.particles.bytes > 0 and eng.volumes.bytes == 0  # the worker's ping fields
     lines = {str(r.line_index) for r in star.read_particle_table(str(tmp_path / "parts.star")).records}
     for r in cal:
         assert set(r["orientation_rows"]) == lines
