@@ -24,6 +24,8 @@
 #include "common.h"
 #include "vdft_layout.h"
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace {
@@ -76,6 +78,36 @@ __device__ __forceinline__ void batched_tile_loop(LoadF load, StoreF store) {
 #pragma unroll
     for (int u = 0; u < UB; ++u) store(i0 + u, v[u]);
   }
+}
+
+// x / D for 0 <= x < XMAX as one 24-bit multiply and a shift: with
+// M = ceil(2^S / D) = (2^S + e) / D, floor(x M / 2^S) = floor(x / D) whenever
+// (XMAX - 1) e < 2^S (the added x e / (D 2^S) stays below 1/D).  The
+// compiler's 32-bit division by a constant costs a quarter-rate multiply-high,
+// and the tile loops divide every element index.
+template <unsigned D, unsigned XMAX>
+struct SmallDiv {
+  static constexpr unsigned long long m_of(unsigned s) { return ((1ull << s) + D - 1) / D; }
+  static constexpr bool ok(unsigned s) {
+    return m_of(s) < (1ull << 24) && (unsigned long long)(XMAX - 1) * m_of(s) < (1ull << 32) &&
+           (unsigned long long)(XMAX - 1) * (m_of(s) * D - (1ull << s)) < (1ull << s);
+  }
+  static constexpr unsigned pick() {
+    for (unsigned s = 1; s < 32; ++s)
+      if (ok(s)) return s;
+    return 0;
+  }
+  static constexpr unsigned S = pick();
+  static constexpr unsigned M = (unsigned)m_of(S);
+  static_assert(S != 0, "no 24-bit reciprocal is exact over the range");
+  static __device__ __forceinline__ unsigned div(unsigned x) { return __umul24(x, M) >> S; }
+};
+
+// (x / D, x % D) for 0 <= x < XMAX
+template <unsigned D, unsigned XMAX>
+__device__ __forceinline__ void divmod_small(int x, int& q, int& r) {
+  q = (int)SmallDiv<D, XMAX>::div((unsigned)x);
+  r = x - (int)__umul24((unsigned)q, D);
 }
 
 constexpr int largest_divisor_upto(int n, int cap) {
@@ -172,11 +204,29 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
   // 32-bit element offsets within the workgroup's scratch slot
   auto uidx = [](int row, int col) -> unsigned {
     const unsigned c = (unsigned)col;
-    return ((c / T::TK1) * N + (unsigned)row) * T::TK1 + c % T::TK1;
+    const unsigned ct = SmallDiv<T::TK1, (unsigned)(T::UT * T::TK1)>::div(c);
+    return (ct * N + (unsigned)row) * T::TK1 + (c - __umul24(ct, (unsigned)T::TK1));
   };
+  using ColDiv = SmallDiv<T::TK3, (unsigned)(T::MH + T::TK3)>;
   auto widx = [](int row, int col) -> unsigned {
-    const unsigned c = (unsigned)col;
-    return ((c / T::TK3) * N + (unsigned)row) * T::TK3 + c % T::TK3;
+    const unsigned c = (unsigned)col, ct = ColDiv::div(c);
+    return (ct * N + (unsigned)row) * T::TK3 + (c - __umul24(ct, (unsigned)T::TK3));
+  };
+  // A column tile of W (k0 a multiple of TK3) is the contiguous block
+  // W[k0 * N + t], t = Y * TK3 + q: tile loops index it linearly and split t
+  // into (row Y, column q) for the LDS side.
+  using TileDiv = SmallDiv<T::TK3, (unsigned)(((T::TK3 * N + kThreads - 1) / kThreads + 3) / 4 * 4 * kThreads)>;
+  auto tile_yq = [](int t, int& Y, int& q) {
+    Y = (int)TileDiv::div((unsigned)t);
+    q = t - (int)__umul24((unsigned)Y, (unsigned)T::TK3);
+  };
+  // Element i of a per-workgroup array addressed as a 32-bit byte offset from
+  // a uniform base (the slot and the tables are < 4 GiB), which the loads and
+  // stores take directly instead of a 64-bit address per element.
+  auto at = [](auto* base, unsigned i) -> decltype(*base)& {
+    using E = std::remove_reference_t<decltype(*base)>;
+    return *reinterpret_cast<E*>(reinterpret_cast<std::conditional_t<std::is_const_v<E>, const char*, char*>>(base) +
+                                 (size_t)(i * (unsigned)sizeof(E)));
   };
   float* Z = (float*)U;  // real image, aliases U once U is consumed
   const int mh_n = n / 2 + 1;
@@ -301,7 +351,8 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       if (io.slices) {
         // the sampled slice itself: row (ip + L/2) mod L (fftshifted order), column kx
         for (int t = tid; t < nk * L2; t += nthr) {
-          const int q = t / L2, ip = t - q * L2;
+          int q, ip;
+          divmod_small<L2, (unsigned)(T::TK1 * L2)>(t, q, ip);
           io.slices[((size_t)d * L2 + (ip + L2 / 2) % L2) * M2 + k0 + q] = buf[q * SP2 + pd(ip)];
         }
         __syncthreads();
@@ -310,10 +361,11 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       fft_lds_inplace<L2, true, kPad>(buf, nk, SP2, tw2, tid, nthr);
       // ifftshift + crop: projection row Y is spatial row (Y + (L-N)/2 + L/2) mod L
       for (int t = tid; t < T::TK1 * N; t += nthr) {
-        const int Y = t / T::TK1, q = t - Y * T::TK1;
+        int Y, q;
+        divmod_small<T::TK1, (unsigned)(T::TK1 * N)>(t, Y, q);
         if (q >= nk) continue;
         const int y = (Y + (L2 - N) / 2 + L2 / 2) % L2;
-        U[uidx(Y, k0 + q)] = buf[q * SP2 + pd(y)];
+        at(U, uidx(Y, k0 + q)) = buf[q * SP2 + pd(y)];
       }
       __syncthreads();
     }
@@ -331,13 +383,15 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
         batched_tile_loop<IT2I, 4>(
             [&](int it) {
               const int t = tid + it * kThreads;
-              const int q = t / N, X = t - q * N;
+              int q, X;
+              divmod_small<N, (unsigned)(((T::RB2 * N + kThreads - 1) / kThreads + 3) / 4 * 4 * kThreads)>(t, q, X);
               const int ya = min(y0 + 2 * q, N - 1), yb = min(y0 + 2 * q + 1, N - 1);
-              return make_float2(im[ya * N + X], im[yb * N + X]);
+              return make_float2(at(im, (unsigned)(ya * N + X)), at(im, (unsigned)(yb * N + X)));
             },
             [&](int it, float2 ab) {
               const int t = tid + it * kThreads;
-              const int q = t / N, X = t - q * N;
+              int q, X;
+              divmod_small<N, (unsigned)(((T::RB2 * N + kThreads - 1) / kThreads + 3) / 4 * 4 * kThreads)>(t, q, X);
               if (2 * q + 1 >= nrow) ab.y = 0.f;
               if (q < nseq) buf[q * SP2 + pd(X)] = ab;
             });
@@ -349,15 +403,17 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       batched_tile_loop<IT2, 4>(
           [&](int it) {
             const int t = tid + it * kThreads;
-            const int q = t / L2, k = t - q * L2;
+            int q, k;
+            divmod_small<L2, (unsigned)(((T::RB2 * L2 + kThreads - 1) / kThreads + 3) / 4 * 4 * kThreads)>(t, q, k);
             const int ya = min(y0 + 2 * q, N - 1), yb = min(y0 + 2 * q + 1, N - 1);
             const int ks = k <= L2 / 2 ? k : L2 - k;
-            const float2 a = U[uidx(ya, ks)], b = U[uidx(yb, ks)];
+            const float2 a = at(U, uidx(ya, ks)), b = at(U, uidx(yb, ks));
             return make_float4(a.x, a.y, b.x, b.y);
           },
           [&](int it, float4 ab) {
             const int t = tid + it * kThreads;
-            const int q = t / L2, k = t - q * L2;
+            int q, k;
+            divmod_small<L2, (unsigned)(((T::RB2 * L2 + kThreads - 1) / kThreads + 3) / 4 * 4 * kThreads)>(t, q, k);
             // Hermitian extension of the half spectrum, Im of columns 0 and L/2 dropped
             const int ks = k <= L2 / 2 ? k : L2 - k;
             float2 a = make_float2(ab.x, ab.y), b = make_float2(ab.z, ab.w);
@@ -396,7 +452,8 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       if (io.proj && !io.whiten && !io.normz) {
         // the cropped projection itself: rows ya (re) and yb (im) of each slot front
         for (int t = tid; t < nseq * N; t += nthr) {
-          const int q = t / N, X = t - q * N;
+          int q, X;
+          divmod_small<N, (unsigned)(T::RB2 * N)>(t, q, X);
           const float2 v = buf[q * SP2 + pd(X)];
           const int ya = y0 + 2 * q;
           io.proj[((size_t)d * N + ya) * N + X] = v.x;
@@ -407,13 +464,14 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       }
       fft_lds_inplace<N, false, kPad>(buf, nseq, SP2, tw1, tid, nthr);
       for (int t = tid; t < nseq * MH; t += nthr) {
-        const int q = t / MH, k = t - q * MH;
+        int q, k;
+        divmod_small<MH, (unsigned)(T::RB2 * MH)>(t, q, k);
         const float2 z = buf[q * SP2 + pd(k)];
         const float2 zc = cconj(buf[q * SP2 + pd((N - k) % N)]);
         const int ya = y0 + 2 * q;
-        W[widx(ya, k)] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+        at(W, widx(ya, k)) = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
         if (ya + 1 < y0 + nrow)
-          W[widx(ya + 1, k)] = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+          at(W, widx(ya + 1, k)) = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
       }
       __syncthreads();
     }
@@ -438,23 +496,22 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       constexpr int IT3 = (T::TK3 * N + kThreads - 1) / kThreads;
       batched_tile_loop<IT3, 4>(
           [&](int it) {
-            const int t = tid + it * kThreads;
-            const int Y = t / T::TK3, q = t - Y * T::TK3;
             // W's last column tile is padded to TK3 columns: the load is in bounds
-            return W[widx(min(Y, N - 1), k0 + q)];
+            return at(W, (unsigned)k0 * N + (unsigned)min(tid + it * kThreads, T::TK3 * N - 1));
           },
           [&](int it, float2 v) {
-            const int t = tid + it * kThreads;
-            const int Y = t / T::TK3, q = t - Y * T::TK3;
+            int Y, q;
+            tile_yq(tid + it * kThreads, Y, q);
             if (Y < N && q < nk) buf[q * SP1 + pd(Y)] = v;
           });
       __syncthreads();
       fft_lds_inplace<N, false, kPad>(buf, nk, SP1, tw1, tid, nthr);
       const float2* res = buf;
       for (int t = tid; t < T::TK3 * N; t += nthr) {
-        const int ky = t / T::TK3, q = t - ky * T::TK3;
+        int ky, q;
+        tile_yq(t, ky, q);
         if (q >= nk) continue;
-        W[widx(ky, k0 + q)] = res[q * SP1 + pd(ky)];
+        at(W, (unsigned)k0 * N + (unsigned)t) = res[q * SP1 + pd(ky)];
       }
       const int lane = tid & 63;
       for (int task = tid >> 6; task < 2 * nk; task += nthr / 64) {
@@ -549,13 +606,11 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       constexpr int IT5 = (T::TK3 * N + kThreads - 1) / kThreads;
       batched_tile_loop<IT5, 4>(
           [&](int it) {
-            const int t = tid + it * kThreads;
-            const int ky = min(t / T::TK3, N - 1), q = t - (t / T::TK3) * T::TK3;
-            return W[widx(ky, k0 + q)];
+            return at(W, (unsigned)k0 * N + (unsigned)min(tid + it * kThreads, T::TK3 * N - 1));
           },
           [&](int it, float2 v) {
-            const int t = tid + it * kThreads;
-            const int ky = t / T::TK3, q = t - ky * T::TK3;
+            int ky, q;
+            tile_yq(tid + it * kThreads, ky, q);
             if (ky < N && q < nk) {
               const int fy = ky < N / 2 ? ky : ky - N;
               const float g = div[ring_of(fy, k0 + q)];
@@ -565,9 +620,10 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       __syncthreads();
       fft_lds_inplace<N, true, kPad>(buf, nk, SP1, tw1, tid, nthr);
       for (int t = tid; t < T::TK3 * N; t += nthr) {
-        const int Y = t / T::TK3, q = t - Y * T::TK3;
+        int Y, q;
+        tile_yq(t, Y, q);
         if (q >= nk) continue;
-        W[widx(Y, k0 + q)] = buf[q * SP1 + pd(Y)];
+        at(W, (unsigned)k0 * N + (unsigned)t) = buf[q * SP1 + pd(Y)];
       }
       __syncthreads();
     }
@@ -582,15 +638,17 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       batched_tile_loop<IT6, 4>(
           [&](int it) {
             const int t = tid + it * kThreads;
-            const int q = t / N, k = t - q * N;
+            int q, k;
+            divmod_small<N, (unsigned)(((T::RB6 * N + kThreads - 1) / kThreads + 3) / 4 * 4 * kThreads)>(t, q, k);
             const int ya = min(y0 + 2 * q, N - 1), yb = min(y0 + 2 * q + 1, N - 1);
             const int ks = k <= N / 2 ? k : N - k;
-            const float2 a = W[widx(ya, ks)], b = W[widx(yb, ks)];
+            const float2 a = at(W, widx(ya, ks)), b = at(W, widx(yb, ks));
             return make_float4(a.x, a.y, b.x, b.y);
           },
           [&](int it, float4 ab) {
             const int t = tid + it * kThreads;
-            const int q = t / N, k = t - q * N;
+            int q, k;
+            divmod_small<N, (unsigned)(((T::RB6 * N + kThreads - 1) / kThreads + 3) / 4 * 4 * kThreads)>(t, q, k);
             const int ks = k <= N / 2 ? k : N - k;
             float2 a = make_float2(ab.x, ab.y), b = make_float2(ab.z, ab.w);
             if (2 * q + 1 >= nrow) b = make_float2(0.f, 0.f);
@@ -602,15 +660,16 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       fft_lds_inplace<N, true, kPad>(buf, nseq, SP1, tw1, tid, nthr);
       const float sc = 1.0f / ((float)N * (float)N);
       for (int t = tid; t < nseq * N; t += nthr) {
-        const int q = t / N, X = t - q * N;
+        int q, X;
+        divmod_small<N, (unsigned)(T::RB6 * N)>(t, q, X);
         const float2 v = buf[q * SP1 + pd(X)];
         const int ya = y0 + 2 * q;
         const float za = v.x * sc;
-        Z[(size_t)ya * N + X] = za;
+        at(Z, (unsigned)(ya * N + X)) = za;
         acc[0] += za; acc[1] += (double)za * za; acc[2] += (double)ya * za; acc[3] += (double)X * za;
         if (ya + 1 < y0 + nrow) {
           const float zb = v.y * sc;
-          Z[(size_t)(ya + 1) * N + X] = zb;
+          at(Z, (unsigned)((ya + 1) * N + X)) = zb;
           acc[0] += zb; acc[1] += (double)zb * zb; acc[2] += (double)(ya + 1) * zb; acc[3] += (double)X * zb;
         }
       }
@@ -661,7 +720,8 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       // real-space output: Z, or (Z - plane - mean) / std
       float* po = io.proj + (size_t)d * N * N;
       for (int t = tid; t < N * N; t += nthr) {
-        const int y = t / N, x = t - y * N;
+        int y, x;
+        divmod_small<N, (unsigned)(N * N)>(t, y, x);
         const float z = Z[t];
         po[t] = io.normz ? ((z - ((fa * (float)y + fb * (float)x) + fc)) - fmean) * fis : z;
       }
@@ -676,14 +736,17 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       batched_tile_loop<IT8, 4>(
           [&](int it) {
             const int t = tid + it * kThreads;
-            const int q = t / N, X = t - q * N;
+            int q, X;
+            divmod_small<N, (unsigned)(((T::RB6 * N + kThreads - 1) / kThreads + 3) / 4 * 4 * kThreads)>(t, q, X);
             const int ya = min(y0 + 2 * q, N - 1), yb = min(y0 + 2 * q + 1, N - 1);
-            return make_float4(Z[ya * N + X], Z[yb * N + X], mask ? mask[ya * N + X] : 1.0f,
-                               mask ? mask[yb * N + X] : 1.0f);
+            return make_float4(at(Z, (unsigned)(ya * N + X)), at(Z, (unsigned)(yb * N + X)),
+                               mask ? at(mask, (unsigned)(ya * N + X)) : 1.0f,
+                               mask ? at(mask, (unsigned)(yb * N + X)) : 1.0f);
           },
           [&](int it, float4 zm) {
             const int t = tid + it * kThreads;
-            const int q = t / N, X = t - q * N;
+            int q, X;
+            divmod_small<N, (unsigned)(((T::RB6 * N + kThreads - 1) / kThreads + 3) / 4 * 4 * kThreads)>(t, q, X);
             const int ya = y0 + 2 * q, yb = ya + 1;
             float va = ((zm.x - ((fa * (float)ya + fb * (float)X) + fc)) - fmean) * fis;
             if (mask) va *= zm.z;
@@ -697,13 +760,14 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       __syncthreads();
       fft_lds_inplace<N, false, kPad>(buf, nseq, SP1, tw1, tid, nthr);
       for (int t = tid; t < nseq * MH; t += nthr) {
-        const int q = t / MH, k = t - q * MH;
+        int q, k;
+        divmod_small<MH, (unsigned)(T::RB6 * MH)>(t, q, k);
         const float2 z = buf[q * SP1 + pd(k)];
         const float2 zc = cconj(buf[q * SP1 + pd((N - k) % N)]);
         const int ya = y0 + 2 * q;
-        W[widx(ya, k)] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+        at(W, widx(ya, k)) = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
         if (ya + 1 < y0 + nrow)
-          W[widx(ya + 1, k)] = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+          at(W, widx(ya + 1, k)) = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
       }
       __syncthreads();
     }
@@ -717,13 +781,11 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       constexpr int IT9 = (T::TK3 * N + kThreads - 1) / kThreads;
       batched_tile_loop<IT9, 4>(
           [&](int it) {
-            const int t = tid + it * kThreads;
-            const int Y = t / T::TK3, q = t - Y * T::TK3;
-            return W[widx(min(Y, N - 1), k0 + q)];
+            return at(W, (unsigned)k0 * N + (unsigned)min(tid + it * kThreads, T::TK3 * N - 1));
           },
           [&](int it, float2 v) {
-            const int t = tid + it * kThreads;
-            const int Y = t / T::TK3, q = t - Y * T::TK3;
+            int Y, q;
+            tile_yq(tid + it * kThreads, Y, q);
             if (Y < N && q < nk) buf[q * SP1 + pd(Y)] = v;
           });
       __syncthreads();
@@ -734,15 +796,18 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       constexpr int IT9S = (T::TK3 * N + kThreads - 1) / kThreads;  // covers TK3 * n (n <= N)
       batched_tile_loop<IT9S, 4>(
           [&](int it) {
-            const int t = tid + it * kThreads;
-            const int r = min(t / T::TK3, n - 1), c = min(k0 + t % T::TK3, mh_n - 1);
-            const float g = fsc ? fsc[r * mh_n + c] : 1.0f;
-            const int j = co.kslot ? co.kslot[r * mh_n + c] : -1;
+            int r, c;
+            tile_yq(tid + it * kThreads, r, c);
+            r = min(r, n - 1);
+            c = min(k0 + c, mh_n - 1);
+            const float g = fsc ? at(fsc, (unsigned)(r * mh_n + c)) : 1.0f;
+            const int j = co.kslot ? at(co.kslot, (unsigned)(r * mh_n + c)) : -1;
             return make_float2(g, __int_as_float(j));
           },
           [&](int it, float2 gj) {
             const int t = tid + it * kThreads;
-            const int r = t / T::TK3, q = t - r * T::TK3;
+            int r, q;
+            tile_yq(t, r, q);
             if (t >= T::TK3 * n || q >= nk) return;
             const int c = k0 + q;
             const int fy = r < n / 2 ? r : r - n;

@@ -29,13 +29,17 @@ __device__ __forceinline__ size_t vdft_brick_x(int kx) { return (size_t)(kx >> 2
 // 32-bit forms of the same offsets (the spectrum holds < 2^32 values for every
 // supported length), so a gather can address D with a scalar base and a
 // 32-bit vector offset.
+// (24-bit multiplies: the strides and brick counts fit, and v_mul_u32_u24 is
+// full rate where a 32-bit multiply is not.)
 template <int L>
 __device__ __forceinline__ unsigned vdft_brick_z32(unsigned kz) {
-  return (kz >> 1) * (unsigned)VdftBricks<L>::ZSTRIDE + (kz & 1u) * 8u;
+  static_assert(VdftBricks<L>::ZSTRIDE < (1u << 24), "24-bit stride");
+  return __umul24(kz >> 1, (unsigned)VdftBricks<L>::ZSTRIDE) + (kz & 1u) * 8u;
 }
 template <int L>
 __device__ __forceinline__ unsigned vdft_brick_y32(unsigned ky) {
-  return (ky >> 1) * (unsigned)VdftBricks<L>::YSTRIDE + (ky & 1u) * 4u;
+  static_assert(VdftBricks<L>::YSTRIDE < (1u << 24), "24-bit stride");
+  return __umul24(ky >> 1, (unsigned)VdftBricks<L>::YSTRIDE) + (ky & 1u) * 4u;
 }
 __device__ __forceinline__ unsigned vdft_brick_x32(unsigned kx) { return (kx >> 2) * 16u + (kx & 3u); }
 static_assert(VdftBricks<512>::ELEMS < (1ull << 32), "32-bit brick offsets");
