@@ -34,6 +34,7 @@ SIGNATURES = {
     "gis_extract_slices": (_i, [_vp, _i, _vp, _i, _vp, _vp, _sz, _vp]),
     "gis_sinc2_filter": (_i, [_vp, _i, _i, _i, _vp]),
     "gis_slice_sample_grid": (_i, [_vp, _i, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp]),
+    "gis_sample_dft_3d": (_i, [_vp, _i, _i, _i, _vp, _sz, _vp, _vp]),
     "gis_volume_dft": (_i, [_vp, _i, _vp, _vp, _sz, _vp]),
     "gis_build_templates_workspace": (_sz, [_i, _i]),
     "gis_build_templates": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _sz, _vp]),

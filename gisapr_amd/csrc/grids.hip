@@ -1,14 +1,19 @@
-// Two element-wise helpers of the grid_sample-based projector that callers of
-// the search module's namespace use directly:
+// Element-wise helpers of the grid_sample-based projectors that callers use
+// directly:
 //
 //   gis_sinc2_filter      volume[z,y,x] *= sinc^2(|f|), f = fftshift(fftfreq(n))
 //                         per axis (v606:837-853; project3d_and_whiten_cuda_nowrite.py:42-81)
 //   gis_slice_sample_grid rotated central-slice sampling grid for a 5-D
 //                         F.grid_sample (align_corners=True) over an rfft
 //                         volume, plus the Friedel-conjugation mask (v606:866-897)
+//   gis_sample_dft_3d     trilinear, border-clamped samples of a complex volume
+//                         at array coordinates (project3d_and_whiten_cuda.py:612-650)
 //
-// Both are HBM-bound streams: the filter reads and writes 8 bytes per voxel;
-// the grid reads one base point per thread and writes 13 bytes per sample.
+// All three are HBM-bound streams: the filter reads and writes 8 bytes per
+// voxel; the grid reads one base point per thread and writes 13 bytes per
+// sample; the sampler reads 12 bytes of coordinates and eight 8-byte taps and
+// writes 8 bytes per sample.  Coordinates are clamped before use, so NaN or
+// infinite ones read the border, never outside the volume.
 // Float arithmetic follows the reference op by op (float32 products, sums and
 // divisions in the same order, a correctly rounded sqrt and division), so the
 // results match torch's float32 values to a few ulp.
@@ -91,6 +96,42 @@ __global__ __launch_bounds__(kGridThreads) void slice_sample_grid_kernel(
   }
 }
 
+// One thread per sample: trilinear interpolation of a natural-layout complex
+// [D][H][W] volume at array coordinates (z, y, x), each clamped to
+// [0, size - 1] first (border padding); taps beyond the last index carry
+// weight 0 and are read clamped.
+__global__ __launch_bounds__(kGridThreads) void sample_dft_3d_kernel(const float2* __restrict__ dft, int D, int H,
+                                                                     int W, const float* __restrict__ coords,
+                                                                     size_t P, float2* __restrict__ out) {
+  for (size_t t = (size_t)blockIdx.x * kGridThreads + threadIdx.x; t < P; t += (size_t)gridDim.x * kGridThreads) {
+    const float cz = fminf(fmaxf(coords[3 * t + 0], 0.0f), (float)(D - 1));
+    const float cy = fminf(fmaxf(coords[3 * t + 1], 0.0f), (float)(H - 1));
+    const float cx = fminf(fmaxf(coords[3 * t + 2], 0.0f), (float)(W - 1));
+    const float fz = floorf(cz), fy = floorf(cy), fx = floorf(cx);
+    const float tz = cz - fz, ty = cy - fy, tx = cx - fx;
+    const int z0 = (int)fz, y0 = (int)fy, x0 = (int)fx;
+    const int z1 = min(z0 + 1, D - 1), y1 = min(y0 + 1, H - 1), x1 = min(x0 + 1, W - 1);
+    auto at = [&](int z, int y, int x) { return dft[((size_t)z * H + y) * W + x]; };
+    const float2 v000 = at(z0, y0, x0), v001 = at(z0, y0, x1), v010 = at(z0, y1, x0), v011 = at(z0, y1, x1);
+    const float2 v100 = at(z1, y0, x0), v101 = at(z1, y0, x1), v110 = at(z1, y1, x0), v111 = at(z1, y1, x1);
+    const float wz0 = 1.0f - tz, wy0 = 1.0f - ty, wx0 = 1.0f - tx;
+    float2 acc = make_float2(0.f, 0.f);
+    auto tap = [&](float2 v, float w) {
+      acc.x += w * v.x;
+      acc.y += w * v.y;
+    };
+    tap(v000, (wz0 * wy0) * wx0);
+    tap(v001, (wz0 * wy0) * tx);
+    tap(v010, (wz0 * ty) * wx0);
+    tap(v011, (wz0 * ty) * tx);
+    tap(v100, (tz * wy0) * wx0);
+    tap(v101, (tz * wy0) * tx);
+    tap(v110, (tz * ty) * wx0);
+    tap(v111, (tz * ty) * tx);
+    out[t] = acc;
+  }
+}
+
 int grid_blocks(size_t work) {
   const size_t want = (work + kGridThreads - 1) / kGridThreads;
   const size_t cap = (size_t)gis_cu_count() * 32;
@@ -133,5 +174,21 @@ GIS_API int gis_slice_sample_grid(const float* base_zyx, int P, const float* R, 
                      base_zyx, P, R, M, (float)img_d, (float)img_h, (float)img_w, (float)(img_d / 2),
                      (float)(img_h / 2), den_d, den_h, den_w, grid_xyz, conj);
   GIS_LAUNCH_CHECK("gis_slice_sample_grid");
+  return GIS_OK;
+}
+
+// Trilinear samples of a complex [D][H][W] volume (natural layout, complex64)
+// at P array coordinates coords [P][3] (z, y, x; border-clamped), into out
+// [P] complex64: sample_dft_3d of project3d_and_whiten_cuda.py:612-650 (its
+// grid_sample with align_corners=True and border padding on the real and
+// imaginary channels).
+GIS_API int gis_sample_dft_3d(const void* dft, int D, int H, int W, const float* coords, size_t P, void* out,
+                              hipStream_t stream) {
+  GIS_REQUIRE(dft && coords && out, "gis_sample_dft_3d: null pointer");
+  GIS_REQUIRE(D > 0 && H > 0 && W > 0, "gis_sample_dft_3d: shape (%d, %d, %d) must be positive", D, H, W);
+  if (P == 0) return GIS_OK;
+  hipLaunchKernelGGL(sample_dft_3d_kernel, dim3(grid_blocks(P)), dim3(kGridThreads), 0, stream, (const float2*)dft, D,
+                     H, W, coords, P, (float2*)out);
+  GIS_LAUNCH_CHECK("gis_sample_dft_3d");
   return GIS_OK;
 }

@@ -178,6 +178,25 @@ def slice_sample_grid(base_zyx: torch.Tensor, rts: torch.Tensor, image_shape, df
     return grid, conj
 
 
+def sample_dft_3d(dft: torch.Tensor, coordinates: torch.Tensor) -> torch.Tensor:
+    """Trilinear, border-clamped samples [...] complex64 of a complex64 [D, H, W]
+    volume at array coordinates [..., 3] (z, y, x)."""
+    require_hip(dft, "dft")
+    if dft.dtype != torch.complex64 or dft.ndim != 3:
+        raise ValueError("dft must be a complex64 [D, H, W] volume")
+    if not isinstance(coordinates, torch.Tensor) or coordinates.shape[-1] != 3:
+        raise ValueError("coordinates must be [..., 3] (z, y, x)")
+    c = coordinates.to(dft.device, torch.float32).contiguous()
+    lead = tuple(c.shape[:-1])
+    out = torch.empty(lead, dtype=torch.complex64, device=dft.device)
+    P = int(out.numel())
+    if P:
+        D, H, W = (int(v) for v in dft.shape)
+        check(lib().gis_sample_dft_3d(ptr(dft), D, H, W, ptr(c), P, ptr(out), stream_of(dft.device)),
+              "gis_sample_dft_3d")
+    return out
+
+
 def whiten_images(images: torch.Tensor, whiten: bool = True, normalize: bool = True, eps: float = 1e-12,
                   out: Optional[torch.Tensor] = None) -> torch.Tensor:
     """Ring division of the spectrum (whiten) and / or least-squares plane +

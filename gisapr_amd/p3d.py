@@ -26,9 +26,12 @@ import torch
 from . import host, kernels, mrc, star
 from .projection import apply_sinc2_filter_inplace_3d  # noqa: F401  (project3d_and_whiten_cuda_nowrite.py:42-81)
 
-__all__ = ["create_project3d_parser", "Euler_angles2matrix", "build_radius_map_torch", "build_xy_grid_torch",
-           "getSpectrum_divideBySpectrum_torch", "normalize_torch", "extract_central_slices_rfft",
-           "apply_sinc2_filter_inplace_3d", "run_project3d", "run_projection", "extract_filename", "main"]
+__all__ = ["create_project3d_parser", "Euler_angles2matrix", "DEG2RAD", "build_radius_map_torch",
+           "build_xy_grid_torch", "getSpectrum_divideBySpectrum_torch", "normalize_torch",
+           "extract_central_slices_rfft", "apply_sinc2_filter_inplace_3d", "run_project3d", "run_projection",
+           "extract_filename", "main", "judge_relion30_or_relion31", "judge_mline0", "judge_mline1",
+           "rfft_shape", "dft_center", "fftshift_2d", "fftshift_3d", "fftfreq_grid", "central_slice_grid",
+           "rotated_central_slice_grid", "fftfreq_to_dft_coordinates", "array_to_grid_sample", "sample_dft_3d"]
 
 
 def create_project3d_parser() -> argparse.ArgumentParser:
@@ -127,6 +130,153 @@ def extract_central_slices_rfft(dft: torch.Tensor, image_shape, rotation_matrice
     natural = torch.fft.ifftshift(dft.to(torch.complex64), dim=(0, 1))
     out = kernels.extract_slices(kernels.natural_to_bricked(natural.contiguous()), L, R.contiguous())
     return out.reshape(*lead, L, L // 2 + 1)
+
+
+# ---------------------------------------------------------------------------
+# Frequency-grid geometry of the projector script (the libtilt helpers of
+# project3d_and_whiten_cuda.py:292-650).  Small index tables and coordinate
+# maps on the caller's device; the sampling itself is a HIP kernel.
+# ---------------------------------------------------------------------------
+
+def DEG2RAD(x):
+    """Degrees to radians with pi written 3.14159265359 (:692-693)."""
+    return x / 180.0 * 3.14159265359
+
+
+def rfft_shape(input_shape: Sequence[int]) -> tuple:
+    """Shape after an rfft over the last axis: last = int(last / 2 + 1) (:298-302)."""
+    out = list(input_shape)
+    out[-1] = int(out[-1] / 2 + 1)
+    return tuple(out)
+
+
+def fftshift_2d(input: torch.Tensor, rfft: bool) -> torch.Tensor:  # noqa: A002 (reference name)
+    """fftshift over the last two axes, or only the rows of an rfft (:292-297)."""
+    return torch.fft.fftshift(input, dim=(-2,) if rfft else (-2, -1))
+
+
+def fftshift_3d(input: torch.Tensor, rfft: bool) -> torch.Tensor:  # noqa: A002 (reference name)
+    """fftshift over the last three axes, or all but the last of an rfft (:444-449)."""
+    return torch.fft.fftshift(input, dim=(-3, -2) if rfft else (-3, -2, -1))
+
+
+def _axis_freqs(n: int, spacing: float, half: bool, device) -> torch.Tensor:
+    return (torch.fft.rfftfreq if half else torch.fft.fftfreq)(n, d=spacing, device=device)
+
+
+def fftfreq_grid(image_shape, rfft: bool, fftshift: bool = False, spacing=1, norm: bool = False,
+                 device=None) -> torch.Tensor:
+    """[*shape, ndim] DFT sample frequencies of a 2-D or 3-D image (the last
+    axis rfftfreq when ``rfft``), optionally fftshifted, or their Euclidean
+    norm [*shape] when ``norm`` (:303-372)."""
+    shape = tuple(int(v) for v in image_shape)
+    if len(shape) not in (2, 3):
+        raise NotImplementedError("Construction of fftfreq grids is currently only supported for 2D and 3D images.")
+    sp = list(spacing) if isinstance(spacing, Sequence) else [spacing] * len(shape)
+    axes = [_axis_freqs(n, float(d), rfft and i == len(shape) - 1, device) for i, (n, d) in enumerate(zip(shape, sp))]
+    grid = torch.stack(torch.meshgrid(*axes, indexing="ij"), dim=-1)
+    if fftshift:
+        grid = torch.movedim((fftshift_2d if len(shape) == 2 else fftshift_3d)(torch.movedim(grid, -1, 0), rfft), 0,
+                             -1)
+    if norm:
+        grid = (grid ** 2).sum(dim=-1) ** 0.5
+    return grid
+
+
+def central_slice_grid(image_shape, rfft: bool, fftshift: bool = False, device=None) -> torch.Tensor:
+    """[h, w(/2+1), 3] (z = 0, y, x) frequencies of the central xy slice (:450-468)."""
+    h, w = (int(v) for v in tuple(image_shape)[-2:])
+    hw = fftfreq_grid((h, w), rfft=rfft, fftshift=fftshift, device=device)
+    return torch.cat([torch.zeros_like(hw[..., :1]), hw], dim=-1)
+
+
+def rotated_central_slice_grid(image_shape, rotation_matrices: torch.Tensor, rotation_matrix_zyx: bool, rfft: bool,
+                               fftshift: bool = False, device=None) -> torch.Tensor:
+    """[..., h, w, 3] (z, y, x) central-slice frequencies rotated by each
+    matrix (acting on zyx vectors, or on xyz when not ``rotation_matrix_zyx``)
+    (:469-489)."""
+    grid = central_slice_grid(image_shape, rfft=rfft, fftshift=fftshift, device=device)
+    R = torch.as_tensor(rotation_matrices).to(grid.device, grid.dtype)
+    if not rotation_matrix_zyx:
+        R = R.flip(-1).flip(-2)  # the same rotation acting on zyx vectors
+    return torch.einsum("...ij,hwj->...hwi", R, grid)
+
+
+def dft_center(image_shape, rfft: bool, fftshifted: bool, device=None) -> torch.Tensor:
+    """Array index of the DFT origin (:524-537): shape // 2 when fftshifted
+    (of the rfft shape when ``rfft``), last axis 0 for an rfft."""
+    shape = rfft_shape(image_shape) if rfft else tuple(image_shape)
+    c = [int(v) // 2 if fftshifted else 0 for v in shape]
+    if rfft:
+        c[-1] = 0
+    return torch.tensor(c, dtype=torch.long, device=device)
+
+
+def fftfreq_to_dft_coordinates(frequencies: torch.Tensor, image_shape, rfft: bool) -> torch.Tensor:
+    """Frequencies [..., d] to array coordinates in a fftshifted DFT (:538-571):
+    f * size (the last axis f * 2 (w_rfft - 1) for an rfft) plus the origin."""
+    shape = [float(v) for v in image_shape]
+    scale = torch.tensor(shape, dtype=frequencies.dtype, device=frequencies.device)
+    if rfft:
+        scale[-1] = 2.0 * (rfft_shape(image_shape)[-1] - 1)
+    dc = dft_center(image_shape, rfft=rfft, fftshifted=True, device=frequencies.device)
+    return frequencies * scale + dc
+
+
+def array_to_grid_sample(array_coordinates: torch.Tensor, array_shape) -> torch.Tensor:
+    """Array coordinates [..., d] to grid_sample's align_corners=True grid,
+    c / (size/2 - 1/2) - 1, in reversed (x, y, z) order (:573-599)."""
+    shape = torch.as_tensor(array_shape, dtype=array_coordinates.dtype, device=array_coordinates.device)
+    return torch.flip(array_coordinates / (0.5 * shape - 0.5) - 1, dims=(-1,))
+
+
+def sample_dft_3d(dft: torch.Tensor, coordinates: torch.Tensor) -> torch.Tensor:
+    """Trilinear, border-clamped samples [...] of a complex [D, H, W] volume at
+    array coordinates [..., 3] (z, y, x) (:612-650), one HIP launch."""
+    return kernels.sample_dft_3d(dft.to(torch.complex64).contiguous(), coordinates)
+
+
+# ---------------------------------------------------------------------------
+# STAR helpers of the projector script (:651-691)
+# ---------------------------------------------------------------------------
+
+def judge_relion30_or_relion31(inline: Sequence[str]) -> int:
+    """1 for a RELION 3.0 STAR, 0 for 3.1 (:651-665): among the first three
+    lines, a '# version N' line with N > 30000 marks 3.1; a '# ... 3.0.8'
+    header ends the check as 3.0."""
+    for line in list(inline)[:3]:
+        tok = line.split()
+        if len(tok) < 2:
+            continue
+        if tok[0] == "#" and len(tok) >= 4 and tok[3] == "3.0.8":
+            break
+        if int(tok[2]) > 30000:
+            return 0
+    return 1
+
+
+def _first_data_after_labels(inline: Sequence[str], start: int, stop: int):
+    seen_label = False
+    for i in range(start, min(stop, len(inline))):
+        tok = inline[i].split()
+        if not tok:
+            continue
+        if tok[0][0] == "_":
+            seen_label = True
+        elif seen_label:
+            return i
+    return None
+
+
+def judge_mline0(inline: Sequence[str]):
+    """Index of the first data line after the first run of '_' label lines,
+    searched in the first 60 lines (:666-678); None when there is none."""
+    return _first_data_after_labels(inline, 0, 60)
+
+
+def judge_mline1(inline: Sequence[str], start: int):
+    """judge_mline0 from line ``start``, searched up to line 70 (:679-691)."""
+    return _first_data_after_labels(inline, int(start), 70)
 
 
 def extract_filename(file_path: str) -> str:

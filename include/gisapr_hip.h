@@ -189,6 +189,14 @@ int gis_sinc2_filter(float* vol, int D, int H, int W, void* stream);
 int gis_slice_sample_grid(const float* base_zyx, int P, const float* R, int M, int img_d, int img_h, int img_w,
                           int dft_d, int dft_h, int dft_w, float* grid_xyz, unsigned char* conj, void* stream);
 
+/*
+ * sample_dft_3d (project3d_and_whiten_cuda.py:612-650): trilinear samples of a
+ * complex64 volume dft [D][H][W] (natural layout) at P array coordinates
+ * coords [P][3] (z, y, x), each clamped to [0, size-1] (grid_sample with
+ * align_corners=True and border padding), into out [P] complex64.
+ */
+int gis_sample_dft_3d(const void* dft, int D, int H, int W, const float* coords, size_t P, void* out, void* stream);
+
 /* Workspace bytes gis_prepare_particles needs for `particles` (= B) particles. */
 size_t gis_prepare_particles_workspace(int N, int particles);
 

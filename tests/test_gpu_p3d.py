@@ -205,3 +205,34 @@ def test_run_projection_library_form(hip_device, tmp_path, monkeypatch, return_a
     out = (tmp_path / "proj").read_text().splitlines(keepends=True)
     assert out[:len(header)] == header
     assert [ln.split("\t")[0] for ln in out[len(header):]] == [f"{i + 1}@proj.mrcs" for i in range(len(ROT))]
+
+
+def test_sample_dft_3d_and_slice_geometry_match_oracle(hip_device):
+    """sample_dft_3d against the oracle's trilinear border sampler (coordinates
+    inside, on and beyond the edges), and the script's own route to a central
+    slice (rotated grid -> mirror x < 0 -> array coordinates -> sample ->
+    conjugate) against the oracle's central slice."""
+    import torch
+    from gisapr_amd import host, p3d
+    say("sample_dft_3d")
+    rng = np.random.default_rng(21)
+    D, H, W = 12, 10, 7
+    vol = (rng.standard_normal((D, H, W)) + 1j * rng.standard_normal((D, H, W))).astype(np.complex64)
+    coords = np.stack([rng.uniform(-2, D + 1, 500), rng.uniform(-2, H + 1, 500), rng.uniform(-2, W + 1, 500)], -1)
+    coords[:3] = [[0, 0, 0], [D - 1, H - 1, W - 1], [5.5, 4.25, 6.0]]
+    got = p3d.sample_dft_3d(torch.from_numpy(vol).to(hip_device),
+                            torch.from_numpy(coords.astype(np.float32)).to(hip_device)).cpu().numpy()
+    want = opj.trilinear_border(vol.astype(np.complex128), coords[:, 0], coords[:, 1], coords[:, 2])
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-5 * np.abs(want).max())
+    L = 32
+    ref = opj.volume_dft(blob_volume(L // 2, 4).astype(np.float64), np.float64)  # fftshifted rows and planes
+    dft = torch.from_numpy(ref.astype(np.complex64)).to(hip_device)
+    R = np.ascontiguousarray(host.rotation_matrices_t(ROT, TILT), np.float32)
+    grid = p3d.rotated_central_slice_grid((L, L, L), torch.from_numpy(R).to(hip_device), False, rfft=True,
+                                          fftshift=True, device=hip_device)
+    conj = grid[..., 2] < 0
+    grid = torch.where(conj[..., None], -grid, grid)
+    s = p3d.sample_dft_3d(dft, p3d.fftfreq_to_dft_coordinates(grid, (L, L, L), rfft=True))
+    s = torch.where(conj, torch.conj(s), s).cpu().numpy()
+    for m in range(len(ROT)):
+        assert rel_l2(s[m], opj.central_slice(ref, R[m], np.float64)) < 2e-5, m
