@@ -42,8 +42,9 @@ from .projection import (ProjectionWhitening, apply_sinc2_filter_inplace_3d,  # 
                          build_projection_base_grid, rotated_grid_to_grid_sample)
 from .host import (calculate_angular_distance, euler_angles_to_matrix, expand_radial_values,  # noqa: F401
                    generate_soft_mask, psi_sampling_for_record)
-from .star import (ModelAngles, ParticleRecord, ParticleTable, find_star_loop, read_model_angles,  # noqa: F401
-                   read_particle_table, select_particle_records)
+from .star import (ModelAngles, ParticleRecord, ParticleTable, StarLoop, find_star_loop,  # noqa: F401
+                   read_model_angles, read_particle_table, read_text_lines, select_particle_records)
+from .star import _label_column as _parse_label_index  # noqa: F401  (v606:86-92)
 
 torch.set_grad_enabled(False)
 
@@ -58,6 +59,88 @@ def build_local_orientation_entries(particle_records, model_rot, model_tilt, loc
     tilt = np.asarray([r.tilt for r in particle_records], np.float64)
     local = host.local_orientation_map(rot, tilt, model_rot, model_tilt, float(local_range))
     return [[r.line_index] + l for r, l in zip(particle_records, local)]
+
+
+def clip_value(a: float, b: float, c: float) -> float:
+    """a clamped to [b, c] as float (v606:314-319)."""
+    return float(min(max(float(a), float(b)), float(c)))
+
+
+def dot_product(v1: Sequence[float], v2: Sequence[float]) -> float:
+    """Sum of products, or -9999999.0 for lengths that differ (v606:322-328)."""
+    if len(v1) != len(v2):
+        return -9999999.0
+    total = 0.0
+    for a, b in zip(v1, v2):
+        total += float(a) * float(b)
+    return total
+
+
+def _local_cache_valid(entries, particle_records, n_models: int) -> bool:
+    """A cached local map matches the selected records row for row and holds
+    valid direction indices (v606:381-393)."""
+    if not isinstance(entries, list) or len(entries) != len(particle_records):
+        return False
+    for entry, rec in zip(entries, particle_records):
+        if not isinstance(entry, list) or not entry or entry[0] != rec.line_index:
+            return False
+        if any(not isinstance(i, (int, np.integer)) or i < 0 or i >= n_models for i in entry[1:]):
+            return False
+    return True
+
+
+class _PlainDataUnpickler:
+    """Reads a pickle of plain lists and ints only: any other object (a class
+    or function reference, which is how a pickle runs code) is refused."""
+
+    def __init__(self, fh):
+        import pickle
+
+        class _Restricted(pickle.Unpickler):
+            def find_class(self, module, name):
+                raise pickle.UnpicklingError(f"refused object {module}.{name} in a local-orientation cache")
+
+        self._u = _Restricted(fh)
+
+    def load(self):
+        return self._u.load()
+
+
+def load_or_build_local_orientation_map(particle_star: str, particle_records, model_angles: ModelAngles,
+                                        local_range: float) -> Dict[int, List[int]]:
+    """{line_index: direction indices} with the v605/v606 cache file
+    ``<star>_LocalSearch_Angle_<int(range)>_data.pkl`` (v606:396-438): a valid
+    cache is reused, otherwise the map is built and the cache rewritten
+    atomically.  The cache is read as plain lists and ints only."""
+    import pickle
+    cache = particle_star + "_LocalSearch_Angle_" + str(int(local_range)) + "_data.pkl"
+    entries = None
+    if os.path.exists(cache):
+        try:
+            with open(cache, "rb") as fh:
+                loaded = _PlainDataUnpickler(fh).load()
+            if _local_cache_valid(loaded, particle_records, len(model_angles.rot)):
+                entries = loaded
+                print("Data loaded from file:", cache)
+            else:
+                print("Existing local-search cache is incompatible; rebuilding:", cache)
+        except Exception as exc:  # unreadable or refused: rebuild
+            print(f"Could not read local-search cache ({exc}); rebuilding: {cache}")
+    if entries is None:
+        entries = build_local_orientation_entries(particle_records, model_angles.rot, model_angles.tilt, local_range)
+        tmp = cache + f".tmp.{os.getpid()}"
+        try:
+            with open(tmp, "wb") as fh:
+                pickle.dump(entries, fh, protocol=pickle.HIGHEST_PROTOCOL)
+            os.replace(tmp, cache)
+            print("Data generated and saved to file:", cache)
+        finally:
+            if os.path.exists(tmp):
+                try:
+                    os.remove(tmp)
+                except OSError:
+                    pass
+    return {int(e[0]): [int(x) for x in e[1:]] for e in entries}
 
 
 def read_fsc_map(fsc_filename: str, original_box: int, new_box: int, ignore_fsc: bool) -> np.ndarray:

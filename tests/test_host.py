@@ -139,3 +139,34 @@ def test_search_session_cache_forms():
     d = {}
     assert _session_get(d, "k", factory) == 2 and _session_get(d, "k", factory) == 2 and d == {"k": 2}
     assert _session_get(None, "k", factory) == 3 and _session_get(None, "k", factory) == 4
+
+
+def test_v606_small_helpers_and_local_cache(tmp_path):
+    """clip_value, dot_product, the label-column parser, and the local
+    orientation cache: built, written, reused; a cache holding anything but
+    plain lists and ints is refused (no object is constructed) and rebuilt."""
+    import pickle
+    import numpy as np
+    from gisapr_amd import search
+    from gisapr_amd.star import ModelAngles, ParticleRecord
+    assert search.clip_value(5, 0, 3) == 3.0 and search.clip_value(-1, 0, 3) == 0.0 and search.clip_value(2, 0, 3) == 2.0
+    assert search.dot_product([1, 2], [3, 4]) == 11.0 and search.dot_product([1], [1, 2]) == -9999999.0
+    assert search._parse_label_index(["_rlnAngleRot", "#3"], 9) == 2 and search._parse_label_index(["_x"], 9) == 9
+    recs = [ParticleRecord(10, "1@a", 0.0, 30.0, 0.0), ParticleRecord(11, "2@a", 90.0, 60.0, 0.0)]
+    angles = ModelAngles([0.0, 5.0, 90.0, 180.0], [30.0, 32.0, 60.0, 90.0], [0.0] * 4, np.zeros((4, 3, 3)))
+    star_path = str(tmp_path / "p.star")
+    m1 = search.load_or_build_local_orientation_map(star_path, recs, angles, 10.0)
+    want = {e[0]: e[1:] for e in search.build_local_orientation_entries(recs, angles.rot, angles.tilt, 10.0)}
+    assert m1 == want
+    cache = tmp_path / "p.star_LocalSearch_Angle_10_data.pkl"
+    assert pickle.loads(cache.read_bytes()) == [[10] + want[10], [11] + want[11]]
+    assert search.load_or_build_local_orientation_map(star_path, recs, angles, 10.0) == m1  # reused
+
+    class Evil:
+        def __reduce__(self):
+            return (os.system, ("true",))
+
+    import os
+    cache.write_bytes(pickle.dumps([[10, Evil()]]))
+    assert search.load_or_build_local_orientation_map(star_path, recs, angles, 10.0) == m1  # refused, rebuilt
+    assert pickle.loads(cache.read_bytes()) == [[10] + want[10], [11] + want[11]]
