@@ -127,3 +127,52 @@ def generate_mrc(pdb_path, output, boxsize, apix, res, device="cuda", do_center=
     from .mrc import write_mrc
     write_mrc(output, data, voxel_size=float(apix))
     return 0
+
+
+def _round_away_from_zero(value):
+    """sign(v) floor(|v| + 0.5) in float64, as int64 (p2m:35-38)."""
+    import torch
+    v = torch.as_tensor(value).to(torch.float64)
+    return (torch.sign(v) * torch.floor(torch.abs(v) + 0.5)).to(torch.int64)
+
+
+def _voxel_offsets(width, device_string):
+    """[(2w+1)^3, 3] int64 (x, y, z) offsets of a splat window (p2m:26-32)."""
+    import torch
+    axis = torch.arange(2 * int(width) + 1, device=device_string, dtype=torch.int64)
+    zz, yy, xx = torch.meshgrid(axis, axis, axis, indexing="ij")
+    return torch.stack((xx.reshape(-1), yy.reshape(-1), zz.reshape(-1)), dim=1)
+
+
+def create_parser():
+    """The pdb2mrc command line (p2m:138-151)."""
+    import argparse
+    p = argparse.ArgumentParser(description="Read text PDB and generate a 3-D MRC on the HIP device.")
+    p.add_argument("--i", required=True, help="Input PDB file")
+    p.add_argument("--o", required=True, help="Output MRC file")
+    p.add_argument("--box", type=int, default=256)
+    p.add_argument("--apix", type=float, default=1.0)
+    p.add_argument("--res", type=float, default=2.0)
+    p.add_argument("--center", action="store_true")
+    p.add_argument("--yflip", action="store_true")
+    p.add_argument("--includeHETATM", action="store_true")
+    p.add_argument("--gpuid", type=int, default=0)
+    p.add_argument("--device", help="HIP device, e.g. cuda:0; overrides --gpuid")
+    p.add_argument("--dontWriteMRC", action="store_true")
+    return p
+
+
+def run_pdb2mrc(args):
+    """generate_mrc with the parsed flags (p2m:154-162); the density array
+    with --dontWriteMRC, else 0 after writing --o."""
+    device = getattr(args, "device", None) or f"cuda:{int(args.gpuid)}"
+    return generate_mrc(args.i, None if args.dontWriteMRC else args.o, args.box, args.apix, args.res, device=device,
+                        do_center=args.center, do_yflip=args.yflip, include_hetatm=args.includeHETATM)
+
+
+def main(argv=None):
+    return run_pdb2mrc(create_parser().parse_args(argv))
+
+
+if __name__ == "__main__":
+    main()

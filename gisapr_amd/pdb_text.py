@@ -48,6 +48,23 @@ class PDBText:
     chains: np.ndarray          # [A] str
     atom_line_indices: np.ndarray
 
+    def write(self, output, coords=None) -> None:
+        """Every original line, with the x/y/z columns (31-54) of this object's
+        atom records replaced by ``coords`` in 8.3 format when given
+        (pdb_text.py:59-75)."""
+        lines = list(self.lines)
+        if coords is not None:
+            xyz = np.asarray(coords)
+            if xyz.shape != self.coords.shape:
+                raise ValueError("Replacement coordinates must have shape %s" % (self.coords.shape,))
+            if not np.all(np.isfinite(xyz)):
+                raise ValueError("PDB coordinates must be finite")
+            for li, row in zip(self.atom_line_indices, xyz):
+                ln = lines[int(li)]
+                lines[int(li)] = ln[:30] + "".join(coordinate_fields(row)) + ln[54:]
+        with open(output, "w", encoding="utf-8", errors="surrogateescape", newline="") as fh:
+            fh.writelines(lines)
+
 
 def read_pdb(path, include_hetatm=True, first_model_only=True) -> PDBText:
     """xyz, element and chain of ATOM (and HETATM) records, in file order."""
@@ -185,3 +202,30 @@ def move_chain(coords, chains, chain_id, pose, quantize=True) -> np.ndarray:
     moved = transform_coordinates(coords[sel], pose)
     coords[sel] = quantize_8_3(moved) if quantize else moved.astype(np.float32)
     return coords
+
+
+def write_transformed_chain(path, chain_id, pose, output, pivot=None) -> np.ndarray:
+    """The whole PDB with only the first-model chain ``chain_id`` moved by
+    ``pose``; every other line is kept byte for byte (pdb_text.py:153-162).
+    Returns the chain's new coordinates."""
+    pdb = read_pdb(path)
+    sel = pdb.chains == str(chain_id)
+    moved = transform_chain_coordinates(pdb, chain_id, pose, pivot=pivot)
+    PDBText(pdb.lines, pdb.coords[sel], pdb.elements[sel], pdb.chains[sel], pdb.atom_line_indices[sel]).write(
+        output, moved)
+    return moved
+
+
+def split_pdb_by_chain(input_pdb, chain_id, output_chain_file, output_rest_file) -> None:
+    """The first model's atom records of chain ``chain_id``, and all the others,
+    verbatim into two files, each ending with END (pdb_text.py:165-183)."""
+    pdb = read_pdb(input_pdb)
+    sel = pdb.chains == str(chain_id)
+    if not np.any(sel):
+        raise ValueError("Chain %r has no atoms in the first PDB model" % chain_id)
+    for output, mask in ((output_chain_file, sel), (output_rest_file, ~sel)):
+        with open(output, "w", encoding="utf-8", errors="surrogateescape", newline="") as fh:
+            for li in pdb.atom_line_indices[mask]:
+                ln = pdb.lines[int(li)]
+                fh.write(ln if ln.endswith(("\n", "\r")) else ln + "\n")
+            fh.write("END\n")

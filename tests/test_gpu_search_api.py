@@ -171,3 +171,19 @@ def test_calibration_bank_round_trip(hip_device, tmp_path):
     with pytest.raises(ValueError):  # a bank for other settings is refused
         search.run_search(parse(base[:-1] + ["3", "--o", str(tmp_path / "bad.txt"), "--orientations-in",
                                              str(tmp_path / "bank.json")]))
+
+
+def test_pdb2mrc_cli_matches_generate_volume(hip_device, tmp_path):
+    """main(--dontWriteMRC) returns generate_volume's density; with --o it
+    writes the same density as an MRC."""
+    from gisapr_amd import density, mrc, synthetic
+    print("[case] pdb2mrc CLI", flush=True)
+    d = synthetic.make_workload("box32")
+    (tmp_path / "m.pdb").write_text(synthetic.pdb_text(d["coords"], d["elements"], d["chains"]))
+    argv = ["--i", str(tmp_path / "m.pdb"), "--o", str(tmp_path / "m.mrc"), "--box", "32", "--apix", "3.0",
+            "--res", "6.0"]
+    arr = density.main(argv + ["--dontWriteMRC"])
+    want = density.generate_volume(str(tmp_path / "m.pdb"), 32, 3.0, 6.0, device=str(hip_device)).cpu().numpy()
+    np.testing.assert_allclose(arr, want, rtol=0, atol=1e-6 * np.abs(want).max())
+    assert density.main(argv) == 0
+    np.testing.assert_allclose(mrc.read(tmp_path / "m.mrc"), want, rtol=0, atol=1e-6 * np.abs(want).max())
