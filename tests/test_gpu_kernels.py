@@ -2,8 +2,8 @@
 
 Tolerances: the FP32 kernels are compared with the oracle run in float64
 (the exact value of the reference's algorithm); FP32 rounding of FFTs and
-sums gives relative errors ~1e-6, the bar is 1e-4 relative (BASELINE metric's
-"within 1e-4 relative"), with integer outputs (argmax, shifts) identical.
+sums gives relative errors ~1e-6, the bar is 1e-4 relative (the stated
+tolerance of the FP32 path), with integer outputs (argmax, shifts) identical.
 """
 import sys
 
