@@ -27,11 +27,13 @@ model, moving chain 4,500, 20,000 particles at 1.3 A/px), a swarm of 64
 candidates x 20 iterations with seeded poses, each member scoring its own
 2,000-20,000-particle subset, one graph-captured batched step per iteration.
 
-Serving leg (``--serve-requests``, 0 to skip): three resident datasets (box
+Serving legs (``--serve-requests``, 0 to skip): three resident datasets (box
 128 / 2,000, box 192 / 10,000, box 256 / 20,000 particles, 3,000-atom moving
 chain), a seeded Poisson stream of requests at ``--serve-rate``/s with 1-64
-candidates each (geometric, mean 8), continuous batching on the mixed path;
-p50/p99 request latency and throughput.
+candidates each (geometric, mean 8), continuous batching; p50/p99 request
+latency and throughput on the split-bf16 path (``serving``, the top-level
+p50/p99) and on the FP32 path (``serving_fp32``, ``--serve-requests-fp32``
+requests: the FP32 search is ~5x slower, so its stream is shorter).
 
 Prints ONE JSON line.  ``--gpus`` above 1 is not supported (one-GPU build).
 """
@@ -225,15 +227,17 @@ def pso_leg(args, dev, log):
 
 
 def serve_leg(args, dev, log):
-    """Serving leg: three resident datasets, seeded Poisson request stream."""
+    """Serving legs: three resident datasets, a seeded Poisson request stream per
+    path.  The datasets are built once; the FP32 path runs first because the
+    split-bf16 path replaces each dataset's spectra by their in-disk
+    coefficients.  Returns {path: stats}."""
     import torch
     from gisapr_amd import synthetic
     from gisapr_amd.batch import Autotuner, RigidBody
     from gisapr_amd.serve import SearchServer, poisson_requests
     t0 = time.perf_counter()
-    server = SearchServer(dev, path="mixed", slots=args.serve_slots, autotuner=Autotuner())
     names = ["serve128", "serve192", "serve256"]
-    sizes = {}
+    built = []
     for name in names:
         wl = synthetic.WORKLOADS[name]
         data = synthetic.make_workload(name, device=dev)
@@ -242,21 +246,37 @@ def serve_leg(args, dev, log):
         ps = eng.add_particles(data["images"], data["rot"], data["tilt"], data["psi"], defocus=dfo)
         del data["images"]
         body = RigidBody.from_arrays(data["coords"], data["numbers"], data["chains"], wl.moving_chain)
-        ds = server.add_dataset(name, eng, ps, body, wl.res)
-        sizes[name] = {"box": wl.box, "particles": ps.B, "resident_GiB": ds.resident_bytes / 2 ** 30}
-        log(f"serving dataset {name} resident ({ds.resident_bytes / 2**30:.1f} GiB)")
+        built.append((name, wl, eng, ps, body))
+        log(f"serving dataset {name} built")
     setup = time.perf_counter() - t0
-    # warm every executor once (library set-up, autotuning) before the stream
-    for name in names:
-        for ex in server.datasets[name].executors:
-            ex.tune(np.zeros((server.slots, 6)))
-            ex.evaluate(np.zeros((1, 6)))
-    torch.cuda.synchronize()
-    reqs = poisson_requests(args.serve_requests, args.serve_rate, names, seed=7)
-    stats = server.serve(reqs)
-    stats.update(datasets=sizes, setup_s=setup, rate_per_s=args.serve_rate, slots=server.slots, path="mixed",
-                 offered_candidates_per_s=args.serve_rate * float(np.mean([r.size for r in reqs])))
-    return stats
+    requests = {"fp32": args.serve_requests_fp32, "mixed": args.serve_requests}
+    out = {}
+    for path in ("fp32", "mixed"):
+        if requests[path] <= 0 or path not in args.serve_paths.split(","):
+            continue
+        t1 = time.perf_counter()
+        server = SearchServer(dev, path=path, slots=args.serve_slots, autotuner=Autotuner())
+        sizes = {}
+        for name, wl, eng, ps, body in built:
+            ds = server.add_dataset(name, eng, ps, body, wl.res)
+            sizes[name] = {"box": wl.box, "particles": ps.B, "resident_GiB": ds.resident_bytes / 2 ** 30}
+            log(f"serving dataset {name} resident on the {path} path ({ds.resident_bytes / 2**30:.1f} GiB)")
+        # warm every executor once (library set-up, autotuning) before the stream
+        for name in names:
+            for ex in server.datasets[name].executors:
+                ex.tune(np.zeros((server.slots, 6)))
+                ex.evaluate(np.zeros((1, 6)))
+        torch.cuda.synchronize()
+        reqs = poisson_requests(requests[path], args.serve_rate, names, seed=7)
+        stats = server.serve(reqs)
+        stats.update(datasets=sizes, setup_s=setup + time.perf_counter() - t1 - stats["wall_s"],
+                     rate_per_s=args.serve_rate, slots=server.slots, path=path,
+                     offered_candidates_per_s=args.serve_rate * float(np.mean([r.size for r in reqs])))
+        out[path] = stats
+        log(f"serving {path}: p50 {stats['p50_ms']:.0f} ms, p99 {stats['p99_ms']:.0f} ms")
+        del server
+        torch.cuda.empty_cache()
+    return out
 
 
 def main() -> int:
@@ -274,7 +294,10 @@ def main() -> int:
     ap.add_argument("--pso-iters", type=int, default=20, help="batched PSO leg iterations (0 to skip)")
     ap.add_argument("--pso-slots", type=int, default=64)
     ap.add_argument("--pso-particles", type=int, default=20000)
-    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-requests", type=int, default=300,
+                    help="split-bf16 serving stream (0 skips the serving legs)")
+    ap.add_argument("--serve-requests-fp32", type=int, default=40, help="FP32 serving stream (0 to skip it)")
+    ap.add_argument("--serve-paths", default="fp32,mixed")
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -477,9 +500,13 @@ def main() -> int:
         out["pso"] = pso_leg(args, dev, log)
         torch.cuda.empty_cache()
     if args.serve_requests > 0:
-        out["serving"] = serve_leg(args, dev, log)
-        out["p50_ms"] = out["serving"]["p50_ms"]
-        out["p99_ms"] = out["serving"]["p99_ms"]
+        served = serve_leg(args, dev, log)
+        if "mixed" in served:
+            out["serving"] = served["mixed"]
+            out["p50_ms"] = served["mixed"]["p50_ms"]
+            out["p99_ms"] = served["mixed"]["p99_ms"]
+        if "fp32" in served:
+            out["serving_fp32"] = served["fp32"]
     print(json.dumps(out), flush=True)
     return 0
 

@@ -48,13 +48,14 @@ def test_poisson_requests_are_seeded_and_bounded():
 
 
 @pytest.mark.gpu
-def test_served_results_equal_alone(hip_device):
+@pytest.mark.parametrize("path", ["mixed", "fp32"])
+def test_served_results_equal_alone(hip_device, path):
     from gisapr_amd import synthetic
     from gisapr_amd.batch import CandidateBatch, RigidBody
     from gisapr_amd.engine import SearchEngine
     from gisapr_amd.serve import SearchServer
-    print("[case] serve mix vs alone", flush=True)
-    server = SearchServer(hip_device, path="mixed", slots=4)
+    print(f"[case] serve mix vs alone ({path})", flush=True)
+    server = SearchServer(hip_device, path=path, slots=4)
     parts = {}
     for name in ("box32", "box64"):
         wl = synthetic.WORKLOADS[name]
@@ -75,7 +76,7 @@ def test_served_results_equal_alone(hip_device):
     assert stats["requests"] == len(reqs) and stats["p99_ms"] >= stats["p50_ms"] > 0
     for r in reqs:
         eng, ps, body, res = parts[r.dataset]
-        alone = CandidateBatch(eng, ps, body, res, slots=r.size, path="mixed").evaluate(r.poses)
+        alone = CandidateBatch(eng, ps, body, res, slots=r.size, path=path).evaluate(r.poses)
         np.testing.assert_allclose(r.scores, alone, rtol=1e-5, err_msg=f"request {r.rid}")
     with pytest.raises(MemoryError):
         SearchServer(hip_device, hbm_budget_bytes=1 << 20, slots=2).add_dataset("box64", *parts["box64"])
