@@ -202,7 +202,7 @@ def test_correlate_matches_oracle(hip_device, n, C):
     from gisapr_amd import kernels
     rng = np.random.default_rng(n * 31 + C)
     m = n // 2 + 1
-    M, B, P = 7, 3, 5
+    M, B, P = 7, 4, 5
 
     def rand_spec(*shape):
         x = rng.standard_normal(shape + (n, m)) + 1j * rng.standard_normal(shape + (n, m))
@@ -211,7 +211,7 @@ def test_correlate_matches_oracle(hip_device, n, C):
     bank_pre = rand_spec(M)
     # particles correlated with a template shifted by a known lag, plus noise
     parts_pre = rand_spec(B, P) * 0.3
-    local = [[0, 2, 5], [1], [3, 4, 5, 6]]
+    local = [[0, 2, 5], [], [1], [3, 4, 5, 6]]  # particle 1 has no local direction (no record, v606:1467-1468)
     off = np.zeros(B + 1, np.int32)
     off[1:] = np.cumsum([len(l) for l in local])
     idx = np.asarray([k for l in local for k in l], np.int32)
@@ -225,6 +225,9 @@ def test_correlate_matches_oracle(hip_device, n, C):
     py = c.peak_y.cpu().numpy()
     px = c.peak_x.cpu().numpy()
     for b in range(B):
+        if not local[b]:
+            assert int(c.best[b]) == -1
+            continue
         s, y, x = ofr.correlate(bank_pre[local[b]], parts_pre[b], C, np.float64)
         g = slice(off[b], off[b + 1])
         np.testing.assert_allclose(sc[g], s, rtol=1e-4, atol=1e-4 * np.abs(s).max())
@@ -232,6 +235,10 @@ def test_correlate_matches_oracle(hip_device, n, C):
         assert int(c.best[b]) == int(np.argmax(s.reshape(-1)))
         np.testing.assert_allclose(float(c.mean[b]), s.mean(), rtol=1e-4, atol=1e-6)
         np.testing.assert_allclose(float(c.std[b]), s.std(), rtol=1e-4, atol=1e-6)
+    # the conformation score skips the particle without a record
+    pk = c.peak.cpu().numpy().astype(np.float64)
+    want = sum(np.exp(pk[b] / 1024.0) for b in range(B) if local[b])
+    np.testing.assert_allclose(float(kernels.score_sum(c.peak, c.best)), want, rtol=1e-12)
 
 
 @pytest.mark.parametrize("name", ["box32", "box64", "box96"])
