@@ -146,6 +146,40 @@ def precision_check(eng, ps, tk, res, subset: int, seed: int):
             "max_rel_err_fp64": worst, "reference": "same functional in float64 (torch complex128)"}
 
 
+def fp16_variant(eng, ps, tk, res_bf16, subset: int, seed: int, repeats: int = 5):
+    """The split-fp16 operand variant of the small-shift search on the same
+    template and particle coefficients: search time (HIP events on the launch
+    stream, mean of ``repeats`` launches after one warm-up, beside the
+    split-bf16 search timed the same way), scores/s of the search stage, top-1
+    against the split-bf16 search and, on the seeded subset, against the
+    float64 functional (precision_check)."""
+    import torch
+
+    def timed():
+        res = eng.search_small_shift(None, ps, tk=tk)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(repeats):
+            eng.search_small_shift(None, ps, tk=tk)
+        e1.record()
+        torch.cuda.synchronize()
+        return res, e0.elapsed_time(e1) / repeats
+
+    _, ms_bf16 = timed()
+    eng.set_small_shift_operand("fp16")
+    try:
+        res_, ms = timed()
+        out = {"operand": "fp16 x3 (split hi+lo, A scaled 2^14, fp32 accumulate)", "search_ms": ms,
+               "bf16_search_ms": ms_bf16, "search_scores_per_s": ps.n_scores / (ms * 1e-3),
+               "top1_same_as_bf16": float(np.mean(res_.corr.best.cpu().numpy() == res_bf16.corr.best.cpu().numpy()))}
+        if subset > 0:
+            out["precision"] = precision_check(eng, ps, tk, res_, subset, seed)
+        return out
+    finally:
+        eng.set_small_shift_operand("bf16")
+
+
 def profiled_traffic(kernel: str, workload: str):
     """HBM bytes per launch of ``kernel`` on ``workload`` from the committed
     rocprofv3 PMC summary profiles/<workload>_<kernel>_traffic.md (its
@@ -289,6 +323,8 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=8, help="candidates per batched step")
     ap.add_argument("--resolution", type=float, default=4.0, help="small-shift disk radius in A")
     ap.add_argument("--precision-subset", type=int, default=256)
+    ap.add_argument("--no-fp16-variant", dest="fp16_variant", action="store_false",
+                    help="skip the split-fp16 operand variant of the small-shift search")
     ap.add_argument("--cpu-sample", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pso-iters", type=int, default=20, help="batched PSO leg iterations (0 to skip)")
@@ -491,6 +527,9 @@ def main() -> int:
             "particles": int(valid.sum())}
     if args.precision_subset > 0 and "mixed" in results:
         out["precision"] = precision_check(eng, ps, tk_last, results["mixed"], args.precision_subset,
+                                           synthetic.SEED_CAND)
+    if args.fp16_variant and "mixed" in results:
+        out["fp16_variant"] = fp16_variant(eng, ps, tk_last, results["mixed"], args.precision_subset,
                                            synthetic.SEED_CAND)
     if not args.no_cpu_baseline and args.cpu_sample > 0 and bank is not None:
         out["cpu_baseline"] = cpu_baseline(eng, ps, bank, args.cpu_sample)

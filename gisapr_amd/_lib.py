@@ -54,6 +54,8 @@ SIGNATURES = {
     "gis_gather_coefficients": (_i, [_vp, _sz, _i, _vp, _i, _i, _vp, _vp, _vp]),
     "gis_correlate_mixed": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _vp,
                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gis_correlate_mixed_ex": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _vp,
+                                    _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
 }
 
 

@@ -343,7 +343,8 @@ class CandidateBatch:
             return kernels.correlate_mixed(self.tk, ps.coeffs, ps.P, dir_off, dir_idx, eng.small_bhi,
                                            eng.small_blo, eng.small.half, validate=False, pmap=pmap,
                                            pt=tile.get("pt", 0), wpb=tile.get("wpb", 0), plast=eng.small_plast,
-                                           rows=self.plan.particle_major(self.slots if k is None else k))
+                                           rows=self.plan.particle_major(self.slots if k is None else k),
+                                           operand=eng.small.operand)
         return kernels.correlate(self.bank, ps.spectra, ps.inv_norm, dir_off, dir_idx, eng.crop,
                                  validate=False, pmap=pmap, jt=tile.get("jt", 0), tinv=self.tinv)
 
@@ -376,7 +377,8 @@ class CandidateBatch:
         self._all_templates(self.slots)
         ps, pl = self.ps, self.plan
         if self.path == "mixed":
-            key = Autotuner.key("mixed", self.engine.n, ps.P, self.engine.small.Kpad, _bucket(int(pl.dir_idx.numel())))
+            key = Autotuner.key("mixed" if self.engine.small.operand == "bf16" else "mixed_fp16", self.engine.n,
+                                ps.P, self.engine.small.Kpad, _bucket(int(pl.dir_idx.numel())))
         else:
             key = Autotuner.key("fp32", self.engine.n, ps.P, _bucket(int(pl.dir_idx.numel())))
         self.tile = self.autotuner.choose(key, _variants_for(self.path, ps.P), self._search)
@@ -537,7 +539,8 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
             engine.small_templates(vol, out=tk[c * M:(c + 1) * M])
         corr = kernels.correlate_mixed(tk, ps.coeffs, ps.P, plan.dir_off, plan.dir_idx, engine.small_bhi,
                                        engine.small_blo, engine.small.half, validate=False, pmap=plan.pmap,
-                                       pt=tile.get("pt", 0), wpb=tile.get("wpb", 0), plast=engine.small_plast)
+                                       pt=tile.get("pt", 0), wpb=tile.get("wpb", 0), plast=engine.small_plast,
+                                       operand=engine.small.operand)
     else:
         if ps.spectra is None:
             engine.prepare(ps)

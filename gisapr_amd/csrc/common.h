@@ -23,6 +23,9 @@ enum {
   GIS_EUNSUPPORTED = -2 // box size / crop size outside the compiled set
 };
 
+// 16-bit operand type of the split small-shift search.
+enum { GIS_OPERAND_BF16 = 0, GIS_OPERAND_F16 = 1 };
+
 void gis_set_error(const char* fmt, ...);
 
 #define GIS_REQUIRE(cond, ...)              \

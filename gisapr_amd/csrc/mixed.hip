@@ -1,4 +1,4 @@
-// Small-shift correlation on split-BF16 MFMA (precision variant).
+// Small-shift correlation on split-BF16 (or split-FP16) MFMA (precision variant).
 //
 // The v606 score at an integer lag l = (ly, lx) is a fixed linear functional
 // of the packed product Q = T * P (correlate_templates_and_particle,
@@ -9,7 +9,11 @@
 //     A[p][2k] = Re Q_pk, A[p][2k+1] = Im Q_pk;  B[2k][l] = Re Phi, B[2k+1][l] = -Im Phi
 // evaluated on v_mfma_f32_16x16x32_bf16 with both operands split into BF16
 // hi + lo parts (hi*hi + hi*lo + lo*hi, FP32 accumulation): ~16 significant
-// bits per product instead of BF16's 8.
+// bits per product instead of BF16's 8.  The FP16 variant
+// (v_mfma_f32_16x16x32_f16, same operand layout) splits into FP16 hi + lo,
+// ~22 bits per product; FP16's range needs A scaled by 2^14 before the split
+// (|A| <= 1 for unit-norm spectra, so |2^14 A| < 65504 and typical products
+// stay normal) and the accumulator scaled back by 2^-14, both exact.
 //
 // gfx950 mapping: one wave per (particle, tile of 16 direction x psi pairs).
 // Each lane forms 4 complex Q values per k-step from the FP32 template and
@@ -23,6 +27,8 @@
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -50,21 +56,46 @@ gather_coeffs_kernel(const float2* __restrict__ src, size_t plane, int rows, con
   }
 }
 
-// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi), both round to nearest
-// even; pairs of values go through one packed conversion each way.
-__device__ __forceinline__ void split8(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
+// x = hi + lo with hi = h(s x) and lo = h(s x - hi), both round to nearest
+// even (h = the 16-bit operand type, s = its A scale, a power of two); pairs
+// of values go through one packed conversion each way.
+template <typename V8, typename V2, int SCALE_EXP>
+__device__ __forceinline__ void split8(const float (&x)[8], V8& hi, V8& lo) {
+  constexpr float s = (float)(1 << SCALE_EXP);
 #pragma unroll
   for (int e = 0; e < 8; e += 2) {
-    const f32x2 v = {x[e], x[e + 1]};
-    const bf16x2 h = __builtin_convertvector(v, bf16x2);
+    const f32x2 v = {x[e] * s, x[e + 1] * s};
+    const V2 h = __builtin_convertvector(v, V2);
     const f32x2 hf = __builtin_convertvector(h, f32x2);
-    const bf16x2 l = __builtin_convertvector(v - hf, bf16x2);
+    const V2 l = __builtin_convertvector(v - hf, V2);
     hi[e] = h[0];
     hi[e + 1] = h[1];
     lo[e] = l[0];
     lo[e + 1] = l[1];
   }
 }
+
+// Operand types of the split product: 8-element fragments, the MFMA, the A
+// scale exponent and the factor from the accumulator to the v606 score
+// (1024 x 2^-scale).
+struct OpBF16 {
+  typedef bf16x8 v8;
+  typedef bf16x2 v2;
+  static constexpr int kScaleExp = 0;
+  static constexpr float kOut = 1024.0f;
+  static __device__ __forceinline__ floatx4 mfma(v8 a, v8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+struct OpF16 {
+  typedef f16x8 v8;
+  typedef f16x2 v2;
+  static constexpr int kScaleExp = 14;
+  static constexpr float kOut = 1024.0f / 16384.0f;
+  static __device__ __forceinline__ floatx4 mfma(v8 a, v8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
 
 // One wave per (group of 16 direction rows, chunk of PT psi values).  Row r
 // is the r-th entry of dir_idx (particle b with dir_off[b] <= r <
@@ -75,19 +106,20 @@ __device__ __forceinline__ void split8(const float (&x)[8], bf16x8& hi, bf16x8& 
 // of a block (register-staged, double-buffered).  NT lag tiles of 16 go to
 // the MFMAs; with VL the last lag (49 = 3 x 16 + 1) is a float32 dot product
 // on the VALU instead of a fourth, almost empty, tile.
-template <int PT, int WPB, int NT, bool VL>
+template <typename OP, int PT, int WPB, int NT, bool VL>
 __global__ void __launch_bounds__(64 * WPB)
 mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__ Pk,
                        const int* __restrict__ dir_off, const int* __restrict__ dir_idx,
                        const int* __restrict__ pmap, const int* __restrict__ rows, int B, int nrows,
-                       int P, int Kpad, const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo,
-                       const float2* __restrict__ Plast, int half,
+                       int P, int Kpad, const typename OP::v8* __restrict__ Bhi,
+                       const typename OP::v8* __restrict__ Blo, const float2* __restrict__ Plast, int half,
                        float* __restrict__ scores, int* __restrict__ peak_y, int* __restrict__ peak_x) {
   constexpr int KC = 4;               // k-steps per LDS chunk
   constexpr int FRAG = NT * 2 * 64;   // lag-table fragments per k-step (hi and lo)
   constexpr int NTH = 64 * WPB;
   constexpr int SPT = (KC * FRAG + NTH - 1) / NTH;  // staged fragments per thread
-  __shared__ bf16x8 blds[2][KC * FRAG];
+  typedef typename OP::v8 h8;
+  __shared__ h8 blds[2][KC * FRAG];
   const int tid = threadIdx.x, lane = tid & 63;
   int group = blockIdx.x * WPB + (tid >> 6);
   const bool live = group * 16 < nrows;
@@ -119,7 +151,7 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
     for (int nt = 0; nt < NT; ++nt) acc[p][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
   }
 
-  bf16x8 stg[SPT];
+  h8 stg[SPT];
   auto fetch = [&](int ch) {
 #pragma unroll
     for (int s = 0; s < SPT; ++s) {
@@ -127,7 +159,7 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
       const int kk = e / FRAG, f = e - kk * FRAG;
       const int ks = ch * KC + kk;
       const int nt = f >> 7, h = (f >> 6) & 1, ln = f & 63;
-      bf16x8 v = {};
+      h8 v = {};
       if (e < KC * FRAG && ks < ksteps) v = (h ? Blo : Bhi)[((size_t)nt * ksteps + ks) * 64 + ln];
       stg[s] = v;
     }
@@ -189,7 +221,7 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
 #pragma unroll
       for (int p = 0; p < PT; ++p) { uc[p][0] = now.u[p][0]; uc[p][1] = now.u[p][1]; }
       const float tr[4] = {t0.x, t0.z, t1.x, t1.z}, ti[4] = {t0.y, t0.w, t1.y, t1.w};
-      bf16x8 ahi[PT], alo[PT];
+      h8 ahi[PT], alo[PT];
 #pragma unroll
       for (int p = 0; p < PT; ++p) {
         const float4 u0 = uc[p][0], u1 = uc[p][1];
@@ -203,17 +235,17 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
         if (VL)
           accl[p] += ((x[0] * f0.x + x[1] * f0.y) + (x[2] * f0.z + x[3] * f0.w)) +
                      ((x[4] * f1.x + x[5] * f1.y) + (x[6] * f1.z + x[7] * f1.w));
-        split8(x, ahi[p], alo[p]);
+        split8<h8, typename OP::v2, OP::kScaleExp>(x, ahi[p], alo[p]);
       }
-      const bf16x8* fr = blds[cur] + kk * FRAG + lane;
+      const h8* fr = blds[cur] + kk * FRAG + lane;
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
-        const bf16x8 bh = fr[(nt * 2) * 64], bl = fr[(nt * 2 + 1) * 64];
+        const h8 bh = fr[(nt * 2) * 64], bl = fr[(nt * 2 + 1) * 64];
 #pragma unroll
         for (int p = 0; p < PT; ++p) {
-          acc[p][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[p], bh, acc[p][nt], 0, 0, 0);
-          acc[p][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[p], bl, acc[p][nt], 0, 0, 0);
-          acc[p][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[p], bh, acc[p][nt], 0, 0, 0);
+          acc[p][nt] = OP::mfma(ahi[p], bh, acc[p][nt]);
+          acc[p][nt] = OP::mfma(ahi[p], bl, acc[p][nt]);
+          acc[p][nt] = OP::mfma(alo[p], bh, acc[p][nt]);
         }
       }
     }
@@ -239,7 +271,7 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
       for (int nt = 0; nt < NT; ++nt) {
         const int s = nt * 16 + i;
         if (s < (VL ? nlag - 1 : nlag)) {
-          const float val = acc[p][nt][v] * 1024.0f;
+          const float val = acc[p][nt][v] * OP::kOut;
           if (val > bv) { bv = val; bs = s; }  // s increases with nt: strict > keeps the first
         }
       }
@@ -320,6 +352,45 @@ GIS_API int gis_gather_coefficients(const void* src, size_t plane, int rows, con
   return GIS_OK;
 }
 
+namespace {
+
+// Launches the tile variant (pt, wpb, lag tiles) of the search for operand type OP.
+template <typename OP>
+void launch_mixed(dim3 grid, int pt, int wpb, int nt, bool vl, hipStream_t st, const void* Tk, const void* Pk,
+                  const int* dir_off, const int* dir_idx, const int* pmap, const int* rows, int B, int nrows, int P,
+                  int Kpad, const void* Bhi, const void* Blo, const float2* pl, int half, float* scores,
+                  int* peak_y, int* peak_x) {
+  typedef const typename OP::v8* BP;
+#define GIS_MIXED_LAUNCH(PTV, WV, NTV, VLV)                                                                   \
+  hipLaunchKernelGGL((mixed_correlate_kernel<OP, PTV, WV, NTV, VLV>), grid, dim3(64 * WV), 0, st,              \
+                     (const float2*)Tk, (const float2*)Pk, dir_off, dir_idx, pmap, rows, B, nrows, P, Kpad,    \
+                     (BP)Bhi, (BP)Blo, pl, half, scores, peak_y, peak_x)
+#define GIS_MIXED_NT(PTV, WV)                                 \
+  if (vl && nt == 3) GIS_MIXED_LAUNCH(PTV, WV, 3, true);      \
+  else if (nt == 1) GIS_MIXED_LAUNCH(PTV, WV, 1, false);      \
+  else if (nt == 2) GIS_MIXED_LAUNCH(PTV, WV, 2, false);      \
+  else if (nt == 3) GIS_MIXED_LAUNCH(PTV, WV, 3, false);      \
+  else GIS_MIXED_LAUNCH(PTV, WV, 4, false);
+#define GIS_MIXED_PT(WV)                     \
+  switch (pt) {                              \
+    case 1: { GIS_MIXED_NT(1, WV) } break;   \
+    case 2: { GIS_MIXED_NT(2, WV) } break;   \
+    case 3: { GIS_MIXED_NT(3, WV) } break;   \
+    case 4: { GIS_MIXED_NT(4, WV) } break;   \
+    default: { GIS_MIXED_NT(5, WV) } break;  \
+  }
+  if (wpb == 2) {
+    GIS_MIXED_PT(2)
+  } else {
+    GIS_MIXED_PT(4)
+  }
+#undef GIS_MIXED_PT
+#undef GIS_MIXED_NT
+#undef GIS_MIXED_LAUNCH
+}
+
+}  // namespace
+
 // Precision variant of correlate_templates_and_particle (v606:1418-1449)
 // over lags [-half, half]^2 (half <= 3) and a truncated coefficient set.
 //   Tk [bank rows][Kpad], Pk [particles*P][Kpad] complex64 (unit-norm, unshifted)
@@ -329,18 +400,22 @@ GIS_API int gis_gather_coefficients(const void* src, size_t plane, int rows, con
 //        cache (several candidates' rows of one particle in a batched launch)
 //   nrows = dir_off[B] = number of dir_idx entries (passed so no device read is needed)
 //   pt = psi values per wave (1..5, 0 = min(P, 5)); wpb = waves per block (2 or 4; 0 = 4)
-//   Bhi, Blo: split lag table, [4][Kpad/16][64] x 8 bf16 (host layout)
+//   Bhi, Blo: split lag table, [4][Kpad/16][64] x 8 16-bit values (host layout)
 //   Plast: float2 [Kpad] (Re Phi, -Im Phi) of the last lag, used when the lag
 //          count is 16 t + 1 (t tiles on MFMA, the last lag on the VALU); else NULL
 //   scores / peak_y / peak_x [dir_off[B]][P] (peak indices are lag + half)
 //   best / peak / mean / std [B] as gis_correlate
-GIS_API int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx,
-                                const int* pmap, const int* rows, int B, int P, int nrows, int Kpad, const void* Bhi, const void* Blo,
-                                const void* Plast, int half, int pt, int wpb, float* scores, int* peak_y,
-                                int* peak_x, int* best,
-                                float* peak, double* mean, double* stdv, void* stream) {
+//   operand: GIS_OPERAND_BF16 (Bhi / Blo are bf16) or GIS_OPERAND_F16 (fp16)
+GIS_API int gis_correlate_mixed_ex(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx,
+                                   const int* pmap, const int* rows, int B, int P, int nrows, int Kpad,
+                                   const void* Bhi, const void* Blo, const void* Plast, int half, int pt, int wpb,
+                                   float* scores, int* peak_y, int* peak_x, int* best, float* peak, double* mean,
+                                   double* stdv, int operand, void* stream) {
   GIS_REQUIRE(B >= 0 && P >= 1 && nrows >= 0, "gis_correlate_mixed: B=%d P=%d nrows=%d", B, P, nrows);
   GIS_REQUIRE(Kpad % 16 == 0 && Kpad >= 16, "gis_correlate_mixed: Kpad %d must be a positive multiple of 16", Kpad);
+  GIS_REQUIRE(operand == GIS_OPERAND_BF16 || operand == GIS_OPERAND_F16,
+              "gis_correlate_mixed: operand %d not bf16 (%d) or fp16 (%d)", operand, GIS_OPERAND_BF16,
+              GIS_OPERAND_F16);
   if (half < 0 || (2 * half + 1) * (2 * half + 1) > 16 * kLagTiles) {
     gis_set_error("gis_correlate_mixed: half %d gives more than %d lags", half, 16 * kLagTiles);
     return GIS_EUNSUPPORTED;
@@ -362,35 +437,24 @@ GIS_API int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_o
     const int rows_per_block = 16 * wpb;
     const dim3 grid((unsigned)((nrows + rows_per_block - 1) / rows_per_block), (unsigned)((P + pt - 1) / pt));
     const float2* pl = (const float2*)Plast;
-#define GIS_MIXED_LAUNCH(PTV, WV, NTV, VLV)                                                                   \
-  hipLaunchKernelGGL((mixed_correlate_kernel<PTV, WV, NTV, VLV>), grid, dim3(64 * WV), 0, st,                  \
-                     (const float2*)Tk, (const float2*)Pk, dir_off, dir_idx, pmap, rows, B, nrows, P, Kpad,            \
-                     (const bf16x8*)Bhi, (const bf16x8*)Blo, pl, half, scores, peak_y, peak_x)
-#define GIS_MIXED_NT(PTV, WV)                                 \
-  if (vl && nt == 3) GIS_MIXED_LAUNCH(PTV, WV, 3, true);      \
-  else if (nt == 1) GIS_MIXED_LAUNCH(PTV, WV, 1, false);      \
-  else if (nt == 2) GIS_MIXED_LAUNCH(PTV, WV, 2, false);      \
-  else if (nt == 3) GIS_MIXED_LAUNCH(PTV, WV, 3, false);      \
-  else GIS_MIXED_LAUNCH(PTV, WV, 4, false);
-#define GIS_MIXED_PT(WV)                     \
-  switch (pt) {                              \
-    case 1: { GIS_MIXED_NT(1, WV) } break;   \
-    case 2: { GIS_MIXED_NT(2, WV) } break;   \
-    case 3: { GIS_MIXED_NT(3, WV) } break;   \
-    case 4: { GIS_MIXED_NT(4, WV) } break;   \
-    default: { GIS_MIXED_NT(5, WV) } break;  \
-  }
-    if (wpb == 2) {
-      GIS_MIXED_PT(2)
-    } else {
-      GIS_MIXED_PT(4)
-    }
-#undef GIS_MIXED_PT
-#undef GIS_MIXED_NT
-#undef GIS_MIXED_LAUNCH
+    if (operand == GIS_OPERAND_F16)
+      launch_mixed<OpF16>(grid, pt, wpb, nt, vl, st, Tk, Pk, dir_off, dir_idx, pmap, rows, B, nrows, P, Kpad, Bhi,
+                          Blo, pl, half, scores, peak_y, peak_x);
+    else
+      launch_mixed<OpBF16>(grid, pt, wpb, nt, vl, st, Tk, Pk, dir_off, dir_idx, pmap, rows, B, nrows, P, Kpad, Bhi,
+                           Blo, pl, half, scores, peak_y, peak_x);
     GIS_LAUNCH_CHECK("gis_correlate_mixed");
   }
   hipLaunchKernelGGL(particle_tail_kernel, dim3(B), dim3(64), 0, st, dir_off, P, scores, best, peak, mean, stdv);
   GIS_LAUNCH_CHECK("gis_correlate_mixed tail");
   return GIS_OK;
+}
+
+GIS_API int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx,
+                                const int* pmap, const int* rows, int B, int P, int nrows, int Kpad, const void* Bhi,
+                                const void* Blo, const void* Plast, int half, int pt, int wpb, float* scores,
+                                int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv,
+                                void* stream) {
+  return gis_correlate_mixed_ex(Tk, Pk, dir_off, dir_idx, pmap, rows, B, P, nrows, Kpad, Bhi, Blo, Plast, half, pt,
+                                wpb, scores, peak_y, peak_x, best, peak, mean, stdv, GIS_OPERAND_BF16, stream);
 }

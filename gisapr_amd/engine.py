@@ -190,11 +190,22 @@ class SearchEngine:
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.crop // 2)
 
     # ---------------------------------------------------------------- small-shift precision search
-    def enable_small_shift(self, resolution: float = 4.0, half: int = 3) -> host.SmallShiftTables:
-        """Set up the split-bf16 search over lags [-half, half]^2 and the
-        coefficients with |k| <= N apix / resolution (capped at n/2)."""
+    def enable_small_shift(self, resolution: float = 4.0, half: int = 3,
+                           operand: str = "bf16") -> host.SmallShiftTables:
+        """Set up the split-bf16 (``operand="fp16"``: split-fp16) search over
+        lags [-half, half]^2 and the coefficients with |k| <= N apix /
+        resolution (capped at n/2)."""
         radius = min(self.N * self.apix / float(resolution), self.n / 2.0)
-        tab = host.SmallShiftTables(self.n, radius, half)
+        return self._install_small(host.SmallShiftTables(self.n, radius, half, operand))
+
+    def set_small_shift_operand(self, operand: str) -> host.SmallShiftTables:
+        """Switch the small-shift search between split-bf16 and split-fp16
+        operands, keeping its coefficient set and lags (prepared particle
+        coefficients stay valid)."""
+        self._require_small()
+        return self._install_small(host.SmallShiftTables(self.n, self.small.radius, self.small.half, operand))
+
+    def _install_small(self, tab: host.SmallShiftTables) -> host.SmallShiftTables:
         dev = self.device
         self.small = tab
         self.small_kidx = torch.from_numpy(tab.kidx).to(dev)
@@ -254,7 +265,7 @@ class SearchEngine:
                 self.device)
         corr = kernels.correlate_mixed(tk, ps.coeffs, ps.P, ps.dir_off, ps.dir_idx, self.small_bhi,
                                        self.small_blo, self.small.half, validate=False, plast=self.small_plast,
-                                       rows=ps.row_order)
+                                       rows=ps.row_order, operand=self.small.operand)
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.small.half)
 
     def score_volume(self, volume: torch.Tensor, ps: ParticleSet) -> SearchResult:

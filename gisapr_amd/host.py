@@ -227,17 +227,23 @@ class SmallShiftTables:
     Only coefficients with |k| <= radius (frequency fy = r or r - n, fx = c,
     with the packed Nyquist column at fx = -n/2) are kept and the lags run
     over [-half, half]^2, row-major.  The real GEMM operand B[2k][s] =
-    Re Phi, B[2k+1][s] = -Im Phi is split into bf16 hi + lo and stored in the
-    v_mfma_f32_16x16x32_bf16 B-operand order [lag tile][k-step][lane][8]:
-    lane l of k-step ks holds rows 32 ks + 8 (l / 16) + 0..7 of column
-    16 tile + l % 16.
+    Re Phi, B[2k+1][s] = -Im Phi is split into 16-bit hi + lo (``operand``
+    "bf16" or "fp16", both round to nearest even; |B| <= 2 needs no scaling
+    in either) and stored in the v_mfma_f32_16x16x32_{bf16,f16} B-operand
+    order [lag tile][k-step][lane][8]: lane l of k-step ks holds rows
+    32 ks + 8 (l / 16) + 0..7 of column 16 tile + l % 16.
     """
 
     LAG_TILES = 4
 
-    def __init__(self, n: int, radius: float, half: int = 3):
+    OPERANDS = {"bf16": 0, "fp16": 1}
+
+    def __init__(self, n: int, radius: float, half: int = 3, operand: str = "bf16"):
         n = int(n)
         half = int(half)
+        if operand not in self.OPERANDS:
+            raise ValueError(f"operand {operand!r} not in {sorted(self.OPERANDS)}")
+        self.operand = operand
         side = 2 * half + 1
         if side * side > 16 * self.LAG_TILES:
             raise ValueError(f"half {half}: at most {16 * self.LAG_TILES} lags")
@@ -274,7 +280,7 @@ class SmallShiftTables:
         self.phi_last = np.zeros((self.Kpad, 2), np.float32)
         self.phi_last[:self.K, 0] = phi[:, S - 1].real
         self.phi_last[:self.K, 1] = -phi[:, S - 1].imag
-        hi, lo = _split_bf16(breal)
+        hi, lo = _split_bf16(breal) if operand == "bf16" else _split_f16(breal)
         self.b_hi = _mfma_b_layout(hi, self.Kpad, self.LAG_TILES)
         self.b_lo = _mfma_b_layout(lo, self.Kpad, self.LAG_TILES)
 
@@ -298,6 +304,13 @@ def _split_bf16(x: np.ndarray):
     hi = _bf16_bits(x.astype(np.float32))
     lo = _bf16_bits((x - _bf16_value(hi)).astype(np.float32))
     return hi, lo
+
+
+def _split_f16(x: np.ndarray):
+    """x = hi + lo in IEEE half, each rounded to nearest even (bit patterns)."""
+    hi = x.astype(np.float16)
+    lo = (x - hi.astype(np.float64)).astype(np.float16)
+    return hi.view(np.uint16), lo.view(np.uint16)
 
 
 def _mfma_b_layout(bits: np.ndarray, kpad: int, tiles: int) -> np.ndarray:

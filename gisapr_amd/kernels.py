@@ -497,10 +497,15 @@ def gather_coefficients(src: torch.Tensor, kidx: torch.Tensor, kpad: int,
 def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.Tensor, dir_idx: torch.Tensor,
                     b_hi: torch.Tensor, b_lo: torch.Tensor, half: int, validate: bool = True,
                     pmap: Optional[torch.Tensor] = None, pt: int = 0, wpb: int = 0,
-                    plast: Optional[torch.Tensor] = None, rows: Optional[torch.Tensor] = None) -> CorrelationResult:
-    """Split-bf16 small-shift search: best lag over [-half, half]^2 of every
-    (direction, psi) pair; peak_y / peak_x are lag + half.  ``validate``
+                    plast: Optional[torch.Tensor] = None, rows: Optional[torch.Tensor] = None,
+                    operand: str = "bf16") -> CorrelationResult:
+    """Split-bf16 (``operand="fp16"``: split-fp16) small-shift search: best
+    lag over [-half, half]^2 of every (direction, psi) pair; peak_y / peak_x
+    are lag + half.  ``b_hi`` / ``b_lo`` are the lag table split in that
+    operand type (``host.SmallShiftTables(..., operand=...)``).  ``validate``
     checks dir_off / dir_idx against the operands on the host."""
+    if operand not in ("bf16", "fp16"):
+        raise ValueError(f"operand {operand!r} not 'bf16' or 'fp16'")
     for t, name in ((tk, "tk"), (pk, "pk"), (dir_off, "dir_off"), (dir_idx, "dir_idx"), (b_hi, "b_hi"),
                     (b_lo, "b_lo")):
         require_hip(t, name)
@@ -546,10 +551,9 @@ def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.T
     peak = torch.empty((B,), dtype=torch.float32, device=dev)
     mean = torch.empty((B,), dtype=torch.float64, device=dev)
     std = torch.empty((B,), dtype=torch.float64, device=dev)
-    check(lib().gis_correlate_mixed(ptr(tk), ptr(pk), ptr(dir_off), ptr(dir_idx), ptr(pmap), ptr(rows), B, int(P),
-                                    total,
-                                    kpad, ptr(b_hi), ptr(b_lo), ptr(plast), int(half), int(pt), int(wpb),
-                                    ptr(scores), ptr(py), ptr(px), ptr(best), ptr(peak), ptr(mean), ptr(std),
-                                    stream_of(dev)),
+    check(lib().gis_correlate_mixed_ex(ptr(tk), ptr(pk), ptr(dir_off), ptr(dir_idx), ptr(pmap), ptr(rows), B,
+                                       int(P), total, kpad, ptr(b_hi), ptr(b_lo), ptr(plast), int(half), int(pt),
+                                       int(wpb), ptr(scores), ptr(py), ptr(px), ptr(best), ptr(peak), ptr(mean),
+                                       ptr(std), 0 if operand == "bf16" else 1, stream_of(dev)),
           "gis_correlate_mixed")
     return CorrelationResult(scores, py, px, best, peak, mean, std)

@@ -307,6 +307,23 @@ int gis_correlate_mixed(const void* Tk, const void* Pk, const int* dir_off, cons
                         int half, int pt, int wpb, float* scores, int* peak_y, int* peak_x, int* best, float* peak,
                         double* mean, double* stdv, void* stream);
 
+/* 16-bit operand type of gis_correlate_mixed_ex. */
+enum { GIS_OPERAND_BF16 = 0, GIS_OPERAND_F16 = 1 };
+
+/*
+ * gis_correlate_mixed with a choice of operand type (the precision variant's
+ * "FP16 variant"; same reference functional as gis_correlate_mixed).
+ * operand = GIS_OPERAND_BF16: as gis_correlate_mixed.  GIS_OPERAND_F16: Bhi /
+ * Blo hold the lag table split into fp16 hi + lo (same layout), the A operand
+ * is scaled by 2^14 before its fp16 split (|T_k P_k| <= 1, so it stays finite)
+ * and the accumulator by 2^-14 (v_mfma_f32_16x16x32_f16, ~22 significant
+ * bits per product against split-bf16's ~16).
+ */
+int gis_correlate_mixed_ex(const void* Tk, const void* Pk, const int* dir_off, const int* dir_idx, const int* pmap,
+                           const int* rows, int B, int P, int nrows, int Kpad, const void* Bhi, const void* Blo,
+                           const void* Plast, int half, int pt, int wpb, float* scores, int* peak_y, int* peak_x,
+                           int* best, float* peak, double* mean, double* stdv, int operand, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,15 +106,16 @@ def test_gather_coefficients(hip_device):
     np.testing.assert_array_equal(out2.cpu().numpy()[:, :300], spec.reshape(rows, -1)[:, kidx])
 
 
+@pytest.mark.parametrize("operand", ["bf16", "fp16"])
 @pytest.mark.parametrize("n,radius,half", [(32, 9.0, 3), (64, 20.5, 3), (96, 30.0, 2), (64, 12.0, 0)])
-def test_mixed_correlate_random(hip_device, n, radius, half):
+def test_mixed_correlate_random(hip_device, n, radius, half, operand):
     import torch
     from gisapr_amd import host, kernels
-    say("mixed random", n, radius, half)
+    say("mixed random", n, radius, half, operand)
     rng = np.random.default_rng(n + half)
     m = n // 2 + 1
     Mtot, B, P = 17, 9, 3
-    tab = host.SmallShiftTables(n, radius, half)
+    tab = host.SmallShiftTables(n, radius, half, operand)
     K, kpad = tab.K, tab.Kpad
 
     def unit(*shape):
@@ -140,14 +141,14 @@ def test_mixed_correlate_random(hip_device, n, radius, half):
                                    torch.from_numpy(idx).to(d),
                                    torch.from_numpy(tab.b_hi.view(np.int16)).to(d),
                                    torch.from_numpy(tab.b_lo.view(np.int16)).to(d), half,
-                                   plast=torch.from_numpy(tab.phi_last).to(d))
+                                   plast=torch.from_numpy(tab.phi_last).to(d), operand=operand)
     # any processing order of the rows gives bit-identical results
     perm = torch.from_numpy(rng.permutation(len(idx)).astype(np.int32)).to(d)
     corr2 = kernels.correlate_mixed(padded(T), padded(Pk), P, torch.from_numpy(off).to(d),
                                     torch.from_numpy(idx).to(d),
                                     torch.from_numpy(tab.b_hi.view(np.int16)).to(d),
                                     torch.from_numpy(tab.b_lo.view(np.int16)).to(d), half,
-                                    plast=torch.from_numpy(tab.phi_last).to(d), rows=perm)
+                                    plast=torch.from_numpy(tab.phi_last).to(d), rows=perm, operand=operand)
     for a, b in zip(corr.__dict__.values(), corr2.__dict__.values()):
         np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
     torch.cuda.synchronize()
@@ -156,6 +157,8 @@ def test_mixed_correlate_random(hip_device, n, radius, half):
     P32 = Pk.astype(np.complex64).astype(np.complex128)
     worst, ties = compare_particles(corr, T32, P32, local, P, phi, half)
     say("worst rel err", worst, "near ties", ties)
+    if operand == "fp16":  # ~22-bit products: well inside the shared bar
+        assert worst <= TOL / 5, worst
 
 
 def test_mixed_correlate_rejects_bad_arguments(hip_device):
@@ -273,7 +276,8 @@ def test_template_coefficients_equal_gathered_bank(hip_device, name):
     assert np.all(b[:rts.shape[0]] == 7.0 + 7.0j)
 
 
-def test_small_shift_box256_subset_precision(hip_device):
+@pytest.mark.parametrize("operand", ["bf16", "fp16"])
+def test_small_shift_box256_subset_precision(hip_device, operand):
     """Box 256 at 1.06 A/px, 4 A disk (K ~ 7.2k), 49 lags, seeded 256-particle
     subset: the split-bf16 path against the same functional in FP64 on the
     same FP32 coefficients.  Top-1 must agree on every particle whose FP64
@@ -282,12 +286,12 @@ def test_small_shift_box256_subset_precision(hip_device):
     from gisapr_amd import synthetic
     from gisapr_amd.engine import SearchEngine
     wl = synthetic.WORKLOADS["box256"]
-    say("small-shift box256 subset 256")
+    say("small-shift box256 subset 256", operand)
     data = synthetic.make_workload("box256", n_particles=256, device=hip_device)
     eng = SearchEngine(box=wl.box, apix=wl.apix, crop=wl.crop, psi_step=wl.psi_step, local_range=wl.local_range,
                        kk=wl.kk, mask_radius=wl.mask_radius, mask_edge=wl.mask_edge, fsc_values=data["fsc"],
                        dir_rot=data["dir_rot"], dir_tilt=data["dir_tilt"], device=hip_device)
-    tab = eng.enable_small_shift(resolution=4.0, half=3)
+    tab = eng.enable_small_shift(resolution=4.0, half=3, operand=operand)
     assert 7000 < tab.K < 7400
     dfo = np.stack([data["dfu"], data["dfv"], data["dfa"]], 1)
     ps = eng.add_particles(data["images"], data["rot"], data["tilt"], data["psi"], defocus=dfo)
@@ -300,4 +304,6 @@ def test_small_shift_box256_subset_precision(hip_device):
     torch.cuda.synchronize()
     phi = opr.phi_table(wl.box, tab.kidx.astype(np.int64), 3)
     worst, ties = compare_particles(res.corr, tk, pk, ps.local, ps.P, phi, 3)
-    say("box256 worst rel err", worst, "near ties", ties)
+    say("box256 worst rel err", worst, "near ties", ties, operand)
+    if operand == "fp16":
+        assert worst <= TOL / 5, worst

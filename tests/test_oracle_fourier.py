@@ -173,3 +173,34 @@ def test_small_shift_host_tables_match_oracle():
     K = host.SmallShiftTables(256, 256 * 1.06 / 4, 3).K   # 4 A at box 256, 1.06 A/px
     assert 7000 < K < 7400
 
+
+def test_small_shift_tables_fp16_split():
+    """The fp16 operand table: the same layout, hi the nearest half to B
+    (no neighbouring half is closer), lo the nearest half to B - hi, and
+    hi + lo within 2^-21 of Phi (|B| <= 2; fp16 hi + lo keeps ~22 bits)."""
+    from gisapr_amd import host
+    from oracle import precision as opr
+    with pytest.raises(ValueError):
+        host.SmallShiftTables(32, 9.0, 3, "fp8")
+    for n, R, half in [(32, 9.0, 3), (256, 67.84, 3)]:
+        t = host.SmallShiftTables(n, R, half, "fp16")
+        assert t.operand == "fp16"
+        tiles, ks = t.b_hi.shape[:2]
+
+        def unlayout(b):
+            return (b.reshape(tiles, ks, 4, 16, 8).transpose(1, 2, 4, 0, 3).reshape(ks * 32, -1)
+                    .view(np.float16).astype(np.float64))
+        hi, lo = unlayout(t.b_hi), unlayout(t.b_lo)
+        phi = opr.phi_table(n, t.kidx.astype(np.int64), half)
+        S = (2 * half + 1) ** 2
+        want = np.zeros_like(hi)
+        want[0:2 * t.K:2, :S] = phi.real
+        want[1:2 * t.K:2, :S] = -phi.imag
+        assert np.abs(hi + lo - want).max() <= 2.0 ** -21
+        for part, target in ((hi, want), (lo, want - hi)):
+            h16 = part.astype(np.float16)
+            for step in (np.inf, -np.inf):
+                nb = np.nextafter(h16, np.float16(step)).astype(np.float64)
+                assert np.all(np.abs(target - part) <= np.abs(target - nb))
+        assert np.abs(lo).max() > 0 and not hi[2 * t.K:].any() and not hi[:, S:].any()
+
