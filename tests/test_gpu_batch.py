@@ -46,11 +46,23 @@ def single(eng, ps, wl, data, pose, path):
     return res.corr.best.cpu().numpy(), res.corr.peak.cpu().numpy()
 
 
-@pytest.mark.parametrize("path,group", [("mixed", 8), ("fp32", 8), ("mixed", 2), ("fp32", 2)])
-def test_batch_slots_match_single_candidates(setup, path, group):
+@pytest.mark.parametrize("path,group,operand", [("mixed", 8, "bf16"), ("fp32", 8, None), ("mixed", 2, "bf16"),
+                                                ("fp32", 2, None), ("mixed", 8, "fp16")])
+def test_batch_slots_match_single_candidates(setup, path, group, operand):
+    wl, data, eng, ps, poses = setup
+    if operand == "fp16":
+        eng.set_small_shift_operand("fp16")
+    try:
+        _batch_slots_match_single_candidates(setup, path, group, operand)
+    finally:
+        if operand == "fp16":
+            eng.set_small_shift_operand("bf16")
+
+
+def _batch_slots_match_single_candidates(setup, path, group, operand):
     from gisapr_amd.batch import CandidateBatch, RigidBody
     wl, data, eng, ps, poses = setup
-    say("batch slots", path, "template group", group)
+    say("batch slots", path, "template group", group, "operand", operand)
     body = RigidBody.from_arrays(data["coords"], data["numbers"], data["chains"], wl.moving_chain)
     subsets = [np.arange(ps.B), np.arange(0, ps.B, 2), np.arange(5, 17)]
     batch = CandidateBatch(eng, ps, body, wl.res, slots=3, path=path, subsets=subsets, template_group=group)
