@@ -146,30 +146,32 @@ def precision_check(eng, ps, tk, res, subset: int, seed: int):
             "max_rel_err_fp64": worst, "reference": "same functional in float64 (torch complex128)"}
 
 
-def fp16_variant(eng, ps, tk, res_bf16, subset: int, seed: int, repeats: int = 5):
+def fp16_variant(eng, ps, tk, res_bf16, subset: int, seed: int, repeats: int = 6):
     """The split-fp16 operand variant of the small-shift search on the same
-    template and particle coefficients: search time (HIP events on the launch
-    stream, mean of ``repeats`` launches after one warm-up, beside the
-    split-bf16 search timed the same way), scores/s of the search stage, top-1
-    against the split-bf16 search and, on the seeded subset, against the
-    float64 functional (precision_check)."""
+    template and particle coefficients: search time (HIP events around each
+    launch on the launch stream, median of ``repeats`` launches per operand,
+    split-bf16 and split-fp16 alternating so clock drift hits both alike),
+    scores/s of the search stage, top-1 against the split-bf16 search and, on
+    the seeded subset, against the float64 functional (precision_check)."""
     import torch
-
-    def timed():
-        res = eng.search_small_shift(None, ps, tk=tk)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(repeats):
-            eng.search_small_shift(None, ps, tk=tk)
-        e1.record()
-        torch.cuda.synchronize()
-        return res, e0.elapsed_time(e1) / repeats
-
-    _, ms_bf16 = timed()
-    eng.set_small_shift_operand("fp16")
+    times = {"bf16": [], "fp16": []}
+    res_ = None
     try:
-        res_, ms = timed()
+        for rep in range(repeats + 1):  # the first round is a warm-up
+            for operand in ("bf16", "fp16"):
+                eng.set_small_shift_operand(operand)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                res = eng.search_small_shift(None, ps, tk=tk)
+                e1.record()
+                torch.cuda.synchronize()
+                if rep:
+                    times[operand].append(e0.elapsed_time(e1))
+                if operand == "fp16":
+                    res_ = res
+        eng.set_small_shift_operand("fp16")
+        ms, ms_bf16 = float(np.median(times["fp16"])), float(np.median(times["bf16"]))
         out = {"operand": "fp16 x3 (split hi+lo, A scaled 2^14, fp32 accumulate)", "search_ms": ms,
                "bf16_search_ms": ms_bf16, "search_scores_per_s": ps.n_scores / (ms * 1e-3),
                "top1_same_as_bf16": float(np.mean(res_.corr.best.cpu().numpy() == res_bf16.corr.best.cpu().numpy()))}

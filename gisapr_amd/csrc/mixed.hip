@@ -56,15 +56,14 @@ gather_coeffs_kernel(const float2* __restrict__ src, size_t plane, int rows, con
   }
 }
 
-// x = hi + lo with hi = h(s x) and lo = h(s x - hi), both round to nearest
-// even (h = the 16-bit operand type, s = its A scale, a power of two); pairs
-// of values go through one packed conversion each way.
-template <typename V8, typename V2, int SCALE_EXP>
+// x = hi + lo with hi = h(x) and lo = h(x - hi), both round to nearest even
+// (h = the 16-bit operand type); pairs of values go through one packed
+// conversion each way.
+template <typename V8, typename V2>
 __device__ __forceinline__ void split8(const float (&x)[8], V8& hi, V8& lo) {
-  constexpr float s = (float)(1 << SCALE_EXP);
 #pragma unroll
   for (int e = 0; e < 8; e += 2) {
-    const f32x2 v = {x[e] * s, x[e + 1] * s};
+    const f32x2 v = {x[e], x[e + 1]};
     const V2 h = __builtin_convertvector(v, V2);
     const f32x2 hf = __builtin_convertvector(h, f32x2);
     const V2 l = __builtin_convertvector(v - hf, V2);
@@ -220,7 +219,11 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
       float4 uc[PT][2];
 #pragma unroll
       for (int p = 0; p < PT; ++p) { uc[p][0] = now.u[p][0]; uc[p][1] = now.u[p][1]; }
-      const float tr[4] = {t0.x, t0.z, t1.x, t1.z}, ti[4] = {t0.y, t0.w, t1.y, t1.w};
+      // the A scale (a power of two, 1 for bf16) goes on the template values
+      // once per k-step; every product, and the last-lag sum, carry it exactly
+      constexpr float as = (float)(1 << OP::kScaleExp);
+      const float tr[4] = {t0.x * as, t0.z * as, t1.x * as, t1.z * as};
+      const float ti[4] = {t0.y * as, t0.w * as, t1.y * as, t1.w * as};
       h8 ahi[PT], alo[PT];
 #pragma unroll
       for (int p = 0; p < PT; ++p) {
@@ -235,7 +238,7 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
         if (VL)
           accl[p] += ((x[0] * f0.x + x[1] * f0.y) + (x[2] * f0.z + x[3] * f0.w)) +
                      ((x[4] * f1.x + x[5] * f1.y) + (x[6] * f1.z + x[7] * f1.w));
-        split8<h8, typename OP::v2, OP::kScaleExp>(x, ahi[p], alo[p]);
+        split8<h8, typename OP::v2>(x, ahi[p], alo[p]);
       }
       const h8* fr = blds[cur] + kk * FRAG + lane;
 #pragma unroll
@@ -284,7 +287,7 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
       const int slot = group * 16 + 4 * g + v;
       const int row = slot < nrows ? (rows ? rows[slot] : slot) : nrows;
       if (VL) {
-        const float vl = __shfl(tl, 4 * g + v, 64) * 1024.0f;  // lane 4g+v holds that row's sum
+        const float vl = __shfl(tl, 4 * g + v, 64) * OP::kOut;  // lane 4g+v holds that row's sum
         if (vl > bv) { bv = vl; bs = nlag - 1; }  // the last lag: strict > keeps earlier ties
       }
       if (live && i == 0 && row < nrows && p0 + p < P) {
