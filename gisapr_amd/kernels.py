@@ -539,6 +539,10 @@ def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.T
         if rows is not None and not torch.equal(torch.sort(rows.long()).values,
                                                 torch.arange(total, device=rows.device)):
             raise ValueError("rows must be a permutation of the dir_idx rows")
+        # fp16 holds 2^14 |T_k P_k| only while it stays below 65504
+        if operand == "fp16" and tk.numel() and pk.numel() and \
+                float(tk.abs().max()) * float(pk.abs().max()) > 4.0:
+            raise ValueError("the fp16 operand needs |T_k P_k| <= 4 (unit-norm spectra)")
     if rows is not None:
         require_hip(rows, "rows")
         if rows.dtype != torch.int32 or rows.numel() != total:

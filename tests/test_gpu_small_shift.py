@@ -182,6 +182,15 @@ def test_mixed_correlate_rejects_bad_arguments(hip_device):
     with pytest.raises(ValueError):
         bad = torch.tensor([0, 4], dtype=torch.int32, device=d)
         kernels.correlate_mixed(tk, pk, 2, off, bad, bh, bl, 3, plast=pl)  # direction 4 of a 4-row bank
+    with pytest.raises(ValueError):
+        kernels.correlate_mixed(tk, pk, 2, off, idx, bh, bl, 3, plast=pl, operand="fp8")
+    with pytest.raises(ValueError, match="fp16"):  # 2^14 * 6 would overflow fp16
+        kernels.correlate_mixed(tk + 3.0, pk + 2.0, 2, off, idx, bh, bl, 3, plast=pl, operand="fp16")
+    # the C entry point refuses an unknown operand before touching any pointer
+    from gisapr_amd._lib import lib
+    rc = lib().gis_correlate_mixed_ex(None, None, None, None, None, None, 1, 1, 0, 16, None, None, None, 3, 0, 0,
+                                      None, None, None, None, None, None, None, 2, None)
+    assert rc < 0 and b"operand" in lib().gis_last_error()
 
 
 def _oracle_inputs(data, wl, tab):
