@@ -304,6 +304,7 @@ class CandidateBatch:
         self.autotuner = autotuner
         self.tile: dict = {}
         self.graph = None
+        self._graph_tables = None
         self.static_out = None
 
     # -- device part ---------------------------------------------------------
@@ -395,6 +396,11 @@ class CandidateBatch:
         with torch.cuda.graph(g):
             self.static_out = self._device_step()
         self.graph = g
+        # the graph holds the device addresses of the engine's lag tables: keep
+        # those tensors alive and refuse replays after the engine switches them
+        eng = self.engine
+        self._graph_tables = ((eng.small_bhi, eng.small_blo, eng.small_plast)
+                              if getattr(eng, "small", None) is not None else None)
         return g
 
     def evaluate(self, poses, sync: bool = True):
@@ -403,6 +409,9 @@ class CandidateBatch:
         the first k slots only; a captured graph always runs every slot."""
         k = self._stage(poses)
         if self.graph is not None:
+            eng = self.engine
+            if self._graph_tables is not None and self._graph_tables[0] is not getattr(eng, "small_bhi", None):
+                raise RuntimeError("the engine's small-shift tables changed since capture(); capture again")
             self.graph.replay()
             scores, corr = self.static_out
         else:

@@ -97,6 +97,10 @@ def test_batch_graph_replays_match_eager(setup, path):
         got = graphed.evaluate(poses[[a, b]])
         torch.cuda.synchronize()
         np.testing.assert_allclose(got, want, rtol=1e-5, err_msg=f"replay {rep}")
+    if path == "mixed":  # new lag tables (same values): the graph holds the old addresses and must be re-captured
+        eng.set_small_shift_operand("bf16")
+        with pytest.raises(RuntimeError, match="capture again"):
+            graphed.evaluate(poses[:2])
 
 
 def test_autotuner_variants_agree(setup, tmp_path):
