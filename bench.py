@@ -10,17 +10,23 @@ splat, 3-D transform, 768-direction template bank, search of every
 (particle, local direction, psi) pair, per-particle reduction and the
 float64 score sum.  Legs, on the same particles and poses:
 
-  fp32           one candidate per step, the v606 FP32 FFT search (+/-8 px)
-  mixed          one candidate per step, the small-shift split-bf16 search
-                 (coefficients inside 1/(4 A), 49 lags, +/-3 px)
+  fp32           one candidate per step, the v606 FP32 search (every packed
+                 coefficient, the 16 x 16 crop: +/-8 px) on FP32 MFMA
+  split          one candidate per step, the same search (same operator,
+                 same crop) on split-bf16 MFMA
+  mixed          one candidate per step, the small-shift split-bf16 search of
+                 the precision variant (coefficients inside 1/(4 A), 49 lags,
+                 +/-3 px): a different, truncated functional
   batched_fp32   ``--slots`` candidates per step, one search launch over all
-  batched_mixed  of them, replayed as a captured HIP graph, tile variant
-                 from the autotuner
+  batched_split  of them, replayed as a captured HIP graph, tile variant
+  batched_mixed  from the autotuner (batched_mixed only when asked for)
 
-``value`` is batched_mixed particle-orientation scores/s; the ratio to the
-plain fp32 leg, top-1 agreement between the mixed and fp32 searches, and the
-mixed search's error against the same functional in float64 on a
-256-particle subset are reported beside it.
+``value`` is batched_split particle-orientation scores/s; beside it: the
+ratio to the plain fp32 leg, top-1 (direction, psi, shift) agreement of the
+split and fp32 searches on every particle, the split search's error against
+the same operator in float64 on a 256-particle subset, and the same for the
+small-shift variant (whose top-1 is not expected to match: it truncates the
+coefficients and the lags).
 
 Batched PSO leg (``--pso-iters``, 0 to skip): box-192 workload (20,000-atom
 model, moving chain 4,500, 20,000 particles at 1.3 A/px), a swarm of 64
@@ -57,6 +63,20 @@ PEAK_HBM_GBS = 8000.0
 def correlation_flops_per_score(n: int) -> float:
     """Algorithmic FP32 work of one v606 score: conj-multiply + full C2R."""
     return 6.0 * n * (n / 2 + 1) + 2.5 * n * n * math.log2(n * n)
+
+
+def split_flops_per_score(n: int, crop: int) -> float:
+    """Split-bf16 work of one full-plane score: stage 1 (columns to lags,
+    complex, 8 flops per MAC) over n x (n/2+1) coefficients x C lags, stage 2
+    (rows to lags, real part, 4 flops) over n rows x C^2 lags, 3 products each."""
+    return 3.0 * (8.0 * n * (n // 2 + 1) * crop + 4.0 * n * crop * crop)
+
+
+def template_bytes_per_direction(N: int, n: int) -> float:
+    """HBM bytes of one template row: the packed row written plus the
+    trilinear slice gather from the padded 3-D half spectrum (8 taps of 8
+    bytes per sample over 2N x (N+1) samples, uncached worst case)."""
+    return 8.0 * n * (n // 2 + 1) + 8.0 * 8.0 * 2 * N * (N + 1)
 
 
 def mixed_flops_per_score(K: int, lags: int) -> float:
@@ -146,6 +166,56 @@ def precision_check(eng, ps, tk, res, subset: int, seed: int):
             "max_rel_err_fp64": worst, "reference": "same functional in float64 (torch complex128)"}
 
 
+def precision_check_split(eng, ps, bank, res, subset: int, seed: int):
+    """Split-search scores against the same operator in float64 (torch
+    complex128 on the device, the separable functional W^T Q E of
+    ``host.crop_functional``) on a seeded particle subset; ``bank`` =
+    (unnormalised rows, 1/norms) of the candidate the split leg searched last.
+    A top-1 difference counts as a near-tie when the float64 scores of the two
+    choices differ by at most 1e-5 of the particle's best score."""
+    import torch
+    from gisapr_amd import host
+    dev = eng.device
+    raw, tinv = bank
+    W, E = host.crop_functional(eng.n, eng.crop)
+    Wt = torch.from_numpy(W.T.copy()).to(dev)
+    Et = torch.from_numpy(E).to(dev)
+    rng = np.random.default_rng(seed)
+    pick = np.sort(rng.choice(ps.B, min(subset, ps.B), replace=False))
+    off = ps.dir_off.cpu().numpy()
+    idx = ps.dir_idx.cpu().numpy()
+    sc = res.corr.scores.cpu().numpy().reshape(-1)
+    py = res.corr.peak_y.cpu().numpy().reshape(-1)
+    px = res.corr.peak_x.cpu().numpy().reshape(-1)
+    best = res.corr.best.cpu().numpy()
+    P, C = ps.P, eng.crop
+    agree, ties, worst, used = 0, 0, 0.0, 0
+    for b in pick:
+        d = idx[off[b]:off[b + 1]]
+        if len(d) == 0:
+            continue
+        dd = torch.from_numpy(d.astype(np.int64)).to(dev)
+        t = raw[dd].to(torch.complex128) * tinv[dd].double()[:, None, None]
+        p = ps.spectra[b].to(torch.complex128) * ps.inv_norm[b].double()[:, None, None]
+        q = t[:, None] * p[None]                                    # [Mi, P, n, m]
+        crop = 1024.0 * (Wt @ q @ Et).real.reshape(len(d) * P, C * C)
+        v, k = crop.max(dim=1)
+        v, k = v.cpu().numpy(), k.cpu().numpy()
+        g = slice(off[b] * P, off[b + 1] * P)
+        worst = max(worst, float(np.abs(sc[g] - v).max() / np.abs(v).max()))
+        bw = int(np.argmax(v))
+        gb = off[b] * P + int(best[b])
+        same = int(best[b]) == bw and int(py[gb]) * C + int(px[gb]) == int(k[bw])
+        agree += int(same)
+        if not same:
+            chosen = float(crop[int(best[b]), int(py[gb]) * C + int(px[gb])])
+            ties += int(v[bw] - chosen <= 1e-5 * abs(v[bw]))
+        used += 1
+    return {"subset": int(used), "seed": seed, "top1_agree_fp64": agree / max(used, 1),
+            "top1_differs_outside_near_ties": used - agree - ties, "max_rel_err_fp64": worst,
+            "reference": "same operator (every coefficient, the full crop) in float64 (torch complex128)"}
+
+
 def fp16_variant(eng, ps, tk, res_bf16, subset: int, seed: int, repeats: int = 6):
     """The split-fp16 operand variant of the small-shift search on the same
     template and particle coefficients: search time (HIP events around each
@@ -212,7 +282,7 @@ def pso_leg(args, dev, log):
     ``--pso-iters`` iterations, poses from a seeded normal distribution (rotation
     capped at 5 deg, translation at 2 A), each swarm member scoring its own
     seeded particle subset of 2,000-20,000 particles; one graph-captured
-    batched step per iteration on the mixed path."""
+    batched step per iteration on the ``--pso-path`` search path."""
     import torch
     from gisapr_amd import synthetic
     from gisapr_amd.batch import Autotuner, CandidateBatch, RigidBody
@@ -223,13 +293,16 @@ def pso_leg(args, dev, log):
     dfo = np.stack([data["dfu"], data["dfv"], data["dfa"]], 1)
     ps = eng.add_particles(data["images"], data["rot"], data["tilt"], data["psi"], defocus=dfo, prepare=False)
     del data["images"]
-    eng.prepare_small_shift(ps, drop_spectra=True)
+    if args.pso_path == "mixed":
+        eng.prepare_small_shift(ps, drop_spectra=True)
+    else:
+        eng.prepare(ps)
     rng = np.random.default_rng(synthetic.SEED_CAND + 3)
     S = args.pso_slots
     lo_n, hi_n = min(2000, ps.B), ps.B
     subsets = [np.sort(rng.choice(ps.B, int(rng.integers(lo_n, hi_n + 1)), replace=False)) for _ in range(S)]
     body = RigidBody.from_arrays(data["coords"], data["numbers"], data["chains"], wl.moving_chain)
-    batch = CandidateBatch(eng, ps, body, wl.res, S, path="mixed", subsets=subsets, autotuner=Autotuner())
+    batch = CandidateBatch(eng, ps, body, wl.res, S, path=args.pso_path, subsets=subsets, autotuner=Autotuner())
 
     def swarm():
         rot = rng.normal(0.0, 2.0, (S, 3))
@@ -259,14 +332,15 @@ def pso_leg(args, dev, log):
             "scores_per_s": pairs / dt, "candidates_per_s": S * args.pso_iters / dt,
             "ms_per_iteration": dt * 1000.0 / args.pso_iters, "pairs_per_iteration": batch.plan.n_pairs,
             "mean_subset": float(np.mean([len(x) for x in subsets])), "tile": dict(batch.tile), "graph": True,
-            "path": "mixed", "setup_s": setup, "best_score": best}
+            "path": args.pso_path, "setup_s": setup, "best_score": best}
 
 
 def serve_leg(args, dev, log):
     """Serving legs: three resident datasets, a seeded Poisson request stream per
-    path.  The datasets are built once; the FP32 path runs first because the
-    split-bf16 path replaces each dataset's spectra by their in-disk
-    coefficients.  Returns {path: stats}."""
+    path.  The datasets are built once; the FP32 and full-plane split-bf16
+    paths share their resident spectra, and the small-shift path (when asked
+    for) runs last because it replaces them by their in-disk coefficients.
+    Returns {path: stats}."""
     import torch
     from gisapr_amd import synthetic
     from gisapr_amd.batch import Autotuner, RigidBody
@@ -285,9 +359,9 @@ def serve_leg(args, dev, log):
         built.append((name, wl, eng, ps, body))
         log(f"serving dataset {name} built")
     setup = time.perf_counter() - t0
-    requests = {"fp32": args.serve_requests_fp32, "mixed": args.serve_requests}
+    requests = {"fp32": args.serve_requests_fp32, "split": args.serve_requests, "mixed": args.serve_requests}
     out = {}
-    for path in ("fp32", "mixed"):
+    for path in ("fp32", "split", "mixed"):
         if requests[path] <= 0 or path not in args.serve_paths.split(","):
             continue
         t1 = time.perf_counter()
@@ -335,13 +409,14 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="split-bf16 serving stream (0 skips the serving legs)")
     ap.add_argument("--serve-requests-fp32", type=int, default=40, help="FP32 serving stream (0 to skip it)")
-    ap.add_argument("--serve-paths", default="fp32,mixed")
+    ap.add_argument("--serve-paths", default="fp32,split")
+    ap.add_argument("--pso-path", default="split", help="search path of the batched PSO leg")
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the legs, run one device-to-device copy of this many MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--legs", default="fp32,mixed,batched_fp32,batched_mixed",
+    ap.add_argument("--legs", default="fp32,split,mixed,batched_fp32,batched_split",
                     help="comma list of legs to run (profiling runs select one)")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
@@ -425,11 +500,15 @@ def main() -> int:
 
     want = set(args.legs.split(","))
     legs, results = {}, {}
-    bank = tk_last = None  # the last candidate's (bank, 1/norms) / template coefficients
+    bank = bank_split = tk_last = None  # the last candidate's (bank, 1/norms) / template coefficients
     if "fp32" in want:
         # FP32 path: full unnormalised bank (1/norm folded into the search)
         legs["fp32"], results["fp32"], bank = run_single(lambda v: eng.templates(v, normalize=False),
                                                          lambda t, p: eng.search(t[0], p, t[1]))
+    if "split" in want:
+        # the same search on split-bf16 MFMA, same bank
+        legs["split"], results["split"], bank_split = run_single(lambda v: eng.templates(v, normalize=False),
+                                                                 lambda t, p: eng.search_split(t[0], p, t[1]))
     if "mixed" in want:
         # split-bf16 path: the template kernel writes the in-disk coefficients directly
         legs["mixed"], results["mixed"], tk_last = run_single(eng.small_templates,
@@ -439,7 +518,7 @@ def main() -> int:
     # batched legs: ``slots`` candidates per step, graph-captured, autotuned search tile
     body = RigidBody.from_arrays(coords, numbers, chains, wl.moving_chain)
     tuner = Autotuner()
-    for path in ("fp32", "mixed"):
+    for path in ("fp32", "split", "mixed"):
         if "batched_" + path not in want:
             continue
         batch = CandidateBatch(eng, ps, body, wl.res, args.slots, path=path, autotuner=tuner)
@@ -473,6 +552,23 @@ def main() -> int:
                                     "frac": ach / PEAK_FP32_MFMA_TFLOPS,
                                     "flops_per_score": correlation_flops_per_score(n), "launch_ms": ms,
                                     "traffic": profiled_traffic("correlate_kernel", wl.name)}
+    if "split" in legs:
+        ms = legs["split"]["stage_ms"]["search"]
+        fps = split_flops_per_score(n, eng.crop)
+        ach = fps * n_scores / (ms * 1e-3) / 1e12
+        legs["split"]["roofline"] = {"bound": "mfma", "kernel": "split_correlate_kernel (+ tables, particle tail)",
+                                     "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
+                                     "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps,
+                                     "launch_ms": ms,
+                                     "traffic": profiled_traffic("split_correlate_kernel", wl.name)}
+        ms_t = legs["split"]["stage_ms"]["templates"]
+        bpd = template_bytes_per_direction(eng.N, n)
+        ach_t = bpd * len(eng.dir_rot) / (ms_t * 1e-3) / 1e9
+        legs["split"]["roofline_templates"] = {
+            "bound": "hbm", "kernel": "volume transform + build_templates_kernel", "achieved": ach_t,
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach_t / PEAK_HBM_GBS,
+            "bytes_per_direction": bpd, "directions": len(eng.dir_rot), "launch_ms": ms_t,
+            "traffic": profiled_traffic("build_templates_kernel", wl.name)}
     if "mixed" in legs:
         ms = legs["mixed"]["stage_ms"]["search"]
         fps = mixed_flops_per_score(tab.K, tab.n_lags)
@@ -491,15 +587,24 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
 
-    head_name = next(k for k in ("batched_mixed", "mixed", "batched_fp32", "fp32") if k in legs)
+    head_name = next(k for k in ("batched_split", "split", "batched_mixed", "mixed", "batched_fp32", "fp32")
+                     if k in legs)
     head = legs[head_name]
-    single = "mixed" if "mixed" in legs else "fp32"
+    single = next(k for k in ("split", "mixed", "fp32") if k in legs)
+    roof = dict(legs[single].get("roofline", {}))
+    if single == "split" and "roofline_templates" in legs["split"]:
+        # the line's roofline is the kernel that takes the larger share of a candidate step
+        st = legs["split"]["stage_ms"]
+        if st["templates"] > st["search"]:
+            roof = dict(legs["split"]["roofline_templates"])
+        roof["from_leg"] = "split (one candidate per step, HIP events on the launch stream)"
     out = {
         "metric": METRIC, "value": head["scores_per_s"], "unit": "scores/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": head.get("ms_per_step", head["ms_per_candidate"]),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3 (split hi+lo, fp32 accumulate)" if "mixed" in head_name else "fp32",
+        "dtype": ("bf16x3 (split hi+lo, fp32 accumulate)" if ("mixed" in head_name or "split" in head_name)
+                  else "fp32"),
         "data": "synthetic (seeded generators; random-walk model)",
         "config": {"workload": f"{wl.name}: {sum(wl.chain_sizes)}-atom model (moving chain "
                                f"{wl.chain_sizes[-1]}), {ps.B} particles {wl.box}x{wl.box} at {wl.apix} A/px, "
@@ -508,44 +613,55 @@ def main() -> int:
                                f"candidates per batched step, particle spectra resident",
                    "box": wl.box, "particles": ps.B, "directions": len(eng.dir_rot), "psi": ps.P,
                    "scores_per_candidate": n_scores, "path": head_name, "slots": args.slots,
-                   "fp32_crop": wl.crop, "mixed_lags": tab.n_lags, "mixed_K": tab.K,
+                   "crop": wl.crop, "mixed_lags": tab.n_lags, "mixed_K": tab.K,
                    "mixed_resolution_A": args.resolution},
         "candidates_per_s": 1000.0 / head["ms_per_candidate"],
         "particle_prep_ms": prep_ms, "coefficient_gather_ms": gather_ms,
         "legs": legs,
         "setup_s": setup_s,
-        "roofline": dict(legs[single].get("roofline", {})),
+        "roofline": roof,
         "cpu_baseline": None,
     }
     if "fp32" in legs:
         out["speedup_vs_fp32"] = {k: v["scores_per_s"] / legs["fp32"]["scores_per_s"] for k, v in legs.items()}
-    if "fp32" in results and "mixed" in results:
-        a = top1(results["fp32"], ps, eng.crop // 2)
-        b = top1(results["mixed"], ps, 3)
-        valid = a[:, 0] >= 0
-        out["top1_same_as_fp32"] = {
-            "orientation": float(np.mean((a[valid, 0] == b[valid, 0]))),
-            "orientation_and_shift": float(np.mean(np.all(a[valid] == b[valid], axis=1))),
-            "particles": int(valid.sum())}
+    agreement = {}
+    for other, center in (("split", eng.crop // 2), ("mixed", 3)):
+        if "fp32" in results and other in results:
+            a = top1(results["fp32"], ps, eng.crop // 2)
+            b = top1(results[other], ps, center)
+            valid = a[:, 0] >= 0
+            agreement[other] = {
+                "orientation": float(np.mean((a[valid, 0] == b[valid, 0]))),
+                "orientation_and_shift": float(np.mean(np.all(a[valid] == b[valid], axis=1))),
+                "particles": int(valid.sum())}
+    if "split" in agreement:
+        out["top1_same_as_fp32"] = dict(agreement["split"], path="split (same operator and crop)")
+    if "mixed" in agreement:
+        out["top1_small_shift_vs_fp32"] = dict(
+            agreement["mixed"], path="mixed small-shift variant (truncated coefficients, +/-3 px lags)")
+    if args.precision_subset > 0 and "split" in results:
+        out["precision"] = precision_check_split(eng, ps, bank_split, results["split"], args.precision_subset,
+                                                 synthetic.SEED_CAND)
     if args.precision_subset > 0 and "mixed" in results:
-        out["precision"] = precision_check(eng, ps, tk_last, results["mixed"], args.precision_subset,
-                                           synthetic.SEED_CAND)
+        out["precision_small_shift"] = precision_check(eng, ps, tk_last, results["mixed"], args.precision_subset,
+                                                       synthetic.SEED_CAND)
     if args.fp16_variant and "mixed" in results:
         out["fp16_variant"] = fp16_variant(eng, ps, tk_last, results["mixed"], args.precision_subset,
                                            synthetic.SEED_CAND)
     if not args.no_cpu_baseline and args.cpu_sample > 0 and bank is not None:
         out["cpu_baseline"] = cpu_baseline(eng, ps, bank, args.cpu_sample)
-    del ps, eng, bank, tk_last, results
+    del ps, eng, bank, bank_split, tk_last, results
     torch.cuda.empty_cache()
     if args.pso_iters > 0:
         out["pso"] = pso_leg(args, dev, log)
         torch.cuda.empty_cache()
     if args.serve_requests > 0:
         served = serve_leg(args, dev, log)
-        if "mixed" in served:
-            out["serving"] = served["mixed"]
-            out["p50_ms"] = served["mixed"]["p50_ms"]
-            out["p99_ms"] = served["mixed"]["p99_ms"]
+        main_path = next((p for p in ("split", "mixed") if p in served), None)
+        if main_path is not None:
+            out["serving"] = served[main_path]
+            out["p50_ms"] = served[main_path]["p50_ms"]
+            out["p99_ms"] = served[main_path]["p99_ms"]
         if "fp32" in served:
             out["serving_fp32"] = served["fp32"]
     print(json.dumps(out), flush=True)

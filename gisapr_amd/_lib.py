@@ -49,6 +49,9 @@ SIGNATURES = {
     "gis_correlate_workspace": (_sz, [_i]),
     "gis_correlate": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp,
                            _vp, _vp, _vp, _sz, _vp]),
+    "gis_correlate_split_workspace": (_sz, [_i, _i, _i]),
+    "gis_correlate_split": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp,
+                                 _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gis_score_sum_segments": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
     "gis_score_sum": (_i, [_vp, _vp, _i, _vp, _vp]),
     "gis_gather_coefficients": (_i, [_vp, _sz, _i, _vp, _i, _i, _vp, _vp, _vp]),

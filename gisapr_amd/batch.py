@@ -93,13 +93,14 @@ class Autotuner:
 
 
 MIXED_VARIANTS = [dict(pt=5, wpb=4), dict(pt=5, wpb=2), dict(pt=3, wpb=4), dict(pt=1, wpb=4)]
+SPLIT_VARIANTS = [dict(dt=1, pt=5), dict(dt=2, pt=3), dict(dt=1, pt=3), dict(dt=2, pt=2)]
 FP32_VARIANTS = [dict(jt=0), dict(jt=2), dict(jt=4), dict(jt=6)]
 
 
 def _variants_for(path: str, P: int) -> List[dict]:
-    if path == "mixed":
+    if path in ("mixed", "split"):
         out = []
-        for v in MIXED_VARIANTS:
+        for v in (MIXED_VARIANTS if path == "mixed" else SPLIT_VARIANTS):
             v = dict(v, pt=min(v["pt"], P))
             if v not in out:
                 out.append(v)
@@ -135,6 +136,7 @@ class SlotPlan:
         return self.dir_off[:v + 1], self.dir_idx[:r], self.pmap[:v], self.slot_off[:k + 1]
 
     particle_rank: np.ndarray = None  # [particles] processing rank of each particle
+    pmap_host: np.ndarray = None      # [V] particle of each virtual particle
 
     def particle_major(self, k: int) -> torch.Tensor:
         """Processing order of the first k slots' rows: each particle's rows of all
@@ -148,6 +150,20 @@ class SlotPlan:
             order = np.argsort(key, kind="stable").astype(np.int32)
             self._orders[k] = torch.from_numpy(order).to(self.dir_idx.device)
         return self._orders[k]
+
+    def virtual_major(self, k: int) -> torch.Tensor:
+        """Processing order of the first k slots' virtual particles: the slots'
+        copies of one particle adjacent (they share its spectra in cache),
+        particles in ``particle_rank`` order."""
+        if self._orders is None:
+            self._orders = {}
+        key = ("virtual", k)
+        if key not in self._orders:
+            v = int(self.slot_off_host[k])
+            pm = self.pmap_host[:v]
+            order = np.argsort(self.particle_rank[pm], kind="stable").astype(np.int32)
+            self._orders[key] = torch.from_numpy(order).to(self.dir_idx.device)
+        return self._orders[key]
 
     @property
     def virtual(self) -> int:
@@ -201,7 +217,7 @@ def build_slot_plan(ps: ParticleSet, M: int, slots: int, subsets: Optional[Seque
                     dir_idx=torch.from_numpy(d_idx).to(dev), pmap=torch.from_numpy(pm).to(dev),
                     slot_off=torch.from_numpy(so).to(dev), slot_off_host=so, row_off_host=d_off[so].copy(),
                     n_pairs=int(len(d_idx)) * ps.P, row_particle=row_particle,
-                    particle_rank=orientation_rank(off, idx))
+                    particle_rank=orientation_rank(off, idx), pmap_host=pm)
 
 
 # ---------------------------------------------------------------------------
@@ -238,8 +254,8 @@ class RigidBody:
 
 class CandidateBatch:
     """``slots`` candidate poses of one moving chain, searched against one
-    resident particle set, on the FP32 ("fp32") or split-bf16 small-shift
-    ("mixed") path.
+    resident particle set, on the FP32 ("fp32"), full-plane split-bf16
+    ("split") or split-bf16 small-shift ("mixed") path.
 
     ``evaluate(poses)`` runs one step eagerly; ``capture()`` records the
     device part as a HIP graph and later ``evaluate`` calls replay it.
@@ -255,8 +271,8 @@ class CandidateBatch:
     def __init__(self, engine: SearchEngine, ps: ParticleSet, body: RigidBody, res: float, slots: int, *,
                  path: str = "mixed", subsets=None, autotuner: Optional[Autotuner] = None,
                  template_group: Optional[int] = None):
-        if path not in ("mixed", "fp32"):
-            raise ValueError("path must be 'mixed' or 'fp32'")
+        if path not in ("mixed", "fp32", "split"):
+            raise ValueError("path must be 'mixed', 'split' or 'fp32'")
         self.engine, self.ps, self.body, self.path = engine, ps, body, path
         self.slots = int(slots)
         if template_group is None:
@@ -346,6 +362,11 @@ class CandidateBatch:
                                            pt=tile.get("pt", 0), wpb=tile.get("wpb", 0), plast=eng.small_plast,
                                            rows=self.plan.particle_major(self.slots if k is None else k),
                                            operand=eng.small.operand)
+        if self.path == "split":
+            return kernels.correlate_split(self.bank, ps.spectra, ps.inv_norm, dir_off, dir_idx, eng.crop,
+                                           validate=False, pmap=pmap, tinv=self.tinv,
+                                           order=self.plan.virtual_major(self.slots if k is None else k),
+                                           dt=tile.get("dt", 0), pt=tile.get("pt", 0))
         return kernels.correlate(self.bank, ps.spectra, ps.inv_norm, dir_off, dir_idx, eng.crop,
                                  validate=False, pmap=pmap, jt=tile.get("jt", 0), tinv=self.tinv)
 
@@ -381,7 +402,7 @@ class CandidateBatch:
             key = Autotuner.key("mixed" if self.engine.small.operand == "bf16" else "mixed_fp16", self.engine.n,
                                 ps.P, self.engine.small.Kpad, _bucket(int(pl.dir_idx.numel())))
         else:
-            key = Autotuner.key("fp32", self.engine.n, ps.P, _bucket(int(pl.dir_idx.numel())))
+            key = Autotuner.key(self.path, self.engine.n, ps.P, _bucket(int(pl.dir_idx.numel())))
         self.tile = self.autotuner.choose(key, _variants_for(self.path, ps.P), self._search)
         return self.tile
 
@@ -396,12 +417,17 @@ class CandidateBatch:
         with torch.cuda.graph(g):
             self.static_out = self._device_step()
         self.graph = g
-        # the graph holds the device addresses of the engine's lag tables: keep
-        # those tensors alive and refuse replays after the engine switches them
-        eng = self.engine
-        self._graph_tables = ((eng.small_bhi, eng.small_blo, eng.small_plast)
-                              if getattr(eng, "small", None) is not None else None)
+        # the graph holds the device addresses of every operand it reads: keep
+        # those tensors alive and refuse replays once any of them is replaced
+        self._graph_tables = self._graph_operands()
         return g
+
+    def _graph_operands(self):
+        """The engine / particle-set tensors a captured step reads, by identity."""
+        eng, ps = self.engine, self.ps
+        if self.path == "mixed":
+            return (ps.coeffs, eng.small_bhi, eng.small_blo, eng.small_plast, eng.small_kslot)
+        return (ps.spectra, ps.inv_norm)
 
     def evaluate(self, poses, sync: bool = True):
         """Score ``poses`` ([k, 6], k <= slots); returns float64 [k] (or the device
@@ -409,9 +435,9 @@ class CandidateBatch:
         the first k slots only; a captured graph always runs every slot."""
         k = self._stage(poses)
         if self.graph is not None:
-            eng = self.engine
-            if self._graph_tables is not None and self._graph_tables[0] is not getattr(eng, "small_bhi", None):
-                raise RuntimeError("the engine's small-shift tables changed since capture(); capture again")
+            now = self._graph_operands()
+            if len(now) != len(self._graph_tables) or any(a is not b for a, b in zip(now, self._graph_tables)):
+                raise RuntimeError("operands of the captured step were replaced since capture(); capture again")
             self.graph.replay()
             scores, corr = self.static_out
         else:

@@ -372,5 +372,51 @@ __device__ __forceinline__ void block_sum_f64(double (&v)[NV], double* scratch) 
   __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// Per-particle tail of a search (the numerics of make_result_records,
+// v606:1466-1491): one wave per particle over its [M_b][P] pair scores:
+// first maximum in (local direction, psi) order, mean and population std in
+// float64.  Internal linkage: each search source gets its own copy.
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void __launch_bounds__(64)
+particle_tail_kernel(const int* __restrict__ dir_off, int P, const float* __restrict__ scores,
+                     int* __restrict__ best, float* __restrict__ peak, double* __restrict__ mean,
+                     double* __restrict__ stdv) {
+  const int b = blockIdx.x, lane = threadIdx.x;
+  const int d0 = dir_off[b];
+  const int np = (dir_off[b + 1] - d0) * P;
+  const float* s = scores + (size_t)d0 * P;
+  float bv = -INFINITY;
+  int bi = 0x7fffffff;
+  double sum = 0.0;
+  for (int k = lane; k < np; k += 64) {
+    const float v = s[k];
+    sum += (double)v;
+    if (v > bv || (v == bv && k < bi)) { bv = v; bi = k; }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const float ov = __shfl_xor(bv, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    sum += __shfl_xor(sum, o, 64);
+  }
+  const double mu = np > 0 ? sum / (double)np : 0.0;
+  double ss = 0.0;
+  for (int k = lane; k < np; k += 64) {
+    const double dv = (double)s[k] - mu;
+    ss += dv * dv;
+  }
+  ss = wave_sum_f64(ss);
+  if (lane == 0) {
+    best[b] = np > 0 ? bi : -1;
+    peak[b] = np > 0 ? bv : 0.0f;
+    mean[b] = mu;
+    stdv[b] = np > 0 ? sqrt(ss / (double)np) : 0.0;
+  }
+}
+}  // namespace
+
 // Number of compute units of the current device (cached per device).
 int gis_cu_count();

@@ -300,44 +300,6 @@ mixed_correlate_kernel(const float2* __restrict__ Tk, const float2* __restrict__
   }
 }
 
-__global__ void __launch_bounds__(64)
-particle_tail_kernel(const int* __restrict__ dir_off, int P, const float* __restrict__ scores,
-                     int* __restrict__ best, float* __restrict__ peak, double* __restrict__ mean,
-                     double* __restrict__ stdv) {
-  const int b = blockIdx.x, lane = threadIdx.x;
-  const int d0 = dir_off[b];
-  const int np = (dir_off[b + 1] - d0) * P;
-  const float* s = scores + (size_t)d0 * P;
-  float bv = -INFINITY;
-  int bi = 0x7fffffff;
-  double sum = 0.0;
-  for (int k = lane; k < np; k += 64) {
-    const float v = s[k];
-    sum += (double)v;
-    if (v > bv || (v == bv && k < bi)) { bv = v; bi = k; }
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const float ov = __shfl_xor(bv, o, 64);
-    const int oi = __shfl_xor(bi, o, 64);
-    if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-    sum += __shfl_xor(sum, o, 64);
-  }
-  const double mu = np > 0 ? sum / (double)np : 0.0;
-  double ss = 0.0;
-  for (int k = lane; k < np; k += 64) {
-    const double dv = (double)s[k] - mu;
-    ss += dv * dv;
-  }
-  ss = wave_sum_f64(ss);
-  if (lane == 0) {
-    best[b] = np > 0 ? bi : -1;
-    peak[b] = np > 0 ? bv : 0.0f;
-    mean[b] = mu;
-    stdv[b] = np > 0 ? sqrt(ss / (double)np) : 0.0;
-  }
-}
-
 }  // namespace
 
 // Gathers the K listed packed coefficients of `rows` spectra into a dense

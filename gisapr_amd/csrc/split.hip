@@ -1,0 +1,505 @@
+// Full-plane correlation search on split-BF16 MFMA.
+//
+// The v606 score of (particle, direction, psi) at crop lag (y, x) is the
+// fixed linear functional of Q = T * P (correlate_templates_and_particle,
+// v606:1418-1449, for UNshifted operands):
+//     ccg[y][x] = 1/n^2 sum_{r,c} Re( Q[r][c] W[r][y] E[c][x] )
+//     W[r][y] = (-1)^l_y exp(+2 pi i r l_y / n)                      (row roll)
+//     E[c][x] = w(s_c) exp(+2 pi i s_c l_x / n),  s_c = (c + (n/2+1)//2) mod (n/2+1)
+//     w(s) = 1 for s in {0, n/2}, else 2                             (C2R rule)
+// over every packed coefficient and every lag of the C x C crop: the same
+// quantity gis_correlate computes on FP32 MFMA.  Here it runs on
+// v_mfma_f32_16x16x32_bf16 as two chained products per 16-column tile of the
+// packed plane:
+//     stage 1 (rows):     G[c][y]  = sum_r Q[r][c] W[r][y]        M = 16 columns, K = 16 rows (re, im) per
+//                                                                  step, N = 16 lags, real and imaginary part
+//     stage 2 (columns):  S[x][y] += sum_c Re(E[c][x] G[c][y])    M = 16 lags, K = 16 columns (re, im), N = 16 lags
+// Every operand is split into BF16 hi + lo and each product takes three
+// MFMAs (hi*hi + hi*lo + lo*hi, FP32 accumulation): ~16 significant bits per
+// product.  The tables are split once per launch (the row table, read every
+// step, is held in LDS); Q is formed from the FP32 spectra and split in
+// registers per step; the stage-1 accumulators, re-split, are the stage-2 B
+// operand register for register (their column index is the K index, their
+// lag index the N index), so G never leaves the wave.
+//
+// Lane layout (16x16x32 fragments: A lane l holds A[l%16][8(l/16) .. +7],
+// B lane l holds B[8(l/16) .. +7][l%16], C lane l holds C[4(l/16) .. +3][l%16]),
+// g = l / 16, i = l % 16:
+//   stage-1 A: column 16 ct + i, slot 2j + ri <-> row 16 ks + 4g + j (j = 0..3):
+//              one load instruction j reads 4 rows x 16 consecutive columns
+//   stage-1 B (table W): lag i, the same row slots; tile "re" holds
+//              (Re W, -Im W), tile "im" holds (Im W, Re W)
+//   stage-2 A (table E): lag i, slot 2v + ri <-> column 16 ct + 4g + v
+//   stage-2 B: G_re / G_im accumulator element v of lane (g, i) = column 16 ct + 4g + v
+// A crop C <= 16 uses one lag tile per axis (LT = 1), C <= 32 two (LT = 2).
+//
+// Work decomposition: one wave per (particle, DT local directions, PT psi
+// values); a particle's waves are consecutive, four per block, and blocks are
+// mapped so consecutive ones share an XCD (and its L2, where a particle's psi
+// spectra are reused by its directions).  Each wave keeps two operand buffers
+// in registers and loads a step's operands two steps ahead.  The pair tail
+// (first maximum over the crop) runs in the wave's epilogue; the particle
+// tail is particle_tail_kernel.
+#include "common.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int kWaves = 4;
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ floatx4 mfma_bf16(bf16x8 a, bf16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// x = hi + lo, hi = bf16(x), lo = bf16(x - hi), round to nearest even; two
+// values per packed conversion.
+__device__ __forceinline__ void split8(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int e = 0; e < 8; e += 2) {
+    const f32x2 v = {x[e], x[e + 1]};
+    const bf16x2 h = __builtin_convertvector(v, bf16x2);
+    const f32x2 hf = __builtin_convertvector(h, f32x2);
+    const bf16x2 l = __builtin_convertvector(v - hf, bf16x2);
+    hi[e] = h[0];
+    hi[e + 1] = h[1];
+    lo[e] = l[0];
+    lo[e + 1] = l[1];
+  }
+}
+
+__host__ __device__ inline int crop_lag(int i, int n, int C) { return i < C / 2 ? n - C / 2 + i : i - C / 2; }
+
+// One table value split from float64: hi = bf16(v), lo = bf16(v - hi).
+__device__ __forceinline__ void split_d(double v, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)(float)v;
+  lo = (__bf16)(float)(v - (double)(float)hi);
+}
+
+// Split tables in fragment order (hi at o, lo at o + 64).
+//   Wt [nks][2 LT tiles][hi, lo][64 lanes] x 8: tile 2 ty + part, lane (g, i) -> lag 16 ty + i,
+//      slot 2j + ri -> row 16 ks + 4g + j; part 0: (Re W, -Im W), part 1: (Im W, Re W)
+//   Et [nct][LT][hi, lo][64 lanes] x 8: lane (g, i) -> lag 16 tx + i, slot 2v + ri -> column 16 ct + 4g + v,
+//      (Re E, -Im E)
+// Lags beyond C and rows / columns beyond the plane are zero.
+__global__ void split_tables_kernel(int n, int C, int LT, int nks, int nct, bf16x8* __restrict__ Wt,
+                                    bf16x8* __restrict__ Et) {
+  const int m = n / 2 + 1, s = m / 2;
+  const int nw = nks * 2 * LT * 64, ne = nct * LT * 64;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nw + ne; t += gridDim.x * blockDim.x) {
+    const int lane = t & 63, g = lane >> 4, i = lane & 15;
+    double v[8];
+    size_t o;
+    if (t < nw) {
+      const int tile = (t >> 6) % (2 * LT), ks = (t >> 6) / (2 * LT);
+      const int y = (tile >> 1) * 16 + i, part = tile & 1;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = 16 * ks + 4 * g + j;
+        double wr = 0.0, wi = 0.0;
+        if (r < n && y < C) {
+          const int l = crop_lag(y, n, C);
+          const long k = ((long)r * l) % n;
+          double sn, cs;
+          sincospi(2.0 * (double)k / (double)n, &sn, &cs);
+          const double sg = (l & 1) ? -1.0 : 1.0;
+          wr = sg * cs;
+          wi = sg * sn;
+        }
+        v[2 * j] = part == 0 ? wr : wi;
+        v[2 * j + 1] = part == 0 ? -wi : wr;
+      }
+      o = ((size_t)(ks * 2 * LT + tile) * 2) * 64 + lane;
+    } else {
+      const int u = t - nw;
+      const int tx = (u >> 6) % LT, ct = (u >> 6) / LT;
+      const int x = tx * 16 + i;
+#pragma unroll
+      for (int vv = 0; vv < 4; ++vv) {
+        const int c = 16 * ct + 4 * g + vv;
+        double er = 0.0, ei = 0.0;
+        if (c < m && x < C) {
+          const int l = crop_lag(x, n, C);
+          const int pos = (c + s) % m;
+          const double w = (pos == 0 || pos == n / 2) ? 1.0 : 2.0;
+          const long k = ((long)pos * l) % n;
+          double sn, cs;
+          sincospi(2.0 * (double)k / (double)n, &sn, &cs);
+          er = w * cs;
+          ei = w * sn;
+        }
+        v[2 * vv] = er;
+        v[2 * vv + 1] = -ei;
+      }
+      o = ((size_t)(ct * LT + tx) * 2) * 64 + lane;
+    }
+    bf16x8* dst = t < nw ? Wt : Et;
+    bf16x8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      __bf16 h, l;
+      split_d(v[e], h, l);
+      hi[e] = h;
+      lo[e] = l;
+    }
+    dst[o] = hi;
+    dst[o + 64] = lo;
+  }
+}
+
+// unit_off[k] = sum over earlier particles (in processing order) of
+// ceil(M_b / DT); unit_off[B] = the total.  One 1024-thread block.
+__global__ void __launch_bounds__(1024)
+split_units_kernel(const int* __restrict__ dir_off, const int* __restrict__ order, int B, int DT,
+                   int* __restrict__ unit_off) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (B + 1023) / 1024;
+  const int k0 = min(t * chunk, B), k1 = min(k0 + chunk, B);
+  int s = 0;
+  for (int k = k0; k < k1; ++k) {
+    const int b = order ? order[k] : k;
+    s += (dir_off[b + 1] - dir_off[b] + DT - 1) / DT;
+  }
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int k = k0; k < k1; ++k) {
+    unit_off[k] = run;
+    const int b = order ? order[k] : k;
+    run += (dir_off[b + 1] - dir_off[b] + DT - 1) / DT;
+  }
+  if (t == 1023) unit_off[B] = part[1023];
+}
+
+// The step operands of one lane: 4 complex values (rows 16 ks + 4g + j of
+// column 16 ct + i) of each of the wave's DT template rows and PT psi spectra.
+template <int DT, int PT>
+struct Frags {
+  float2 t[DT][4];
+  float2 p[PT][4];
+};
+
+template <int LT, int DT, int PT>
+__global__ void __launch_bounds__(64 * kWaves)
+split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spectra,
+                       const float* __restrict__ inv_norm, const float* __restrict__ tinv,
+                       const int* __restrict__ dir_off, const int* __restrict__ dir_idx,
+                       const int* __restrict__ pmap, const int* __restrict__ order,
+                       const int* __restrict__ unit_off, int B, int P, int n, int C, int nblk,
+                       const bf16x8* __restrict__ Wt, const bf16x8* __restrict__ Et, float* __restrict__ scores,
+                       int* __restrict__ peak_y, int* __restrict__ peak_x) {
+  // the row table (read every step) lives in LDS; the column table (read
+  // once per column tile) is read from global memory (cached)
+  extern __shared__ bf16x8 tab[];
+  const int m = n / 2 + 1, nct = (m + 15) >> 4;
+  const int nks = (((n + 15) >> 4) + 1) & ~1;  // row steps, padded to even (zero table rows)
+  const int nw = nks * 4 * LT * 64;
+  for (int e = threadIdx.x; e < nw; e += 64 * kWaves) tab[e] = Wt[e];
+  __syncthreads();
+  const bf16x8* Wl = tab;
+
+  // consecutive logical blocks on one XCD (blocks are dealt to the XCDs round robin)
+  const int per = (int)(gridDim.x / kXcds);
+  const int lb = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  const int lane = threadIdx.x & 63;
+  // wave-uniform from here on: unit, particle and row pointers live in SGPRs
+  const int u = __builtin_amdgcn_readfirstlane(lb * kWaves + (int)(threadIdx.x >> 6));
+  const int total = unit_off[B];
+  if (lb >= nblk || u >= total) return;  // no barrier below
+  int lo = 0, hi = B;  // unit_off[lo] <= u < unit_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (unit_off[mid] <= u) lo = mid; else hi = mid;
+  }
+  const int b = order ? order[lo] : lo;
+  const int d0 = dir_off[b], Mi = dir_off[b + 1] - d0;
+  const int j0 = (u - unit_off[lo]) * DT;
+  const int p0 = (int)blockIdx.y * PT;
+  const int pb = pmap ? pmap[b] : b;
+  const size_t plane = (size_t)n * m;
+  const float2* tp[DT];
+  const float2* pp[PT];
+#pragma unroll
+  for (int d = 0; d < DT; ++d) tp[d] = bank + (size_t)dir_idx[d0 + min(j0 + d, Mi - 1)] * plane;
+#pragma unroll
+  for (int p = 0; p < PT; ++p) pp[p] = spectra + ((size_t)pb * P + min(p0 + p, P - 1)) * plane;
+  const int g = lane >> 4, i = lane & 15;
+
+  // Operands of step (ct, ks).  Loads are unconditional: an offset past the
+  // plane (rows >= n, columns >= n/2+1) is clamped to its last element, and
+  // the tables hold zeros at those rows and columns, so the (finite) values
+  // read there contribute exactly nothing.
+  const int last = (int)plane - 1;
+  auto load = [&](int ct, int ks, Frags<DT, PT>& f) {
+    const int base = (16 * ks + 4 * g) * m + 16 * ct + i;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int off = min(base + j * m, last);
+#pragma unroll
+      for (int d = 0; d < DT; ++d) f.t[d][j] = tp[d][off];
+#pragma unroll
+      for (int p = 0; p < PT; ++p) f.p[p][j] = pp[p][off];
+    }
+  };
+
+  floatx4 out[DT][PT][LT * LT], gacc[DT][PT][2 * LT];
+#pragma unroll
+  for (int d = 0; d < DT; ++d)
+#pragma unroll
+    for (int p = 0; p < PT; ++p)
+#pragma unroll
+      for (int t = 0; t < LT * LT; ++t) out[d][p][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  // stage 1 of one step from buffer f, then f is refilled with step (ct2, ks2)
+  auto step = [&](int ks, Frags<DT, PT>& f, int ct2, int ks2) {
+    bf16x8 bh[2 * LT], bl[2 * LT];
+#pragma unroll
+    for (int t = 0; t < 2 * LT; ++t) {
+      bh[t] = Wl[((ks * 2 * LT + t) * 2) * 64 + lane];
+      bl[t] = Wl[((ks * 2 * LT + t) * 2 + 1) * 64 + lane];
+    }
+#pragma unroll
+    for (int d = 0; d < DT; ++d) {
+#pragma unroll
+      for (int p = 0; p < PT; ++p) {
+        float x[8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float2 a = f.t[d][j], q = f.p[p][j];
+          x[2 * j] = a.x * q.x - a.y * q.y;
+          x[2 * j + 1] = a.x * q.y + a.y * q.x;
+        }
+        bf16x8 ahi, alo;
+        split8(x, ahi, alo);
+#pragma unroll
+        for (int t = 0; t < 2 * LT; ++t) {
+          gacc[d][p][t] = mfma_bf16(ahi, bh[t], gacc[d][p][t]);
+          gacc[d][p][t] = mfma_bf16(ahi, bl[t], gacc[d][p][t]);
+          gacc[d][p][t] = mfma_bf16(alo, bh[t], gacc[d][p][t]);
+        }
+      }
+    }
+    load(ct2, ks2, f);
+  };
+
+  // two buffers, loaded two steps ahead; nks is even, so buffer a always
+  // holds the even steps and b the odd ones, across column tiles too
+  Frags<DT, PT> fa, fb;
+  load(0, 0, fa);
+  load(0, 1, fb);
+  for (int ct = 0; ct < nct; ++ct) {
+    const int ctn = min(ct + 1, nct - 1);
+#pragma unroll
+    for (int d = 0; d < DT; ++d)
+#pragma unroll
+      for (int p = 0; p < PT; ++p)
+#pragma unroll
+        for (int t = 0; t < 2 * LT; ++t) gacc[d][p][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < nks; ks += 2) {
+      const bool wrap = ks + 2 >= nks;
+      step(ks, fa, wrap ? ctn : ct, wrap ? 0 : ks + 2);
+      step(ks + 1, fb, wrap ? ctn : ct, wrap ? 1 : ks + 3);
+    }
+    // stage 2: S += Re(E^T G) over this column tile
+    bf16x8 eh[LT], el[LT];
+#pragma unroll
+    for (int tx = 0; tx < LT; ++tx) {
+      eh[tx] = Et[((ct * LT + tx) * 2) * 64 + lane];
+      el[tx] = Et[((ct * LT + tx) * 2 + 1) * 64 + lane];
+    }
+#pragma unroll
+    for (int d = 0; d < DT; ++d) {
+#pragma unroll
+      for (int p = 0; p < PT; ++p) {
+#pragma unroll
+        for (int ty = 0; ty < LT; ++ty) {
+          float x[8];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            x[2 * v] = gacc[d][p][2 * ty][v];
+            x[2 * v + 1] = gacc[d][p][2 * ty + 1][v];
+          }
+          bf16x8 gh, gl;
+          split8(x, gh, gl);
+#pragma unroll
+          for (int tx = 0; tx < LT; ++tx) {
+            floatx4 o = out[d][p][tx * LT + ty];
+            o = mfma_bf16(eh[tx], gh, o);
+            o = mfma_bf16(eh[tx], gl, o);
+            o = mfma_bf16(el[tx], gh, o);
+            out[d][p][tx * LT + ty] = o;
+          }
+        }
+      }
+    }
+  }
+
+  // pair tail: first maximum over the crop in y * C + x order; lane (g, i)
+  // holds S[x = 16 tx + 4g + v][y = 16 ty + i]
+#pragma unroll
+  for (int d = 0; d < DT; ++d) {
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+      if (j0 + d >= Mi || p0 + p >= P) continue;  // wave-uniform
+      const int dir = dir_idx[d0 + j0 + d];
+      const float sc = 1024.0f * (inv_norm ? inv_norm[(size_t)pb * P + p0 + p] : 1.0f) * (tinv ? tinv[dir] : 1.0f);
+      float bv = -INFINITY;
+      int bi = 0x7fffffff;
+#pragma unroll
+      for (int tx = 0; tx < LT; ++tx)
+#pragma unroll
+        for (int ty = 0; ty < LT; ++ty)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int x = 16 * tx + 4 * g + v, y = 16 * ty + i;
+            if (y < C && x < C) {
+              const float val = out[d][p][tx * LT + ty][v] * sc;
+              const int idx = y * C + x;
+              if (val > bv || (val == bv && idx < bi)) { bv = val; bi = idx; }
+            }
+          }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(bv, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      if (lane == 0) {
+        const size_t gidx = (size_t)(d0 + j0 + d) * P + p0 + p;
+        scores[gidx] = bv;
+        peak_y[gidx] = bi / C;
+        peak_x[gidx] = bi % C;
+      }
+    }
+  }
+}
+
+inline int split_lt(int C) { return C <= 16 ? 1 : 2; }
+inline int split_nct(int n) { return (n / 2 + 1 + 15) / 16; }
+inline int split_nks(int n) { return (((n + 15) / 16) + 1) & ~1; }
+inline size_t split_w_frags(int n, int C) { return (size_t)split_nks(n) * 4 * split_lt(C) * 64; }
+inline size_t split_e_frags(int n, int C) { return (size_t)split_nct(n) * 2 * split_lt(C) * 64; }
+inline size_t split_lds_bytes(int n, int C) { return split_w_frags(n, C) * sizeof(bf16x8); }
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+template <int LT, int DT, int PT>
+int launch_split(dim3 grid, size_t lds, hipStream_t st, const float2* bank, const float2* spectra,
+                 const float* inv_norm, const float* tinv, const int* dir_off, const int* dir_idx, const int* pmap,
+                 const int* order, const int* unit_off, int B, int P, int n, int C, int nblk, const bf16x8* Wt,
+                 const bf16x8* Et, float* scores, int* peak_y, int* peak_x) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)split_correlate_kernel<LT, DT, PT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)split_lds_bytes(256, LT == 1 ? 16 : 32));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      gis_set_error("gis_correlate_split: cannot reserve LDS: %s", hipGetErrorString(e));
+      return (int)e;
+    }
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((split_correlate_kernel<LT, DT, PT>), grid, dim3(64 * kWaves), lds, st, bank, spectra,
+                     inv_norm, tinv, dir_off, dir_idx, pmap, order, unit_off, B, P, n, C, nblk, Wt, Et, scores,
+                     peak_y, peak_x);
+  return GIS_OK;
+}
+
+}  // namespace
+
+GIS_API size_t gis_correlate_split_workspace(int n, int C, int B) {
+  if (n < 2 || C < 2 || B < 0) return 0;
+  return align256((split_w_frags(n, C) + split_e_frags(n, C)) * sizeof(bf16x8)) +
+         align256((size_t)(B + 1) * sizeof(int));
+}
+
+// Full-plane correlate_templates_and_particle (v606:1418-1449) plus the
+// numerics of make_result_records (v606:1452-1491) on split-BF16 MFMA; the
+// arguments and outputs of gis_correlate, plus
+//   order  [B] processing order of the particles (a permutation) or NULL
+//   nrows  dir_off[B], the number of dir_idx entries
+//   dt     directions per wave (1 or 2; 0 = automatic)
+//   pt     psi values per wave (1..5, <= 3 with dt 2; 0 = automatic)
+GIS_API int gis_correlate_split(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
+                                const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B,
+                                int P, int n, int C, int nrows, int dt, int pt, float* scores, int* peak_y,
+                                int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
+                                size_t ws_bytes, void* stream) {
+  GIS_REQUIRE(B >= 0 && P >= 1 && nrows >= 0, "gis_correlate_split: B=%d P=%d nrows=%d", B, P, nrows);
+  GIS_REQUIRE(n >= 4 && n % 2 == 0 && n <= 256, "gis_correlate_split: newbox %d must be even, in [4, 256]", n);
+  if (C > 32 || C < 2 || C % 2 != 0 || C > n) {
+    gis_set_error("gis_correlate_split: crop %d must be even, in [2, 32] and <= newbox", C);
+    return GIS_EUNSUPPORTED;
+  }
+  GIS_REQUIRE(dt >= 0 && dt <= 2, "gis_correlate_split: dt %d not in [0, 2]", dt);
+  GIS_REQUIRE(pt >= 0 && pt <= 5, "gis_correlate_split: pt %d not in [0, 5]", pt);
+  if (B == 0) return GIS_OK;
+  GIS_REQUIRE(bank && spectra && dir_off && dir_idx && scores && peak_y && peak_x && best && peak && mean && stdv &&
+                  ws,
+              "gis_correlate_split: null pointer");
+  GIS_REQUIRE(ws_bytes >= gis_correlate_split_workspace(n, C, B), "gis_correlate_split: workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  const int LT = split_lt(C);
+  if (pt == 0) pt = std::min(P, 5);
+  if (dt == 0 || LT == 2) dt = 1;
+  if (dt == 2) pt = std::min(pt, 3);  // register budget of two directions per wave
+  bf16x8* Wt = (bf16x8*)ws;
+  bf16x8* Et = Wt + split_w_frags(n, C);
+  int* unit_off = (int*)((char*)ws + align256((split_w_frags(n, C) + split_e_frags(n, C)) * sizeof(bf16x8)));
+  hipLaunchKernelGGL(split_tables_kernel, dim3(64), dim3(256), 0, st, n, C, LT, split_nks(n), split_nct(n), Wt, Et);
+  GIS_LAUNCH_CHECK("gis_correlate_split tables");
+  hipLaunchKernelGGL(split_units_kernel, dim3(1), dim3(1024), 0, st, dir_off, order, B, dt, unit_off);
+  GIS_LAUNCH_CHECK("gis_correlate_split units");
+  if (nrows > 0) {
+    // units <= sum_b ceil(M_b / dt) <= (nrows + B (dt - 1)) / dt
+    const long bound = ((long)nrows + (long)B * (dt - 1) + dt - 1) / dt;
+    const int nblk = (int)((bound + kWaves - 1) / kWaves);
+    const int grid_x = (nblk + kXcds - 1) / kXcds * kXcds;
+    const dim3 grid((unsigned)grid_x, (unsigned)((P + pt - 1) / pt));
+    const size_t lds = split_lds_bytes(n, C);
+    int rc = GIS_OK;
+#define GIS_SPLIT_ARGS                                                                                          \
+  grid, lds, st, (const float2*)bank, (const float2*)spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order,    \
+      unit_off, B, P, n, C, nblk, Wt, Et, scores, peak_y, peak_x
+#define GIS_SPLIT_PT5(LTV, DTV)                                          \
+  switch (pt) {                                                         \
+    case 1: rc = launch_split<LTV, DTV, 1>(GIS_SPLIT_ARGS); break;      \
+    case 2: rc = launch_split<LTV, DTV, 2>(GIS_SPLIT_ARGS); break;      \
+    case 3: rc = launch_split<LTV, DTV, 3>(GIS_SPLIT_ARGS); break;      \
+    case 4: rc = launch_split<LTV, DTV, 4>(GIS_SPLIT_ARGS); break;      \
+    default: rc = launch_split<LTV, DTV, 5>(GIS_SPLIT_ARGS); break;     \
+  }
+#define GIS_SPLIT_PT3(LTV, DTV)                                         \
+  switch (pt) {                                                         \
+    case 1: rc = launch_split<LTV, DTV, 1>(GIS_SPLIT_ARGS); break;      \
+    case 2: rc = launch_split<LTV, DTV, 2>(GIS_SPLIT_ARGS); break;      \
+    default: rc = launch_split<LTV, DTV, 3>(GIS_SPLIT_ARGS); break;     \
+  }
+    if (LT == 2) {
+      GIS_SPLIT_PT5(2, 1)
+    } else if (dt == 1) {
+      GIS_SPLIT_PT5(1, 1)
+    } else {
+      GIS_SPLIT_PT3(1, 2)
+    }
+#undef GIS_SPLIT_PT5
+#undef GIS_SPLIT_PT3
+#undef GIS_SPLIT_ARGS
+    if (rc != GIS_OK) return rc;
+    GIS_LAUNCH_CHECK("gis_correlate_split");
+  }
+  hipLaunchKernelGGL(particle_tail_kernel, dim3(B), dim3(64), 0, st, dir_off, P, scores, best, peak, mean, stdv);
+  GIS_LAUNCH_CHECK("gis_correlate_split tail");
+  return GIS_OK;
+}

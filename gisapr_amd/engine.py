@@ -37,6 +37,7 @@ class ParticleSet:
     inv_norm: Optional[torch.Tensor] = None  # [B, P] float32
     coeffs: Optional[torch.Tensor] = None    # [B*P, Kpad] complex64, unit norm (small-shift search)
     row_order: Optional[torch.Tensor] = None # [sum M_i] int32 processing order of the small-shift search
+    particle_order: Optional[torch.Tensor] = None  # [B] int32 processing order of the split search
 
     @property
     def B(self) -> int:
@@ -54,6 +55,15 @@ class ParticleSet:
         t = [self.images, self.rotp, self.dir_off, self.dir_idx, self.spectra, self.inv_norm, self.defocus,
              self.coeffs]
         return sum(x.numel() * x.element_size() for x in t if x is not None)
+
+
+def particle_order(dir_off: np.ndarray, dir_idx: np.ndarray, device) -> torch.Tensor:
+    """Particles sorted by their first local direction (int32 [B] on ``device``):
+    neighbours in this order share most of their template rows."""
+    off = np.asarray(dir_off, np.int64)
+    idx = np.asarray(dir_idx, np.int64)
+    first = np.where(off[1:] > off[:-1], idx[np.minimum(off[:-1], max(len(idx) - 1, 0))], -1)
+    return torch.from_numpy(np.argsort(first, kind="stable").astype(np.int32)).to(device)
 
 
 @dataclass
@@ -187,6 +197,20 @@ class SearchEngine:
         # dir_off / dir_idx were built on the host against this direction table
         corr = kernels.correlate(bank, ps.spectra, ps.inv_norm, ps.dir_off, ps.dir_idx, self.crop,
                                  validate=False, tinv=tinv)
+        return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.crop // 2)
+
+    def search_split(self, bank: torch.Tensor, ps: ParticleSet, tinv: Optional[torch.Tensor] = None,
+                     dt: int = 0, pt: int = 0) -> SearchResult:
+        """The search of ``search`` (every coefficient, every lag of the crop)
+        on split-bf16 MFMA; particles are processed in orientation order."""
+        if ps.spectra is None:
+            self.prepare(ps)
+        if int(bank.shape[0]) != len(self.dir_rot):
+            raise ValueError("template bank does not cover the engine's direction table")
+        if ps.particle_order is None:
+            ps.particle_order = particle_order(ps.dir_off.cpu().numpy(), ps.dir_idx.cpu().numpy(), self.device)
+        corr = kernels.correlate_split(bank, ps.spectra, ps.inv_norm, ps.dir_off, ps.dir_idx, self.crop,
+                                       validate=False, tinv=tinv, order=ps.particle_order, dt=dt, pt=pt)
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.crop // 2)
 
     # ---------------------------------------------------------------- small-shift precision search

@@ -212,6 +212,30 @@ class CTFTables:
         return np.asarray([self.k1, self.k2, self.k3, self.k5, self.kk], np.float32)
 
 
+def crop_functional(n: int, crop: int):
+    """The v606 crop as a separable linear functional of the UNshifted packed
+    product Q = T * P (correlate_templates_and_particle, v606:1418-1449 with
+    the pre-shift of v606:535-537 folded in):
+
+        score[y][x] = 1024 * Re( (W^T Q E)[y][x] ),
+        W[r][y] = (-1)^l_y exp(2 pi i r l_y / n),
+        E[c][x] = w(pos_c) exp(2 pi i pos_c l_x / n),  pos_c = (c + (n/2+1)//2) mod (n/2+1),
+
+    w = 1 for pos in {0, n/2} (C2R drops the imaginary part of those
+    columns) else 2, l the crop lags [n-C/2 .. n-1, 0 .. C/2-1].  Returns
+    complex128 W [n, C] and E [n/2+1, C]."""
+    n, crop = int(n), int(crop)
+    m = n // 2 + 1
+    h = crop // 2
+    lags = np.concatenate((np.arange(n - h, n), np.arange(0, h)))
+    pos = (np.arange(m) + m // 2) % m
+    w = np.where((pos == 0) | (pos == n // 2), 1.0, 2.0)
+    sign = np.where(lags % 2 == 0, 1.0, -1.0)
+    W = np.exp(2j * np.pi * (np.outer(np.arange(n), lags) % n) / n) * sign[None, :]
+    E = np.exp(2j * np.pi * (np.outer(pos, lags) % n) / n) * w[:, None]
+    return W, E
+
+
 class SmallShiftTables:
     """Coefficient set and split lag table of the small-shift precision search.
 

@@ -444,6 +444,62 @@ def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torc
     return CorrelationResult(scores, py, px, best, peak, mean, std)
 
 
+def correlate_split(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torch.Tensor],
+                    dir_off: torch.Tensor, dir_idx: torch.Tensor, crop: int, validate: bool = True,
+                    pmap: Optional[torch.Tensor] = None, tinv: Optional[torch.Tensor] = None,
+                    order: Optional[torch.Tensor] = None, dt: int = 0, pt: int = 0) -> CorrelationResult:
+    """The v606 search of ``correlate`` (every packed coefficient, every lag of
+    the crop, crop <= 32) on split-bf16 MFMA.  ``order`` is the particles'
+    processing order (a permutation of the B searched particles, e.g. by
+    orientation so neighbouring particles share template rows in cache);
+    ``dt`` / ``pt`` are the directions / psi values per wave (0 = automatic;
+    results do not depend on them)."""
+    require_hip(bank, "bank")
+    require_hip(spectra, "spectra")
+    P, n = int(spectra.shape[1]), int(spectra.shape[2])
+    B = int(dir_off.shape[0]) - 1 if pmap is not None else int(spectra.shape[0])
+    if tuple(bank.shape[1:]) != (n, n // 2 + 1):
+        raise ValueError("bank and spectra layouts differ")
+    if dir_off.dtype != torch.int32 or dir_idx.dtype != torch.int32:
+        raise ValueError("dir_off / dir_idx must be int32")
+    if tuple(dir_off.shape) != (B + 1,):
+        raise ValueError("dir_off must be [B+1]")
+    total = int(dir_idx.shape[0])
+    if order is not None:
+        require_hip(order, "order")
+        if order.dtype != torch.int32 or tuple(order.shape) != (B,):
+            raise ValueError("order must be int32 [B]")
+    if validate:
+        off = dir_off.cpu().numpy()
+        if off[0] != 0 or off[-1] != total or np.any(np.diff(off) < 0):
+            raise ValueError("dir_off must start at 0, be non-decreasing and end at len(dir_idx)")
+        if total and (int(dir_idx.min()) < 0 or int(dir_idx.max()) >= int(bank.shape[0])):
+            raise ValueError("dir_idx out of range of the template bank")
+        _check_pmap(pmap, B, int(spectra.shape[0]))
+        if order is not None and not torch.equal(torch.sort(order.long()).values,
+                                                 torch.arange(B, device=order.device)):
+            raise ValueError("order must be a permutation of the particles")
+    if tinv is not None:
+        require_hip(tinv, "tinv")
+        if tinv.dtype != torch.float32 or tinv.numel() != int(bank.shape[0]):
+            raise ValueError("tinv must be float32 with one value per bank row")
+    dev = spectra.device
+    scores = torch.empty((total, P), dtype=torch.float32, device=dev)
+    py = torch.empty((total, P), dtype=torch.int32, device=dev)
+    px = torch.empty((total, P), dtype=torch.int32, device=dev)
+    best = torch.empty((B,), dtype=torch.int32, device=dev)
+    peak = torch.empty((B,), dtype=torch.float32, device=dev)
+    mean = torch.empty((B,), dtype=torch.float64, device=dev)
+    std = torch.empty((B,), dtype=torch.float64, device=dev)
+    wsb = int(lib().gis_correlate_split_workspace(n, int(crop), B))
+    ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dev)
+    check(lib().gis_correlate_split(ptr(bank), ptr(spectra), ptr(inv_norm), ptr(tinv), ptr(dir_off), ptr(dir_idx),
+                                    ptr(pmap), ptr(order), B, P, n, int(crop), total, int(dt), int(pt),
+                                    ptr(scores), ptr(py), ptr(px), ptr(best), ptr(peak), ptr(mean), ptr(std),
+                                    ptr(ws), wsb, stream_of(dev)), "gis_correlate_split")
+    return CorrelationResult(scores, py, px, best, peak, mean, std)
+
+
 def score_sum(peak: torch.Tensor, best: Optional[torch.Tensor] = None) -> torch.Tensor:
     """sum exp(peak/1024) over particles with best >= 0, as a float64 device scalar."""
     require_hip(peak, "peak")

@@ -256,6 +256,32 @@ int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, 
                   int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
                   size_t ws_bytes, void* stream);
 
+/* Workspace bytes gis_correlate_split needs at newbox n, crop C, B particles. */
+size_t gis_correlate_split_workspace(int n, int C, int B);
+
+/*
+ * (d) The search of gis_correlate -- the full v606 operator: every packed
+ * coefficient, every lag of the C x C crop -- on split-BF16 MFMA (operands
+ * split into bf16 hi + lo, hi*hi + hi*lo + lo*hi, FP32 accumulation; ~16
+ * significant bits per product).  Replaces correlate_templates_and_particle
+ * (v606:1418-1449) and the numerics of make_result_records (v606:1452-1491)
+ * like gis_correlate, with the same arguments and outputs, plus
+ *   order  [B] processing order of the searched particles (a permutation) or
+ *          NULL: particles ranked by orientation share template rows in cache
+ *   nrows  dir_off[B], the number of dir_idx entries
+ *   dt     local directions per wave (1 or 2), or 4: one block of four
+ *          waves per four directions of a particle, sharing its psi spectra
+ *          through LDS; 0 = automatic
+ *   pt     psi values per wave (1..5, <= 3 with dt 2; 0 = automatic).  dt and
+ *          pt are autotuner knobs: results do not depend on them
+ * C (--ccgCropSize) must be even and <= 32 (the reference default is 32,
+ * v606:1562); n <= 256.
+ */
+int gis_correlate_split(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
+                        const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B, int P,
+                        int n, int C, int nrows, int dt, int pt, float* scores, int* peak_y, int* peak_x,
+                        int* best, float* peak, double* mean, double* stdv, void* ws, size_t ws_bytes, void* stream);
+
 /*
  * Cross-particle conformation score sum_b exp(peak_b / 1024) in float64 over
  * particles with best_b >= 0 (best may be NULL).  Replaces
