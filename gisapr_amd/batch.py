@@ -528,11 +528,11 @@ class RigidDensityCache:
             sel = first.chains == chain
             if not sel.any():
                 raise ValueError(f"Chain {chain!r} has no atoms in the first PDB model")
-            moving_lines = first.atom_line_indices[sel]
+            moving_lines = first.line_numbers[sel]
             self.moving_coords = first.coords[sel]
             positions = {int(line): i for i, line in enumerate(moving_lines)}
-            moving = np.isin(dens.atom_line_indices, moving_lines)
-            self.transform_rows = np.asarray([positions[int(line)] for line in dens.atom_line_indices[moving]],
+            moving = np.isin(dens.line_numbers, moving_lines)
+            self.transform_rows = np.asarray([positions[int(line)] for line in dens.line_numbers[moving]],
                                              dtype=np.int64)
             numbers, _ = atoms_to_atomic_numbers_and_masses(dens.elements)
             self.moving_numbers = numbers[moving]

@@ -33,21 +33,25 @@ def splat_parameters(apix: float, res: float):
 
 
 def _hip_device(device):
+    """The HIP device to splat on.  The signatures keep p2m's default
+    device='cpu' (p2m:41), but this engine has no CPU path: a call that does
+    not name a HIP device (e.g. device='cuda:0') is refused."""
     import torch
     dev = torch.device(device)
     if dev.type != "cuda":
-        raise ValueError(f"gisapr_amd computes densities on HIP devices only (got {dev})")
+        raise ValueError(f"gisapr_amd splats on HIP devices only: pass device='cuda' or 'cuda:N' (got {dev})")
     if dev.index is None:
         dev = torch.device("cuda", torch.cuda.current_device())
     return dev
 
 
-def density_from_coords(coordinates, atomic_numbers, boxsize, apix, res, device="cuda",
+def density_from_coords(coordinates, atomic_numbers, boxsize, apix, res, device="cpu",
                         max_chunk_voxels=524288, _initial_volume=None, backend="hip",
                         backend_metrics=None):
     """Radial-exponential density of atoms as a float32 [box, box, box] tensor ([z, y, x]).
 
-    Same validation and rule as p2m:41-98.  ``_initial_volume`` (the zip
+    Same validation and rule as p2m:41-98; ``device`` must name a HIP device
+    (the default 'cpu' of p2m:41 is refused).  ``_initial_volume`` (the zip
     release's fixed-body reuse) is cloned and the atoms are added on top.
     ``max_chunk_voxels`` is accepted for signature compatibility; the kernel
     needs no chunking.
@@ -97,7 +101,7 @@ def yflip_vectorized(arr):
     return arr.index_select(1, torch.as_tensor(idx, device=arr.device))
 
 
-def generate_volume(pdb_path, boxsize, apix, res, device="cuda", do_center=False, do_yflip=False,
+def generate_volume(pdb_path, boxsize, apix, res, device="cpu", do_center=False, do_yflip=False,
                     include_hetatm=False, source_pdb=None, chain_id=None, pose=None):
     """Density tensor of a PDB file, or of ``source_pdb`` with ``chain_id`` moved by ``pose``."""
     from .pdb_text import read_pdb, transformed_density_atoms
@@ -117,7 +121,7 @@ def generate_volume(pdb_path, boxsize, apix, res, device="cuda", do_center=False
     return vol
 
 
-def generate_mrc(pdb_path, output, boxsize, apix, res, device="cuda", do_center=False, do_yflip=False,
+def generate_mrc(pdb_path, output, boxsize, apix, res, device="cpu", do_center=False, do_yflip=False,
                  include_hetatm=False):
     """Write the density as MRC (returns 0) or return it as a numpy array when output is None."""
     vol = generate_volume(pdb_path, boxsize, apix, res, device, do_center, do_yflip, include_hetatm)
@@ -127,21 +131,6 @@ def generate_mrc(pdb_path, output, boxsize, apix, res, device="cuda", do_center=
     from .mrc import write_mrc
     write_mrc(output, data, voxel_size=float(apix))
     return 0
-
-
-def _round_away_from_zero(value):
-    """sign(v) floor(|v| + 0.5) in float64, as int64 (p2m:35-38)."""
-    import torch
-    v = torch.as_tensor(value).to(torch.float64)
-    return (torch.sign(v) * torch.floor(torch.abs(v) + 0.5)).to(torch.int64)
-
-
-def _voxel_offsets(width, device_string):
-    """[(2w+1)^3, 3] int64 (x, y, z) offsets of a splat window (p2m:26-32)."""
-    import torch
-    axis = torch.arange(2 * int(width) + 1, device=device_string, dtype=torch.int64)
-    zz, yy, xx = torch.meshgrid(axis, axis, axis, indexing="ij")
-    return torch.stack((xx.reshape(-1), yy.reshape(-1), zz.reshape(-1)), dim=1)
 
 
 def create_parser():

@@ -1,18 +1,23 @@
-"""Per-particle Top-K direction banks (the zip release's calibration mode,
-ZL orientation_bank_v1.py; used by run_search's ``orientation_rows`` /
-``calibrate`` and ``--orientations-in`` / ``--orientations-out``).
+"""Per-particle Top-K direction banks for calibrated searches.
 
-A calibration search ranks, for every particle, its local directions by the
-best score over psi and keeps the first ``keep`` distinct (rot, tilt) pairs;
-later searches score only those directions.  The bank is JSON:
+A calibration search ranks every particle's local directions by their best
+score over psi and stores the first ``keep`` directions with distinct
+(rot, tilt); later searches score only those.  Used by run_search's
+``orientation_rows`` / ``calibrate`` arguments and ``--orientations-in`` /
+``--orientations-out`` flags (the zip release's calibration mode).
 
-    {"settings": {...inputs and search settings...},
-     "rows": {"<STAR line>": {"image": "...", "directions": [[i, rot, tilt, score], ...]}},
-     "mrc": "<model path>", "sha256": "<digest of everything but the digest>"}
+On disk the bank is one JSON object (this layout and the digest rule are a
+file format shared with the zip release, so banks move between the two):
 
-and is refused when its digest or settings do not match the current inputs.
-The format (field names, digest rule) follows ZL orientation_bank_v1.py so
-banks written by either implementation read in the other.
+    {"settings": {"schema", "particle_sha256", "angle_sha256", "keep", "local",
+                  "local_range", "psi_step"},
+     "rows": {"<STAR line>": {"image": "<image name>",
+                              "directions": [[index, rot, tilt, score], ...]}},
+     ...origin fields..., "sha256": "<digest>"}
+
+The digest is SHA-256 of the object without "sha256", serialised as JSON
+with sorted keys and no whitespace.  A bank is refused when its digest or
+its settings do not match the current inputs.
 """
 from __future__ import annotations
 
@@ -21,9 +26,9 @@ import hashlib
 import json
 import math
 import os
+import tempfile
 from functools import lru_cache
 from pathlib import Path
-from tempfile import NamedTemporaryFile
 from typing import Dict, List, Sequence
 
 import numpy as np
@@ -34,125 +39,127 @@ __all__ = ["positive_int", "file_key", "file_hash", "settings", "pack_bank", "ba
            "restricted_indices", "ranked_row", "atomic_json", "SCHEMA"]
 
 
-def positive_int(value) -> int:
-    """argparse type: an integer >= 1 (ZL orientation_bank_v1.py:14-18)."""
-    number = int(value)
-    if number < 1:
-        raise argparse.ArgumentTypeError("must be >= 1")
-    return number
+def positive_int(text) -> int:
+    """argparse type of --keep-orientations: an integer of at least 1."""
+    value = int(text)
+    if value >= 1:
+        return value
+    raise argparse.ArgumentTypeError(f"{text!r}: must be >= 1")
 
 
 def file_key(path) -> tuple:
-    """Identity of a file's current contents: resolved path, size, mtime and
-    ctime in ns (ZL gisapr_cache.py:9-12)."""
-    p = Path(path).expanduser().resolve()
-    st = p.stat()
-    return str(p), st.st_size, st.st_mtime_ns, st.st_ctime_ns
+    """What identifies a file's current contents without reading it: resolved
+    path, size, modification and change times (ns)."""
+    resolved = Path(path).expanduser().resolve()
+    info = resolved.stat()
+    return str(resolved), info.st_size, info.st_mtime_ns, info.st_ctime_ns
 
 
 @lru_cache(maxsize=16)
-def _sha256_of(identity: tuple) -> str:
+def _digest_of_file(key: tuple) -> str:
     h = hashlib.sha256()
-    with open(identity[0], "rb") as fh:
-        for block in iter(lambda: fh.read(1 << 20), b""):
-            h.update(block)
-    return h.hexdigest()
+    with open(key[0], "rb") as fh:
+        while True:
+            chunk = fh.read(1 << 20)
+            if not chunk:
+                return h.hexdigest()
+            h.update(chunk)
 
 
 def file_hash(path) -> str:
-    """SHA-256 of a file, cached per identity (ZL orientation_bank_v1.py:39-45)."""
-    return _sha256_of(file_key(path))
+    """SHA-256 of a file's bytes, computed once per file_key."""
+    return _digest_of_file(file_key(path))
 
 
 def settings(particles, angles, keep: int, local: bool, angular_range: float, psi_step: float) -> Dict:
-    """The inputs and search settings a bank is valid for (:48-51)."""
-    return {"schema": SCHEMA, "particle_sha256": file_hash(particles), "angle_sha256": file_hash(angles),
-            "keep": int(keep), "local": bool(local), "local_range": float(angular_range),
-            "psi_step": float(psi_step)}
+    """The inputs and search settings a bank belongs to."""
+    return dict(schema=SCHEMA, particle_sha256=file_hash(particles), angle_sha256=file_hash(angles),
+                keep=int(keep), local=bool(local), local_range=float(angular_range), psi_step=float(psi_step))
 
 
 def bank_digest(bank: Dict) -> str:
-    """SHA-256 of the canonical JSON of every field but the digest (:60-63)."""
-    body = {k: v for k, v in bank.items() if k != "sha256"}
-    text = json.dumps(body, sort_keys=True, allow_nan=False, separators=(",", ":"))
-    return hashlib.sha256(text.encode()).hexdigest()
+    """The bank's digest: SHA-256 of its canonical JSON without "sha256"."""
+    canonical = json.dumps({k: bank[k] for k in bank if k != "sha256"}, sort_keys=True, separators=(",", ":"),
+                           allow_nan=False)
+    return hashlib.sha256(canonical.encode("utf-8")).hexdigest()
 
 
 def pack_bank(context: Dict, rows: Dict, **origin) -> Dict:
-    """Bank object with its digest (:54-57)."""
-    bank = {"settings": context, "rows": rows, **origin}
+    """A bank of ``rows`` for ``context`` (settings), with its digest."""
+    bank = dict(origin, settings=context, rows=rows)
     bank["sha256"] = bank_digest(bank)
     return bank
 
 
 def check_bank(bank: Dict, context: Dict) -> None:
-    """Refuse a bank built for other inputs or settings, or altered (:66-68)."""
-    if bank.get("settings") != context or bank.get("sha256") != bank_digest(bank):
-        raise ValueError("Orientation bank digest or input/K/search settings mismatch")
+    """ValueError unless the bank was built for ``context`` and is unaltered."""
+    if bank.get("settings") == context and bank.get("sha256") == bank_digest(bank):
+        return
+    raise ValueError("orientation bank does not match the inputs, K or search settings (or was altered)")
 
 
 def restricted_indices(row, rec, allowed: Sequence[int], angles, keep: int) -> List[int]:
-    """Direction indices a bank row allows for one particle, validated against
-    the particle's STAR image name, its local directions ``allowed``, the
-    angle table and ``keep``; ascending, so tie order follows the angle file
-    (:71-90)."""
-    if row is None or row.get("image") != rec.image_name:
-        raise ValueError(f"Missing or mismatched bank entry at STAR line {rec.line_index}")
-    directions = row["directions"]
-    allowed_set = set(int(i) for i in allowed)
-    if (not directions and allowed_set) or len(directions) > min(int(keep), len(allowed_set)):
-        raise ValueError(f"Invalid direction count at STAR line {rec.line_index}")
-    seen, out = set(), []
-    for entry in directions:
-        i, rot, tilt, score = entry
-        if (type(i) is not int or i not in allowed_set or (rot, tilt) in seen
-                or (rot, tilt) != (angles.rot[i], angles.tilt[i]) or not math.isfinite(score)):
-            raise ValueError(f"Invalid bank direction at STAR line {rec.line_index}")
-        seen.add((rot, tilt))
-        out.append(i)
-    return sorted(out)
+    """The direction indices a bank row lets particle ``rec`` search, in
+    ascending order (ties then fall in angle-file order).
+
+    The row must name the particle's image, list at most min(keep, number of
+    local directions) entries -- none only when the particle has no local
+    direction -- and each entry must be a local direction index whose
+    (rot, tilt) equals the angle table's, appear once by (rot, tilt), and
+    carry a finite score."""
+    where = f"STAR line {rec.line_index}"
+    if not isinstance(row, dict) or row.get("image") != rec.image_name:
+        raise ValueError(f"orientation bank has no entry for the image at {where}")
+    local = {int(i) for i in allowed}
+    entries = row["directions"]
+    if len(entries) > min(int(keep), len(local)) or (local and not entries):
+        raise ValueError(f"orientation bank lists {len(entries)} directions at {where}")
+    picked: Dict[tuple, int] = {}
+    for index, rot, tilt, score in entries:
+        valid = (type(index) is int and index in local and (rot, tilt) == (angles.rot[index], angles.tilt[index])
+                 and (rot, tilt) not in picked and math.isfinite(score))
+        if not valid:
+            raise ValueError(f"orientation bank entry {[index, rot, tilt, score]} is not valid at {where}")
+        picked[(rot, tilt)] = index
+    return sorted(picked.values())
 
 
 def ranked_row(rec, indices: Sequence[int], angles, scores, keep: int) -> Dict:
-    """A particle's bank row from its score table ``scores`` [len(indices), P]
-    (translation peak already taken): the best score over psi per direction,
-    ranked descending with ties in table order, first ``keep`` distinct
-    (rot, tilt) pairs (:93-112).  NaN scores mark the row invalid; an infinite
-    score is an error."""
-    s = np.asarray(scores.detach().cpu().numpy() if hasattr(scores, "detach") else scores, np.float64)
-    peaks = s.reshape(len(indices), -1).max(axis=1) if len(indices) else np.zeros(0)
-    if np.isinf(peaks).any():
-        raise ValueError(f"Infinite directional peak at STAR line {rec.line_index}")
-    if np.isnan(peaks).any():
+    """Bank row of one particle from its score table ``scores`` [len(indices),
+    P] (each entry already the best over translations): directions ranked by
+    their best score over psi, descending, ties in table order, keeping the
+    first ``keep`` with distinct (rot, tilt).  A NaN best score marks the row
+    invalid (no directions); an infinite one is an error."""
+    if len(indices) == 0:
+        return {"image": rec.image_name, "directions": []}
+    table = scores.detach().cpu().numpy() if hasattr(scores, "detach") else scores
+    best = np.asarray(table, np.float64).reshape(len(indices), -1).max(axis=1)
+    if np.isinf(best).any():
+        raise ValueError(f"infinite directional peak at STAR line {rec.line_index}")
+    if np.isnan(best).any():
         return {"image": rec.image_name, "directions": [], "invalid_rawcc": True}
-    directions, seen = [], set()
-    for j in np.argsort(-peaks, kind="stable"):
-        i = int(indices[j])
-        pair = (angles.rot[i], angles.tilt[i])
-        if pair in seen:
-            continue
-        seen.add(pair)
-        directions.append([i, *pair, float(peaks[j])])
-        if len(directions) == int(keep):
+    kept: Dict[tuple, list] = {}
+    for j in np.argsort(-best, kind="stable"):
+        if len(kept) == int(keep):
             break
-    return {"image": rec.image_name, "directions": directions}
+        i = int(indices[j])
+        key = (angles.rot[i], angles.tilt[i])
+        kept.setdefault(key, [i, key[0], key[1], float(best[j])])
+    return {"image": rec.image_name, "directions": list(kept.values())}
 
 
 def atomic_json(path, obj) -> None:
-    """Write JSON through a temporary file in the target folder and rename it
-    into place (:21-36): readers never see a partial bank."""
-    path = Path(path)
-    path.parent.mkdir(parents=True, exist_ok=True)
-    with NamedTemporaryFile("w", encoding="utf-8", dir=path.parent, delete=False) as fh:
-        tmp = Path(fh.name)
-        try:
+    """Write ``obj`` as JSON so readers never see a partial file: into a
+    temporary file in the same folder, then renamed over ``path``."""
+    target = Path(path)
+    target.parent.mkdir(parents=True, exist_ok=True)
+    fd, tmp = tempfile.mkstemp(dir=target.parent, prefix=target.name + ".", suffix=".tmp")
+    try:
+        with os.fdopen(fd, "w", encoding="utf-8") as fh:
             json.dump(obj, fh, allow_nan=False, ensure_ascii=False, indent=2)
             fh.write("\n")
-        except BaseException:
-            fh.close()
-            tmp.unlink(missing_ok=True)
-            raise
-    try:
-        os.replace(tmp, path)
-    finally:
-        tmp.unlink(missing_ok=True)
+        os.replace(tmp, target)
+    except BaseException:
+        if os.path.exists(tmp):
+            os.unlink(tmp)
+        raise

@@ -1,9 +1,17 @@
-"""Text PDB input and the rigid-body pose transform that feeds the splat.
+"""Atom records of a text PDB and the rigid-body pose that moves one chain.
 
-Mirrors pdb_text.py (read_pdb :78-120, transform_coordinates :123-142,
-transform_chain_coordinates :145-150) and the zip release's
-transformed_density_atoms (first-model transform, all-model density rows,
-PDB 8.3 coordinate round trip).
+The candidate path needs three things from a PDB file: the coordinates,
+element symbols and chain letters of its atom records (for the splat), and
+the move of one chain by a pose -- rotation vector in degrees about the
+chain's own centre, then a translation in A -- followed by the fixed-column
+8.3 text round trip the reference does when it writes the moved model and
+reads it back (pdb_text.py:123-150; Z3 pdb_text.py:179-198).
+
+Record selection (the PDB format): columns 1-6 name the record; atom
+coordinates sit in columns 31-38, 39-46, 47-54; the chain letter in column
+22; the element symbol in columns 77-78.  When the file has MODEL records,
+only atoms inside MODEL ... ENDMDL blocks count, and the pose pivot uses the
+first block only.
 """
 from __future__ import annotations
 
@@ -11,153 +19,174 @@ import math
 import os
 from dataclasses import dataclass
 from functools import lru_cache
+from typing import List, Optional, Tuple
 
 import numpy as np
 
-_ELEMENTS = set((
-    "H HE LI BE B C N O F NE NA MG AL SI P S CL AR K CA SC TI V CR MN FE "
-    "CO NI CU ZN GA GE AS SE BR KR RB SR Y ZR NB MO TC RU RH PD AG CD IN "
-    "SN SB TE I XE CS BA LA CE PR ND PM SM EU GD TB DY HO ER TM YB LU HF "
-    "TA W RE OS IR PT AU HG TL PB BI PO AT RN FR RA AC TH PA U NP PU AM "
-    "CM BK CF ES FM MD NO LR RF DB SG BH HS MT DS RG CN NH FL MC LV TS OG D").split())
+# symbols accepted from columns 77-78 or derived from the atom name
+_KNOWN = frozenset("""
+H HE LI BE B C N O F NE NA MG AL SI P S CL AR K CA SC TI V CR MN FE CO NI CU ZN GA GE AS SE BR KR RB SR Y ZR
+NB MO TC RU RH PD AG CD IN SN SB TE I XE CS BA LA CE PR ND PM SM EU GD TB DY HO ER TM YB LU HF TA W RE OS IR
+PT AU HG TL PB BI PO AT RN FR RA AC TH PA U NP PU AM CM BK CF ES FM MD NO LR RF DB SG BH HS MT DS RG CN NH FL
+MC LV TS OG D""".split())
+
+# one-letter elements that an ATOM (polymer) record's atom name starts with
+_POLYMER_FIRST = frozenset("CNOHPS")
 
 
-def infer_element(line: str) -> str:
-    """Element of an ATOM/HETATM line: columns 77-78, else the aligned atom name."""
-    explicit = line[76:78].strip().upper()
-    if explicit in _ELEMENTS:
-        return explicit
-    name = line[12:16].ljust(4).upper()
-    letters = "".join(ch for ch in name if ch.isalpha())
+def infer_element(record: str) -> str:
+    """Element symbol of an ATOM / HETATM line ("" when none can be read).
+
+    Columns 77-78 win when they hold a known symbol.  Otherwise the atom name
+    (columns 13-16) decides: a name whose first column is blank or a digit is
+    a one-letter element in column 14; an ATOM record's name starting with
+    C, N, O, H, P or S is that element (CA in a protein is carbon alpha); any
+    other name is the two-letter symbol it starts with when that is known,
+    else its first letter."""
+    sym = record[76:78].strip().upper()
+    if sym in _KNOWN:
+        return sym
+    name = record[12:16].ljust(4).upper()
+    letters = [ch for ch in name if ch.isalpha()]
     if not letters:
         return ""
+    first = letters[0]
+    one = first if first in _KNOWN else ""
     if name[0].isspace() or name[0].isdigit():
-        return letters[0] if letters[0] in _ELEMENTS else ""
-    if line[:6] == "ATOM  " and letters[0] in "CNOHPS":
-        return letters[0]
-    if letters[:2] in _ELEMENTS:
-        return letters[:2]
-    return letters[0] if letters[0] in _ELEMENTS else ""
+        return one
+    if record.startswith("ATOM  ") and first in _POLYMER_FIRST:
+        return first
+    two = "".join(letters[:2])
+    return two if len(two) == 2 and two in _KNOWN else one
 
 
 @dataclass
-class PDBText:
-    lines: list
-    coords: np.ndarray          # [A, 3] float32
-    elements: np.ndarray        # [A] str
-    chains: np.ndarray          # [A] str
-    atom_line_indices: np.ndarray
+class AtomRecords:
+    """Atom records in file order: [A, 3] float32 coordinates, element and
+    chain per atom, and the 0-based line number of each record."""
+    lines: List[str]
+    coords: np.ndarray
+    elements: np.ndarray
+    chains: np.ndarray
+    line_numbers: np.ndarray
 
-    def write(self, output, coords=None) -> None:
-        """Every original line, with the x/y/z columns (31-54) of this object's
-        atom records replaced by ``coords`` in 8.3 format when given
-        (pdb_text.py:59-75)."""
-        lines = list(self.lines)
-        if coords is not None:
-            xyz = np.asarray(coords)
-            if xyz.shape != self.coords.shape:
-                raise ValueError("Replacement coordinates must have shape %s" % (self.coords.shape,))
-            if not np.all(np.isfinite(xyz)):
-                raise ValueError("PDB coordinates must be finite")
-            for li, row in zip(self.atom_line_indices, xyz):
-                ln = lines[int(li)]
-                lines[int(li)] = ln[:30] + "".join(coordinate_fields(row)) + ln[54:]
-        with open(output, "w", encoding="utf-8", errors="surrogateescape", newline="") as fh:
-            fh.writelines(lines)
+    def chain_mask(self, chain_id) -> np.ndarray:
+        return self.chains == str(chain_id)
 
 
-def read_pdb(path, include_hetatm=True, first_model_only=True) -> PDBText:
-    """xyz, element and chain of ATOM (and HETATM) records, in file order."""
+def _model_blocks(records: List[str]) -> Optional[List[Tuple[int, int]]]:
+    """[start, end) line ranges of the MODEL ... ENDMDL blocks, or None when
+    the file has no MODEL record (then every line counts)."""
+    blocks, start = [], None
+    for n, line in enumerate(records):
+        tag = line[:6].strip()
+        if tag == "MODEL":
+            if start is not None:  # an unterminated block ends at the next MODEL
+                blocks.append((start, n))
+            start = n + 1
+        elif tag == "ENDMDL" and start is not None:
+            blocks.append((start, n))
+            start = None
+    if start is not None:
+        blocks.append((start, len(records)))
+    if not blocks and not any(line[:6].strip() == "MODEL" for line in records):
+        return None
+    return blocks
+
+
+def _coordinates(line: str, where: str) -> List[float]:
+    try:
+        xyz = [float(line[30:38]), float(line[38:46]), float(line[46:54])]
+    except ValueError as exc:
+        raise ValueError(f"unreadable atom coordinates at {where}") from exc
+    if not all(math.isfinite(v) for v in xyz):
+        raise ValueError(f"non-finite atom coordinates at {where}")
+    return xyz
+
+
+def read_pdb(path, include_hetatm=True, first_model_only=True) -> AtomRecords:
+    """ATOM (and, with ``include_hetatm``, HETATM) records of a text PDB; only
+    the first MODEL block with ``first_model_only`` (the pose pivot's atoms,
+    pdb_text.py:78-120), every block otherwise (the density's atoms)."""
     with open(path, "r", encoding="utf-8", errors="surrogateescape", newline="") as fh:
-        lines = fh.readlines()
-    first_model = next((i for i, ln in enumerate(lines) if ln[:6].strip() == "MODEL"), None)
-    active = first_model is None
-    kinds = ("ATOM", "HETATM") if include_hetatm else ("ATOM",)
-    coords, elements, chains, idx = [], [], [], []
-    for i, ln in enumerate(lines):
-        rec = ln[:6].strip()
-        if rec == "MODEL":
-            if first_model_only and i != first_model:
-                break
-            active = True
-            continue
-        if rec == "ENDMDL" and first_model is not None:
-            if first_model_only:
-                break
-            active = False
-            continue
-        if not active or rec not in kinds:
-            continue
-        try:
-            xyz = [float(ln[s:s + 8]) for s in (30, 38, 46)]
-        except ValueError as exc:
-            raise ValueError("Invalid PDB coordinates in %s, line %d" % (os.fspath(path), i + 1)) from exc
-        if not all(math.isfinite(v) for v in xyz):
-            raise ValueError("Non-finite PDB coordinates in %s, line %d" % (os.fspath(path), i + 1))
-        coords.append(xyz)
-        elements.append(infer_element(ln))
-        chains.append(ln[21:22] or " ")
-        idx.append(i)
-    return PDBText(lines, np.asarray(coords, np.float32).reshape(-1, 3), np.asarray(elements, "U2"),
-                   np.asarray(chains, "U1"), np.asarray(idx, np.int64))
+        records = fh.readlines()
+    blocks = _model_blocks(records)
+    if blocks is None:
+        spans = [(0, len(records))]
+    else:
+        spans = blocks[:1] if first_model_only else blocks
+    wanted = ("ATOM", "HETATM") if include_hetatm else ("ATOM",)
+    xyz, elem, chain, where = [], [], [], []
+    for lo, hi in spans:
+        for n in range(lo, hi):
+            line = records[n]
+            if line[:6].strip() not in wanted:
+                continue
+            xyz.append(_coordinates(line, f"{os.fspath(path)}:{n + 1}"))
+            elem.append(infer_element(line))
+            chain.append(line[21:22] or " ")
+            where.append(n)
+    return AtomRecords(records, np.asarray(xyz, np.float32).reshape(-1, 3), np.asarray(elem, "U2"),
+                       np.asarray(chain, "U1"), np.asarray(where, np.int64))
 
 
 @lru_cache(maxsize=16)
-def _read_pdb_cached(path, mtime, include_hetatm, first_model_only):
+def _read_cached(path, mtime_ns, include_hetatm, first_model_only):
     return read_pdb(path, include_hetatm, first_model_only)
 
 
-def read_pdb_cached(path, include_hetatm=True, first_model_only=True) -> PDBText:
-    return _read_pdb_cached(os.fspath(path), os.path.getmtime(path), include_hetatm, first_model_only)
+def read_pdb_cached(path, include_hetatm=True, first_model_only=True) -> AtomRecords:
+    """read_pdb, reused while the file is unchanged (the optimizers re-read one
+    model for every candidate)."""
+    return _read_cached(os.fspath(path), os.stat(path).st_mtime_ns, include_hetatm, first_model_only)
 
 
 def rotvec_matrix(rotvec_deg) -> np.ndarray:
-    """Rotation matrix of a rotation vector given in degrees (Rodrigues)."""
-    v = np.deg2rad(np.asarray(rotvec_deg, np.float64))
-    th = float(np.linalg.norm(v))
-    if th == 0.0:
+    """3x3 matrix of the rotation by |v| degrees about v (Rodrigues),
+    acting on column vectors (scipy's Rotation.from_rotvec(...).as_matrix())."""
+    v = np.radians(np.asarray(rotvec_deg, np.float64).reshape(3))
+    angle = float(np.sqrt(v @ v))
+    if angle == 0.0:
         return np.eye(3)
-    k = v / th
-    K = np.array([[0.0, -k[2], k[1]], [k[2], 0.0, -k[0]], [-k[1], k[0], 0.0]])
-    return np.eye(3) * math.cos(th) + (1.0 - math.cos(th)) * np.outer(k, k) + math.sin(th) * K
+    a = v / angle
+    cross = np.array([[0.0, -a[2], a[1]], [a[2], 0.0, -a[0]], [-a[1], a[0], 0.0]])
+    return np.eye(3) + math.sin(angle) * cross + (1.0 - math.cos(angle)) * (cross @ cross)
 
 
 def transform_coordinates(coords, pose, pivot=None) -> np.ndarray:
-    """(coords - pivot) @ R(rotvec deg) + pivot + translation (pdb_text.py:123-142)."""
-    coords = np.asarray(coords)
-    pose = np.asarray(pose, np.float64)
-    if coords.ndim != 2 or coords.shape[1] != 3 or not len(coords):
-        raise ValueError("A nonempty (N, 3) coordinate array is required")
-    if pose.shape != (6,) or not np.all(np.isfinite(pose)):
-        raise ValueError("Pose must contain six finite rotation/translation values")
-    center = np.mean(coords, axis=0) if pivot is None else np.asarray(pivot, np.float64)
-    if np.shape(center) != (3,) or not np.all(np.isfinite(center)):
-        raise ValueError("Pivot must contain three finite coordinates")
-    return (coords - center) @ rotvec_matrix(pose[:3]) + center + pose[3:]
+    """Rigid move of row-vector coordinates: (x - pivot) R + pivot + t, with R
+    the rotation matrix of pose[:3] (degrees) and t = pose[3:] (A); the pivot
+    defaults to the coordinates' mean (pdb_text.py:123-142: row vectors times
+    the matrix, i.e. the inverse rotation of each point)."""
+    xyz = np.asarray(coords)
+    if xyz.ndim != 2 or xyz.shape[1] != 3 or xyz.shape[0] == 0:
+        raise ValueError("coordinates must be a nonempty (N, 3) array")
+    p = np.asarray(pose, np.float64)
+    if p.shape != (6,) or not np.isfinite(p).all():
+        raise ValueError("pose must be six finite numbers: rotation vector (deg) and translation (A)")
+    centre = xyz.mean(axis=0) if pivot is None else np.asarray(pivot, np.float64)
+    if centre.shape != (3,) or not np.isfinite(centre).all():
+        raise ValueError("pivot must be three finite numbers")
+    return (xyz - centre) @ rotvec_matrix(p[:3]) + centre + p[3:]
 
 
-def transform_chain_coordinates(pdb: PDBText, chain_id, pose, pivot=None) -> np.ndarray:
-    sel = pdb.chains == str(chain_id)
-    if not np.any(sel):
-        raise ValueError("Chain %r has no atoms in the first PDB model" % chain_id)
-    return transform_coordinates(pdb.coords[sel], pose, pivot=pivot)
-
-
-def coordinate_fields(xyz):
-    """PDB 8.3 text fields of one coordinate triple."""
-    fields = [f"{float(v):8.3f}" for v in xyz]
-    if any(len(f) != 8 for f in fields):
-        raise ValueError("Coordinate exceeds the PDB 8.3 field width")
-    return fields
+def transform_chain_coordinates(pdb: AtomRecords, chain_id, pose, pivot=None) -> np.ndarray:
+    """The moved coordinates of one chain of a parsed model (pdb_text.py:145-150)."""
+    mask = pdb.chain_mask(chain_id)
+    if not mask.any():
+        raise ValueError(f"chain {chain_id!r} has no atom records")
+    return transform_coordinates(pdb.coords[mask], pose, pivot=pivot)
 
 
 def quantize_8_3(xyz) -> np.ndarray:
-    """float(f"{x:8.3f}") of every coordinate (the write/re-read round trip), float32.
+    """float(f"{x:8.3f}") of every coordinate -- the PDB write / re-read round
+    trip -- as float32 [N, 3].
 
     Vectorised: k = round-half-even(1000 x) on the exact binary value of x,
     then k / 1000 (correctly rounded, as float() of the 8.3 text is).  The
     few values whose 1000 x lies within 1e-6 of a tie are decided exactly
-    from the formatted text."""
+    from the formatted text.  Values that do not fit the 8-character field
+    raise, as writing them would."""
     xyz = np.asarray(xyz, np.float64)
     if xyz.size and not np.all(np.isfinite(xyz)):
         raise ValueError("PDB coordinates must be finite")
@@ -170,62 +199,33 @@ def quantize_8_3(xyz) -> np.ndarray:
         k[idx] = [round(float(f"{v:.3f}") * 1000.0) for v in xyz[idx]]
     out = k / 1000.0
     if out.size and (out.max() >= 9999.9995 or out.min() <= -999.9995):
-        for r in xyz.reshape(-1, 3):
-            coordinate_fields(r)  # raises with the reference's message
+        raise ValueError("a coordinate exceeds the PDB 8.3 field width")
     return out.astype(np.float32).reshape(-1, 3)
 
 
 def transformed_density_atoms(path, chain_id, pose, include_hetatm=False):
-    """Density atoms after moving ``chain_id`` by ``pose``: the first-model
-    transform (pivot over first-model rows incl. HETATM), written to 8.3 text
-    and re-read over all models."""
+    """Density atoms (all models) after moving ``chain_id`` of the first model
+    by ``pose`` about its first-model centre (every first-model record of the
+    chain, HETATM included, sets the pivot) and the 8.3 round trip
+    (Z3 pdb_text.py:179-198).  Returns (coords [A, 3] float32, elements)."""
     first = read_pdb_cached(path)
     dens = read_pdb_cached(path, include_hetatm=include_hetatm, first_model_only=False)
-    sel = first.chains == str(chain_id)
-    moved = transform_chain_coordinates(first, chain_id, pose)
-    if not np.isfinite(moved).all():
-        raise ValueError("PDB coordinates must be finite")
-    rows = {int(li): p for p, li in enumerate(dens.atom_line_indices)}
+    mask = first.chain_mask(chain_id)
+    moved = quantize_8_3(transform_chain_coordinates(first, chain_id, pose))
+    slot = {int(n): k for k, n in enumerate(dens.line_numbers)}
     coords = dens.coords.copy()
-    q = quantize_8_3(moved)
-    for li, xyz in zip(first.atom_line_indices[sel], q):
-        p = rows.get(int(li))
-        if p is not None:
-            coords[p] = xyz
+    for n, xyz in zip(first.line_numbers[mask], moved):
+        k = slot.get(int(n))
+        if k is not None:
+            coords[k] = xyz
     return coords, dens.elements
 
 
 def move_chain(coords, chains, chain_id, pose, quantize=True) -> np.ndarray:
-    """In-memory form of transformed_density_atoms for an already-parsed model."""
+    """transformed_density_atoms for an already-parsed model: the rows of
+    ``chain_id`` moved by ``pose`` about their mean (and 8.3-rounded)."""
     coords = np.asarray(coords, np.float32).copy()
-    sel = np.asarray(chains) == str(chain_id)
-    moved = transform_coordinates(coords[sel], pose)
-    coords[sel] = quantize_8_3(moved) if quantize else moved.astype(np.float32)
+    mask = np.asarray(chains) == str(chain_id)
+    moved = transform_coordinates(coords[mask], pose)
+    coords[mask] = quantize_8_3(moved) if quantize else moved.astype(np.float32)
     return coords
-
-
-def write_transformed_chain(path, chain_id, pose, output, pivot=None) -> np.ndarray:
-    """The whole PDB with only the first-model chain ``chain_id`` moved by
-    ``pose``; every other line is kept byte for byte (pdb_text.py:153-162).
-    Returns the chain's new coordinates."""
-    pdb = read_pdb(path)
-    sel = pdb.chains == str(chain_id)
-    moved = transform_chain_coordinates(pdb, chain_id, pose, pivot=pivot)
-    PDBText(pdb.lines, pdb.coords[sel], pdb.elements[sel], pdb.chains[sel], pdb.atom_line_indices[sel]).write(
-        output, moved)
-    return moved
-
-
-def split_pdb_by_chain(input_pdb, chain_id, output_chain_file, output_rest_file) -> None:
-    """The first model's atom records of chain ``chain_id``, and all the others,
-    verbatim into two files, each ending with END (pdb_text.py:165-183)."""
-    pdb = read_pdb(input_pdb)
-    sel = pdb.chains == str(chain_id)
-    if not np.any(sel):
-        raise ValueError("Chain %r has no atoms in the first PDB model" % chain_id)
-    for output, mask in ((output_chain_file, sel), (output_rest_file, ~sel)):
-        with open(output, "w", encoding="utf-8", errors="surrogateescape", newline="") as fh:
-            for li in pdb.atom_line_indices[mask]:
-                ln = pdb.lines[int(li)]
-                fh.write(ln if ln.endswith(("\n", "\r")) else ln + "\n")
-            fh.write("END\n")

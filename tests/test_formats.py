@@ -41,3 +41,32 @@ def test_direction_star_round_trip(tmp_path):
     a = star.read_model_angles(str(p))
     assert a.rot == [-180.0, 12.5] and a.tilt == [0.0, 90.0]
     assert a.rotation_matrices_t.shape == (2, 3, 3) and a.rotation_matrices_t.dtype == np.float32
+
+
+def test_star_loop_selection_rules(tmp_path):
+    """v606:96-152 selection: the first loop with the required labels (a loop
+    without them is passed over), #n columns or label order, comments and
+    blank lines skipped, a loop_ / label line before the first row passed
+    over, and rows ending at the next label once one was read."""
+    lines = ["data_x\n", "loop_\n", "_rlnAngleRot\n", "_rlnOther\n", "1 2\n",
+             "loop_\n", "# c\n", "_rlnAngleTilt #2\n", "\n", "_rlnAngleRot #1\n", "_rlnX\n",
+             "loop_\n", "_rlnY\n", "# before rows\n",
+             "10 20 x\n", "# mid\n", "11 21 y\n", "_rlnZ\n", "12 22 z\n"]
+    loop = star.find_star_loop(lines, ["_rlnAngleRot", "_rlnAngleTilt"])
+    assert loop.labels == {"_rlnAngleTilt": 1, "_rlnAngleRot": 0, "_rlnX": 2}
+    assert loop.data_line_indices == [14, 16] and loop.data_start == 14
+    with pytest.raises(ValueError):
+        star.find_star_loop(["loop_\n", "_rlnAngleRot\n", "_rlnAngleTilt\n", "data_y\n"],
+                            ["_rlnAngleRot", "_rlnAngleTilt"])
+
+
+def test_particle_rows_need_every_used_column(tmp_path):
+    """Rows too short for the image / angle (or, with defocus labels, the
+    defocus) columns are skipped; psi defaults to 0 when its column is absent."""
+    p = tmp_path / "q.star"
+    p.write_text("data_\nloop_\n_rlnImageName #1\n_rlnAngleRot #2\n_rlnAngleTilt #3\n_rlnAnglePsi #4\n"
+                 "1@a 1 2 3\n2@a 4 5\n3@a 6\n")
+    t = star.read_particle_table(str(p))
+    assert not t.has_ctf
+    assert [(r.image_name, r.rot, r.tilt, r.psi) for r in t.records] == [("1@a", 1.0, 2.0, 3.0),
+                                                                         ("2@a", 4.0, 5.0, 0.0)]
