@@ -27,7 +27,8 @@
 // r+1 are in flight (in registers) while the MFMAs consume chunk r.
 // The per-particle tail (make_result_records numerics, v606:1452-1491) runs
 // in the same workgroup: first-maximum argmax over (direction, psi), mean
-// and population std in float64.
+// and population std in float64.  A crop of up to 32 runs as four launches,
+// one per 16 x 16 lag window, each merged into the stored maxima.
 #include "common.h"
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -45,14 +46,16 @@ __device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-__global__ void build_corr_tables(int n, int C, int nrow_pad, int mpad, float2* __restrict__ Wt,
+// W / E tables of the 16 x 16 lag window whose first crop row / column are
+// (y0, x0): table column y holds crop index y0 + y (zero beyond the crop).
+__global__ void build_corr_tables(int n, int C, int y0, int x0, int nrow_pad, int mpad, float2* __restrict__ Wt,
                                   float2* __restrict__ Et) {
   const int m = n / 2 + 1;
   const int s = m / 2;
   const int total = nrow_pad * 16 + mpad * 16;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     if (t < nrow_pad * 16) {
-      const int r = t / 16, y = t % 16;
+      const int r = t / 16, y = y0 + t % 16;
       float2 v = make_float2(0.f, 0.f);
       if (r < n && y < C) {
         const int l = y < C / 2 ? n - C / 2 + y : y - C / 2;
@@ -65,7 +68,7 @@ __global__ void build_corr_tables(int n, int C, int nrow_pad, int mpad, float2* 
       Wt[t] = v;
     } else {
       const int u = t - nrow_pad * 16;
-      const int c = u / 16, x = u % 16;
+      const int c = u / 16, x = x0 + u % 16;
       float2 v = make_float2(0.f, 0.f);
       if (c < m && x < C) {
         const int l = x < C / 2 ? n - C / 2 + x : x - C / 2;
@@ -85,7 +88,7 @@ __global__ void __launch_bounds__(kThreads)
 correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spectra,
                  const float* __restrict__ inv_norm, const float* __restrict__ tinv,
                  const int* __restrict__ dir_off, const int* __restrict__ dir_idx, const int* __restrict__ pmap,
-                 int P, int n, int C, int JT, int PT,
+                 int P, int n, int C, int JT, int PT, int y0, int x0, int merge, int tail,
                  const float2* __restrict__ Wt, const float2* __restrict__ Et,
                  float* __restrict__ scores, int* __restrict__ peak_y, int* __restrict__ peak_x,
                  int* __restrict__ best, float* __restrict__ peak, double* __restrict__ mean,
@@ -223,7 +226,7 @@ correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spe
         int bi = 0x7fffffff;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int x = 4 * lr + v, y = lc;
+          const int x = x0 + 4 * lr + v, y = y0 + lc;
           if (x < C && y < C) {
             const float val = outv[i][v] * sc;
             const int idx = y * C + x;
@@ -238,9 +241,15 @@ correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spe
         }
         if (lane == 0) {
           const size_t g = (size_t)(d0 + j0 + j) * P + p0 + p;
-          scores[g] = bv;
-          peak_y[g] = bi / C;
-          peak_x[g] = bi % C;
+          // a later lag window keeps the earlier windows' maximum unless it
+          // is larger, or equal at a smaller crop index (first maximum)
+          const bool keep_old =
+              merge && (scores[g] > bv || (scores[g] == bv && peak_y[g] * C + peak_x[g] < bi));
+          if (!keep_old) {
+            scores[g] = bv;
+            peak_y[g] = bi / C;
+            peak_x[g] = bi % C;
+          }
         }
       }
     }
@@ -248,7 +257,7 @@ correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spe
 
   // ---- per-particle tail -------------------------------------------------
   __syncthreads();
-  if (wave == 0) {
+  if (tail && wave == 0) {
     const int np = Mi * P;
     const float* s = scores + (size_t)d0 * P;
     float bv = -INFINITY;
@@ -351,8 +360,8 @@ GIS_API int gis_correlate(const void* bank, const void* spectra, const float* in
                           size_t ws_bytes, void* stream) {
   GIS_REQUIRE(B >= 0 && P >= 1, "gis_correlate: B=%d P=%d", B, P);
   GIS_REQUIRE(n >= 4 && n % 2 == 0 && n <= 256, "gis_correlate: newbox %d must be even, in [4, 256]", n);
-  if (C > 16 || C < 2 || C % 2 != 0 || C > n) {
-    gis_set_error("gis_correlate: crop %d must be even, in [2, 16] and <= newbox", C);
+  if (C > 32 || C < 2 || C % 2 != 0 || C > n) {
+    gis_set_error("gis_correlate: crop %d must be even, in [2, 32] and <= newbox", C);
     return GIS_EUNSUPPORTED;
   }
   if (B == 0) return GIS_OK;
@@ -363,8 +372,6 @@ GIS_API int gis_correlate(const void* bank, const void* spectra, const float* in
   hipStream_t st = (hipStream_t)stream;
   float2* Wt = (float2*)ws;
   float2* Et = Wt + (size_t)row_pad(n) * 16;
-  hipLaunchKernelGGL(build_corr_tables, dim3(64), dim3(256), 0, st, n, C, row_pad(n), col_pad(n), Wt, Et);
-  GIS_LAUNCH_CHECK("gis_correlate tables");
   const int PT = std::min(P, 8);
   const int jt_max = kMaxTiles - PT;
   GIS_REQUIRE(jt >= 0 && jt <= jt_max, "gis_correlate: jt %d not in [0, %d]", jt, jt_max);
@@ -381,10 +388,20 @@ GIS_API int gis_correlate(const void* bank, const void* spectra, const float* in
     }
     attr_set = true;
   }
-  hipLaunchKernelGGL(correlate_kernel, dim3(B), dim3(kThreads), lds, st, (const float2*)bank,
-                     (const float2*)spectra, inv_norm, tinv, dir_off, dir_idx, pmap, P, n, C, JT, PT, Wt, Et, scores,
-                     peak_y, peak_x, best, peak, mean, stdv);
-  GIS_LAUNCH_CHECK("gis_correlate");
+  // one 16 x 16 lag window per launch: a crop above 16 takes four, each
+  // merged into the stored pair maxima; the last one runs the particle tail
+  const int nw = C > 16 ? 2 : 1;
+  for (int q = 0; q < nw * nw; ++q) {
+    const int y0 = 16 * (q / nw), x0 = 16 * (q % nw);
+    hipLaunchKernelGGL(build_corr_tables, dim3(64), dim3(256), 0, st, n, C, y0, x0, row_pad(n), col_pad(n), Wt,
+                       Et);
+    GIS_LAUNCH_CHECK("gis_correlate tables");
+    hipLaunchKernelGGL(correlate_kernel, dim3(B), dim3(kThreads), lds, st, (const float2*)bank,
+                       (const float2*)spectra, inv_norm, tinv, dir_off, dir_idx, pmap, P, n, C, JT, PT, y0, x0,
+                       q > 0 ? 1 : 0, q == nw * nw - 1 ? 1 : 0, Wt, Et, scores, peak_y, peak_x, best, peak, mean,
+                       stdv);
+    GIS_LAUNCH_CHECK("gis_correlate");
+  }
   return GIS_OK;
 }
 

@@ -89,6 +89,8 @@ class SearchEngine:
             raise ValueError("psiStep must be positive")
         if crop <= 0 or crop % 2 or crop > self.n:
             raise ValueError("CCG crop size must be positive, even and <= newboxsize")
+        if crop > 32:
+            raise ValueError(f"--ccgCropSize {crop}: the searches cover crops of up to 32 x 32 lags")
         if self.N not in kernels.SUPPORTED_BOXES:
             raise ValueError(f"box {self.N} is not one of {kernels.SUPPORTED_BOXES}")
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)

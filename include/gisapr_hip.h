@@ -248,7 +248,8 @@ size_t gis_correlate_workspace(int n);
  *                        (y, x) = divmod(argmax, C) in the C x C crop
  *   best [B]: flat argmax over (local direction, psi) or -1 if none
  *   peak [B] float32, mean/std [B] float64 (population std)
- * C (--ccgCropSize) must be even and <= 16.
+ * C (--ccgCropSize) must be even and <= 32 (the reference default is 32,
+ * v606:1562); a crop above 16 runs as four 16 x 16 lag windows.
  */
 int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
                   const int* dir_off, const int* dir_idx, const int* pmap, int B, int P, int n, int C, int jt,

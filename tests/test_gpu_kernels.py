@@ -196,7 +196,8 @@ def test_particles_match_oracle(hip_device, N, n, with_ctf):
         assert e < 1e-4, (b, e)
 
 
-@pytest.mark.parametrize("n,C", [(32, 8), (64, 10), (64, 16), (96, 16), (192, 16), (256, 16)])
+@pytest.mark.parametrize("n,C", [(32, 8), (64, 10), (64, 16), (96, 16), (192, 16), (256, 16), (64, 20), (160, 32),
+                                 (256, 32)])
 def test_correlate_matches_oracle(hip_device, n, C):
     import torch
     from gisapr_amd import kernels

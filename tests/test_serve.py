@@ -48,7 +48,7 @@ def test_poisson_requests_are_seeded_and_bounded():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("path", ["mixed", "fp32"])
+@pytest.mark.parametrize("path", ["mixed", "fp32", "split"])
 def test_served_results_equal_alone(hip_device, path):
     from gisapr_amd import synthetic
     from gisapr_amd.batch import CandidateBatch, RigidBody
