@@ -351,6 +351,23 @@ int gis_correlate_mixed_ex(const void* Tk, const void* Pk, const int* dir_off, c
                            const void* Plast, int half, int pt, int wpb, float* scores, int* peak_y, int* peak_x,
                            int* best, float* peak, double* mean, double* stdv, int operand, void* stream);
 
+/*
+ * Geometric restraint (func_check_boundary_for_testing_v8.py:62-126):
+ * gis_threshold_mask: mask[i] = vol[i] > thr (the body / chain masks, :84, :112).
+ * gis_edt_squared: squared Euclidean distance (voxels^2) from every voxel of
+ *   an [N]^3 grid to the nearest voxel with mask != 0, exact (1e30 when the
+ *   mask is empty); replaces scipy.ndimage.distance_transform_edt(~mask)
+ *   (:90-92) up to the square root.  N <= 1024.
+ * gis_restraint_eval: one pose of GeometryRestraint.evaluate (:96-126):
+ *   overlap[0] = #(chain_vol > thr and main_mask), min_d2[0] = float32 bits of
+ *   the smallest main_d2 over voxels with chain_vol > thr (1e30 if none),
+ *   any[0] = 1 if some voxel has chain_vol > thr.  Device outputs, set here.
+ */
+int gis_threshold_mask(const float* vol, size_t n, float thr, unsigned char* mask, void* stream);
+int gis_edt_squared(const unsigned char* mask, int N, float* d2, void* stream);
+int gis_restraint_eval(const float* chain_vol, float thr, const unsigned char* main_mask, const float* main_d2,
+                       size_t n, unsigned long long* overlap, unsigned int* min_d2, int* any, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
