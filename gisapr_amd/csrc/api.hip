@@ -38,5 +38,5 @@ GIS_API int gis_init(int device) {
     return (int)e;
   }
   (void)gis_cu_count();
-  return GIS_OK;
+  return gis_split_scratch_setup();
 }

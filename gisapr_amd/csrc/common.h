@@ -420,3 +420,5 @@ particle_tail_kernel(const int* __restrict__ dir_off, int P, const float* __rest
 
 // Number of compute units of the current device (cached per device).
 int gis_cu_count();
+// Looks up the allocation-free search workspace of the current device.
+int gis_split_scratch_setup();

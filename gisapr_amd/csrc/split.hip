@@ -42,6 +42,8 @@
 // tail is particle_tail_kernel.
 #include "common.h"
 
+#include <mutex>
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -424,32 +426,28 @@ GIS_API size_t gis_correlate_split_workspace(int n, int C, int B) {
          align256((size_t)(B + 1) * sizeof(int));
 }
 
-// Full-plane correlate_templates_and_particle (v606:1418-1449) plus the
-// numerics of make_result_records (v606:1452-1491) on split-BF16 MFMA; the
-// arguments and outputs of gis_correlate, plus
-//   order  [B] processing order of the particles (a permutation) or NULL
-//   nrows  dir_off[B], the number of dir_idx entries
-//   dt     directions per wave (1 or 2; 0 = automatic)
-//   pt     psi values per wave (1..5, <= 3 with dt 2; 0 = automatic)
-GIS_API int gis_correlate_split(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
-                                const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B,
-                                int P, int n, int C, int nrows, int dt, int pt, float* scores, int* peak_y,
-                                int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
-                                size_t ws_bytes, void* stream) {
-  GIS_REQUIRE(B >= 0 && P >= 1 && nrows >= 0, "gis_correlate_split: B=%d P=%d nrows=%d", B, P, nrows);
-  GIS_REQUIRE(n >= 4 && n % 2 == 0 && n <= 256, "gis_correlate_split: newbox %d must be even, in [4, 256]", n);
+namespace {
+
+// Argument checks shared by both entry points; GIS_OK when the call may go on.
+int split_check(const char* who, int B, int P, int n, int C, int nrows, int dt, int pt) {
+  GIS_REQUIRE(B >= 0 && P >= 1 && nrows >= 0, "%s: B=%d P=%d nrows=%d", who, B, P, nrows);
+  GIS_REQUIRE(n >= 4 && n % 2 == 0 && n <= 256, "%s: newbox %d must be even, in [4, 256]", who, n);
   if (C > 32 || C < 2 || C % 2 != 0 || C > n) {
-    gis_set_error("gis_correlate_split: crop %d must be even, in [2, 32] and <= newbox", C);
+    gis_set_error("%s: crop %d must be even, in [2, 32] and <= newbox", who, C);
     return GIS_EUNSUPPORTED;
   }
-  GIS_REQUIRE(dt >= 0 && dt <= 2, "gis_correlate_split: dt %d not in [0, 2]", dt);
-  GIS_REQUIRE(pt >= 0 && pt <= 5, "gis_correlate_split: pt %d not in [0, 5]", pt);
-  if (B == 0) return GIS_OK;
-  GIS_REQUIRE(bank && spectra && dir_off && dir_idx && scores && peak_y && peak_x && best && peak && mean && stdv &&
-                  ws,
-              "gis_correlate_split: null pointer");
-  GIS_REQUIRE(ws_bytes >= gis_correlate_split_workspace(n, C, B), "gis_correlate_split: workspace too small");
-  hipStream_t st = (hipStream_t)stream;
+  GIS_REQUIRE(dt >= 0 && dt <= 2, "%s: dt %d not in [0, 2]", who, dt);
+  GIS_REQUIRE(pt >= 0 && pt <= 5, "%s: pt %d not in [0, 5]", who, pt);
+  return GIS_OK;
+}
+
+// The launches of one search; ws holds gis_correlate_split_workspace(n, C, B)
+// bytes of device memory (tables, then the unit offsets), all of which the
+// first two kernels write before the search reads them.
+int split_run(const void* bank, const void* spectra, const float* inv_norm, const float* tinv, const int* dir_off,
+              const int* dir_idx, const int* pmap, const int* order, int B, int P, int n, int C, int nrows, int dt,
+              int pt, float* scores, int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv,
+              void* ws, hipStream_t st) {
   const int LT = split_lt(C);
   if (pt == 0) pt = std::min(P, 5);
   if (dt == 0 || LT == 2) dt = 1;
@@ -502,4 +500,96 @@ GIS_API int gis_correlate_split(const void* bank, const void* spectra, const flo
   hipLaunchKernelGGL(particle_tail_kernel, dim3(B), dim3(64), 0, st, dir_off, P, scores, best, peak, mean, stdv);
   GIS_LAUNCH_CHECK("gis_correlate_split tail");
   return GIS_OK;
+}
+
+}  // namespace
+
+// Workspace of the allocation-free entry point: one array per GPU, resident
+// until the process ends.  Host code must not use the array's name as a
+// device address; split_scratch() returns the device's own address of it.
+constexpr size_t kSplitScratchBytes = (size_t)1 << 20;
+static __device__ __align__(256) unsigned char scratch[kSplitScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current; NULL (and the error set) on failure.
+void* split_scratch(const char* who) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    gis_set_error("%s: no usable current device (%d): %s", who, dev, hipGetErrorString(e));
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      gis_set_error("%s: hipGetSymbolAddress(scratch) on device %d: %s", who, dev, hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+// Looks up the allocation-free workspace of the current device (called by
+// gis_init; the entry point also does it on its first call on a device).
+int gis_split_scratch_setup() { return split_scratch("gis_init") ? GIS_OK : GIS_EINVAL; }
+
+// Full-plane correlate_templates_and_particle (v606:1418-1449) plus the
+// numerics of make_result_records (v606:1452-1491) on split-BF16 MFMA; the
+// arguments and outputs of gis_correlate, plus
+//   order  [B] processing order of the particles (a permutation) or NULL
+//   nrows  dir_off[B], the number of dir_idx entries
+//   dt     directions per wave (1 or 2; 0 = automatic)
+//   pt     psi values per wave (1..5, <= 3 with dt 2; 0 = automatic)
+GIS_API int gis_correlate_split(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
+                                const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B,
+                                int P, int n, int C, int nrows, int dt, int pt, float* scores, int* peak_y,
+                                int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
+                                size_t ws_bytes, void* stream) {
+  const int rc = split_check("gis_correlate_split", B, P, n, C, nrows, dt, pt);
+  if (rc != GIS_OK) return rc;
+  if (B == 0) return GIS_OK;
+  GIS_REQUIRE(bank && spectra && dir_off && dir_idx && scores && peak_y && peak_x && best && peak && mean && stdv &&
+                  ws,
+              "gis_correlate_split: null pointer");
+  GIS_REQUIRE(ws_bytes >= gis_correlate_split_workspace(n, C, B), "gis_correlate_split: workspace too small");
+  return split_run(bank, spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order, B, P, n, C, nrows, dt, pt, scores,
+                   peak_y, peak_x, best, peak, mean, stdv, ws, (hipStream_t)stream);
+}
+
+// gis_correlate_split without a workspace argument and without allocating:
+// the workspace is a 1 MiB array per GPU inside the library, so calls with
+// gis_correlate_split_workspace(n, C, B) > 1 MiB (about 220000 particles at
+// newbox 256, crop 32) are refused.  Calls that overlap on one GPU (other
+// streams or threads) share the array and race.
+GIS_API int gis_correlate_split_noalloc(const void* bank, const void* spectra, const float* inv_norm,
+                                        const float* tinv, const int* dir_off, const int* dir_idx, const int* pmap,
+                                        const int* order, int B, int P, int n, int C, int nrows, int dt, int pt,
+                                        float* scores, int* peak_y, int* peak_x, int* best, float* peak,
+                                        double* mean, double* stdv, void* stream) {
+  const int rc = split_check("gis_correlate_split_noalloc", B, P, n, C, nrows, dt, pt);
+  if (rc != GIS_OK) return rc;
+  if (B == 0) return GIS_OK;
+  GIS_REQUIRE(bank && spectra && dir_off && dir_idx && scores && peak_y && peak_x && best && peak && mean && stdv,
+              "gis_correlate_split_noalloc: null pointer");
+  const size_t need = gis_correlate_split_workspace(n, C, B);
+  GIS_REQUIRE(need <= kSplitScratchBytes,
+              "gis_correlate_split_noalloc: %d particles at newbox %d, crop %d need %zu bytes of workspace, above "
+              "the %zu-byte limit of the allocation-free path",
+              B, n, C, need, kSplitScratchBytes);
+  void* ws = split_scratch("gis_correlate_split_noalloc");
+  if (!ws) return GIS_EINVAL;
+  return split_run(bank, spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order, B, P, n, C, nrows, dt, pt, scores,
+                   peak_y, peak_x, best, peak, mean, stdv, ws, (hipStream_t)stream);
 }

@@ -33,7 +33,8 @@ extern "C" {
 const char* gis_last_error(void);
 /* ABI version (1). */
 int gis_version(void);
-/* Makes `device` current for the calling thread. */
+/* Makes `device` current for the calling thread and looks up the device's
+ * workspace of gis_correlate_split_noalloc. */
 int gis_init(int device);
 
 /*
@@ -270,9 +271,7 @@ size_t gis_correlate_split_workspace(int n, int C, int B);
  *   order  [B] processing order of the searched particles (a permutation) or
  *          NULL: particles ranked by orientation share template rows in cache
  *   nrows  dir_off[B], the number of dir_idx entries
- *   dt     local directions per wave (1 or 2), or 4: one block of four
- *          waves per four directions of a particle, sharing its psi spectra
- *          through LDS; 0 = automatic
+ *   dt     local directions per wave (1 or 2; 0 = automatic)
  *   pt     psi values per wave (1..5, <= 3 with dt 2; 0 = automatic).  dt and
  *          pt are autotuner knobs: results do not depend on them
  * C (--ccgCropSize) must be even and <= 32 (the reference default is 32,
@@ -282,6 +281,21 @@ int gis_correlate_split(const void* bank, const void* spectra, const float* inv_
                         const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B, int P,
                         int n, int C, int nrows, int dt, int pt, float* scores, int* peak_y, int* peak_x,
                         int* best, float* peak, double* mean, double* stdv, void* ws, size_t ws_bytes, void* stream);
+
+/*
+ * gis_correlate_split without the workspace argument, allocating nothing:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The workspace
+ * is a 1 MiB array per GPU inside the library (held until the process
+ * ends); a call whose gis_correlate_split_workspace(n, C, B) exceeds 1 MiB
+ * (about 220000 particles at n = 256, C = 32) is refused with GIS_EINVAL.
+ * Calls that overlap on one GPU share that array and race: serialise them
+ * (one stream, or the caller's own lock).
+ */
+int gis_correlate_split_noalloc(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
+                                const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B,
+                                int P, int n, int C, int nrows, int dt, int pt, float* scores, int* peak_y,
+                                int* peak_x, int* best, float* peak, double* mean, double* stdv, void* stream);
 
 /*
  * Cross-particle conformation score sum_b exp(peak_b / 1024) in float64 over
