@@ -242,6 +242,23 @@ class RigidBody:
         numbers = np.asarray(numbers, np.float32)
         return cls(coords[~sel], numbers[~sel], coords[sel], numbers[sel])
 
+    @classmethod
+    def from_pdb(cls, source_pdb, chain_id, include_hetatm=False):
+        """The split RigidDensityCache makes of a PDB file: the moving chain is
+        the first model's atoms of ``chain_id``; every other atom of every
+        model (with HETATM records when ``include_hetatm``) is fixed."""
+        first = read_pdb_cached(source_pdb)
+        dens = read_pdb_cached(source_pdb, include_hetatm=include_hetatm, first_model_only=False)
+        sel = first.chains == str(chain_id)
+        if not sel.any():
+            raise ValueError(f"Chain {chain_id!r} has no atoms in the first PDB model")
+        moving_lines = first.line_numbers[sel]
+        moving = np.isin(dens.line_numbers, moving_lines)
+        numbers, _ = atoms_to_atomic_numbers_and_masses(dens.elements)
+        numbers = np.asarray(numbers, np.float32)
+        coords = np.asarray(dens.coords, np.float32)
+        return cls(coords[~moving], numbers[~moving], coords[moving], numbers[moving])
+
     def moved(self, pose) -> np.ndarray:
         """Moving-chain coordinates after ``pose`` (rotvec deg, translation A) about
         the chain's mean, 8.3-rounded (pdb_text.py:123-142)."""
@@ -566,6 +583,8 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
     tile = tile or {}
     C, M = len(volumes), len(engine.dir_rot)
     plan = build_slot_plan(ps, M, C)
+    if path not in ("mixed", "fp32", "split"):
+        raise ValueError("path must be 'fp32', 'split' or 'mixed'")
     if path == "mixed":
         if ps.coeffs is None:
             engine.prepare_small_shift(ps)
@@ -587,8 +606,13 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
             _, norms = kernels.build_templates(dft, n, engine.rts, engine.mask, engine.fsc_packed,
                                                out=bank[c * M:(c + 1) * M], normalize=False)
             torch.reciprocal(norms, out=tinv[c * M:(c + 1) * M])
-        corr = kernels.correlate(bank, ps.spectra, ps.inv_norm, plan.dir_off, plan.dir_idx, engine.crop,
-                                 validate=False, pmap=plan.pmap, jt=tile.get("jt", 0), tinv=tinv)
+        if path == "split":
+            corr = kernels.correlate_split(bank, ps.spectra, ps.inv_norm, plan.dir_off, plan.dir_idx, engine.crop,
+                                           validate=False, pmap=plan.pmap, tinv=tinv, order=plan.virtual_major(C),
+                                           dt=tile.get("dt", 0), pt=tile.get("pt", 0))
+        else:
+            corr = kernels.correlate(bank, ps.spectra, ps.inv_norm, plan.dir_off, plan.dir_idx, engine.crop,
+                                     validate=False, pmap=plan.pmap, jt=tile.get("jt", 0), tinv=tinv)
     scores = kernels.score_sum_segments(corr.peak, corr.best, plan.slot_off)
     return scores, corr, plan
 
@@ -616,7 +640,8 @@ class BatchSearchEngine:
     Returns one dict per model: ``score`` (float64 sum exp(rawCC/1024)),
     ``search_text`` (the three v606 records per particle), ``metrics``.
     ``options``: ``batch_candidates`` (group size), ``path`` ("fp32", the
-    reference's search, or "mixed"), ``fixed_density_cache_mb``,
+    reference's search; "split", the same operator on split-bf16 MFMA; or
+    "mixed", the small-shift window), ``fixed_density_cache_mb``,
     ``batch_density_mode`` ("fixed" reuses the fixed body).
     ``orientation_rows`` (a calibration bank's rows) restricts every
     particle to its bank directions; ``calibrate`` returns each model's
@@ -704,7 +729,7 @@ class BatchSearchEngine:
         if orientation_rows is not None:
             ps = self._restricted(ctx, orientation_rows, keep)
         path = self._opt("path")
-        center = engine.crop // 2 if path == "fp32" else engine.small.half
+        center = engine.small.half if path == "mixed" else engine.crop // 2
         P, M = ps.P, len(angles.rot)
         results = []
         cap = max(1, int(self._opt("batch_candidates")))

@@ -676,7 +676,7 @@ def run_search(args: argparse.Namespace, *, model_volume=None, session_cache=Non
 
 def main() -> None:
     if "--worker-connect" in sys.argv[1:]:
-        from gisapr_worker import run_worker  # the caller's worker loop (transport is out of scope here)
+        from .worker import run_worker
         run_worker(create_search_parser, run_search, choose_device)
         return
     args = create_search_parser().parse_args()

@@ -174,6 +174,10 @@ class SearchServer:
         self.autotuner = autotuner
         self.datasets: Dict[str, Dataset] = {}
         self.scheduler = Scheduler()
+        self._inflight: Dict[Tuple[str, int], tuple] = {}  # (dataset, executor) -> (items, host scores, event)
+        self._reported = 0
+        self.pairs = 0                    # (direction, psi) pairs searched so far
+        self.steps = 0
 
     def add_dataset(self, name: str, engine, particles, body, res: float) -> Dataset:
         from .batch import CandidateBatch
@@ -203,56 +207,81 @@ class SearchServer:
         self.datasets[name] = ds
         return ds
 
+    # -- incremental loop (the worker drives it between messages) --------------
+    def submit(self, req: Request) -> None:
+        """Queue a request (its ``arrival`` on the caller's clock)."""
+        if req.dataset not in self.datasets:
+            raise KeyError(f"request {req.rid}: dataset {req.dataset!r} is not resident")
+        self.scheduler.arrive(req)
+
+    def busy(self) -> bool:
+        return not self.scheduler.idle() or bool(self._inflight)
+
+    def pump(self, now: Callable[[], float]) -> bool:
+        """Retire finished steps, then launch a step on every idle executor
+        whose dataset has pending candidates.  ``now()`` stamps completions.
+        Returns whether a step was launched."""
+        torch = self.torch
+        sched = self.scheduler
+        for key in list(self._inflight):
+            items, host, ev = self._inflight[key]
+            if ev.query():
+                sched.finish(items, host[:len(items)].tolist(), now())
+                del self._inflight[key]
+        launched = False
+        for name, ds in self.datasets.items():
+            for e, ex in enumerate(ds.executors):
+                if (name, e) in self._inflight or sched.pending(name) == 0:
+                    continue
+                items = sched.take(name, self.slots)
+                poses = np.stack([req.poses[j] for req, j in items])
+                with torch.cuda.stream(ds.streams[e]):
+                    scores, corr = ex.evaluate(poses, sync=False)
+                    host = torch.empty(scores.shape, dtype=scores.dtype, pin_memory=True)
+                    host.copy_(scores, non_blocking=True)
+                    ev = torch.cuda.Event()
+                    ev.record()
+                self._inflight[(name, e)] = (items, host, ev)
+                self.pairs += ex.plan.n_pairs * len(items) // ex.slots
+                self.steps += 1
+                launched = True
+        return launched
+
+    def take_completed(self) -> List[Request]:
+        """Requests completed since the last call, in completion order."""
+        done = self.scheduler.completed[self._reported:]
+        self._reported = len(self.scheduler.completed)
+        return done
+
     def serve(self, requests: Sequence[Request], clock: Callable[[], float] = time.perf_counter,
               poll_s: float = 0.0002) -> dict:
-        torch = self.torch
+        """Replay ``requests`` against the wall clock (each visible from its
+        arrival time) until all complete; returns latency and throughput."""
         pending = sorted(requests, key=lambda r: r.arrival)
         for r in pending:
             if r.dataset not in self.datasets:
                 raise KeyError(f"request {r.rid}: dataset {r.dataset!r} is not resident")
-        sched = self.scheduler
         t0 = clock()
+        rel = lambda: clock() - t0
+        first_done = len(self.scheduler.completed)
+        pairs0, steps0 = self.pairs, self.steps
         nxt = 0
-        inflight: Dict[Tuple[str, int], tuple] = {}  # (dataset, executor) -> (items, host scores, event)
-        pairs = 0
-        steps = 0
-        while nxt < len(pending) or not sched.idle() or inflight:
-            now = clock() - t0
+        while nxt < len(pending) or self.busy():
+            now = rel()
             while nxt < len(pending) and pending[nxt].arrival <= now:
-                sched.arrive(pending[nxt])
+                self.submit(pending[nxt])
                 nxt += 1
-            # retire finished steps
-            for key in list(inflight):
-                items, host, ev = inflight[key]
-                if ev.query():
-                    sched.finish(items, host[:len(items)].tolist(), clock() - t0)
-                    del inflight[key]
-            # launch steps on idle executors
-            launched = False
-            for name, ds in self.datasets.items():
-                for e, ex in enumerate(ds.executors):
-                    if (name, e) in inflight or sched.pending(name) == 0:
-                        continue
-                    items = sched.take(name, self.slots)
-                    poses = np.stack([req.poses[j] for req, j in items])
-                    with torch.cuda.stream(ds.streams[e]):
-                        scores, corr = ex.evaluate(poses, sync=False)
-                        host = torch.empty(scores.shape, dtype=scores.dtype, pin_memory=True)
-                        host.copy_(scores, non_blocking=True)
-                        ev = torch.cuda.Event()
-                        ev.record()
-                    inflight[(name, e)] = (items, host, ev)
-                    pairs += ex.plan.n_pairs * len(items) // ex.slots
-                    steps += 1
-                    launched = True
-            if not launched and inflight:
+            launched = self.pump(rel)
+            if not launched and self._inflight:
                 time.sleep(poll_s)
             elif not launched and nxt < len(pending):
-                time.sleep(max(0.0, min(pending[nxt].arrival - (clock() - t0), 0.002)))
-        wall = clock() - t0
-        done = sched.completed
+                time.sleep(max(0.0, min(pending[nxt].arrival - rel(), 0.002)))
+        wall = rel()
+        done = self.scheduler.completed[first_done:]
+        self._reported = len(self.scheduler.completed)
         stats = latency_stats(done)
         cands = int(sum(r.size for r in done))
+        pairs, steps = self.pairs - pairs0, self.steps - steps0
         stats.update(wall_s=wall, requests=len(done), candidates=cands, steps=steps,
                      candidates_per_s=cands / wall if wall > 0 else None,
                      scores_per_s=pairs / wall if wall > 0 else None)
