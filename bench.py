@@ -85,37 +85,137 @@ def mixed_flops_per_score(K: int, lags: int) -> float:
     return 3.0 * 4.0 * K * lags
 
 
-def cpu_baseline(eng, ps, bank, sample: int):
-    """The oracle (float32 NumPy port, one core) on the first ``sample`` particles:
-    particle preparation + correlation + per-particle record, templates given
-    (``bank`` = (rows, 1/norms) from the GPU)."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def _median_time(fn, runs: int):
+    times, out = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), out
+
+
+def cpu_baseline(gpu_scores_per_candidate: int, box256_particles: int, runs: int = 5, template_dirs: int = 64,
+                 log=print):
+    """The v606 FP32 path restated in torch on the host CPU (oracle.torch_cpu,
+    the reference's chunking: 16 templates x 32 psi, CTF batches of 256,
+    8 projections per batch), timed at the default thread count and at the
+    reference default of 4 threads, median of ``runs`` runs after one
+    warm-up:
+      box96  (the CPU-runnable workload, 200 particles): search only, and
+             density + templates (all 192 directions) + search;
+      box256 (``box256_particles`` particles of the 5,000): search only; the
+             volume transform timed, the per-direction template work timed
+             on ``template_dirs`` of the 768 directions and scaled to 768
+             (density and templates: median of 3); the 5,000-particle
+             candidate extrapolated
+             from the per-score search time and the GPU run's score count.
+    ``value``: the extrapolated box256 candidate (density + 768 templates +
+    every score of the 5,000 particles) in scores/s at the default thread
+    count."""
+    import torch
+    from gisapr_amd import synthetic
     from oracle import fourier as ofr
     from oracle import particles as opt
+    from oracle import project as opj
     from oracle import search as osr
-    raw, tinv = bank
-    bank_pre = ofr.pre_shift((raw * tinv[:, None, None]).cpu().numpy())  # unit-norm rows
-    mask = eng.mask.cpu().numpy() if eng.mask is not None else None
-    ctf = opt.PackedCTF(eng.n, eng.new_apix, 2.7, 300.0, eng.ctf.kk)
-    sample = min(sample, ps.B)
-    imgs = ps.images[:sample].cpu().numpy()
-    dfo = ps.defocus[:sample].cpu().numpy()
-    scores = 0
-    t0 = time.perf_counter()
-    for b in range(sample):
-        idx = ps.local[b]
-        if not idx:
-            continue
-        c = ctf.one(*dfo[b])
-        pf = opt.prepare_particle(imgs[b], list(ps.eff_psi[b]), eng.n, mask, c, np.float32)
-        sc, ys, xs = ofr.correlate(bank_pre[idx], pf, eng.crop, np.float32)
-        osr.make_record(b, (ps.rot[b], ps.tilt[b], ps.psi[b]), list(ps.fake_psi[b]), eng.dir_rot,
-                        eng.dir_tilt, idx, sc, ys, xs, eng.crop, eng.trans_scale)
-        scores += sc.size
-    dt = time.perf_counter() - t0
-    return {"value": scores / dt, "unit": "scores/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (numpy float32, 1 thread) v606 FP32 path: particle prep + correlation + "
-                      f"record for the first {sample} particles of the same workload, templates from the "
-                      f"GPU bank: {scores} scores in {dt:.1f} s"}
+    from oracle import splat as osp
+    from oracle import torch_cpu as otc
+
+    def inputs(name, B):
+        wl = synthetic.WORKLOADS[name]
+        d = synthetic.make_workload(name, n_particles=B)
+        N = wl.box
+        mask = opj.generate_soft_mask((N, N), wl.mask_radius, wl.mask_edge)
+        fscv = d.get("fsc")
+        fscv = np.ones(N, np.float32) if fscv is None else np.asarray(fscv, np.float32)
+        fsc = ofr.centered_full_to_packed(opj.expand_radial_values(fscv, N))
+        local = [osr.local_orientations(float(d["rot"][b]), float(d["tilt"][b]), d["dir_rot"], d["dir_tilt"],
+                                        wl.local_range) for b in range(B)]
+        eff = [opt.psi_sampling_for_record(float(d["psi"][b]), wl.psi_step, True, wl.local_range)[1]
+               for b in range(B)]
+        ctf = opt.PackedCTF(N, wl.apix, wl.cs, wl.voltage, wl.kk)
+        dfo = np.stack([d["dfu"], d["dfv"], d["dfa"]], 1)
+        # the searched bank: the directions some particle searches (rows are
+        # independent of each other), local lists renumbered into it
+        used = sorted({i for l in local for i in l})
+        where = {i: k for k, i in enumerate(used)}
+        vol = osp.density_from_coords(d["coords"], d["numbers"], N, wl.apix, wl.res)
+        bank = otc.build_templates(vol, [d["dir_rot"][i] for i in used], [d["dir_tilt"][i] for i in used], N, mask,
+                                   fsc)
+        local = [[where[i] for i in l] for l in local]
+        return wl, d, mask, fsc, local, eff, ctf, dfo, d["images"].numpy(), vol, bank
+
+    default_threads = torch.get_num_threads()
+    results = {}
+    w96 = inputs("box96", synthetic.WORKLOADS["box96"].n_particles)
+    w256 = inputs("box256", int(box256_particles))
+    for threads in sorted({default_threads, 4}, reverse=True):
+        torch.set_num_threads(threads)
+        row = {}
+        for key, (wl, d, mask, fsc, local, eff, ctf, dfo, imgs, vol, bank) in (("box96", w96), ("box256", w256)):
+            N = wl.box
+            M = len(d["dir_rot"])
+            dirs = M if key == "box96" else min(int(template_dirs), M)
+
+            def density():
+                return osp.density_from_coords(d["coords"], d["numbers"], N, wl.apix, wl.res)
+
+            def spectrum():
+                return otc.volume_dft(vol)
+
+            dft = spectrum()
+
+            def templates():
+                return otc.build_templates(vol, d["dir_rot"][:dirs], d["dir_tilt"][:dirs], N, mask, fsc, dft=dft)
+
+            def search():
+                return otc.search(bank, imgs, dfo, eff, local, N, mask, ctf, wl.crop)
+
+            search()  # warm-up (FFT plans)
+            t_search, (_, n_sc) = _median_time(search, runs)
+            few = runs if key == "box96" else min(runs, 3)
+            t_density, _ = _median_time(density, few)
+            t_dft, _ = _median_time(spectrum, few)
+            t_tpl, _ = _median_time(templates, few)
+            t_tpl_all = t_dft + t_tpl * M / dirs  # one volume transform, then the per-direction work
+            r = {"particles": len(imgs), "scores": int(n_sc), "directions": M, "search_s": t_search,
+                 "search_scores_per_s": n_sc / t_search, "density_s": t_density, "templates_s": t_tpl_all}
+            if dirs != M:
+                r["templates_timed_directions"] = dirs
+            if key == "box96":
+                total = t_density + t_tpl_all + t_search
+                r.update(candidate_s=total, candidates_per_s=1.0 / total, end_to_end_scores_per_s=n_sc / total)
+            else:
+                per_score = t_search / n_sc
+                total = t_density + t_tpl_all + per_score * gpu_scores_per_candidate
+                r.update(extrapolated_scores=int(gpu_scores_per_candidate), candidate_s_extrapolated=total,
+                         candidates_per_s_extrapolated=1.0 / total,
+                         end_to_end_scores_per_s_extrapolated=gpu_scores_per_candidate / total)
+            row[key] = r
+            log("cpu baseline", threads, key, r)
+        results[str(threads)] = row
+    torch.set_num_threads(default_threads)
+    head = results[str(default_threads)]["box256"]
+    return {"value": head["end_to_end_scores_per_s_extrapolated"], "unit": "scores/s", "cores": default_threads,
+            "kind": "port", "cpu_model": _cpu_model(), "os_cpu_count": os.cpu_count(), "runs": runs,
+            "sample": f"oracle.torch_cpu (torch {torch.__version__} float32 on the host; v606 chunking) -- value: "
+                      f"box256 candidate extrapolated to {gpu_scores_per_candidate} scores (5,000 particles) "
+                      f"from {box256_particles} timed particles, density + 768 templates "
+                      f"(timed on {min(int(template_dirs), 768)}) + search, {default_threads} threads; "
+                      f"median of {runs}",
+            "threads": results}
 
 
 def top1(res, ps, center):
@@ -401,7 +501,7 @@ def main() -> int:
     ap.add_argument("--precision-subset", type=int, default=256)
     ap.add_argument("--no-fp16-variant", dest="fp16_variant", action="store_false",
                     help="skip the split-fp16 operand variant of the small-shift search")
-    ap.add_argument("--cpu-sample", type=int, default=1000)
+    ap.add_argument("--cpu-sample", type=int, default=100, help="box256 particles timed on the CPU (0 to skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pso-iters", type=int, default=20, help="batched PSO leg iterations (0 to skip)")
     ap.add_argument("--pso-slots", type=int, default=64)
@@ -648,8 +748,8 @@ def main() -> int:
     if args.fp16_variant and "mixed" in results:
         out["fp16_variant"] = fp16_variant(eng, ps, tk_last, results["mixed"], args.precision_subset,
                                            synthetic.SEED_CAND)
-    if not args.no_cpu_baseline and args.cpu_sample > 0 and bank is not None:
-        out["cpu_baseline"] = cpu_baseline(eng, ps, bank, args.cpu_sample)
+    if not args.no_cpu_baseline and args.cpu_sample > 0:
+        out["cpu_baseline"] = cpu_baseline(n_scores, args.cpu_sample, log=log)
     del ps, eng, bank, bank_split, tk_last, results
     torch.cuda.empty_cache()
     if args.pso_iters > 0:

@@ -113,6 +113,23 @@ def latency_stats(reqs: Sequence[Request]) -> dict:
                 mean_ms=float(lat.mean()), max_ms=float(lat.max()))
 
 
+def queue_growth(reqs: Sequence[Request], wall: float) -> dict:
+    """Whether the server kept up with the stream: the requests (and their
+    candidates) still open when the last one arrived, the time to drain
+    them, and the slope of latency against arrival time (about 0 when the
+    server keeps up; 1 - capacity / offered load when the queue grows)."""
+    if not reqs:
+        return {}
+    last = max(r.arrival for r in reqs)
+    open_reqs = [r for r in reqs if r.t_done is not None and r.t_done > last]
+    arr = np.asarray([r.arrival for r in reqs], np.float64)
+    lat = np.asarray([r.latency for r in reqs], np.float64)
+    slope = float(np.polyfit(arr, lat, 1)[0]) if len(reqs) > 2 and np.ptp(arr) > 0 else 0.0
+    return dict(last_arrival_s=float(last), open_requests_at_last_arrival=len(open_reqs),
+                open_candidates_at_last_arrival=int(sum(r.size for r in open_reqs)),
+                drain_s=float(wall - last), latency_growth_s_per_s=slope)
+
+
 def poisson_requests(n: int, rate: float, datasets: Sequence[str], seed: int = 0, mean_candidates: float = 8.0,
                      max_candidates: int = 64, pose_sigma=(2.0, 0.7), weights=None) -> List[Request]:
     """Seeded stream: Poisson arrivals at ``rate``/s, 1..max_candidates candidates
@@ -285,4 +302,5 @@ class SearchServer:
         stats.update(wall_s=wall, requests=len(done), candidates=cands, steps=steps,
                      candidates_per_s=cands / wall if wall > 0 else None,
                      scores_per_s=pairs / wall if wall > 0 else None)
+        stats.update(queue_growth(done, wall))
         return stats

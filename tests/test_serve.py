@@ -80,3 +80,18 @@ def test_served_results_equal_alone(hip_device, path):
         np.testing.assert_allclose(r.scores, alone, rtol=1e-5, err_msg=f"request {r.rid}")
     with pytest.raises(MemoryError):
         SearchServer(hip_device, hbm_budget_bytes=1 << 20, slots=2).add_dataset("box64", *parts["box64"])
+
+
+def test_queue_growth_reports_a_growing_queue():
+    from gisapr_amd.serve import queue_growth
+    kept = [Request(i, "a", np.zeros((2, 6)), float(i)) for i in range(5)]
+    for r in kept:
+        r.t_done = r.arrival + 0.1  # constant latency: the server keeps up
+    g = queue_growth(kept, 4.1)
+    assert g["open_requests_at_last_arrival"] == 1 and abs(g["latency_growth_s_per_s"]) < 1e-12
+    grow = [Request(i, "a", np.zeros((3, 6)), float(i)) for i in range(5)]
+    for i, r in enumerate(grow):
+        r.t_done = r.arrival + 0.5 * i + 0.1  # each request waits for all earlier ones
+    g = queue_growth(grow, 6.1)
+    assert g["open_requests_at_last_arrival"] == 2 and g["open_candidates_at_last_arrival"] == 6
+    assert abs(g["latency_growth_s_per_s"] - 0.5) < 1e-9 and abs(g["drain_s"] - 2.1) < 1e-9
