@@ -550,6 +550,10 @@ def gather_coefficients(src: torch.Tensor, kidx: torch.Tensor, kpad: int,
     return out
 
 
+# Largest |T_k P_k| the fp16 operand represents after its 2^14 scale (65504 / 2^14).
+FP16_PRODUCT_BOUND = 65504.0 / 16384.0
+
+
 def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.Tensor, dir_idx: torch.Tensor,
                     b_hi: torch.Tensor, b_lo: torch.Tensor, half: int, validate: bool = True,
                     pmap: Optional[torch.Tensor] = None, pt: int = 0, wpb: int = 0,
@@ -559,7 +563,11 @@ def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.T
     lag over [-half, half]^2 of every (direction, psi) pair; peak_y / peak_x
     are lag + half.  ``b_hi`` / ``b_lo`` are the lag table split in that
     operand type (``host.SmallShiftTables(..., operand=...)``).  ``validate``
-    checks dir_off / dir_idx against the operands on the host."""
+    checks dir_off / dir_idx against the operands on the host, and for fp16
+    that |T_k P_k| <= 65504 / 2^14; callers with ``validate=False`` must pass
+    unit-norm coefficient rows (every |T_k|, |P_k| <= 1), which is what
+    ``gather_coefficients`` with 1/norm scales and the template-coefficient
+    kernel produce."""
     if operand not in ("bf16", "fp16"):
         raise ValueError(f"operand {operand!r} not 'bf16' or 'fp16'")
     for t, name in ((tk, "tk"), (pk, "pk"), (dir_off, "dir_off"), (dir_idx, "dir_idx"), (b_hi, "b_hi"),
@@ -595,10 +603,10 @@ def correlate_mixed(tk: torch.Tensor, pk: torch.Tensor, P: int, dir_off: torch.T
         if rows is not None and not torch.equal(torch.sort(rows.long()).values,
                                                 torch.arange(total, device=rows.device)):
             raise ValueError("rows must be a permutation of the dir_idx rows")
-        # fp16 holds 2^14 |T_k P_k| only while it stays below 65504
+        # fp16 holds 2^14 |T_k P_k| only while it stays at most 65504
         if operand == "fp16" and tk.numel() and pk.numel() and \
-                float(tk.abs().max()) * float(pk.abs().max()) > 4.0:
-            raise ValueError("the fp16 operand needs |T_k P_k| <= 4 (unit-norm spectra)")
+                float(tk.abs().max()) * float(pk.abs().max()) > FP16_PRODUCT_BOUND:
+            raise ValueError(f"the fp16 operand needs |T_k P_k| <= {FP16_PRODUCT_BOUND} (unit-norm spectra)")
     if rows is not None:
         require_hip(rows, "rows")
         if rows.dtype != torch.int32 or rows.numel() != total:
