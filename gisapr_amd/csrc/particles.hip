@@ -13,11 +13,25 @@
 // ([B][P][n][n/2+1]); inv_norm[b][p] = 1 / ||spectrum|| is applied by the
 // correlation kernel (the reference divides before correlating).
 //
-// gfx950 mapping: one 256-thread workgroup per particle (persistent grid).
-// Row pass: for each psi, two rotated rows per complex LDS FFT, written to
-// the workgroup's scratch slot.  Column pass: per 8-column tile the CTF is
-// evaluated once into LDS and applied to all psi images of the particle.
-// Tile widths are compile-time powers of two (no runtime divisions).
+// gfx950 mapping: two kernels per chunk of particles, the row-transformed
+// images staged in a workspace small enough to stay in the last-level cache
+// (a chunk of at most 512 images):
+//   rows    one 256-thread block per (image, 16-row band): bilinear
+//           rotation of the band (8x8 blocked lanes, all loads in flight
+//           together), mask, two real rows per complex FFT (one wave per two
+//           sequences, in place in LDS), the half-spectrum split, written
+//           column-tile-major ([tile][row][8], one contiguous 16 KiB block
+//           per tile) so the column pass reads whole tiles;
+//   columns one 256-thread block per (particle, 8-column tile): the tile's
+//           CTF is evaluated once into LDS, then for each psi image the tile
+//           is loaded (the next image's tile is in flight during the
+//           current one's FFT), transformed along the rows, cropped,
+//           packed, multiplied by the CTF, conjugated and written; the
+//           block's sum of |v|^2 per image goes to a partial slot;
+//   norm    inv_norm = 1 / sqrt(sum of the partials in tile order)
+//           (deterministic).
+// Many small independent blocks keep every CU busy where one block per
+// particle waited on its own chain of barriers and scratch round trips.
 #include "common.h"
 
 #pragma clang fp contract(off)
@@ -25,20 +39,90 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kMaxPsiGroup = 8;  // psi images per pass (scratch slots per workgroup)
-constexpr int kTK = 8;           // columns per column tile
-constexpr int kRB = 8;           // complex rows (= 2 real rows) per row batch
+constexpr int kTK = 8;            // columns per column tile
+constexpr int kRows = 16;         // real rows per rows-kernel block (8 complex sequences)
+constexpr int kPad = 3;           // one LDS pad slot per 8 elements
+constexpr int kMaxChunkImages = 512;
 
-// Scratch layout of one row-transformed image: tile-major [NT][N][kTK] with
-// NT = ceil((N/2+1)/kTK), so a column tile is one contiguous 16 KiB block.
+// Stage layout of one row-transformed image: tile-major [NT][N][kTK],
+// NT = ceil((N/2+1)/kTK), a column tile one contiguous block.  In LDS a
+// block's sequences start SL float2 apart, SL = 8 mod 32, so the eight
+// sequences' words of one row fall on different banks.
 template <int N>
 struct PrepLayout {
   static constexpr int MH = N / 2 + 1;
-  static constexpr int SP = N + 1;                 // padded column stride in LDS
-  static constexpr int NT = (MH + kTK - 1) / kTK;  // column tiles
-  static constexpr size_t IMG = (size_t)NT * N * kTK;            // float2 per image
-  static constexpr size_t SLOT = (size_t)kMaxPsiGroup * IMG;     // float2 per workgroup
+  static constexpr int NT = (MH + kTK - 1) / kTK;               // column tiles
+  static constexpr size_t IMG = (size_t)NT * N * kTK;           // float2 per image
+  static constexpr int PL = lds_padded_len<N, kPad>();          // padded sequence length
+  static constexpr int SL = PL + ((8 - PL % 32) + 32) % 32;     // sequence stride
 };
+
+// One stage of fft_wave_inplace over two sequences of one wave at once:
+// butterfly j2 < 2 L/R of the stage belongs to sequence j2 / (L/R), so a
+// stage with L/R = 32 butterflies keeps all 64 lanes busy.  Each butterfly
+// is computed exactly as in stockham_stage_inplace (same results).
+template <int L, int R, bool INV, int PAD>
+__device__ __forceinline__ void stage_inplace2(float2* s0, float2* s1, int Ns, const float2* __restrict__ tw,
+                                               int lane) {
+  constexpr int LR = L / R;
+  constexpr int NB = (2 * LR + 63) / 64;
+  const int step = L / (Ns * R);
+  float2 v[NB][R];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j2 = lane + 64 * b;
+    if ((2 * LR) % 64 == 0 || j2 < 2 * LR) {
+      const int sq = j2 >= LR ? 1 : 0;
+      const float2* s = sq ? s1 : s0;
+      const int j = j2 - sq * LR;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = s[lds_pad<PAD>(j + r * LR)];
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j2 = lane + 64 * b;
+    if ((2 * LR) % 64 == 0 || j2 < 2 * LR) {
+      const int sq = j2 >= LR ? 1 : 0;
+      float2* s = sq ? s1 : s0;
+      const int j = j2 - sq * LR;
+      const int k = j % Ns;
+      if (Ns > 1)  // the first stage's twiddles are all 1
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[k * r * step];
+          if (INV) w.y = -w.y;
+          v[b][r] = cmul(v[b][r], w);
+        }
+      dft_radix<R, INV>(v[b]);
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[lds_pad<PAD>(base + r * Ns)] = v[b][r];
+    }
+  }
+  wave_lds_sync();
+}
+
+template <int L, bool INV, int PAD>
+__device__ __forceinline__ void fft_wave_inplace2(float2* s0, float2* s1, const float2* __restrict__ tw, int lane) {
+  using S = FftShape<L>;
+  int Ns = 1;
+#pragma unroll
+  for (int st = 0; st < S::n8; ++st) {
+    stage_inplace2<L, 8, INV, PAD>(s0, s1, Ns, tw, lane);
+    Ns *= 8;
+  }
+  if constexpr (S::n4) {
+    stage_inplace2<L, 4, INV, PAD>(s0, s1, Ns, tw, lane);
+    Ns *= 4;
+  }
+  if constexpr (S::n2) {
+    stage_inplace2<L, 2, INV, PAD>(s0, s1, Ns, tw, lane);
+    Ns *= 2;
+  }
+  if constexpr (S::three) stage_inplace2<L, 3, INV, PAD>(s0, s1, Ns, tw, lane);
+}
 
 typedef float f2a4 __attribute__((ext_vector_type(2), aligned(4)));
 
@@ -50,195 +134,265 @@ __device__ __forceinline__ float ctf_weight(float x, float y, float u4, float nc
   return cv * sqrtf(1.0f / (nc + kk * cv * cv));
 }
 
+// rotp: [B][P][6] = c, -s, s, c, off_y, off_x of the psi sampling grid.
+// Block = (image i0 + blockIdx.x / bands, band of kRows rows); X holds the
+// chunk's images from image i0 on.
+template <int N, bool MASK>
+__global__ void __launch_bounds__(kThreads)
+prep_rows_kernel(const float* __restrict__ images, int P, int n, int i0, const float* __restrict__ rotp,
+                 const float* __restrict__ mask, const float2* __restrict__ twg, float2* __restrict__ X) {
+  using T = PrepLayout<N>;
+  constexpr int MH = T::MH, SL = T::SL, bands = N / kRows;
+  __shared__ float2 tw[N];
+  __shared__ float2 seq[(kRows / 2) * SL];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = blockIdx.x / bands, band = blockIdx.x - li * bands;
+  const int img = i0 + li, b = img / P;
+  const int y0 = band * kRows;
+  for (int k = tid; k < N; k += kThreads) tw[k] = twg[k];
+  const float* src = images + (size_t)b * N * N;
+  const float* rp = rotp + (size_t)img * 6;
+  const float m00 = rp[0], m01 = rp[1], m10 = rp[2], m11 = rp[3], oy = rp[4], ox = rp[5];
+  // Each wave samples 8x8 output blocks, so one load instruction's addresses
+  // stay within a compact rotated source patch; the two horizontal taps of a
+  // row come from one 8-byte load.  Branch-free: the 2x2 footprint is read at
+  // clamped coordinates (always in bounds) and taps outside the image are
+  // zeroed by selects.  The loads of a group of GI pixels are all issued
+  // before any is used.
+  constexpr int BPR = N / 8;
+  constexpr int ITER = 2 * BPR / (kThreads / 64);
+  constexpr int GI = (ITER % 2 == 0 && ITER > 8) ? ITER / 2 : ITER;
+  float* seq_f = (float*)seq;
+  auto coords = [&](int it, int& yl, int& Y, int& Xc, float& sy, float& sx) {
+    const int blk = wave + it * (kThreads / 64);
+    const int hb = blk / BPR, bc = blk - hb * BPR;
+    yl = hb * 8 + (lane >> 3);
+    Y = y0 + yl;
+    Xc = bc * 8 + (lane & 7);
+    sy = (float)Y * m00 + (float)Xc * m01 + oy;
+    sx = (float)Y * m10 + (float)Xc * m11 + ox;
+  };
+#pragma unroll
+  for (int g0 = 0; g0 < ITER; g0 += GI) {
+    f2a4 r0[GI], r1[GI];
+    float mv[GI];
+#pragma unroll
+    for (int u = 0; u < GI; ++u) {
+      int yl, Y, Xc;
+      float sy, sx;
+      coords(g0 + u, yl, Y, Xc, sy, sx);
+      // clamp before converting so far-outside samples stay finite ints
+      const int iy = (int)fminf(fmaxf(floorf(sy), -2.0f), (float)N);
+      const int ix = (int)fminf(fmaxf(floorf(sx), -2.0f), (float)N);
+      const int cy = min(max(iy, 0), N - 2), cx = min(max(ix, 0), N - 2);
+      const float* s0 = src + (size_t)cy * N + cx;
+      r0[u] = *(const f2a4*)s0;
+      r1[u] = *(const f2a4*)(s0 + N);
+      if constexpr (MASK) mv[u] = mask[(size_t)Y * N + Xc];
+    }
+#pragma unroll
+    for (int u = 0; u < GI; ++u) {
+      int yl, Y, Xc;
+      float sy, sx;
+      coords(g0 + u, yl, Y, Xc, sy, sx);
+      const float fy0 = floorf(sy), fx0 = floorf(sx);
+      const float ty = sy - fy0, tx = sx - fx0;
+      const int iy = (int)fminf(fmaxf(fy0, -2.0f), (float)N);
+      const int ix = (int)fminf(fmaxf(fx0, -2.0f), (float)N);
+      // the loaded 2x2 block starts at (clamp(iy), clamp(ix)); inside the
+      // image tap row iy is its row 0 except at the bottom edge (iy = N-1,
+      // block at N-2), and tap row iy+1 its row 1 except at the top edge
+      // (iy = -1, block at 0); likewise for columns; taps outside are zero
+      const bool y0ok = (unsigned)iy < (unsigned)N, y1ok = (unsigned)(iy + 1) < (unsigned)N;
+      const bool x0ok = (unsigned)ix < (unsigned)N, x1ok = (unsigned)(ix + 1) < (unsigned)N;
+      const bool top = iy < 0, bottom = iy >= N - 1, left = ix < 0, right = ix >= N - 1;
+      const f2a4 a0 = r0[u], a1 = r1[u];
+      const f2a4 row0 = bottom ? a1 : a0;   // tap row iy
+      const f2a4 row1 = top ? a0 : a1;      // tap row iy + 1
+      const float a00 = (y0ok && x0ok) ? (right ? row0.y : row0.x) : 0.f;
+      const float a01 = (y0ok && x1ok) ? (left ? row0.x : row0.y) : 0.f;
+      const float a10 = (y1ok && x0ok) ? (right ? row1.y : row1.x) : 0.f;
+      const float a11 = (y1ok && x1ok) ? (left ? row1.x : row1.y) : 0.f;
+      float v = ((1.0f - ty) * (1.0f - tx)) * a00 + ((1.0f - ty) * tx) * a01 +
+                (ty * (1.0f - tx)) * a10 + (ty * tx) * a11;
+      if constexpr (MASK) v *= mv[u];
+      seq_f[2 * ((yl >> 1) * SL + lds_pad<kPad>(Xc)) + (yl & 1)] = v;  // row pair -> (re, im)
+    }
+  }
+  __syncthreads();
+  static_assert(kRows / 2 == 2 * (kThreads / 64), "two sequences per wave");
+  fft_wave_inplace2<N, false, kPad>(seq + (2 * wave) * SL, seq + (2 * wave + 1) * SL, tw, lane);
+  __syncthreads();
+  // split the two real rows' spectra; write the tiles the crop uses
+  const int tiles = n / 2 / kTK + 1;
+  float2* Xi = X + (size_t)li * T::IMG;
+  constexpr int KW = T::NT * kTK;
+  for (int t = tid; t < (kRows / 2) * KW; t += kThreads) {
+    const int q = t / KW, k = t - q * KW;
+    if (k >= MH || k / kTK >= tiles) continue;
+    const float2 z = seq[q * SL + lds_pad<kPad>(k)];
+    const float2 zc = cconj(seq[q * SL + lds_pad<kPad>((N - k) % N)]);
+    const int ya = y0 + 2 * q;
+    float2* Wk = Xi + ((size_t)(k / kTK) * N) * kTK + (k % kTK);
+    Wk[(size_t)ya * kTK] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+    Wk[(size_t)(ya + 1) * kTK] = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+  }
+}
+
 // ctf_tab with defocus:  [4][n][n/2+1] = x, y, u4, ncurve (PackedCTFTorch.__init__)
 // ctf_tab without:       [B][n][n/2+1] explicit packed CTF, or NULL (ones)
-// rotp:                  [B][P][6] = c, -s, s, c, off_y, off_x of the psi sampling grid
+// Block = (particle b0 + blockIdx.x / tiles, column tile); the chunk starts
+// at particle b0, whose first image is X's image 0.
 template <int N>
 __global__ void __launch_bounds__(kThreads)
-prepare_particles_kernel(const float* __restrict__ images, int B, int P, int n,
-                         const float* __restrict__ rotp, const float* __restrict__ mask,
-                         const float* __restrict__ ctf_tab, const float* __restrict__ defocus,
-                         float k1, float k2, float k3, float k5, float kk,
-                         float2* __restrict__ spectra, float* __restrict__ inv_norm,
-                         float2* __restrict__ scratch) {
+prep_cols_kernel(int P, int n, int b0, const float* __restrict__ ctf_tab, const float* __restrict__ defocus,
+                 float k1, float k2, float k3, float k5, float kk, const float2* __restrict__ twg,
+                 const float2* __restrict__ X, float2* __restrict__ spectra, double* __restrict__ partial) {
   using T = PrepLayout<N>;
-  constexpr int MH = T::MH, SP = T::SP;
+  constexpr int MH = T::MH, SL = T::SL;
   __shared__ float2 tw[N];
-  __shared__ float2 bufA[kTK * SP > kRB * N ? kTK * SP : kRB * N];
-  __shared__ float2 bufB[kTK * SP > kRB * N ? kTK * SP : kRB * N];
+  __shared__ float2 seq[kTK * SL];
   __shared__ float ctf_l[256 * kTK];  // [row][q], rows < n <= N <= 256
-  __shared__ double red[kMaxPsiGroup * (kThreads / 64)];
-  const int tid = threadIdx.x;
-  constexpr int nthr = kThreads;  // launched with exactly kThreads threads
-  fill_twiddles(tw, N, tid, nthr);
-  float2* Wbase = scratch + (size_t)blockIdx.x * T::SLOT;
+  __shared__ double red[kThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tiles = n / 2 / kTK + 1;
+  const int lb = blockIdx.x / tiles, tile = blockIdx.x - lb * tiles;
+  const int b = b0 + lb;
+  const int k0 = tile * kTK, kmax = n / 2;
   const int mh_n = n / 2 + 1;
   const size_t plane_n = (size_t)n * mh_n;
+  for (int k = tid; k < N; k += kThreads) tw[k] = twg[k];
   const bool has_ctf = defocus != nullptr;
   const bool explicit_ctf = !has_ctf && ctf_tab != nullptr;
-
-  for (int b = blockIdx.x; b < B; b += gridDim.x) {
-    const float* src = images + (size_t)b * N * N;
-    float axx = 0.f, ayy = 0.f, axy = 0.f;
-    if (has_ctf) {
-      const float dfu = defocus[3 * b + 0], dfv = defocus[3 * b + 1];
-      const float ang = defocus[3 * b + 2] * 0.017453292519943295f;
-      const float sa = sinf(ang), ca = cosf(ang);
-      axx = (-dfu * ca) * ca - (dfv * sa) * sa;
-      ayy = (-dfu * sa) * sa - (dfv * ca) * ca;
-      axy = (-dfu * sa) * ca + (dfv * sa) * ca;
+  float axx = 0.f, ayy = 0.f, axy = 0.f;
+  if (has_ctf) {
+    const float dfu = defocus[3 * b + 0], dfv = defocus[3 * b + 1];
+    const float ang = defocus[3 * b + 2] * 0.017453292519943295f;
+    const float sa = sinf(ang), ca = cosf(ang);
+    axx = (-dfu * ca) * ca - (dfv * sa) * sa;
+    ayy = (-dfu * sa) * sa - (dfv * ca) * ca;
+    axy = (-dfu * sa) * ca + (dfv * sa) * ca;
+  }
+  for (int t = tid; t < n * kTK; t += kThreads) {
+    const int r = t / kTK, q = t - r * kTK;
+    const int c = k0 + q;
+    float w = 1.0f;
+    if (c <= kmax) {
+      const size_t e = (size_t)r * mh_n + c;
+      if (has_ctf)
+        w = ctf_weight(ctf_tab[e], ctf_tab[plane_n + e], ctf_tab[2 * plane_n + e], ctf_tab[3 * plane_n + e], axx,
+                       axy, ayy, k1, k2, k3, k5, kk);
+      else if (explicit_ctf)
+        w = ctf_tab[(size_t)b * plane_n + e];
     }
-    for (int pg = 0; pg < P; pg += kMaxPsiGroup) {
-      const int np = min(kMaxPsiGroup, P - pg);
-
-      // ---- rows: bilinear rotation (zeros padding), mask, forward FFT ---------
-      for (int pi = 0; pi < np; ++pi) {
-        const float* rp = rotp + ((size_t)b * P + pg + pi) * 6;
-        const float m00 = rp[0], m01 = rp[1], m10 = rp[2], m11 = rp[3], oy = rp[4], ox = rp[5];
-        float2* W = Wbase + (size_t)pi * T::IMG;
-        for (int y0 = 0; y0 < N; y0 += 2 * kRB) {
-          __syncthreads();
-          // Each wave samples 8x8 output blocks, so one load instruction's
-          // addresses stay within a compact rotated source patch; the two
-          // horizontal taps of a row come from one 8-byte load.
-          // Branch-free: the 2x2 footprint is read at clamped coordinates (always
-          // in bounds) and taps outside the image are zeroed by selects, so
-          // all ITER iterations' loads can be in flight together.
-          constexpr int BPR = N / 8;
-          constexpr int ITER = 2 * BPR / (kThreads / 64);
-          float* bufA_f = (float*)bufA;
-          const int lane = tid & 63;
+    ctf_l[t] = w;
+  }
+  constexpr int PER = N * kTK / kThreads;  // tile elements per thread
+  static_assert(PER * kThreads == N * kTK, "tile not divisible by the block");
+  float2 nxt[PER];
+  auto load_tile = [&](int p) {
+    const float2* Xt = X + ((size_t)lb * P + p) * T::IMG + (size_t)tile * N * kTK;
 #pragma unroll
-          for (int it = 0; it < ITER; ++it) {
-            const int blk = (tid >> 6) + it * (kThreads / 64);
-            const int band = blk / BPR, bc = blk - band * BPR;
-            const int yl = band * 8 + (lane >> 3);
-            const int Y = y0 + yl, X = bc * 8 + (lane & 7);
-            const float sy = (float)Y * m00 + (float)X * m01 + oy;
-            const float sx = (float)Y * m10 + (float)X * m11 + ox;
-            const float fy0 = floorf(sy), fx0 = floorf(sx);
-            const float ty = sy - fy0, tx = sx - fx0;
-            // clamp before converting so far-outside samples stay finite ints
-            const int iy = (int)fminf(fmaxf(fy0, -2.0f), (float)N);
-            const int ix = (int)fminf(fmaxf(fx0, -2.0f), (float)N);
-            const int cy = min(max(iy, 0), N - 2), cx = min(max(ix, 0), N - 2);
-            const float* s0 = src + (size_t)cy * N + cx;
-            const f2a4 r0 = *(const f2a4*)s0;
-            const f2a4 r1 = *(const f2a4*)(s0 + N);
-            const bool y0ok = iy >= 0 && iy < N, y1ok = iy + 1 >= 0 && iy + 1 < N;
-            const bool x0ok = ix >= 0 && ix < N, x1ok = ix + 1 >= 0 && ix + 1 < N;
-            // offsets of each valid tap inside the loaded 2x2 block are 0 or 1
-            const int dy0 = iy - cy, dy1 = iy + 1 - cy, dx0 = ix - cx, dx1 = ix + 1 - cx;
-            auto pick = [&](int dy, int dx) { return dy ? (dx ? r1.y : r1.x) : (dx ? r0.y : r0.x); };
-            const float a00 = (y0ok && x0ok) ? pick(dy0, dx0) : 0.f;
-            const float a01 = (y0ok && x1ok) ? pick(dy0, dx1) : 0.f;
-            const float a10 = (y1ok && x0ok) ? pick(dy1, dx0) : 0.f;
-            const float a11 = (y1ok && x1ok) ? pick(dy1, dx1) : 0.f;
-            float v = ((1.0f - ty) * (1.0f - tx)) * a00 + ((1.0f - ty) * tx) * a01 +
-                      (ty * (1.0f - tx)) * a10 + (ty * tx) * a11;
-            if (mask) v *= mask[(size_t)Y * N + X];
-            bufA_f[2 * ((yl >> 1) * N + X) + (yl & 1)] = v;  // row pair -> (re, im)
-          }
-          __syncthreads();
-          float2* res = fft_lds<N, false>(bufA, bufB, kRB, N, tw, tid, nthr);
-          for (int t = tid; t < kRB * MH; t += nthr) {
-            const int q = t / MH, k = t - q * MH;
-            const float2 z = res[q * N + k];
-            const float2 zc = cconj(res[q * N + (N - k) % N]);
-            const int ya = y0 + 2 * q;
-            float2* Wk = W + ((size_t)(k / kTK) * N) * kTK + (k % kTK);
-            Wk[(size_t)ya * kTK] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
-            Wk[(size_t)(ya + 1) * kTK] = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
-          }
-        }
+    for (int u = 0; u < PER; ++u) nxt[u] = Xt[tid + u * kThreads];
+  };
+  load_tile(0);
+  for (int p = 0; p < P; ++p) {
+    __syncthreads();  // the previous image's output reads are done
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int t = tid + u * kThreads;
+      const int Y = t / kTK, q = t - Y * kTK;
+      seq[q * SL + lds_pad<kPad>(Y)] = k0 + q < MH ? nxt[u] : make_float2(0.f, 0.f);
+    }
+    if (p + 1 < P) load_tile(p + 1);  // in flight during this image's transform
+    __syncthreads();
+    static_assert(kTK == 2 * (kThreads / 64), "two sequences per wave");
+    fft_wave_inplace2<N, false, kPad>(seq + (2 * wave) * SL, seq + (2 * wave + 1) * SL, tw, lane);
+    __syncthreads();
+    float2* out = spectra + ((size_t)b * P + p) * plane_n;
+    double acc = 0.0;
+    for (int t = tid; t < n * kTK; t += kThreads) {
+      const int r = t / kTK, q = t - r * kTK;
+      const int c = k0 + q;
+      if (c > kmax) continue;
+      const int fy = r < n / 2 ? r : r - n;
+      float2 v;
+      if (c < n / 2 || n == N) {
+        v = seq[q * SL + lds_pad<kPad>(fy < 0 ? fy + N : fy)];
+      } else {  // packed column n/2 holds frequency -n/2 = conj of (+n/2) at -fy
+        const int my = -fy;
+        v = cconj(seq[q * SL + lds_pad<kPad>(my < 0 ? my + N : my)]);
       }
-
-      // ---- columns: CTF tile once, then FFT/crop/pack/CTF/conj per psi ------
-      double nrm[kMaxPsiGroup];
-#pragma unroll
-      for (int i = 0; i < kMaxPsiGroup; ++i) nrm[i] = 0.0;
-      const int kmax = n / 2;
-      for (int k0 = 0; k0 <= kmax; k0 += kTK) {
-        __syncthreads();
-        for (int t = tid; t < n * kTK; t += nthr) {
-          const int r = t / kTK, q = t - r * kTK;
-          const int c = k0 + q;
-          float w = 1.0f;
-          if (c <= kmax) {
-            const size_t e = (size_t)r * mh_n + c;
-            if (has_ctf)
-              w = ctf_weight(ctf_tab[e], ctf_tab[plane_n + e], ctf_tab[2 * plane_n + e], ctf_tab[3 * plane_n + e],
-                             axx, axy, ayy, k1, k2, k3, k5, kk);
-            else if (explicit_ctf)
-              w = ctf_tab[(size_t)b * plane_n + e];
-          }
-          ctf_l[t] = w;
-        }
-#pragma unroll 1
-        for (int pi = 0; pi < np; ++pi) {
-          const float2* Wt = Wbase + (size_t)pi * T::IMG + (size_t)(k0 / kTK) * N * kTK;
-          for (int t = tid; t < kTK * N; t += nthr) {  // contiguous tile block
-            const int Y = t / kTK, q = t - Y * kTK;
-            const int c = k0 + q;
-            bufA[q * SP + Y] = c < MH ? Wt[t] : make_float2(0.f, 0.f);
-          }
-          __syncthreads();
-          float2* res = fft_lds<N, false>(bufA, bufB, kTK, SP, tw, tid, nthr);
-          float2* out = spectra + ((size_t)b * P + pg + pi) * plane_n;
-          double acc = 0.0;
-          for (int t = tid; t < n * kTK; t += nthr) {
-            const int r = t / kTK, q = t - r * kTK;
-            const int c = k0 + q;
-            if (c > kmax) continue;
-            const int fy = r < n / 2 ? r : r - n;
-            float2 v;
-            if (c < n / 2 || n == N) {
-              v = res[q * SP + (fy < 0 ? fy + N : fy)];
-            } else {  // packed column n/2 holds frequency -n/2 = conj of (+n/2) at -fy
-              const int my = -fy;
-              v = cconj(res[q * SP + (my < 0 ? my + N : my)]);
-            }
-            const float w = ctf_l[t];
-            v.x *= w;
-            v.y = -(v.y * w);  // conj
-            out[(size_t)r * mh_n + c] = v;
-            acc += (double)v.x * v.x + (double)v.y * v.y;
-          }
-#pragma unroll
-          for (int i = 0; i < kMaxPsiGroup; ++i)
-            if (i == pi) nrm[i] += acc;
-          __syncthreads();
-        }
-      }
-      block_sum_f64<kMaxPsiGroup>(nrm, red);
-      if (tid < np) {
-        double v = 0.0;
-#pragma unroll
-        for (int i = 0; i < kMaxPsiGroup; ++i)
-          if (i == tid) v = nrm[i];
-        inv_norm[(size_t)b * P + pg + tid] = 1.0f / (float)sqrt(v);
-      }
+      const float w = ctf_l[t];
+      v.x *= w;
+      v.y = -(v.y * w);  // conj
+      out[(size_t)r * mh_n + c] = v;
+      acc += (double)v.x * v.x + (double)v.y * v.y;
+    }
+    acc = wave_sum_f64(acc);
+    if (lane == 0) red[wave] = acc;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) s += red[w];
+      partial[((size_t)lb * P + p) * T::NT + tile] = s;
     }
   }
 }
+
+// The forward twiddle table of length N (the same values every FFT here uses).
+__global__ void prep_twiddle_kernel(float2* __restrict__ tw, int N) {
+  fill_twiddles(tw, N, threadIdx.x, blockDim.x);
+}
+
+// inv_norm[b][p] = 1 / sqrt(sum over tiles of partial), tiles in order.
+__global__ void prep_norm_kernel(const double* __restrict__ partial, int nimg, int stride, int tiles, int i0,
+                                 float* __restrict__ inv_norm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nimg) return;
+  double s = 0.0;
+  for (int t = 0; t < tiles; ++t) s += partial[(size_t)i * stride + t];
+  inv_norm[i0 + i] = 1.0f / (float)sqrt(s);
+}
+
+template <int N>
+size_t prep_image_bytes() { return PrepLayout<N>::IMG * sizeof(float2) + PrepLayout<N>::NT * sizeof(double); }
+constexpr size_t kTwiddleBytes = 256 * sizeof(float2);  // table at the workspace's head (N <= 256)
 
 template <int N>
 int launch_prepare(const float* images, int B, int P, int n, const float* rotp, const float* mask,
                    const float* ctf_tab, const float* defocus, const float* k, float2* spectra,
                    float* inv_norm, void* ws, size_t ws_bytes, hipStream_t st) {
   using T = PrepLayout<N>;
-  const size_t slot = T::SLOT * sizeof(float2);
-  int grid = (int)std::min<size_t>((size_t)B, ws_bytes / slot);
-  grid = std::min(grid, 4 * gis_cu_count());
-  GIS_REQUIRE(grid >= 1, "gis_prepare_particles: workspace of %zu bytes is below one slot (%zu)", ws_bytes, slot);
-  hipLaunchKernelGGL(prepare_particles_kernel<N>, dim3(grid), dim3(kThreads), 0, st, images, B, P, n, rotp,
-                     mask, ctf_tab, defocus, k[0], k[1], k[2], k[3], k[4], spectra, inv_norm, (float2*)ws);
-  GIS_LAUNCH_CHECK("gis_prepare_particles");
+  const size_t cap = ws_bytes > kTwiddleBytes ? (ws_bytes - kTwiddleBytes) / prep_image_bytes<N>() : 0;
+  const int chunk = (int)std::min<size_t>((size_t)B, cap / (size_t)P);
+  GIS_REQUIRE(chunk >= 1, "gis_prepare_particles: workspace of %zu bytes holds %zu images, below one particle (%d)",
+              ws_bytes, cap, P);
+  float2* twg = (float2*)ws;
+  float2* X = (float2*)((char*)ws + kTwiddleBytes);
+  double* partial = (double*)((char*)X + (size_t)chunk * P * T::IMG * sizeof(float2));
+  const int tiles = n / 2 / kTK + 1;
+  hipLaunchKernelGGL(prep_twiddle_kernel, dim3(1), dim3(256), 0, st, twg, N);
+  GIS_LAUNCH_CHECK("gis_prepare_particles twiddles");
+  for (int b0 = 0; b0 < B; b0 += chunk) {
+    const int nb = std::min(chunk, B - b0);
+    const int i0 = b0 * P, nimg = nb * P;
+    if (mask)
+      hipLaunchKernelGGL((prep_rows_kernel<N, true>), dim3((unsigned)(nimg * (N / kRows))), dim3(kThreads), 0, st,
+                         images, P, n, i0, rotp, mask, twg, X);
+    else
+      hipLaunchKernelGGL((prep_rows_kernel<N, false>), dim3((unsigned)(nimg * (N / kRows))), dim3(kThreads), 0, st,
+                         images, P, n, i0, rotp, mask, twg, X);
+    GIS_LAUNCH_CHECK("gis_prepare_particles rows");
+    hipLaunchKernelGGL(prep_cols_kernel<N>, dim3((unsigned)(nb * tiles)), dim3(kThreads), 0, st, P, n, b0,
+                       ctf_tab, defocus, k[0], k[1], k[2], k[3], k[4], twg, X, spectra, partial);
+    GIS_LAUNCH_CHECK("gis_prepare_particles columns");
+    hipLaunchKernelGGL(prep_norm_kernel, dim3((unsigned)((nimg + 255) / 256)), dim3(256), 0, st, partial, nimg,
+                       T::NT, tiles, i0, inv_norm);
+    GIS_LAUNCH_CHECK("gis_prepare_particles norm");
+  }
   return GIS_OK;
 }
-
-template <int N>
-size_t prep_slot_bytes() { return PrepLayout<N>::SLOT * sizeof(float2); }
 
 __global__ void __launch_bounds__(256)
 ctf_packed_kernel(const float* __restrict__ tab, const float* __restrict__ defocus, int B, int n, float k1,
@@ -277,20 +431,28 @@ GIS_API int gis_ctf_packed(const float* ctf_tab, const float* defocus, int B, in
   return GIS_OK;
 }
 
-// Workspace bytes for gis_prepare_particles over `particles` particles.
-GIS_API size_t gis_prepare_particles_workspace(int N, int particles) {
+// Workspace bytes for gis_prepare_particles over `particles` particles of P
+// psi images each: a chunk of min(particles * P, max(512, P)) row-transformed
+// images and their norm partials (the call processes the particles in
+// chunks that fit, at least one particle).
+GIS_API size_t gis_prepare_particles_workspace_ex(int N, int particles, int P) {
   size_t s = 0;
   switch (N) {
-    case 32: s = prep_slot_bytes<32>(); break;
-    case 64: s = prep_slot_bytes<64>(); break;
-    case 96: s = prep_slot_bytes<96>(); break;
-    case 128: s = prep_slot_bytes<128>(); break;
-    case 192: s = prep_slot_bytes<192>(); break;
-    case 256: s = prep_slot_bytes<256>(); break;
+    case 32: s = prep_image_bytes<32>(); break;
+    case 64: s = prep_image_bytes<64>(); break;
+    case 96: s = prep_image_bytes<96>(); break;
+    case 128: s = prep_image_bytes<128>(); break;
+    case 192: s = prep_image_bytes<192>(); break;
+    case 256: s = prep_image_bytes<256>(); break;
     default: return 0;
   }
-  if (particles <= 0) return 0;
-  return std::min<size_t>((size_t)particles, (size_t)4 * gis_cu_count()) * s;
+  if (particles <= 0 || P <= 0) return 0;
+  return kTwiddleBytes + std::min<size_t>((size_t)particles * P, (size_t)std::max(kMaxChunkImages, P)) * s;
+}
+
+// The same for up to 64 psi images per particle.
+GIS_API size_t gis_prepare_particles_workspace(int N, int particles) {
+  return gis_prepare_particles_workspace_ex(N, particles, 64);
 }
 
 // Replaces ParticleFourierPreparer.prepare (v606:1340-1374) together with the

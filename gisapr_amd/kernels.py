@@ -358,7 +358,7 @@ def prepare_particles(images: torch.Tensor, new_box: int, rotp: torch.Tensor, ma
     if inv_norm is None:
         inv_norm = torch.empty((B, P), dtype=torch.float32, device=images.device)
     kc = (C.c_float * 5)(*[float(v) for v in np.asarray(kconst, np.float32)])
-    wsb = int(lib().gis_prepare_particles_workspace(N, B))
+    wsb = int(lib().gis_prepare_particles_workspace_ex(N, B, P))
     ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=images.device)
     check(lib().gis_prepare_particles(ptr(images), B, N, n, P, ptr(rotp), ptr(mask), ptr(ctf_tab),
                                       ptr(defocus), C.cast(kc, C.c_void_p), ptr(out), ptr(inv_norm), ptr(ws),

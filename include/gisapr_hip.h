@@ -198,8 +198,12 @@ int gis_slice_sample_grid(const float* base_zyx, int P, const float* R, int M, i
  */
 int gis_sample_dft_3d(const void* dft, int D, int H, int W, const float* coords, size_t P, void* out, void* stream);
 
-/* Workspace bytes gis_prepare_particles needs for `particles` (= B) particles. */
+/* Workspace bytes gis_prepare_particles needs for `particles` (= B) particles
+ * of up to 64 psi images each; _ex for P psi images.  The call works through
+ * the particles in chunks of the images the workspace holds (at least one
+ * particle's P images). */
 size_t gis_prepare_particles_workspace(int N, int particles);
+size_t gis_prepare_particles_workspace_ex(int N, int particles, int P);
 
 /*
  * (b') Particle preparation.  Replaces ParticleFourierPreparer.prepare
