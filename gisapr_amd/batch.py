@@ -93,7 +93,7 @@ class Autotuner:
 
 
 MIXED_VARIANTS = [dict(pt=5, wpb=4), dict(pt=5, wpb=2), dict(pt=3, wpb=4), dict(pt=1, wpb=4)]
-SPLIT_VARIANTS = [dict(dt=1, pt=5), dict(dt=2, pt=3), dict(dt=1, pt=3), dict(dt=2, pt=2)]
+SPLIT_VARIANTS = [dict(dt=2, pt=5), dict(dt=1, pt=5), dict(dt=4, pt=5), dict(dt=2, pt=3)]
 FP32_VARIANTS = [dict(jt=0), dict(jt=2), dict(jt=4), dict(jt=6)]
 
 

@@ -436,7 +436,7 @@ int split_check(const char* who, int B, int P, int n, int C, int nrows, int dt, 
     gis_set_error("%s: crop %d must be even, in [2, 32] and <= newbox", who, C);
     return GIS_EUNSUPPORTED;
   }
-  GIS_REQUIRE(dt >= 0 && dt <= 2, "%s: dt %d not in [0, 2]", who, dt);
+  GIS_REQUIRE(dt == 0 || dt == 1 || dt == 2 || dt == 4, "%s: dt %d not in {0, 1, 2, 4}", who, dt);
   GIS_REQUIRE(pt >= 0 && pt <= 5, "%s: pt %d not in [0, 5]", who, pt);
   return GIS_OK;
 }
@@ -450,8 +450,8 @@ int split_run(const void* bank, const void* spectra, const float* inv_norm, cons
               void* ws, hipStream_t st) {
   const int LT = split_lt(C);
   if (pt == 0) pt = std::min(P, 5);
-  if (dt == 0 || LT == 2) dt = 1;
-  if (dt == 2) pt = std::min(pt, 3);  // register budget of two directions per wave
+  if (dt == 0) dt = 2;  // measured fastest at box 256 (psi spectra shared by two directions)
+  if (LT == 2) dt = 1;  // two lag tiles per axis: one direction per wave (registers)
   bf16x8* Wt = (bf16x8*)ws;
   bf16x8* Et = Wt + split_w_frags(n, C);
   int* unit_off = (int*)((char*)ws + align256((split_w_frags(n, C) + split_e_frags(n, C)) * sizeof(bf16x8)));
@@ -488,8 +488,10 @@ int split_run(const void* bank, const void* spectra, const float* inv_norm, cons
       GIS_SPLIT_PT5(2, 1)
     } else if (dt == 1) {
       GIS_SPLIT_PT5(1, 1)
+    } else if (dt == 2) {
+      GIS_SPLIT_PT5(1, 2)
     } else {
-      GIS_SPLIT_PT3(1, 2)
+      GIS_SPLIT_PT5(1, 4)
     }
 #undef GIS_SPLIT_PT5
 #undef GIS_SPLIT_PT3
@@ -550,8 +552,8 @@ int gis_split_scratch_setup() { return split_scratch("gis_init") ? GIS_OK : GIS_
 // arguments and outputs of gis_correlate, plus
 //   order  [B] processing order of the particles (a permutation) or NULL
 //   nrows  dir_off[B], the number of dir_idx entries
-//   dt     directions per wave (1 or 2; 0 = automatic)
-//   pt     psi values per wave (1..5, <= 3 with dt 2; 0 = automatic)
+//   dt     directions per wave (1, 2 or 4; 0 = automatic)
+//   pt     psi values per wave (1..5; 0 = automatic)
 GIS_API int gis_correlate_split(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
                                 const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B,
                                 int P, int n, int C, int nrows, int dt, int pt, float* scores, int* peak_y,

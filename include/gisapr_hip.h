@@ -275,8 +275,9 @@ size_t gis_correlate_split_workspace(int n, int C, int B);
  *   order  [B] processing order of the searched particles (a permutation) or
  *          NULL: particles ranked by orientation share template rows in cache
  *   nrows  dir_off[B], the number of dir_idx entries
- *   dt     local directions per wave (1 or 2; 0 = automatic)
- *   pt     psi values per wave (1..5, <= 3 with dt 2; 0 = automatic).  dt and
+ *   dt     local directions per wave (1, 2 or 4; 0 = automatic; crops above
+ *          16 use 1)
+ *   pt     psi values per wave (1..5; 0 = automatic).  dt and
  *          pt are autotuner knobs: results do not depend on them
  * C (--ccgCropSize) must be even and <= 32 (the reference default is 32,
  * v606:1562); n <= 256.

@@ -124,7 +124,7 @@ def test_split_tiles_are_bit_identical(hip_device):
     idx = np.asarray([k for l in local for k in l], np.int32)
     d_off, d_idx = torch.from_numpy(off).to(dev), torch.from_numpy(idx).to(dev)
     ref = kernels.correlate_split(bank, spec, None, d_off, d_idx, C, dt=1, pt=1)
-    for dt in (1, 2):
+    for dt in (1, 2, 4):
         for pt in (1, 2, 3, 4, 5):
             say("split tile", dt, pt)
             got = kernels.correlate_split(bank, spec, None, d_off, d_idx, C, dt=dt, pt=pt,
