@@ -204,7 +204,11 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
   // the row table (read every step) lives in LDS; the column table (read
   // once per column tile) is read from global memory (cached)
   extern __shared__ bf16x8 tab[];
-  const int m = n / 2 + 1, nct = (m + 15) >> 4;
+  const int m = n / 2 + 1;
+  // when the packed plane's last column is alone in its tile (n a multiple
+  // of 32), it is searched separately: the wave's pairs share one tile
+  const bool lone = (m & 15) == 1;
+  const int nct = lone ? (m >> 4) : (m + 15) >> 4;
   const int nks = (((n + 15) >> 4) + 1) & ~1;  // row steps, padded to even (zero table rows)
   const int nw = nks * 4 * LT * 64;
   for (int e = threadIdx.x; e < nw; e += 64 * kWaves) tab[e] = Wt[e];
@@ -344,6 +348,97 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
           }
         }
       }
+    }
+  }
+
+  if (lone) {
+    // The last column c = m - 1 of the wave's pairs, sixteen pairs per tile:
+    // M slot i of group q holds pair k = 16 q + i = d * PT + p (slots past the
+    // last pair hold a copy of pair 0, never read).  Stage 1 as above on
+    // split-bf16 MFMA; stage 2 is the rank-one update
+    // S[x][y] += Re(E[c][x] G[c][y]) in FP32, G[c][y] of pair k fetched from
+    // lane ((k % 16) / 4, y) of the group's accumulator.
+    // E[m - 1][x] for this lane's lags x = 16 tx + 4g + v, in float64 -> float
+    float er[LT][4], ei[LT][4];
+    {
+      const int sh = (m - 1 + m / 2) % m;
+      const double w = (sh == 0 || sh == n / 2) ? 1.0 : 2.0;
+#pragma unroll
+      for (int tx = 0; tx < LT; ++tx)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int x = 16 * tx + 4 * g + v;
+          double sn = 0.0, cs = 0.0;
+          if (x < C) {
+            const long kk = ((long)sh * crop_lag(x, n, C)) % n;
+            sincospi(2.0 * (double)kk / (double)n, &sn, &cs);
+          }
+          er[tx][v] = (float)(w * cs);
+          ei[tx][v] = (float)(w * sn);
+        }
+    }
+    constexpr int NG = (DT * PT + 15) / 16;
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+      const int k = 16 * q + i < DT * PT ? 16 * q + i : 0;
+      const float2* lt = tp[0];
+      const float2* lq = pp[0];
+#pragma unroll
+      for (int d = 1; d < DT; ++d)
+        if (k / PT == d) lt = tp[d];
+#pragma unroll
+      for (int p = 1; p < PT; ++p)
+        if (k % PT == p) lq = pp[p];
+      floatx4 gn[2 * LT];
+#pragma unroll
+      for (int t = 0; t < 2 * LT; ++t) gn[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+      float2 na[4], nq[4];
+      auto nload = [&](int ks) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int off = min((16 * ks + 4 * g + j) * m + (m - 1), last);
+          na[j] = lt[off];
+          nq[j] = lq[off];
+        }
+      };
+      nload(0);
+      for (int ks = 0; ks < nks; ++ks) {
+        float x[8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          x[2 * j] = na[j].x * nq[j].x - na[j].y * nq[j].y;
+          x[2 * j + 1] = na[j].x * nq[j].y + na[j].y * nq[j].x;
+        }
+        if (ks + 1 < nks) nload(ks + 1);
+        bf16x8 ahi, alo;
+        split8(x, ahi, alo);
+#pragma unroll
+        for (int t = 0; t < 2 * LT; ++t) {
+          const bf16x8 bh = Wl[((ks * 2 * LT + t) * 2) * 64 + lane];
+          const bf16x8 bl = Wl[((ks * 2 * LT + t) * 2 + 1) * 64 + lane];
+          gn[t] = mfma_bf16(ahi, bh, gn[t]);
+          gn[t] = mfma_bf16(ahi, bl, gn[t]);
+          gn[t] = mfma_bf16(alo, bh, gn[t]);
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < DT; ++d)
+#pragma unroll
+        for (int p = 0; p < PT; ++p) {
+          const int kp = d * PT + p;
+          if (kp / 16 != q) continue;  // compile-time after unrolling
+          const int src = ((kp & 15) >> 2) * 16 + i;
+#pragma unroll
+          for (int ty = 0; ty < LT; ++ty) {
+            const float gr = __shfl(gn[2 * ty][kp & 3], src, 64);
+            const float gi = __shfl(gn[2 * ty + 1][kp & 3], src, 64);
+#pragma unroll
+            for (int tx = 0; tx < LT; ++tx)
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                out[d][p][tx * LT + ty][v] += er[tx][v] * gr - ei[tx][v] * gi;
+          }
+        }
     }
   }
 
