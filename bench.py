@@ -295,7 +295,7 @@ def precision_check_split(eng, ps, bank, res, subset: int, seed: int):
         if len(d) == 0:
             continue
         dd = torch.from_numpy(d.astype(np.int64)).to(dev)
-        t = raw[dd].to(torch.complex128) * tinv[dd].double()[:, None, None]
+        t = raw[dd][..., :eng.n // 2 + 1].to(torch.complex128) * tinv[dd].double()[:, None, None]
         p = ps.spectra[b].to(torch.complex128) * ps.inv_norm[b].double()[:, None, None]
         q = t[:, None] * p[None]                                    # [Mi, P, n, m]
         crop = 1024.0 * (Wt @ q @ Et).real.reshape(len(d) * P, C * C)
@@ -606,9 +606,9 @@ def main() -> int:
         legs["fp32"], results["fp32"], bank = run_single(lambda v: eng.templates(v, normalize=False),
                                                          lambda t, p: eng.search(t[0], p, t[1]))
     if "split" in want:
-        # the same search on split-bf16 MFMA, same bank
-        legs["split"], results["split"], bank_split = run_single(lambda v: eng.templates(v, normalize=False),
-                                                                 lambda t, p: eng.search_split(t[0], p, t[1]))
+        # the same search on split-bf16 MFMA, same bank values in zero-padded rows
+        legs["split"], results["split"], bank_split = run_single(
+            lambda v: eng.templates(v, normalize=False, padded=True), lambda t, p: eng.search_split(t[0], p, t[1]))
     if "mixed" in want:
         # split-bf16 path: the template kernel writes the in-disk coefficients directly
         legs["mixed"], results["mixed"], tk_last = run_single(eng.small_templates,

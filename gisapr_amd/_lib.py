@@ -41,6 +41,8 @@ SIGNATURES = {
     "gis_build_template_coefficients": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _sz,
                                              _vp]),
     "gis_build_templates_batched": (_i, [_vp, _sz, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _sz, _vp]),
+    "gis_build_templates_batched_ld": (_i, [_vp, _sz, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _sz,
+                                            _vp]),
     "gis_build_template_coefficients_batched": (_i, [_vp, _sz, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _vp, _vp,
                                                      _vp, _sz, _vp]),
     "gis_prepare_particles_workspace": (_sz, [_i, _i]),
@@ -51,10 +53,10 @@ SIGNATURES = {
     "gis_correlate": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp,
                            _vp, _vp, _vp, _sz, _vp]),
     "gis_correlate_split_workspace": (_sz, [_i, _i, _i]),
-    "gis_correlate_split": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp,
-                                 _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
-    "gis_correlate_split_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp,
-                                         _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gis_correlate_split": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp,
+                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "gis_correlate_split_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i,
+                                         _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gis_threshold_mask": (_i, [_vp, _sz, _f, _vp, _vp]),
     "gis_edt_squared": (_i, [_vp, _i, _vp, _vp]),
     "gis_restraint_eval": (_i, [_vp, _f, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),

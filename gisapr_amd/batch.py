@@ -328,8 +328,11 @@ class CandidateBatch:
             self.bank = None
             self.tk = torch.empty((self.slots * M, engine.small.Kpad), dtype=torch.complex64, device=dev)
             self.norms = torch.empty((self.slots * M,), dtype=torch.float32, device=dev)
-        else:
-            self.bank = torch.empty((self.slots * M, n, n // 2 + 1), dtype=torch.complex64, device=dev)
+        else:  # the split search reads zero-padded rows (aligned 128-byte segments)
+            ld = kernels.padded_row_stride(n) if path == "split" else n // 2 + 1
+            self.bank = torch.empty((self.slots * M, n, ld), dtype=torch.complex64, device=dev)
+            if path == "split":
+                engine.split_spectra(ps)
         self.tinv = torch.empty((self.slots * M,), dtype=torch.float32, device=dev)  # 1/norm per bank row
         self.tpl_ws = torch.empty((max(int(kernels.lib().gis_build_templates_workspace(N, self.group * M)), 1),),
                                   dtype=torch.uint8, device=dev)
@@ -380,7 +383,7 @@ class CandidateBatch:
                                            rows=self.plan.particle_major(self.slots if k is None else k),
                                            operand=eng.small.operand)
         if self.path == "split":
-            return kernels.correlate_split(self.bank, ps.spectra, ps.inv_norm, dir_off, dir_idx, eng.crop,
+            return kernels.correlate_split(self.bank, eng.split_spectra(ps), ps.inv_norm, dir_off, dir_idx, eng.crop,
                                            validate=False, pmap=pmap, tinv=self.tinv,
                                            order=self.plan.virtual_major(self.slots if k is None else k),
                                            dt=tile.get("dt", 0), pt=tile.get("pt", 0))
@@ -444,6 +447,8 @@ class CandidateBatch:
         eng, ps = self.engine, self.ps
         if self.path == "mixed":
             return (ps.coeffs, eng.small_bhi, eng.small_blo, eng.small_plast, eng.small_kslot)
+        if self.path == "split":
+            return (ps.spectra_split, ps.inv_norm)
         return (ps.spectra, ps.inv_norm)
 
     def evaluate(self, poses, sync: bool = True):
@@ -599,7 +604,8 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
         if ps.spectra is None:
             engine.prepare(ps)
         n = engine.n
-        bank = torch.empty((C * M, n, n // 2 + 1), dtype=torch.complex64, device=engine.device)
+        ld = kernels.padded_row_stride(n) if path == "split" else n // 2 + 1
+        bank = torch.empty((C * M, n, ld), dtype=torch.complex64, device=engine.device)
         tinv = torch.empty((C * M,), dtype=torch.float32, device=engine.device)
         for c, vol in enumerate(volumes):
             dft = kernels.volume_dft(vol)
@@ -607,8 +613,9 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
                                                out=bank[c * M:(c + 1) * M], normalize=False)
             torch.reciprocal(norms, out=tinv[c * M:(c + 1) * M])
         if path == "split":
-            corr = kernels.correlate_split(bank, ps.spectra, ps.inv_norm, plan.dir_off, plan.dir_idx, engine.crop,
-                                           validate=False, pmap=plan.pmap, tinv=tinv, order=plan.virtual_major(C),
+            corr = kernels.correlate_split(bank, engine.split_spectra(ps), ps.inv_norm, plan.dir_off, plan.dir_idx,
+                                           engine.crop, validate=False, pmap=plan.pmap, tinv=tinv,
+                                           order=plan.virtual_major(C),
                                            dt=tile.get("dt", 0), pt=tile.get("pt", 0))
         else:
             corr = kernels.correlate(bank, ps.spectra, ps.inv_norm, plan.dir_off, plan.dir_idx, engine.crop,

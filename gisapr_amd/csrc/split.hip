@@ -198,7 +198,7 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
                        const float* __restrict__ inv_norm, const float* __restrict__ tinv,
                        const int* __restrict__ dir_off, const int* __restrict__ dir_idx,
                        const int* __restrict__ pmap, const int* __restrict__ order,
-                       const int* __restrict__ unit_off, int B, int P, int n, int C, int nblk,
+                       const int* __restrict__ unit_off, int B, int P, int n, int C, int ldt, int ldp, int nblk,
                        const bf16x8* __restrict__ Wt, const bf16x8* __restrict__ Et, float* __restrict__ scores,
                        int* __restrict__ peak_y, int* __restrict__ peak_x) {
   // the row table (read every step) lives in LDS; the column table (read
@@ -233,29 +233,31 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
   const int j0 = (u - unit_off[lo]) * DT;
   const int p0 = (int)blockIdx.y * PT;
   const int pb = pmap ? pmap[b] : b;
-  const size_t plane = (size_t)n * m;
+  // template rows are ldt complex values apart, psi spectrum rows ldp
+  // (ld > n/2+1: zero padding, so 16-column segments start on 128-byte lines)
   const float2* tp[DT];
   const float2* pp[PT];
 #pragma unroll
-  for (int d = 0; d < DT; ++d) tp[d] = bank + (size_t)dir_idx[d0 + min(j0 + d, Mi - 1)] * plane;
+  for (int d = 0; d < DT; ++d) tp[d] = bank + (size_t)dir_idx[d0 + min(j0 + d, Mi - 1)] * ((size_t)n * ldt);
 #pragma unroll
-  for (int p = 0; p < PT; ++p) pp[p] = spectra + ((size_t)pb * P + min(p0 + p, P - 1)) * plane;
+  for (int p = 0; p < PT; ++p) pp[p] = spectra + ((size_t)pb * P + min(p0 + p, P - 1)) * ((size_t)n * ldp);
   const int g = lane >> 4, i = lane & 15;
 
   // Operands of step (ct, ks).  Loads are unconditional: an offset past the
-  // plane (rows >= n, columns >= n/2+1) is clamped to its last element, and
-  // the tables hold zeros at those rows and columns, so the (finite) values
-  // read there contribute exactly nothing.
-  const int last = (int)plane - 1;
+  // plane (rows >= n) is clamped to its last element, columns >= n/2+1 read
+  // the next row or the zero padding, and the tables hold zeros at those rows
+  // and columns, so the (finite) values read there contribute exactly
+  // nothing.
+  const int lastt = n * ldt - 1, lastp = n * ldp - 1;
   auto load = [&](int ct, int ks, Frags<DT, PT>& f) {
-    const int base = (16 * ks + 4 * g) * m + 16 * ct + i;
+    const int r0 = 16 * ks + 4 * g, c = 16 * ct + i;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int off = min(base + j * m, last);
+      const int ot = min((r0 + j) * ldt + c, lastt), op = min((r0 + j) * ldp + c, lastp);
 #pragma unroll
-      for (int d = 0; d < DT; ++d) f.t[d][j] = tp[d][off];
+      for (int d = 0; d < DT; ++d) f.t[d][j] = tp[d][ot];
 #pragma unroll
-      for (int p = 0; p < PT; ++p) f.p[p][j] = pp[p][off];
+      for (int p = 0; p < PT; ++p) f.p[p][j] = pp[p][op];
     }
   };
 
@@ -396,9 +398,9 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
       auto nload = [&](int ks) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const int off = min((16 * ks + 4 * g + j) * m + (m - 1), last);
-          na[j] = lt[off];
-          nq[j] = lq[off];
+          const int r = 16 * ks + 4 * g + j;
+          na[j] = lt[min(r * ldt + (m - 1), lastt)];
+          nq[j] = lq[min(r * ldp + (m - 1), lastp)];
         }
       };
       nload(0);
@@ -493,7 +495,8 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 template <int LT, int DT, int PT>
 int launch_split(dim3 grid, size_t lds, hipStream_t st, const float2* bank, const float2* spectra,
                  const float* inv_norm, const float* tinv, const int* dir_off, const int* dir_idx, const int* pmap,
-                 const int* order, const int* unit_off, int B, int P, int n, int C, int nblk, const bf16x8* Wt,
+                 const int* order, const int* unit_off, int B, int P, int n, int C, int ldt, int ldp, int nblk,
+                 const bf16x8* Wt,
                  const bf16x8* Et, float* scores, int* peak_y, int* peak_x) {
   static bool attr_set = false;
   if (!attr_set) {
@@ -508,7 +511,7 @@ int launch_split(dim3 grid, size_t lds, hipStream_t st, const float2* bank, cons
     attr_set = true;
   }
   hipLaunchKernelGGL((split_correlate_kernel<LT, DT, PT>), grid, dim3(64 * kWaves), lds, st, bank, spectra,
-                     inv_norm, tinv, dir_off, dir_idx, pmap, order, unit_off, B, P, n, C, nblk, Wt, Et, scores,
+                     inv_norm, tinv, dir_off, dir_idx, pmap, order, unit_off, B, P, n, C, ldt, ldp, nblk, Wt, Et, scores,
                      peak_y, peak_x);
   return GIS_OK;
 }
@@ -524,7 +527,9 @@ GIS_API size_t gis_correlate_split_workspace(int n, int C, int B) {
 namespace {
 
 // Argument checks shared by both entry points; GIS_OK when the call may go on.
-int split_check(const char* who, int B, int P, int n, int C, int nrows, int dt, int pt) {
+int split_check(const char* who, int B, int P, int n, int C, int nrows, int ldt, int ldp, int dt, int pt) {
+  GIS_REQUIRE(ldt >= n / 2 + 1 && ldp >= n / 2 + 1, "%s: row strides %d, %d below n/2+1 = %d", who, ldt, ldp,
+              n / 2 + 1);
   GIS_REQUIRE(B >= 0 && P >= 1 && nrows >= 0, "%s: B=%d P=%d nrows=%d", who, B, P, nrows);
   GIS_REQUIRE(n >= 4 && n % 2 == 0 && n <= 256, "%s: newbox %d must be even, in [4, 256]", who, n);
   if (C > 32 || C < 2 || C % 2 != 0 || C > n) {
@@ -540,9 +545,9 @@ int split_check(const char* who, int B, int P, int n, int C, int nrows, int dt, 
 // bytes of device memory (tables, then the unit offsets), all of which the
 // first two kernels write before the search reads them.
 int split_run(const void* bank, const void* spectra, const float* inv_norm, const float* tinv, const int* dir_off,
-              const int* dir_idx, const int* pmap, const int* order, int B, int P, int n, int C, int nrows, int dt,
-              int pt, float* scores, int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv,
-              void* ws, hipStream_t st) {
+              const int* dir_idx, const int* pmap, const int* order, int B, int P, int n, int C, int nrows, int ldt,
+              int ldp, int dt, int pt, float* scores, int* peak_y, int* peak_x, int* best, float* peak, double* mean,
+              double* stdv, void* ws, hipStream_t st) {
   const int LT = split_lt(C);
   if (pt == 0) pt = std::min(P, 5);
   if (dt == 0) dt = 2;  // measured fastest at box 256 (psi spectra shared by two directions)
@@ -564,7 +569,7 @@ int split_run(const void* bank, const void* spectra, const float* inv_norm, cons
     int rc = GIS_OK;
 #define GIS_SPLIT_ARGS                                                                                          \
   grid, lds, st, (const float2*)bank, (const float2*)spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order,    \
-      unit_off, B, P, n, C, nblk, Wt, Et, scores, peak_y, peak_x
+      unit_off, B, P, n, C, ldt, ldp, nblk, Wt, Et, scores, peak_y, peak_x
 #define GIS_SPLIT_PT5(LTV, DTV)                                          \
   switch (pt) {                                                         \
     case 1: rc = launch_split<LTV, DTV, 1>(GIS_SPLIT_ARGS); break;      \
@@ -647,22 +652,24 @@ int gis_split_scratch_setup() { return split_scratch("gis_init") ? GIS_OK : GIS_
 // arguments and outputs of gis_correlate, plus
 //   order  [B] processing order of the particles (a permutation) or NULL
 //   nrows  dir_off[B], the number of dir_idx entries
+//   ldt    complex values between template rows (n/2+1, or more: zero padding)
+//   ldp    complex values between spectrum rows (likewise)
 //   dt     directions per wave (1, 2 or 4; 0 = automatic)
 //   pt     psi values per wave (1..5; 0 = automatic)
 GIS_API int gis_correlate_split(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
                                 const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B,
-                                int P, int n, int C, int nrows, int dt, int pt, float* scores, int* peak_y,
-                                int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
+                                int P, int n, int C, int nrows, int ldt, int ldp, int dt, int pt, float* scores,
+                                int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
                                 size_t ws_bytes, void* stream) {
-  const int rc = split_check("gis_correlate_split", B, P, n, C, nrows, dt, pt);
+  const int rc = split_check("gis_correlate_split", B, P, n, C, nrows, ldt, ldp, dt, pt);
   if (rc != GIS_OK) return rc;
   if (B == 0) return GIS_OK;
   GIS_REQUIRE(bank && spectra && dir_off && dir_idx && scores && peak_y && peak_x && best && peak && mean && stdv &&
                   ws,
               "gis_correlate_split: null pointer");
   GIS_REQUIRE(ws_bytes >= gis_correlate_split_workspace(n, C, B), "gis_correlate_split: workspace too small");
-  return split_run(bank, spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order, B, P, n, C, nrows, dt, pt, scores,
-                   peak_y, peak_x, best, peak, mean, stdv, ws, (hipStream_t)stream);
+  return split_run(bank, spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order, B, P, n, C, nrows, ldt, ldp, dt, pt,
+                   scores, peak_y, peak_x, best, peak, mean, stdv, ws, (hipStream_t)stream);
 }
 
 // gis_correlate_split without a workspace argument and without allocating:
@@ -672,10 +679,10 @@ GIS_API int gis_correlate_split(const void* bank, const void* spectra, const flo
 // streams or threads) share the array and race.
 GIS_API int gis_correlate_split_noalloc(const void* bank, const void* spectra, const float* inv_norm,
                                         const float* tinv, const int* dir_off, const int* dir_idx, const int* pmap,
-                                        const int* order, int B, int P, int n, int C, int nrows, int dt, int pt,
-                                        float* scores, int* peak_y, int* peak_x, int* best, float* peak,
-                                        double* mean, double* stdv, void* stream) {
-  const int rc = split_check("gis_correlate_split_noalloc", B, P, n, C, nrows, dt, pt);
+                                        const int* order, int B, int P, int n, int C, int nrows, int ldt, int ldp,
+                                        int dt, int pt, float* scores, int* peak_y, int* peak_x, int* best,
+                                        float* peak, double* mean, double* stdv, void* stream) {
+  const int rc = split_check("gis_correlate_split_noalloc", B, P, n, C, nrows, ldt, ldp, dt, pt);
   if (rc != GIS_OK) return rc;
   if (B == 0) return GIS_OK;
   GIS_REQUIRE(bank && spectra && dir_off && dir_idx && scores && peak_y && peak_x && best && peak && mean && stdv,
@@ -687,6 +694,6 @@ GIS_API int gis_correlate_split_noalloc(const void* bank, const void* spectra, c
               B, n, C, need, kSplitScratchBytes);
   void* ws = split_scratch("gis_correlate_split_noalloc");
   if (!ws) return GIS_EINVAL;
-  return split_run(bank, spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order, B, P, n, C, nrows, dt, pt, scores,
-                   peak_y, peak_x, best, peak, mean, stdv, ws, (hipStream_t)stream);
+  return split_run(bank, spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order, B, P, n, C, nrows, ldt, ldp, dt, pt,
+                   scores, peak_y, peak_x, best, peak, mean, stdv, ws, (hipStream_t)stream);
 }

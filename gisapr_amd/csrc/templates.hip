@@ -124,6 +124,7 @@ struct CoefOut {
   const int* kslot;  // [n * (n/2+1)], -1 = not kept; null = write the bank
   int K, Kpad;
   float2* coeffs;    // [M][Kpad]
+  int ldb = 0;       // bank row stride in complex values (0: n/2+1); rows are n of them
 };
 
 // Real-space modes (the standalone projector's path): `img` replaces the
@@ -773,7 +774,8 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
     }
 
     // ---- phase 9: forward column FFT (N), crop to n, pack, FSC -----------
-    float2* out = co.kslot ? nullptr : bank + (size_t)w * n * mh_n;
+    const int ldb = co.ldb > 0 ? co.ldb : mh_n;
+    float2* out = co.kslot ? nullptr : bank + (size_t)w * n * ldb;
     double nrm[1] = {0.0};
     const int kmax = n / 2;  // columns 0..n/2 are needed
     for (int k0 = 0; k0 <= kmax; k0 += T::TK3) {
@@ -824,11 +826,19 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
               const int j = __float_as_int(gj.y);
               if (j >= 0) co.coeffs[(size_t)w * co.Kpad + j] = v;
             } else {
-              out[(size_t)r * mh_n + c] = v;
+              out[(size_t)r * ldb + c] = v;
             }
             nrm[0] += (double)v.x * v.x + (double)v.y * v.y;
           });
       __syncthreads();
+    }
+
+    if (out && ldb > mh_n) {  // zero padding past column n/2 (read by searches as a clamp target)
+      const int pw = ldb - mh_n;
+      for (int t = tid; t < n * pw; t += nthr) {
+        const int r = t / pw, c = t - r * pw;
+        out[(size_t)r * ldb + mh_n + c] = make_float2(0.f, 0.f);
+      }
     }
 
     // ---- phase 10: unit L2 norm over the packed half plane (or, when the
@@ -853,8 +863,9 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
       }
     } else if (normalize) {
       for (int t = tid; t < n * mh_n; t += nthr) {
-        float2 v = out[t];
-        out[t] = make_float2(v.x / norm, v.y / norm);
+        const int r = t / mh_n, c = t - r * mh_n;
+        float2 v = out[(size_t)r * ldb + c];
+        out[(size_t)r * ldb + c] = make_float2(v.x / norm, v.y / norm);
       }
     }
     if (norms && tid == 0) norms[w] = norm;
@@ -1048,6 +1059,27 @@ GIS_API int gis_build_templates_batched(const void* dft, size_t dft_stride, int 
   if (M == 0 || C == 0) return GIS_OK;
   return build_templates_dispatch(dft, VolSet{C, dft_stride}, N, 1, n, rts, M, mask, fsc, bank, norms, normalize, ws,
                                   ws_bytes, stream, CoefOut{nullptr, 0, 0, nullptr}, kBankIO);
+}
+
+// gis_build_templates_batched with bank rows ldb >= n/2+1 complex values
+// apart (columns past n/2 are left unwritten): the padded row layout the
+// split search reads in aligned 128-byte segments.
+GIS_API int gis_build_templates_batched_ld(const void* dft, size_t dft_stride, int C, int N, int n,
+                                           const float* rts, int M, const float* mask, const float* fsc, void* bank,
+                                           int ldb, float* norms, int normalize, void* ws, size_t ws_bytes,
+                                           void* stream) {
+  GIS_REQUIRE(dft && rts && bank && ws, "gis_build_templates_batched_ld: null pointer");
+  GIS_REQUIRE(normalize || norms, "gis_build_templates_batched_ld: an unnormalised bank needs the norms output");
+  GIS_REQUIRE(M >= 0 && C >= 0, "gis_build_templates_batched_ld: M = %d, C = %d", M, C);
+  GIS_REQUIRE((size_t)C * (size_t)M < (1u << 31), "gis_build_templates_batched_ld: C * M too large");
+  GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0, "gis_build_templates_batched_ld: newbox %d must be even and <= box %d",
+              n, N);
+  GIS_REQUIRE(ldb >= n / 2 + 1, "gis_build_templates_batched_ld: row stride %d below n/2+1 = %d", ldb, n / 2 + 1);
+  if (M == 0 || C == 0) return GIS_OK;
+  CoefOut co{nullptr, 0, 0, nullptr};
+  co.ldb = ldb;
+  return build_templates_dispatch(dft, VolSet{C, dft_stride}, N, 1, n, rts, M, mask, fsc, bank, norms, normalize, ws,
+                                  ws_bytes, stream, co, kBankIO);
 }
 
 // gis_build_template_coefficients over C spectra in one launch (layout as

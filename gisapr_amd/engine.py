@@ -38,6 +38,7 @@ class ParticleSet:
     coeffs: Optional[torch.Tensor] = None    # [B*P, Kpad] complex64, unit norm (small-shift search)
     row_order: Optional[torch.Tensor] = None # [sum M_i] int32 processing order of the small-shift search
     particle_order: Optional[torch.Tensor] = None  # [B] int32 processing order of the split search
+    spectra_split: Optional[torch.Tensor] = None   # [B, P, n, ld] the spectra in zero-padded rows (split search)
 
     @property
     def B(self) -> int:
@@ -165,6 +166,7 @@ class SearchEngine:
         return ps
 
     def prepare(self, ps: ParticleSet) -> ParticleSet:
+        ps.spectra_split = None  # a padded copy of older spectra is stale
         ctf_tab = self.ctf_tab if ps.defocus is not None else None
         ps.spectra, ps.inv_norm = kernels.prepare_particles(
             ps.images, self.n, ps.rotp, self.mask, ctf_tab, ps.defocus, self.ctf.kconst,
@@ -179,11 +181,17 @@ class SearchEngine:
         vol = base.clone() if base is not None else None
         return kernels.splat(xyz, zn, self.N, width, self.apix, rp, kn, vol)
 
-    def templates(self, volume: torch.Tensor, with_norms=False, normalize=True):
+    def templates(self, volume: torch.Tensor, with_norms=False, normalize=True, padded=False):
         """Unit-norm bank (and norms); with ``normalize=False`` returns
-        (bank, 1/norms) with unnormalised rows, for ``search(..., tinv=)``."""
+        (bank, 1/norms) with unnormalised rows, for ``search(..., tinv=)``.
+        ``padded``: rows zero padded to ``kernels.padded_row_stride(n)``
+        (the split search's aligned layout)."""
         dft = kernels.volume_dft(volume)
-        out = kernels.build_templates(dft, self.n, self.rts, self.mask, self.fsc_packed, with_norms,
+        out = None
+        if padded:
+            out = torch.empty((len(self.dir_rot), self.n, kernels.padded_row_stride(self.n)), dtype=torch.complex64,
+                              device=self.device)
+        out = kernels.build_templates(dft, self.n, self.rts, self.mask, self.fsc_packed, with_norms, out=out,
                                       normalize=normalize)
         if normalize:
             return out
@@ -201,6 +209,16 @@ class SearchEngine:
                                  validate=False, tinv=tinv)
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.crop // 2)
 
+    def split_spectra(self, ps: ParticleSet) -> torch.Tensor:
+        """The particle spectra in zero-padded rows (made once per set): the
+        split search reads them in aligned 128-byte segments."""
+        if ps.spectra is None:
+            self.prepare(ps)
+        ld = kernels.padded_row_stride(self.n)
+        if ps.spectra_split is None or int(ps.spectra_split.shape[-1]) != ld:
+            ps.spectra_split = kernels.pad_rows(ps.spectra, ld)
+        return ps.spectra_split
+
     def search_split(self, bank: torch.Tensor, ps: ParticleSet, tinv: Optional[torch.Tensor] = None,
                      dt: int = 0, pt: int = 0) -> SearchResult:
         """The search of ``search`` (every coefficient, every lag of the crop)
@@ -211,7 +229,7 @@ class SearchEngine:
             raise ValueError("template bank does not cover the engine's direction table")
         if ps.particle_order is None:
             ps.particle_order = particle_order(ps.dir_off.cpu().numpy(), ps.dir_idx.cpu().numpy(), self.device)
-        corr = kernels.correlate_split(bank, ps.spectra, ps.inv_norm, ps.dir_off, ps.dir_idx, self.crop,
+        corr = kernels.correlate_split(bank, self.split_spectra(ps), ps.inv_norm, ps.dir_off, ps.dir_idx, self.crop,
                                        validate=False, tinv=tinv, order=ps.particle_order, dt=dt, pt=pt)
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.crop // 2)
 

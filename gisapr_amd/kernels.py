@@ -239,6 +239,20 @@ def _spectra(dft: torch.Tensor):
     return N, (int(dft.shape[0]) if dft.ndim == 2 else 1)
 
 
+def padded_row_stride(n: int) -> int:
+    """Row stride (complex values) of the padded packed layout: n/2+1 rounded
+    up to a multiple of 16, so every 16-column segment of a row starts on a
+    128-byte line."""
+    return (n // 2 + 1 + 15) // 16 * 16
+
+
+def pad_rows(x: torch.Tensor, ld: int) -> torch.Tensor:
+    """A contiguous copy of ``x`` [..., n/2+1] with rows ``ld`` wide, zero padded."""
+    out = torch.zeros(tuple(x.shape[:-1]) + (int(ld),), dtype=x.dtype, device=x.device)
+    out[..., :x.shape[-1]] = x
+    return out
+
+
 def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Optional[torch.Tensor],
                     fsc_packed: Optional[torch.Tensor], with_norms: bool = False,
                     out: Optional[torch.Tensor] = None, normalize: bool = True,
@@ -258,9 +272,11 @@ def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Op
         bank = torch.empty((rows, n, n // 2 + 1), dtype=torch.complex64, device=dft.device)
     else:
         require_hip(out, "out")
-        if tuple(out.shape) != (rows, n, n // 2 + 1) or out.dtype != torch.complex64:
-            raise ValueError("out must be complex64 [C * M, n, n/2+1]")
+        if tuple(out.shape[:2]) != (rows, n) or out.ndim != 3 or int(out.shape[2]) < n // 2 + 1 \
+                or out.dtype != torch.complex64 or not out.is_contiguous():
+            raise ValueError("out must be contiguous complex64 [C * M, n, ld], ld >= n/2+1")
         bank = out
+    ldb = int(bank.shape[2])
     norms = torch.empty((rows,), dtype=torch.float32, device=dft.device) if with_norms else None
     if mask is not None:
         require_hip(mask, "mask")
@@ -271,7 +287,13 @@ def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Op
         if tuple(fsc_packed.shape) != (n, n // 2 + 1):
             raise ValueError("fsc_packed must be [n, n/2+1]")
     ws, wsb = _template_workspace(N, rows, dft.device, workspace)
-    if dft.ndim == 1:
+    if ldb != n // 2 + 1:  # padded rows (zeros past column n/2)
+        stride = int(dft.shape[1]) if dft.ndim > 1 else int(dft.numel())
+        check(lib().gis_build_templates_batched_ld(ptr(dft), stride, C, N, n, ptr(rts), M, ptr(mask),
+                                                   ptr(fsc_packed), ptr(bank), ldb, ptr(norms),
+                                                   int(bool(normalize)), ptr(ws), wsb, stream_of(dft.device)),
+              "gis_build_templates_batched_ld")
+    elif dft.ndim == 1:
         check(lib().gis_build_templates(ptr(dft), N, n, ptr(rts), M, ptr(mask), ptr(fsc_packed), ptr(bank),
                                         ptr(norms), int(bool(normalize)), ptr(ws), wsb, stream_of(dft.device)),
               "gis_build_templates")
@@ -410,8 +432,9 @@ def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torc
     require_hip(spectra, "spectra")
     P, n = int(spectra.shape[1]), int(spectra.shape[2])
     B = int(dir_off.shape[0]) - 1 if pmap is not None else int(spectra.shape[0])
-    if tuple(bank.shape[1:]) != (n, n // 2 + 1):
-        raise ValueError("bank and spectra layouts differ")
+    ldt, ldp = int(bank.shape[-1]), int(spectra.shape[-1])
+    if int(bank.shape[1]) != n or min(ldt, ldp) < n // 2 + 1 or not (bank.is_contiguous() and spectra.is_contiguous()):
+        raise ValueError("bank [M, n, ldt] and spectra [B, P, n, ldp] must be contiguous, ld >= n/2+1")
     if dir_off.dtype != torch.int32 or dir_idx.dtype != torch.int32:
         raise ValueError("dir_off / dir_idx must be int32")
     if tuple(dir_off.shape) != (B + 1,):
@@ -449,7 +472,9 @@ def correlate_split(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optiona
                     pmap: Optional[torch.Tensor] = None, tinv: Optional[torch.Tensor] = None,
                     order: Optional[torch.Tensor] = None, dt: int = 0, pt: int = 0) -> CorrelationResult:
     """The v606 search of ``correlate`` (every packed coefficient, every lag of
-    the crop, crop <= 32) on split-bf16 MFMA.  ``order`` is the particles'
+    the crop, crop <= 32) on split-bf16 MFMA.  ``bank`` [M, n, ldt] and
+    ``spectra`` [B, P, n, ldp] may carry zero-padded rows (ld > n/2+1;
+    ``padded_row_stride(n)`` aligns them to 128-byte lines).  ``order`` is the particles'
     processing order (a permutation of the B searched particles, e.g. by
     orientation so neighbouring particles share template rows in cache);
     ``dt`` / ``pt`` are the directions / psi values per wave (0 = automatic;
@@ -458,8 +483,9 @@ def correlate_split(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optiona
     require_hip(spectra, "spectra")
     P, n = int(spectra.shape[1]), int(spectra.shape[2])
     B = int(dir_off.shape[0]) - 1 if pmap is not None else int(spectra.shape[0])
-    if tuple(bank.shape[1:]) != (n, n // 2 + 1):
-        raise ValueError("bank and spectra layouts differ")
+    ldt, ldp = int(bank.shape[-1]), int(spectra.shape[-1])
+    if int(bank.shape[1]) != n or min(ldt, ldp) < n // 2 + 1 or not (bank.is_contiguous() and spectra.is_contiguous()):
+        raise ValueError("bank [M, n, ldt] and spectra [B, P, n, ldp] must be contiguous, ld >= n/2+1")
     if dir_off.dtype != torch.int32 or dir_idx.dtype != torch.int32:
         raise ValueError("dir_off / dir_idx must be int32")
     if tuple(dir_off.shape) != (B + 1,):
@@ -494,7 +520,7 @@ def correlate_split(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optiona
     wsb = int(lib().gis_correlate_split_workspace(n, int(crop), B))
     ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dev)
     check(lib().gis_correlate_split(ptr(bank), ptr(spectra), ptr(inv_norm), ptr(tinv), ptr(dir_off), ptr(dir_idx),
-                                    ptr(pmap), ptr(order), B, P, n, int(crop), total, int(dt), int(pt),
+                                    ptr(pmap), ptr(order), B, P, n, int(crop), total, ldt, ldp, int(dt), int(pt),
                                     ptr(scores), ptr(py), ptr(px), ptr(best), ptr(peak), ptr(mean), ptr(std),
                                     ptr(ws), wsb, stream_of(dev)), "gis_correlate_split")
     return CorrelationResult(scores, py, px, best, peak, mean, std)

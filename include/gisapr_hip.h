@@ -130,6 +130,15 @@ int gis_build_template_coefficients(const void* dft, int N, int n, const float* 
 int gis_build_templates_batched(const void* dft, size_t dft_stride, int C, int N, int n, const float* rts, int M,
                                 const float* mask, const float* fsc, void* bank, float* norms, int normalize, void* ws,
                                 size_t ws_bytes, void* stream);
+
+/*
+ * gis_build_templates_batched with bank rows `ldb` >= n/2+1 complex values
+ * apart ([C*M][n][ldb]); columns past n/2 are written as zeros.  For the
+ * split search's aligned operand loads (ldb a multiple of 16).
+ */
+int gis_build_templates_batched_ld(const void* dft, size_t dft_stride, int C, int N, int n, const float* rts, int M,
+                                   const float* mask, const float* fsc, void* bank, int ldb, float* norms,
+                                   int normalize, void* ws, size_t ws_bytes, void* stream);
 int gis_build_template_coefficients_batched(const void* dft, size_t dft_stride, int C, int N, int n, const float* rts,
                                             int M, const float* mask, const float* fsc, const int* kslot, int K,
                                             int Kpad, void* coeffs, float* norms, void* ws, size_t ws_bytes,
@@ -275,6 +284,11 @@ size_t gis_correlate_split_workspace(int n, int C, int B);
  *   order  [B] processing order of the searched particles (a permutation) or
  *          NULL: particles ranked by orientation share template rows in cache
  *   nrows  dir_off[B], the number of dir_idx entries
+ *   ldt    complex values between the rows of a template (n/2+1 for the
+ *          [n][n/2+1] layout, or a padded stride: columns past n/2 zero);
+ *          a stride that is a multiple of 16 starts every 16-column segment
+ *          on a 128-byte line (see gis_build_templates_batched_ld)
+ *   ldp    the same for the rows of a psi spectrum
  *   dt     local directions per wave (1, 2 or 4; 0 = automatic; crops above
  *          16 use 1)
  *   pt     psi values per wave (1..5; 0 = automatic).  dt and
@@ -284,8 +298,9 @@ size_t gis_correlate_split_workspace(int n, int C, int B);
  */
 int gis_correlate_split(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
                         const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B, int P,
-                        int n, int C, int nrows, int dt, int pt, float* scores, int* peak_y, int* peak_x,
-                        int* best, float* peak, double* mean, double* stdv, void* ws, size_t ws_bytes, void* stream);
+                        int n, int C, int nrows, int ldt, int ldp, int dt, int pt, float* scores, int* peak_y,
+                        int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws, size_t ws_bytes,
+                        void* stream);
 
 /*
  * gis_correlate_split without the workspace argument, allocating nothing:
@@ -299,8 +314,9 @@ int gis_correlate_split(const void* bank, const void* spectra, const float* inv_
  */
 int gis_correlate_split_noalloc(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
                                 const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B,
-                                int P, int n, int C, int nrows, int dt, int pt, float* scores, int* peak_y,
-                                int* peak_x, int* best, float* peak, double* mean, double* stdv, void* stream);
+                                int P, int n, int C, int nrows, int ldt, int ldp, int dt, int pt, float* scores,
+                                int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv,
+                                void* stream);
 
 /*
  * Cross-particle conformation score sum_b exp(peak_b / 1024) in float64 over

@@ -131,6 +131,13 @@ def test_split_tiles_are_bit_identical(hip_device):
                                           order=torch.tensor([4, 1, 0, 5, 3, 2], dtype=torch.int32, device=dev))
             for f in ("scores", "peak_y", "peak_x", "best", "peak", "mean", "std"):
                 assert torch.equal(getattr(got, f), getattr(ref, f)), (dt, pt, f)
+    # zero-padded rows (the aligned layout): the same results bit for bit
+    for ld in (kernels.padded_row_stride(n), n // 2 + 1 + 5):
+        say("split padded rows", ld)
+        got = kernels.correlate_split(kernels.pad_rows(bank, ld), kernels.pad_rows(spec, ld), None, d_off, d_idx, C,
+                                      dt=2, pt=5)
+        for f in ("scores", "peak_y", "peak_x", "best", "peak", "mean", "std"):
+            assert torch.equal(getattr(got, f), getattr(ref, f)), (ld, f)
     # pmap: searched particles 0..B-1 mapped onto spectra rows reversed
     rev = torch.arange(B - 1, -1, -1, dtype=torch.int32, device=dev)
     spec_rev = spec.flip(0).contiguous()
@@ -198,3 +205,29 @@ def test_split_end_to_end_matches_oracle(hip_device, name):
         assert abs(r["rawcc"] - w["rawcc"]) <= 1e-4 * abs(w["rawcc"]) + 1e-3
     s_want = osr.score_sum([w["rawcc"] for w in want])
     assert abs(float(res.score.cpu()) - s_want) <= 1e-4 * s_want
+
+
+@pytest.mark.parametrize("N,n", [(64, 64), (96, 64), (256, 256)])
+def test_padded_template_bank_equals_packed(hip_device, N, n):
+    """The template kernel's padded-row output (gis_build_templates_batched_ld)
+    holds the packed bank bit for bit, zeros past column n/2."""
+    import torch
+    from gisapr_amd import host, kernels
+    rng = np.random.default_rng(N + n)
+    say("padded bank", N, n)
+    pts = rng.integers(N // 4, 3 * N // 4, (40, 3))
+    vol = torch.from_numpy(osp.density_from_coords((pts - N // 2) * 1.0, np.full(40, 6.0), N, 1.0, 2.0).astype(np.float32))
+    vol = vol.to(hip_device)
+    rot, tilt = rng.uniform(0, 360, 12), rng.uniform(0, 180, 12)
+    rts = torch.from_numpy(host.rotation_matrices_t(rot, tilt).astype(np.float32)).to(hip_device).contiguous()
+    mask = torch.from_numpy(host.generate_soft_mask((N, N), N // 3, 3).astype(np.float32)).to(hip_device)
+    dft = kernels.volume_dft(vol)
+    ref, rn = kernels.build_templates(dft, n, rts, mask, None, normalize=False)
+    ld = kernels.padded_row_stride(n)
+    out = torch.full((12, n, ld), float("nan"), dtype=torch.complex64, device=hip_device)
+    got, gn = kernels.build_templates(dft, n, rts, mask, None, out=out, normalize=False)
+    assert torch.equal(got[..., :n // 2 + 1], ref) and torch.equal(gn, rn)
+    assert bool((got[..., n // 2 + 1:] == 0).all())
+    unit = kernels.build_templates(dft, n, rts, mask, None,
+                                   out=torch.empty((12, n, ld), dtype=torch.complex64, device=hip_device))
+    assert torch.equal(unit[..., :n // 2 + 1], kernels.build_templates(dft, n, rts, mask, None))

@@ -49,8 +49,8 @@ def outputs(torch, dev, case):
 def run(lib, case, out, crop, dt, pt, stream, ws=None):
     from gisapr_amd._lib import ptr
     args = [ptr(case["bank"]), ptr(case["spectra"]), ptr(case["inv_norm"]), ptr(case["tinv"]), ptr(case["dir_off"]),
-            ptr(case["dir_idx"]), None, ptr(case["order"]), case["B"], case["P"], case["n"], crop, case["nrows"], dt,
-            pt, ptr(out["scores"]), ptr(out["py"]), ptr(out["px"]), ptr(out["best"]), ptr(out["peak"]),
+            ptr(case["dir_idx"]), None, ptr(case["order"]), case["B"], case["P"], case["n"], crop, case["nrows"],
+            case["n"] // 2 + 1, case["n"] // 2 + 1, dt, pt, ptr(out["scores"]), ptr(out["py"]), ptr(out["px"]), ptr(out["best"]), ptr(out["peak"]),
             ptr(out["mean"]), ptr(out["std"])]
     if ws is None:
         return lib.gis_correlate_split_noalloc(*args, stream)
@@ -105,7 +105,7 @@ def test_noalloc_refuses_above_its_workspace(hip_device):
     mean = torch.zeros((B,), dtype=f64, device=dev)
     std = torch.zeros((B,), dtype=f64, device=dev)
     rc = L.gis_correlate_split_noalloc(ptr(bank), ptr(spectra), ptr(inv_norm), None, ptr(dir_off), ptr(dir_idx),
-                                       ptr(pmap), None, B, 1, n, crop, 0, 0, 0, ptr(scores), ptr(pyx),
+                                       ptr(pmap), None, B, 1, n, crop, 0, m, m, 0, 0, ptr(scores), ptr(pyx),
                                        C.c_void_p(pyx.data_ptr() + 4), ptr(best), ptr(peak), ptr(mean), ptr(std),
                                        stream_of(dev))
     torch.cuda.synchronize(dev)
