@@ -603,7 +603,7 @@ def main() -> int:
     bank = bank_split = tk_last = None  # the last candidate's (bank, 1/norms) / template coefficients
     if "fp32" in want:
         # FP32 path: full unnormalised bank (1/norm folded into the search)
-        legs["fp32"], results["fp32"], bank = run_single(lambda v: eng.templates(v, normalize=False),
+        legs["fp32"], results["fp32"], bank = run_single(lambda v: eng.templates(v, normalize=False, padded=True),
                                                          lambda t, p: eng.search(t[0], p, t[1]))
     if "split" in want:
         # the same search on split-bf16 MFMA, same bank values in zero-padded rows

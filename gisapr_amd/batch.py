@@ -94,19 +94,17 @@ class Autotuner:
 
 MIXED_VARIANTS = [dict(pt=5, wpb=4), dict(pt=5, wpb=2), dict(pt=3, wpb=4), dict(pt=1, wpb=4)]
 SPLIT_VARIANTS = [dict(dt=2, pt=5), dict(dt=1, pt=5), dict(dt=4, pt=5), dict(dt=2, pt=3)]
-FP32_VARIANTS = [dict(jt=0), dict(jt=2), dict(jt=4), dict(jt=6)]
+FP32_VARIANTS = [dict(dt=1, pt=5), dict(dt=2, pt=5), dict(dt=1, pt=3), dict(dt=2, pt=3)]
 
 
 def _variants_for(path: str, P: int) -> List[dict]:
-    if path in ("mixed", "split"):
-        out = []
-        for v in (MIXED_VARIANTS if path == "mixed" else SPLIT_VARIANTS):
-            v = dict(v, pt=min(v["pt"], P))
-            if v not in out:
-                out.append(v)
-        return out
-    jt_max = 16 - min(P, 8)
-    return [v for v in FP32_VARIANTS if v["jt"] <= jt_max]
+    table = {"mixed": MIXED_VARIANTS, "split": SPLIT_VARIANTS}.get(path, FP32_VARIANTS)
+    out = []
+    for v in table:
+        v = dict(v, pt=min(v["pt"], P))
+        if v not in out:
+            out.append(v)
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -329,10 +327,9 @@ class CandidateBatch:
             self.tk = torch.empty((self.slots * M, engine.small.Kpad), dtype=torch.complex64, device=dev)
             self.norms = torch.empty((self.slots * M,), dtype=torch.float32, device=dev)
         else:  # the split search reads zero-padded rows (aligned 128-byte segments)
-            ld = kernels.padded_row_stride(n) if path == "split" else n // 2 + 1
+            ld = kernels.padded_row_stride(n)
             self.bank = torch.empty((self.slots * M, n, ld), dtype=torch.complex64, device=dev)
-            if path == "split":
-                engine.split_spectra(ps)
+            engine.split_spectra(ps)
         self.tinv = torch.empty((self.slots * M,), dtype=torch.float32, device=dev)  # 1/norm per bank row
         self.tpl_ws = torch.empty((max(int(kernels.lib().gis_build_templates_workspace(N, self.group * M)), 1),),
                                   dtype=torch.uint8, device=dev)
@@ -387,8 +384,10 @@ class CandidateBatch:
                                            validate=False, pmap=pmap, tinv=self.tinv,
                                            order=self.plan.virtual_major(self.slots if k is None else k),
                                            dt=tile.get("dt", 0), pt=tile.get("pt", 0))
-        return kernels.correlate(self.bank, ps.spectra, ps.inv_norm, dir_off, dir_idx, eng.crop,
-                                 validate=False, pmap=pmap, jt=tile.get("jt", 0), tinv=self.tinv)
+        return kernels.correlate(self.bank, eng.split_spectra(ps), ps.inv_norm, dir_off, dir_idx, eng.crop,
+                                 validate=False, pmap=pmap, tinv=self.tinv,
+                                 order=self.plan.virtual_major(self.slots if k is None else k),
+                                 dt=tile.get("dt", 0), pt=tile.get("pt", 0))
 
     def _device_step(self, k: Optional[int] = None):
         k = self.slots if k is None else k
@@ -447,9 +446,7 @@ class CandidateBatch:
         eng, ps = self.engine, self.ps
         if self.path == "mixed":
             return (ps.coeffs, eng.small_bhi, eng.small_blo, eng.small_plast, eng.small_kslot)
-        if self.path == "split":
-            return (ps.spectra_split, ps.inv_norm)
-        return (ps.spectra, ps.inv_norm)
+        return (ps.spectra_split, ps.inv_norm)
 
     def evaluate(self, poses, sync: bool = True):
         """Score ``poses`` ([k, 6], k <= slots); returns float64 [k] (or the device
@@ -604,8 +601,7 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
         if ps.spectra is None:
             engine.prepare(ps)
         n = engine.n
-        ld = kernels.padded_row_stride(n) if path == "split" else n // 2 + 1
-        bank = torch.empty((C * M, n, ld), dtype=torch.complex64, device=engine.device)
+        bank = torch.empty((C * M, n, kernels.padded_row_stride(n)), dtype=torch.complex64, device=engine.device)
         tinv = torch.empty((C * M,), dtype=torch.float32, device=engine.device)
         for c, vol in enumerate(volumes):
             dft = kernels.volume_dft(vol)
@@ -618,8 +614,10 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
                                            order=plan.virtual_major(C),
                                            dt=tile.get("dt", 0), pt=tile.get("pt", 0))
         else:
-            corr = kernels.correlate(bank, ps.spectra, ps.inv_norm, plan.dir_off, plan.dir_idx, engine.crop,
-                                     validate=False, pmap=plan.pmap, jt=tile.get("jt", 0), tinv=tinv)
+            corr = kernels.correlate(bank, engine.split_spectra(ps), ps.inv_norm, plan.dir_off, plan.dir_idx,
+                                     engine.crop,
+                                     validate=False, pmap=plan.pmap, tinv=tinv, order=plan.virtual_major(C),
+                                     dt=tile.get("dt", 0), pt=tile.get("pt", 0))
     scores = kernels.score_sum_segments(corr.peak, corr.best, plan.slot_off)
     return scores, corr, plan
 

@@ -422,3 +422,11 @@ particle_tail_kernel(const int* __restrict__ dir_off, int P, const float* __rest
 int gis_cu_count();
 // Looks up the allocation-free search workspace of the current device.
 int gis_split_scratch_setup();
+// The FP32 search on the split search's work decomposition (split.hip);
+// ws holds gis_correlate_split_workspace(n, C, B) bytes.
+int gis_correlate_units(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
+                        const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B, int P, int n,
+                        int C, int nrows, int ldt, int ldp, int dt, int pt, float* scores, int* peak_y, int* peak_x,
+                        int* best, float* peak, double* mean, double* stdv, void* ws, size_t ws_bytes,
+                        hipStream_t st);
+GIS_API size_t gis_correlate_split_workspace(int n, int C, int B);

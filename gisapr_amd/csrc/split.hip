@@ -57,6 +57,9 @@ constexpr int kXcds = 8;
 __device__ __forceinline__ floatx4 mfma_bf16(bf16x8 a, bf16x8 b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
+__device__ __forceinline__ floatx4 mfma_f32(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
 
 // x = hi + lo, hi = bf16(x), lo = bf16(x - hi), round to nearest even; two
 // values per packed conversion.
@@ -88,8 +91,10 @@ __device__ __forceinline__ void split_d(double v, __bf16& hi, __bf16& lo) {
 //   Et [nct][LT][hi, lo][64 lanes] x 8: lane (g, i) -> lag 16 tx + i, slot 2v + ri -> column 16 ct + 4g + v,
 //      (Re E, -Im E)
 // Lags beyond C and rows / columns beyond the plane are zero.
-__global__ void split_tables_kernel(int n, int C, int LT, int nks, int nct, bf16x8* __restrict__ Wt,
-                                    bf16x8* __restrict__ Et) {
+//   FP32 tables (f32): the same eight values per lane and fragment, rounded
+//   to float and stored [fragment][64 lanes][8] (one 32-byte run per lane).
+__global__ void split_tables_kernel(int n, int C, int LT, int nks, int nct, int f32, void* __restrict__ Wt,
+                                    void* __restrict__ Et) {
   const int m = n / 2 + 1, s = m / 2;
   const int nw = nks * 2 * LT * 64, ne = nct * LT * 64;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nw + ne; t += gridDim.x * blockDim.x) {
@@ -139,7 +144,14 @@ __global__ void split_tables_kernel(int n, int C, int LT, int nks, int nct, bf16
       }
       o = ((size_t)(ct * LT + tx) * 2) * 64 + lane;
     }
-    bf16x8* dst = t < nw ? Wt : Et;
+    if (f32) {
+      float* dst = (float*)(t < nw ? Wt : Et);
+      const size_t blk = (o - lane) / 128;  // fragment index
+#pragma unroll
+      for (int e = 0; e < 8; ++e) dst[(blk * 64 + lane) * 8 + e] = (float)v[e];
+      continue;
+    }
+    bf16x8* dst = (bf16x8*)(t < nw ? Wt : Et);
     bf16x8 hi, lo;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -192,18 +204,22 @@ struct Frags {
   float2 p[PT][4];
 };
 
-template <int LT, int DT, int PT>
+template <int LT, int DT, int PT, bool F32>
 __global__ void __launch_bounds__(64 * kWaves)
 split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spectra,
                        const float* __restrict__ inv_norm, const float* __restrict__ tinv,
                        const int* __restrict__ dir_off, const int* __restrict__ dir_idx,
                        const int* __restrict__ pmap, const int* __restrict__ order,
                        const int* __restrict__ unit_off, int B, int P, int n, int C, int ldt, int ldp, int nblk,
-                       const bf16x8* __restrict__ Wt, const bf16x8* __restrict__ Et, float* __restrict__ scores,
+                       const void* __restrict__ Wtv, const void* __restrict__ Etv, float* __restrict__ scores,
                        int* __restrict__ peak_y, int* __restrict__ peak_x) {
   // the row table (read every step) lives in LDS; the column table (read
   // once per column tile) is read from global memory (cached)
   extern __shared__ bf16x8 tab[];
+  const bf16x8* Wt = (const bf16x8*)Wtv;
+  const bf16x8* Et = (const bf16x8*)Etv;
+  const float4* Wf = (const float4*)tab;   // FP32 tables: [fragment][lane][2] float4
+  const float4* Ef = (const float4*)Etv;
   const int m = n / 2 + 1;
   // when the packed plane's last column is alone in its tile (n a multiple
   // of 32), it is searched separately: the wave's pairs share one tile
@@ -271,30 +287,57 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
 
   // stage 1 of one step from buffer f, then f is refilled with step (ct2, ks2)
   auto step = [&](int ks, Frags<DT, PT>& f, int ct2, int ks2) {
-    bf16x8 bh[2 * LT], bl[2 * LT];
+    if constexpr (F32) {
+      float w[2 * LT][8];
 #pragma unroll
-    for (int t = 0; t < 2 * LT; ++t) {
-      bh[t] = Wl[((ks * 2 * LT + t) * 2) * 64 + lane];
-      bl[t] = Wl[((ks * 2 * LT + t) * 2 + 1) * 64 + lane];
-    }
+      for (int t = 0; t < 2 * LT; ++t) {
+        const float4 w0 = Wf[((ks * 2 * LT + t) * 64 + lane) * 2], w1 = Wf[((ks * 2 * LT + t) * 64 + lane) * 2 + 1];
+        w[t][0] = w0.x; w[t][1] = w0.y; w[t][2] = w0.z; w[t][3] = w0.w;
+        w[t][4] = w1.x; w[t][5] = w1.y; w[t][6] = w1.z; w[t][7] = w1.w;
+      }
 #pragma unroll
-    for (int d = 0; d < DT; ++d) {
+      for (int d = 0; d < DT; ++d) {
 #pragma unroll
-      for (int p = 0; p < PT; ++p) {
-        float x[8];
+        for (int p = 0; p < PT; ++p) {
+          float x[8];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float2 a = f.t[d][j], q = f.p[p][j];
-          x[2 * j] = a.x * q.x - a.y * q.y;
-          x[2 * j + 1] = a.x * q.y + a.y * q.x;
+          for (int j = 0; j < 4; ++j) {
+            const float2 a = f.t[d][j], q = f.p[p][j];
+            x[2 * j] = a.x * q.x - a.y * q.y;
+            x[2 * j + 1] = a.x * q.y + a.y * q.x;
+          }
+#pragma unroll
+          for (int t = 0; t < 2 * LT; ++t)
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) gacc[d][p][t] = mfma_f32(x[kk], w[t][kk], gacc[d][p][t]);
         }
-        bf16x8 ahi, alo;
-        split8(x, ahi, alo);
+      }
+    } else {
+      bf16x8 bh[2 * LT], bl[2 * LT];
 #pragma unroll
-        for (int t = 0; t < 2 * LT; ++t) {
-          gacc[d][p][t] = mfma_bf16(ahi, bh[t], gacc[d][p][t]);
-          gacc[d][p][t] = mfma_bf16(ahi, bl[t], gacc[d][p][t]);
-          gacc[d][p][t] = mfma_bf16(alo, bh[t], gacc[d][p][t]);
+      for (int t = 0; t < 2 * LT; ++t) {
+        bh[t] = Wl[((ks * 2 * LT + t) * 2) * 64 + lane];
+        bl[t] = Wl[((ks * 2 * LT + t) * 2 + 1) * 64 + lane];
+      }
+#pragma unroll
+      for (int d = 0; d < DT; ++d) {
+#pragma unroll
+        for (int p = 0; p < PT; ++p) {
+          float x[8];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float2 a = f.t[d][j], q = f.p[p][j];
+            x[2 * j] = a.x * q.x - a.y * q.y;
+            x[2 * j + 1] = a.x * q.y + a.y * q.x;
+          }
+          bf16x8 ahi, alo;
+          split8(x, ahi, alo);
+#pragma unroll
+          for (int t = 0; t < 2 * LT; ++t) {
+            gacc[d][p][t] = mfma_bf16(ahi, bh[t], gacc[d][p][t]);
+            gacc[d][p][t] = mfma_bf16(ahi, bl[t], gacc[d][p][t]);
+            gacc[d][p][t] = mfma_bf16(alo, bh[t], gacc[d][p][t]);
+          }
         }
       }
     }
@@ -320,6 +363,32 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
       step(ks + 1, fb, wrap ? ctn : ct, wrap ? 1 : ks + 3);
     }
     // stage 2: S += Re(E^T G) over this column tile
+    if constexpr (F32) {
+      float e[LT][8];
+#pragma unroll
+      for (int tx = 0; tx < LT; ++tx) {
+        const float4 e0 = Ef[((ct * LT + tx) * 64 + lane) * 2], e1 = Ef[((ct * LT + tx) * 64 + lane) * 2 + 1];
+        e[tx][0] = e0.x; e[tx][1] = e0.y; e[tx][2] = e0.z; e[tx][3] = e0.w;
+        e[tx][4] = e1.x; e[tx][5] = e1.y; e[tx][6] = e1.z; e[tx][7] = e1.w;
+      }
+#pragma unroll
+      for (int d = 0; d < DT; ++d)
+#pragma unroll
+        for (int p = 0; p < PT; ++p)
+#pragma unroll
+          for (int ty = 0; ty < LT; ++ty)
+#pragma unroll
+            for (int tx = 0; tx < LT; ++tx) {
+              floatx4 o = out[d][p][tx * LT + ty];
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                o = mfma_f32(e[tx][2 * v], gacc[d][p][2 * ty][v], o);
+                o = mfma_f32(e[tx][2 * v + 1], gacc[d][p][2 * ty + 1][v], o);
+              }
+              out[d][p][tx * LT + ty] = o;
+            }
+      continue;
+    }
     bf16x8 eh[LT], el[LT];
 #pragma unroll
     for (int tx = 0; tx < LT; ++tx) {
@@ -412,15 +481,26 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
           x[2 * j + 1] = na[j].x * nq[j].y + na[j].y * nq[j].x;
         }
         if (ks + 1 < nks) nload(ks + 1);
-        bf16x8 ahi, alo;
-        split8(x, ahi, alo);
+        if constexpr (F32) {
 #pragma unroll
-        for (int t = 0; t < 2 * LT; ++t) {
-          const bf16x8 bh = Wl[((ks * 2 * LT + t) * 2) * 64 + lane];
-          const bf16x8 bl = Wl[((ks * 2 * LT + t) * 2 + 1) * 64 + lane];
-          gn[t] = mfma_bf16(ahi, bh, gn[t]);
-          gn[t] = mfma_bf16(ahi, bl, gn[t]);
-          gn[t] = mfma_bf16(alo, bh, gn[t]);
+          for (int t = 0; t < 2 * LT; ++t) {
+            const float4 w0 = Wf[((ks * 2 * LT + t) * 64 + lane) * 2];
+            const float4 w1 = Wf[((ks * 2 * LT + t) * 64 + lane) * 2 + 1];
+            const float w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) gn[t] = mfma_f32(x[kk], w[kk], gn[t]);
+          }
+        } else {
+          bf16x8 ahi, alo;
+          split8(x, ahi, alo);
+#pragma unroll
+          for (int t = 0; t < 2 * LT; ++t) {
+            const bf16x8 bh = Wl[((ks * 2 * LT + t) * 2) * 64 + lane];
+            const bf16x8 bl = Wl[((ks * 2 * LT + t) * 2 + 1) * 64 + lane];
+            gn[t] = mfma_bf16(ahi, bh, gn[t]);
+            gn[t] = mfma_bf16(ahi, bl, gn[t]);
+            gn[t] = mfma_bf16(alo, bh, gn[t]);
+          }
         }
       }
 #pragma unroll
@@ -492,15 +572,14 @@ inline size_t split_e_frags(int n, int C) { return (size_t)split_nct(n) * 2 * sp
 inline size_t split_lds_bytes(int n, int C) { return split_w_frags(n, C) * sizeof(bf16x8); }
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-template <int LT, int DT, int PT>
+template <int LT, int DT, int PT, bool F32>
 int launch_split(dim3 grid, size_t lds, hipStream_t st, const float2* bank, const float2* spectra,
                  const float* inv_norm, const float* tinv, const int* dir_off, const int* dir_idx, const int* pmap,
                  const int* order, const int* unit_off, int B, int P, int n, int C, int ldt, int ldp, int nblk,
-                 const bf16x8* Wt,
-                 const bf16x8* Et, float* scores, int* peak_y, int* peak_x) {
+                 const void* Wt, const void* Et, float* scores, int* peak_y, int* peak_x) {
   static bool attr_set = false;
   if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)split_correlate_kernel<LT, DT, PT>,
+    hipError_t e = hipFuncSetAttribute((const void*)split_correlate_kernel<LT, DT, PT, F32>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)split_lds_bytes(256, LT == 1 ? 16 : 32));
     if (e != hipSuccess) {
@@ -510,7 +589,7 @@ int launch_split(dim3 grid, size_t lds, hipStream_t st, const float2* bank, cons
     }
     attr_set = true;
   }
-  hipLaunchKernelGGL((split_correlate_kernel<LT, DT, PT>), grid, dim3(64 * kWaves), lds, st, bank, spectra,
+  hipLaunchKernelGGL((split_correlate_kernel<LT, DT, PT, F32>), grid, dim3(64 * kWaves), lds, st, bank, spectra,
                      inv_norm, tinv, dir_off, dir_idx, pmap, order, unit_off, B, P, n, C, ldt, ldp, nblk, Wt, Et, scores,
                      peak_y, peak_x);
   return GIS_OK;
@@ -547,15 +626,16 @@ int split_check(const char* who, int B, int P, int n, int C, int nrows, int ldt,
 int split_run(const void* bank, const void* spectra, const float* inv_norm, const float* tinv, const int* dir_off,
               const int* dir_idx, const int* pmap, const int* order, int B, int P, int n, int C, int nrows, int ldt,
               int ldp, int dt, int pt, float* scores, int* peak_y, int* peak_x, int* best, float* peak, double* mean,
-              double* stdv, void* ws, hipStream_t st) {
+              double* stdv, void* ws, hipStream_t st, bool f32 = false) {
   const int LT = split_lt(C);
   if (pt == 0) pt = std::min(P, 5);
-  if (dt == 0) dt = 2;  // measured fastest at box 256 (psi spectra shared by two directions)
-  if (LT == 2) dt = 1;  // two lag tiles per axis: one direction per wave (registers)
+  if (dt == 0) dt = f32 ? 1 : 2;  // measured fastest at box 256
+  if (LT == 2 || (f32 && dt > 2)) dt = 1;  // registers: one direction per wave with two lag tiles
   bf16x8* Wt = (bf16x8*)ws;
   bf16x8* Et = Wt + split_w_frags(n, C);
   int* unit_off = (int*)((char*)ws + align256((split_w_frags(n, C) + split_e_frags(n, C)) * sizeof(bf16x8)));
-  hipLaunchKernelGGL(split_tables_kernel, dim3(64), dim3(256), 0, st, n, C, LT, split_nks(n), split_nct(n), Wt, Et);
+  hipLaunchKernelGGL(split_tables_kernel, dim3(64), dim3(256), 0, st, n, C, LT, split_nks(n), split_nct(n),
+                     f32 ? 1 : 0, (void*)Wt, (void*)Et);
   GIS_LAUNCH_CHECK("gis_correlate_split tables");
   hipLaunchKernelGGL(split_units_kernel, dim3(1), dim3(1024), 0, st, dir_off, order, B, dt, unit_off);
   GIS_LAUNCH_CHECK("gis_correlate_split units");
@@ -570,31 +650,32 @@ int split_run(const void* bank, const void* spectra, const float* inv_norm, cons
 #define GIS_SPLIT_ARGS                                                                                          \
   grid, lds, st, (const float2*)bank, (const float2*)spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order,    \
       unit_off, B, P, n, C, ldt, ldp, nblk, Wt, Et, scores, peak_y, peak_x
-#define GIS_SPLIT_PT5(LTV, DTV)                                          \
-  switch (pt) {                                                         \
-    case 1: rc = launch_split<LTV, DTV, 1>(GIS_SPLIT_ARGS); break;      \
-    case 2: rc = launch_split<LTV, DTV, 2>(GIS_SPLIT_ARGS); break;      \
-    case 3: rc = launch_split<LTV, DTV, 3>(GIS_SPLIT_ARGS); break;      \
-    case 4: rc = launch_split<LTV, DTV, 4>(GIS_SPLIT_ARGS); break;      \
-    default: rc = launch_split<LTV, DTV, 5>(GIS_SPLIT_ARGS); break;     \
+#define GIS_SPLIT_PT5(LTV, DTV, F)                                         \
+  switch (pt) {                                                           \
+    case 1: rc = launch_split<LTV, DTV, 1, F>(GIS_SPLIT_ARGS); break;     \
+    case 2: rc = launch_split<LTV, DTV, 2, F>(GIS_SPLIT_ARGS); break;     \
+    case 3: rc = launch_split<LTV, DTV, 3, F>(GIS_SPLIT_ARGS); break;     \
+    case 4: rc = launch_split<LTV, DTV, 4, F>(GIS_SPLIT_ARGS); break;     \
+    default: rc = launch_split<LTV, DTV, 5, F>(GIS_SPLIT_ARGS); break;    \
   }
-#define GIS_SPLIT_PT3(LTV, DTV)                                         \
-  switch (pt) {                                                         \
-    case 1: rc = launch_split<LTV, DTV, 1>(GIS_SPLIT_ARGS); break;      \
-    case 2: rc = launch_split<LTV, DTV, 2>(GIS_SPLIT_ARGS); break;      \
-    default: rc = launch_split<LTV, DTV, 3>(GIS_SPLIT_ARGS); break;     \
-  }
-    if (LT == 2) {
-      GIS_SPLIT_PT5(2, 1)
+    if (f32) {
+      if (LT == 2) {
+        GIS_SPLIT_PT5(2, 1, true)
+      } else if (dt == 1) {
+        GIS_SPLIT_PT5(1, 1, true)
+      } else {
+        GIS_SPLIT_PT5(1, 2, true)
+      }
+    } else if (LT == 2) {
+      GIS_SPLIT_PT5(2, 1, false)
     } else if (dt == 1) {
-      GIS_SPLIT_PT5(1, 1)
+      GIS_SPLIT_PT5(1, 1, false)
     } else if (dt == 2) {
-      GIS_SPLIT_PT5(1, 2)
+      GIS_SPLIT_PT5(1, 2, false)
     } else {
-      GIS_SPLIT_PT5(1, 4)
+      GIS_SPLIT_PT5(1, 4, false)
     }
 #undef GIS_SPLIT_PT5
-#undef GIS_SPLIT_PT3
 #undef GIS_SPLIT_ARGS
     if (rc != GIS_OK) return rc;
     GIS_LAUNCH_CHECK("gis_correlate_split");
@@ -605,6 +686,22 @@ int split_run(const void* bank, const void* spectra, const float* inv_norm, cons
 }
 
 }  // namespace
+
+// The FP32 search of gis_correlate with the work decomposition of the split
+// search (one wave per particle, direction and psi group; no padded pair
+// slots): FP32 MFMA, the same two-stage functional, zero-padded rows.
+int gis_correlate_units(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
+                        const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B, int P, int n,
+                        int C, int nrows, int ldt, int ldp, int dt, int pt, float* scores, int* peak_y, int* peak_x,
+                        int* best, float* peak, double* mean, double* stdv, void* ws, size_t ws_bytes,
+                        hipStream_t st) {
+  const int rc = split_check("gis_correlate", B, P, n, C, nrows, ldt, ldp, dt, pt);
+  if (rc != GIS_OK) return rc;
+  if (B == 0) return GIS_OK;
+  GIS_REQUIRE(ws_bytes >= gis_correlate_split_workspace(n, C, B), "gis_correlate: workspace too small");
+  return split_run(bank, spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order, B, P, n, C, nrows, ldt, ldp, dt, pt,
+                   scores, peak_y, peak_x, best, peak, mean, stdv, ws, st, true);
+}
 
 // Workspace of the allocation-free entry point: one array per GPU, resident
 // until the process ends.  Host code must not use the array's name as a

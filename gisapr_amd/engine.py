@@ -205,8 +205,10 @@ class SearchEngine:
         if int(bank.shape[0]) != len(self.dir_rot):
             raise ValueError("template bank does not cover the engine's direction table")
         # dir_off / dir_idx were built on the host against this direction table
-        corr = kernels.correlate(bank, ps.spectra, ps.inv_norm, ps.dir_off, ps.dir_idx, self.crop,
-                                 validate=False, tinv=tinv)
+        if ps.particle_order is None:
+            ps.particle_order = particle_order(ps.dir_off.cpu().numpy(), ps.dir_idx.cpu().numpy(), self.device)
+        corr = kernels.correlate(bank, self.split_spectra(ps), ps.inv_norm, ps.dir_off, ps.dir_idx, self.crop,
+                                 validate=False, tinv=tinv, order=ps.particle_order)
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.crop // 2)
 
     def split_spectra(self, ps: ParticleSet) -> torch.Tensor:

@@ -423,11 +423,13 @@ def _check_pmap(pmap, B, rows):
 
 def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torch.Tensor],
               dir_off: torch.Tensor, dir_idx: torch.Tensor, crop: int, validate: bool = True,
-              pmap: Optional[torch.Tensor] = None, jt: int = 0,
-              tinv: Optional[torch.Tensor] = None) -> CorrelationResult:
+              pmap: Optional[torch.Tensor] = None, tinv: Optional[torch.Tensor] = None,
+              order: Optional[torch.Tensor] = None, dt: int = 0, pt: int = 0) -> CorrelationResult:
     """FP32 v606 search.  With ``pmap`` the searched particles are
     B = len(dir_off) - 1 entries each mapped to a row of ``spectra``;
-    ``tinv`` gives 1/norm per bank row for an unnormalised bank."""
+    ``tinv`` gives 1/norm per bank row for an unnormalised bank; ``bank`` and
+    ``spectra`` may carry zero-padded rows (``padded_row_stride``);
+    ``order`` / ``dt`` / ``pt`` as in ``correlate_split`` (dt 1 or 2)."""
     require_hip(bank, "bank")
     require_hip(spectra, "spectra")
     P, n = int(spectra.shape[1]), int(spectra.shape[2])
@@ -447,6 +449,13 @@ def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torc
         if total and (int(dir_idx.min()) < 0 or int(dir_idx.max()) >= int(bank.shape[0])):
             raise ValueError("dir_idx out of range of the template bank")
         _check_pmap(pmap, B, int(spectra.shape[0]))
+        if order is not None and not torch.equal(torch.sort(order.long()).values,
+                                                 torch.arange(B, device=order.device)):
+            raise ValueError("order must be a permutation of the particles")
+    if order is not None:
+        require_hip(order, "order")
+        if order.dtype != torch.int32 or tuple(order.shape) != (B,):
+            raise ValueError("order must be int32 [B]")
     if tinv is not None:
         require_hip(tinv, "tinv")
         if tinv.dtype != torch.float32 or tinv.numel() != int(bank.shape[0]):
@@ -459,11 +468,12 @@ def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torc
     peak = torch.empty((B,), dtype=torch.float32, device=dev)
     mean = torch.empty((B,), dtype=torch.float64, device=dev)
     std = torch.empty((B,), dtype=torch.float64, device=dev)
-    wsb = int(lib().gis_correlate_workspace(n))
+    wsb = int(lib().gis_correlate_workspace(n, int(crop), B))
     ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dev)
     check(lib().gis_correlate(ptr(bank), ptr(spectra), ptr(inv_norm), ptr(tinv), ptr(dir_off), ptr(dir_idx),
-                              ptr(pmap), B, P, n, int(crop), int(jt), ptr(scores), ptr(py), ptr(px), ptr(best),
-                              ptr(peak), ptr(mean), ptr(std), ptr(ws), wsb, stream_of(dev)), "gis_correlate")
+                              ptr(pmap), ptr(order), B, P, n, int(crop), total, ldt, ldp, int(dt), int(pt),
+                              ptr(scores), ptr(py), ptr(px), ptr(best), ptr(peak), ptr(mean), ptr(std), ptr(ws),
+                              wsb, stream_of(dev)), "gis_correlate")
     return CorrelationResult(scores, py, px, best, peak, mean, std)
 
 

@@ -244,32 +244,38 @@ int gis_prepare_particles(const float* images, int B, int N, int n, int P, const
 int gis_ctf_packed(const float* ctf_tab, const float* defocus, int B, int n, const float* kconst, float* out,
                    void* stream);
 
-/* Workspace bytes gis_correlate needs at newbox n. */
-size_t gis_correlate_workspace(int n);
+/* Workspace bytes gis_correlate needs at newbox n, crop C, B particles. */
+size_t gis_correlate_workspace(int n, int C, int B);
 
 /*
  * (d) FFT cross-correlation search + per-particle reduction.  Replaces
  * correlate_templates_and_particle (v606:1418-1449) and the numerics of
- * make_result_records (v606:1452-1491), batched over B particles.  Particle
- * b is correlated against bank rows dir_idx[dir_off[b] .. dir_off[b+1]).
+ * make_result_records (v606:1452-1491), batched over B particles, on FP32
+ * MFMA (the operator as a two-stage linear functional of the unshifted
+ * product).  Particle b is correlated against bank rows
+ * dir_idx[dir_off[b] .. dir_off[b+1]).
+ *   bank [rows][n][ldt], spectra [particles][P][n][ldp] complex64, rows ld
+ *        >= n/2+1 complex values apart (columns past n/2 zero when padded)
  *   inv_norm [particles][P] 1/norm of each spectrum or NULL; tinv [bank rows]
  *        1/norm of each template row or NULL (bank already unit norm)
  *   pmap [B] spectra row of searched particle b, or NULL (identity): one
  *        launch can search several candidates' banks (rows c*M + direction)
  *        against one resident particle stack (ZL:gisapr_batch_search.py:318-340)
- *   jt   template tiles per sweep, 0 = automatic (autotuner knob)
+ *   order [B] processing order of the searched particles or NULL
+ *   nrows dir_off[B], the number of dir_idx entries
+ *   dt, pt directions (1 or 2) / psi values (1..5) per wave, 0 = automatic
+ *        (autotuner knobs: results do not depend on them)
  *   scores/peak_y/peak_x [dir_off[B]][P]: score = max(ccg crop) * 1024 n^2,
  *                        (y, x) = divmod(argmax, C) in the C x C crop
  *   best [B]: flat argmax over (local direction, psi) or -1 if none
  *   peak [B] float32, mean/std [B] float64 (population std)
  * C (--ccgCropSize) must be even and <= 32 (the reference default is 32,
- * v606:1562); a crop above 16 runs as four 16 x 16 lag windows.
+ * v606:1562); n <= 256.
  */
 int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,
-                  const int* dir_off, const int* dir_idx, const int* pmap, int B, int P, int n, int C, int jt,
-                  float* scores,
-                  int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
-                  size_t ws_bytes, void* stream);
+                  const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B, int P, int n,
+                  int C, int nrows, int ldt, int ldp, int dt, int pt, float* scores, int* peak_y, int* peak_x,
+                  int* best, float* peak, double* mean, double* stdv, void* ws, size_t ws_bytes, void* stream);
 
 /* Workspace bytes gis_correlate_split needs at newbox n, crop C, B particles. */
 size_t gis_correlate_split_workspace(int n, int C, int B);

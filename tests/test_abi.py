@@ -46,8 +46,8 @@ def test_version_and_argument_errors_without_gpu(library):
     # argument validation happens before any device call
     rc = library.gis_splat(None, None, 0, 0, 6, 1.0, 1.0, 1.0, None, None)
     assert rc == -1 and b"box" in library.gis_last_error()
-    rc = library.gis_correlate(None, None, None, None, None, None, None, 1, 5, 64, 34, 0, None, None, None,
-                               None, None, None, None, None, 0, None)
+    rc = library.gis_correlate(None, None, None, None, None, None, None, None, 1, 5, 64, 34, 1, 33, 33, 0, 0, None,
+                               None, None, None, None, None, None, None, 0, None)
     assert rc == -2 and b"crop" in library.gis_last_error()
     rc = library.gis_correlate_split(None, None, None, None, None, None, None, None, 1, 5, 64, 34, 1, 33, 33, 0, 0,
                                      None, None, None, None, None, None, None, None, 0, None)
