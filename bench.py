@@ -274,7 +274,7 @@ def precision_check_split(eng, ps, bank, res, subset: int, seed: int):
     A top-1 difference counts as a near-tie when the float64 scores of the two
     choices differ by at most 1e-5 of the particle's best score."""
     import torch
-    from gisapr_amd import host
+    from gisapr_amd import host, kernels
     dev = eng.device
     raw, tinv = bank
     W, E = host.crop_functional(eng.n, eng.crop)
@@ -295,7 +295,8 @@ def precision_check_split(eng, ps, bank, res, subset: int, seed: int):
         if len(d) == 0:
             continue
         dd = torch.from_numpy(d.astype(np.int64)).to(dev)
-        t = raw[dd][..., :eng.n // 2 + 1].to(torch.complex128) * tinv[dd].double()[:, None, None]
+        rows = raw[dd].unpack() if isinstance(raw, kernels.FragmentPlanes) else raw[dd][..., :eng.n // 2 + 1]
+        t = rows.to(torch.complex128) * tinv[dd].double()[:, None, None]
         p = ps.spectra[b].to(torch.complex128) * ps.inv_norm[b].double()[:, None, None]
         q = t[:, None] * p[None]                                    # [Mi, P, n, m]
         crop = 1024.0 * (Wt @ q @ Et).real.reshape(len(d) * P, C * C)
@@ -603,12 +604,12 @@ def main() -> int:
     bank = bank_split = tk_last = None  # the last candidate's (bank, 1/norms) / template coefficients
     if "fp32" in want:
         # FP32 path: full unnormalised bank (1/norm folded into the search)
-        legs["fp32"], results["fp32"], bank = run_single(lambda v: eng.templates(v, normalize=False, padded=True),
+        legs["fp32"], results["fp32"], bank = run_single(lambda v: eng.templates(v, normalize=False, fragments=True),
                                                          lambda t, p: eng.search(t[0], p, t[1]))
     if "split" in want:
-        # the same search on split-bf16 MFMA, same bank values in zero-padded rows
+        # the same search on split-bf16 MFMA, the same bank values
         legs["split"], results["split"], bank_split = run_single(
-            lambda v: eng.templates(v, normalize=False, padded=True), lambda t, p: eng.search_split(t[0], p, t[1]))
+            lambda v: eng.templates(v, normalize=False, fragments=True), lambda t, p: eng.search_split(t[0], p, t[1]))
     if "mixed" in want:
         # split-bf16 path: the template kernel writes the in-disk coefficients directly
         legs["mixed"], results["mixed"], tk_last = run_single(eng.small_templates,

@@ -53,6 +53,8 @@ SIGNATURES = {
     "gis_correlate": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp, _vp,
                            _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gis_correlate_split_workspace": (_sz, [_i, _i, _i]),
+    "gis_fragment_plane": (_sz, [_i]),
+    "gis_pack_fragments": (_i, [_vp, _i, _i, _i, _vp, _vp]),
     "gis_correlate_split": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp,
                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gis_correlate_split_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i,

@@ -326,10 +326,10 @@ class CandidateBatch:
             self.bank = None
             self.tk = torch.empty((self.slots * M, engine.small.Kpad), dtype=torch.complex64, device=dev)
             self.norms = torch.empty((self.slots * M,), dtype=torch.float32, device=dev)
-        else:  # the split search reads zero-padded rows (aligned 128-byte segments)
-            ld = kernels.padded_row_stride(n)
-            self.bank = torch.empty((self.slots * M, n, ld), dtype=torch.complex64, device=dev)
-            engine.split_spectra(ps)
+        else:  # the searches' operand layout, written by the template kernel
+            self.bank = kernels.FragmentPlanes(torch.empty((self.slots * M, kernels.fragment_plane(n)),
+                                                           dtype=torch.complex64, device=dev), n)
+            engine.search_spectra(ps)
         self.tinv = torch.empty((self.slots * M,), dtype=torch.float32, device=dev)  # 1/norm per bank row
         self.tpl_ws = torch.empty((max(int(kernels.lib().gis_build_templates_workspace(N, self.group * M)), 1),),
                                   dtype=torch.uint8, device=dev)
@@ -380,11 +380,11 @@ class CandidateBatch:
                                            rows=self.plan.particle_major(self.slots if k is None else k),
                                            operand=eng.small.operand)
         if self.path == "split":
-            return kernels.correlate_split(self.bank, eng.split_spectra(ps), ps.inv_norm, dir_off, dir_idx, eng.crop,
+            return kernels.correlate_split(self.bank, eng.search_spectra(ps), ps.inv_norm, dir_off, dir_idx, eng.crop,
                                            validate=False, pmap=pmap, tinv=self.tinv,
                                            order=self.plan.virtual_major(self.slots if k is None else k),
                                            dt=tile.get("dt", 0), pt=tile.get("pt", 0))
-        return kernels.correlate(self.bank, eng.split_spectra(ps), ps.inv_norm, dir_off, dir_idx, eng.crop,
+        return kernels.correlate(self.bank, eng.search_spectra(ps), ps.inv_norm, dir_off, dir_idx, eng.crop,
                                  validate=False, pmap=pmap, tinv=self.tinv,
                                  order=self.plan.virtual_major(self.slots if k is None else k),
                                  dt=tile.get("dt", 0), pt=tile.get("pt", 0))
@@ -446,7 +446,7 @@ class CandidateBatch:
         eng, ps = self.engine, self.ps
         if self.path == "mixed":
             return (ps.coeffs, eng.small_bhi, eng.small_blo, eng.small_plast, eng.small_kslot)
-        return (ps.spectra_split, ps.inv_norm)
+        return (ps.spectra_frag.data, ps.inv_norm)
 
     def evaluate(self, poses, sync: bool = True):
         """Score ``poses`` ([k, 6], k <= slots); returns float64 [k] (or the device
@@ -601,7 +601,8 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
         if ps.spectra is None:
             engine.prepare(ps)
         n = engine.n
-        bank = torch.empty((C * M, n, kernels.padded_row_stride(n)), dtype=torch.complex64, device=engine.device)
+        bank = kernels.FragmentPlanes(torch.empty((C * M, kernels.fragment_plane(n)), dtype=torch.complex64,
+                                                  device=engine.device), n)
         tinv = torch.empty((C * M,), dtype=torch.float32, device=engine.device)
         for c, vol in enumerate(volumes):
             dft = kernels.volume_dft(vol)
@@ -609,12 +610,12 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
                                                out=bank[c * M:(c + 1) * M], normalize=False)
             torch.reciprocal(norms, out=tinv[c * M:(c + 1) * M])
         if path == "split":
-            corr = kernels.correlate_split(bank, engine.split_spectra(ps), ps.inv_norm, plan.dir_off, plan.dir_idx,
+            corr = kernels.correlate_split(bank, engine.search_spectra(ps), ps.inv_norm, plan.dir_off, plan.dir_idx,
                                            engine.crop, validate=False, pmap=plan.pmap, tinv=tinv,
                                            order=plan.virtual_major(C),
                                            dt=tile.get("dt", 0), pt=tile.get("pt", 0))
         else:
-            corr = kernels.correlate(bank, engine.split_spectra(ps), ps.inv_norm, plan.dir_off, plan.dir_idx,
+            corr = kernels.correlate(bank, engine.search_spectra(ps), ps.inv_norm, plan.dir_off, plan.dir_idx,
                                      engine.crop,
                                      validate=False, pmap=plan.pmap, tinv=tinv, order=plan.virtual_major(C),
                                      dt=tile.get("dt", 0), pt=tile.get("pt", 0))

@@ -419,6 +419,45 @@ particle_tail_kernel(const int* __restrict__ dir_off, int P, const float* __rest
 }  // namespace
 
 // Number of compute units of the current device (cached per device).
+// Fragment order of a packed plane (n rows, m = n/2+1 columns), the search
+// operand layout read with 16-byte loads: tile (ks, ct) of rows
+// [16 ks, 16 ks + 16) x columns [16 ct, 16 ct + 16) is 256 complex values at
+// (ks * nct + ct) * 256, value (row 16 ks + 4g + 2h + e, column 16 ct + i) at
+// ((h * 4 + g) * 16 + i) * 2 + e; when the last column is alone in its tile
+// (m % 16 == 1) nct counts the tiles before it and its values follow the
+// tiles, row 16 ks + 4g + j at nks * nct * 256 + 16 ks + 4g + j.  Rows >= n
+// and columns >= m hold zeros.  nks = the searches' even-padded row steps.
+__host__ __device__ inline int frag_nks(int n) { return (((n + 15) / 16) + 1) & ~1; }
+__host__ __device__ inline int frag_nct(int n) {
+  const int m = n / 2 + 1;
+  return (m & 15) == 1 ? (m >> 4) : (m + 15) >> 4;
+}
+__host__ __device__ inline int frag_plane(int n) {
+  const int m = n / 2 + 1;
+  return frag_nks(n) * frag_nct(n) * 256 + ((m & 15) == 1 ? frag_nks(n) * 16 : 0);
+}
+
+// Position of (row r, column c) in a fragment-order plane (nct = frag_nct(n)).
+__host__ __device__ inline int frag_pos(int n, int nct, int r, int c) {
+  const int m = n / 2 + 1;
+  if ((m & 15) == 1 && c == m - 1) return frag_nks(n) * nct * 256 + r;
+  return ((r >> 4) * nct + (c >> 4)) * 256 + ((((r >> 1) & 1) * 4 + ((r >> 2) & 3)) * 16 + (c & 15)) * 2 + (r & 1);
+}
+
+// (row, column) of position e of a fragment-order plane (tiles = frag_nks(n)
+// * nct * 256); a row >= n or column >= n/2+1 marks a zero pad.
+__host__ __device__ inline void frag_rc(int n, int nct, int tiles, int e, int& r, int& c) {
+  if (e < tiles) {
+    const int tl = e >> 8, w = e & 255;
+    const int ks = tl / nct, hg = w >> 5;
+    r = 16 * ks + 4 * (hg & 3) + 2 * (hg >> 2) + (w & 1);
+    c = 16 * (tl - ks * nct) + ((w >> 1) & 15);
+  } else {
+    r = e - tiles;  // the lone last column
+    c = n / 2;
+  }
+}
+
 int gis_cu_count();
 // Looks up the allocation-free search workspace of the current device.
 int gis_split_scratch_setup();

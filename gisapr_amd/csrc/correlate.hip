@@ -61,8 +61,10 @@ GIS_API size_t gis_correlate_workspace(int n, int C, int B) { return gis_correla
 // Replaces correlate_templates_and_particle (v606:1418-1449) and the numeric
 // part of make_result_records (v606:1466-1491), batched over particles.
 //   bank      [Mtot][n][ldt] complex64 templates (UNshifted rfft2 layout,
-//             rows ldt >= n/2+1 apart, columns past n/2 zero)
-//   spectra   [B'][P][n][ldp] complex64 conjugated particle spectra (UNshifted)
+//             rows ldt >= n/2+1 apart, columns past n/2 zero), or with
+//             ldt = 0 [Mtot][gis_fragment_plane(n)] in fragment order
+//   spectra   [B'][P][n][ldp] complex64 conjugated particle spectra
+//             (UNshifted; ldp = 0: fragment order, as the bank)
 //   inv_norm  [B'][P] float32 per-spectrum 1/L2 norm, or NULL (already unit norm)
 //   tinv      [Mtot] float32 per-template 1/L2 norm, or NULL (bank already unit norm)
 //   dir_off   [B+1] int32 offsets into dir_idx; dir_idx global template indices

@@ -85,6 +85,20 @@ __device__ __forceinline__ void split_d(double v, __bf16& hi, __bf16& lo) {
   lo = (__bf16)(float)(v - (double)(float)hi);
 }
 
+// dst [rows][frag_plane(n)] from src [rows][n][ld] (row major, ld >= n/2+1).
+__global__ void pack_fragments_kernel(const float2* __restrict__ src, int rows, int n, int ld,
+                                      float2* __restrict__ dst) {
+  const int m = n / 2 + 1, nct = frag_nct(n), F = frag_plane(n);
+  const int tiles = frag_nks(n) * nct * 256;
+  const size_t total = (size_t)rows * F;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t row = t / F;
+    int r, c;
+    frag_rc(n, nct, tiles, (int)(t - row * F), r, c);
+    dst[t] = (r < n && c < m) ? src[row * (size_t)n * ld + (size_t)r * ld + c] : make_float2(0.f, 0.f);
+  }
+}
+
 // Split tables in fragment order (hi at o, lo at o + 64).
 //   Wt [nks][2 LT tiles][hi, lo][64 lanes] x 8: tile 2 ty + part, lane (g, i) -> lag 16 ty + i,
 //      slot 2j + ri -> row 16 ks + 4g + j; part 0: (Re W, -Im W), part 1: (Im W, Re W)
@@ -204,7 +218,7 @@ struct Frags {
   float2 p[PT][4];
 };
 
-template <int LT, int DT, int PT, bool F32>
+template <int LT, int DT, int PT, bool F32, bool FR>
 __global__ void __launch_bounds__(64 * kWaves)
 split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spectra,
                        const float* __restrict__ inv_norm, const float* __restrict__ tinv,
@@ -250,13 +264,18 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
   const int p0 = (int)blockIdx.y * PT;
   const int pb = pmap ? pmap[b] : b;
   // template rows are ldt complex values apart, psi spectrum rows ldp
-  // (ld > n/2+1: zero padding, so 16-column segments start on 128-byte lines)
+  // (ld > n/2+1: zero padding, so 16-column segments start on 128-byte
+  // lines); ld = 0: the operand is in fragment order (frag_plane), each
+  // step's tile two contiguous 1 KiB runs read with 16-byte loads
+  const size_t fplane = (size_t)frag_plane(n);
   const float2* tp[DT];
   const float2* pp[PT];
 #pragma unroll
-  for (int d = 0; d < DT; ++d) tp[d] = bank + (size_t)dir_idx[d0 + min(j0 + d, Mi - 1)] * ((size_t)n * ldt);
+  for (int d = 0; d < DT; ++d)
+    tp[d] = bank + (size_t)dir_idx[d0 + min(j0 + d, Mi - 1)] * (FR ? fplane : (size_t)n * ldt);
 #pragma unroll
-  for (int p = 0; p < PT; ++p) pp[p] = spectra + ((size_t)pb * P + min(p0 + p, P - 1)) * ((size_t)n * ldp);
+  for (int p = 0; p < PT; ++p)
+    pp[p] = spectra + ((size_t)pb * P + min(p0 + p, P - 1)) * (FR ? fplane : (size_t)n * ldp);
   const int g = lane >> 4, i = lane & 15;
 
   // Operands of step (ct, ks).  Loads are unconditional: an offset past the
@@ -267,13 +286,40 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
   const int lastt = n * ldt - 1, lastp = n * ldp - 1;
   auto load = [&](int ct, int ks, Frags<DT, PT>& f) {
     const int r0 = 16 * ks + 4 * g, c = 16 * ct + i;
+    const int tile = (ks * nct + ct) * 256 + 2 * lane;  // fragment order: [h][lane][2 rows]
+    if constexpr (FR) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int ot = min((r0 + j) * ldt + c, lastt), op = min((r0 + j) * ldp + c, lastp);
+      for (int d = 0; d < DT; ++d) {
+        const float4 q0 = *(const float4*)(tp[d] + tile), q1 = *(const float4*)(tp[d] + tile + 128);
+        f.t[d][0] = make_float2(q0.x, q0.y);
+        f.t[d][1] = make_float2(q0.z, q0.w);
+        f.t[d][2] = make_float2(q1.x, q1.y);
+        f.t[d][3] = make_float2(q1.z, q1.w);
+      }
+    } else {
 #pragma unroll
-      for (int d = 0; d < DT; ++d) f.t[d][j] = tp[d][ot];
+      for (int j = 0; j < 4; ++j) {
+        const int ot = min((r0 + j) * ldt + c, lastt);
 #pragma unroll
-      for (int p = 0; p < PT; ++p) f.p[p][j] = pp[p][op];
+        for (int d = 0; d < DT; ++d) f.t[d][j] = tp[d][ot];
+      }
+    }
+    if constexpr (FR) {
+#pragma unroll
+      for (int p = 0; p < PT; ++p) {
+        const float4 q0 = *(const float4*)(pp[p] + tile), q1 = *(const float4*)(pp[p] + tile + 128);
+        f.p[p][0] = make_float2(q0.x, q0.y);
+        f.p[p][1] = make_float2(q0.z, q0.w);
+        f.p[p][2] = make_float2(q1.x, q1.y);
+        f.p[p][3] = make_float2(q1.z, q1.w);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int op = min((r0 + j) * ldp + c, lastp);
+#pragma unroll
+        for (int p = 0; p < PT; ++p) f.p[p][j] = pp[p][op];
+      }
     }
   };
 
@@ -468,8 +514,9 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int r = 16 * ks + 4 * g + j;
-          na[j] = lt[min(r * ldt + (m - 1), lastt)];
-          nq[j] = lq[min(r * ldp + (m - 1), lastp)];
+          const int lone_at = nks * nct * 256 + 16 * ks + 4 * g + j;  // fragment order: the lone column
+          na[j] = FR ? lt[lone_at] : lt[min(r * ldt + (m - 1), lastt)];
+          nq[j] = FR ? lq[lone_at] : lq[min(r * ldp + (m - 1), lastp)];
         }
       };
       nload(0);
@@ -572,14 +619,14 @@ inline size_t split_e_frags(int n, int C) { return (size_t)split_nct(n) * 2 * sp
 inline size_t split_lds_bytes(int n, int C) { return split_w_frags(n, C) * sizeof(bf16x8); }
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-template <int LT, int DT, int PT, bool F32>
+template <int LT, int DT, int PT, bool F32, bool FR>
 int launch_split(dim3 grid, size_t lds, hipStream_t st, const float2* bank, const float2* spectra,
                  const float* inv_norm, const float* tinv, const int* dir_off, const int* dir_idx, const int* pmap,
                  const int* order, const int* unit_off, int B, int P, int n, int C, int ldt, int ldp, int nblk,
                  const void* Wt, const void* Et, float* scores, int* peak_y, int* peak_x) {
   static bool attr_set = false;
   if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)split_correlate_kernel<LT, DT, PT, F32>,
+    hipError_t e = hipFuncSetAttribute((const void*)split_correlate_kernel<LT, DT, PT, F32, FR>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)split_lds_bytes(256, LT == 1 ? 16 : 32));
     if (e != hipSuccess) {
@@ -589,13 +636,32 @@ int launch_split(dim3 grid, size_t lds, hipStream_t st, const float2* bank, cons
     }
     attr_set = true;
   }
-  hipLaunchKernelGGL((split_correlate_kernel<LT, DT, PT, F32>), grid, dim3(64 * kWaves), lds, st, bank, spectra,
+  hipLaunchKernelGGL((split_correlate_kernel<LT, DT, PT, F32, FR>), grid, dim3(64 * kWaves), lds, st, bank, spectra,
                      inv_norm, tinv, dir_off, dir_idx, pmap, order, unit_off, B, P, n, C, ldt, ldp, nblk, Wt, Et, scores,
                      peak_y, peak_x);
   return GIS_OK;
 }
 
 }  // namespace
+
+// Complex values of one plane in fragment order at newbox n.
+GIS_API size_t gis_fragment_plane(int n) { return n >= 2 && n % 2 == 0 ? (size_t)frag_plane(n) : 0; }
+
+// dst [rows][gis_fragment_plane(n)] complex64, the planes of src
+// [rows][n][ld] (ld >= n/2+1) in fragment order: the searches' operand
+// layout read with 16-byte loads (pass ld = 0 to them).
+GIS_API int gis_pack_fragments(const void* src, int rows, int n, int ld, void* dst, void* stream) {
+  GIS_REQUIRE(rows >= 0 && n >= 2 && n % 2 == 0 && ld >= n / 2 + 1, "gis_pack_fragments: rows %d n %d ld %d", rows,
+              n, ld);
+  if (rows == 0) return GIS_OK;
+  GIS_REQUIRE(src && dst, "gis_pack_fragments: null pointer");
+  const size_t total = (size_t)rows * frag_plane(n);
+  const int blocks = (int)std::min<size_t>((total + 255) / 256, (size_t)gis_cu_count() * 32);
+  hipLaunchKernelGGL(pack_fragments_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const float2*)src, rows,
+                     n, ld, (float2*)dst);
+  GIS_LAUNCH_CHECK("gis_pack_fragments");
+  return GIS_OK;
+}
 
 GIS_API size_t gis_correlate_split_workspace(int n, int C, int B) {
   if (n < 2 || C < 2 || B < 0) return 0;
@@ -607,8 +673,8 @@ namespace {
 
 // Argument checks shared by both entry points; GIS_OK when the call may go on.
 int split_check(const char* who, int B, int P, int n, int C, int nrows, int ldt, int ldp, int dt, int pt) {
-  GIS_REQUIRE(ldt >= n / 2 + 1 && ldp >= n / 2 + 1, "%s: row strides %d, %d below n/2+1 = %d", who, ldt, ldp,
-              n / 2 + 1);
+  GIS_REQUIRE((ldt == 0 && ldp == 0) || (ldt >= n / 2 + 1 && ldp >= n / 2 + 1),
+              "%s: row strides %d, %d: both 0 (fragment order) or both >= n/2+1 = %d", who, ldt, ldp, n / 2 + 1);
   GIS_REQUIRE(B >= 0 && P >= 1 && nrows >= 0, "%s: B=%d P=%d nrows=%d", who, B, P, nrows);
   GIS_REQUIRE(n >= 4 && n % 2 == 0 && n <= 256, "%s: newbox %d must be even, in [4, 256]", who, n);
   if (C > 32 || C < 2 || C % 2 != 0 || C > n) {
@@ -629,7 +695,7 @@ int split_run(const void* bank, const void* spectra, const float* inv_norm, cons
               double* stdv, void* ws, hipStream_t st, bool f32 = false) {
   const int LT = split_lt(C);
   if (pt == 0) pt = std::min(P, 5);
-  if (dt == 0) dt = f32 ? 1 : 2;  // measured fastest at box 256
+  if (dt == 0) dt = 1;  // measured fastest at box 256
   if (LT == 2 || (f32 && dt > 2)) dt = 1;  // registers: one direction per wave with two lag tiles
   bf16x8* Wt = (bf16x8*)ws;
   bf16x8* Et = Wt + split_w_frags(n, C);
@@ -650,13 +716,19 @@ int split_run(const void* bank, const void* spectra, const float* inv_norm, cons
 #define GIS_SPLIT_ARGS                                                                                          \
   grid, lds, st, (const float2*)bank, (const float2*)spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order,    \
       unit_off, B, P, n, C, ldt, ldp, nblk, Wt, Et, scores, peak_y, peak_x
-#define GIS_SPLIT_PT5(LTV, DTV, F)                                         \
-  switch (pt) {                                                           \
-    case 1: rc = launch_split<LTV, DTV, 1, F>(GIS_SPLIT_ARGS); break;     \
-    case 2: rc = launch_split<LTV, DTV, 2, F>(GIS_SPLIT_ARGS); break;     \
-    case 3: rc = launch_split<LTV, DTV, 3, F>(GIS_SPLIT_ARGS); break;     \
-    case 4: rc = launch_split<LTV, DTV, 4, F>(GIS_SPLIT_ARGS); break;     \
-    default: rc = launch_split<LTV, DTV, 5, F>(GIS_SPLIT_ARGS); break;    \
+#define GIS_SPLIT_PT(LTV, DTV, F, R)                                         \
+  switch (pt) {                                                              \
+    case 1: rc = launch_split<LTV, DTV, 1, F, R>(GIS_SPLIT_ARGS); break;     \
+    case 2: rc = launch_split<LTV, DTV, 2, F, R>(GIS_SPLIT_ARGS); break;     \
+    case 3: rc = launch_split<LTV, DTV, 3, F, R>(GIS_SPLIT_ARGS); break;     \
+    case 4: rc = launch_split<LTV, DTV, 4, F, R>(GIS_SPLIT_ARGS); break;     \
+    default: rc = launch_split<LTV, DTV, 5, F, R>(GIS_SPLIT_ARGS); break;    \
+  }
+#define GIS_SPLIT_PT5(LTV, DTV, F)          \
+  if (ldt == 0) {                           \
+    GIS_SPLIT_PT(LTV, DTV, F, true)         \
+  } else {                                  \
+    GIS_SPLIT_PT(LTV, DTV, F, false)        \
   }
     if (f32) {
       if (LT == 2) {
@@ -676,6 +748,7 @@ int split_run(const void* bank, const void* spectra, const float* inv_norm, cons
       GIS_SPLIT_PT5(1, 4, false)
     }
 #undef GIS_SPLIT_PT5
+#undef GIS_SPLIT_PT
 #undef GIS_SPLIT_ARGS
     if (rc != GIS_OK) return rc;
     GIS_LAUNCH_CHECK("gis_correlate_split");
@@ -749,8 +822,10 @@ int gis_split_scratch_setup() { return split_scratch("gis_init") ? GIS_OK : GIS_
 // arguments and outputs of gis_correlate, plus
 //   order  [B] processing order of the particles (a permutation) or NULL
 //   nrows  dir_off[B], the number of dir_idx entries
-//   ldt    complex values between template rows (n/2+1, or more: zero padding)
-//   ldp    complex values between spectrum rows (likewise)
+//   ldt    complex values between template rows (n/2+1, or more: zero
+//          padding), or 0: the bank's planes are in fragment order
+//          (gis_pack_fragments, gis_fragment_plane(n) values per plane)
+//   ldp    the same for the spectra
 //   dt     directions per wave (1, 2 or 4; 0 = automatic)
 //   pt     psi values per wave (1..5; 0 = automatic)
 GIS_API int gis_correlate_split(const void* bank, const void* spectra, const float* inv_norm, const float* tinv,

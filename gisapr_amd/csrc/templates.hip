@@ -125,6 +125,7 @@ struct CoefOut {
   int K, Kpad;
   float2* coeffs;    // [M][Kpad]
   int ldb = 0;       // bank row stride in complex values (0: n/2+1); rows are n of them
+  bool frag = false; // bank planes in fragment order (frag_pos; ldb unused)
 };
 
 // Real-space modes (the standalone projector's path): `img` replaces the
@@ -775,7 +776,10 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
 
     // ---- phase 9: forward column FFT (N), crop to n, pack, FSC -----------
     const int ldb = co.ldb > 0 ? co.ldb : mh_n;
-    float2* out = co.kslot ? nullptr : bank + (size_t)w * n * ldb;
+    const int fnct = frag_nct(n);
+    const size_t plane = co.frag ? (size_t)frag_plane(n) : (size_t)n * ldb;
+    float2* out = co.kslot ? nullptr : bank + (size_t)w * plane;
+    auto bidx = [&](int r, int c) -> size_t { return co.frag ? (size_t)frag_pos(n, fnct, r, c) : (size_t)r * ldb + c; };
     double nrm[1] = {0.0};
     const int kmax = n / 2;  // columns 0..n/2 are needed
     for (int k0 = 0; k0 <= kmax; k0 += T::TK3) {
@@ -826,14 +830,21 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
               const int j = __float_as_int(gj.y);
               if (j >= 0) co.coeffs[(size_t)w * co.Kpad + j] = v;
             } else {
-              out[(size_t)r * ldb + c] = v;
+              out[bidx(r, c)] = v;
             }
             nrm[0] += (double)v.x * v.x + (double)v.y * v.y;
           });
       __syncthreads();
     }
 
-    if (out && ldb > mh_n) {  // zero padding past column n/2 (read by searches as a clamp target)
+    if (out && co.frag && plane > (size_t)n * mh_n) {  // fragment order: zeros past row n / column n/2
+      const int tiles = frag_nks(n) * fnct * 256;
+      for (int e = tid; e < (int)plane; e += nthr) {
+        int r, c;
+        frag_rc(n, fnct, tiles, e, r, c);
+        if (r >= n || c >= mh_n) out[e] = make_float2(0.f, 0.f);
+      }
+    } else if (out && !co.frag && ldb > mh_n) {  // zero padding past column n/2 (read by searches as a clamp target)
       const int pw = ldb - mh_n;
       for (int t = tid; t < n * pw; t += nthr) {
         const int r = t / pw, c = t - r * pw;
@@ -864,8 +875,8 @@ build_templates_kernel(const float2* __restrict__ D0, VolSet vs, const float* __
     } else if (normalize) {
       for (int t = tid; t < n * mh_n; t += nthr) {
         const int r = t / mh_n, c = t - r * mh_n;
-        float2 v = out[(size_t)r * ldb + c];
-        out[(size_t)r * ldb + c] = make_float2(v.x / norm, v.y / norm);
+        float2 v = out[bidx(r, c)];
+        out[bidx(r, c)] = make_float2(v.x / norm, v.y / norm);
       }
     }
     if (norms && tid == 0) norms[w] = norm;
@@ -1062,8 +1073,9 @@ GIS_API int gis_build_templates_batched(const void* dft, size_t dft_stride, int 
 }
 
 // gis_build_templates_batched with bank rows ldb >= n/2+1 complex values
-// apart (columns past n/2 are left unwritten): the padded row layout the
-// split search reads in aligned 128-byte segments.
+// apart (columns past n/2 zero): the padded row layout the searches read in
+// aligned 128-byte segments; ldb = 0: the planes in fragment order
+// (frag_pos, frag_plane(n) complex values each), read with 16-byte loads.
 GIS_API int gis_build_templates_batched_ld(const void* dft, size_t dft_stride, int C, int N, int n,
                                            const float* rts, int M, const float* mask, const float* fsc, void* bank,
                                            int ldb, float* norms, int normalize, void* ws, size_t ws_bytes,
@@ -1074,10 +1086,12 @@ GIS_API int gis_build_templates_batched_ld(const void* dft, size_t dft_stride, i
   GIS_REQUIRE((size_t)C * (size_t)M < (1u << 31), "gis_build_templates_batched_ld: C * M too large");
   GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0, "gis_build_templates_batched_ld: newbox %d must be even and <= box %d",
               n, N);
-  GIS_REQUIRE(ldb >= n / 2 + 1, "gis_build_templates_batched_ld: row stride %d below n/2+1 = %d", ldb, n / 2 + 1);
+  GIS_REQUIRE(ldb == 0 || ldb >= n / 2 + 1, "gis_build_templates_batched_ld: row stride %d below n/2+1 = %d", ldb,
+              n / 2 + 1);
   if (M == 0 || C == 0) return GIS_OK;
   CoefOut co{nullptr, 0, 0, nullptr};
   co.ldb = ldb;
+  co.frag = ldb == 0;
   return build_templates_dispatch(dft, VolSet{C, dft_stride}, N, 1, n, rts, M, mask, fsc, bank, norms, normalize, ws,
                                   ws_bytes, stream, co, kBankIO);
 }

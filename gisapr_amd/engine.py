@@ -38,7 +38,7 @@ class ParticleSet:
     coeffs: Optional[torch.Tensor] = None    # [B*P, Kpad] complex64, unit norm (small-shift search)
     row_order: Optional[torch.Tensor] = None # [sum M_i] int32 processing order of the small-shift search
     particle_order: Optional[torch.Tensor] = None  # [B] int32 processing order of the split search
-    spectra_split: Optional[torch.Tensor] = None   # [B, P, n, ld] the spectra in zero-padded rows (split search)
+    spectra_frag: Optional[kernels.FragmentPlanes] = None  # [B, P] the spectra in fragment order (the searches')
 
     @property
     def B(self) -> int:
@@ -54,8 +54,14 @@ class ParticleSet:
 
     def nbytes_resident(self) -> int:
         t = [self.images, self.rotp, self.dir_off, self.dir_idx, self.spectra, self.inv_norm, self.defocus,
-             self.coeffs]
+             self.coeffs, None if self.spectra_frag is None else self.spectra_frag.data]
         return sum(x.numel() * x.element_size() for x in t if x is not None)
+
+
+def search_bank(bank) -> kernels.FragmentPlanes:
+    """A template bank in the searches' operand layout: FragmentPlanes as
+    they are, a row-major [M, n, ld] bank packed (one permuting copy)."""
+    return bank if isinstance(bank, kernels.FragmentPlanes) else kernels.pack_fragments(bank.contiguous())
 
 
 def particle_order(dir_off: np.ndarray, dir_idx: np.ndarray, device) -> torch.Tensor:
@@ -166,11 +172,12 @@ class SearchEngine:
         return ps
 
     def prepare(self, ps: ParticleSet) -> ParticleSet:
-        ps.spectra_split = None  # a padded copy of older spectra is stale
         ctf_tab = self.ctf_tab if ps.defocus is not None else None
         ps.spectra, ps.inv_norm = kernels.prepare_particles(
             ps.images, self.n, ps.rotp, self.mask, ctf_tab, ps.defocus, self.ctf.kconst,
             out=ps.spectra, inv_norm=ps.inv_norm)
+        if ps.spectra_frag is not None:  # the search copy follows, in place
+            kernels.pack_fragments(ps.spectra, out=ps.spectra_frag.data)
         return ps
 
     # ---------------------------------------------------------------- model
@@ -181,16 +188,16 @@ class SearchEngine:
         vol = base.clone() if base is not None else None
         return kernels.splat(xyz, zn, self.N, width, self.apix, rp, kn, vol)
 
-    def templates(self, volume: torch.Tensor, with_norms=False, normalize=True, padded=False):
+    def templates(self, volume: torch.Tensor, with_norms=False, normalize=True, fragments=False):
         """Unit-norm bank (and norms); with ``normalize=False`` returns
         (bank, 1/norms) with unnormalised rows, for ``search(..., tinv=)``.
-        ``padded``: rows zero padded to ``kernels.padded_row_stride(n)``
-        (the split search's aligned layout)."""
+        ``fragments``: the bank as FragmentPlanes, the searches' operand
+        layout, written by the template kernel directly."""
         dft = kernels.volume_dft(volume)
         out = None
-        if padded:
-            out = torch.empty((len(self.dir_rot), self.n, kernels.padded_row_stride(self.n)), dtype=torch.complex64,
-                              device=self.device)
+        if fragments:
+            out = kernels.FragmentPlanes(torch.empty((len(self.dir_rot), kernels.fragment_plane(self.n)),
+                                                     dtype=torch.complex64, device=self.device), self.n)
         out = kernels.build_templates(dft, self.n, self.rts, self.mask, self.fsc_packed, with_norms, out=out,
                                       normalize=normalize)
         if normalize:
@@ -207,19 +214,18 @@ class SearchEngine:
         # dir_off / dir_idx were built on the host against this direction table
         if ps.particle_order is None:
             ps.particle_order = particle_order(ps.dir_off.cpu().numpy(), ps.dir_idx.cpu().numpy(), self.device)
-        corr = kernels.correlate(bank, self.split_spectra(ps), ps.inv_norm, ps.dir_off, ps.dir_idx, self.crop,
-                                 validate=False, tinv=tinv, order=ps.particle_order)
+        corr = kernels.correlate(search_bank(bank), self.search_spectra(ps), ps.inv_norm, ps.dir_off, ps.dir_idx,
+                                 self.crop, validate=False, tinv=tinv, order=ps.particle_order)
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.crop // 2)
 
-    def split_spectra(self, ps: ParticleSet) -> torch.Tensor:
-        """The particle spectra in zero-padded rows (made once per set): the
-        split search reads them in aligned 128-byte segments."""
+    def search_spectra(self, ps: ParticleSet) -> kernels.FragmentPlanes:
+        """The particle spectra in fragment order, the searches' operand
+        layout (made once per set, kept in step by ``prepare``)."""
         if ps.spectra is None:
             self.prepare(ps)
-        ld = kernels.padded_row_stride(self.n)
-        if ps.spectra_split is None or int(ps.spectra_split.shape[-1]) != ld:
-            ps.spectra_split = kernels.pad_rows(ps.spectra, ld)
-        return ps.spectra_split
+        if ps.spectra_frag is None:
+            ps.spectra_frag = kernels.pack_fragments(ps.spectra)
+        return ps.spectra_frag
 
     def search_split(self, bank: torch.Tensor, ps: ParticleSet, tinv: Optional[torch.Tensor] = None,
                      dt: int = 0, pt: int = 0) -> SearchResult:
@@ -231,8 +237,9 @@ class SearchEngine:
             raise ValueError("template bank does not cover the engine's direction table")
         if ps.particle_order is None:
             ps.particle_order = particle_order(ps.dir_off.cpu().numpy(), ps.dir_idx.cpu().numpy(), self.device)
-        corr = kernels.correlate_split(bank, self.split_spectra(ps), ps.inv_norm, ps.dir_off, ps.dir_idx, self.crop,
-                                       validate=False, tinv=tinv, order=ps.particle_order, dt=dt, pt=pt)
+        corr = kernels.correlate_split(search_bank(bank), self.search_spectra(ps), ps.inv_norm, ps.dir_off,
+                                       ps.dir_idx, self.crop, validate=False, tinv=tinv, order=ps.particle_order,
+                                       dt=dt, pt=pt)
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.crop // 2)
 
     # ---------------------------------------------------------------- small-shift precision search

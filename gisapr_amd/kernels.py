@@ -253,12 +253,70 @@ def pad_rows(x: torch.Tensor, ld: int) -> torch.Tensor:
     return out
 
 
+@dataclass
+class FragmentPlanes:
+    """Packed planes (n rows, n/2+1 columns) in fragment order, the searches'
+    operand layout: each 16 x 16 coefficient tile one contiguous 2 KiB run
+    that a wave reads with 16-byte loads.  ``data`` [..., fragment_plane(n)]
+    complex64; the leading dimensions are those of the row-major planes."""
+    data: torch.Tensor
+    n: int
+
+    @property
+    def shape(self):
+        return tuple(self.data.shape[:-1])
+
+    def __getitem__(self, idx) -> "FragmentPlanes":
+        """Planes selected along the leading dimensions (a view)."""
+        return FragmentPlanes(self.data[idx], self.n)
+
+    def unpack(self) -> torch.Tensor:
+        """The planes row major, [..., n, n/2+1] (a gather; for checks)."""
+        pos = torch.from_numpy(fragment_positions(self.n)).to(self.data.device)
+        return self.data[..., pos.reshape(-1)].reshape(self.shape + tuple(pos.shape))
+
+
+def fragment_positions(n: int) -> np.ndarray:
+    """[n, n/2+1] position of each packed coefficient in fragment order."""
+    m = n // 2 + 1
+    nks = ((n + 15) // 16 + 1) & ~1
+    lone = m % 16 == 1
+    nct = m // 16 if lone else (m + 15) // 16
+    r, c = np.meshgrid(np.arange(n), np.arange(m), indexing="ij")
+    pos = ((r >> 4) * nct + (c >> 4)) * 256 + ((((r >> 1) & 1) * 4 + ((r >> 2) & 3)) * 16 + (c & 15)) * 2 + (r & 1)
+    if lone:
+        pos[:, m - 1] = nks * nct * 256 + np.arange(n)
+    return pos.astype(np.int64)
+
+
+def fragment_plane(n: int) -> int:
+    """Complex values of one plane in fragment order at newbox n."""
+    return int(lib().gis_fragment_plane(int(n)))
+
+
+def pack_fragments(x: torch.Tensor, out: Optional[torch.Tensor] = None) -> FragmentPlanes:
+    """``x`` [..., n, ld] (ld >= n/2+1, contiguous) in fragment order."""
+    require_hip(x, "planes")
+    n, ld = int(x.shape[-2]), int(x.shape[-1])
+    if x.dtype != torch.complex64 or not x.is_contiguous() or ld < n // 2 + 1:
+        raise ValueError("planes must be contiguous complex64 [..., n, ld >= n/2+1]")
+    lead = tuple(x.shape[:-2])
+    rows = int(np.prod(lead)) if lead else 1
+    F = fragment_plane(n)
+    if out is None:
+        out = torch.empty(lead + (F,), dtype=torch.complex64, device=x.device)
+    elif tuple(out.shape) != lead + (F,) or out.dtype != torch.complex64 or not out.is_contiguous():
+        raise ValueError(f"out must be contiguous complex64 {lead + (F,)}")
+    check(lib().gis_pack_fragments(ptr(x), rows, n, ld, ptr(out), stream_of(x.device)), "gis_pack_fragments")
+    return FragmentPlanes(out, n)
+
+
 def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Optional[torch.Tensor],
                     fsc_packed: Optional[torch.Tensor], with_norms: bool = False,
                     out: Optional[torch.Tensor] = None, normalize: bool = True,
                     workspace: Optional[torch.Tensor] = None):
     """Packed template bank [M, n, n/2+1] complex64 (unshifted), written into
-    ``out`` when given.  Unit-norm rows (the reference's bank) unless
+    ``out`` when given ([M, n, ld] padded rows, or FragmentPlanes).  Unit-norm rows (the reference's bank) unless
     ``normalize`` is False: then the rows keep their scale and the norms are
     returned (callers fold 1/norm into the search).  ``dft`` [C, E] (C
     spectra) gives the C banks in one launch, rows c * M + d."""
@@ -270,13 +328,20 @@ def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Op
     rows = C * M
     if out is None:
         bank = torch.empty((rows, n, n // 2 + 1), dtype=torch.complex64, device=dft.device)
+        ldb = n // 2 + 1
+    elif isinstance(out, FragmentPlanes):  # fragment order (ldb = 0)
+        require_hip(out.data, "out")
+        if out.n != n or tuple(out.data.shape) != (rows, fragment_plane(n)) or \
+                out.data.dtype != torch.complex64 or not out.data.is_contiguous():
+            raise ValueError("out must be contiguous complex64 fragment planes [C * M, fragment_plane(n)]")
+        bank, ldb = out.data, 0
     else:
         require_hip(out, "out")
         if tuple(out.shape[:2]) != (rows, n) or out.ndim != 3 or int(out.shape[2]) < n // 2 + 1 \
                 or out.dtype != torch.complex64 or not out.is_contiguous():
             raise ValueError("out must be contiguous complex64 [C * M, n, ld], ld >= n/2+1")
         bank = out
-    ldb = int(bank.shape[2])
+        ldb = int(bank.shape[2])
     norms = torch.empty((rows,), dtype=torch.float32, device=dft.device) if with_norms else None
     if mask is not None:
         require_hip(mask, "mask")
@@ -287,7 +352,7 @@ def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Op
         if tuple(fsc_packed.shape) != (n, n // 2 + 1):
             raise ValueError("fsc_packed must be [n, n/2+1]")
     ws, wsb = _template_workspace(N, rows, dft.device, workspace)
-    if ldb != n // 2 + 1:  # padded rows (zeros past column n/2)
+    if ldb != n // 2 + 1:  # padded rows (zeros past column n/2) or fragment order
         stride = int(dft.shape[1]) if dft.ndim > 1 else int(dft.numel())
         check(lib().gis_build_templates_batched_ld(ptr(dft), stride, C, N, n, ptr(rts), M, ptr(mask),
                                                    ptr(fsc_packed), ptr(bank), ldb, ptr(norms),
@@ -301,6 +366,8 @@ def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Op
         check(lib().gis_build_templates_batched(ptr(dft), int(dft.shape[1]), C, N, n, ptr(rts), M, ptr(mask),
                                                 ptr(fsc_packed), ptr(bank), ptr(norms), int(bool(normalize)),
                                                 ptr(ws), wsb, stream_of(dft.device)), "gis_build_templates_batched")
+    if ldb == 0:
+        bank = out
     return (bank, norms) if with_norms else bank
 
 
@@ -421,6 +488,29 @@ def _check_pmap(pmap, B, rows):
         raise ValueError("pmap out of range of the particle rows")
 
 
+def _search_operands(bank, spectra):
+    """(bank, spectra, n, ldt, ldp) of the searches: row-major tensors
+    [M, n, ld] / [B, P, n, ld] (ld >= n/2+1) or FragmentPlanes (ld = 0)."""
+    def one(x, what, lead):
+        if isinstance(x, FragmentPlanes):
+            require_hip(x.data, what)
+            if x.data.dim() != lead + 1 or int(x.data.shape[-1]) != fragment_plane(x.n) or \
+                    not x.data.is_contiguous() or x.data.dtype != torch.complex64:
+                raise ValueError(f"{what}: fragment planes must be contiguous complex64 [..., fragment_plane(n)]")
+            return x.data, x.n, 0
+        require_hip(x, what)
+        if x.dim() != lead + 2 or not x.is_contiguous() or int(x.shape[-1]) < int(x.shape[-2]) // 2 + 1:
+            raise ValueError("bank [M, n, ldt] and spectra [B, P, n, ldp] must be contiguous, ld >= n/2+1")
+        return x, int(x.shape[-2]), int(x.shape[-1])
+    bank, nt, ldt = one(bank, "bank", 1)
+    spectra, n, ldp = one(spectra, "spectra", 2)
+    if nt != n:
+        raise ValueError(f"bank newbox {nt} != spectra newbox {n}")
+    if (ldt == 0) != (ldp == 0):
+        raise ValueError("bank and spectra row strides: both 0 (fragment order) or both row major")
+    return bank, spectra, n, ldt, ldp
+
+
 def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torch.Tensor],
               dir_off: torch.Tensor, dir_idx: torch.Tensor, crop: int, validate: bool = True,
               pmap: Optional[torch.Tensor] = None, tinv: Optional[torch.Tensor] = None,
@@ -428,15 +518,12 @@ def correlate(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optional[torc
     """FP32 v606 search.  With ``pmap`` the searched particles are
     B = len(dir_off) - 1 entries each mapped to a row of ``spectra``;
     ``tinv`` gives 1/norm per bank row for an unnormalised bank; ``bank`` and
-    ``spectra`` may carry zero-padded rows (``padded_row_stride``);
+    ``spectra`` may carry zero-padded rows (``padded_row_stride``) or be
+    FragmentPlanes (``pack_fragments``);
     ``order`` / ``dt`` / ``pt`` as in ``correlate_split`` (dt 1 or 2)."""
-    require_hip(bank, "bank")
-    require_hip(spectra, "spectra")
-    P, n = int(spectra.shape[1]), int(spectra.shape[2])
+    bank, spectra, n, ldt, ldp = _search_operands(bank, spectra)
+    P = int(spectra.shape[1])
     B = int(dir_off.shape[0]) - 1 if pmap is not None else int(spectra.shape[0])
-    ldt, ldp = int(bank.shape[-1]), int(spectra.shape[-1])
-    if int(bank.shape[1]) != n or min(ldt, ldp) < n // 2 + 1 or not (bank.is_contiguous() and spectra.is_contiguous()):
-        raise ValueError("bank [M, n, ldt] and spectra [B, P, n, ldp] must be contiguous, ld >= n/2+1")
     if dir_off.dtype != torch.int32 or dir_idx.dtype != torch.int32:
         raise ValueError("dir_off / dir_idx must be int32")
     if tuple(dir_off.shape) != (B + 1,):
@@ -484,18 +571,15 @@ def correlate_split(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optiona
     """The v606 search of ``correlate`` (every packed coefficient, every lag of
     the crop, crop <= 32) on split-bf16 MFMA.  ``bank`` [M, n, ldt] and
     ``spectra`` [B, P, n, ldp] may carry zero-padded rows (ld > n/2+1;
-    ``padded_row_stride(n)`` aligns them to 128-byte lines).  ``order`` is the particles'
-    processing order (a permutation of the B searched particles, e.g. by
+    ``padded_row_stride(n)`` aligns them to 128-byte lines) or be
+    FragmentPlanes (``pack_fragments``: 16-byte loads).  ``order`` is the
+    particles' processing order (a permutation of the B searched particles, e.g. by
     orientation so neighbouring particles share template rows in cache);
     ``dt`` / ``pt`` are the directions / psi values per wave (0 = automatic;
     results do not depend on them)."""
-    require_hip(bank, "bank")
-    require_hip(spectra, "spectra")
-    P, n = int(spectra.shape[1]), int(spectra.shape[2])
+    bank, spectra, n, ldt, ldp = _search_operands(bank, spectra)
+    P = int(spectra.shape[1])
     B = int(dir_off.shape[0]) - 1 if pmap is not None else int(spectra.shape[0])
-    ldt, ldp = int(bank.shape[-1]), int(spectra.shape[-1])
-    if int(bank.shape[1]) != n or min(ldt, ldp) < n // 2 + 1 or not (bank.is_contiguous() and spectra.is_contiguous()):
-        raise ValueError("bank [M, n, ldt] and spectra [B, P, n, ldp] must be contiguous, ld >= n/2+1")
     if dir_off.dtype != torch.int32 or dir_idx.dtype != torch.int32:
         raise ValueError("dir_off / dir_idx must be int32")
     if tuple(dir_off.shape) != (B + 1,):

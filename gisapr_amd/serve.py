@@ -163,10 +163,6 @@ class Dataset:
     resident_bytes: int = 0
 
 
-def _tensor_bytes(*ts) -> int:
-    return sum(int(t.numel()) * t.element_size() for t in ts if t is not None)
-
-
 class SearchServer:
     """Continuous-batching evaluation of candidate requests on one GPU.
 

@@ -134,7 +134,9 @@ int gis_build_templates_batched(const void* dft, size_t dft_stride, int C, int N
 /*
  * gis_build_templates_batched with bank rows `ldb` >= n/2+1 complex values
  * apart ([C*M][n][ldb]); columns past n/2 are written as zeros.  For the
- * split search's aligned operand loads (ldb a multiple of 16).
+ * searches' aligned operand loads (ldb a multiple of 16).  ldb = 0: the
+ * planes in fragment order ([C*M][gis_fragment_plane(n)], see
+ * gis_pack_fragments), the searches' 16-byte-load layout.
  */
 int gis_build_templates_batched_ld(const void* dft, size_t dft_stride, int C, int N, int n, const float* rts, int M,
                                    const float* mask, const float* fsc, void* bank, int ldb, float* norms,
@@ -255,7 +257,9 @@ size_t gis_correlate_workspace(int n, int C, int B);
  * product).  Particle b is correlated against bank rows
  * dir_idx[dir_off[b] .. dir_off[b+1]).
  *   bank [rows][n][ldt], spectra [particles][P][n][ldp] complex64, rows ld
- *        >= n/2+1 complex values apart (columns past n/2 zero when padded)
+ *        >= n/2+1 complex values apart (columns past n/2 zero when padded);
+ *        ld = 0: the planes are in fragment order (gis_pack_fragments),
+ *        gis_fragment_plane(n) complex values each
  *   inv_norm [particles][P] 1/norm of each spectrum or NULL; tinv [bank rows]
  *        1/norm of each template row or NULL (bank already unit norm)
  *   pmap [B] spectra row of searched particle b, or NULL (identity): one
@@ -277,6 +281,22 @@ int gis_correlate(const void* bank, const void* spectra, const float* inv_norm, 
                   int C, int nrows, int ldt, int ldp, int dt, int pt, float* scores, int* peak_y, int* peak_x,
                   int* best, float* peak, double* mean, double* stdv, void* ws, size_t ws_bytes, void* stream);
 
+/*
+ * Fragment order: the searches' operand layout, in which each step's 16 x 16
+ * coefficient tile is two contiguous 1 KiB runs read with 16-byte loads.
+ * A plane of n rows and m = n/2+1 columns is cut into tiles of 16 rows
+ * (row steps padded to an even count) by 16 columns, tile by tile; inside a
+ * tile, the value at row 16 ks + 4g + 2h + e, column 16 ct + i is at
+ * ((h * 4 + g) * 16 + i) * 2 + e.  When m % 16 == 1 the last column is kept
+ * apart after the tiles, one value per row.  Rows >= n and columns >= m are
+ * zero.  No counterpart in the reference (its planes are row major).
+ * gis_fragment_plane: complex values per plane (0 for an odd or tiny n).
+ * gis_pack_fragments: dst [rows][gis_fragment_plane(n)] complex64 from src
+ * [rows][n][ld] (ld >= n/2+1) on the stream.
+ */
+size_t gis_fragment_plane(int n);
+int gis_pack_fragments(const void* src, int rows, int n, int ld, void* dst, void* stream);
+
 /* Workspace bytes gis_correlate_split needs at newbox n, crop C, B particles. */
 size_t gis_correlate_split_workspace(int n, int C, int B);
 
@@ -294,7 +314,8 @@ size_t gis_correlate_split_workspace(int n, int C, int B);
  *          [n][n/2+1] layout, or a padded stride: columns past n/2 zero);
  *          a stride that is a multiple of 16 starts every 16-column segment
  *          on a 128-byte line (see gis_build_templates_batched_ld)
- *   ldp    the same for the rows of a psi spectrum
+ *   ldp    the same for the rows of a psi spectrum; ldt / ldp = 0: the
+ *          bank's / spectra's planes are in fragment order (gis_pack_fragments)
  *   dt     local directions per wave (1, 2 or 4; 0 = automatic; crops above
  *          16 use 1)
  *   pt     psi values per wave (1..5; 0 = automatic).  dt and

@@ -145,6 +145,67 @@ def test_split_tiles_are_bit_identical(hip_device):
     assert torch.equal(got.scores, ref.scores) and torch.equal(got.best, ref.best)
 
 
+def fragment_index(n):
+    """Position in fragment order of each (row, column) of a packed plane, from
+    the layout's definition: tile (ks, ct) of 16 rows x 16 columns at
+    (ks * nct + ct) * 256, row 16 ks + 4g + 2h + e / column 16 ct + i inside
+    it at ((h * 4 + g) * 16 + i) * 2 + e; a lone last column (m % 16 == 1)
+    after the tiles, one value per row."""
+    m = n // 2 + 1
+    nks = ((n + 15) // 16 + 1) & ~1
+    lone = m % 16 == 1
+    nct = m // 16 if lone else (m + 15) // 16
+    r, c = np.meshgrid(np.arange(n), np.arange(m), indexing="ij")
+    ks, rr = np.divmod(r, 16)
+    ct, i = np.divmod(c, 16)
+    g, j = np.divmod(rr, 4)
+    h, e = np.divmod(j, 2)
+    idx = (ks * nct + ct) * 256 + ((h * 4 + g) * 16 + i) * 2 + e
+    if lone:
+        idx[:, m - 1] = nks * nct * 256 + np.arange(n)
+    return idx, nks * nct * 256 + (nks * 16 if lone else 0)
+
+
+@pytest.mark.parametrize("n", [64, 62, 40, 256])
+def test_fragment_order_is_bit_identical(hip_device, n):
+    """gis_pack_fragments places every coefficient where the layout's
+    definition puts it (zeros elsewhere), and both searches give the same
+    results bit for bit on fragment-order operands as on row-major ones."""
+    import torch
+    from gisapr_amd import kernels
+    rng = np.random.default_rng(n)
+    C, M, B, P = min(16, n), 9, 6, 5
+    dev = hip_device
+    bank = torch.from_numpy(rand_spec(rng, n, M).astype(np.complex64)).to(dev)
+    spec = torch.from_numpy(rand_spec(rng, n, B, P).astype(np.complex64)).to(dev)
+    idx, F = fragment_index(n)
+    assert kernels.fragment_plane(n) == F
+    say("pack fragments", n)
+    fb = kernels.pack_fragments(kernels.pad_rows(bank, kernels.padded_row_stride(n)))
+    fs = kernels.pack_fragments(spec)
+    want = np.zeros((M, F), np.complex64)
+    want[:, idx.reshape(-1)] = bank.cpu().numpy().reshape(M, -1)
+    np.testing.assert_array_equal(fb.data.cpu().numpy(), want)
+    want = np.zeros((B * P, F), np.complex64)
+    want[:, idx.reshape(-1)] = spec.cpu().numpy().reshape(B * P, -1)
+    np.testing.assert_array_equal(fs.data.cpu().numpy().reshape(B * P, F), want)
+    local = [[0, 1, 2, 3, 4], [8], [], [2, 5, 7], [0, 1, 2, 3, 4, 5, 6, 7, 8], [6, 7]]
+    off = np.zeros(B + 1, np.int32)
+    off[1:] = np.cumsum([len(l) for l in local])
+    d_off = torch.from_numpy(off).to(dev)
+    d_idx = torch.from_numpy(np.asarray([k for l in local for k in l], np.int32)).to(dev)
+    fields = ("scores", "peak_y", "peak_x", "best", "peak", "mean", "std")
+    for fn, dts in ((kernels.correlate_split, (1, 2, 4)), (kernels.correlate, (1, 2))):
+        for dt in dts:
+            ref = fn(bank, spec, None, d_off, d_idx, C, dt=dt, pt=5)
+            say("fragment search", n, fn.__name__, dt)
+            got = fn(fb, fs, None, d_off, d_idx, C, dt=dt, pt=5)
+            for f in fields:
+                assert torch.equal(getattr(got, f), getattr(ref, f)), (fn.__name__, dt, f)
+    with pytest.raises(ValueError, match="both 0"):  # one operand in each layout
+        kernels.correlate_split(fb, spec, None, d_off, d_idx, C)
+
+
 def test_split_agrees_with_fp32_search(hip_device):
     """Same operands through the FP32 search: identical peaks and best pairs
     (both evaluate the same functional; scores within 1e-4)."""
@@ -207,10 +268,11 @@ def test_split_end_to_end_matches_oracle(hip_device, name):
     assert abs(float(res.score.cpu()) - s_want) <= 1e-4 * s_want
 
 
-@pytest.mark.parametrize("N,n", [(64, 64), (96, 64), (256, 256)])
+@pytest.mark.parametrize("N,n", [(64, 64), (96, 64), (64, 40), (96, 62), (256, 256)])
 def test_padded_template_bank_equals_packed(hip_device, N, n):
-    """The template kernel's padded-row output (gis_build_templates_batched_ld)
-    holds the packed bank bit for bit, zeros past column n/2."""
+    """The template kernel's padded-row and fragment-order outputs
+    (gis_build_templates_batched_ld, ldb > n/2+1 and ldb = 0) hold the packed
+    bank bit for bit, with zeros in every pad."""
     import torch
     from gisapr_amd import host, kernels
     rng = np.random.default_rng(N + n)
@@ -230,4 +292,15 @@ def test_padded_template_bank_equals_packed(hip_device, N, n):
     assert bool((got[..., n // 2 + 1:] == 0).all())
     unit = kernels.build_templates(dft, n, rts, mask, None,
                                    out=torch.empty((12, n, ld), dtype=torch.complex64, device=hip_device))
-    assert torch.equal(unit[..., :n // 2 + 1], kernels.build_templates(dft, n, rts, mask, None))
+    unit_ref = kernels.build_templates(dft, n, rts, mask, None)
+    assert torch.equal(unit[..., :n // 2 + 1], unit_ref)
+    say("fragment-order bank", N, n)
+    F = kernels.fragment_plane(n)
+    frag = kernels.FragmentPlanes(torch.full((12, F), float("nan"), dtype=torch.complex64, device=hip_device), n)
+    fg, fnrm = kernels.build_templates(dft, n, rts, mask, None, out=frag, normalize=False)
+    assert torch.equal(fg.data, kernels.pack_fragments(ref).data) and torch.equal(fnrm, rn)  # pads zero
+    assert torch.equal(fg.unpack(), ref)
+    fu = kernels.build_templates(dft, n, rts, mask, None,
+                                 out=kernels.FragmentPlanes(torch.empty((12, F), dtype=torch.complex64,
+                                                                        device=hip_device), n))
+    assert torch.equal(fu.unpack(), unit_ref)
