@@ -297,7 +297,7 @@ def precision_check_split(eng, ps, bank, res, subset: int, seed: int):
         dd = torch.from_numpy(d.astype(np.int64)).to(dev)
         rows = raw[dd].unpack() if isinstance(raw, kernels.FragmentPlanes) else raw[dd][..., :eng.n // 2 + 1]
         t = rows.to(torch.complex128) * tinv[dd].double()[:, None, None]
-        p = ps.spectra[b].to(torch.complex128) * ps.inv_norm[b].double()[:, None, None]
+        p = ps.spectra_frag[b].unpack().to(torch.complex128) * ps.inv_norm[b].double()[:, None, None]
         q = t[:, None] * p[None]                                    # [Mi, P, n, m]
         crop = 1024.0 * (Wt @ q @ Et).real.reshape(len(d) * P, C * C)
         v, k = crop.max(dim=1)

@@ -48,6 +48,7 @@ SIGNATURES = {
     "gis_prepare_particles_workspace": (_sz, [_i, _i]),
     "gis_prepare_particles_workspace_ex": (_sz, [_i, _i, _i]),
     "gis_prepare_particles": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "gis_prepare_particles_ld": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _sz, _vp]),
     "gis_ctf_packed": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "gis_correlate_workspace": (_sz, [_i, _i, _i]),
     "gis_correlate": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp, _vp,

@@ -308,7 +308,7 @@ class CandidateBatch:
                 engine.enable_small_shift()
             if ps.coeffs is None:
                 engine.prepare_small_shift(ps)
-        elif ps.spectra is None:
+        elif ps.spectra_frag is None:
             engine.prepare(ps)
         self.plan = build_slot_plan(ps, M, self.slots, subsets, dev)
         # resident buffers
@@ -598,7 +598,7 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
                                        pt=tile.get("pt", 0), wpb=tile.get("wpb", 0), plast=engine.small_plast,
                                        operand=engine.small.operand)
     else:
-        if ps.spectra is None:
+        if ps.spectra_frag is None:
             engine.prepare(ps)
         n = engine.n
         bank = kernels.FragmentPlanes(torch.empty((C * M, kernels.fragment_plane(n)), dtype=torch.complex64,

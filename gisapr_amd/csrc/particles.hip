@@ -247,7 +247,8 @@ template <int N>
 __global__ void __launch_bounds__(kThreads)
 prep_cols_kernel(int P, int n, int b0, const float* __restrict__ ctf_tab, const float* __restrict__ defocus,
                  float k1, float k2, float k3, float k5, float kk, const float2* __restrict__ twg,
-                 const float2* __restrict__ X, float2* __restrict__ spectra, double* __restrict__ partial) {
+                 const float2* __restrict__ X, float2* __restrict__ spectra, int ldo,
+                 double* __restrict__ partial) {
   using T = PrepLayout<N>;
   constexpr int MH = T::MH, SL = T::SL;
   __shared__ float2 tw[N];
@@ -261,6 +262,7 @@ prep_cols_kernel(int P, int n, int b0, const float* __restrict__ ctf_tab, const 
   const int k0 = tile * kTK, kmax = n / 2;
   const int mh_n = n / 2 + 1;
   const size_t plane_n = (size_t)n * mh_n;
+  const int n16 = (n + 15) & ~15, fnct = frag_nct(n);
   for (int k = tid; k < N; k += kThreads) tw[k] = twg[k];
   const bool has_ctf = defocus != nullptr;
   const bool explicit_ctf = !has_ctf && ctf_tab != nullptr;
@@ -309,12 +311,17 @@ prep_cols_kernel(int P, int n, int b0, const float* __restrict__ ctf_tab, const 
     static_assert(kTK == 2 * (kThreads / 64), "two sequences per wave");
     fft_wave_inplace2<N, false, kPad>(seq + (2 * wave) * SL, seq + (2 * wave + 1) * SL, tw, lane);
     __syncthreads();
-    float2* out = spectra + ((size_t)b * P + p) * plane_n;
+    // output: rows ldo apart, or (ldo = 0) fragment order
+    float2* out = spectra + ((size_t)b * P + p) * (ldo ? (size_t)n * ldo : (size_t)frag_plane(n));
     double acc = 0.0;
-    for (int t = tid; t < n * kTK; t += kThreads) {
-      const int r = t / kTK, q = t - r * kTK;
+    // element t of the tile is (row 16 ks + 4g + 2h + e, column k0 + i) for
+    // t = (((ks * 2 + h) * 4 + g) * 8 + i) * 2 + e: sixteen consecutive t
+    // are one 128-byte run of a fragment-order tile
+    for (int t = tid; t < n16 * kTK; t += kThreads) {
+      const int q = (t >> 1) & 7, rest = t >> 4;
+      const int r = 16 * (rest >> 3) + 4 * (rest & 3) + 2 * ((rest >> 2) & 1) + (t & 1);
       const int c = k0 + q;
-      if (c > kmax) continue;
+      if (c > kmax || r >= n) continue;
       const int fy = r < n / 2 ? r : r - n;
       float2 v;
       if (c < n / 2 || n == N) {
@@ -323,10 +330,10 @@ prep_cols_kernel(int P, int n, int b0, const float* __restrict__ ctf_tab, const 
         const int my = -fy;
         v = cconj(seq[q * SL + lds_pad<kPad>(my < 0 ? my + N : my)]);
       }
-      const float w = ctf_l[t];
+      const float w = ctf_l[r * kTK + q];
       v.x *= w;
       v.y = -(v.y * w);  // conj
-      out[(size_t)r * mh_n + c] = v;
+      out[ldo ? (size_t)r * ldo + c : (size_t)frag_pos(n, fnct, r, c)] = v;
       acc += (double)v.x * v.x + (double)v.y * v.y;
     }
     acc = wave_sum_f64(acc);
@@ -337,6 +344,31 @@ prep_cols_kernel(int P, int n, int b0, const float* __restrict__ ctf_tab, const 
       for (int w = 0; w < kThreads / 64; ++w) s += red[w];
       partial[((size_t)lb * P + p) * T::NT + tile] = s;
     }
+  }
+}
+
+// Zeros at the pads of fragment-order planes (rows n .. 16 nks, and columns
+// n/2+1 .. 16 nct when the last column is not kept apart): the positions no
+// preparation block writes.
+__global__ void frag_zero_pads_kernel(float2* __restrict__ dst, size_t rows, int n) {
+  const int m = n / 2 + 1, nct = frag_nct(n), nks = frag_nks(n);
+  const bool lone = (m & 15) == 1;
+  const int rpad = 16 * nks - n, wcol = 16 * nct + (lone ? 1 : 0);
+  const int padA = rpad * wcol, padB = lone ? 0 : n * (16 * nct - m);
+  const size_t per = (size_t)(padA + padB), total = rows * per;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t row = t / per;
+    const int e = (int)(t - row * per);
+    int r, c;
+    if (e < padA) {
+      r = n + e / wcol;
+      c = e % wcol;
+      if (lone && c == 16 * nct) c = m - 1;
+    } else {
+      r = (e - padA) / (16 * nct - m);
+      c = m + (e - padA) % (16 * nct - m);
+    }
+    dst[row * (size_t)frag_plane(n) + frag_pos(n, nct, r, c)] = make_float2(0.f, 0.f);
   }
 }
 
@@ -361,7 +393,7 @@ constexpr size_t kTwiddleBytes = 256 * sizeof(float2);  // table at the workspac
 
 template <int N>
 int launch_prepare(const float* images, int B, int P, int n, const float* rotp, const float* mask,
-                   const float* ctf_tab, const float* defocus, const float* k, float2* spectra,
+                   const float* ctf_tab, const float* defocus, const float* k, float2* spectra, int ldo,
                    float* inv_norm, void* ws, size_t ws_bytes, hipStream_t st) {
   using T = PrepLayout<N>;
   const size_t cap = ws_bytes > kTwiddleBytes ? (ws_bytes - kTwiddleBytes) / prep_image_bytes<N>() : 0;
@@ -374,6 +406,11 @@ int launch_prepare(const float* images, int B, int P, int n, const float* rotp, 
   const int tiles = n / 2 / kTK + 1;
   hipLaunchKernelGGL(prep_twiddle_kernel, dim3(1), dim3(256), 0, st, twg, N);
   GIS_LAUNCH_CHECK("gis_prepare_particles twiddles");
+  if (ldo == 0 && (size_t)frag_plane(n) > (size_t)n * (n / 2 + 1)) {
+    hipLaunchKernelGGL(frag_zero_pads_kernel, dim3(std::max(1, std::min(B * P, 4096))), dim3(256), 0, st, spectra,
+                       (size_t)B * P, n);
+    GIS_LAUNCH_CHECK("gis_prepare_particles pads");
+  }
   for (int b0 = 0; b0 < B; b0 += chunk) {
     const int nb = std::min(chunk, B - b0);
     const int i0 = b0 * P, nimg = nb * P;
@@ -385,7 +422,7 @@ int launch_prepare(const float* images, int B, int P, int n, const float* rotp, 
                          images, P, n, i0, rotp, mask, twg, X);
     GIS_LAUNCH_CHECK("gis_prepare_particles rows");
     hipLaunchKernelGGL(prep_cols_kernel<N>, dim3((unsigned)(nb * tiles)), dim3(kThreads), 0, st, P, n, b0,
-                       ctf_tab, defocus, k[0], k[1], k[2], k[3], k[4], twg, X, spectra, partial);
+                       ctf_tab, defocus, k[0], k[1], k[2], k[3], k[4], twg, X, spectra, ldo, partial);
     GIS_LAUNCH_CHECK("gis_prepare_particles columns");
     hipLaunchKernelGGL(prep_norm_kernel, dim3((unsigned)((nimg + 255) / 256)), dim3(256), 0, st, partial, nimg,
                        T::NT, tiles, i0, inv_norm);
@@ -464,28 +501,43 @@ GIS_API size_t gis_prepare_particles_workspace(int N, int particles) {
 //            without: [B][n][n/2+1] explicit packed CTF, or NULL (CTF = 1)
 //   defocus  [B][3] float32 (dfU, dfV, angle deg) or NULL
 //   kconst   [5] float32 (host): k1, k2, k3, k5, kk
-//   spectra  [B][P][n][n/2+1] complex64 output (conjugated, unnormalised, unshifted)
+//   spectra  [B][P][n][ld] complex64 output (conjugated, unnormalised, unshifted;
+//            ld >= n/2+1, columns past n/2 untouched), or with ld = 0
+//            [B][P][frag_plane(n)] in fragment order (the searches' operand
+//            layout, pads zero); gis_prepare_particles: ld = n/2+1
 //   inv_norm [B][P] float32 output
-GIS_API int gis_prepare_particles(const float* images, int B, int N, int n, int P, const float* rotp,
-                                  const float* mask, const float* ctf_tab, const float* defocus,
-                                  const float* kconst, void* spectra, float* inv_norm, void* ws,
-                                  size_t ws_bytes, void* stream) {
+GIS_API int gis_prepare_particles_ld(const float* images, int B, int N, int n, int P, const float* rotp,
+                                     const float* mask, const float* ctf_tab, const float* defocus,
+                                     const float* kconst, void* spectra, int ld, float* inv_norm, void* ws,
+                                     size_t ws_bytes, void* stream) {
   GIS_REQUIRE(B >= 0 && P >= 1, "gis_prepare_particles: B=%d P=%d", B, P);
   GIS_REQUIRE(n >= 2 && n <= N && n % 2 == 0, "gis_prepare_particles: newbox %d must be even and <= box %d", n, N);
+  GIS_REQUIRE(ld == 0 || ld >= n / 2 + 1, "gis_prepare_particles: row stride %d below n/2+1 = %d", ld, n / 2 + 1);
   if (B == 0) return GIS_OK;
   GIS_REQUIRE(images && rotp && spectra && inv_norm && ws && kconst, "gis_prepare_particles: null pointer");
   GIS_REQUIRE(defocus == nullptr || ctf_tab != nullptr, "gis_prepare_particles: defocus without CTF tables");
   hipStream_t st = (hipStream_t)stream;
   float2* S = (float2*)spectra;
+#define GIS_PREP(NV) \
+  launch_prepare<NV>(images, B, P, n, rotp, mask, ctf_tab, defocus, kconst, S, ld, inv_norm, ws, ws_bytes, st)
   switch (N) {
-    case 32: return launch_prepare<32>(images, B, P, n, rotp, mask, ctf_tab, defocus, kconst, S, inv_norm, ws, ws_bytes, st);
-    case 64: return launch_prepare<64>(images, B, P, n, rotp, mask, ctf_tab, defocus, kconst, S, inv_norm, ws, ws_bytes, st);
-    case 96: return launch_prepare<96>(images, B, P, n, rotp, mask, ctf_tab, defocus, kconst, S, inv_norm, ws, ws_bytes, st);
-    case 128: return launch_prepare<128>(images, B, P, n, rotp, mask, ctf_tab, defocus, kconst, S, inv_norm, ws, ws_bytes, st);
-    case 192: return launch_prepare<192>(images, B, P, n, rotp, mask, ctf_tab, defocus, kconst, S, inv_norm, ws, ws_bytes, st);
-    case 256: return launch_prepare<256>(images, B, P, n, rotp, mask, ctf_tab, defocus, kconst, S, inv_norm, ws, ws_bytes, st);
+    case 32: return GIS_PREP(32);
+    case 64: return GIS_PREP(64);
+    case 96: return GIS_PREP(96);
+    case 128: return GIS_PREP(128);
+    case 192: return GIS_PREP(192);
+    case 256: return GIS_PREP(256);
     default:
       gis_set_error("gis_prepare_particles: box %d not in {32, 64, 96, 128, 192, 256}", N);
       return GIS_EUNSUPPORTED;
   }
+#undef GIS_PREP
+}
+
+GIS_API int gis_prepare_particles(const float* images, int B, int N, int n, int P, const float* rotp,
+                                  const float* mask, const float* ctf_tab, const float* defocus,
+                                  const float* kconst, void* spectra, float* inv_norm, void* ws,
+                                  size_t ws_bytes, void* stream) {
+  return gis_prepare_particles_ld(images, B, N, n, P, rotp, mask, ctf_tab, defocus, kconst, spectra, n / 2 + 1,
+                                  inv_norm, ws, ws_bytes, stream);
 }

@@ -33,12 +33,12 @@ class ParticleSet:
     defocus: Optional[torch.Tensor]    # [B, 3] or None
     dir_off: torch.Tensor              # [B+1] int32
     dir_idx: torch.Tensor              # [sum M_i] int32
-    spectra: Optional[torch.Tensor] = None   # [B, P, n, n/2+1] complex64
+    spectra: Optional[torch.Tensor] = None   # [B, P, n, n/2+1] complex64 row-major copy (on demand)
     inv_norm: Optional[torch.Tensor] = None  # [B, P] float32
     coeffs: Optional[torch.Tensor] = None    # [B*P, Kpad] complex64, unit norm (small-shift search)
     row_order: Optional[torch.Tensor] = None # [sum M_i] int32 processing order of the small-shift search
     particle_order: Optional[torch.Tensor] = None  # [B] int32 processing order of the split search
-    spectra_frag: Optional[kernels.FragmentPlanes] = None  # [B, P] the spectra in fragment order (the searches')
+    spectra_frag: Optional[kernels.FragmentPlanes] = None  # [B, P] psi spectra in fragment order (searched)
 
     @property
     def B(self) -> int:
@@ -172,13 +172,29 @@ class SearchEngine:
         return ps
 
     def prepare(self, ps: ParticleSet) -> ParticleSet:
+        """Psi spectra in fragment order (the searches' operand layout) and
+        their 1/norms; a row-major copy made earlier (``row_spectra``) is
+        refreshed."""
         ctf_tab = self.ctf_tab if ps.defocus is not None else None
-        ps.spectra, ps.inv_norm = kernels.prepare_particles(
+        if ps.spectra_frag is None:
+            ps.spectra_frag = kernels.FragmentPlanes(
+                torch.empty((ps.B, ps.P, kernels.fragment_plane(self.n)), dtype=torch.complex64,
+                            device=self.device), self.n)
+        _, ps.inv_norm = kernels.prepare_particles(
             ps.images, self.n, ps.rotp, self.mask, ctf_tab, ps.defocus, self.ctf.kconst,
-            out=ps.spectra, inv_norm=ps.inv_norm)
-        if ps.spectra_frag is not None:  # the search copy follows, in place
-            kernels.pack_fragments(ps.spectra, out=ps.spectra_frag.data)
+            out=ps.spectra_frag, inv_norm=ps.inv_norm)
+        if ps.spectra is not None:
+            ps.spectra.copy_(ps.spectra_frag.unpack())
         return ps
+
+    def row_spectra(self, ps: ParticleSet) -> torch.Tensor:
+        """The psi spectra row major [B, P, n, n/2+1] (unpacked from the
+        fragment-order planes once, then kept in step by ``prepare``)."""
+        if ps.spectra_frag is None:
+            self.prepare(ps)
+        if ps.spectra is None:
+            ps.spectra = ps.spectra_frag.unpack()
+        return ps.spectra
 
     # ---------------------------------------------------------------- model
     def density(self, coords, numbers, res, base: Optional[torch.Tensor] = None) -> torch.Tensor:
@@ -207,7 +223,7 @@ class SearchEngine:
 
     # ---------------------------------------------------------------- search
     def search(self, bank: torch.Tensor, ps: ParticleSet, tinv: Optional[torch.Tensor] = None) -> SearchResult:
-        if ps.spectra is None:
+        if ps.spectra_frag is None:
             self.prepare(ps)
         if int(bank.shape[0]) != len(self.dir_rot):
             raise ValueError("template bank does not cover the engine's direction table")
@@ -220,18 +236,16 @@ class SearchEngine:
 
     def search_spectra(self, ps: ParticleSet) -> kernels.FragmentPlanes:
         """The particle spectra in fragment order, the searches' operand
-        layout (made once per set, kept in step by ``prepare``)."""
-        if ps.spectra is None:
-            self.prepare(ps)
+        layout (written by ``prepare``)."""
         if ps.spectra_frag is None:
-            ps.spectra_frag = kernels.pack_fragments(ps.spectra)
+            self.prepare(ps)
         return ps.spectra_frag
 
     def search_split(self, bank: torch.Tensor, ps: ParticleSet, tinv: Optional[torch.Tensor] = None,
                      dt: int = 0, pt: int = 0) -> SearchResult:
         """The search of ``search`` (every coefficient, every lag of the crop)
         on split-bf16 MFMA; particles are processed in orientation order."""
-        if ps.spectra is None:
+        if ps.spectra_frag is None:
             self.prepare(ps)
         if int(bank.shape[0]) != len(self.dir_rot):
             raise ValueError("template bank does not cover the engine's direction table")
@@ -275,11 +289,9 @@ class SearchEngine:
     def prepare_small_shift(self, ps: ParticleSet, drop_spectra: bool = False) -> ParticleSet:
         """Gather the unit-norm in-disk coefficients of every psi spectrum (once per dataset)."""
         self._require_small()
-        if ps.spectra is None:
-            self.prepare(ps)
-        ps.coeffs = kernels.gather_coefficients(ps.spectra, self.small_kidx, self.small.Kpad, ps.inv_norm)
+        ps.coeffs = kernels.gather_coefficients(self.row_spectra(ps), self.small_kidx, self.small.Kpad, ps.inv_norm)
         if drop_spectra:
-            ps.spectra = None
+            ps.spectra = ps.spectra_frag = None
         return ps
 
     def template_coefficients(self, bank: torch.Tensor, tinv: Optional[torch.Tensor] = None) -> torch.Tensor:

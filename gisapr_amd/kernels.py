@@ -434,7 +434,9 @@ def build_template_coefficients(dft: torch.Tensor, new_box: int, rts: torch.Tens
 def prepare_particles(images: torch.Tensor, new_box: int, rotp: torch.Tensor, mask: Optional[torch.Tensor],
                       ctf_tab: Optional[torch.Tensor], defocus: Optional[torch.Tensor], kconst: np.ndarray,
                       out: Optional[torch.Tensor] = None, inv_norm: Optional[torch.Tensor] = None):
-    """Packed conjugated particle spectra [B, P, n, n/2+1] and 1/norms [B, P]."""
+    """Packed conjugated particle spectra [B, P, n, n/2+1] and 1/norms [B, P];
+    ``out`` may be a row-major [B, P, n, ld] tensor (ld >= n/2+1) or
+    FragmentPlanes [B, P] (the searches' layout, written directly)."""
     require_hip(images, "images")
     B, N = int(images.shape[0]), int(images.shape[1])
     P = int(rotp.shape[1])
@@ -444,14 +446,25 @@ def prepare_particles(images: torch.Tensor, new_box: int, rotp: torch.Tensor, ma
         raise ValueError("rotp must be [B, P, 6]")
     if out is None:
         out = torch.empty((B, P, n, n // 2 + 1), dtype=torch.complex64, device=images.device)
+    if isinstance(out, FragmentPlanes):
+        dst, ld = out.data, 0
+        if out.n != n or tuple(dst.shape) != (B, P, fragment_plane(n)) or dst.dtype != torch.complex64 \
+                or not dst.is_contiguous():
+            raise ValueError("out must be contiguous complex64 fragment planes [B, P, fragment_plane(n)]")
+    else:
+        dst, ld = out, int(out.shape[-1])
+        if tuple(dst.shape[:3]) != (B, P, n) or dst.dim() != 4 or ld < n // 2 + 1 or \
+                dst.dtype != torch.complex64 or not dst.is_contiguous():
+            raise ValueError("out must be contiguous complex64 [B, P, n, ld >= n/2+1]")
+    require_hip(dst, "out")
     if inv_norm is None:
         inv_norm = torch.empty((B, P), dtype=torch.float32, device=images.device)
     kc = (C.c_float * 5)(*[float(v) for v in np.asarray(kconst, np.float32)])
     wsb = int(lib().gis_prepare_particles_workspace_ex(N, B, P))
     ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=images.device)
-    check(lib().gis_prepare_particles(ptr(images), B, N, n, P, ptr(rotp), ptr(mask), ptr(ctf_tab),
-                                      ptr(defocus), C.cast(kc, C.c_void_p), ptr(out), ptr(inv_norm), ptr(ws),
-                                      wsb, stream_of(images.device)), "gis_prepare_particles")
+    check(lib().gis_prepare_particles_ld(ptr(images), B, N, n, P, ptr(rotp), ptr(mask), ptr(ctf_tab),
+                                         ptr(defocus), C.cast(kc, C.c_void_p), ptr(dst), ld, ptr(inv_norm), ptr(ws),
+                                         wsb, stream_of(images.device)), "gis_prepare_particles")
     return out, inv_norm
 
 

@@ -204,7 +204,7 @@ class SearchServer:
                 engine.enable_small_shift()
             if particles.coeffs is None:
                 engine.prepare_small_shift(particles, drop_spectra=True)
-        elif particles.spectra is None:
+        elif particles.spectra_frag is None:
             engine.prepare(particles)
         for _ in range(2):
             ds.executors.append(CandidateBatch(engine, particles, body, res, self.slots, path=self.path,

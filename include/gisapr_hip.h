@@ -239,6 +239,16 @@ int gis_prepare_particles(const float* images, int B, int N, int n, int P, const
                           size_t ws_bytes, void* stream);
 
 /*
+ * gis_prepare_particles with the spectra written rows `ld` >= n/2+1 complex
+ * values apart ([B][P][n][ld], columns past n/2 untouched), or with ld = 0 in
+ * fragment order ([B][P][gis_fragment_plane(n)], pads zero): the searches'
+ * operand layout, written directly (no permuting copy).
+ */
+int gis_prepare_particles_ld(const float* images, int B, int N, int n, int P, const float* rotp, const float* mask,
+                             const float* ctf_tab, const float* defocus, const float* kconst, void* spectra, int ld,
+                             float* inv_norm, void* ws, size_t ws_bytes, void* stream);
+
+/*
  * Packed CTF of B particles.  Replaces PackedCTFTorch.batch (v606:778-829).
  *   ctf_tab [4][n][n/2+1] float32 (x, y, u4, ncurve);  defocus [B][3]
  *   kconst  HOST [5] float32 (k1, k2, k3, k5, kk);     out [B][n][n/2+1] float32

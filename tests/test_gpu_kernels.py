@@ -164,7 +164,8 @@ def test_batched_template_builds_equal_single_calls(hip_device, N, n, M):
         assert torch.equal(coef[rows], k1) and torch.equal(cn[rows], kn1), c
 
 
-@pytest.mark.parametrize("N,n,with_ctf", [(32, 32, True), (64, 64, True), (64, 48, True), (64, 64, False)])
+@pytest.mark.parametrize("N,n,with_ctf", [(32, 32, True), (64, 64, True), (64, 48, True), (64, 64, False),
+                                          (64, 40, True)])
 def test_particles_match_oracle(hip_device, N, n, with_ctf):
     import torch
     from gisapr_amd import host, kernels
@@ -194,6 +195,19 @@ def test_particles_match_oracle(hip_device, N, n, with_ctf):
         want = opt.prepare_particle(imgs[b].astype(np.float64), effs[b], n, mask, c, np.float64)
         e = rel_l2(got[b], want)
         assert e < 1e-4, (b, e)
+    # the same spectra written in fragment order (pads zeroed) and in padded rows, bit for bit
+    args = (torch.from_numpy(imgs).to(dev), n, torch.from_numpy(rotp).to(dev), torch.from_numpy(mask).to(dev),
+            torch.from_numpy(ctf.table).to(dev) if with_ctf else None,
+            torch.from_numpy(dfo).to(dev) if with_ctf else None, ctf.kconst)
+    say("particles fragment order", N, n, with_ctf)
+    frag = kernels.FragmentPlanes(torch.full((B, len(effs[0]), kernels.fragment_plane(n)), float("nan"),
+                                             dtype=torch.complex64, device=dev), n)
+    fo, finv = kernels.prepare_particles(*args, out=frag)
+    assert torch.equal(fo.data, kernels.pack_fragments(spec).data) and torch.equal(finv, inv)
+    ld = kernels.padded_row_stride(n)
+    po, pinv = kernels.prepare_particles(*args, out=torch.zeros((B, len(effs[0]), n, ld), dtype=torch.complex64,
+                                                                 device=dev))
+    assert torch.equal(po[..., :n // 2 + 1], spec) and torch.equal(pinv, inv)
 
 
 @pytest.mark.parametrize("n,C", [(32, 8), (64, 10), (64, 16), (96, 16), (192, 16), (256, 16), (64, 20), (160, 32),
