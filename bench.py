@@ -14,19 +14,25 @@ float64 score sum.  Legs, on the same particles and poses:
                  coefficient, the 16 x 16 crop: +/-8 px) on FP32 MFMA
   split          one candidate per step, the same search (same operator,
                  same crop) on split-bf16 MFMA
+  split16        one candidate per step, the same search with fp16 operands
+                 (packed fp16 products, FP32 accumulation), then each
+                 particle's best pair and its near ties rescored on
+                 split-bf16 MFMA: the split leg's answers, bit for bit
   mixed          one candidate per step, the small-shift split-bf16 search of
                  the precision variant (coefficients inside 1/(4 A), 49 lags,
                  +/-3 px): a different, truncated functional
-  batched_fp32   ``--slots`` candidates per step, one search launch over all
-  batched_split  of them, replayed as a captured HIP graph, tile variant
-  batched_mixed  from the autotuner (batched_mixed only when asked for)
+  batched_fp32     ``--slots`` candidates per step, one search launch over
+  batched_split    all of them, replayed as a captured HIP graph, tile
+  batched_split16  variant from the autotuner (batched_mixed only when
+  batched_mixed    asked for)
 
-``value`` is batched_split particle-orientation scores/s; beside it: the
-ratio to the plain fp32 leg, top-1 (direction, psi, shift) agreement of the
-split and fp32 searches on every particle, the split search's error against
-the same operator in float64 on a 256-particle subset, and the same for the
-small-shift variant (whose top-1 is not expected to match: it truncates the
-coefficients and the lags).
+``value`` is batched_split16 particle-orientation scores/s (batched_split
+when that leg is not run); beside it: the ratio to the plain fp32 leg, top-1
+(direction, psi, shift) agreement of the split / split16 and fp32 searches
+on every particle, the split searches' error against the same operator in
+float64 on a 256-particle subset, and the same for the small-shift variant
+(whose top-1 is not expected to match: it truncates the coefficients and the
+lags).
 
 Batched PSO leg (``--pso-iters``, 0 to skip): box-192 workload (20,000-atom
 model, moving chain 4,500, 20,000 particles at 1.3 A/px), a swarm of 64
@@ -70,6 +76,15 @@ def split_flops_per_score(n: int, crop: int) -> float:
     complex, 8 flops per MAC) over n x (n/2+1) coefficients x C lags, stage 2
     (rows to lags, real part, 4 flops) over n rows x C^2 lags, 3 products each."""
     return 3.0 * (8.0 * n * (n // 2 + 1) * crop + 4.0 * n * crop * crop)
+
+
+def split16_flops_per_score(n: int, crop: int) -> float:
+    """MFMA work of one fp16-operand score: stage 1 with one fp16 product
+    (8 flops per complex MAC) over n x (n/2+1) coefficients x C lags, stage 2
+    on split-bf16 (3 products) over n rows x C^2 lags.  The split-bf16
+    rescoring of each particle's best pair and near ties is extra work, not
+    counted."""
+    return 8.0 * n * (n // 2 + 1) * crop + 3.0 * 4.0 * n * crop * crop
 
 
 def template_bytes_per_direction(N: int, n: int) -> float:
@@ -272,7 +287,10 @@ def precision_check_split(eng, ps, bank, res, subset: int, seed: int):
     ``host.crop_functional``) on a seeded particle subset; ``bank`` =
     (unnormalised rows, 1/norms) of the candidate the split leg searched last.
     A top-1 difference counts as a near-tie when the float64 scores of the two
-    choices differ by at most 1e-5 of the particle's best score."""
+    choices differ by at most 1e-5 of the particle's best score.
+    ``max_rel_err_fp64``: the particle's reported best score (rawCC) against
+    the float64 best; ``max_rel_err_fp64_all_pairs``: every pair's score
+    against the float64 one, relative to the particle's best."""
     import torch
     from gisapr_amd import host, kernels
     dev = eng.device
@@ -289,7 +307,8 @@ def precision_check_split(eng, ps, bank, res, subset: int, seed: int):
     px = res.corr.peak_x.cpu().numpy().reshape(-1)
     best = res.corr.best.cpu().numpy()
     P, C = ps.P, eng.crop
-    agree, ties, worst, used = 0, 0, 0.0, 0
+    peak = res.corr.peak.cpu().numpy()
+    agree, ties, worst, worst_best, used = 0, 0, 0.0, 0.0, 0
     for b in pick:
         d = idx[off[b]:off[b + 1]]
         if len(d) == 0:
@@ -305,6 +324,7 @@ def precision_check_split(eng, ps, bank, res, subset: int, seed: int):
         g = slice(off[b] * P, off[b + 1] * P)
         worst = max(worst, float(np.abs(sc[g] - v).max() / np.abs(v).max()))
         bw = int(np.argmax(v))
+        worst_best = max(worst_best, abs(float(peak[b]) - float(v[bw])) / abs(float(v[bw])))
         gb = off[b] * P + int(best[b])
         same = int(best[b]) == bw and int(py[gb]) * C + int(px[gb]) == int(k[bw])
         agree += int(same)
@@ -313,7 +333,8 @@ def precision_check_split(eng, ps, bank, res, subset: int, seed: int):
             ties += int(v[bw] - chosen <= 1e-5 * abs(v[bw]))
         used += 1
     return {"subset": int(used), "seed": seed, "top1_agree_fp64": agree / max(used, 1),
-            "top1_differs_outside_near_ties": used - agree - ties, "max_rel_err_fp64": worst,
+            "top1_differs_outside_near_ties": used - agree - ties, "max_rel_err_fp64": worst_best,
+            "max_rel_err_fp64_all_pairs": worst,
             "reference": "same operator (every coefficient, the full crop) in float64 (torch complex128)"}
 
 
@@ -510,14 +531,14 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="split-bf16 serving stream (0 skips the serving legs)")
     ap.add_argument("--serve-requests-fp32", type=int, default=40, help="FP32 serving stream (0 to skip it)")
-    ap.add_argument("--serve-paths", default="fp32,split")
-    ap.add_argument("--pso-path", default="split", help="search path of the batched PSO leg")
+    ap.add_argument("--serve-paths", default="fp32,split16")
+    ap.add_argument("--pso-path", default="split16", help="search path of the batched PSO leg")
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the legs, run one device-to-device copy of this many MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--legs", default="fp32,split,mixed,batched_fp32,batched_split",
+    ap.add_argument("--legs", default="fp32,split,split16,mixed,batched_fp32,batched_split,batched_split16",
                     help="comma list of legs to run (profiling runs select one)")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
@@ -601,7 +622,7 @@ def main() -> int:
 
     want = set(args.legs.split(","))
     legs, results = {}, {}
-    bank = bank_split = tk_last = None  # the last candidate's (bank, 1/norms) / template coefficients
+    bank = bank_split = bank_split16 = tk_last = None  # last candidate's (bank, 1/norms) / template coefficients
     if "fp32" in want:
         # FP32 path: full unnormalised bank (1/norm folded into the search)
         legs["fp32"], results["fp32"], bank = run_single(lambda v: eng.templates(v, normalize=False, fragments=True),
@@ -610,6 +631,11 @@ def main() -> int:
         # the same search on split-bf16 MFMA, the same bank values
         legs["split"], results["split"], bank_split = run_single(
             lambda v: eng.templates(v, normalize=False, fragments=True), lambda t, p: eng.search_split(t[0], p, t[1]))
+    if "split16" in want:
+        # fp16 operands, then the best pair and near ties of each particle on split-bf16
+        legs["split16"], results["split16"], bank_split16 = run_single(
+            lambda v: eng.templates(v, normalize=False, fragments=True),
+            lambda t, p: eng.search_split(t[0], p, t[1], operand="fp16"))
     if "mixed" in want:
         # split-bf16 path: the template kernel writes the in-disk coefficients directly
         legs["mixed"], results["mixed"], tk_last = run_single(eng.small_templates,
@@ -619,7 +645,7 @@ def main() -> int:
     # batched legs: ``slots`` candidates per step, graph-captured, autotuned search tile
     body = RigidBody.from_arrays(coords, numbers, chains, wl.moving_chain)
     tuner = Autotuner()
-    for path in ("fp32", "split", "mixed"):
+    for path in ("fp32", "split", "split16", "mixed"):
         if "batched_" + path not in want:
             continue
         batch = CandidateBatch(eng, ps, body, wl.res, args.slots, path=path, autotuner=tuner)
@@ -662,14 +688,26 @@ def main() -> int:
                                      "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps,
                                      "launch_ms": ms,
                                      "traffic": profiled_traffic("split_correlate_kernel", wl.name)}
-        ms_t = legs["split"]["stage_ms"]["templates"]
+    for leg in ("split", "split16"):
+        if leg not in legs:
+            continue
+        ms_t = legs[leg]["stage_ms"]["templates"]
         bpd = template_bytes_per_direction(eng.N, n)
         ach_t = bpd * len(eng.dir_rot) / (ms_t * 1e-3) / 1e9
-        legs["split"]["roofline_templates"] = {
+        legs[leg]["roofline_templates"] = {
             "bound": "hbm", "kernel": "volume transform + build_templates_kernel", "achieved": ach_t,
             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach_t / PEAK_HBM_GBS,
             "bytes_per_direction": bpd, "directions": len(eng.dir_rot), "launch_ms": ms_t,
             "traffic": profiled_traffic("build_templates_kernel", wl.name)}
+    if "split16" in legs:
+        ms = legs["split16"]["stage_ms"]["search"]
+        fps = split16_flops_per_score(n, eng.crop)
+        ach = fps * n_scores / (ms * 1e-3) / 1e12
+        legs["split16"]["roofline"] = {
+            "bound": "mfma", "kernel": "split16_correlate_kernel (+ tables, particle tail, split-bf16 rescoring)",
+            "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps, "launch_ms": ms,
+            "traffic": profiled_traffic("split16_correlate_kernel", wl.name)}
     if "mixed" in legs:
         ms = legs["mixed"]["stage_ms"]["search"]
         fps = mixed_flops_per_score(tab.K, tab.n_lags)
@@ -688,23 +726,25 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
 
-    head_name = next(k for k in ("batched_split", "split", "batched_mixed", "mixed", "batched_fp32", "fp32")
-                     if k in legs)
+    head_name = next(k for k in ("batched_split16", "batched_split", "split16", "split", "batched_mixed", "mixed",
+                                 "batched_fp32", "fp32") if k in legs)
     head = legs[head_name]
-    single = next(k for k in ("split", "mixed", "fp32") if k in legs)
+    single = next(k for k in ("split16", "split", "mixed", "fp32") if k in legs)
     roof = dict(legs[single].get("roofline", {}))
-    if single == "split" and "roofline_templates" in legs["split"]:
+    if single in ("split", "split16") and legs[single].get("roofline_templates"):
         # the line's roofline is the kernel that takes the larger share of a candidate step
-        st = legs["split"]["stage_ms"]
+        st = legs[single]["stage_ms"]
         if st["templates"] > st["search"]:
-            roof = dict(legs["split"]["roofline_templates"])
-        roof["from_leg"] = "split (one candidate per step, HIP events on the launch stream)"
+            roof = dict(legs[single]["roofline_templates"])
+        roof["from_leg"] = f"{single} (one candidate per step, HIP events on the launch stream)"
     out = {
         "metric": METRIC, "value": head["scores_per_s"], "unit": "scores/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": head.get("ms_per_step", head["ms_per_candidate"]),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": ("bf16x3 (split hi+lo, fp32 accumulate)" if ("mixed" in head_name or "split" in head_name)
+        "dtype": ("fp16 operands (packed fp16 products, fp32 accumulate) + bf16x3 rescoring of each particle's "
+                  "best pair and near ties" if "split16" in head_name
+                  else "bf16x3 (split hi+lo, fp32 accumulate)" if ("mixed" in head_name or "split" in head_name)
                   else "fp32"),
         "data": "synthetic (seeded generators; random-walk model)",
         "config": {"workload": f"{wl.name}: {sum(wl.chain_sizes)}-atom model (moving chain "
@@ -726,7 +766,7 @@ def main() -> int:
     if "fp32" in legs:
         out["speedup_vs_fp32"] = {k: v["scores_per_s"] / legs["fp32"]["scores_per_s"] for k, v in legs.items()}
     agreement = {}
-    for other, center in (("split", eng.crop // 2), ("mixed", 3)):
+    for other, center in (("split", eng.crop // 2), ("split16", eng.crop // 2), ("mixed", 3)):
         if "fp32" in results and other in results:
             a = top1(results["fp32"], ps, eng.crop // 2)
             b = top1(results[other], ps, center)
@@ -735,14 +775,26 @@ def main() -> int:
                 "orientation": float(np.mean((a[valid, 0] == b[valid, 0]))),
                 "orientation_and_shift": float(np.mean(np.all(a[valid] == b[valid], axis=1))),
                 "particles": int(valid.sum())}
+    if "split16" in agreement:
+        out["top1_same_as_fp32"] = dict(agreement["split16"], path="split16 (same operator and crop)")
     if "split" in agreement:
-        out["top1_same_as_fp32"] = dict(agreement["split"], path="split (same operator and crop)")
+        out["top1_split_bf16_vs_fp32"] = dict(agreement["split"], path="split (same operator and crop)")
+    if "split" in results and "split16" in results:
+        a, b = results["split"].corr, results["split16"].corr
+        valid = a.best >= 0
+        out["split16_vs_split"] = {
+            "same_best": float((a.best == b.best)[valid].float().mean()),
+            "bit_identical_peak": float((a.peak == b.peak)[valid].float().mean()),
+            "particles": int(valid.sum())}
     if "mixed" in agreement:
         out["top1_small_shift_vs_fp32"] = dict(
             agreement["mixed"], path="mixed small-shift variant (truncated coefficients, +/-3 px lags)")
-    if args.precision_subset > 0 and "split" in results:
-        out["precision"] = precision_check_split(eng, ps, bank_split, results["split"], args.precision_subset,
+    if args.precision_subset > 0 and "split16" in results:
+        out["precision"] = precision_check_split(eng, ps, bank_split16, results["split16"], args.precision_subset,
                                                  synthetic.SEED_CAND)
+    if args.precision_subset > 0 and "split" in results:
+        out["precision_split_bf16"] = precision_check_split(eng, ps, bank_split, results["split"],
+                                                            args.precision_subset, synthetic.SEED_CAND)
     if args.precision_subset > 0 and "mixed" in results:
         out["precision_small_shift"] = precision_check(eng, ps, tk_last, results["mixed"], args.precision_subset,
                                                        synthetic.SEED_CAND)
@@ -751,14 +803,14 @@ def main() -> int:
                                            synthetic.SEED_CAND)
     if not args.no_cpu_baseline and args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(n_scores, args.cpu_sample, log=log)
-    del ps, eng, bank, bank_split, tk_last, results
+    del ps, eng, bank, bank_split, bank_split16, tk_last, results
     torch.cuda.empty_cache()
     if args.pso_iters > 0:
         out["pso"] = pso_leg(args, dev, log)
         torch.cuda.empty_cache()
     if args.serve_requests > 0:
         served = serve_leg(args, dev, log)
-        main_path = next((p for p in ("split", "mixed") if p in served), None)
+        main_path = next((p for p in ("split16", "split", "mixed") if p in served), None)
         if main_path is not None:
             out["serving"] = served[main_path]
             out["p50_ms"] = served[main_path]["p50_ms"]

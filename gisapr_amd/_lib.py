@@ -60,6 +60,10 @@ SIGNATURES = {
                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gis_correlate_split_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i,
                                          _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gis_pack_fragments_f16": (_i, [_vp, _vp, _i, _i, _f, _vp, _vp]),
+    "gis_correlate_split_f16_workspace": (_sz, [_i, _i, _i, C.c_long]),
+    "gis_correlate_split_f16": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i,
+                                     _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "gis_threshold_mask": (_i, [_vp, _sz, _f, _vp, _vp]),
     "gis_edt_squared": (_i, [_vp, _i, _vp, _vp]),
     "gis_restraint_eval": (_i, [_vp, _f, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),

@@ -94,11 +94,13 @@ class Autotuner:
 
 MIXED_VARIANTS = [dict(pt=5, wpb=4), dict(pt=5, wpb=2), dict(pt=3, wpb=4), dict(pt=1, wpb=4)]
 SPLIT_VARIANTS = [dict(dt=2, pt=5), dict(dt=1, pt=5), dict(dt=4, pt=5), dict(dt=2, pt=3)]
+SPLIT16_VARIANTS = [dict(dt=1, pt=5), dict(dt=2, pt=5), dict(dt=1, pt=3), dict(dt=2, pt=2)]
+PATHS = ("fp32", "split", "split16", "mixed")
 FP32_VARIANTS = [dict(dt=1, pt=5), dict(dt=2, pt=5), dict(dt=1, pt=3), dict(dt=2, pt=3)]
 
 
 def _variants_for(path: str, P: int) -> List[dict]:
-    table = {"mixed": MIXED_VARIANTS, "split": SPLIT_VARIANTS}.get(path, FP32_VARIANTS)
+    table = {"mixed": MIXED_VARIANTS, "split": SPLIT_VARIANTS, "split16": SPLIT16_VARIANTS}.get(path, FP32_VARIANTS)
     out = []
     for v in table:
         v = dict(v, pt=min(v["pt"], P))
@@ -270,7 +272,9 @@ class RigidBody:
 class CandidateBatch:
     """``slots`` candidate poses of one moving chain, searched against one
     resident particle set, on the FP32 ("fp32"), full-plane split-bf16
-    ("split") or split-bf16 small-shift ("mixed") path.
+    ("split"), full-plane fp16 with split-bf16 rescoring of each particle's
+    best pair and near ties ("split16") or split-bf16 small-shift ("mixed")
+    path.
 
     ``evaluate(poses)`` runs one step eagerly; ``capture()`` records the
     device part as a HIP graph and later ``evaluate`` calls replay it.
@@ -286,8 +290,8 @@ class CandidateBatch:
     def __init__(self, engine: SearchEngine, ps: ParticleSet, body: RigidBody, res: float, slots: int, *,
                  path: str = "mixed", subsets=None, autotuner: Optional[Autotuner] = None,
                  template_group: Optional[int] = None):
-        if path not in ("mixed", "fp32", "split"):
-            raise ValueError("path must be 'mixed', 'split' or 'fp32'")
+        if path not in PATHS:
+            raise ValueError(f"path must be one of {PATHS}")
         self.engine, self.ps, self.body, self.path = engine, ps, body, path
         self.slots = int(slots)
         if template_group is None:
@@ -321,6 +325,7 @@ class CandidateBatch:
         self.dft = torch.empty((self.group, int(kernels.lib().gis_volume_dft_bytes(N)) // 8), dtype=torch.complex64,
                                device=dev)
         self.dft_ws = torch.empty((int(kernels.lib().gis_volume_dft_workspace(N)),), dtype=torch.uint8, device=dev)
+        self.bank16 = None
         if path == "mixed":
             # template coefficients written straight from the template kernel (no bank)
             self.bank = None
@@ -330,6 +335,10 @@ class CandidateBatch:
             self.bank = kernels.FragmentPlanes(torch.empty((self.slots * M, kernels.fragment_plane(n)),
                                                            dtype=torch.complex64, device=dev), n)
             engine.search_spectra(ps)
+            if path == "split16":  # the bank as fp16 pairs for the fp16 pass, packed after each template launch
+                engine.search_spectra16(ps)
+                self.bank16 = kernels.HalfFragmentPlanes(torch.empty((self.slots * M, kernels.fragment_plane(n)),
+                                                                     dtype=torch.complex32, device=dev), n)
         self.tinv = torch.empty((self.slots * M,), dtype=torch.float32, device=dev)  # 1/norm per bank row
         self.tpl_ws = torch.empty((max(int(kernels.lib().gis_build_templates_workspace(N, self.group * M)), 1),),
                                   dtype=torch.uint8, device=dev)
@@ -365,6 +374,8 @@ class CandidateBatch:
         _, norms = kernels.build_templates(dft, eng.n, eng.rts, eng.mask, eng.fsc_packed, out=self.bank[rows],
                                            normalize=False, workspace=self.tpl_ws)
         torch.reciprocal(norms, out=self.tinv[rows])
+        if self.bank16 is not None:
+            kernels.pack_fragments_f16(self.bank[rows], self.tinv[rows], out=self.bank16.data[rows])
 
     def _all_templates(self, k: int):
         for c0 in range(0, k, self.group):
@@ -384,6 +395,13 @@ class CandidateBatch:
                                            validate=False, pmap=pmap, tinv=self.tinv,
                                            order=self.plan.virtual_major(self.slots if k is None else k),
                                            dt=tile.get("dt", 0), pt=tile.get("pt", 0))
+        if self.path == "split16":
+            return kernels.correlate_split_f16(self.bank, eng.search_spectra16(ps), dir_off, dir_idx, eng.crop,
+                                               validate=False, pmap=pmap, tinv=self.tinv,
+                                               order=self.plan.virtual_major(self.slots if k is None else k),
+                                               dt=tile.get("dt", 0), pt=tile.get("pt", 0),
+                                               refine=eng.search_spectra(ps), inv_norm=ps.inv_norm,
+                                               bank16=self.bank16)
         return kernels.correlate(self.bank, eng.search_spectra(ps), ps.inv_norm, dir_off, dir_idx, eng.crop,
                                  validate=False, pmap=pmap, tinv=self.tinv,
                                  order=self.plan.virtual_major(self.slots if k is None else k),
@@ -446,6 +464,8 @@ class CandidateBatch:
         eng, ps = self.engine, self.ps
         if self.path == "mixed":
             return (ps.coeffs, eng.small_bhi, eng.small_blo, eng.small_plast, eng.small_kslot)
+        if self.path == "split16":
+            return (ps.spectra_frag.data, ps.inv_norm, ps.spectra16.data)
         return (ps.spectra_frag.data, ps.inv_norm)
 
     def evaluate(self, poses, sync: bool = True):
@@ -585,8 +605,8 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
     tile = tile or {}
     C, M = len(volumes), len(engine.dir_rot)
     plan = build_slot_plan(ps, M, C)
-    if path not in ("mixed", "fp32", "split"):
-        raise ValueError("path must be 'fp32', 'split' or 'mixed'")
+    if path not in PATHS:
+        raise ValueError(f"path must be one of {PATHS}")
     if path == "mixed":
         if ps.coeffs is None:
             engine.prepare_small_shift(ps)
@@ -614,6 +634,12 @@ def search_volumes(engine: SearchEngine, ps: ParticleSet, volumes: Sequence[torc
                                            engine.crop, validate=False, pmap=plan.pmap, tinv=tinv,
                                            order=plan.virtual_major(C),
                                            dt=tile.get("dt", 0), pt=tile.get("pt", 0))
+        elif path == "split16":
+            corr = kernels.correlate_split_f16(bank, engine.search_spectra16(ps), plan.dir_off, plan.dir_idx,
+                                               engine.crop, validate=False, pmap=plan.pmap, tinv=tinv,
+                                               order=plan.virtual_major(C), dt=tile.get("dt", 0),
+                                               pt=tile.get("pt", 0), refine=engine.search_spectra(ps),
+                                               inv_norm=ps.inv_norm, bank16=kernels.pack_fragments_f16(bank, tinv))
         else:
             corr = kernels.correlate(bank, engine.search_spectra(ps), ps.inv_norm, plan.dir_off, plan.dir_idx,
                                      engine.crop,

@@ -45,6 +45,8 @@
 #include <mutex>
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -105,10 +107,13 @@ __global__ void pack_fragments_kernel(const float2* __restrict__ src, int rows, 
 //   Et [nct][LT][hi, lo][64 lanes] x 8: lane (g, i) -> lag 16 tx + i, slot 2v + ri -> column 16 ct + 4g + v,
 //      (Re E, -Im E)
 // Lags beyond C and rows / columns beyond the plane are zero.
-//   FP32 tables (f32): the same eight values per lane and fragment, rounded
-//   to float and stored [fragment][64 lanes][8] (one 32-byte run per lane).
-__global__ void split_tables_kernel(int n, int C, int LT, int nks, int nct, int f32, void* __restrict__ Wt,
+//   mode kTabF32: the same eight values per lane and fragment, rounded to
+//   float and stored [fragment][64 lanes][8] (one 32-byte run per lane).
+//   mode kTabF16: Wt split into fp16 hi + lo (same layout), Et as bf16.
+enum { kTabBF16 = 0, kTabF32 = 1, kTabF16 = 2 };
+__global__ void split_tables_kernel(int n, int C, int LT, int nks, int nct, int mode, void* __restrict__ Wt,
                                     void* __restrict__ Et) {
+  const int f32 = mode == kTabF32;
   const int m = n / 2 + 1, s = m / 2;
   const int nw = nks * 2 * LT * 64, ne = nct * LT * 64;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nw + ne; t += gridDim.x * blockDim.x) {
@@ -165,6 +170,18 @@ __global__ void split_tables_kernel(int n, int C, int LT, int nks, int nct, int 
       for (int e = 0; e < 8; ++e) dst[(blk * 64 + lane) * 8 + e] = (float)v[e];
       continue;
     }
+    if (mode == kTabF16 && t < nw) {
+      f16x8* dst = (f16x8*)Wt;
+      f16x8 hi, lo;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        hi[e] = (_Float16)(float)v[e];
+        lo[e] = (_Float16)(float)(v[e] - (double)(float)hi[e]);
+      }
+      dst[o] = hi;
+      dst[o + 64] = lo;
+      continue;
+    }
     bf16x8* dst = (bf16x8*)(t < nw ? Wt : Et);
     bf16x8 hi, lo;
 #pragma unroll
@@ -183,9 +200,10 @@ __global__ void split_tables_kernel(int n, int C, int LT, int nks, int nct, int 
 // ceil(M_b / DT); unit_off[B] = the total.  One 1024-thread block.
 __global__ void __launch_bounds__(1024)
 split_units_kernel(const int* __restrict__ dir_off, const int* __restrict__ order, int B, int DT,
-                   int* __restrict__ unit_off) {
+                   int* __restrict__ unit_off, int* __restrict__ zero = nullptr) {
   __shared__ int part[1024];
   const int t = threadIdx.x;
+  if (zero && t == 0) *zero = 0;  // the pair-list counter of a refined search
   const int chunk = (B + 1023) / 1024;
   const int k0 = min(t * chunk, B), k1 = min(k0 + chunk, B);
   int s = 0;
@@ -218,7 +236,7 @@ struct Frags {
   float2 p[PT][4];
 };
 
-template <int LT, int DT, int PT, bool F32, bool FR>
+template <int LT, int DT, int PT, bool F32, bool FR, bool LIST = false>
 __global__ void __launch_bounds__(64 * kWaves)
 split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spectra,
                        const float* __restrict__ inv_norm, const float* __restrict__ tinv,
@@ -226,7 +244,8 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
                        const int* __restrict__ pmap, const int* __restrict__ order,
                        const int* __restrict__ unit_off, int B, int P, int n, int C, int ldt, int ldp, int nblk,
                        const void* __restrict__ Wtv, const void* __restrict__ Etv, float* __restrict__ scores,
-                       int* __restrict__ peak_y, int* __restrict__ peak_x) {
+                       int* __restrict__ peak_y, int* __restrict__ peak_x, const int2* __restrict__ list = nullptr,
+                       const int* __restrict__ list_count = nullptr) {
   // the row table (read every step) lives in LDS; the column table (read
   // once per column tile) is read from global memory (cached)
   extern __shared__ bf16x8 tab[];
@@ -240,20 +259,527 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
   const bool lone = (m & 15) == 1;
   const int nct = lone ? (m >> 4) : (m + 15) >> 4;
   const int nks = (((n + 15) >> 4) + 1) & ~1;  // row steps, padded to even (zero table rows)
-  const int nw = nks * 4 * LT * 64;
-  for (int e = threadIdx.x; e < nw; e += 64 * kWaves) tab[e] = Wt[e];
-  __syncthreads();
-  const bf16x8* Wl = tab;
+  // the pair-list pass (few waves, latency bound) reads the row table
+  // through the caches instead, so no LDS limits its occupancy
+  static_assert(!(LIST && F32), "the pair list mode is split-bf16");
+  const bf16x8* Wl = LIST ? Wt : tab;
+  if constexpr (!LIST) {
+    const int nw = nks * 4 * LT * 64;
+    for (int e = threadIdx.x; e < nw; e += 64 * kWaves) tab[e] = Wt[e];
+    __syncthreads();
+  }
 
   // consecutive logical blocks on one XCD (blocks are dealt to the XCDs round robin)
   const int per = (int)(gridDim.x / kXcds);
   const int lb = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
   const int lane = threadIdx.x & 63;
-  // wave-uniform from here on: unit, particle and row pointers live in SGPRs
+  // LIST (pair list mode, DT = PT = 1): each wave scores the listed pairs
+  // (particle b, pair k = local direction * P + psi) wave, wave + waves, ...
+  // of the list_count[0] entries; otherwise one unit per wave
+  const int wave = lb * kWaves + (int)(threadIdx.x >> 6);
+  const int nlist = LIST ? list_count[0] : 0;
+  for (int it = wave;; it += (int)gridDim.x * kWaves) {
+    int b, j0, p0;
+    if constexpr (LIST) {
+      if (it >= nlist) break;  // no barrier below
+      const int2 e = list[__builtin_amdgcn_readfirstlane(it)];
+      b = __builtin_amdgcn_readfirstlane(e.x);
+      const int k = __builtin_amdgcn_readfirstlane(e.y);
+      j0 = k / P;
+      p0 = k - j0 * P;
+    } else {
+      // wave-uniform from here on: unit, particle and row pointers live in SGPRs
+      const int u = __builtin_amdgcn_readfirstlane(it);
+      const int total = unit_off[B];
+      if (lb >= nblk || u >= total) return;  // no barrier below
+      int lo = 0, hi = B;  // unit_off[lo] <= u < unit_off[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (unit_off[mid] <= u) lo = mid; else hi = mid;
+      }
+      b = order ? order[lo] : lo;
+      j0 = (u - unit_off[lo]) * DT;
+      p0 = (int)blockIdx.y * PT;
+    }
+    const int d0 = dir_off[b], Mi = dir_off[b + 1] - d0;
+    const int pb = pmap ? pmap[b] : b;
+    // template rows are ldt complex values apart, psi spectrum rows ldp
+    // (ld > n/2+1: zero padding, so 16-column segments start on 128-byte
+    // lines); ld = 0: the operand is in fragment order (frag_plane), each
+    // step's tile two contiguous 1 KiB runs read with 16-byte loads
+    const size_t fplane = (size_t)frag_plane(n);
+    const float2* tp[DT];
+    const float2* pp[PT];
+#pragma unroll
+    for (int d = 0; d < DT; ++d)
+      tp[d] = bank + (size_t)dir_idx[d0 + min(j0 + d, Mi - 1)] * (FR ? fplane : (size_t)n * ldt);
+#pragma unroll
+    for (int p = 0; p < PT; ++p)
+      pp[p] = spectra + ((size_t)pb * P + min(p0 + p, P - 1)) * (FR ? fplane : (size_t)n * ldp);
+    const int g = lane >> 4, i = lane & 15;
+
+    // Operands of step (ct, ks).  Loads are unconditional: an offset past the
+    // plane (rows >= n) is clamped to its last element, columns >= n/2+1 read
+    // the next row or the zero padding, and the tables hold zeros at those rows
+    // and columns, so the (finite) values read there contribute exactly
+    // nothing.
+    const int lastt = n * ldt - 1, lastp = n * ldp - 1;
+    auto load = [&](int ct, int ks, Frags<DT, PT>& f) {
+      const int r0 = 16 * ks + 4 * g, c = 16 * ct + i;
+      const int tile = (ks * nct + ct) * 256 + 2 * lane;  // fragment order: [h][lane][2 rows]
+      if constexpr (FR) {
+#pragma unroll
+        for (int d = 0; d < DT; ++d) {
+          const float4 q0 = *(const float4*)(tp[d] + tile), q1 = *(const float4*)(tp[d] + tile + 128);
+          f.t[d][0] = make_float2(q0.x, q0.y);
+          f.t[d][1] = make_float2(q0.z, q0.w);
+          f.t[d][2] = make_float2(q1.x, q1.y);
+          f.t[d][3] = make_float2(q1.z, q1.w);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int ot = min((r0 + j) * ldt + c, lastt);
+#pragma unroll
+          for (int d = 0; d < DT; ++d) f.t[d][j] = tp[d][ot];
+        }
+      }
+      if constexpr (FR) {
+#pragma unroll
+        for (int p = 0; p < PT; ++p) {
+          const float4 q0 = *(const float4*)(pp[p] + tile), q1 = *(const float4*)(pp[p] + tile + 128);
+          f.p[p][0] = make_float2(q0.x, q0.y);
+          f.p[p][1] = make_float2(q0.z, q0.w);
+          f.p[p][2] = make_float2(q1.x, q1.y);
+          f.p[p][3] = make_float2(q1.z, q1.w);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int op = min((r0 + j) * ldp + c, lastp);
+#pragma unroll
+          for (int p = 0; p < PT; ++p) f.p[p][j] = pp[p][op];
+        }
+      }
+    };
+
+    floatx4 out[DT][PT][LT * LT], gacc[DT][PT][2 * LT];
+#pragma unroll
+    for (int d = 0; d < DT; ++d)
+#pragma unroll
+      for (int p = 0; p < PT; ++p)
+#pragma unroll
+        for (int t = 0; t < LT * LT; ++t) out[d][p][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    // stage 1 of one step from buffer f, then f is refilled with step (ct2, ks2)
+    auto step = [&](int ks, Frags<DT, PT>& f, int ct2, int ks2) {
+      if constexpr (F32) {
+        float w[2 * LT][8];
+#pragma unroll
+        for (int t = 0; t < 2 * LT; ++t) {
+          const float4 w0 = Wf[((ks * 2 * LT + t) * 64 + lane) * 2], w1 = Wf[((ks * 2 * LT + t) * 64 + lane) * 2 + 1];
+          w[t][0] = w0.x; w[t][1] = w0.y; w[t][2] = w0.z; w[t][3] = w0.w;
+          w[t][4] = w1.x; w[t][5] = w1.y; w[t][6] = w1.z; w[t][7] = w1.w;
+        }
+#pragma unroll
+        for (int d = 0; d < DT; ++d) {
+#pragma unroll
+          for (int p = 0; p < PT; ++p) {
+            float x[8];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const float2 a = f.t[d][j], q = f.p[p][j];
+              x[2 * j] = a.x * q.x - a.y * q.y;
+              x[2 * j + 1] = a.x * q.y + a.y * q.x;
+            }
+#pragma unroll
+            for (int t = 0; t < 2 * LT; ++t)
+#pragma unroll
+              for (int kk = 0; kk < 8; ++kk) gacc[d][p][t] = mfma_f32(x[kk], w[t][kk], gacc[d][p][t]);
+          }
+        }
+      } else {
+        bf16x8 bh[2 * LT], bl[2 * LT];
+#pragma unroll
+        for (int t = 0; t < 2 * LT; ++t) {
+          bh[t] = Wl[((ks * 2 * LT + t) * 2) * 64 + lane];
+          bl[t] = Wl[((ks * 2 * LT + t) * 2 + 1) * 64 + lane];
+        }
+#pragma unroll
+        for (int d = 0; d < DT; ++d) {
+#pragma unroll
+          for (int p = 0; p < PT; ++p) {
+            float x[8];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const float2 a = f.t[d][j], q = f.p[p][j];
+              x[2 * j] = a.x * q.x - a.y * q.y;
+              x[2 * j + 1] = a.x * q.y + a.y * q.x;
+            }
+            bf16x8 ahi, alo;
+            split8(x, ahi, alo);
+#pragma unroll
+            for (int t = 0; t < 2 * LT; ++t) {
+              gacc[d][p][t] = mfma_bf16(ahi, bh[t], gacc[d][p][t]);
+              gacc[d][p][t] = mfma_bf16(ahi, bl[t], gacc[d][p][t]);
+              gacc[d][p][t] = mfma_bf16(alo, bh[t], gacc[d][p][t]);
+            }
+          }
+        }
+      }
+      load(ct2, ks2, f);
+    };
+
+    // two buffers, loaded two steps ahead; nks is even, so buffer a always
+    // holds the even steps and b the odd ones, across column tiles too
+    Frags<DT, PT> fa, fb;
+    load(0, 0, fa);
+    load(0, 1, fb);
+    for (int ct = 0; ct < nct; ++ct) {
+      const int ctn = min(ct + 1, nct - 1);
+#pragma unroll
+      for (int d = 0; d < DT; ++d)
+#pragma unroll
+        for (int p = 0; p < PT; ++p)
+#pragma unroll
+          for (int t = 0; t < 2 * LT; ++t) gacc[d][p][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int ks = 0; ks < nks; ks += 2) {
+        const bool wrap = ks + 2 >= nks;
+        step(ks, fa, wrap ? ctn : ct, wrap ? 0 : ks + 2);
+        step(ks + 1, fb, wrap ? ctn : ct, wrap ? 1 : ks + 3);
+      }
+      // stage 2: S += Re(E^T G) over this column tile
+      if constexpr (F32) {
+        float e[LT][8];
+#pragma unroll
+        for (int tx = 0; tx < LT; ++tx) {
+          const float4 e0 = Ef[((ct * LT + tx) * 64 + lane) * 2], e1 = Ef[((ct * LT + tx) * 64 + lane) * 2 + 1];
+          e[tx][0] = e0.x; e[tx][1] = e0.y; e[tx][2] = e0.z; e[tx][3] = e0.w;
+          e[tx][4] = e1.x; e[tx][5] = e1.y; e[tx][6] = e1.z; e[tx][7] = e1.w;
+        }
+#pragma unroll
+        for (int d = 0; d < DT; ++d)
+#pragma unroll
+          for (int p = 0; p < PT; ++p)
+#pragma unroll
+            for (int ty = 0; ty < LT; ++ty)
+#pragma unroll
+              for (int tx = 0; tx < LT; ++tx) {
+                floatx4 o = out[d][p][tx * LT + ty];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                  o = mfma_f32(e[tx][2 * v], gacc[d][p][2 * ty][v], o);
+                  o = mfma_f32(e[tx][2 * v + 1], gacc[d][p][2 * ty + 1][v], o);
+                }
+                out[d][p][tx * LT + ty] = o;
+              }
+        continue;
+      }
+      bf16x8 eh[LT], el[LT];
+#pragma unroll
+      for (int tx = 0; tx < LT; ++tx) {
+        eh[tx] = Et[((ct * LT + tx) * 2) * 64 + lane];
+        el[tx] = Et[((ct * LT + tx) * 2 + 1) * 64 + lane];
+      }
+#pragma unroll
+      for (int d = 0; d < DT; ++d) {
+#pragma unroll
+        for (int p = 0; p < PT; ++p) {
+#pragma unroll
+          for (int ty = 0; ty < LT; ++ty) {
+            float x[8];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              x[2 * v] = gacc[d][p][2 * ty][v];
+              x[2 * v + 1] = gacc[d][p][2 * ty + 1][v];
+            }
+            bf16x8 gh, gl;
+            split8(x, gh, gl);
+#pragma unroll
+            for (int tx = 0; tx < LT; ++tx) {
+              floatx4 o = out[d][p][tx * LT + ty];
+              o = mfma_bf16(eh[tx], gh, o);
+              o = mfma_bf16(eh[tx], gl, o);
+              o = mfma_bf16(el[tx], gh, o);
+              out[d][p][tx * LT + ty] = o;
+            }
+          }
+        }
+      }
+    }
+
+    if (lone) {
+      // The last column c = m - 1 of the wave's pairs, sixteen pairs per tile:
+      // M slot i of group q holds pair k = 16 q + i = d * PT + p (slots past the
+      // last pair hold a copy of pair 0, never read).  Stage 1 as above on
+      // split-bf16 MFMA; stage 2 is the rank-one update
+      // S[x][y] += Re(E[c][x] G[c][y]) in FP32, G[c][y] of pair k fetched from
+      // lane ((k % 16) / 4, y) of the group's accumulator.
+      // E[m - 1][x] for this lane's lags x = 16 tx + 4g + v, in float64 -> float
+      float er[LT][4], ei[LT][4];
+      {
+        const int sh = (m - 1 + m / 2) % m;
+        const double w = (sh == 0 || sh == n / 2) ? 1.0 : 2.0;
+#pragma unroll
+        for (int tx = 0; tx < LT; ++tx)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int x = 16 * tx + 4 * g + v;
+            double sn = 0.0, cs = 0.0;
+            if (x < C) {
+              const long kk = ((long)sh * crop_lag(x, n, C)) % n;
+              sincospi(2.0 * (double)kk / (double)n, &sn, &cs);
+            }
+            er[tx][v] = (float)(w * cs);
+            ei[tx][v] = (float)(w * sn);
+          }
+      }
+      constexpr int NG = (DT * PT + 15) / 16;
+#pragma unroll
+      for (int q = 0; q < NG; ++q) {
+        const int k = 16 * q + i < DT * PT ? 16 * q + i : 0;
+        const float2* lt = tp[0];
+        const float2* lq = pp[0];
+#pragma unroll
+        for (int d = 1; d < DT; ++d)
+          if (k / PT == d) lt = tp[d];
+#pragma unroll
+        for (int p = 1; p < PT; ++p)
+          if (k % PT == p) lq = pp[p];
+        floatx4 gn[2 * LT];
+#pragma unroll
+        for (int t = 0; t < 2 * LT; ++t) gn[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        float2 na[4], nq[4];
+        auto nload = [&](int ks) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int r = 16 * ks + 4 * g + j;
+            const int lone_at = nks * nct * 256 + 16 * ks + 4 * g + j;  // fragment order: the lone column
+            na[j] = FR ? lt[lone_at] : lt[min(r * ldt + (m - 1), lastt)];
+            nq[j] = FR ? lq[lone_at] : lq[min(r * ldp + (m - 1), lastp)];
+          }
+        };
+        nload(0);
+        for (int ks = 0; ks < nks; ++ks) {
+          float x[8];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            x[2 * j] = na[j].x * nq[j].x - na[j].y * nq[j].y;
+            x[2 * j + 1] = na[j].x * nq[j].y + na[j].y * nq[j].x;
+          }
+          if (ks + 1 < nks) nload(ks + 1);
+          if constexpr (F32) {
+#pragma unroll
+            for (int t = 0; t < 2 * LT; ++t) {
+              const float4 w0 = Wf[((ks * 2 * LT + t) * 64 + lane) * 2];
+              const float4 w1 = Wf[((ks * 2 * LT + t) * 64 + lane) * 2 + 1];
+              const float w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+              for (int kk = 0; kk < 8; ++kk) gn[t] = mfma_f32(x[kk], w[kk], gn[t]);
+            }
+          } else {
+            bf16x8 ahi, alo;
+            split8(x, ahi, alo);
+#pragma unroll
+            for (int t = 0; t < 2 * LT; ++t) {
+              const bf16x8 bh = Wl[((ks * 2 * LT + t) * 2) * 64 + lane];
+              const bf16x8 bl = Wl[((ks * 2 * LT + t) * 2 + 1) * 64 + lane];
+              gn[t] = mfma_bf16(ahi, bh, gn[t]);
+              gn[t] = mfma_bf16(ahi, bl, gn[t]);
+              gn[t] = mfma_bf16(alo, bh, gn[t]);
+            }
+          }
+        }
+#pragma unroll
+        for (int d = 0; d < DT; ++d)
+#pragma unroll
+          for (int p = 0; p < PT; ++p) {
+            const int kp = d * PT + p;
+            if (kp / 16 != q) continue;  // compile-time after unrolling
+            const int src = ((kp & 15) >> 2) * 16 + i;
+#pragma unroll
+            for (int ty = 0; ty < LT; ++ty) {
+              const float gr = __shfl(gn[2 * ty][kp & 3], src, 64);
+              const float gi = __shfl(gn[2 * ty + 1][kp & 3], src, 64);
+#pragma unroll
+              for (int tx = 0; tx < LT; ++tx)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                  out[d][p][tx * LT + ty][v] += er[tx][v] * gr - ei[tx][v] * gi;
+            }
+          }
+      }
+    }
+
+    // pair tail: first maximum over the crop in y * C + x order; lane (g, i)
+    // holds S[x = 16 tx + 4g + v][y = 16 ty + i]
+#pragma unroll
+    for (int d = 0; d < DT; ++d) {
+#pragma unroll
+      for (int p = 0; p < PT; ++p) {
+        if (j0 + d >= Mi || p0 + p >= P) continue;  // wave-uniform
+        const int dir = dir_idx[d0 + j0 + d];
+        const float sc = 1024.0f * (inv_norm ? inv_norm[(size_t)pb * P + p0 + p] : 1.0f) * (tinv ? tinv[dir] : 1.0f);
+        float bv = -INFINITY;
+        int bi = 0x7fffffff;
+#pragma unroll
+        for (int tx = 0; tx < LT; ++tx)
+#pragma unroll
+          for (int ty = 0; ty < LT; ++ty)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int x = 16 * tx + 4 * g + v, y = 16 * ty + i;
+              if (y < C && x < C) {
+                const float val = out[d][p][tx * LT + ty][v] * sc;
+                const int idx = y * C + x;
+                if (val > bv || (val == bv && idx < bi)) { bv = val; bi = idx; }
+              }
+            }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          const float ov = __shfl_xor(bv, o, 64);
+          const int oi = __shfl_xor(bi, o, 64);
+          if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        }
+        if (lane == 0) {
+          const size_t gidx = (size_t)(d0 + j0 + d) * P + p0 + p;
+          scores[gidx] = bv;
+          peak_y[gidx] = bi / C;
+          peak_x[gidx] = bi % C;
+        }
+      }
+    }
+    if constexpr (!LIST) break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP16-operand variant of the search.  The psi spectra are stored as fp16
+// pairs, normalised and scaled by 2^7 (pack_fragments16_kernel), in the fp16
+// fragment order: tile value (row 16 ks + 4g + j, column 16 ct + i) at
+// tile * 256 + 4 (16 g + i) + j, one 16-byte run per lane and tile (the lone
+// last column as in fragment order).  The template rows stay float32 in
+// fragment order; each step scales a lane's four values by 2^7 / norm and
+// rounds them to fp16.  Q' = T' P' (= 2^14 Q / (|T| |P|)) is formed with
+// packed fp16 multiply-adds and is the stage-1 A operand as it is; the row
+// table W is split into fp16 hi + lo (WS: two MFMAs per tile) or used as fp16
+// hi alone.  Stage 2 and the pair tail are those of the split-bf16 search;
+// the pair score is 1024 * 2^-14 * S'.
+// ---------------------------------------------------------------------------
+constexpr float kH16Scale = 128.0f;          // 2^7 per operand
+constexpr float kH16Unscale = 1024.0f / 16384.0f;  // 1024 * 2^-14
+
+__device__ __forceinline__ floatx4 mfma_f16(f16x8 a, f16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// s t as the fp16 pairs th = (tr, ti) and tn = (-ti, tr) (i s t)
+__device__ __forceinline__ void t_pair(float2 t, float s, f16x2& th, f16x2& tn) {
+  const float tr = t.x * s, ti = t.y * s;
+  const f32x2 v0 = {tr, ti}, v1 = {-ti, tr};
+  th = __builtin_convertvector(v0, f16x2);
+  tn = __builtin_convertvector(v1, f16x2);
+}
+
+// t (pr + i pi) = pr t + pi (i t): one packed fp16 multiply (pi (i t),
+// rounded to fp16) and one packed fused multiply-add, in this order whatever
+// the tile shape (bit-identical results across the kernel's variants)
+__device__ __forceinline__ f16x2 cmul_h(f16x2 th, f16x2 tn, _Float16 pr, _Float16 pi) {
+  const f16x2 r = {pr, pr}, i = {pi, pi};
+  return __builtin_elementwise_fma(r, th, i * tn);
+}
+
+__device__ __forceinline__ f16x2 to_h2(float2 v, float s) {
+  const f32x2 x = {v.x * s, v.y * s};
+  return __builtin_convertvector(x, f16x2);
+}
+
+// dst [rows][frag_plane(n)] fp16 pairs: src [rows][frag_plane(n)] complex64
+// (fragment order) times scale * inv_norm[row] (inv_norm may be NULL), in the
+// fp16 fragment order.
+__global__ void pack_fragments16_kernel(const float2* __restrict__ src, const float* __restrict__ inv_norm, int rows,
+                                        int n, float scale, f16x2* __restrict__ dst) {
+  const int F = frag_plane(n), tiles = frag_nks(n) * frag_nct(n) * 256;
+  const size_t total = (size_t)rows * F;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t row = t / F;
+    const int e = (int)(t - row * F);
+    int s = e;
+    if (e < tiles) {
+      const int w = e & 255, lane = w >> 2, j = w & 3;
+      s = (e & ~255) + (j >> 1) * 128 + 2 * lane + (j & 1);
+    }
+    const float sc = scale * (inv_norm ? inv_norm[row] : 1.0f);
+    dst[t] = to_h2(src[row * F + s], sc);
+  }
+}
+
+// Near-tie selection after an fp16-operand search: particle b's pairs whose
+// score is within max(rel |peak_b|, abs_margin) of its best go on the list
+// (b, pair index) for the split-bf16 rescoring; the best pair always does.
+__global__ void __launch_bounds__(64)
+split_select_kernel(const int* __restrict__ dir_off, int P, const float* __restrict__ scores,
+                    const int* __restrict__ best, const float* __restrict__ peak, float rel, float abs_margin,
+                    int2* __restrict__ list, int* __restrict__ count) {
+  const int b = blockIdx.x, lane = threadIdx.x;
+  if (best[b] < 0) return;
+  const float pk = peak[b];
+  const float thr = pk - fmaxf(rel * fabsf(pk), abs_margin);
+  const int d0 = dir_off[b], np = (dir_off[b + 1] - d0) * P;
+  const float* s = scores + (size_t)d0 * P;
+  for (int k = lane; k < np; k += 64)
+    if (s[k] >= thr) list[atomicAdd(count, 1)] = make_int2(b, k);
+}
+
+template <int DT, int PT>
+struct Frags16 {
+  float2 t[DT][4];  // float32 template values (T16 = false)
+  f16x8 t16[DT];    // fp16 template pairs (T16 = true)
+  f16x8 p[PT];      // four fp16 pairs: rows 16 ks + 4g + j, j = 0..3
+};
+
+// i t = (-ti, tr) of an fp16 pair t = (tr, ti): one packed multiply
+__device__ __forceinline__ f16x2 times_i(f16x2 t) {
+  const f16x2 sw = {t[1], t[0]};
+  const f16x2 ng = {(_Float16)-1.0f, (_Float16)1.0f};
+  return sw * ng;
+}
+
+// T16: the bank is fp16 pairs too (gis_pack_fragments_f16 of the float32
+// bank with scale 128 and the rows' 1/norms), read like the spectra.
+template <int LT, int DT, int PT, bool WS, bool T16>
+__global__ void __launch_bounds__(64 * kWaves)
+// one direction per wave fits 168 registers: three waves per SIMD
+__attribute__((amdgpu_waves_per_eu(DT * PT * LT <= 5 ? 3 : 1)))
+split16_correlate_kernel(const void* __restrict__ bankv, const f16x2* __restrict__ spectra,
+                         const float* __restrict__ tinv, const int* __restrict__ dir_off,
+                         const int* __restrict__ dir_idx, const int* __restrict__ pmap,
+                         const int* __restrict__ order, const int* __restrict__ unit_off, int B, int P, int n, int C,
+                         int nblk, const void* __restrict__ Wtv, const void* __restrict__ Etv,
+                         float* __restrict__ scores, int* __restrict__ peak_y, int* __restrict__ peak_x) {
+  // row table in LDS: [ks][2 LT tiles][hi, lo][64 lanes] (WS) or hi alone
+  extern __shared__ f16x8 tab16[];
+  const f16x8* Wt = (const f16x8*)Wtv;
+  const bf16x8* Et = (const bf16x8*)Etv;
+  const int m = n / 2 + 1;
+  const bool lone = (m & 15) == 1;
+  const int nct = lone ? (m >> 4) : (m + 15) >> 4;
+  const int nks = (((n + 15) >> 4) + 1) & ~1;
+  constexpr int NH = WS ? 2 : 1;
+  const int nw = nks * 2 * LT * NH * 64;
+  for (int e = threadIdx.x; e < nw; e += 64 * kWaves) {
+    const int lane = e & 63, f = e >> 6;  // f = (ks * 2 LT + t) * NH + h
+    tab16[e] = Wt[((f / NH) * 2 + (f % NH)) * 64 + lane];
+  }
+  __syncthreads();
+
+  const int per = (int)(gridDim.x / kXcds);
+  const int lb = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  const int lane = threadIdx.x & 63;
   const int u = __builtin_amdgcn_readfirstlane(lb * kWaves + (int)(threadIdx.x >> 6));
   const int total = unit_off[B];
   if (lb >= nblk || u >= total) return;  // no barrier below
-  int lo = 0, hi = B;  // unit_off[lo] <= u < unit_off[hi]
+  int lo = 0, hi = B;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
     if (unit_off[mid] <= u) lo = mid; else hi = mid;
@@ -263,64 +789,39 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
   const int j0 = (u - unit_off[lo]) * DT;
   const int p0 = (int)blockIdx.y * PT;
   const int pb = pmap ? pmap[b] : b;
-  // template rows are ldt complex values apart, psi spectrum rows ldp
-  // (ld > n/2+1: zero padding, so 16-column segments start on 128-byte
-  // lines); ld = 0: the operand is in fragment order (frag_plane), each
-  // step's tile two contiguous 1 KiB runs read with 16-byte loads
   const size_t fplane = (size_t)frag_plane(n);
   const float2* tp[DT];
-  const float2* pp[PT];
+  const f16x2* tp16[DT];
+  float ts[DT];
+  const f16x2* pp[PT];
 #pragma unroll
-  for (int d = 0; d < DT; ++d)
-    tp[d] = bank + (size_t)dir_idx[d0 + min(j0 + d, Mi - 1)] * (FR ? fplane : (size_t)n * ldt);
+  for (int d = 0; d < DT; ++d) {
+    const int dir = dir_idx[d0 + min(j0 + d, Mi - 1)];
+    tp[d] = (const float2*)bankv + (size_t)dir * fplane;
+    tp16[d] = (const f16x2*)bankv + (size_t)dir * fplane;
+    ts[d] = kH16Scale * (tinv && !T16 ? tinv[dir] : 1.0f);
+  }
 #pragma unroll
-  for (int p = 0; p < PT; ++p)
-    pp[p] = spectra + ((size_t)pb * P + min(p0 + p, P - 1)) * (FR ? fplane : (size_t)n * ldp);
+  for (int p = 0; p < PT; ++p) pp[p] = spectra + ((size_t)pb * P + min(p0 + p, P - 1)) * fplane;
   const int g = lane >> 4, i = lane & 15;
 
-  // Operands of step (ct, ks).  Loads are unconditional: an offset past the
-  // plane (rows >= n) is clamped to its last element, columns >= n/2+1 read
-  // the next row or the zero padding, and the tables hold zeros at those rows
-  // and columns, so the (finite) values read there contribute exactly
-  // nothing.
-  const int lastt = n * ldt - 1, lastp = n * ldp - 1;
-  auto load = [&](int ct, int ks, Frags<DT, PT>& f) {
-    const int r0 = 16 * ks + 4 * g, c = 16 * ct + i;
-    const int tile = (ks * nct + ct) * 256 + 2 * lane;  // fragment order: [h][lane][2 rows]
-    if constexpr (FR) {
+  auto load = [&](int ct, int ks, Frags16<DT, PT>& f) {
+    const int tile = (ks * nct + ct) * 256;
 #pragma unroll
-      for (int d = 0; d < DT; ++d) {
-        const float4 q0 = *(const float4*)(tp[d] + tile), q1 = *(const float4*)(tp[d] + tile + 128);
+    for (int d = 0; d < DT; ++d) {
+      if constexpr (T16) {
+        f.t16[d] = *(const f16x8*)(tp16[d] + tile + 4 * lane);
+      } else {
+        const float4 q0 = *(const float4*)(tp[d] + tile + 2 * lane);
+        const float4 q1 = *(const float4*)(tp[d] + tile + 128 + 2 * lane);
         f.t[d][0] = make_float2(q0.x, q0.y);
         f.t[d][1] = make_float2(q0.z, q0.w);
         f.t[d][2] = make_float2(q1.x, q1.y);
         f.t[d][3] = make_float2(q1.z, q1.w);
       }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int ot = min((r0 + j) * ldt + c, lastt);
-#pragma unroll
-        for (int d = 0; d < DT; ++d) f.t[d][j] = tp[d][ot];
-      }
     }
-    if constexpr (FR) {
 #pragma unroll
-      for (int p = 0; p < PT; ++p) {
-        const float4 q0 = *(const float4*)(pp[p] + tile), q1 = *(const float4*)(pp[p] + tile + 128);
-        f.p[p][0] = make_float2(q0.x, q0.y);
-        f.p[p][1] = make_float2(q0.z, q0.w);
-        f.p[p][2] = make_float2(q1.x, q1.y);
-        f.p[p][3] = make_float2(q1.z, q1.w);
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int op = min((r0 + j) * ldp + c, lastp);
-#pragma unroll
-        for (int p = 0; p < PT; ++p) f.p[p][j] = pp[p][op];
-      }
-    }
+    for (int p = 0; p < PT; ++p) f.p[p] = *(const f16x8*)(pp[p] + tile + 4 * lane);
   };
 
   floatx4 out[DT][PT][LT * LT], gacc[DT][PT][2 * LT];
@@ -331,110 +832,67 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
 #pragma unroll
       for (int t = 0; t < LT * LT; ++t) out[d][p][t] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  // stage 1 of one step from buffer f, then f is refilled with step (ct2, ks2)
-  auto step = [&](int ks, Frags<DT, PT>& f, int ct2, int ks2) {
-    if constexpr (F32) {
-      float w[2 * LT][8];
+  auto step = [&](int ks, Frags16<DT, PT>& f, int ct2, int ks2, bool first) {
+    f16x8 wh[2 * LT], wl[2 * LT];
 #pragma unroll
-      for (int t = 0; t < 2 * LT; ++t) {
-        const float4 w0 = Wf[((ks * 2 * LT + t) * 64 + lane) * 2], w1 = Wf[((ks * 2 * LT + t) * 64 + lane) * 2 + 1];
-        w[t][0] = w0.x; w[t][1] = w0.y; w[t][2] = w0.z; w[t][3] = w0.w;
-        w[t][4] = w1.x; w[t][5] = w1.y; w[t][6] = w1.z; w[t][7] = w1.w;
-      }
+    for (int t = 0; t < 2 * LT; ++t) {
+      wh[t] = tab16[((ks * 2 * LT + t) * NH) * 64 + lane];
+      if constexpr (WS) wl[t] = tab16[((ks * 2 * LT + t) * NH + 1) * 64 + lane];
+    }
 #pragma unroll
-      for (int d = 0; d < DT; ++d) {
+    for (int d = 0; d < DT; ++d) {
+      // the template values as fp16 (Tr, Ti) and (-Ti, Tr): Q = Pr T + Pi (i T)
+      f16x2 th[4], tn[4];
 #pragma unroll
-        for (int p = 0; p < PT; ++p) {
-          float x[8];
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const float2 a = f.t[d][j], q = f.p[p][j];
-            x[2 * j] = a.x * q.x - a.y * q.y;
-            x[2 * j + 1] = a.x * q.y + a.y * q.x;
-          }
-#pragma unroll
-          for (int t = 0; t < 2 * LT; ++t)
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk) gacc[d][p][t] = mfma_f32(x[kk], w[t][kk], gacc[d][p][t]);
+      for (int j = 0; j < 4; ++j) {
+        if constexpr (T16) {
+          th[j] = f16x2{f.t16[d][2 * j], f.t16[d][2 * j + 1]};
+          tn[j] = times_i(th[j]);
+        } else {
+          t_pair(f.t[d][j], ts[d], th[j], tn[j]);
         }
       }
-    } else {
-      bf16x8 bh[2 * LT], bl[2 * LT];
 #pragma unroll
-      for (int t = 0; t < 2 * LT; ++t) {
-        bh[t] = Wl[((ks * 2 * LT + t) * 2) * 64 + lane];
-        bl[t] = Wl[((ks * 2 * LT + t) * 2 + 1) * 64 + lane];
-      }
+      for (int p = 0; p < PT; ++p) {
+        f16x8 a;
 #pragma unroll
-      for (int d = 0; d < DT; ++d) {
+        for (int j = 0; j < 4; ++j) {
+          const f16x2 q = cmul_h(th[j], tn[j], f.p[p][2 * j], f.p[p][2 * j + 1]);
+          a[2 * j] = q[0];
+          a[2 * j + 1] = q[1];
+        }
+        // the first step of a column tile starts the accumulators from zero
+        // (an explicit operand rather than a cleared register keeps the
+        // accumulators in place across the loop)
 #pragma unroll
-        for (int p = 0; p < PT; ++p) {
-          float x[8];
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const float2 a = f.t[d][j], q = f.p[p][j];
-            x[2 * j] = a.x * q.x - a.y * q.y;
-            x[2 * j + 1] = a.x * q.y + a.y * q.x;
-          }
-          bf16x8 ahi, alo;
-          split8(x, ahi, alo);
-#pragma unroll
-          for (int t = 0; t < 2 * LT; ++t) {
-            gacc[d][p][t] = mfma_bf16(ahi, bh[t], gacc[d][p][t]);
-            gacc[d][p][t] = mfma_bf16(ahi, bl[t], gacc[d][p][t]);
-            gacc[d][p][t] = mfma_bf16(alo, bh[t], gacc[d][p][t]);
-          }
+        for (int t = 0; t < 2 * LT; ++t) {
+          gacc[d][p][t] = mfma_f16(a, wh[t], first ? floatx4{0.f, 0.f, 0.f, 0.f} : gacc[d][p][t]);
+          if constexpr (WS) gacc[d][p][t] = mfma_f16(a, wl[t], gacc[d][p][t]);
         }
       }
     }
     load(ct2, ks2, f);
   };
 
-  // two buffers, loaded two steps ahead; nks is even, so buffer a always
-  // holds the even steps and b the odd ones, across column tiles too
-  Frags<DT, PT> fa, fb;
+  Frags16<DT, PT> fa, fb;
   load(0, 0, fa);
   load(0, 1, fb);
   for (int ct = 0; ct < nct; ++ct) {
     const int ctn = min(ct + 1, nct - 1);
-#pragma unroll
-    for (int d = 0; d < DT; ++d)
-#pragma unroll
-      for (int p = 0; p < PT; ++p)
-#pragma unroll
-        for (int t = 0; t < 2 * LT; ++t) gacc[d][p][t] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < nks; ks += 2) {
+    {
+      const bool wrap = 2 >= nks;  // nks >= 2 and even
+      step(0, fa, wrap ? ctn : ct, wrap ? 0 : 2, true);
+      step(1, fb, wrap ? ctn : ct, wrap ? 1 : 3, false);
+    }
+    for (int ks = 2; ks < nks; ks += 2) {
       const bool wrap = ks + 2 >= nks;
-      step(ks, fa, wrap ? ctn : ct, wrap ? 0 : ks + 2);
-      step(ks + 1, fb, wrap ? ctn : ct, wrap ? 1 : ks + 3);
+      step(ks, fa, wrap ? ctn : ct, wrap ? 0 : ks + 2, false);
+      step(ks + 1, fb, wrap ? ctn : ct, wrap ? 1 : ks + 3, false);
+      // keeps the scheduler from moving work across the loop edge, which
+      // otherwise makes the register allocator copy every accumulator per trip
+      __builtin_amdgcn_sched_barrier(0);
     }
-    // stage 2: S += Re(E^T G) over this column tile
-    if constexpr (F32) {
-      float e[LT][8];
-#pragma unroll
-      for (int tx = 0; tx < LT; ++tx) {
-        const float4 e0 = Ef[((ct * LT + tx) * 64 + lane) * 2], e1 = Ef[((ct * LT + tx) * 64 + lane) * 2 + 1];
-        e[tx][0] = e0.x; e[tx][1] = e0.y; e[tx][2] = e0.z; e[tx][3] = e0.w;
-        e[tx][4] = e1.x; e[tx][5] = e1.y; e[tx][6] = e1.z; e[tx][7] = e1.w;
-      }
-#pragma unroll
-      for (int d = 0; d < DT; ++d)
-#pragma unroll
-        for (int p = 0; p < PT; ++p)
-#pragma unroll
-          for (int ty = 0; ty < LT; ++ty)
-#pragma unroll
-            for (int tx = 0; tx < LT; ++tx) {
-              floatx4 o = out[d][p][tx * LT + ty];
-#pragma unroll
-              for (int v = 0; v < 4; ++v) {
-                o = mfma_f32(e[tx][2 * v], gacc[d][p][2 * ty][v], o);
-                o = mfma_f32(e[tx][2 * v + 1], gacc[d][p][2 * ty + 1][v], o);
-              }
-              out[d][p][tx * LT + ty] = o;
-            }
-      continue;
-    }
+    // stage 2 (split-bf16, as in split_correlate_kernel)
     bf16x8 eh[LT], el[LT];
 #pragma unroll
     for (int tx = 0; tx < LT; ++tx) {
@@ -469,13 +927,9 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
   }
 
   if (lone) {
-    // The last column c = m - 1 of the wave's pairs, sixteen pairs per tile:
-    // M slot i of group q holds pair k = 16 q + i = d * PT + p (slots past the
-    // last pair hold a copy of pair 0, never read).  Stage 1 as above on
-    // split-bf16 MFMA; stage 2 is the rank-one update
-    // S[x][y] += Re(E[c][x] G[c][y]) in FP32, G[c][y] of pair k fetched from
-    // lane ((k % 16) / 4, y) of the group's accumulator.
-    // E[m - 1][x] for this lane's lags x = 16 tx + 4g + v, in float64 -> float
+    // the last column c = m - 1 of the wave's pairs, sixteen pairs per tile
+    // (as in split_correlate_kernel): stage 1 on fp16 MFMA, stage 2 the
+    // rank-one update in FP32
     float er[LT][4], ei[LT][4];
     {
       const int sh = (m - 1 + m / 2) % m;
@@ -499,55 +953,49 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
     for (int q = 0; q < NG; ++q) {
       const int k = 16 * q + i < DT * PT ? 16 * q + i : 0;
       const float2* lt = tp[0];
-      const float2* lq = pp[0];
+      const f16x2* lt16 = tp16[0];
+      float lts = ts[0];
+      const f16x2* lq = pp[0];
 #pragma unroll
       for (int d = 1; d < DT; ++d)
-        if (k / PT == d) lt = tp[d];
+        if (k / PT == d) { lt = tp[d]; lt16 = tp16[d]; lts = ts[d]; }
 #pragma unroll
       for (int p = 1; p < PT; ++p)
         if (k % PT == p) lq = pp[p];
       floatx4 gn[2 * LT];
 #pragma unroll
       for (int t = 0; t < 2 * LT; ++t) gn[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-      float2 na[4], nq[4];
+      float2 na[4];
+      f16x2 na16[4], nq[4];
       auto nload = [&](int ks) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const int r = 16 * ks + 4 * g + j;
-          const int lone_at = nks * nct * 256 + 16 * ks + 4 * g + j;  // fragment order: the lone column
-          na[j] = FR ? lt[lone_at] : lt[min(r * ldt + (m - 1), lastt)];
-          nq[j] = FR ? lq[lone_at] : lq[min(r * ldp + (m - 1), lastp)];
+          const int lone_at = nks * nct * 256 + 16 * ks + 4 * g + j;
+          if constexpr (T16) na16[j] = lt16[lone_at]; else na[j] = lt[lone_at];
+          nq[j] = lq[lone_at];
         }
       };
       nload(0);
       for (int ks = 0; ks < nks; ++ks) {
-        float x[8];
+        f16x8 a;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          x[2 * j] = na[j].x * nq[j].x - na[j].y * nq[j].y;
-          x[2 * j + 1] = na[j].x * nq[j].y + na[j].y * nq[j].x;
+          f16x2 th, tn;
+          if constexpr (T16) {
+            th = na16[j];
+            tn = times_i(th);
+          } else {
+            t_pair(na[j], lts, th, tn);
+          }
+          const f16x2 qv = cmul_h(th, tn, nq[j][0], nq[j][1]);
+          a[2 * j] = qv[0];
+          a[2 * j + 1] = qv[1];
         }
         if (ks + 1 < nks) nload(ks + 1);
-        if constexpr (F32) {
 #pragma unroll
-          for (int t = 0; t < 2 * LT; ++t) {
-            const float4 w0 = Wf[((ks * 2 * LT + t) * 64 + lane) * 2];
-            const float4 w1 = Wf[((ks * 2 * LT + t) * 64 + lane) * 2 + 1];
-            const float w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk) gn[t] = mfma_f32(x[kk], w[kk], gn[t]);
-          }
-        } else {
-          bf16x8 ahi, alo;
-          split8(x, ahi, alo);
-#pragma unroll
-          for (int t = 0; t < 2 * LT; ++t) {
-            const bf16x8 bh = Wl[((ks * 2 * LT + t) * 2) * 64 + lane];
-            const bf16x8 bl = Wl[((ks * 2 * LT + t) * 2 + 1) * 64 + lane];
-            gn[t] = mfma_bf16(ahi, bh, gn[t]);
-            gn[t] = mfma_bf16(ahi, bl, gn[t]);
-            gn[t] = mfma_bf16(alo, bh, gn[t]);
-          }
+        for (int t = 0; t < 2 * LT; ++t) {
+          gn[t] = mfma_f16(a, tab16[((ks * 2 * LT + t) * NH) * 64 + lane], gn[t]);
+          if constexpr (WS) gn[t] = mfma_f16(a, tab16[((ks * 2 * LT + t) * NH + 1) * 64 + lane], gn[t]);
         }
       }
 #pragma unroll
@@ -571,15 +1019,12 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
     }
   }
 
-  // pair tail: first maximum over the crop in y * C + x order; lane (g, i)
-  // holds S[x = 16 tx + 4g + v][y = 16 ty + i]
+  // pair tail: first maximum over the crop in y * C + x order
 #pragma unroll
   for (int d = 0; d < DT; ++d) {
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
       if (j0 + d >= Mi || p0 + p >= P) continue;  // wave-uniform
-      const int dir = dir_idx[d0 + j0 + d];
-      const float sc = 1024.0f * (inv_norm ? inv_norm[(size_t)pb * P + p0 + p] : 1.0f) * (tinv ? tinv[dir] : 1.0f);
       float bv = -INFINITY;
       int bi = 0x7fffffff;
 #pragma unroll
@@ -590,7 +1035,7 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
           for (int v = 0; v < 4; ++v) {
             const int x = 16 * tx + 4 * g + v, y = 16 * ty + i;
             if (y < C && x < C) {
-              const float val = out[d][p][tx * LT + ty][v] * sc;
+              const float val = out[d][p][tx * LT + ty][v] * kH16Unscale;
               const int idx = y * C + x;
               if (val > bv || (val == bv && idx < bi)) { bv = val; bi = idx; }
             }
@@ -619,14 +1064,15 @@ inline size_t split_e_frags(int n, int C) { return (size_t)split_nct(n) * 2 * sp
 inline size_t split_lds_bytes(int n, int C) { return split_w_frags(n, C) * sizeof(bf16x8); }
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-template <int LT, int DT, int PT, bool F32, bool FR>
+template <int LT, int DT, int PT, bool F32, bool FR, bool LIST = false>
 int launch_split(dim3 grid, size_t lds, hipStream_t st, const float2* bank, const float2* spectra,
                  const float* inv_norm, const float* tinv, const int* dir_off, const int* dir_idx, const int* pmap,
                  const int* order, const int* unit_off, int B, int P, int n, int C, int ldt, int ldp, int nblk,
-                 const void* Wt, const void* Et, float* scores, int* peak_y, int* peak_x) {
+                 const void* Wt, const void* Et, float* scores, int* peak_y, int* peak_x,
+                 const int2* list = nullptr, const int* list_count = nullptr) {
   static bool attr_set = false;
   if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)split_correlate_kernel<LT, DT, PT, F32, FR>,
+    hipError_t e = hipFuncSetAttribute((const void*)split_correlate_kernel<LT, DT, PT, F32, FR, LIST>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)split_lds_bytes(256, LT == 1 ? 16 : 32));
     if (e != hipSuccess) {
@@ -636,9 +1082,32 @@ int launch_split(dim3 grid, size_t lds, hipStream_t st, const float2* bank, cons
     }
     attr_set = true;
   }
-  hipLaunchKernelGGL((split_correlate_kernel<LT, DT, PT, F32, FR>), grid, dim3(64 * kWaves), lds, st, bank, spectra,
-                     inv_norm, tinv, dir_off, dir_idx, pmap, order, unit_off, B, P, n, C, ldt, ldp, nblk, Wt, Et, scores,
-                     peak_y, peak_x);
+  hipLaunchKernelGGL((split_correlate_kernel<LT, DT, PT, F32, FR, LIST>), grid, dim3(64 * kWaves), lds, st, bank,
+                     spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order, unit_off, B, P, n, C, ldt, ldp, nblk, Wt,
+                     Et, scores, peak_y, peak_x, list, list_count);
+  return GIS_OK;
+}
+
+template <int LT, int DT, int PT, bool WS, bool T16>
+int launch_split16(dim3 grid, size_t lds, hipStream_t st, const void* bank, const f16x2* spectra, const float* tinv,
+                   const int* dir_off, const int* dir_idx, const int* pmap, const int* order, const int* unit_off,
+                   int B, int P, int n, int C, int nblk, const void* Wt, const void* Et, float* scores, int* peak_y,
+                   int* peak_x) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)split16_correlate_kernel<LT, DT, PT, WS, T16>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)split_lds_bytes(256, LT == 1 ? 16 : 32));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      gis_set_error("gis_correlate_split_f16: cannot reserve LDS: %s", hipGetErrorString(e));
+      return (int)e;
+    }
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((split16_correlate_kernel<LT, DT, PT, WS, T16>), grid, dim3(64 * kWaves), lds, st, bank,
+                     spectra, tinv, dir_off, dir_idx, pmap, order, unit_off, B, P, n, C, nblk, Wt, Et, scores, peak_y,
+                     peak_x);
   return GIS_OK;
 }
 
@@ -660,6 +1129,25 @@ GIS_API int gis_pack_fragments(const void* src, int rows, int n, int ld, void* d
   hipLaunchKernelGGL(pack_fragments_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const float2*)src, rows,
                      n, ld, (float2*)dst);
   GIS_LAUNCH_CHECK("gis_pack_fragments");
+  return GIS_OK;
+}
+
+// dst [rows][gis_fragment_plane(n)] fp16 pairs (real, imaginary): the
+// fragment-order planes src [rows][gis_fragment_plane(n)] complex64 times
+// scale * inv_norm[row] (inv_norm may be NULL), in the fp16 fragment order
+// gis_correlate_split_f16 reads (tile value (row 16 ks + 4g + j, column
+// 16 ct + i) at tile * 256 + 4 (16 g + i) + j).
+GIS_API int gis_pack_fragments_f16(const void* src, const float* inv_norm, int rows, int n, float scale, void* dst,
+                                   void* stream) {
+  GIS_REQUIRE(rows >= 0 && n >= 2 && n % 2 == 0 && scale > 0.0f, "gis_pack_fragments_f16: rows %d n %d scale %g",
+              rows, n, (double)scale);
+  if (rows == 0) return GIS_OK;
+  GIS_REQUIRE(src && dst, "gis_pack_fragments_f16: null pointer");
+  const size_t total = (size_t)rows * frag_plane(n);
+  const int blocks = (int)std::min<size_t>((total + 255) / 256, (size_t)gis_cu_count() * 32);
+  hipLaunchKernelGGL(pack_fragments16_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const float2*)src,
+                     inv_norm, rows, n, scale, (f16x2*)dst);
+  GIS_LAUNCH_CHECK("gis_pack_fragments_f16");
   return GIS_OK;
 }
 
@@ -701,7 +1189,7 @@ int split_run(const void* bank, const void* spectra, const float* inv_norm, cons
   bf16x8* Et = Wt + split_w_frags(n, C);
   int* unit_off = (int*)((char*)ws + align256((split_w_frags(n, C) + split_e_frags(n, C)) * sizeof(bf16x8)));
   hipLaunchKernelGGL(split_tables_kernel, dim3(64), dim3(256), 0, st, n, C, LT, split_nks(n), split_nct(n),
-                     f32 ? 1 : 0, (void*)Wt, (void*)Et);
+                     f32 ? kTabF32 : kTabBF16, (void*)Wt, (void*)Et);
   GIS_LAUNCH_CHECK("gis_correlate_split tables");
   hipLaunchKernelGGL(split_units_kernel, dim3(1), dim3(1024), 0, st, dir_off, order, B, dt, unit_off);
   GIS_LAUNCH_CHECK("gis_correlate_split units");
@@ -759,6 +1247,164 @@ int split_run(const void* bank, const void* spectra, const float* inv_norm, cons
 }
 
 }  // namespace
+
+namespace {
+
+// Near-tie margin of the refined fp16 search (rawCC units): a pair is
+// rescored on split-bf16 MFMA when its fp16 score is within
+// max(kRefineRel |best|, kRefineAbs / sqrt(n (n/2+1))) of its particle's best
+// fp16 score.  The fp16 products carry ~11 significant bits: at box 256 the
+// fp16 scores deviate from the FP32 search's by at most 1.6e-4 of the peak
+// (0.04 rawCC), so the margin (2e-3 of the peak, >= 0.35 rawCC at n = 256)
+// keeps every pair that could be the best in exact arithmetic.
+constexpr float kRefineRel = 1.0f / 512.0f;
+constexpr float kRefineAbs = 64.0f;
+constexpr int kRefineBlocks = 1280;  // persistent grid of the rescoring pass: 5 waves per SIMD (a multiple of kXcds)
+
+size_t split16_list_offset(int n, int C, int B) { return align256(gis_correlate_split_workspace(n, C, B)); }
+
+// The launches of one fp16-operand search (ws as in split_run, plus the pair
+// list when `spectra32` is given: the refining pass rescoring each particle's
+// best pair and its near ties with the split-bf16 operator from the float32
+// spectra).
+int split16_run(const void* bank, const void* bank16, const void* spectra, const void* spectra32,
+                const float* inv_norm, const float* tinv, const int* dir_off, const int* dir_idx, const int* pmap,
+                const int* order, int B, int P, int n, int C, int nrows, int dt, int pt, int wsplit, float* scores,
+                int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv, void* ws,
+                hipStream_t st) {
+  const int LT = split_lt(C);
+  if (pt == 0) pt = std::min(P, 5);
+  if (dt == 0) dt = 1;
+  if (LT == 2) dt = 1;
+  if (dt > 2) dt = 2;  // four directions per wave exceed the registers
+  bf16x8* Wt = (bf16x8*)ws;
+  bf16x8* Et = Wt + split_w_frags(n, C);
+  int* unit_off = (int*)((char*)ws + align256((split_w_frags(n, C) + split_e_frags(n, C)) * sizeof(bf16x8)));
+  int* count = (int*)((char*)ws + split16_list_offset(n, C, B));
+  int2* list = (int2*)((char*)count + 256);
+  hipLaunchKernelGGL(split_tables_kernel, dim3(64), dim3(256), 0, st, n, C, LT, split_nks(n), split_nct(n), kTabF16,
+                     (void*)Wt, (void*)Et);
+  GIS_LAUNCH_CHECK("gis_correlate_split_f16 tables");
+  hipLaunchKernelGGL(split_units_kernel, dim3(1), dim3(1024), 0, st, dir_off, order, B, dt, unit_off,
+                     spectra32 ? count : nullptr);
+  GIS_LAUNCH_CHECK("gis_correlate_split_f16 units");
+  if (nrows > 0) {
+    const long bound = ((long)nrows + (long)B * (dt - 1) + dt - 1) / dt;
+    const int nblk = (int)((bound + kWaves - 1) / kWaves);
+    const int grid_x = (nblk + kXcds - 1) / kXcds * kXcds;
+    const dim3 grid((unsigned)grid_x, (unsigned)((P + pt - 1) / pt));
+    const size_t lds = split_lds_bytes(n, C) / (wsplit ? 1 : 2);
+    int rc = GIS_OK;
+#define GIS_S16_ARGS                                                                                          \
+  grid, lds, st, bank16 ? bank16 : bank, (const f16x2*)spectra, tinv, dir_off, dir_idx, pmap, order, unit_off, \
+      B, P, n, C, nblk, (const void*)Wt, (const void*)Et, scores, peak_y, peak_x
+#define GIS_S16_PTT(LTV, DTV, WSV, TV)                                           \
+  switch (pt) {                                                                  \
+    case 1: rc = launch_split16<LTV, DTV, 1, WSV, TV>(GIS_S16_ARGS); break;      \
+    case 2: rc = launch_split16<LTV, DTV, 2, WSV, TV>(GIS_S16_ARGS); break;      \
+    case 3: rc = launch_split16<LTV, DTV, 3, WSV, TV>(GIS_S16_ARGS); break;      \
+    case 4: rc = launch_split16<LTV, DTV, 4, WSV, TV>(GIS_S16_ARGS); break;      \
+    default: rc = launch_split16<LTV, DTV, 5, WSV, TV>(GIS_S16_ARGS); break;     \
+  }
+#define GIS_S16_PT(LTV, DTV, WSV)        \
+  if (bank16) {                          \
+    GIS_S16_PTT(LTV, DTV, WSV, true)     \
+  } else {                               \
+    GIS_S16_PTT(LTV, DTV, WSV, false)    \
+  }
+#define GIS_S16_WS(LTV, DTV)    \
+  if (wsplit) {                 \
+    GIS_S16_PT(LTV, DTV, true)  \
+  } else {                      \
+    GIS_S16_PT(LTV, DTV, false) \
+  }
+    if (LT == 2) {
+      GIS_S16_WS(2, 1)
+    } else if (dt == 1) {
+      GIS_S16_WS(1, 1)
+    } else {
+      GIS_S16_WS(1, 2)
+    }
+#undef GIS_S16_WS
+#undef GIS_S16_PT
+#undef GIS_S16_PTT
+#undef GIS_S16_ARGS
+    if (rc != GIS_OK) return rc;
+    GIS_LAUNCH_CHECK("gis_correlate_split_f16");
+  }
+  hipLaunchKernelGGL(particle_tail_kernel, dim3(B), dim3(64), 0, st, dir_off, P, scores, best, peak, mean, stdv);
+  GIS_LAUNCH_CHECK("gis_correlate_split_f16 tail");
+  if (!spectra32 || nrows == 0) return GIS_OK;
+
+  // refine: the best pair and its near ties of every particle on split-bf16
+  const float abs_margin = kRefineAbs / sqrtf((float)n * (float)(n / 2 + 1));
+  hipLaunchKernelGGL(split_select_kernel, dim3(B), dim3(64), 0, st, dir_off, P, scores, best, peak, kRefineRel,
+                     abs_margin, list, count);
+  GIS_LAUNCH_CHECK("gis_correlate_split_f16 select");
+  hipLaunchKernelGGL(split_tables_kernel, dim3(64), dim3(256), 0, st, n, C, LT, split_nks(n), split_nct(n), kTabBF16,
+                     (void*)Wt, (void*)Et);
+  GIS_LAUNCH_CHECK("gis_correlate_split_f16 refine tables");
+  {
+    int rc = GIS_OK;
+    const dim3 grid(kRefineBlocks, 1);
+    const size_t lds = 0;
+#define GIS_REFINE(LTV)                                                                                          \
+  rc = launch_split<LTV, 1, 1, false, true, true>(grid, lds, st, (const float2*)bank, (const float2*)spectra32,   \
+                                                  inv_norm, tinv, dir_off, dir_idx, pmap, order, unit_off, B, P, \
+                                                  n, C, 0, 0, 0, (const void*)Wt, (const void*)Et, scores,       \
+                                                  peak_y, peak_x, list, count)
+    if (LT == 2) {
+      GIS_REFINE(2);
+    } else {
+      GIS_REFINE(1);
+    }
+#undef GIS_REFINE
+    if (rc != GIS_OK) return rc;
+    GIS_LAUNCH_CHECK("gis_correlate_split_f16 refine");
+  }
+  hipLaunchKernelGGL(particle_tail_kernel, dim3(B), dim3(64), 0, st, dir_off, P, scores, best, peak, mean, stdv);
+  GIS_LAUNCH_CHECK("gis_correlate_split_f16 refine tail");
+  return GIS_OK;
+}
+
+}  // namespace
+
+// Workspace bytes of gis_correlate_split_f16 at newbox n, crop C, B
+// particles and npairs = dir_off[B] * P searched pairs.
+GIS_API size_t gis_correlate_split_f16_workspace(int n, int C, int B, long npairs) {
+  if (n < 2 || C < 2 || B < 0 || npairs < 0) return 0;
+  return split16_list_offset(n, C, B) + 256 + align256((size_t)npairs * sizeof(int2));
+}
+
+// The search of gis_correlate_split with fp16 operands: the psi spectra from
+// gis_pack_fragments_f16 (scale 128, with their 1/norms: unit norm times
+// 2^7), the template bank float32 in fragment order (tinv: 1/norm per row,
+// or NULL for a unit-norm bank) or, when bank16 is given, that bank packed
+// the same way (gis_pack_fragments_f16 with scale 128 and tinv), products in
+// packed fp16, the row table split into fp16 hi + lo (wsplit = 1) or fp16
+// alone (wsplit = 0), FP32 accumulation.  With spectra32 (the float32 fragment-order spectra, and
+// their inv_norm) the best pair of every particle and the pairs within the
+// near-tie margin are rescored with the split-bf16 operator of
+// gis_correlate_split; without it the fp16 scores stand.  Outputs as
+// gis_correlate_split; ws: gis_correlate_split_f16_workspace bytes.
+GIS_API int gis_correlate_split_f16(const void* bank, const void* bank16, const void* spectra16,
+                                    const void* spectra32, const float* inv_norm, const float* tinv,
+                                    const int* dir_off, const int* dir_idx, const int* pmap, const int* order, int B,
+                                    int P, int n, int C, int nrows, int dt, int pt, int wsplit, float* scores,
+                                    int* peak_y, int* peak_x, int* best, float* peak, double* mean, double* stdv,
+                                    void* ws, size_t ws_bytes, void* stream) {
+  const int rc = split_check("gis_correlate_split_f16", B, P, n, C, nrows, 0, 0, dt, pt);
+  if (rc != GIS_OK) return rc;
+  if (B == 0) return GIS_OK;
+  GIS_REQUIRE(bank && spectra16 && dir_off && dir_idx && scores && peak_y && peak_x && best && peak && mean &&
+                  stdv && ws,
+              "gis_correlate_split_f16: null pointer");
+  GIS_REQUIRE(wsplit == 0 || wsplit == 1, "gis_correlate_split_f16: wsplit %d not 0 or 1", wsplit);
+  GIS_REQUIRE(ws_bytes >= gis_correlate_split_f16_workspace(n, C, B, (long)nrows * P),
+              "gis_correlate_split_f16: workspace too small");
+  return split16_run(bank, bank16, spectra16, spectra32, inv_norm, tinv, dir_off, dir_idx, pmap, order, B, P, n, C,
+                     nrows, dt, pt, wsplit, scores, peak_y, peak_x, best, peak, mean, stdv, ws, (hipStream_t)stream);
+}
 
 // The FP32 search of gis_correlate with the work decomposition of the split
 // search (one wave per particle, direction and psi group; no padded pair

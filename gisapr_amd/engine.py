@@ -39,6 +39,7 @@ class ParticleSet:
     row_order: Optional[torch.Tensor] = None # [sum M_i] int32 processing order of the small-shift search
     particle_order: Optional[torch.Tensor] = None  # [B] int32 processing order of the split search
     spectra_frag: Optional[kernels.FragmentPlanes] = None  # [B, P] psi spectra in fragment order (searched)
+    spectra16: Optional[kernels.HalfFragmentPlanes] = None  # [B, P] unit-norm x 2^7 fp16 spectra (fp16 search)
 
     @property
     def B(self) -> int:
@@ -54,7 +55,8 @@ class ParticleSet:
 
     def nbytes_resident(self) -> int:
         t = [self.images, self.rotp, self.dir_off, self.dir_idx, self.spectra, self.inv_norm, self.defocus,
-             self.coeffs, None if self.spectra_frag is None else self.spectra_frag.data]
+             self.coeffs, None if self.spectra_frag is None else self.spectra_frag.data,
+             None if self.spectra16 is None else self.spectra16.data]
         return sum(x.numel() * x.element_size() for x in t if x is not None)
 
 
@@ -185,6 +187,8 @@ class SearchEngine:
             out=ps.spectra_frag, inv_norm=ps.inv_norm)
         if ps.spectra is not None:
             ps.spectra.copy_(ps.spectra_frag.unpack())
+        if ps.spectra16 is not None:
+            kernels.pack_fragments_f16(ps.spectra_frag, ps.inv_norm, out=ps.spectra16.data)
         return ps
 
     def row_spectra(self, ps: ParticleSet) -> torch.Tensor:
@@ -241,19 +245,42 @@ class SearchEngine:
             self.prepare(ps)
         return ps.spectra_frag
 
+    def search_spectra16(self, ps: ParticleSet) -> kernels.HalfFragmentPlanes:
+        """The particle spectra as the fp16-operand search reads them (unit
+        norm x 2^7, fp16), made from the fragment-order spectra once and kept
+        in step by ``prepare``."""
+        if ps.spectra_frag is None:
+            self.prepare(ps)
+        if ps.spectra16 is None:
+            ps.spectra16 = kernels.pack_fragments_f16(ps.spectra_frag, ps.inv_norm)
+        return ps.spectra16
+
     def search_split(self, bank: torch.Tensor, ps: ParticleSet, tinv: Optional[torch.Tensor] = None,
-                     dt: int = 0, pt: int = 0) -> SearchResult:
+                     dt: int = 0, pt: int = 0, operand: str = "bf16", wsplit: bool = False,
+                     refine: bool = True) -> SearchResult:
         """The search of ``search`` (every coefficient, every lag of the crop)
-        on split-bf16 MFMA; particles are processed in orientation order."""
+        on split-bf16 MFMA; particles are processed in orientation order.
+        ``operand="fp16"``: fp16 operands and packed fp16 products (the row
+        table split into fp16 hi + lo with ``wsplit``), then, with ``refine``,
+        each particle's best pair and its near ties rescored on split-bf16."""
+        if operand not in ("bf16", "fp16"):
+            raise ValueError("operand must be 'bf16' or 'fp16'")
         if ps.spectra_frag is None:
             self.prepare(ps)
         if int(bank.shape[0]) != len(self.dir_rot):
             raise ValueError("template bank does not cover the engine's direction table")
         if ps.particle_order is None:
             ps.particle_order = particle_order(ps.dir_off.cpu().numpy(), ps.dir_idx.cpu().numpy(), self.device)
-        corr = kernels.correlate_split(search_bank(bank), self.search_spectra(ps), ps.inv_norm, ps.dir_off,
-                                       ps.dir_idx, self.crop, validate=False, tinv=tinv, order=ps.particle_order,
-                                       dt=dt, pt=pt)
+        if operand == "fp16":
+            fb = search_bank(bank)
+            corr = kernels.correlate_split_f16(fb, self.search_spectra16(ps), ps.dir_off, ps.dir_idx, self.crop,
+                                               validate=False, tinv=tinv, order=ps.particle_order, dt=dt, pt=pt,
+                                               wsplit=wsplit, refine=ps.spectra_frag if refine else None,
+                                               inv_norm=ps.inv_norm, bank16=kernels.pack_fragments_f16(fb, tinv))
+        else:
+            corr = kernels.correlate_split(search_bank(bank), self.search_spectra(ps), ps.inv_norm, ps.dir_off,
+                                           ps.dir_idx, self.crop, validate=False, tinv=tinv,
+                                           order=ps.particle_order, dt=dt, pt=pt)
         return SearchResult(corr=corr, score=kernels.score_sum(corr.peak, corr.best), center=self.crop // 2)
 
     # ---------------------------------------------------------------- small-shift precision search

@@ -311,6 +311,66 @@ def pack_fragments(x: torch.Tensor, out: Optional[torch.Tensor] = None) -> Fragm
     return FragmentPlanes(out, n)
 
 
+H16_SCALE = 128.0  # fp16 operands: unit-norm planes times 2^7
+
+
+@dataclass
+class HalfFragmentPlanes:
+    """Psi spectra as the fp16-operand search reads them: unit-norm planes
+    times 2^7 as fp16 (real, imaginary) pairs, ``data`` [..., fragment_plane(n)]
+    complex32, each tile's values one 16-byte run per lane."""
+    data: torch.Tensor
+    n: int
+
+    @property
+    def shape(self):
+        return tuple(self.data.shape[:-1])
+
+    def __getitem__(self, idx) -> "HalfFragmentPlanes":
+        return HalfFragmentPlanes(self.data[idx], self.n)
+
+    def unpack(self) -> torch.Tensor:
+        """The planes row major as complex64 [..., n, n/2+1], still x 2^7 (for checks)."""
+        pos = torch.from_numpy(half_fragment_positions(self.n)).to(self.data.device)
+        return self.data.to(torch.complex64)[..., pos.reshape(-1)].reshape(self.shape + tuple(pos.shape))
+
+
+def half_fragment_positions(n: int) -> np.ndarray:
+    """[n, n/2+1] position of each packed coefficient in the fp16 fragment order."""
+    m = n // 2 + 1
+    nks = ((n + 15) // 16 + 1) & ~1
+    lone = m % 16 == 1
+    nct = m // 16 if lone else (m + 15) // 16
+    r, c = np.meshgrid(np.arange(n), np.arange(m), indexing="ij")
+    pos = ((r >> 4) * nct + (c >> 4)) * 256 + (((r >> 2) & 3) * 16 + (c & 15)) * 4 + (r & 3)
+    if lone:
+        pos[:, m - 1] = nks * nct * 256 + np.arange(n)
+    return pos.astype(np.int64)
+
+
+def pack_fragments_f16(x: FragmentPlanes, inv_norm: Optional[torch.Tensor], scale: float = H16_SCALE,
+                       out: Optional[torch.Tensor] = None) -> HalfFragmentPlanes:
+    """Fragment-order planes times ``scale`` * ``inv_norm`` (one value per
+    plane, or None) as fp16 pairs in the fp16 fragment order."""
+    if not isinstance(x, FragmentPlanes):
+        raise ValueError("pack_fragments_f16 takes FragmentPlanes")
+    require_hip(x.data, "planes")
+    lead = x.shape
+    rows = int(np.prod(lead)) if lead else 1
+    F = fragment_plane(x.n)
+    if inv_norm is not None:
+        require_hip(inv_norm, "inv_norm")
+        if inv_norm.dtype != torch.float32 or inv_norm.numel() != rows or not inv_norm.is_contiguous():
+            raise ValueError("inv_norm must be contiguous float32 with one value per plane")
+    if out is None:
+        out = torch.empty(lead + (F,), dtype=torch.complex32, device=x.data.device)
+    elif tuple(out.shape) != lead + (F,) or out.dtype != torch.complex32 or not out.is_contiguous():
+        raise ValueError(f"out must be contiguous complex32 {lead + (F,)}")
+    check(lib().gis_pack_fragments_f16(ptr(x.data), ptr(inv_norm), rows, x.n, float(scale), ptr(out),
+                                       stream_of(x.data.device)), "gis_pack_fragments_f16")
+    return HalfFragmentPlanes(out, x.n)
+
+
 def build_templates(dft: torch.Tensor, new_box: int, rts: torch.Tensor, mask: Optional[torch.Tensor],
                     fsc_packed: Optional[torch.Tensor], with_norms: bool = False,
                     out: Optional[torch.Tensor] = None, normalize: bool = True,
@@ -630,6 +690,105 @@ def correlate_split(bank: torch.Tensor, spectra: torch.Tensor, inv_norm: Optiona
                                     ptr(pmap), ptr(order), B, P, n, int(crop), total, ldt, ldp, int(dt), int(pt),
                                     ptr(scores), ptr(py), ptr(px), ptr(best), ptr(peak), ptr(mean), ptr(std),
                                     ptr(ws), wsb, stream_of(dev)), "gis_correlate_split")
+    return CorrelationResult(scores, py, px, best, peak, mean, std)
+
+
+def correlate_split_f16(bank: FragmentPlanes, spectra: HalfFragmentPlanes, dir_off: torch.Tensor,
+                        dir_idx: torch.Tensor, crop: int, validate: bool = True, pmap: Optional[torch.Tensor] = None,
+                        tinv: Optional[torch.Tensor] = None, order: Optional[torch.Tensor] = None, dt: int = 0,
+                        pt: int = 0, wsplit: bool = False, refine: Optional[FragmentPlanes] = None,
+                        inv_norm: Optional[torch.Tensor] = None,
+                        bank16: Optional[HalfFragmentPlanes] = None,
+                        out: Optional[CorrelationResult] = None) -> CorrelationResult:
+    """The v606 search of ``correlate`` with fp16 operands: ``bank``
+    float32 fragment planes [M] (``tinv`` 1/norm per row, or None for unit
+    rows), ``spectra`` from ``pack_fragments_f16`` (unit norm x 2^7),
+    products in packed fp16, the row table split into fp16 hi + lo
+    (``wsplit``) or fp16 alone; ``order`` / ``dt`` / ``pt`` as in
+    ``correlate_split``.  ``refine``: the float32 fragment-order spectra
+    (with ``inv_norm``) the fp16 ones came from; each particle's best pair
+    and its near ties are then rescored on split-bf16 MFMA.  ``bank16``: the
+    bank as fp16 pairs (``pack_fragments_f16(bank, tinv)``), read by the fp16
+    pass instead of rounding the float32 rows per step.  ``out``: result
+    tensors to write into (shapes and dtypes as returned)."""
+    if not isinstance(bank, FragmentPlanes) or not isinstance(spectra, HalfFragmentPlanes):
+        raise ValueError("correlate_split_f16 takes FragmentPlanes (bank) and HalfFragmentPlanes (spectra)")
+    require_hip(bank.data, "bank")
+    require_hip(spectra.data, "spectra")
+    n = spectra.n
+    if bank.n != n:
+        raise ValueError(f"bank newbox {bank.n} != spectra newbox {n}")
+    F = fragment_plane(n)
+    if bank.data.dim() != 2 or int(bank.data.shape[-1]) != F or bank.data.dtype != torch.complex64 or \
+            not bank.data.is_contiguous():
+        raise ValueError("bank must be contiguous complex64 [M, fragment_plane(n)]")
+    if spectra.data.dim() != 3 or int(spectra.data.shape[-1]) != F or spectra.data.dtype != torch.complex32 or \
+            not spectra.data.is_contiguous():
+        raise ValueError("spectra must be contiguous complex32 [B, P, fragment_plane(n)]")
+    P = int(spectra.data.shape[1])
+    B = int(dir_off.shape[0]) - 1 if pmap is not None else int(spectra.data.shape[0])
+    if dir_off.dtype != torch.int32 or dir_idx.dtype != torch.int32:
+        raise ValueError("dir_off / dir_idx must be int32")
+    if tuple(dir_off.shape) != (B + 1,):
+        raise ValueError("dir_off must be [B+1]")
+    total = int(dir_idx.shape[0])
+    if order is not None:
+        require_hip(order, "order")
+        if order.dtype != torch.int32 or tuple(order.shape) != (B,):
+            raise ValueError("order must be int32 [B]")
+    if validate:
+        off = dir_off.cpu().numpy()
+        if off[0] != 0 or off[-1] != total or np.any(np.diff(off) < 0):
+            raise ValueError("dir_off must start at 0, be non-decreasing and end at len(dir_idx)")
+        if total and (int(dir_idx.min()) < 0 or int(dir_idx.max()) >= int(bank.data.shape[0])):
+            raise ValueError("dir_idx out of range of the template bank")
+        _check_pmap(pmap, B, int(spectra.data.shape[0]))
+        if order is not None and not torch.equal(torch.sort(order.long()).values,
+                                                 torch.arange(B, device=order.device)):
+            raise ValueError("order must be a permutation of the particles")
+    if tinv is not None:
+        require_hip(tinv, "tinv")
+        if tinv.dtype != torch.float32 or tinv.numel() != int(bank.data.shape[0]):
+            raise ValueError("tinv must be float32 with one value per bank row")
+    dev = spectra.data.device
+    if out is None:
+        out = CorrelationResult(torch.empty((total, P), dtype=torch.float32, device=dev),
+                                torch.empty((total, P), dtype=torch.int32, device=dev),
+                                torch.empty((total, P), dtype=torch.int32, device=dev),
+                                torch.empty((B,), dtype=torch.int32, device=dev),
+                                torch.empty((B,), dtype=torch.float32, device=dev),
+                                torch.empty((B,), dtype=torch.float64, device=dev),
+                                torch.empty((B,), dtype=torch.float64, device=dev))
+    want = [((total, P), torch.float32), ((total, P), torch.int32), ((total, P), torch.int32), ((B,), torch.int32),
+            ((B,), torch.float32), ((B,), torch.float64), ((B,), torch.float64)]
+    fields = [out.scores, out.peak_y, out.peak_x, out.best, out.peak, out.mean, out.std]
+    for t, (shape, dtype) in zip(fields, want):
+        if tuple(t.shape) != shape or t.dtype != dtype or not t.is_contiguous() or t.device != dev:
+            raise ValueError("out tensors must match the result's shapes and dtypes")
+    scores, py, px, best, peak, mean, std = fields
+    b16 = None
+    if bank16 is not None:
+        if not isinstance(bank16, HalfFragmentPlanes) or bank16.n != n or \
+                tuple(bank16.data.shape) != tuple(bank.data.shape) or bank16.data.dtype != torch.complex32:
+            raise ValueError("bank16 must be the bank packed by pack_fragments_f16")
+        require_hip(bank16.data, "bank16")
+        b16 = bank16.data
+    s32 = None
+    if refine is not None:
+        if not isinstance(refine, FragmentPlanes) or refine.n != n or \
+                tuple(refine.data.shape) != tuple(spectra.data.shape) or refine.data.dtype != torch.complex64:
+            raise ValueError("refine must be the float32 FragmentPlanes the fp16 spectra were packed from")
+        if inv_norm is None:
+            raise ValueError("refine needs the spectra's inv_norm")
+        require_hip(refine.data, "refine")
+        s32 = refine.data
+    wsb = int(lib().gis_correlate_split_f16_workspace(n, int(crop), B, total * P))
+    ws = torch.empty((max(wsb, 1),), dtype=torch.uint8, device=dev)
+    check(lib().gis_correlate_split_f16(ptr(bank.data), ptr(b16), ptr(spectra.data), ptr(s32), ptr(inv_norm), ptr(tinv),
+                                        ptr(dir_off), ptr(dir_idx), ptr(pmap), ptr(order), B, P, n, int(crop), total,
+                                        int(dt), int(pt), 1 if wsplit else 0, ptr(scores), ptr(py), ptr(px),
+                                        ptr(best), ptr(peak), ptr(mean), ptr(std), ptr(ws), wsb, stream_of(dev)),
+          "gis_correlate_split_f16")
     return CorrelationResult(scores, py, px, best, peak, mean, std)
 
 

@@ -356,6 +356,57 @@ int gis_correlate_split_noalloc(const void* bank, const void* spectra, const flo
                                 void* stream);
 
 /*
+ * gis_pack_fragments_f16: the psi spectra as the fp16-operand search reads
+ * them.  dst [rows][gis_fragment_plane(n)] pairs of fp16 (real, imaginary):
+ * the fragment-order planes src [rows][gis_fragment_plane(n)] complex64 times
+ * scale * inv_norm[row] (inv_norm may be NULL), rearranged so that a tile's
+ * value (row 16 ks + 4g + j, column 16 ct + i) sits at tile * 256 +
+ * 4 (16 g + i) + j (one 16-byte run per lane); the lone last column as in
+ * fragment order.  No counterpart in the reference (a storage format).
+ */
+int gis_pack_fragments_f16(const void* src, const float* inv_norm, int rows, int n, float scale, void* dst,
+                           void* stream);
+
+/* Workspace bytes gis_correlate_split_f16 needs at newbox n, crop C, B
+ * particles and npairs = dir_off[B] * P searched pairs. */
+size_t gis_correlate_split_f16_workspace(int n, int C, int B, long npairs);
+
+/*
+ * (d) The search of gis_correlate_split with fp16 operands on
+ * v_mfma_f32_16x16x32_f16: the same operator (every packed coefficient,
+ * every lag of the C x C crop) and outputs, replacing
+ * correlate_templates_and_particle (v606:1418-1449) and the numerics of
+ * make_result_records (v606:1452-1491).
+ *   bank      [M][gis_fragment_plane(n)] complex64, fragment order
+ *   bank16    NULL, or bank packed by gis_pack_fragments_f16 with scale 128
+ *             and tinv (unit norm x 2^7): the fp16 pass then reads it
+ *             instead of rounding the float32 values per step
+ *   spectra16 [rows][P][gis_fragment_plane(n)] from gis_pack_fragments_f16
+ *             with scale 128 and the spectra's 1/norms (unit norm x 2^7)
+ *   spectra32 the float32 fragment-order spectra the fp16 ones were made
+ *             from, with their inv_norm, or NULL (see below)
+ *   tinv      [M] 1/norm per bank row, or NULL for a unit-norm bank
+ *   wsplit    1: the row table W split into fp16 hi + lo (two MFMAs per
+ *             tile), 0: fp16 W alone
+ * Each step rounds a lane's template values (x 2^7 / norm) to fp16 and forms
+ * Q' = T' P' with packed fp16 multiply-adds (Q' = 2^14 Q / (|T| |P|), the A
+ * operand as it is); FP32 accumulation; the column stage on split-bf16 MFMA
+ * as in gis_correlate_split.  fp16 products keep ~11 significant bits (scores
+ * within ~2e-4 of the peak).  With spectra32, each particle's best pair and
+ * every pair within max(2^-9 |best|, 64 / sqrt(n (n/2+1))) of it are
+ * rescored with the split-bf16 operator of gis_correlate_split before the
+ * per-particle tail, so best, peak and the best pair's lag are those of the
+ * split-bf16 search.  dir_off, dir_idx, pmap, order, nrows, dt, pt and the
+ * outputs as in gis_correlate_split; ws: gis_correlate_split_f16_workspace
+ * bytes; C even and <= 32, n <= 256.
+ */
+int gis_correlate_split_f16(const void* bank, const void* bank16, const void* spectra16, const void* spectra32,
+                            const float* inv_norm, const float* tinv, const int* dir_off, const int* dir_idx,
+                            const int* pmap, const int* order, int B, int P, int n, int C, int nrows, int dt, int pt,
+                            int wsplit, float* scores, int* peak_y, int* peak_x, int* best, float* peak,
+                            double* mean, double* stdv, void* ws, size_t ws_bytes, void* stream);
+
+/*
  * Cross-particle conformation score sum_b exp(peak_b / 1024) in float64 over
  * particles with best_b >= 0 (best may be NULL).  Replaces
  * new_method_to_fit_the_2nd_Gaussian_PEAK_v4.py:133-134.  out: 1 double.

@@ -55,6 +55,16 @@ def test_version_and_argument_errors_without_gpu(library):
     rc = library.gis_correlate_split(None, None, None, None, None, None, None, None, 1, 5, 64, 16, 1, 32, 33, 0, 0,
                                      None, None, None, None, None, None, None, None, 0, None)
     assert rc == -1 and b"row strides" in library.gis_last_error()
+    rc = library.gis_correlate_split_f16(None, None, None, None, None, None, None, None, None, None, 1, 5, 64, 34,
+                                         1, 0, 0, 0, None, None, None, None, None, None, None, None, 0, None)
+    assert rc == -2 and b"crop" in library.gis_last_error()
+    rc = library.gis_correlate_split_f16(None, None, None, None, None, None, None, None, None, None, 1, 5, 64, 16,
+                                         1, 0, 0, 2, None, None, None, None, None, None, None, None, 0, None)
+    assert rc == -1
+    assert library.gis_correlate_split_f16_workspace(256, 16, 10, 100) > library.gis_correlate_split_workspace(
+        256, 16, 10)
+    rc = library.gis_pack_fragments_f16(None, None, 1, 64, 0.0, None, None)
+    assert rc == -1 and b"scale" in library.gis_last_error()
     rc = library.gis_volume_dft(None, 32, None, None, 0, None)
     assert rc == -1
     rc = library.gis_build_template_coefficients(None, 32, 32, None, 1, None, None, None, 0, 16, None, None, None,

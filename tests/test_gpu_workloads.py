@@ -5,7 +5,7 @@ oracle finishes in seconds) and runs the oracle's search over the union of
 their local directions (an order-preserving subset of the direction table,
 so every particle's local list, and its local indices, are unchanged):
   * box 256 / 1.06 A (templates of transform length 512, particle spectra at
-    256), FP32 and split-bf16 searches, crop 16;
+    256), FP32, split-bf16 and refined fp16 (split16) searches, crop 16;
   * the same particles at the reference's defaults newbox 160 and crop 32;
   * box 192 / 1.3 A (radix-3 transforms of length 384);
   * the batched step: two candidate poses on uneven particle subsets, each
@@ -84,7 +84,12 @@ def engine_records(eng, data, pick, path, coords=None):
     ps = eng.add_particles(imgs, data["rot"][pick], data["tilt"][pick], data["psi"][pick], defocus=dfo)
     vol = eng.density(data["coords"] if coords is None else coords, data["numbers"], data["workload"].res)
     bank, tinv = eng.templates(vol, normalize=False)
-    res = eng.search_split(bank, ps, tinv) if path == "split" else eng.search(bank, ps, tinv)
+    if path == "split":
+        res = eng.search_split(bank, ps, tinv)
+    elif path == "split16":
+        res = eng.search_split(bank, ps, tinv, operand="fp16")
+    else:
+        res = eng.search(bank, ps, tinv)
     recs = eng.records(res, ps)
     return {int(pick[r["line"]]): r for r in recs}, float(res.score.cpu())
 
@@ -98,7 +103,7 @@ def box256(hip_device):
     return wl, data, pick, want, s_want
 
 
-@pytest.mark.parametrize("path", ["fp32", "split"])
+@pytest.mark.parametrize("path", ["fp32", "split", "split16"])
 def test_box256_search_matches_oracle(hip_device, box256, path):
     wl, data, pick, want, s_want = box256
     say("box256 search", path)
@@ -107,7 +112,7 @@ def test_box256_search_matches_oracle(hip_device, box256, path):
     assert abs(s - s_want) <= RTOL * s_want
 
 
-@pytest.mark.parametrize("path", ["fp32", "split"])
+@pytest.mark.parametrize("path", ["fp32", "split", "split16"])
 def test_reference_defaults_newbox160_crop32(hip_device, box256, path):
     """--newboxsize 160 (read_pdb_index_generate_sh_3DEG_local_v37.py:36) and
     --ccgCropSize 32 (v606:1562) on box-256 particles."""
@@ -125,7 +130,7 @@ def test_box192_search_matches_oracle(hip_device):
     pick = np.arange(6)
     say("box192 oracle (6 particles)")
     want, s_want = oracle_run(wl, data, data["coords"], pick)
-    for path in ("fp32", "split"):
+    for path in ("fp32", "split", "split16"):
         say("box192 search", path)
         got, s = engine_records(make_engine(wl, data, hip_device), data, pick, path)
         compare_records(got, want, f"box192 {path}")
@@ -150,7 +155,7 @@ def test_batched_step_matches_oracle(hip_device):
         say("batched step oracle, slot", c)
         moved = move_chain(data["coords"], data["chains"], wl.moving_chain, poses[c])
         oracle.append(oracle_run(wl, data, moved, subsets[c]))
-    for path in ("split", "fp32"):
+    for path in ("split16", "split", "fp32"):
         say("batched step vs oracle", path)
         batch = CandidateBatch(eng, ps, body, wl.res, slots=2, path=path, subsets=subsets)
         got = batch.evaluate(poses)
@@ -167,11 +172,11 @@ def test_batched_step_matches_oracle(hip_device):
 def test_serving_matches_oracle(hip_device):
     """One request per resident dataset (box 128 / 192 / 256, the serving
     workload's generators), served in one continuous-batching stream on the
-    split path; each candidate's score against the oracle on its moved model."""
+    refined fp16 path; each candidate's score against the oracle on its moved model."""
     from gisapr_amd.batch import RigidBody
     from gisapr_amd.pdb_text import move_chain
     from gisapr_amd.serve import Request, SearchServer
-    server = SearchServer(hip_device, path="split", slots=2)
+    server = SearchServer(hip_device, path="split16", slots=2)
     sets = {}
     for name in ("serve128", "serve192", "serve256"):
         wl, data = subset_data(name, 3)
@@ -218,9 +223,12 @@ def test_search_recovers_on_grid_truth_box96(hip_device):
                        device=hip_device)
     ps = eng.add_particles(imgs, drot[k], dtilt[k], psi)
     vol = eng.density(coords, numbers, wl.res)
-    for path in ("fp32", "split"):
+    for path in ("fp32", "split", "split16"):
         bank, tinv = eng.templates(vol, normalize=False)
-        res = eng.search_split(bank, ps, tinv) if path == "split" else eng.search(bank, ps, tinv)
+        if path == "fp32":
+            res = eng.search(bank, ps, tinv)
+        else:
+            res = eng.search_split(bank, ps, tinv, operand="fp16" if path == "split16" else "bf16")
         recs = eng.records(res, ps)
         hit_dir = sum(r["direction"] == k[r["line"]] for r in recs)
         hit_psi = sum(r["psi_fake"] == psi[r["line"]] for r in recs)

@@ -178,3 +178,25 @@ def test_pdb2mrc_cli_helpers():
     a = density.create_parser().parse_args(["--i", "m.pdb", "--o", "m.mrc", "--box", "64", "--center",
                                             "--dontWriteMRC"])
     assert (a.box, a.center, a.dontWriteMRC, a.gpuid, a.device) == (64, True, True, 0, None)
+
+
+@pytest.mark.parametrize("n", [4, 32, 40, 64, 96, 160, 192, 256])
+def test_fragment_orders_are_permutations(n):
+    """Both operand orders place every packed coefficient at a distinct slot
+    of the plane; the fp16 order keeps a lane's four rows of one column
+    contiguous and the same tiles and lone column as the float32 order."""
+    from gisapr_amd import kernels
+    m = n // 2 + 1
+    nks = ((n + 15) // 16 + 1) & ~1
+    lone = m % 16 == 1
+    nct = m // 16 if lone else (m + 15) // 16
+    F = nks * nct * 256 + (nks * 16 if lone else 0)
+    for pos in (kernels.fragment_positions(n), kernels.half_fragment_positions(n)):
+        assert pos.shape == (n, m)
+        assert len(np.unique(pos)) == n * m and pos.min() >= 0 and pos.max() < F
+    p32, p16 = kernels.fragment_positions(n), kernels.half_fragment_positions(n)
+    np.testing.assert_array_equal(p32 // 256, p16 // 256)  # same tile (and lone column) of every value
+    r, c = np.meshgrid(np.arange(n), np.arange(m), indexing="ij")
+    inside = p16 < nks * nct * 256
+    lane = (((r >> 2) & 3) * 16 + (c & 15))
+    np.testing.assert_array_equal((p16 % 256)[inside], (lane * 4 + (r & 3))[inside])
