@@ -43,9 +43,10 @@ Serving legs (``--serve-requests``, 0 to skip): three resident datasets (box
 128 / 2,000, box 192 / 10,000, box 256 / 20,000 particles, 3,000-atom moving
 chain), a seeded Poisson stream of requests at ``--serve-rate``/s with 1-64
 candidates each (geometric, mean 8), continuous batching; p50/p99 request
-latency and throughput on the split-bf16 path (``serving``, the top-level
-p50/p99) and on the FP32 path (``serving_fp32``, ``--serve-requests-fp32``
-requests: the FP32 search is ~5x slower, so its stream is shorter).
+latency and throughput on the refined fp16 path (``serving``: the full
+2,000-request stream, the top-level p50/p99) and on the FP32 path
+(``serving_fp32``, ``--serve-requests-fp32`` requests: the FP32 search is
+~3x slower, so its stream is shorter).
 
 Prints ONE JSON line.  ``--gpus`` above 1 is not supported (one-GPU build).
 """
@@ -459,9 +460,10 @@ def pso_leg(args, dev, log):
 
 def serve_leg(args, dev, log):
     """Serving legs: three resident datasets, a seeded Poisson request stream per
-    path.  The datasets are built once; the FP32 and full-plane split-bf16
-    paths share their resident spectra, and the small-shift path (when asked
-    for) runs last because it replaces them by their in-disk coefficients.
+    path.  The datasets are built once; the FP32 and full-plane split paths
+    share their resident spectra (split16 adds the fp16 copy), and the
+    small-shift path (when asked for) runs last because it replaces them by
+    their in-disk coefficients.
     Returns {path: stats}."""
     import torch
     from gisapr_amd import synthetic
@@ -481,9 +483,10 @@ def serve_leg(args, dev, log):
         built.append((name, wl, eng, ps, body))
         log(f"serving dataset {name} built")
     setup = time.perf_counter() - t0
-    requests = {"fp32": args.serve_requests_fp32, "split": args.serve_requests, "mixed": args.serve_requests}
+    requests = {"fp32": args.serve_requests_fp32, "split": args.serve_requests, "split16": args.serve_requests,
+                "mixed": args.serve_requests}
     out = {}
-    for path in ("fp32", "split", "mixed"):
+    for path in ("fp32", "split", "split16", "mixed"):
         if requests[path] <= 0 or path not in args.serve_paths.split(","):
             continue
         t1 = time.perf_counter()
@@ -528,8 +531,8 @@ def main() -> int:
     ap.add_argument("--pso-iters", type=int, default=20, help="batched PSO leg iterations (0 to skip)")
     ap.add_argument("--pso-slots", type=int, default=64)
     ap.add_argument("--pso-particles", type=int, default=20000)
-    ap.add_argument("--serve-requests", type=int, default=300,
-                    help="split-bf16 serving stream (0 skips the serving legs)")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="split / split16 serving stream (0 skips the serving legs)")
     ap.add_argument("--serve-requests-fp32", type=int, default=40, help="FP32 serving stream (0 to skip it)")
     ap.add_argument("--serve-paths", default="fp32,split16")
     ap.add_argument("--pso-path", default="split16", help="search path of the batched PSO leg")
@@ -779,13 +782,20 @@ def main() -> int:
         out["top1_same_as_fp32"] = dict(agreement["split16"], path="split16 (same operator and crop)")
     if "split" in agreement:
         out["top1_split_bf16_vs_fp32"] = dict(agreement["split"], path="split (same operator and crop)")
-    if "split" in results and "split16" in results:
-        a, b = results["split"].corr, results["split16"].corr
+    if "split16" in results:
+        # the split-bf16 search on the split16 leg's own last bank: the rescored
+        # answers are that search's, bit for bit
+        a = eng.search_split(bank_split16[0], ps, bank_split16[1]).corr
+        b = results["split16"].corr
         valid = a.best >= 0
+        g = (ps.dir_off[:-1].long() * ps.P + a.best.long().clamp(min=0))
+        same_lag = (a.peak_y.reshape(-1)[g] == b.peak_y.reshape(-1)[g]) & \
+                   (a.peak_x.reshape(-1)[g] == b.peak_x.reshape(-1)[g])
         out["split16_vs_split"] = {
             "same_best": float((a.best == b.best)[valid].float().mean()),
+            "same_best_lag": float(same_lag[valid].float().mean()),
             "bit_identical_peak": float((a.peak == b.peak)[valid].float().mean()),
-            "particles": int(valid.sum())}
+            "particles": int(valid.sum()), "bank": "split16 leg's last candidate"}
     if "mixed" in agreement:
         out["top1_small_shift_vs_fp32"] = dict(
             agreement["mixed"], path="mixed small-shift variant (truncated coefficients, +/-3 px lags)")

@@ -79,6 +79,14 @@ __device__ __forceinline__ void split8(const float (&x)[8], bf16x8& hi, bf16x8& 
   }
 }
 
+// a * q in float32 with a fixed operation order (one product rounded, then a
+// fused multiply-add), whatever the tile shape: results are bit-identical
+// across the kernel's variants
+__device__ __forceinline__ void cprod(float2 a, float2 q, float& re, float& im) {
+  re = __builtin_fmaf(a.x, q.x, -(a.y * q.y));
+  im = __builtin_fmaf(a.x, q.y, a.y * q.x);
+}
+
 __host__ __device__ inline int crop_lag(int i, int n, int C) { return i < C / 2 ? n - C / 2 + i : i - C / 2; }
 
 // One table value split from float64: hi = bf16(v), lo = bf16(v - hi).
@@ -236,7 +244,7 @@ struct Frags {
   float2 p[PT][4];
 };
 
-template <int LT, int DT, int PT, bool F32, bool FR, bool LIST = false>
+template <int LT, int DT, int PT, bool F32, bool FR, bool LIST = false, int PF = 2>
 __global__ void __launch_bounds__(64 * kWaves)
 split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict__ spectra,
                        const float* __restrict__ inv_norm, const float* __restrict__ tinv,
@@ -259,11 +267,9 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
   const bool lone = (m & 15) == 1;
   const int nct = lone ? (m >> 4) : (m + 15) >> 4;
   const int nks = (((n + 15) >> 4) + 1) & ~1;  // row steps, padded to even (zero table rows)
-  // the pair-list pass (few waves, latency bound) reads the row table
-  // through the caches instead, so no LDS limits its occupancy
   static_assert(!(LIST && F32), "the pair list mode is split-bf16");
-  const bf16x8* Wl = LIST ? Wt : tab;
-  if constexpr (!LIST) {
+  const bf16x8* Wl = tab;
+  {
     const int nw = nks * 4 * LT * 64;
     for (int e = threadIdx.x; e < nw; e += 64 * kWaves) tab[e] = Wt[e];
     __syncthreads();
@@ -388,9 +394,7 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
             float x[8];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-              const float2 a = f.t[d][j], q = f.p[p][j];
-              x[2 * j] = a.x * q.x - a.y * q.y;
-              x[2 * j + 1] = a.x * q.y + a.y * q.x;
+              cprod(f.t[d][j], f.p[p][j], x[2 * j], x[2 * j + 1]);
             }
 #pragma unroll
             for (int t = 0; t < 2 * LT; ++t)
@@ -412,9 +416,7 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
             float x[8];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-              const float2 a = f.t[d][j], q = f.p[p][j];
-              x[2 * j] = a.x * q.x - a.y * q.y;
-              x[2 * j + 1] = a.x * q.y + a.y * q.x;
+              cprod(f.t[d][j], f.p[p][j], x[2 * j], x[2 * j + 1]);
             }
             bf16x8 ahi, alo;
             split8(x, ahi, alo);
@@ -430,11 +432,14 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
       load(ct2, ks2, f);
     };
 
-    // two buffers, loaded two steps ahead; nks is even, so buffer a always
-    // holds the even steps and b the odd ones, across column tiles too
-    Frags<DT, PT> fa, fb;
-    load(0, 0, fa);
-    load(0, 1, fb);
+    // PF buffers, each step's operands loaded PF steps ahead; nks is a
+    // multiple of PF (even always; the host launches PF = 4, for the
+    // latency-bound list pass, only where nks % 4 == 0), so buffer q always
+    // holds the steps ks = q (mod PF), across column tiles too.
+    static_assert(PF == 2 || PF == 4, "prefetch depth");
+    Frags<DT, PT> fr[PF];
+#pragma unroll
+    for (int q = 0; q < PF; ++q) load(0, q, fr[q]);
     for (int ct = 0; ct < nct; ++ct) {
       const int ctn = min(ct + 1, nct - 1);
 #pragma unroll
@@ -443,10 +448,10 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
         for (int p = 0; p < PT; ++p)
 #pragma unroll
           for (int t = 0; t < 2 * LT; ++t) gacc[d][p][t] = floatx4{0.f, 0.f, 0.f, 0.f};
-      for (int ks = 0; ks < nks; ks += 2) {
-        const bool wrap = ks + 2 >= nks;
-        step(ks, fa, wrap ? ctn : ct, wrap ? 0 : ks + 2);
-        step(ks + 1, fb, wrap ? ctn : ct, wrap ? 1 : ks + 3);
+      for (int ks = 0; ks < nks; ks += PF) {
+        const bool wrap = ks + PF >= nks;
+#pragma unroll
+        for (int q = 0; q < PF; ++q) step(ks + q, fr[q], wrap ? ctn : ct, wrap ? q : ks + PF + q);
       }
       // stage 2: S += Re(E^T G) over this column tile
       if constexpr (F32) {
@@ -564,8 +569,7 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
           float x[8];
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            x[2 * j] = na[j].x * nq[j].x - na[j].y * nq[j].y;
-            x[2 * j + 1] = na[j].x * nq[j].y + na[j].y * nq[j].x;
+            cprod(na[j], nq[j], x[2 * j], x[2 * j + 1]);
           }
           if (ks + 1 < nks) nload(ks + 1);
           if constexpr (F32) {
@@ -605,7 +609,7 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
               for (int tx = 0; tx < LT; ++tx)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                  out[d][p][tx * LT + ty][v] += er[tx][v] * gr - ei[tx][v] * gi;
+                  out[d][p][tx * LT + ty][v] += __builtin_fmaf(er[tx][v], gr, -(ei[tx][v] * gi));
             }
           }
       }
@@ -717,18 +721,26 @@ __global__ void pack_fragments16_kernel(const float2* __restrict__ src, const fl
 // Near-tie selection after an fp16-operand search: particle b's pairs whose
 // score is within max(rel |peak_b|, abs_margin) of its best go on the list
 // (b, pair index) for the split-bf16 rescoring; the best pair always does.
-__global__ void __launch_bounds__(64)
-split_select_kernel(const int* __restrict__ dir_off, int P, const float* __restrict__ scores,
+__global__ void __launch_bounds__(256)
+split_select_kernel(const int* __restrict__ dir_off, int B, int P, const float* __restrict__ scores,
                     const int* __restrict__ best, const float* __restrict__ peak, float rel, float abs_margin,
                     int2* __restrict__ list, int* __restrict__ count) {
-  const int b = blockIdx.x, lane = threadIdx.x;
-  if (best[b] < 0) return;
+  const int b = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;  // one wave per particle
+  if (b >= B || best[b] < 0) return;  // no barrier below
   const float pk = peak[b];
   const float thr = pk - fmaxf(rel * fabsf(pk), abs_margin);
   const int d0 = dir_off[b], np = (dir_off[b + 1] - d0) * P;
   const float* s = scores + (size_t)d0 * P;
-  for (int k = lane; k < np; k += 64)
-    if (s[k] >= thr) list[atomicAdd(count, 1)] = make_int2(b, k);
+  for (int k0 = 0; k0 < np; k0 += 64) {  // one atomic per wave and 64 pairs
+    const int k = k0 + lane;
+    const bool keep = k < np && s[k] >= thr;
+    const unsigned long long m = __ballot(keep);
+    if (m == 0) continue;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, 0, 64);
+    if (keep) list[base + __popcll(m & ((1ull << lane) - 1))] = make_int2(b, k);
+  }
 }
 
 template <int DT, int PT>
@@ -749,8 +761,9 @@ __device__ __forceinline__ f16x2 times_i(f16x2 t) {
 // bank with scale 128 and the rows' 1/norms), read like the spectra.
 template <int LT, int DT, int PT, bool WS, bool T16>
 __global__ void __launch_bounds__(64 * kWaves)
-// one direction per wave fits 168 registers: three waves per SIMD
-__attribute__((amdgpu_waves_per_eu(DT * PT * LT <= 5 ? 3 : 1)))
+// one direction per wave fits 168 registers (three waves per SIMD), two
+// directions and five psi 256 (two waves)
+__attribute__((amdgpu_waves_per_eu(DT * PT * LT * LT <= 5 ? 3 : DT * PT * LT * LT <= 10 ? 2 : 1)))
 split16_correlate_kernel(const void* __restrict__ bankv, const f16x2* __restrict__ spectra,
                          const float* __restrict__ tinv, const int* __restrict__ dir_off,
                          const int* __restrict__ dir_idx, const int* __restrict__ pmap,
@@ -1013,7 +1026,7 @@ split16_correlate_kernel(const void* __restrict__ bankv, const f16x2* __restrict
             for (int tx = 0; tx < LT; ++tx)
 #pragma unroll
               for (int v = 0; v < 4; ++v)
-                out[d][p][tx * LT + ty][v] += er[tx][v] * gr - ei[tx][v] * gi;
+                out[d][p][tx * LT + ty][v] += __builtin_fmaf(er[tx][v], gr, -(ei[tx][v] * gi));
           }
         }
     }
@@ -1064,7 +1077,7 @@ inline size_t split_e_frags(int n, int C) { return (size_t)split_nct(n) * 2 * sp
 inline size_t split_lds_bytes(int n, int C) { return split_w_frags(n, C) * sizeof(bf16x8); }
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-template <int LT, int DT, int PT, bool F32, bool FR, bool LIST = false>
+template <int LT, int DT, int PT, bool F32, bool FR, bool LIST = false, int PF = 2>
 int launch_split(dim3 grid, size_t lds, hipStream_t st, const float2* bank, const float2* spectra,
                  const float* inv_norm, const float* tinv, const int* dir_off, const int* dir_idx, const int* pmap,
                  const int* order, const int* unit_off, int B, int P, int n, int C, int ldt, int ldp, int nblk,
@@ -1072,7 +1085,7 @@ int launch_split(dim3 grid, size_t lds, hipStream_t st, const float2* bank, cons
                  const int2* list = nullptr, const int* list_count = nullptr) {
   static bool attr_set = false;
   if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)split_correlate_kernel<LT, DT, PT, F32, FR, LIST>,
+    hipError_t e = hipFuncSetAttribute((const void*)split_correlate_kernel<LT, DT, PT, F32, FR, LIST, PF>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)split_lds_bytes(256, LT == 1 ? 16 : 32));
     if (e != hipSuccess) {
@@ -1082,7 +1095,7 @@ int launch_split(dim3 grid, size_t lds, hipStream_t st, const float2* bank, cons
     }
     attr_set = true;
   }
-  hipLaunchKernelGGL((split_correlate_kernel<LT, DT, PT, F32, FR, LIST>), grid, dim3(64 * kWaves), lds, st, bank,
+  hipLaunchKernelGGL((split_correlate_kernel<LT, DT, PT, F32, FR, LIST, PF>), grid, dim3(64 * kWaves), lds, st, bank,
                      spectra, inv_norm, tinv, dir_off, dir_idx, pmap, order, unit_off, B, P, n, C, ldt, ldp, nblk, Wt,
                      Et, scores, peak_y, peak_x, list, list_count);
   return GIS_OK;
@@ -1259,7 +1272,7 @@ namespace {
 // keeps every pair that could be the best in exact arithmetic.
 constexpr float kRefineRel = 1.0f / 512.0f;
 constexpr float kRefineAbs = 64.0f;
-constexpr int kRefineBlocks = 1280;  // persistent grid of the rescoring pass: 5 waves per SIMD (a multiple of kXcds)
+constexpr int kRefineBlocks = 512;  // persistent grid of the rescoring pass: the resident blocks (a multiple of kXcds)
 
 size_t split16_list_offset(int n, int C, int B) { return align256(gis_correlate_split_workspace(n, C, B)); }
 
@@ -1274,7 +1287,7 @@ int split16_run(const void* bank, const void* bank16, const void* spectra, const
                 hipStream_t st) {
   const int LT = split_lt(C);
   if (pt == 0) pt = std::min(P, 5);
-  if (dt == 0) dt = 1;
+  if (dt == 0) dt = 2;  // measured fastest at box 256
   if (LT == 2) dt = 1;
   if (dt > 2) dt = 2;  // four directions per wave exceed the registers
   bf16x8* Wt = (bf16x8*)ws;
@@ -1338,8 +1351,8 @@ int split16_run(const void* bank, const void* bank16, const void* spectra, const
 
   // refine: the best pair and its near ties of every particle on split-bf16
   const float abs_margin = kRefineAbs / sqrtf((float)n * (float)(n / 2 + 1));
-  hipLaunchKernelGGL(split_select_kernel, dim3(B), dim3(64), 0, st, dir_off, P, scores, best, peak, kRefineRel,
-                     abs_margin, list, count);
+  hipLaunchKernelGGL(split_select_kernel, dim3((B + 3) / 4), dim3(256), 0, st, dir_off, B, P, scores, best, peak,
+                     kRefineRel, abs_margin, list, count);
   GIS_LAUNCH_CHECK("gis_correlate_split_f16 select");
   hipLaunchKernelGGL(split_tables_kernel, dim3(64), dim3(256), 0, st, n, C, LT, split_nks(n), split_nct(n), kTabBF16,
                      (void*)Wt, (void*)Et);
@@ -1347,16 +1360,17 @@ int split16_run(const void* bank, const void* bank16, const void* spectra, const
   {
     int rc = GIS_OK;
     const dim3 grid(kRefineBlocks, 1);
-    const size_t lds = 0;
-#define GIS_REFINE(LTV)                                                                                          \
-  rc = launch_split<LTV, 1, 1, false, true, true>(grid, lds, st, (const float2*)bank, (const float2*)spectra32,   \
-                                                  inv_norm, tinv, dir_off, dir_idx, pmap, order, unit_off, B, P, \
-                                                  n, C, 0, 0, 0, (const void*)Wt, (const void*)Et, scores,       \
-                                                  peak_y, peak_x, list, count)
+    const size_t lds = split_lds_bytes(n, C);
+#define GIS_REFINE(LTV, PFV)                                                                                      \
+  rc = launch_split<LTV, 1, 1, false, true, true, PFV>(grid, lds, st, (const float2*)bank,                         \
+                                                       (const float2*)spectra32, inv_norm, tinv, dir_off, dir_idx, \
+                                                       pmap, order, unit_off, B, P, n, C, 0, 0, 0, (const void*)Wt, \
+                                                       (const void*)Et, scores, peak_y, peak_x, list, count)
+    const bool deep = split_nks(n) % 4 == 0;  // four steps ahead where the row steps allow it
     if (LT == 2) {
-      GIS_REFINE(2);
+      if (deep) GIS_REFINE(2, 4); else GIS_REFINE(2, 2);
     } else {
-      GIS_REFINE(1);
+      if (deep) GIS_REFINE(1, 4); else GIS_REFINE(1, 2);
     }
 #undef GIS_REFINE
     if (rc != GIS_OK) return rc;
