@@ -208,14 +208,14 @@ __global__ void split_tables_kernel(int n, int C, int LT, int nks, int nct, int 
 // ceil(M_b / DT); unit_off[B] = the total.  One 1024-thread block.
 __global__ void __launch_bounds__(1024)
 split_units_kernel(const int* __restrict__ dir_off, const int* __restrict__ order, int B, int DT,
-                   int* __restrict__ unit_off, int* __restrict__ zero = nullptr) {
+                   int* __restrict__ unit_off) {
   __shared__ int part[1024];
   const int t = threadIdx.x;
-  if (zero && t == 0) *zero = 0;  // the pair-list counter of a refined search
   const int chunk = (B + 1023) / 1024;
   const int k0 = min(t * chunk, B), k1 = min(k0 + chunk, B);
   int s = 0;
-  for (int k = k0; k < k1; ++k) {
+#pragma unroll 8
+  for (int k = k0; k < k1; ++k) {  // independent loads: unrolled so they are in flight together
     const int b = order ? order[k] : k;
     s += (dir_off[b + 1] - dir_off[b] + DT - 1) / DT;
   }
@@ -721,26 +721,63 @@ __global__ void pack_fragments16_kernel(const float2* __restrict__ src, const fl
 // Near-tie selection after an fp16-operand search: particle b's pairs whose
 // score is within max(rel |peak_b|, abs_margin) of its best go on the list
 // (b, pair index) for the split-bf16 rescoring; the best pair always does.
+// Two passes without atomics: WRITE = false counts each particle's pairs
+// (cnt[b]); split_scan_kernel turns the counts into offsets in the
+// particles' processing order; WRITE = true writes each particle's entries
+// there, so a particle's entries (and, in a batched search, the slots'
+// copies of one particle) are adjacent on the list.
+template <bool WRITE>
 __global__ void __launch_bounds__(256)
 split_select_kernel(const int* __restrict__ dir_off, int B, int P, const float* __restrict__ scores,
                     const int* __restrict__ best, const float* __restrict__ peak, float rel, float abs_margin,
-                    int2* __restrict__ list, int* __restrict__ count) {
+                    int* __restrict__ cnt, const int* __restrict__ off, int2* __restrict__ list) {
   const int b = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;  // one wave per particle
-  if (b >= B || best[b] < 0) return;  // no barrier below
+  if (b >= B) return;  // no barrier below
+  if (best[b] < 0) {
+    if (!WRITE && lane == 0) cnt[b] = 0;
+    return;
+  }
   const float pk = peak[b];
   const float thr = pk - fmaxf(rel * fabsf(pk), abs_margin);
   const int d0 = dir_off[b], np = (dir_off[b + 1] - d0) * P;
   const float* s = scores + (size_t)d0 * P;
-  for (int k0 = 0; k0 < np; k0 += 64) {  // one atomic per wave and 64 pairs
+  int base = WRITE ? off[b] : 0;
+  for (int k0 = 0; k0 < np; k0 += 64) {
     const int k = k0 + lane;
     const bool keep = k < np && s[k] >= thr;
     const unsigned long long m = __ballot(keep);
-    if (m == 0) continue;
-    int base = 0;
-    if (lane == 0) base = atomicAdd(count, __popcll(m));
-    base = __shfl(base, 0, 64);
-    if (keep) list[base + __popcll(m & ((1ull << lane) - 1))] = make_int2(b, k);
+    if (WRITE && keep) list[base + __popcll(m & ((1ull << lane) - 1))] = make_int2(b, k);
+    base += __popcll(m);
   }
+  if (!WRITE && lane == 0) cnt[b] = base;
+}
+
+// off[order[k]] = sum of cnt[order[j]] over j < k (order NULL: identity);
+// off[B] = the total.  One 1024-thread block.
+__global__ void __launch_bounds__(1024)
+split_scan_kernel(const int* __restrict__ cnt, const int* __restrict__ order, int B, int* __restrict__ off) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (B + 1023) / 1024;
+  const int k0 = min(t * chunk, B), k1 = min(k0 + chunk, B);
+  int s = 0;
+#pragma unroll 8
+  for (int k = k0; k < k1; ++k) s += cnt[order ? order[k] : k];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int k = k0; k < k1; ++k) {
+    const int b = order ? order[k] : k;
+    off[b] = run;
+    run += cnt[b];
+  }
+  if (t == 1023) off[B] = part[1023];
 }
 
 template <int DT, int PT>
@@ -1275,6 +1312,8 @@ constexpr float kRefineAbs = 64.0f;
 constexpr int kRefineBlocks = 512;  // persistent grid of the rescoring pass: the resident blocks (a multiple of kXcds)
 
 size_t split16_list_offset(int n, int C, int B) { return align256(gis_correlate_split_workspace(n, C, B)); }
+// the selection's counts and offsets: 64 spare ints, cnt[B], off[B+1]
+size_t split16_count_bytes(int B) { return align256((size_t)(64 + 2 * B + 1) * sizeof(int)); }
 
 // The launches of one fp16-operand search (ws as in split_run, plus the pair
 // list when `spectra32` is given: the refining pass rescoring each particle's
@@ -1294,12 +1333,11 @@ int split16_run(const void* bank, const void* bank16, const void* spectra, const
   bf16x8* Et = Wt + split_w_frags(n, C);
   int* unit_off = (int*)((char*)ws + align256((split_w_frags(n, C) + split_e_frags(n, C)) * sizeof(bf16x8)));
   int* count = (int*)((char*)ws + split16_list_offset(n, C, B));
-  int2* list = (int2*)((char*)count + 256);
+  int2* list = (int2*)((char*)count + split16_count_bytes(B));
   hipLaunchKernelGGL(split_tables_kernel, dim3(64), dim3(256), 0, st, n, C, LT, split_nks(n), split_nct(n), kTabF16,
                      (void*)Wt, (void*)Et);
   GIS_LAUNCH_CHECK("gis_correlate_split_f16 tables");
-  hipLaunchKernelGGL(split_units_kernel, dim3(1), dim3(1024), 0, st, dir_off, order, B, dt, unit_off,
-                     spectra32 ? count : nullptr);
+  hipLaunchKernelGGL(split_units_kernel, dim3(1), dim3(1024), 0, st, dir_off, order, B, dt, unit_off);
   GIS_LAUNCH_CHECK("gis_correlate_split_f16 units");
   if (nrows > 0) {
     const long bound = ((long)nrows + (long)B * (dt - 1) + dt - 1) / dt;
@@ -1351,9 +1389,16 @@ int split16_run(const void* bank, const void* bank16, const void* spectra, const
 
   // refine: the best pair and its near ties of every particle on split-bf16
   const float abs_margin = kRefineAbs / sqrtf((float)n * (float)(n / 2 + 1));
-  hipLaunchKernelGGL(split_select_kernel, dim3((B + 3) / 4), dim3(256), 0, st, dir_off, B, P, scores, best, peak,
-                     kRefineRel, abs_margin, list, count);
+  int* cnt = count + 64;  // [B] pair counts, then [B+1] offsets (off[B] = the list length)
+  int* off = cnt + B;
+  hipLaunchKernelGGL(split_select_kernel<false>, dim3((B + 3) / 4), dim3(256), 0, st, dir_off, B, P, scores, best,
+                     peak, kRefineRel, abs_margin, cnt, (const int*)nullptr, (int2*)nullptr);
   GIS_LAUNCH_CHECK("gis_correlate_split_f16 select");
+  hipLaunchKernelGGL(split_scan_kernel, dim3(1), dim3(1024), 0, st, cnt, order, B, off);
+  GIS_LAUNCH_CHECK("gis_correlate_split_f16 select scan");
+  hipLaunchKernelGGL(split_select_kernel<true>, dim3((B + 3) / 4), dim3(256), 0, st, dir_off, B, P, scores, best,
+                     peak, kRefineRel, abs_margin, cnt, off, list);
+  GIS_LAUNCH_CHECK("gis_correlate_split_f16 select write");
   hipLaunchKernelGGL(split_tables_kernel, dim3(64), dim3(256), 0, st, n, C, LT, split_nks(n), split_nct(n), kTabBF16,
                      (void*)Wt, (void*)Et);
   GIS_LAUNCH_CHECK("gis_correlate_split_f16 refine tables");
@@ -1365,7 +1410,7 @@ int split16_run(const void* bank, const void* bank16, const void* spectra, const
   rc = launch_split<LTV, 1, 1, false, true, true, PFV>(grid, lds, st, (const float2*)bank,                         \
                                                        (const float2*)spectra32, inv_norm, tinv, dir_off, dir_idx, \
                                                        pmap, order, unit_off, B, P, n, C, 0, 0, 0, (const void*)Wt, \
-                                                       (const void*)Et, scores, peak_y, peak_x, list, count)
+                                                       (const void*)Et, scores, peak_y, peak_x, list, off + B)
     const bool deep = split_nks(n) % 4 == 0;  // four steps ahead where the row steps allow it
     if (LT == 2) {
       if (deep) GIS_REFINE(2, 4); else GIS_REFINE(2, 2);
@@ -1387,7 +1432,7 @@ int split16_run(const void* bank, const void* bank16, const void* spectra, const
 // particles and npairs = dir_off[B] * P searched pairs.
 GIS_API size_t gis_correlate_split_f16_workspace(int n, int C, int B, long npairs) {
   if (n < 2 || C < 2 || B < 0 || npairs < 0) return 0;
-  return split16_list_offset(n, C, B) + 256 + align256((size_t)npairs * sizeof(int2));
+  return split16_list_offset(n, C, B) + split16_count_bytes(B) + align256((size_t)npairs * sizeof(int2));
 }
 
 // The search of gis_correlate_split with fp16 operands: the psi spectra from
