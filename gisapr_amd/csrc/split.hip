@@ -204,36 +204,64 @@ __global__ void split_tables_kernel(int n, int C, int LT, int nks, int nct, int 
   }
 }
 
-// unit_off[k] = sum over earlier particles (in processing order) of
-// ceil(M_b / DT); unit_off[B] = the total.  One 1024-thread block.
+// Exclusive scans over the particles in processing order (order[k], or k
+// when order is NULL), in two launches of ceil(B / 1024) blocks of 1024
+// threads, one entry per thread.  Values: kScanUnits, the search units of
+// particle b, ceil((src[b+1] - src[b]) / DT) for src = dir_off, written at
+// the position k (unit_off[k]); kScanCounts, src[b] written at b (off[b]).
+// out[B] = the total.  WRITE = false stores each tile's total in
+// tile_sum[tile]; WRITE = true adds the totals of the earlier tiles to the
+// tile's own scan.  B = 0 takes one block and writes out[0] = 0.
+enum ScanValue { kScanUnits = 0, kScanCounts = 1 };
+
+template <int MODE, bool WRITE>
 __global__ void __launch_bounds__(1024)
-split_units_kernel(const int* __restrict__ dir_off, const int* __restrict__ order, int B, int DT,
-                   int* __restrict__ unit_off) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int chunk = (B + 1023) / 1024;
-  const int k0 = min(t * chunk, B), k1 = min(k0 + chunk, B);
-  int s = 0;
-#pragma unroll 8
-  for (int k = k0; k < k1; ++k) {  // independent loads: unrolled so they are in flight together
-    const int b = order ? order[k] : k;
-    s += (dir_off[b + 1] - dir_off[b] + DT - 1) / DT;
+order_scan_kernel(const int* __restrict__ src, const int* __restrict__ order, int B, int DT,
+                  int* __restrict__ tile_sum, int* __restrict__ out) {
+  __shared__ int wave_sum[16];
+  __shared__ int base_s;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int k = (int)blockIdx.x * 1024 + t;
+  int b = k, v = 0;
+  if (k < B) {
+    b = order ? order[k] : k;
+    v = MODE == kScanUnits ? (src[b + 1] - src[b] + DT - 1) / DT : src[b];
   }
-  part[t] = s;
+  int x = v;  // inclusive scan within the wave
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wave_sum[w] = x;
+  if (WRITE && w == 0) {  // the totals of the earlier tiles
+    int s = 0;
+    for (int i = lane; i < (int)blockIdx.x; i += 64) s += tile_sum[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) base_s = s;
+  }
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int v = t >= o ? part[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  int before = 0;  // the earlier waves of this tile
+  for (int i = 0; i < w; ++i) before += wave_sum[i];
+  if (!WRITE) {
+    if (t == 1023) tile_sum[blockIdx.x] = before + x;
+    return;
   }
-  int run = part[t] - s;
-  for (int k = k0; k < k1; ++k) {
-    unit_off[k] = run;
-    const int b = order ? order[k] : k;
-    run += (dir_off[b + 1] - dir_off[b] + DT - 1) / DT;
-  }
-  if (t == 1023) unit_off[B] = part[1023];
+  const int excl = base_s + before + x - v;
+  if (k < B) out[MODE == kScanUnits ? k : b] = excl;
+  if (k == B - 1) out[B] = excl + v;
+  if (B == 0 && t == 0) out[0] = 0;
+}
+
+// The two launches of order_scan_kernel; tile_sum holds scan_tiles(B) ints.
+int scan_tiles(int B) { return std::max(1, (B + 1023) / 1024); }
+
+template <int MODE>
+void launch_order_scan(const int* src, const int* order, int B, int DT, int* tile_sum, int* out, hipStream_t st) {
+  const dim3 grid((unsigned)scan_tiles(B));
+  hipLaunchKernelGGL((order_scan_kernel<MODE, false>), grid, dim3(1024), 0, st, src, order, B, DT, tile_sum, out);
+  hipLaunchKernelGGL((order_scan_kernel<MODE, true>), grid, dim3(1024), 0, st, src, order, B, DT, tile_sum, out);
 }
 
 // The step operands of one lane: 4 complex values (rows 16 ks + 4g + j of
@@ -722,7 +750,7 @@ __global__ void pack_fragments16_kernel(const float2* __restrict__ src, const fl
 // score is within max(rel |peak_b|, abs_margin) of its best go on the list
 // (b, pair index) for the split-bf16 rescoring; the best pair always does.
 // Two passes without atomics: WRITE = false counts each particle's pairs
-// (cnt[b]); split_scan_kernel turns the counts into offsets in the
+// (cnt[b]); order_scan_kernel turns the counts into offsets in the
 // particles' processing order; WRITE = true writes each particle's entries
 // there, so a particle's entries (and, in a batched search, the slots'
 // copies of one particle) are adjacent on the list.
@@ -750,34 +778,6 @@ split_select_kernel(const int* __restrict__ dir_off, int B, int P, const float* 
     base += __popcll(m);
   }
   if (!WRITE && lane == 0) cnt[b] = base;
-}
-
-// off[order[k]] = sum of cnt[order[j]] over j < k (order NULL: identity);
-// off[B] = the total.  One 1024-thread block.
-__global__ void __launch_bounds__(1024)
-split_scan_kernel(const int* __restrict__ cnt, const int* __restrict__ order, int B, int* __restrict__ off) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int chunk = (B + 1023) / 1024;
-  const int k0 = min(t * chunk, B), k1 = min(k0 + chunk, B);
-  int s = 0;
-#pragma unroll 8
-  for (int k = k0; k < k1; ++k) s += cnt[order ? order[k] : k];
-  part[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int v = t >= o ? part[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int run = part[t] - s;
-  for (int k = k0; k < k1; ++k) {
-    const int b = order ? order[k] : k;
-    off[b] = run;
-    run += cnt[b];
-  }
-  if (t == 1023) off[B] = part[1023];
 }
 
 template <int DT, int PT>
@@ -1203,8 +1203,9 @@ GIS_API int gis_pack_fragments_f16(const void* src, const float* inv_norm, int r
 
 GIS_API size_t gis_correlate_split_workspace(int n, int C, int B) {
   if (n < 2 || C < 2 || B < 0) return 0;
+  // tables, unit offsets [B + 1], the scans' tile totals
   return align256((split_w_frags(n, C) + split_e_frags(n, C)) * sizeof(bf16x8)) +
-         align256((size_t)(B + 1) * sizeof(int));
+         align256((size_t)(B + 1 + scan_tiles(B)) * sizeof(int));
 }
 
 namespace {
@@ -1241,7 +1242,7 @@ int split_run(const void* bank, const void* spectra, const float* inv_norm, cons
   hipLaunchKernelGGL(split_tables_kernel, dim3(64), dim3(256), 0, st, n, C, LT, split_nks(n), split_nct(n),
                      f32 ? kTabF32 : kTabBF16, (void*)Wt, (void*)Et);
   GIS_LAUNCH_CHECK("gis_correlate_split tables");
-  hipLaunchKernelGGL(split_units_kernel, dim3(1), dim3(1024), 0, st, dir_off, order, B, dt, unit_off);
+  launch_order_scan<kScanUnits>(dir_off, order, B, dt, unit_off + B + 1, unit_off, st);
   GIS_LAUNCH_CHECK("gis_correlate_split units");
   if (nrows > 0) {
     // units <= sum_b ceil(M_b / dt) <= (nrows + B (dt - 1)) / dt
@@ -1337,7 +1338,7 @@ int split16_run(const void* bank, const void* bank16, const void* spectra, const
   hipLaunchKernelGGL(split_tables_kernel, dim3(64), dim3(256), 0, st, n, C, LT, split_nks(n), split_nct(n), kTabF16,
                      (void*)Wt, (void*)Et);
   GIS_LAUNCH_CHECK("gis_correlate_split_f16 tables");
-  hipLaunchKernelGGL(split_units_kernel, dim3(1), dim3(1024), 0, st, dir_off, order, B, dt, unit_off);
+  launch_order_scan<kScanUnits>(dir_off, order, B, dt, unit_off + B + 1, unit_off, st);
   GIS_LAUNCH_CHECK("gis_correlate_split_f16 units");
   if (nrows > 0) {
     const long bound = ((long)nrows + (long)B * (dt - 1) + dt - 1) / dt;
@@ -1394,7 +1395,7 @@ int split16_run(const void* bank, const void* bank16, const void* spectra, const
   hipLaunchKernelGGL(split_select_kernel<false>, dim3((B + 3) / 4), dim3(256), 0, st, dir_off, B, P, scores, best,
                      peak, kRefineRel, abs_margin, cnt, (const int*)nullptr, (int2*)nullptr);
   GIS_LAUNCH_CHECK("gis_correlate_split_f16 select");
-  hipLaunchKernelGGL(split_scan_kernel, dim3(1), dim3(1024), 0, st, cnt, order, B, off);
+  launch_order_scan<kScanCounts>(cnt, order, B, 1, unit_off + B + 1, off, st);  // the units' tile totals are spent
   GIS_LAUNCH_CHECK("gis_correlate_split_f16 select scan");
   hipLaunchKernelGGL(split_select_kernel<true>, dim3((B + 3) / 4), dim3(256), 0, st, dir_off, B, P, scores, best,
                      peak, kRefineRel, abs_margin, cnt, off, list);
