@@ -21,6 +21,7 @@ SIGNATURES = {
     "gis_version": (_i, []),
     "gis_init": (_i, [_i]),
     "gis_splat": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _d, _vp, _vp]),
+    "gis_move_chain": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp]),
     "gis_volume_dft_bytes": (_sz, [_i]),
     "gis_volume_dft_workspace": (_sz, [_i]),
     "gis_volume_dft_bytes_ex": (_sz, [_i, _i]),

@@ -38,7 +38,7 @@ import torch
 from . import kernels
 from .density import atoms_to_atomic_numbers_and_masses, density_from_coords, splat_parameters, yflip_vectorized
 from .engine import ParticleSet, SearchEngine
-from .pdb_text import quantize_8_3, read_pdb_cached, transform_coordinates
+from .pdb_text import quantize_8_3, read_pdb_cached, rotvec_matrix, transform_coordinates
 
 
 # ---------------------------------------------------------------------------
@@ -264,6 +264,28 @@ class RigidBody:
         the chain's mean, 8.3-rounded (pdb_text.py:123-142)."""
         return quantize_8_3(transform_coordinates(self.moving_coords, pose))
 
+    @property
+    def centre(self) -> np.ndarray:
+        """The pivot ``moved`` uses: the float32 mean of the chain's coordinates."""
+        return np.asarray(self.moving_coords).mean(axis=0)
+
+    def pose_rt(self, pose) -> np.ndarray:
+        """float64 [12] operand of ``kernels.move_chain`` for ``pose``: the rotation
+        matrix ``moved`` applies (row-major), then the translation.  Poses whose
+        moved chain may leave the PDB 8.3 field are checked on the host first
+        (``moved`` raises for them, as writing the coordinates would)."""
+        p = np.asarray(pose, np.float64)
+        if p.shape != (6,) or not np.isfinite(p).all():
+            raise ValueError("pose must be six finite numbers: rotation vector (deg) and translation (A)")
+        if not hasattr(self, "_radius"):
+            d = np.asarray(self.moving_coords, np.float64) - self.centre
+            self._radius = float(np.sqrt((d * d).sum(axis=1).max())) if len(d) else 0.0
+        base = self.centre.astype(np.float64) + p[3:]
+        # a rotation keeps every |x - centre| <= radius; 0.01 A covers the rounding
+        if np.any(base + self._radius + 0.01 >= 9999.9995) or np.any(base - self._radius - 0.01 <= -999.9995):
+            self.moved(p)  # raises when a coordinate really is outside the field
+        return np.concatenate([rotvec_matrix(p[:3]).reshape(9), p[3:]])
+
 
 # ---------------------------------------------------------------------------
 # The batched step
@@ -319,7 +341,12 @@ class CandidateBatch:
         self.fixed = density_from_coords(body.fixed_coords, body.fixed_numbers, N, engine.apix, res, device=dev)
         self.moving_numbers = torch.as_tensor(body.moving_numbers, dtype=torch.float32).to(dev).contiguous()
         A = len(body.moving_numbers)
-        self.coords_host = torch.empty((self.slots, A, 3), dtype=torch.float32).pin_memory()
+        # per slot the pose (rotation matrix and translation); the step moves the chain on the device
+        self.rt_host = torch.empty((self.slots, 12), dtype=torch.float64).pin_memory()
+        self.rt_dev = torch.zeros((self.slots, 12), dtype=torch.float64, device=dev)
+        self.rt_dev[:, 0] = self.rt_dev[:, 4] = self.rt_dev[:, 8] = 1.0
+        self.moving_dev = torch.as_tensor(np.ascontiguousarray(body.moving_coords, np.float32)).to(dev)
+        self.centre = body.centre
         self.coords_dev = torch.zeros((self.slots, A, 3), dtype=torch.float32, device=dev)
         self.volume = torch.empty((N, N, N), dtype=torch.float32, device=dev)
         self.dft = torch.empty((self.group, int(kernels.lib().gis_volume_dft_bytes(N)) // 8), dtype=torch.complex64,
@@ -378,6 +405,7 @@ class CandidateBatch:
             kernels.pack_fragments_f16(self.bank[rows], self.tinv[rows], out=self.bank16.data[rows])
 
     def _all_templates(self, k: int):
+        kernels.move_chain(self.moving_dev, self.centre, self.rt_dev[:k], out=self.coords_dev[:k])
         for c0 in range(0, k, self.group):
             self._templates(c0, min(c0 + self.group, k))
 
@@ -423,8 +451,8 @@ class CandidateBatch:
         fill = self.slots if self.graph is not None else k  # a graph runs every slot: repeat the last pose
         self._copied.synchronize()  # the previous step's copy out of the pinned buffer has run
         for c in range(fill):
-            self.coords_host[c].copy_(torch.from_numpy(self.body.moved(poses[min(c, k - 1)])))
-        self.coords_dev[:fill].copy_(self.coords_host[:fill], non_blocking=True)
+            self.rt_host[c].copy_(torch.from_numpy(self.body.pose_rt(poses[min(c, k - 1)])))
+        self.rt_dev[:fill].copy_(self.rt_host[:fill], non_blocking=True)
         self._copied.record()
         return k
 

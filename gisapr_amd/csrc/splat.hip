@@ -73,3 +73,75 @@ GIS_API int gis_splat(const float* xyz, const float* numbers, int n_atoms, int b
   GIS_LAUNCH_CHECK("gis_splat");
   return GIS_OK;
 }
+
+// Moving-chain coordinates of `poses` rigid poses (pose p, atom a, axis j),
+// computed as the host does (pdb_text.py:123-142, then the PDB 8.3 write /
+// re-read round trip, Z3 pdb_text.py:179-198):
+//   d   = xyz[a] - centre                                    (float32)
+//   v_j = fma(d2, R[2][j], fma(d1, R[1][j], d0 R[0][j])) + centre_j + t_j
+//   out = float(k / 1000), k = the integer nearest v_j * 1000 (ties to even)
+// The product chain is the one NumPy's float64 matmul of (x - c) @ R gives
+// (OpenBLAS: one multiply, then fused multiply-adds along the 3-term dot);
+// the nearest integer is decided on the exact product v_j * 1000, as
+// float(f"{v:8.3f}") does: rint of the rounded product, then the exact sign
+// of v * 1000 - (k +- 1/2) from one fma each (a single rounding keeps the
+// sign, and the result is zero only at an exact tie).
+__device__ __forceinline__ double round_8_3(double v) {
+#pragma clang fp contract(off)
+  double k = rint(v * 1000.0);
+  const double up = fma(v, 1000.0, -(k + 0.5));
+  const double dn = fma(v, 1000.0, -(k - 0.5));
+  if (up > 0.0) {
+    k += 1.0;
+  } else if (up == 0.0) {
+    if (fmod(k, 2.0) != 0.0) k += 1.0;  // tie between k and k + 1
+  } else if (dn < 0.0) {
+    k -= 1.0;
+  } else if (dn == 0.0) {
+    if (fmod(k, 2.0) != 0.0) k -= 1.0;  // tie between k - 1 and k
+  }
+  return k / 1000.0;
+}
+
+__global__ void __launch_bounds__(256)
+move_chain_kernel(const float* __restrict__ xyz, int n_atoms, float cx, float cy, float cz,
+                  const double* __restrict__ rt, int poses, float* __restrict__ out) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)n_atoms * poses) return;
+  const int p = (int)(i / n_atoms), a = (int)(i - (long)p * n_atoms);
+  const double* m = rt + 12 * (size_t)p;
+  const float c[3] = {cx, cy, cz};
+  const double d0 = (double)(xyz[3 * (size_t)a] - cx);
+  const double d1 = (double)(xyz[3 * (size_t)a + 1] - cy);
+  const double d2 = (double)(xyz[3 * (size_t)a + 2] - cz);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double v = d0 * m[j];
+    v = fma(d1, m[3 + j], v);
+    v = fma(d2, m[6 + j], v);
+    v = v + (double)c[j];
+    v = v + m[9 + j];
+    out[3 * (size_t)i + j] = (float)round_8_3(v);
+  }
+}
+
+// out [poses][n_atoms][3] float32: the coordinates xyz [n_atoms][3] moved by
+// each pose and 8.3-rounded.  rt [poses][12] float64 device array: the
+// rotation matrix R row-major (row vectors: (x - centre) R) then the
+// translation t; centre: 3 floats (host), the chain's mean.  Replaces
+// transform_coordinates (pdb_text.py:123-142) and the 8.3 round trip of the
+// moved chain (Z3 pdb_text.py:179-198) for a batch of candidate poses; the
+// caller checks that the results fit the 8.3 field (|R| = 1 bounds them).
+GIS_API int gis_move_chain(const float* xyz, int n_atoms, const float* centre, const double* rt, int poses,
+                           float* out, void* stream) {
+  GIS_REQUIRE(n_atoms >= 0 && poses >= 0, "gis_move_chain: n_atoms %d poses %d", n_atoms, poses);
+  if (n_atoms == 0 || poses == 0) return GIS_OK;
+  GIS_REQUIRE(xyz && centre && rt && out, "gis_move_chain: null pointer");
+  const long total = (long)n_atoms * poses;
+  GIS_REQUIRE(total <= (long)INT32_MAX - 255, "gis_move_chain: %ld atom poses", total);
+  hipLaunchKernelGGL(move_chain_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     xyz, n_atoms, centre[0], centre[1], centre[2], rt, poses, out);
+  GIS_LAUNCH_CHECK("gis_move_chain");
+  return GIS_OK;
+}

@@ -40,6 +40,28 @@ def splat(xyz: torch.Tensor, numbers: torch.Tensor, box: int, width: int, apix: 
     return volume
 
 
+def move_chain(xyz: torch.Tensor, centre, rt: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Coordinates [k, n, 3] float32 of the chain ``xyz`` ([n, 3] float32) moved
+    by each pose and 8.3-rounded: ``rt`` [k, 12] float64 holds R row-major
+    ((x - centre) R) and t; ``centre`` is the chain's float32 mean (3 values)."""
+    require_hip(xyz, "xyz")
+    require_hip(rt, "rt")
+    if xyz.dtype != torch.float32 or xyz.ndim != 2 or xyz.shape[1] != 3 or not xyz.is_contiguous():
+        raise ValueError("xyz must be contiguous float32 [n, 3]")
+    if rt.dtype != torch.float64 or rt.ndim != 2 or rt.shape[1] != 12 or not rt.is_contiguous():
+        raise ValueError("rt must be contiguous float64 [k, 12]")
+    c = np.ascontiguousarray(np.asarray(centre, np.float32).reshape(3))
+    k, n = int(rt.shape[0]), int(xyz.shape[0])
+    if out is None:
+        out = torch.empty((k, n, 3), dtype=torch.float32, device=xyz.device)
+    require_hip(out, "out")
+    if tuple(out.shape) != (k, n, 3) or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError("out must be contiguous float32 [k, n, 3]")
+    check(lib().gis_move_chain(ptr(xyz), n, c.ctypes.data, ptr(rt), k, ptr(out), stream_of(xyz.device)),
+          "gis_move_chain")
+    return out
+
+
 def volume_dft(volume: torch.Tensor, out: Optional[torch.Tensor] = None,
                workspace: Optional[torch.Tensor] = None, pad: bool = True) -> torch.Tensor:
     """[N]^3 float32 -> bricked half spectrum of [L, L, L/2+1] complex64

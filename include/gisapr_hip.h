@@ -49,6 +49,21 @@ int gis_splat(const float* xyz, const float* numbers, int n_atoms, int box, int 
               double apix, double rp, double kn, float* volume, void* stream);
 
 /*
+ * (a) Rigid move + 8.3 round trip of a chain for a batch of poses.  Replaces
+ * transform_coordinates (pdb_text.py:123-142: (x - centre) @ R + centre + t)
+ * and the PDB write / re-read of the moved chain (Z3 pdb_text.py:179-198:
+ * float(f"{v:8.3f}")) for each candidate pose.  xyz [n][3] float32, centre
+ * 3 floats (host memory; the chain's float32 mean), rt [poses][12] float64
+ * DEVICE array (R row-major, then t), out [poses][n][3] float32.  The float64
+ * product is the fixed chain d0 R0j, then fma(d1, R1j, .), fma(d2, R2j, .)
+ * (NumPy's float64 matmul of the same operands), the rounding to 3 decimals
+ * is exact (ties to even on the exact value).  The caller checks that the
+ * moved coordinates fit the 8.3 field.
+ */
+int gis_move_chain(const float* xyz, int n_atoms, const float* centre, const double* rt, int poses, float* out,
+                   void* stream);
+
+/*
  * (b, prefix) Pad / sinc^2 / fftshift / rfftn of the model volume.
  * Replaces IntegratedProjector.build_templates v606:1068-1073.
  * volume [N][N][N] float32 -> dft: the [2N][2N][N+1] complex64 half spectrum

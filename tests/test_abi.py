@@ -65,6 +65,9 @@ def test_version_and_argument_errors_without_gpu(library):
         256, 16, 10)
     rc = library.gis_pack_fragments_f16(None, None, 1, 64, 0.0, None, None)
     assert rc == -1 and b"scale" in library.gis_last_error()
+    rc = library.gis_move_chain(None, -1, None, None, 1, None, None)
+    assert rc == -1 and b"n_atoms" in library.gis_last_error()
+    assert library.gis_move_chain(None, 0, None, None, 4, None, None) == 0  # nothing to move
     rc = library.gis_volume_dft(None, 32, None, None, 0, None)
     assert rc == -1
     rc = library.gis_build_template_coefficients(None, 32, 32, None, 1, None, None, None, 0, 16, None, None, None,

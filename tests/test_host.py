@@ -111,6 +111,26 @@ def test_quantize_8_3_matches_text_round_trip():
         quantize_8_3([[10000.0, 0.0, 0.0]])
 
 
+def test_pose_operand_of_the_device_move():
+    """RigidBody.pose_rt: the rotation matrix ``moved`` applies and the
+    translation; poses that could leave the 8.3 field go through the host
+    check, which raises for a coordinate that really leaves it."""
+    from gisapr_amd.batch import RigidBody
+    from gisapr_amd.pdb_text import rotvec_matrix
+    xyz = np.array([[9990.0, 0.0, 0.0], [9980.0, 1.0, 2.0], [9985.0, -3.0, 1.0]], np.float32)
+    body = RigidBody(np.zeros((0, 3), np.float32), np.zeros(0, np.float32), xyz, np.ones(3, np.float32))
+    pose = np.array([10.0, -20.0, 30.0, 1.0, 2.0, 3.0])
+    rt = body.pose_rt(pose)
+    assert rt.shape == (12,) and rt.dtype == np.float64
+    np.testing.assert_array_equal(rt[:9], rotvec_matrix(pose[:3]).reshape(9))
+    np.testing.assert_array_equal(rt[9:], pose[3:])
+    body.pose_rt([0, 0, 0, 9.9, 0, 0])  # the bound trips (9985 + 9.9 + 5.4), the host check passes (x <= 9999.9)
+    with pytest.raises(ValueError):
+        body.pose_rt([0, 0, 0, 20.0, 0, 0])
+    with pytest.raises(ValueError):
+        body.pose_rt([0, 0, np.nan, 0, 0, 0])
+
+
 def test_search_session_cache_forms():
     """run_search's metadata cache accepts the worker's SearchCache shape
     (``metadata.get_or_build``), a dict, or None."""
