@@ -677,11 +677,12 @@ def main() -> int:
     if "fp32" in legs:
         ms = legs["fp32"]["stage_ms"]["search"]
         ach = correlation_flops_per_score(n) * n_scores / (ms * 1e-3) / 1e12
-        legs["fp32"]["roofline"] = {"bound": "mfma", "kernel": "correlate_kernel", "achieved": ach,
-                                    "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+        legs["fp32"]["roofline"] = {"bound": "mfma",
+                                    "kernel": "split_correlate_kernel, FP32 operands (gis_correlate; + tables, tail)",
+                                    "achieved": ach, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
                                     "frac": ach / PEAK_FP32_MFMA_TFLOPS,
                                     "flops_per_score": correlation_flops_per_score(n), "launch_ms": ms,
-                                    "traffic": profiled_traffic("correlate_kernel", wl.name)}
+                                    "traffic": profiled_traffic("fp32_search", wl.name)}
     if "split" in legs:
         ms = legs["split"]["stage_ms"]["search"]
         fps = split_flops_per_score(n, eng.crop)
@@ -690,7 +691,7 @@ def main() -> int:
                                      "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
                                      "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps,
                                      "launch_ms": ms,
-                                     "traffic": profiled_traffic("split_correlate_kernel", wl.name)}
+                                     "traffic": profiled_traffic("split_search", wl.name)}
     for leg in ("split", "split16"):
         if leg not in legs:
             continue
@@ -701,7 +702,7 @@ def main() -> int:
             "bound": "hbm", "kernel": "volume transform + build_templates_kernel", "achieved": ach_t,
             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach_t / PEAK_HBM_GBS,
             "bytes_per_direction": bpd, "directions": len(eng.dir_rot), "launch_ms": ms_t,
-            "traffic": profiled_traffic("build_templates_kernel", wl.name)}
+            "traffic": profiled_traffic("templates_stage", wl.name)}
     if "split16" in legs:
         ms = legs["split16"]["stage_ms"]["search"]
         fps = split16_flops_per_score(n, eng.crop)
@@ -710,7 +711,7 @@ def main() -> int:
             "bound": "mfma", "kernel": "split16_correlate_kernel (+ tables, particle tail, split-bf16 rescoring)",
             "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
             "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps, "launch_ms": ms,
-            "traffic": profiled_traffic("split16_correlate_kernel", wl.name)}
+            "traffic": profiled_traffic("split16_search", wl.name)}
     if "mixed" in legs:
         ms = legs["mixed"]["stage_ms"]["search"]
         fps = mixed_flops_per_score(tab.K, tab.n_lags)
@@ -719,7 +720,7 @@ def main() -> int:
                                      "kernel": "mixed_correlate_kernel (+ particle tail)",
                                      "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
                                      "frac": ach / PEAK_BF16_MFMA_TFLOPS, "flops_per_score": fps,
-                                     "launch_ms": ms, "traffic": profiled_traffic("mixed_correlate_kernel", wl.name)}
+                                     "launch_ms": ms, "traffic": profiled_traffic("mixed_search", wl.name)}
 
     if args.copy_calibration_mb > 0:
         src = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device=dev)
@@ -732,8 +733,8 @@ def main() -> int:
     head_name = next(k for k in ("batched_split16", "batched_split", "split16", "split", "batched_mixed", "mixed",
                                  "batched_fp32", "fp32") if k in legs)
     head = legs[head_name]
-    single = next(k for k in ("split16", "split", "mixed", "fp32") if k in legs)
-    roof = dict(legs[single].get("roofline", {}))
+    single = next((k for k in ("split16", "split", "mixed", "fp32") if k in legs), None)
+    roof = dict(legs[single].get("roofline", {})) if single else None  # no single-candidate leg: no roofline
     if single in ("split", "split16") and legs[single].get("roofline_templates"):
         # the line's roofline is the kernel that takes the larger share of a candidate step
         st = legs[single]["stage_ms"]
