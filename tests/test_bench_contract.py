@@ -25,9 +25,9 @@ def test_split_search_work_per_score():
 
 
 def test_traffic_summaries_are_read():
-    for kernel in ("mixed_correlate_kernel", "build_templates_kernel", "correlate_kernel"):
-        t = bench.profiled_traffic(kernel, "box256")
-        assert t is not None and t > 1e8, kernel
+    for stage in ("mixed_search", "templates_stage", "fp32_search", "split_search", "split16_search"):
+        t = bench.profiled_traffic(stage, "box256")
+        assert t is not None and t > 1e8, stage
     assert bench.profiled_traffic("no_such_kernel", "box256") is None
 
 
