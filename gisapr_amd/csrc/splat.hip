@@ -9,7 +9,7 @@
 //   value = float( double(Z)*kn * double(expf(float(-double(r)*rp))) )
 // The volume is indexed x + box*(y + box*z), i.e. [z][y][x].
 //
-// gfx950 mapping: one 64-lane wave per atom, the (<= 13^3) voxel window is
+// gfx950 mapping: one 64-lane wave per atom, the (2 width)^3 voxel window is
 // strided over the lanes and accumulated with hardware float atomics
 // (global_atomic_add_f32) into the caller's volume, which the caller either
 // zeroes or pre-fills with a resident fixed-body density.
