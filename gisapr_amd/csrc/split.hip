@@ -855,7 +855,7 @@ split16_correlate_kernel(const void* __restrict__ bankv, const f16x2* __restrict
   for (int p = 0; p < PT; ++p) pp[p] = spectra + ((size_t)pb * P + min(p0 + p, P - 1)) * fplane;
   const int g = lane >> 4, i = lane & 15;
 
-  auto load = [&](int ct, int ks, Frags16<DT, PT>& f) {
+  auto load_tpl = [&](int ct, int ks, Frags16<DT, PT>& f) {
     const int tile = (ks * nct + ct) * 256;
 #pragma unroll
     for (int d = 0; d < DT; ++d) {
@@ -870,8 +870,14 @@ split16_correlate_kernel(const void* __restrict__ bankv, const f16x2* __restrict
         f.t[d][3] = make_float2(q1.z, q1.w);
       }
     }
+  };
+  auto load_psi = [&](int ct, int ks, int p, Frags16<DT, PT>& f) {
+    f.p[p] = *(const f16x8*)(pp[p] + (ks * nct + ct) * 256 + 4 * lane);
+  };
+  auto load = [&](int ct, int ks, Frags16<DT, PT>& f) {
+    load_tpl(ct, ks, f);
 #pragma unroll
-    for (int p = 0; p < PT; ++p) f.p[p] = *(const f16x8*)(pp[p] + tile + 4 * lane);
+    for (int p = 0; p < PT; ++p) load_psi(ct, ks, p, f);
   };
 
   floatx4 out[DT][PT][LT * LT], gacc[DT][PT][2 * LT];
@@ -889,25 +895,30 @@ split16_correlate_kernel(const void* __restrict__ bankv, const f16x2* __restrict
       wh[t] = tab16[((ks * 2 * LT + t) * NH) * 64 + lane];
       if constexpr (WS) wl[t] = tab16[((ks * 2 * LT + t) * NH + 1) * 64 + lane];
     }
+    // the template values as fp16 (Tr, Ti) and (-Ti, Tr): Q = Pr T + Pi (i T)
+    f16x2 th[DT][4], tn[DT][4];
 #pragma unroll
-    for (int d = 0; d < DT; ++d) {
-      // the template values as fp16 (Tr, Ti) and (-Ti, Tr): Q = Pr T + Pi (i T)
-      f16x2 th[4], tn[4];
+    for (int d = 0; d < DT; ++d)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         if constexpr (T16) {
-          th[j] = f16x2{f.t16[d][2 * j], f.t16[d][2 * j + 1]};
-          tn[j] = times_i(th[j]);
+          th[d][j] = f16x2{f.t16[d][2 * j], f.t16[d][2 * j + 1]};
+          tn[d][j] = times_i(th[d][j]);
         } else {
-          t_pair(f.t[d][j], ts[d], th[j], tn[j]);
+          t_pair(f.t[d][j], ts[d], th[d][j], tn[d][j]);
         }
       }
+    // psi outermost: once a psi fragment has met every direction its
+    // registers take the fragment of step ks2, so the psi loads (the bulk of
+    // the step's memory traffic) leave as early as the products allow
 #pragma unroll
-      for (int p = 0; p < PT; ++p) {
+    for (int p = 0; p < PT; ++p) {
+#pragma unroll
+      for (int d = 0; d < DT; ++d) {
         f16x8 a;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const f16x2 q = cmul_h(th[j], tn[j], f.p[p][2 * j], f.p[p][2 * j + 1]);
+          const f16x2 q = cmul_h(th[d][j], tn[d][j], f.p[p][2 * j], f.p[p][2 * j + 1]);
           a[2 * j] = q[0];
           a[2 * j + 1] = q[1];
         }
@@ -920,8 +931,9 @@ split16_correlate_kernel(const void* __restrict__ bankv, const f16x2* __restrict
           if constexpr (WS) gacc[d][p][t] = mfma_f16(a, wl[t], gacc[d][p][t]);
         }
       }
+      load_psi(ct2, ks2, p, f);
     }
-    load(ct2, ks2, f);
+    load_tpl(ct2, ks2, f);
   };
 
   Frags16<DT, PT> fa, fb;
