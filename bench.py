@@ -12,6 +12,9 @@ float64 score sum.  Legs, on the same particles and poses:
 
   fp32           one candidate per step, the v606 FP32 search (every packed
                  coefficient, the 16 x 16 crop: +/-8 px) on FP32 MFMA
+  fft32          the same search on the fp32 leg's last bank, computed the
+                 way the reference computes it (per pair a float32 irfft2 of
+                 the product, torch / rocFFT on the GPU); search timed alone
   split          one candidate per step, the same search (same operator,
                  same crop) on split-bf16 MFMA
   split16        one candidate per step, the same search with fp16 operands
@@ -247,6 +250,46 @@ def top1(res, ps, center):
         g = off[b] * ps.P + best[b]
         out[b] = (best[b], center - py[g], center - px[g])
     return out
+
+
+def fft32_search(eng, ps, bank, tinv, chunk: int = 1024):
+    """The v606 search computed as the reference computes it (v606:1418-1449),
+    in torch on the GPU: per (particle, local direction, psi) pair the product
+    of the pre-shifted template and particle spectra (v606:535-537), one
+    float32 ``torch.fft.irfft2`` (rocFFT), the C x C crop of the unshifted
+    correlation and its maximum times 1024 n^2; pairs in chunks of ``chunk``.
+    Returns (scores [pairs], y, x, best [B], peak [B]) in the layout of the
+    searches' results (first maximum per particle in (direction, psi) order).
+    A measurement baseline, not a product path."""
+    import torch
+    from gisapr_amd.search import PackedFourierLayout
+    n, C, P, dev = eng.n, eng.crop, ps.P, eng.device
+    lay = PackedFourierLayout(n, C, dev)
+    Ts = torch.fft.fftshift(bank.unpack() * tinv[:, None, None], dim=(-2, -1))
+    Ps = torch.fft.fftshift(eng.row_spectra(ps) * ps.inv_norm[..., None, None], dim=(-2, -1)).reshape(-1, n, n // 2 + 1)
+    rows = ps.dir_idx.long()
+    counts = (ps.dir_off[1:] - ps.dir_off[:-1]).long()
+    row_particle = torch.repeat_interleave(torch.arange(ps.B, device=dev), counts)
+    npairs = int(rows.numel()) * P
+    scores = torch.empty(npairs, dtype=torch.float32, device=dev)
+    index = torch.empty(npairs, dtype=torch.long, device=dev)
+    mult = float(1024 * n * n)
+    for k0 in range(0, npairs, chunk):
+        k = torch.arange(k0, min(k0 + chunk, npairs), device=dev)
+        r, p = k // P, k % P
+        prod = Ts.index_select(0, rows[r]) * Ps.index_select(0, row_particle[r] * P + p)
+        ccg = torch.fft.irfft2(prod, s=(n, n), dim=(-2, -1))
+        flat = lay.centered_ccg_crop_from_unshifted(ccg).reshape(len(k), -1)
+        v, i = torch.max(flat, dim=-1)
+        scores[k0:k0 + len(k)] = v * mult
+        index[k0:k0 + len(k)] = i
+    seg = torch.repeat_interleave(row_particle, P)
+    peak = torch.full((ps.B,), -float("inf"), device=dev).scatter_reduce(0, seg, scores, "amax")
+    pos = torch.arange(npairs, device=dev)
+    first = torch.where(scores == peak[seg], pos, torch.full_like(pos, npairs))
+    gbest = torch.full((ps.B,), npairs, dtype=torch.long, device=dev).scatter_reduce(0, seg, first, "amin")
+    best = torch.where(counts > 0, gbest - ps.dir_off[:-1].long() * P, torch.full_like(gbest, -1))
+    return scores, index // C, index % C, best, peak
 
 
 def precision_check(eng, ps, tk, res, subset: int, seed: int):
@@ -541,7 +584,7 @@ def main() -> int:
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the legs, run one device-to-device copy of this many MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--legs", default="fp32,split,split16,mixed,batched_fp32,batched_split,batched_split16",
+    ap.add_argument("--legs", default="fp32,fft32,split,split16,mixed,batched_fp32,batched_split,batched_split16",
                     help="comma list of legs to run (profiling runs select one)")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
@@ -643,6 +686,47 @@ def main() -> int:
         # split-bf16 path: the template kernel writes the in-disk coefficients directly
         legs["mixed"], results["mixed"], tk_last = run_single(eng.small_templates,
                                                               lambda t, p: eng.search_small_shift(None, p, tk=t))
+    if "fft32" in want and "fp32" in legs:
+        # the reference's own formulation of the same search (irfft2 per pair) on the same bank, timed alone
+        t_bank, t_inv = bank
+        fft32_search(eng, ps, t_bank, t_inv)  # warm-up (rocFFT plans, the row-major spectra)
+        torch.cuda.synchronize()
+        reps = max(1, min(args.steps, 2))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fsc_, fy, fx, fbest, fpeak = fft32_search(eng, ps, t_bank, t_inv)
+        torch.cuda.synchronize()
+        ms_fft = (time.perf_counter() - t0) * 1000.0 / reps
+        ref = results["fp32"]
+        b32, bf = ref.corr.best.cpu().numpy(), fbest.cpu().numpy()
+        off = ps.dir_off.cpu().numpy()
+        py32, px32 = ref.corr.peak_y.cpu().numpy().reshape(-1), ref.corr.peak_x.cpu().numpy().reshape(-1)
+        fyn, fxn = fy.cpu().numpy(), fx.cpu().numpy()
+        used = same_o = same_os = 0
+        for i in range(ps.B):
+            if b32[i] < 0:
+                continue
+            used += 1
+            if b32[i] == bf[i]:
+                same_o += 1
+                g = off[i] * ps.P + b32[i]
+                same_os += int(py32[g] == fyn[g] and px32[g] == fxn[g])
+        m = b32 >= 0
+        p32 = ref.corr.peak.cpu().numpy()[m]
+        dens_tpl = legs["fp32"]["stage_ms"]["density"] + legs["fp32"]["stage_ms"]["templates"]
+        legs["fft32"] = {
+            "formulation": "v606 literally: pre-shifted product, torch.fft.irfft2 (rocFFT, float32) per pair, "
+                           "C x C crop, max; torch on the GPU, 1,024 pairs per chunk (a baseline, not a product path)",
+            "search_ms": ms_fft, "search_scores_per_s": n_scores / (ms_fft * 1e-3),
+            "ms_per_candidate": dens_tpl + ms_fft, "scores_per_s": n_scores / ((dens_tpl + ms_fft) * 1e-3),
+            "candidate_note": "density + templates from the fp32 leg's stage times, plus this search",
+            "top1_same_as_fp32": {"orientation": same_o / max(used, 1), "orientation_and_shift": same_os / max(used, 1),
+                                  "particles": used},
+            "max_rel_peak_diff_vs_fp32": float(np.max(np.abs(fpeak.cpu().numpy()[m] - p32) / np.abs(p32))),
+        }
+        del fsc_, fy, fx, fbest, fpeak
+        ps.spectra = None  # the row-major copy was for this leg only
+        torch.cuda.empty_cache()
     log("single legs", {k: v["ms_per_candidate"] for k, v in legs.items()})
 
     # batched legs: ``slots`` candidates per step, graph-captured, autotuned search tile
@@ -769,6 +853,12 @@ def main() -> int:
     }
     if "fp32" in legs:
         out["speedup_vs_fp32"] = {k: v["scores_per_s"] / legs["fp32"]["scores_per_s"] for k, v in legs.items()}
+    if "fft32" in legs:
+        f = legs["fft32"]
+        out["speedup_vs_fft32"] = {
+            "candidate": {k: v["scores_per_s"] / f["scores_per_s"] for k, v in legs.items()},
+            "search": {k: f["search_ms"] / v["stage_ms"]["search"] for k, v in legs.items() if "stage_ms" in v},
+            "note": "against the reference's own formulation of the FP32 search (irfft2 per pair, torch/rocFFT)"}
     agreement = {}
     for other, center in (("split", eng.crop // 2), ("split16", eng.crop // 2), ("mixed", 3)):
         if "fp32" in results and other in results:
