@@ -5,6 +5,8 @@ one moving chain against the same particle stack.  ``CandidateBatch`` keeps
 everything of such a step resident in HBM and runs it as a fixed sequence of
 launches that can be captured once as a HIP graph and replayed per step:
 
+  all slots at once:     the moving chain moved by each slot's pose and
+                         8.3-rounded (gis_move_chain)
   per candidate slot c:  fixed-body density (copy)  ->  splat of the moving
                          chain  ->  3-D transform  ->  768-direction template
                          bank  ->  (small-shift path) in-disk coefficients
@@ -14,9 +16,9 @@ launches that can be captured once as a HIP graph and replayed per step:
                          particle spectra through a particle map
                          one segmented float64 score sum per slot.
 
-The only host work per step is the rigid-body transform and 8.3 rounding of
-the moving chain (ZL:gisapr_batch_density.py:41-49) and one host-to-device
-copy of the coordinates.
+The only host work per step is each slot's rotation matrix and translation
+(12 doubles; the rigid-body move and 8.3 rounding of ZL:gisapr_batch_density.py:
+41-49 then run on the device) and one host-to-device copy of them.
 
 ``RigidDensityCache`` and ``BatchSearchEngine`` keep the call surface of
 ZL:gisapr_batch_density.py:18-117 and ZL:gisapr_batch_search.py:142-364.
