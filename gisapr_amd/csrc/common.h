@@ -378,6 +378,14 @@ __device__ __forceinline__ void block_sum_f64(double (&v)[NV], double* scratch) 
 // first maximum in (local direction, psi) order, mean and population std in
 // float64.  Internal linkage: each search source gets its own copy.
 // ---------------------------------------------------------------------------
+// (bv, bi) := the first maximum of (bv, bi) and (v, i): the larger value,
+// the smaller index on a tie; selects rather than branches.
+__device__ __forceinline__ void first_max(float& bv, int& bi, float v, int i) {
+  const bool take = (v > bv) | ((v == bv) & (i < bi));
+  bv = take ? v : bv;
+  bi = take ? i : bi;
+}
+
 namespace {
 __global__ void __launch_bounds__(64)
 particle_tail_kernel(const int* __restrict__ dir_off, int P, const float* __restrict__ scores,
@@ -399,7 +407,7 @@ particle_tail_kernel(const int* __restrict__ dir_off, int P, const float* __rest
   for (int o = 32; o > 0; o >>= 1) {
     const float ov = __shfl_xor(bv, o, 64);
     const int oi = __shfl_xor(bi, o, 64);
-    if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    first_max(bv, bi, ov, oi);
     sum += __shfl_xor(sum, o, 64);
   }
   const double mu = np > 0 ? sum / (double)np : 0.0;

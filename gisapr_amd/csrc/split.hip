@@ -661,17 +661,14 @@ split_correlate_kernel(const float2* __restrict__ bank, const float2* __restrict
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
               const int x = 16 * tx + 4 * g + v, y = 16 * ty + i;
-              if (y < C && x < C) {
-                const float val = out[d][p][tx * LT + ty][v] * sc;
-                const int idx = y * C + x;
-                if (val > bv || (val == bv && idx < bi)) { bv = val; bi = idx; }
-              }
+              const bool in = (y < C) & (x < C);
+              first_max(bv, bi, in ? out[d][p][tx * LT + ty][v] * sc : -INFINITY, in ? y * C + x : 0x7fffffff);
             }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
           const float ov = __shfl_xor(bv, o, 64);
           const int oi = __shfl_xor(bi, o, 64);
-          if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+          first_max(bv, bi, ov, oi);
         }
         if (lane == 0) {
           const size_t gidx = (size_t)(d0 + j0 + d) * P + p0 + p;
@@ -1096,17 +1093,14 @@ split16_correlate_kernel(const void* __restrict__ bankv, const f16x2* __restrict
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int x = 16 * tx + 4 * g + v, y = 16 * ty + i;
-            if (y < C && x < C) {
-              const float val = out[d][p][tx * LT + ty][v] * kH16Unscale;
-              const int idx = y * C + x;
-              if (val > bv || (val == bv && idx < bi)) { bv = val; bi = idx; }
-            }
+            const bool in = (y < C) & (x < C);
+            first_max(bv, bi, in ? out[d][p][tx * LT + ty][v] * kH16Unscale : -INFINITY, in ? y * C + x : 0x7fffffff);
           }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
         const float ov = __shfl_xor(bv, o, 64);
         const int oi = __shfl_xor(bi, o, 64);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        first_max(bv, bi, ov, oi);
       }
       if (lane == 0) {
         const size_t gidx = (size_t)(d0 + j0 + d) * P + p0 + p;
